@@ -1,0 +1,164 @@
+/*
+ * les_amd.h -- C ABI of the MI355X (gfx950) LES step library.
+ *
+ * The library runs the per-time-step hot path of MPI-LES on one GPU:
+ *   velnw -> bondv1 -> velfg (vel2) -> feedbf -> les (boundsm) -> adam ->
+ *   press (bondfg, rhs, red-black SOR + boundp1/boundp2, mean removal)
+ * with every field resident in device memory across steps.
+ *
+ * Two layers:
+ *
+ * 1. Reference-signature entry points (les_velnw ... les_press).  Each takes
+ *    the argument list of the Fortran subroutine it replaces, in the same
+ *    order, with every array passed as a pointer to its lower-bound corner
+ *    in the reference's layout (Fortran order, explicit lower bounds,
+ *    extents from ip,jp,kp == im,jm,km; ifdata.f95:238-241) and scalars by
+ *    value.  A Fortran iso_c_binding shim module with the reference's module
+ *    and subroutine names forwards to them unchanged.  The library keeps a
+ *    device mirror of every array it is handed: the first call that sees a
+ *    host address uploads its contents; after that the device copy is
+ *    authoritative and the host array is stale until les_sync_host().
+ *    Arrays the reference writes but never reads again (cov1..9, nou1..9,
+ *    dfu1/dfv1/dfw1, fx/fy/fz) are accepted and left untouched.
+ *
+ * 2. Domain API (les_domain_*): create a domain from its set-up arrays,
+ *    advance it by whole steps, read and write fields by reference name.
+ *
+ * Every function returns 0 on success and a nonzero status otherwise;
+ * les_last_error() describes the last failure of the calling thread.
+ * Calls are asynchronous on the domain's stream unless stated otherwise.
+ */
+#ifndef LES_AMD_H
+#define LES_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ----------------------------------------------------------- errors */
+const char* les_last_error(void);
+
+/* ------------------------------------- reference-signature entry points */
+
+/* velnw.f95:5  subroutine velnw(km,jm,im,p,ro,dxs,u,dt,f,dys,v,g,dzs,w,h) */
+int les_velnw(int km, int jm, int im, const float* p, float ro, const float* dxs, float* u, float dt,
+              const float* f, const float* dys, float* v, const float* g, const float* dzs, float* w,
+              const float* h);
+
+/* bondv1.f95:5  subroutine bondv1(jm,u,z2,dzn,v,w,km,n,im,dt,dxs).
+ * The inflow is the reference's 5 m/s (bondv1.f95:33-54) unless a profile
+ * was set with les_bondv1_set_inflow for a domain of these extents. */
+int les_bondv1(int jm, float* u, const float* z2, const float* dzn, float* v, float* w, int km, int n,
+               int im, float dt, const float* dxs);
+
+/* velFG.f95:6-8  subroutine velfg(km,jm,im,dx1,cov1,...,uspd,vspd) (calls vel2, vel2.f95:4) */
+int les_velfg(int km, int jm, int im, const float* dx1, float* cov1, float* cov2, float* cov3,
+              float* dfu1, float* diu1, float* diu2, const float* dy1, float* diu3, const float* dzn,
+              float vn, float* f, float* cov4, float* cov5, float* cov6, float* dfv1, float* diu4,
+              float* diu5, float* diu6, float* g, float* cov7, float* cov8, float* cov9, float* dfw1,
+              float* diu7, float* diu8, float* diu9, const float* dzs, float* h, float* nou1,
+              const float* u, float* nou5, const float* v, float* nou9, const float* w, float* nou2,
+              float* nou3, float* nou4, float* nou6, float* nou7, float* nou8, float* uspd,
+              float* vspd);
+
+/* feedbf.f95:5-6  subroutine feedbf(km,jm,im,usum,u,bmask1,vsum,v,cmask1,wsum,w,dmask1,alpha,dt,beta,fx,fy,fz,f,g,h) */
+int les_feedbf(int km, int jm, int im, float* usum, const float* u, const float* bmask1, float* vsum,
+               const float* v, const float* cmask1, float* wsum, const float* w, const float* dmask1,
+               float alpha, float dt, float beta, float* fx, float* fy, float* fz, float* f, float* g,
+               float* h);
+
+/* les.f95:6-7  subroutine les(km,delx1,dx1,dy1,dzn,jm,im,diu1,...,diu9,sm,f,g,h,uspd,vspd,dxs,dys)
+ * (calls boundsm, boundsm.f95:5).  delx1 (intent out) is written on the host. */
+int les_les(int km, float* delx1, const float* dx1, const float* dy1, const float* dzn, int jm, int im,
+            const float* diu1, const float* diu2, const float* diu3, const float* diu4,
+            const float* diu5, const float* diu6, const float* diu7, const float* diu8,
+            const float* diu9, float* sm, float* f, float* g, float* h, const float* uspd,
+            const float* vspd, const float* dxs, const float* dys);
+
+/* adam.f95:5  subroutine adam(n,nmax,data21,fold,im,jm,km,gold,hold,fghold,f,g,h).
+ * The character argument data21 names the file of the adam.f95:20-29 dump,
+ * which stays on the Fortran side; it is not passed. */
+int les_adam(int n, int nmax, float* fold, int im, int jm, int km, float* gold, float* hold,
+             const float* fghold, float* f, float* g, float* h);
+
+/* press.f95:9-10  subroutine press(km,jm,im,rhs,u,dx1,v,dy1,w,dzn,f,g,h,dt,cn1,cn2l,p,cn2s,
+ * cn3l,cn3s,cn4l,cn4s,n,nmax,data20,usum,vsum,wsum) (calls bondfg, bondFG.f95:5, and
+ * boundp1/boundp2, boundp.f95:7,28).  data20 (file name of the press.f95:205-217
+ * dump) stays on the Fortran side. */
+int les_press(int km, int jm, int im, float* rhs, const float* u, const float* dx1, const float* v,
+              const float* dy1, const float* w, const float* dzn, float* f, float* g, float* h,
+              float dt, const float* cn1, const float* cn2l, float* p, const float* cn2s,
+              const float* cn3l, const float* cn3s, const float* cn4l, const float* cn4s, int n,
+              int nmax, const float* usum, const float* vsum, const float* wsum);
+
+/* Replaces the values of the hard-coded inflow (bondv1.f95:33-54) with
+ * u_in(j,k), j=1..jm fastest then k=1..km, for the domain of extents
+ * (im,jm,km); prof == NULL restores 5 m/s.  No reference counterpart: the
+ * reference's inflow is a literal. */
+int les_bondv1_set_inflow(int im, int jm, int km, const float* prof);
+
+/* SOR controls fixed by parameters in the reference (press.f95:43-45):
+ * nmaxp = 50, pjuge = 1e-4, omega = 1.  Applies to domains created after
+ * the call and to reference-signature domains. */
+int les_press_config(int nmaxp, float pjuge, float omega);
+
+/* Statistics of the last press call on the reference-signature domain of
+ * extents (im,jm,km): SOR iterations, sum(reltmp^2) per iteration, the
+ * volume-weighted means of rhs and p.  Synchronises. */
+int les_press_stats(int im, int jm, int km, int* iters, double* sor_hist, int hist_len, double* rhsav,
+                    double* pav, int* nonfinite);
+
+/* Copies the device state of a host array handed to an earlier call back
+ * into it.  Synchronises.  Returns 2 (and leaves the array alone) when no
+ * call has received this address: its host contents are then current. */
+int les_sync_host(const void* host_array);
+/* The next call that receives this host array uploads it again. */
+int les_mark_host_dirty(const void* host_array);
+/* Waits for all work of every reference-signature domain. */
+int les_synchronize(void);
+/* Frees every reference-signature domain and forgets all host arrays. */
+int les_release_all(void);
+
+/* --------------------------------------------------------- domain API */
+typedef struct les_domain les_domain;
+
+typedef struct {
+  int im, jm, km;
+  /* 1-D metrics in reference layout (grid.f95:5-172, init.f95:82-107):
+   * dx1(-1:im+1) dy1(0:jm+1) dzn(-1:km+2) dzs(-1:km+2) dxs(0:im) dys(0:jm)
+   * cn2l,cn2s(1:im) cn3l,cn3s(1:jm) cn4l,cn4s(1:km) */
+  const float *dx1, *dy1, *dzn, *dzs, *dxs, *dys;
+  const float *cn2l, *cn2s, *cn3l, *cn3s, *cn4l, *cn4s;
+  const float* cn1;      /* cn1(1:im,1:jm,1:km) */
+  const float* bmask1;   /* (-1:im+1,0:jm+1,0:km+1) */
+  const float* cmask1;   /* (0:im+1,-1:jm+1,0:km+1) */
+  const float* dmask1;   /* (0:im+1,0:jm+1,0:km+1) */
+  const float* inflow;   /* u_in(j,k), NULL = 5 m/s */
+  float ro, vn, alpha, beta, dt;   /* set.f95:89-95 */
+  int nmaxp;
+  float pjuge, omega;
+} les_domain_desc;
+
+int les_domain_create(const les_domain_desc* desc, les_domain** out);
+int les_domain_destroy(les_domain* d);
+/* One time step n (main.f95:260-296): the seven stages in order. */
+int les_domain_step(les_domain* d, int n, int nmax);
+/* Advance steps n0..n1 inclusive. */
+int les_domain_run(les_domain* d, int n0, int n1, int nmax);
+/* Stage s of step n: 0 velnw, 1 bondv1, 2 velfg, 3 feedbf, 4 les, 5 adam, 6 press. */
+int les_domain_stage(les_domain* d, int stage, int n, int nmax);
+/* Field by reference name ("u","v","w","p","f","g","h","usum","vsum","wsum",
+ * "fold","gold","hold","rhs","sm","uspd","vspd","diu1".."diu9","cn1",
+ * "bmask1","cmask1","dmask1"), host array in reference layout. */
+int les_domain_upload(les_domain* d, const char* field, const float* host);
+int les_domain_download(les_domain* d, const char* field, float* host);
+int les_domain_synchronize(les_domain* d);
+int les_domain_press_stats(les_domain* d, int* iters, double* sor_hist, int hist_len, double* rhsav,
+                           double* pav, int* nonfinite);
+/* Bytes of device memory the domain holds. */
+long long les_domain_bytes(const les_domain* d);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

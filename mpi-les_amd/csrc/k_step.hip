@@ -1,0 +1,678 @@
+// Stage kernels of the LES step (velnw, bondv1, velfg+vel2, feedbf, les,
+// adam) and the layout-conversion kernels.  Expressions are evaluated in the
+// order of the cited Fortran lines and the library is built without FMA
+// contraction, so each FP32 result equals the reference's.
+#include "les_kernels.h"
+
+namespace les {
+
+#define BX 64
+#define BY 4
+#define NT (BX * BY)
+
+static inline dim3 block2d() { return dim3(BX, BY, 1); }
+static inline dim3 grid2d(int ni, int nj, int nk) {
+  return dim3((unsigned)((ni + BX - 1) / BX), (unsigned)((nj + BY - 1) / BY), (unsigned)nk);
+}
+
+// ------------------------------------------------------------------ velnw
+// velnw.f95:27-59: u,v for k=1..km, w for k=1..km-1.
+__global__ __launch_bounds__(NT) void k_velnw(Grid g, Metrics m, const float* __restrict__ p,
+                                              const float* __restrict__ f,
+                                              const float* __restrict__ gg,
+                                              const float* __restrict__ h, float* __restrict__ u,
+                                              float* __restrict__ v, float* __restrict__ w,
+                                              float ro, float dt) {
+  const int i = 1 + blockIdx.x * BX + threadIdx.x, j = 1 + blockIdx.y * BY + threadIdx.y;
+  if (i > g.im || j > g.jm) return;
+  const float rdx = M1(m.dxs, i), rdy = M1(m.dys, j);
+  for (int k = 1 + blockIdx.z; k <= g.km; k += gridDim.z) {
+    const int c = ix(g, i, j, k);
+    const float pc = p[c];
+    float pz = (-pc + p[c + 1]) / ro / rdx;
+    u[c] = u[c] + dt * (f[c] - pz);
+    pz = (-pc + p[c + g.sj]) / ro / rdy;
+    v[c] = v[c] + dt * (gg[c] - pz);
+    if (k <= g.km - 1) {
+      pz = (-pc + p[c + g.sk]) / ro / M1(m.dzs, k);
+      w[c] = w[c] + dt * (h[c] - pz);
+    }
+  }
+}
+
+void launch_velnw(const DevDomain& d, float ro, float dt) {
+  const Grid& g = d.g;
+  hipLaunchKernelGGL(k_velnw, grid2d(g.im, g.jm, g.km), block2d(), 0, d.stream, g, d.m, d.F.p,
+                     d.F.f, d.F.g, d.F.h, d.F.u, d.F.v, d.F.w, ro, dt);
+}
+
+// ----------------------------------------------------------------- bondv1
+// Inflow planes i=0,1 (bondv1.f95:33-54) and, at n==1, the copy of column
+// i=1 over i=2..im (bondv1.f95:61-88): u=u_in(j,k), v=w=0 for i=0..ilast.
+// Also clears the outflow max/min accumulators.
+__global__ __launch_bounds__(NT) void k_bondv1_inflow(Grid g, Metrics m, float* __restrict__ u,
+                                                      float* __restrict__ v, float* __restrict__ w,
+                                                      int ilast, Ctrl* ctrl) {
+  if (blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0 && threadIdx.x == 0 && threadIdx.y == 0) {
+    ctrl->uout_max = 0u;
+    ctrl->uout_nmin = 0u;
+  }
+  const int i = blockIdx.x * BX + threadIdx.x, j = 1 + blockIdx.y * BY + threadIdx.y;
+  const int k = 1 + blockIdx.z;
+  if (i > ilast || j > g.jm) return;
+  const float uin = m.inflow ? m.inflow[(j - 1) + (k - 1) * g.jm] : 5.0f;
+  const int c = ix(g, i, j, k);
+  u[c] = uin;
+  v[c] = 0.0f;
+  w[c] = 0.0f;
+}
+
+// aaa = max(0, max u(im,j,k)), bbb = min(0, min u(im,j,k)) (bondv1.f95:98-114).
+// Max/min are exact in any order; non-negative floats order like their bits.
+__global__ __launch_bounds__(256) void k_bondv1_uout(Grid g, const float* __restrict__ u, Ctrl* ctrl) {
+  __shared__ float smax[4], smin[4];
+  float mx = 0.0f, mn = 0.0f;
+  const int n = g.jm * g.km;
+  for (int t = blockIdx.x * 256 + threadIdx.x; t < n; t += gridDim.x * 256) {
+    const int j = 1 + t % g.jm, k = 1 + t / g.jm;
+    const float x = u[ix(g, g.im, j, k)];
+    mx = (x > mx) ? x : mx;
+    mn = (x < mn) ? x : mn;
+  }
+  for (int o = kWave / 2; o > 0; o >>= 1) {
+    const float a = __shfl_xor(mx, o, kWave), b = __shfl_xor(mn, o, kWave);
+    mx = (a > mx) ? a : mx;
+    mn = (b < mn) ? b : mn;
+  }
+  if (threadIdx.x % kWave == 0) {
+    smax[threadIdx.x / kWave] = mx;
+    smin[threadIdx.x / kWave] = mn;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int q = 1; q < 4; q++) {
+      mx = (smax[q] > mx) ? smax[q] : mx;
+      mn = (smin[q] < mn) ? smin[q] : mn;
+    }
+    atomicMax(&ctrl->uout_max, __float_as_uint(mx));
+    atomicMax(&ctrl->uout_nmin, __float_as_uint(0.0f - mn));
+  }
+}
+
+// Advective outflow at i=im (u) and i=im+1 (v,w), bondv1.f95:116-134.
+__global__ __launch_bounds__(256) void k_bondv1_outflow(Grid g, Metrics m, float* __restrict__ u,
+                                                        float* __restrict__ v, float* __restrict__ w,
+                                                        const Ctrl* ctrl, float dt) {
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  if (t >= g.jm * g.km) return;
+  const int j = 1 + t % g.jm, k = 1 + t / g.jm;
+  const float aaa = __uint_as_float(ctrl->uout_max);
+  const float bbb = -__uint_as_float(ctrl->uout_nmin);
+  const float uout = (aaa + bbb) / 2.0f;
+  const float dxs = M1(m.dxs, g.im);
+  int c = ix(g, g.im, j, k);
+  u[c] = u[c] - dt * uout * (u[c] - u[c - 1]) / dxs;
+  c = ix(g, g.im + 1, j, k);
+  v[c] = v[c] - dt * uout * (v[c] - v[c - 1]) / dxs;
+  w[c] = w[c] - dt * uout * (w[c] - w[c - 1]) / dxs;
+}
+
+// Periodic j ghosts (bondv1.f95:137-156) followed by the ground/top rows
+// (bondv1.f95:176-195), written as their final values:
+//   part A: j=0 and j=jm+1 for k=1..km (w: k=1..km-1), i=0..im+1
+//   part B: k=0 and k=km+1 rows of u,v for j=0..jm+1 (periodic image of
+//           j folded in), w(k=0)=w(k=km)=0 for j=-1..jm+1.
+// Every value read is an interior j row, none of which is written here.
+__global__ __launch_bounds__(256) void k_bondv1_ghost(Grid g, float* __restrict__ u,
+                                                      float* __restrict__ v, float* __restrict__ w) {
+  const int ni = g.im + 2;
+  const int nA = ni * g.km, nB = ni * (g.jm + 3);
+  int t = blockIdx.x * 256 + threadIdx.x;
+  if (t < nA) {
+    const int i = t % ni, k = 1 + t / ni;
+    const int c0 = ix(g, i, 0, k), cjm = ix(g, i, g.jm, k), c1 = ix(g, i, 1, k), cj1 = ix(g, i, g.jm + 1, k);
+    u[c0] = u[cjm];
+    u[cj1] = u[c1];
+    v[c0] = v[cjm];
+    v[cj1] = v[c1];
+    if (k <= g.km - 1) {
+      w[c0] = w[cjm];
+      w[cj1] = w[c1];
+    }
+    return;
+  }
+  t -= nA;
+  if (t >= nB) return;
+  const int i = t % ni, j = -1 + t / ni;
+  if (j >= 0) {
+    const int cj = (j == 0) ? g.jm : ((j == g.jm + 1) ? 1 : j);
+    u[ix(g, i, j, 0)] = -u[ix(g, i, cj, 1)];
+    u[ix(g, i, j, g.km + 1)] = u[ix(g, i, cj, g.km)];
+    v[ix(g, i, j, 0)] = -v[ix(g, i, cj, 1)];
+    v[ix(g, i, j, g.km + 1)] = v[ix(g, i, cj, g.km)];
+  }
+  w[ix(g, i, j, 0)] = 0.0f;
+  w[ix(g, i, j, g.km)] = 0.0f;
+}
+
+void launch_bondv1(const DevDomain& d, int n, float dt) {
+  const Grid& g = d.g;
+  const int ilast = (n == 1) ? g.im : 1;
+  hipLaunchKernelGGL(k_bondv1_inflow, grid2d(ilast + 1, g.jm, g.km), block2d(), 0, d.stream, g, d.m,
+                     d.F.u, d.F.v, d.F.w, ilast, d.ctrl);
+  const int nplane = g.jm * g.km;
+  int nb = (nplane + 255) / 256;
+  if (nb > 256) nb = 256;
+  hipLaunchKernelGGL(k_bondv1_uout, dim3(nb), dim3(256), 0, d.stream, g, d.F.u, d.ctrl);
+  hipLaunchKernelGGL(k_bondv1_outflow, dim3((nplane + 255) / 256), dim3(256), 0, d.stream, g, d.m,
+                     d.F.u, d.F.v, d.F.w, (const Ctrl*)d.ctrl, dt);
+  const int ng = (g.im + 2) * (g.km + g.jm + 3);
+  hipLaunchKernelGGL(k_bondv1_ghost, dim3((ng + 255) / 256), dim3(256), 0, d.stream, g, d.F.u,
+                     d.F.v, d.F.w);
+}
+
+// ------------------------------------------------------------ vel2/velfg
+struct P10 {
+  float* a[10];
+};
+
+// vel2.f95:54-197: diu1..9 and cov1..9 (nou only as a temporary), each over
+// the k range of its reference loop; at k=1 the wall layer of diu3/diu6
+// (vel2.f95:135-141, 170-176) with |U_h| computed in place, and the final
+// uspd=u/|U_h|, vspd=v/|V_h| (vel2.f95:54-65, 351-356).
+__global__ __launch_bounds__(NT) void k_vel2(Grid g, Metrics m, const float* __restrict__ u,
+                                             const float* __restrict__ v,
+                                             const float* __restrict__ w, P10 diu, P10 cov,
+                                             float* __restrict__ uspd, float* __restrict__ vspd,
+                                             float wl) {
+  const int i = 1 + blockIdx.x * BX + threadIdx.x, j = 1 + blockIdx.y * BY + threadIdx.y;
+  if (i > g.im || j > g.jm) return;
+  const float dxi = M1(m.dx1, i), dxp = M1(m.dx1, i + 1), dxm = M1(m.dx1, i - 1);
+  const float dyj = M1(m.dy1, j), dyp = M1(m.dy1, j + 1), dym = M1(m.dy1, j - 1);
+  const int sj = g.sj, sk = g.sk;
+  for (int k = 1 + blockIdx.z; k <= g.km + 1; k += gridDim.z) {
+    const int c = ix(g, i, j, k);
+    if (k <= g.km) {
+      float nou = (u[c - 1] + u[c]) / 2.0f;  // vel2.f95:77-86
+      float d = (-u[c - 1] + u[c]) / dxi;
+      diu.a[1][c] = d;
+      cov.a[1][c] = nou * d;
+      nou = (v[c - sj] + v[c]) / 2.0f;
+      d = (-v[c - sj] + v[c]) / dyj;
+      diu.a[5][c] = d;
+      cov.a[5][c] = nou * d;
+      nou = (w[c - sk] + w[c]) / 2.0f;
+      d = (-w[c - sk] + w[c]) / M1(m.dzn, k);
+      diu.a[9][c] = d;
+      cov.a[9][c] = nou * d;
+      nou = (dxp * v[c - sj] + dxi * v[c + 1 - sj]) / (dxi + dxp);  // vel2.f95:106-108
+      d = 2.0f * (-u[c - sj] + u[c]) / (dym + dyj);
+      diu.a[2][c] = d;
+      cov.a[2][c] = nou * d;
+      nou = (dyp * u[c - 1] + dyj * u[c - 1 + sj]) / (dyj + dyp);  // vel2.f95:147-149
+      d = 2.0f * (-v[c - 1] + v[c]) / (dxm + dxi);
+      diu.a[4][c] = d;
+      cov.a[4][c] = (nou - 0.0f) * d;
+    }
+    if (k >= 2) {
+      const float dzsk = M1(m.dzs, k - 1);
+      float nou = (dxp * w[c - sk] + dxi * w[c + 1 - sk]) / (dxi + dxp);  // vel2.f95:122-124
+      float d = (-u[c - sk] + u[c]) / dzsk;
+      diu.a[3][c] = d;
+      cov.a[3][c] = nou * d;
+      nou = (dyp * w[c - sk] + dyj * w[c + sj - sk]) / (dyj + dyp);  // vel2.f95:157-159
+      d = (-v[c - sk] + v[c]) / dzsk;
+      diu.a[6][c] = d;
+      cov.a[6][c] = nou * d;
+    } else {
+      // |U_h| and |V_h| at k=1 (vel2.f95:54-65)
+      float t = (0.5f * (v[c - sj] + v[c]) * dxp + 0.5f * (v[c + 1 - sj] + v[c + 1]) * dxi) / (dxi + dxp);
+      const float us = sqrtf(u[c] * u[c] + t * t);
+      t = (0.5f * (u[c - 1] + u[c]) * dyp + 0.5f * (u[c - 1 + sj] + u[c + sj]) * dyj) / (dyj + dyp);
+      const float vs = sqrtf(v[c] * v[c] + t * t);
+      const float dz1 = M1(m.dzn, 1);
+      float nou = 0.5f * (dxp * w[c] + dxi * w[c + 1]) / (dxi + dxp);  // vel2.f95:137-139
+      float d = us * 0.4f / wl / (0.5f * dz1) / 0.4f * u[c] / us;
+      diu.a[3][c] = d;
+      cov.a[3][c] = nou * d;
+      nou = 0.5f * (dyp * w[c] + dyj * w[c + sj]) / (dyj + dyp);  // vel2.f95:172-174
+      d = vs * 0.4f / wl / (0.5f * dz1) / 0.4f * v[c] / vs;
+      diu.a[6][c] = d;
+      cov.a[6][c] = nou * d;
+      const int c2 = (i + 1) + (j + 1) * sj;  // vel2.f95:351-356
+      uspd[c2] = u[c] / us;
+      vspd[c2] = v[c] / vs;
+    }
+    if (k <= g.km - 1) {
+      const float dzk = M1(m.dzn, k), dzp = M1(m.dzn, k + 1);
+      float nou = (dzp * u[c - 1] + dzk * u[c - 1 + sk]) / (dzk + dzp);  // vel2.f95:182-184
+      float d = 2.0f * (-w[c - 1] + w[c]) / (dxm + dxi);
+      diu.a[7][c] = d;
+      cov.a[7][c] = (nou - 0.0f) * d;
+      nou = (dzp * v[c - sj] + dzk * v[c - sj + sk]) / (dzk + dzp);  // vel2.f95:192-194
+      d = 2.0f * (-w[c - sj] + w[c]) / (dym + dyj);
+      diu.a[8][c] = d;
+      cov.a[8][c] = nou * d;
+    }
+  }
+}
+
+// vel2.f95:202-286: i=im+1 copies of 1,4,7 and periodic j copies of 2,5,8.
+__global__ __launch_bounds__(256) void k_vel2_ghost1(Grid g, P10 diu, P10 cov) {
+  const int njk = g.jm * g.km, nik = g.im * g.km;
+  int t = blockIdx.x * 256 + threadIdx.x;
+  if (t < njk) {
+    const int j = 1 + t % g.jm, k = 1 + t / g.jm;
+    const int c = ix(g, g.im + 1, j, k);
+    diu.a[1][c] = diu.a[1][c - 1];
+    cov.a[1][c] = cov.a[1][c - 1];
+    diu.a[4][c] = diu.a[4][c - 1];
+    cov.a[4][c] = cov.a[4][c - 1];
+    if (k <= g.km - 1) {
+      diu.a[7][c] = diu.a[7][c - 1];
+      cov.a[7][c] = cov.a[7][c - 1];
+    }
+    return;
+  }
+  t -= njk;
+  if (t >= nik) return;
+  const int i = 1 + t % g.im, k = 1 + t / g.im;
+  const int c0 = ix(g, i, 0, k), cjm = ix(g, i, g.jm, k), c1 = ix(g, i, 1, k), cj1 = ix(g, i, g.jm + 1, k);
+  for (int q = 2; q <= 8; q += 3) {  // 2, 5, 8
+    if (q == 8 && k > g.km - 1) break;
+    diu.a[q][c0] = diu.a[q][cjm];
+    cov.a[q][c0] = cov.a[q][cjm];
+    diu.a[q][cj1] = diu.a[q][c1];
+    cov.a[q][cj1] = cov.a[q][c1];
+  }
+}
+
+// vel2.f95:299-314: diu2,diu3 at i=im+1 (j=1..jm+1, k=1..km+1), then
+// diu4,diu6 at j=0 (i=1..im+1, k=1..km+1).  Reads values the first ghost
+// pass produced or the reference never writes.
+__global__ __launch_bounds__(256) void k_vel2_ghost2(Grid g, P10 diu) {
+  const int nA = (g.jm + 1) * (g.km + 1), nB = (g.im + 1) * (g.km + 1);
+  int t = blockIdx.x * 256 + threadIdx.x;
+  if (t < nA) {
+    const int j = 1 + t % (g.jm + 1), k = 1 + t / (g.jm + 1);
+    const int c = ix(g, g.im + 1, j, k);
+    diu.a[2][c] = diu.a[2][c - 1];
+    diu.a[3][c] = diu.a[3][c - 1];
+    return;
+  }
+  t -= nA;
+  if (t >= nB) return;
+  const int i = 1 + t % (g.im + 1), k = 1 + t / (g.im + 1);
+  const int c0 = ix(g, i, 0, k), cjm = ix(g, i, g.jm, k);
+  diu.a[4][c0] = diu.a[4][cjm];
+  diu.a[6][c0] = diu.a[6][cjm];
+}
+
+// velFG.f95:64-110: f,g for k=1..km and h for k=1..km-1.
+__global__ __launch_bounds__(NT) void k_velfg(Grid g, Metrics m, P10 diu, P10 cov,
+                                              float* __restrict__ f, float* __restrict__ gg,
+                                              float* __restrict__ h, float vn) {
+  const int i = 1 + blockIdx.x * BX + threadIdx.x, j = 1 + blockIdx.y * BY + threadIdx.y;
+  if (i > g.im || j > g.jm) return;
+  const float dxi = M1(m.dx1, i), dxp = M1(m.dx1, i + 1);
+  const float dyj = M1(m.dy1, j), dyp = M1(m.dy1, j + 1);
+  const int sj = g.sj, sk = g.sk;
+  for (int k = 1 + blockIdx.z; k <= g.km; k += gridDim.z) {
+    const int c = ix(g, i, j, k);
+    const float dzk = M1(m.dzn, k);
+    {
+      const float covx1 = (dxp * cov.a[1][c] + dxi * cov.a[1][c + 1]) / (dxi + dxp);
+      const float covy1 = (cov.a[2][c] + cov.a[2][c + sj]) / 2.0f;
+      const float covz1 = (cov.a[3][c] + cov.a[3][c + sk]) / 2.0f;
+      const float covc = covx1 + covy1 + covz1;
+      const float dfu = 2.0f * (-diu.a[1][c] + diu.a[1][c + 1]) / (dxi + dxp) +
+                        (-diu.a[2][c] + diu.a[2][c + sj]) / dyj + (-diu.a[3][c] + diu.a[3][c + sk]) / dzk;
+      const float df = vn * dfu;
+      f[c] = (-covc + df);
+    }
+    {
+      const float covx1 = (cov.a[4][c] + cov.a[4][c + 1]) / 2.0f;
+      const float covy1 = (dyp * cov.a[5][c] + dyj * cov.a[5][c + sj]) / (dyj + dyp);
+      const float covz1 = (cov.a[6][c] + cov.a[6][c + sk]) / 2.0f;
+      const float covc = covx1 + covy1 + covz1;
+      const float dfv = (-diu.a[4][c] + diu.a[4][c + 1]) / dxi +
+                        2.0f * (-diu.a[5][c] + diu.a[5][c + sj]) / (dyj + dyp) +
+                        (-diu.a[6][c] + diu.a[6][c + sk]) / dzk;
+      const float df = vn * dfv;
+      gg[c] = (-covc + df);
+    }
+    if (k <= g.km - 1) {
+      const float dzp = M1(m.dzn, k + 1);
+      const float covx1 = (cov.a[7][c] + cov.a[7][c + 1]) / 2.0f;
+      const float covy1 = (cov.a[8][c] + cov.a[8][c + sj]) / 2.0f;
+      const float covz1 = (dzp * cov.a[9][c] + dzk * cov.a[9][c + sk]) / (dzk + dzp);
+      const float covc = covx1 + covy1 + covz1;
+      const float dfw = (-diu.a[7][c] + diu.a[7][c + 1]) / dxi + (-diu.a[8][c] + diu.a[8][c + sj]) / dyj +
+                        (-diu.a[9][c] + diu.a[9][c + sk]) / M1(m.dzs, k);
+      const float df = vn * dfw;
+      h[c] = (-covc + df);
+    }
+  }
+}
+
+static P10 p10(float* const* a) {
+  P10 r;
+  for (int q = 0; q < 10; q++) r.a[q] = a[q];
+  return r;
+}
+
+void launch_velfg(const DevDomain& d, float vn) {
+  const Grid& g = d.g;
+  const P10 diu = p10(d.F.diu), cov = p10(d.F.cov);
+  hipLaunchKernelGGL(k_vel2, grid2d(g.im, g.jm, g.km + 1), block2d(), 0, d.stream, g, d.m, d.F.u,
+                     d.F.v, d.F.w, diu, cov, d.F.uspd, d.F.vspd, d.wl);
+  const int n1 = g.jm * g.km + g.im * g.km;
+  hipLaunchKernelGGL(k_vel2_ghost1, dim3((n1 + 255) / 256), dim3(256), 0, d.stream, g, diu, cov);
+  const int n2 = (g.jm + 1) * (g.km + 1) + (g.im + 1) * (g.km + 1);
+  hipLaunchKernelGGL(k_vel2_ghost2, dim3((n2 + 255) / 256), dim3(256), 0, d.stream, g, diu);
+  hipLaunchKernelGGL(k_velfg, grid2d(g.im, g.jm, g.km), block2d(), 0, d.stream, g, d.m, diu, cov,
+                     d.F.f, d.F.g, d.F.h, vn);
+}
+
+// ----------------------------------------------------------------- feedbf
+// feedbf.f95:35-61 (fx,fy,fz are dead outputs and are not stored).
+__global__ __launch_bounds__(NT) void k_feedbf(Grid g, const float* __restrict__ u,
+                                               const float* __restrict__ v,
+                                               const float* __restrict__ w,
+                                               const float* __restrict__ bm,
+                                               const float* __restrict__ cm,
+                                               const float* __restrict__ dm, float* __restrict__ us,
+                                               float* __restrict__ vs, float* __restrict__ ws,
+                                               float* __restrict__ f, float* __restrict__ gg,
+                                               float* __restrict__ h, float alpha, float dt,
+                                               float beta) {
+  const int i = 1 + blockIdx.x * BX + threadIdx.x, j = 1 + blockIdx.y * BY + threadIdx.y;
+  if (i > g.im || j > g.jm) return;
+  for (int k = 1 + blockIdx.z; k <= g.km; k += gridDim.z) {
+    const int c = ix(g, i, j, k);
+    const float uc = u[c], vc = v[c], wc = w[c], b = bm[c], cc = cm[c], dd = dm[c];
+    const float su = (us[c] + uc) * b, sv = (vs[c] + vc) * cc, sw = (ws[c] + wc) * dd;
+    us[c] = su;
+    vs[c] = sv;
+    ws[c] = sw;
+    const float fx = alpha * su * dt + beta * uc * b;
+    const float fy = alpha * sv * dt + beta * vc * cc;
+    const float fz = alpha * sw * dt + beta * wc * dd;
+    f[c] = f[c] + fx;
+    gg[c] = gg[c] + fy;
+    h[c] = h[c] + fz;
+  }
+}
+
+void launch_feedbf(const DevDomain& d, float alpha, float dt, float beta) {
+  const Grid& g = d.g;
+  hipLaunchKernelGGL(k_feedbf, grid2d(g.im, g.jm, g.km), block2d(), 0, d.stream, g, d.F.u, d.F.v,
+                     d.F.w, d.F.bmask, d.F.cmask, d.F.dmask, d.F.usum, d.F.vsum, d.F.wsum, d.F.f,
+                     d.F.g, d.F.h, alpha, dt, beta);
+}
+
+// -------------------------------------------------------------------- les
+// Eddy viscosity sm (les.f95:45-65); delx1 arrives precomputed per level.
+__global__ __launch_bounds__(NT) void k_les_sm(Grid g, P10 diu, const float* __restrict__ delx1,
+                                               float* __restrict__ sm) {
+  const int i = 1 + blockIdx.x * BX + threadIdx.x, j = 1 + blockIdx.y * BY + threadIdx.y;
+  if (i > g.im || j > g.jm) return;
+  const int sj = g.sj, sk = g.sk;
+  const float* d2 = diu.a[2];
+  const float* d3 = diu.a[3];
+  const float* d4 = diu.a[4];
+  const float* d6 = diu.a[6];
+  const float* d7 = diu.a[7];
+  const float* d8 = diu.a[8];
+  for (int k = 1 + blockIdx.z; k <= g.km; k += gridDim.z) {
+    const int c = ix(g, i, j, k);
+    const float dudxx1 = diu.a[1][c];
+    const float dudyx1 = (d2[c - 1] + d2[c - 1 + sj] + d2[c] + d2[c + sj]) * 0.25f;
+    const float dudzx1 = (d3[c - 1] + d3[c - 1 + sk] + d3[c] + d3[c + sk]) * 0.25f;
+    const float dvdxx1 = (d4[c] + d4[c - sj] + d4[c + 1] + d4[c + 1 - sj]) * 0.25f;
+    const float dvdyx1 = diu.a[5][c];
+    const float dvdzx1 = (d6[c - sj] + d6[c - sj + sk] + d6[c] + d6[c + sk]) * 0.25f;
+    const float dwdxx1 = (d7[c] + d7[c - sk] + d7[c + 1] + d7[c + 1 - sk]) * 0.25f;
+    const float dwdyx1 = (d8[c] + d8[c - sk] + d8[c + sj] + d8[c + sj - sk]) * 0.25f;
+    const float dwdzx1 = diu.a[9][c];
+    const float a = 0.1f * delx1[k];
+    const float s = 2.0f * (dudxx1 * dudxx1 + dvdyx1 * dvdyx1 + dwdzx1 * dwdzx1) +
+                    (dudyx1 + dvdxx1) * (dudyx1 + dvdxx1) + (dwdyx1 + dvdzx1) * (dwdyx1 + dvdzx1) +
+                    (dudzx1 + dwdxx1) * (dudzx1 + dwdxx1);
+    sm[c] = a * a * sqrtf(s);
+  }
+}
+
+// boundsm.f95:18-64 as one pass: the serial build runs the i-Neumann
+// copies (j=-1..jm+1, k=0..km+1), then the j-Neumann copies (i=0..im+1,
+// k=0..km+1), then sm(k=0)=-sm(1), sm(km+1)=sm(km) (j=-1..jm+1,
+// i=0..im+1).  Each written cell receives the composite of those copies,
+// whose source is always a cell none of them writes.
+__device__ __forceinline__ bool sm_written(const Grid& g, int i, int j, int k) {
+  const bool ib = (i == 0 || i == g.im + 1), jb = (j == 0 || j == g.jm + 1), kb = (k == 0 || k == g.km + 1);
+  const bool iin = (i >= 0 && i <= g.im + 1), jin = (j >= -1 && j <= g.jm + 1), kin = (k >= 0 && k <= g.km + 1);
+  return (ib && jin && kin) || (jb && iin && kin) || (kb && jin && iin);
+}
+
+__global__ __launch_bounds__(256) void k_boundsm(Grid g, float* __restrict__ sm) {
+  // enumerate the box i=0..im+1, j=-1..jm+1, k=0..km+1 face by face:
+  // k faces (2 planes), then j faces (rows j=-1,0,jm+1 for k=1..km), then
+  // i faces (i=0,im+1 for j=1..jm, k=1..km)
+  const int ni = g.im + 2, nj = g.jm + 3;
+  const int nK = 2 * ni * nj, nJ = 3 * ni * g.km, nI = 2 * g.jm * g.km;
+  int t = blockIdx.x * 256 + threadIdx.x;
+  int i, j, k;
+  if (t < nK) {
+    i = t % ni;
+    j = -1 + (t / ni) % nj;
+    k = (t / (ni * nj)) ? g.km + 1 : 0;
+  } else if ((t -= nK) < nJ) {
+    i = t % ni;
+    const int r = (t / ni) % 3;
+    j = (r == 0) ? -1 : ((r == 1) ? 0 : g.jm + 1);
+    k = 1 + t / (ni * 3);
+  } else if ((t -= nJ) < nI) {
+    i = (t & 1) ? g.im + 1 : 0;
+    j = 1 + (t >> 1) % g.jm;
+    k = 1 + (t >> 1) / g.jm;
+  } else {
+    return;
+  }
+  if (!sm_written(g, i, j, k)) return;
+  float sgn = 1.0f;
+  int kk = k;
+  if (k == 0) {
+    kk = 1;
+    sgn = -1.0f;
+  } else if (k == g.km + 1) {
+    kk = g.km;
+  }
+  int jj = j;
+  if (j == 0) jj = 1;
+  else if (j == g.jm + 1) jj = g.jm;
+  int ii = i;
+  if (i == 0) ii = 1;
+  else if (i == g.im + 1) ii = g.im;
+  const float s = sm[ix(g, ii, jj, kk)];
+  sm[ix(g, i, j, k)] = (sgn < 0.0f) ? -s : s;
+}
+
+// Viscous terms added to f (k=2..km, wall k=1), g (same) and h (k=1..km),
+// les.f95:83-242.
+__global__ __launch_bounds__(NT) void k_les_visc(Grid g, Metrics m, P10 diu,
+                                                 const float* __restrict__ sm,
+                                                 const float* __restrict__ uspd,
+                                                 const float* __restrict__ vspd,
+                                                 float* __restrict__ f, float* __restrict__ gg,
+                                                 float* __restrict__ h, float wl) {
+  const int i = 1 + blockIdx.x * BX + threadIdx.x, j = 1 + blockIdx.y * BY + threadIdx.y;
+  if (i > g.im || j > g.jm) return;
+  const int sj = g.sj, sk = g.sk;
+  const float dxi = M1(m.dx1, i), dxp = M1(m.dx1, i + 1), dxm = M1(m.dx1, i - 1);
+  const float dyj = M1(m.dy1, j), dyp = M1(m.dy1, j + 1), dym = M1(m.dy1, j - 1);
+  const float dxs = M1(m.dxs, i), dys = M1(m.dys, j);
+  const float* d1 = diu.a[1];
+  const float* d2 = diu.a[2];
+  const float* d3 = diu.a[3];
+  const float* d4 = diu.a[4];
+  const float* d5 = diu.a[5];
+  const float* d6 = diu.a[6];
+  const float* d7 = diu.a[7];
+  const float* d8 = diu.a[8];
+  const float* d9 = diu.a[9];
+#define SMX(c) ((dxp * sm[(c)] + dxi * sm[(c) + 1]) / (dxi + dxp))      // x-interp at i+1/2
+#define SMXM(c) ((dxi * sm[(c) - 1] + dxm * sm[(c)]) / (dxm + dxi))     // x-interp at i-1/2
+#define SMY(c) ((dyp * sm[(c)] + dyj * sm[(c) + sj]) / (dyj + dyp))     // y-interp at j+1/2
+#define SMYM(c) ((dyj * sm[(c) - sj] + dym * sm[(c)]) / (dym + dyj))    // y-interp at j-1/2
+  for (int k = 1 + blockIdx.z; k <= g.km; k += gridDim.z) {
+    const int c = ix(g, i, j, k);
+    const float dzk = M1(m.dzn, k), dzp = M1(m.dzn, k + 1), dzm = M1(m.dzn, k - 1);
+    // ---- f (les.f95:83-114; wall les.f95:118-139)
+    {
+      const float evsx2 = sm[c + 1];
+      const float evsx1 = sm[c];
+      const float evsy2 = (dyp * SMX(c) + dyj * SMX(c + sj)) / (dyj + dyp);
+      const float evsy1 = (dyp * SMX(c - sj) + dyj * SMX(c)) / (dyj + dyp);
+      const float evsz2 = (dzp * SMX(c) + dzk * SMX(c + sk)) / (dzk + dzp);
+      const float visux2 = (evsx2) * 2.0f * d1[c + 1];
+      const float visux1 = (evsx1) * 2.0f * d1[c];
+      const float visuy2 = (evsy2) * (d2[c + sj] + d4[c + 1]);
+      const float visuy1 = (evsy1) * (d2[c] + d4[c + 1 - sj]);
+      const float visuz2 = (evsz2) * (d3[c + sk] + d7[c + 1]);
+      float vfu;
+      if (k >= 2) {
+        const float evsz1 = (dzk * SMX(c - sk) + dzm * SMX(c)) / (dzm + dzk);
+        const float visuz1 = (evsz1) * (d3[c] + d7[c + 1 - sk]);
+        vfu = (visux2 - visux1) / dxs + (visuy2 - visuy1) / dyj + (visuz2 - visuz1) / dzk;
+      } else {
+        const float t = 0.4f * uspd[(i + 1) + (j + 1) * sj] / wl;
+        const float visuz1 = t * t * uspd[(i + 1) + (j + 1) * sj];
+        vfu = (visux2 - visux1) / dxs + (visuy2 - visuy1) / dyj + (visuz2 - visuz1) / dzk +
+              (visuy2 - visuy1) / dyj + (visuz2 - visuz1) / dzk;
+      }
+      f[c] = (f[c] + vfu);
+    }
+    // ---- g (les.f95:143-174; wall les.f95:178-201)
+    {
+      const float evsy2 = sm[c + sj];
+      const float evsy1 = sm[c];
+      const float evsx2 = (dyp * SMX(c) + dyj * SMX(c + sj)) / (dyj + dyp);
+      const float evsx1 = (dyp * SMXM(c) + dyj * SMXM(c + sj)) / (dyj + dyp);
+      const float evsz2 = (dzp * SMX(c) + dzk * SMX(c + sk)) / (dzk + dzp);
+      const float visvx2 = (evsx2) * (d2[c + sj] + d4[c + 1]);
+      const float visvx1 = (evsx1) * (d2[c - 1 + sj] + d4[c]);
+      const float visvy2 = (evsy2) * 2.0f * d5[c + sj];
+      const float visvy1 = (evsy1) * 2.0f * d5[c];
+      const float visvz2 = (evsz2) * (d6[c + sk] + d8[c + sj]);
+      float visvz1;
+      if (k >= 2) {
+        const float evsz1 = (dzk * SMX(c - sk) + dzm * SMX(c)) / (dzm + dzk);
+        visvz1 = (evsz1) * (d6[c] + d8[c + sj - sk]);
+      } else {
+        const float t = 0.4f * vspd[(i + 1) + (j + 1) * sj] / wl;
+        visvz1 = t * t * vspd[(i + 1) + (j + 1) * sj];
+      }
+      const float vfv = (visvx2 - visvx1) / dxi + (visvy2 - visvy1) / dys + (visvz2 - visvz1) / dzk;
+      gg[c] = (gg[c] + vfv);
+    }
+    // ---- h (les.f95:206-242)
+    {
+      const float evsz2 = sm[c + sk];
+      const float evsz1 = sm[c];
+      const float evsx2 = (dzp * SMX(c) + dzk * SMX(c + sk)) / (dzk + dzp);
+      const float evsx1 = (dzp * SMXM(c) + dzk * SMXM(c + sk)) / (dzk + dzp);
+      const float evsy2 = (dzp * SMY(c) + dzk * SMY(c + sk)) / (dzk + dzp);
+      const float evsy1 = (dzp * SMYM(c) + dzk * SMYM(c + sk)) / (dzk + dzp);
+      const float viswx2 = (evsx2) * (d3[c + sk] + d7[c + 1]);
+      const float viswx1 = (evsx1) * (d3[c - 1 + sk] + d7[c]);
+      const float viswy2 = (evsy2) * (d6[c + sk] + d8[c + sj]);
+      const float viswy1 = (evsy1) * (d6[c - sj + sk] + d8[c]);
+      const float viswz2 = (evsz2) * 2.0f * d9[c + sk];
+      const float viswz1 = (evsz1) * 2.0f * d9[c];
+      const float vfw = (viswx2 - viswx1) / dxi + (viswy2 - viswy1) / dyj + (viswz2 - viswz1) / dzk;
+      h[c] = (h[c] + vfw);
+    }
+  }
+#undef SMX
+#undef SMXM
+#undef SMY
+#undef SMYM
+}
+
+void launch_les(const DevDomain& d) {
+  const Grid& g = d.g;
+  const P10 diu = p10(d.F.diu);
+  hipLaunchKernelGGL(k_les_sm, grid2d(g.im, g.jm, g.km), block2d(), 0, d.stream, g, diu,
+                     d.m.delx1 + 1, d.F.sm);
+  const int ni = g.im + 2, nj = g.jm + 3;
+  const int nb = 2 * ni * nj + 3 * ni * g.km + 2 * g.jm * g.km;
+  hipLaunchKernelGGL(k_boundsm, dim3((nb + 255) / 256), dim3(256), 0, d.stream, g, d.F.sm);
+  hipLaunchKernelGGL(k_les_visc, grid2d(g.im, g.jm, g.km), block2d(), 0, d.stream, g, d.m, diu,
+                     d.F.sm, d.F.uspd, d.F.vspd, d.F.f, d.F.g, d.F.h, d.wl);
+}
+
+// ------------------------------------------------------------------- adam
+// adam.f95:30-44.
+__global__ __launch_bounds__(NT) void k_adam(Grid g, float* __restrict__ f, float* __restrict__ gg,
+                                             float* __restrict__ h, float* __restrict__ fo,
+                                             float* __restrict__ go, float* __restrict__ ho) {
+  const int i = 1 + blockIdx.x * BX + threadIdx.x, j = 1 + blockIdx.y * BY + threadIdx.y;
+  if (i > g.im || j > g.jm) return;
+  for (int k = 1 + blockIdx.z; k <= g.km; k += gridDim.z) {
+    const int c = ix(g, i, j, k);
+    const float fd = f[c], gd = gg[c], hd = h[c];
+    f[c] = 1.5f * fd - 0.5f * fo[c];
+    gg[c] = 1.5f * gd - 0.5f * go[c];
+    h[c] = 1.5f * hd - 0.5f * ho[c];
+    fo[c] = fd;
+    go[c] = gd;
+    ho[c] = hd;
+  }
+}
+
+void launch_adam(const DevDomain& d) {
+  const Grid& g = d.g;
+  hipLaunchKernelGGL(k_adam, grid2d(g.im, g.jm, g.km), block2d(), 0, d.stream, g, d.F.f, d.F.g,
+                     d.F.h, d.F.fold, d.F.gold, d.F.hold);
+}
+
+// ------------------------------------------------------ layout conversion
+__global__ __launch_bounds__(256) void k_unpack(Grid g, const float* __restrict__ src,
+                                                float* __restrict__ dst, int li, int lj, int lk,
+                                                int ni, int nj, int nk) {
+  const long n = (long)ni * nj * nk;
+  for (long t = blockIdx.x * 256L + threadIdx.x; t < n; t += (long)gridDim.x * 256) {
+    const int a = (int)(t % ni), b = (int)((t / ni) % nj), c = (int)(t / ((long)ni * nj));
+    dst[ix(g, li + a, lj + b, lk + c)] = src[t];
+  }
+}
+
+__global__ __launch_bounds__(256) void k_pack(Grid g, const float* __restrict__ src,
+                                              float* __restrict__ dst, int li, int lj, int lk,
+                                              int ni, int nj, int nk) {
+  const long n = (long)ni * nj * nk;
+  for (long t = blockIdx.x * 256L + threadIdx.x; t < n; t += (long)gridDim.x * 256) {
+    const int a = (int)(t % ni), b = (int)((t / ni) % nj), c = (int)(t / ((long)ni * nj));
+    dst[t] = src[ix(g, li + a, lj + b, lk + c)];
+  }
+}
+
+void launch_unpack(const DevDomain& d, const float* src, float* dst, int li, int lj, int lk, int ni,
+                   int nj, int nk) {
+  const long n = (long)ni * nj * nk;
+  long nb = (n + 255) / 256;
+  if (nb > 65536) nb = 65536;
+  hipLaunchKernelGGL(k_unpack, dim3((unsigned)nb), dim3(256), 0, d.stream, d.g, src, dst, li, lj, lk,
+                     ni, nj, nk);
+}
+
+void launch_pack(const DevDomain& d, const float* src, float* dst, int li, int lj, int lk, int ni,
+                 int nj, int nk) {
+  const long n = (long)ni * nj * nk;
+  long nb = (n + 255) / 256;
+  if (nb > 65536) nb = 65536;
+  hipLaunchKernelGGL(k_pack, dim3((unsigned)nb), dim3(256), 0, d.stream, d.g, src, dst, li, lj, lk,
+                     ni, nj, nk);
+}
+
+}  // namespace les

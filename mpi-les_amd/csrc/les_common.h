@@ -1,0 +1,106 @@
+// Shared device-side definitions of the LES step library (gfx950).
+//
+// Device layout: every 3-D field of a domain uses one layout that covers
+// i,j,k = -1 .. n+2 in each direction, which contains every reference
+// extent (u,v: (0:ip+1,-1:jp+1,0:kp+1); w: (0:ip+1,-1:jp+1,-1:kp+1);
+// p: (0:ip+2,0:jp+2,0:kp+1); diu1: (-1:ip+2,0:jp+2,0:kp+2); sm:
+// (-1:ip+1,-1:jp+1,0:kp+1); ... main.f95:63-189).  i is fastest and the row
+// pitch sj is padded to a multiple of 64 floats (256 B).  Cells a reference
+// array does not have are never read by the kernels.  1-D metrics are
+// stored with index x at slot x+1.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace les {
+
+struct Grid {
+  int im, jm, km;
+  int sj;  // floats between consecutive j rows
+  int sk;  // floats between consecutive k planes
+};
+
+__host__ __device__ __forceinline__ int ix(const Grid& g, int i, int j, int k) {
+  return (i + 1) + (j + 1) * g.sj + (k + 1) * g.sk;
+}
+
+// 1-D metrics (slot x+1 holds index x).  Pointers into one device buffer.
+struct Metrics {
+  const float* dx1;     // dx1(-1:ip+1)
+  const float* dy1;     // dy1(0:jp+1)
+  const float* dzn;     // dzn(-1:kp+2)
+  const float* dzs;     // dzs(-1:kp+2)
+  const float* dxs;     // dxs(0:ip)
+  const float* dys;     // dys(0:jp)
+  const float* cn2l;    // cn2l(1:ip)
+  const float* cn2s;
+  const float* cn3l;    // cn3l(1:jp)
+  const float* cn3s;
+  const float* cn4l;    // cn4l(1:kp)
+  const float* cn4s;
+  const float* delx1;   // delx1(1:kp), (dx1(0)*dy1(0)*dzn(k))**(1/3), les.f95:38-42
+  const float* inflow;  // u_in(j,k) at (j-1)+(k-1)*jm, nullptr = 5 m/s
+};
+#define M1(a, x) ((a)[(x) + 1])
+
+constexpr int kSorHist = 1024;
+
+// Per-domain device scalars of the reductions and the SOR loop.
+struct Ctrl {
+  unsigned int ticket;     // last-workgroup counter of the running reduction
+  unsigned int uout_max;   // bits of max(0, max u(im,:,:))   (>= +0)
+  unsigned int uout_nmin;  // bits of -min(0, min u(im,:,:))  (>= +0)
+  int sor_done;            // SOR iteration (1-based) whose residual fell below pjuge, 0 = none
+  int sor_iters;           // iterations executed in the current press call
+  int nonfinite;           // a non-finite residual was met
+  double rhsav;            // volume-weighted mean of rhs
+  double pav;              // volume-weighted mean of p
+  double sor_first;        // sum of reltmp^2 of the current iteration's first colour
+  double sor_hist[kSorHist];
+};
+
+constexpr int kWave = 64;
+
+// Deterministic sum over a block of NT threads (tid = linear thread id):
+// wavefront butterfly, then wave totals through LDS added by thread 0 in
+// wave order.  The result is valid in thread 0.
+template <int NT>
+__device__ __forceinline__ double block_sum(double v, int tid, double* lds) {
+  static_assert(NT % kWave == 0, "block must be whole wavefronts");
+  for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+  if (tid % kWave == 0) lds[tid / kWave] = v;
+  __syncthreads();
+  double s = 0.0;
+  if (tid == 0)
+    for (int w = 0; w < NT / kWave; w++) s += lds[w];
+  __syncthreads();
+  return s;
+}
+
+// Finish a grid-wide deterministic sum: every block stores its partial,
+// the last block to arrive (ticket) adds all partials in index order.
+// Returns true in thread 0 of the last block, with *total set.
+template <int NT>
+__device__ __forceinline__ bool grid_sum(double block_total, int tid, int bid, int nblocks,
+                                         double* partials, unsigned int* ticket, double* total,
+                                         double* lds) {
+  __shared__ int last;
+  if (tid == 0) {
+    __hip_atomic_store(&partials[bid], block_total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    unsigned int t = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    last = (t == (unsigned int)nblocks - 1u);
+  }
+  __syncthreads();
+  if (!last) return false;
+  double s = 0.0;
+  for (int b = tid; b < nblocks; b += NT)
+    s += __hip_atomic_load(&partials[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  s = block_sum<NT>(s, tid, lds);
+  if (tid == 0) {
+    *total = s;
+    __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  return tid == 0;
+}
+
+}  // namespace les

@@ -1,0 +1,750 @@
+// Host runtime of the LES step library: device domains, the mirrors of
+// host arrays handed to the reference-signature entry points, layout
+// conversion, and the C ABI declared in include/les_amd.h.
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <map>
+#include <mutex>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../include/les_amd.h"
+#include "les_kernels.h"
+
+using namespace les;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(const std::string& msg) {
+  g_err = msg;
+  return 1;
+}
+
+#define HIP_TRY(expr)                                                              \
+  do {                                                                             \
+    hipError_t _e = (expr);                                                        \
+    if (_e != hipSuccess)                                                          \
+      return fail(std::string(#expr) + ": " + hipGetErrorString(_e));              \
+  } while (0)
+
+// ------------------------------------------------------------ field table
+enum Kind { KU, KW, KP, KF, KI, KR, KS, KD1, KD, KB, K2D };
+
+struct KindInfo {
+  int li, lj, lk;    // reference lower bounds
+  int ei, ej, ek;    // extents minus (im, jm, km)
+};
+// main.f95:63-189 and the callee dummy declarations
+const KindInfo kKinds[] = {
+    {0, -1, 0, 2, 3, 2},    // KU  u, v, cmask1   (0:ip+1,-1:jp+1,0:kp+1)
+    {0, -1, -1, 2, 3, 3},   // KW  w              (0:ip+1,-1:jp+1,-1:kp+1)
+    {0, 0, 0, 3, 3, 2},     // KP  p              (0:ip+2,0:jp+2,0:kp+1)
+    {0, 0, 0, 1, 1, 1},     // KF  f,g,h,usum..   (0:ip,0:jp,0:kp)
+    {1, 1, 1, 0, 0, 0},     // KI  fold..,cn1     (ip,jp,kp)
+    {0, 0, 0, 2, 2, 2},     // KR  rhs, dmask1    (0:ip+1,0:jp+1,0:kp+1)
+    {-1, -1, 0, 3, 3, 2},   // KS  sm             (-1:ip+1,-1:jp+1,0:kp+1)
+    {-1, 0, 0, 4, 3, 3},    // KD1 diu1, diu5     (-1:ip+2,0:jp+2,0:kp+2)
+    {0, 0, 0, 3, 3, 3},     // KD  other diu      (0:ip+2,0:jp+2,0:kp+2)
+    {-1, 0, 0, 3, 2, 2},    // KB  bmask1         (-1:ip+1,0:jp+1,0:kp+1)
+    {0, 0, -1, 2, 2, -1},   // K2D uspd, vspd     (0:ip+1,0:jp+1), stored in plane k=-1
+};
+
+enum Field {
+  FU, FV, FW, FP, FF, FG, FH, FUSUM, FVSUM, FWSUM, FFOLD, FGOLD, FHOLD, FRHS, FSM, FUSPD, FVSPD,
+  FDIU1, FDIU2, FDIU3, FDIU4, FDIU5, FDIU6, FDIU7, FDIU8, FDIU9, FBMASK, FCMASK, FDMASK, FCN1,
+  NFIELDS
+};
+struct FieldInfo {
+  const char* name;
+  Kind kind;
+};
+const FieldInfo kFields[NFIELDS] = {
+    {"u", KU},       {"v", KU},       {"w", KW},       {"p", KP},       {"f", KF},
+    {"g", KF},       {"h", KF},       {"usum", KF},    {"vsum", KF},    {"wsum", KF},
+    {"fold", KI},    {"gold", KI},    {"hold", KI},    {"rhs", KR},     {"sm", KS},
+    {"uspd", K2D},   {"vspd", K2D},   {"diu1", KD1},   {"diu2", KD},    {"diu3", KD},
+    {"diu4", KD},    {"diu5", KD1},   {"diu6", KD},    {"diu7", KD},    {"diu8", KD},
+    {"diu9", KD},    {"bmask1", KB},  {"cmask1", KU},  {"dmask1", KR},  {"cn1", KI},
+};
+
+int field_by_name(const char* n) {
+  for (int f = 0; f < NFIELDS; f++)
+    if (strcmp(kFields[f].name, n) == 0) return f;
+  return -1;
+}
+
+// 1-D metrics: reference lower bound, length minus n, which n (0 i, 1 j, 2 k)
+enum Metric { MDX1, MDY1, MDZN, MDZS, MDXS, MDYS, MCN2L, MCN2S, MCN3L, MCN3S, MCN4L, MCN4S, MDELX1, NMETRICS };
+struct MetricInfo {
+  int lb, elen, axis;
+};
+const MetricInfo kMetrics[NMETRICS] = {
+    {-1, 3, 0},  // dx1(-1:ip+1)
+    {0, 2, 1},   // dy1(0:jp+1)
+    {-1, 4, 2},  // dzn(-1:kp+2)
+    {-1, 4, 2},  // dzs(-1:kp+2)
+    {0, 1, 0},   // dxs(0:ip)
+    {0, 1, 1},   // dys(0:jp)
+    {1, 0, 0},   // cn2l(ip)
+    {1, 0, 0},   // cn2s(ip)
+    {1, 0, 1},   // cn3l(jp)
+    {1, 0, 1},   // cn3s(jp)
+    {1, 0, 2},   // cn4l(kp)
+    {1, 0, 2},   // cn4s(kp)
+    {1, 0, 2},   // delx1(kp)
+};
+
+}  // namespace
+
+struct les_domain {
+  DevDomain d{};
+  int n3 = 0;                       // floats per 3-D field
+  float* fields[NFIELDS] = {};
+  float* cov[10] = {};
+  float* mbuf = nullptr;            // metric pack on device
+  float* inflow_dev = nullptr;
+  std::vector<float> mhost;         // metric pack on host
+  int moff[NMETRICS] = {};          // offset of each metric in the pack
+  bool mdirty = true;
+  bool have_inflow = false;
+  float* staging = nullptr;         // reference-layout staging buffer
+  size_t staging_floats = 0;
+  const void* bound[NFIELDS] = {};  // host address each field mirrors (reference-signature layer)
+  float ro = 1.1763f, vn = 0.0f, alpha = -10.0f, beta = -1.0f, dt = 0.05f;
+  long long bytes = 0;
+};
+
+namespace {
+
+int axis_n(const les_domain* D, int axis) { return axis == 0 ? D->d.g.im : (axis == 1 ? D->d.g.jm : D->d.g.km); }
+
+void ref_extent(const les_domain* D, Kind k, int* ni, int* nj, int* nk) {
+  const KindInfo& K = kKinds[k];
+  *ni = D->d.g.im + K.ei;
+  *nj = D->d.g.jm + K.ej;
+  *nk = (k == K2D) ? 1 : D->d.g.km + K.ek;
+}
+
+size_t ref_floats(const les_domain* D, Kind k) {
+  int a, b, c;
+  ref_extent(D, k, &a, &b, &c);
+  return (size_t)a * b * c;
+}
+
+int ensure_staging(les_domain* D, size_t n) {
+  if (D->staging_floats >= n) return 0;
+  if (D->staging) hipFree(D->staging);
+  D->staging = nullptr;
+  HIP_TRY(hipMalloc(&D->staging, n * sizeof(float)));
+  D->staging_floats = n;
+  return 0;
+}
+
+int upload_field(les_domain* D, int f, const float* host) {
+  const Kind k = kFields[f].kind;
+  int ni, nj, nk;
+  ref_extent(D, k, &ni, &nj, &nk);
+  const size_t n = (size_t)ni * nj * nk;
+  if (ensure_staging(D, n)) return 1;
+  HIP_TRY(hipMemcpyAsync(D->staging, host, n * sizeof(float), hipMemcpyHostToDevice, D->d.stream));
+  const KindInfo& K = kKinds[k];
+  launch_unpack(D->d, D->staging, D->fields[f], K.li, K.lj, K.lk, ni, nj, nk);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(D->d.stream));  // the host array may change after return
+  return 0;
+}
+
+int download_field(les_domain* D, int f, float* host) {
+  const Kind k = kFields[f].kind;
+  int ni, nj, nk;
+  ref_extent(D, k, &ni, &nj, &nk);
+  const size_t n = (size_t)ni * nj * nk;
+  if (ensure_staging(D, n)) return 1;
+  const KindInfo& K = kKinds[k];
+  launch_pack(D->d, D->fields[f], D->staging, K.li, K.lj, K.lk, ni, nj, nk);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(host, D->staging, n * sizeof(float), hipMemcpyDeviceToHost, D->d.stream));
+  HIP_TRY(hipStreamSynchronize(D->d.stream));
+  return 0;
+}
+
+// Copy a reference-layout 1-D metric into the host pack; marks the pack
+// dirty when a value changed.
+void set_metric(les_domain* D, int mi, const float* host) {
+  if (!host) return;
+  const MetricInfo& M = kMetrics[mi];
+  const int len = axis_n(D, M.axis) + M.elen;
+  float* dst = D->mhost.data() + D->moff[mi] + (M.lb + 1);  // slot x+1 holds index x
+  if (memcmp(dst, host, len * sizeof(float)) != 0) {
+    memcpy(dst, host, len * sizeof(float));
+    D->mdirty = true;
+  }
+}
+
+float metric_at(const les_domain* D, int mi, int x) { return D->mhost[D->moff[mi] + x + 1]; }
+
+// Host-side constants derived from the metrics: delx1 (les.f95:38-42),
+// the wall-function log (vel2.f95:138) and sum(dx1*dy1*dzn) (press.f95:63-72,
+// accumulated in FP64 in i-fastest order).
+void derive(les_domain* D) {
+  const Grid& g = D->d.g;
+  const float dx0 = metric_at(D, MDX1, 0), dy0 = metric_at(D, MDY1, 0);
+  for (int k = 1; k <= g.km; k++) {
+    const float v = powf(dx0 * dy0 * metric_at(D, MDZN, k), 1.0f / 3.0f);
+    D->mhost[D->moff[MDELX1] + k + 1] = v;
+  }
+  D->d.wl = logf(0.5f * metric_at(D, MDZN, 1) / 0.1f);
+  double area = 0.0;
+  for (int k = 1; k <= g.km; k++) {
+    const float dz = metric_at(D, MDZN, k);
+    for (int j = 1; j <= g.jm; j++) {
+      const float dy = metric_at(D, MDY1, j);
+      for (int i = 1; i <= g.im; i++) area = area + (double)(metric_at(D, MDX1, i) * dy * dz);
+    }
+  }
+  D->d.area = area;
+}
+
+int flush_metrics(les_domain* D) {
+  if (!D->mdirty) return 0;
+  derive(D);
+  HIP_TRY(hipMemcpyAsync(D->mbuf, D->mhost.data(), D->mhost.size() * sizeof(float),
+                         hipMemcpyHostToDevice, D->d.stream));
+  HIP_TRY(hipStreamSynchronize(D->d.stream));
+  D->mdirty = false;
+  return 0;
+}
+
+int set_inflow(les_domain* D, const float* prof) {
+  const Grid& g = D->d.g;
+  if (!prof) {
+    D->have_inflow = false;
+    D->d.m.inflow = nullptr;
+    return 0;
+  }
+  if (!D->inflow_dev) HIP_TRY(hipMalloc(&D->inflow_dev, (size_t)g.jm * g.km * sizeof(float)));
+  HIP_TRY(hipMemcpy(D->inflow_dev, prof, (size_t)g.jm * g.km * sizeof(float), hipMemcpyHostToDevice));
+  D->have_inflow = true;
+  D->d.m.inflow = D->inflow_dev;
+  return 0;
+}
+
+int nmaxp_default = 50;
+float pjuge_default = 1.0e-4f;
+float omega_default = 1.0f;
+
+void domain_free(les_domain* D) {
+  if (!D) return;
+  for (int f = 0; f < NFIELDS; f++)
+    if (D->fields[f]) hipFree(D->fields[f]);
+  for (int q = 1; q <= 9; q++)
+    if (D->cov[q]) hipFree(D->cov[q]);
+  if (D->mbuf) hipFree(D->mbuf);
+  if (D->inflow_dev) hipFree(D->inflow_dev);
+  if (D->staging) hipFree(D->staging);
+  if (D->d.ctrl) hipFree(D->d.ctrl);
+  if (D->d.partials) hipFree(D->d.partials);
+  if (D->d.stream) hipStreamDestroy(D->d.stream);
+  delete D;
+}
+
+int domain_alloc(int im, int jm, int km, les_domain** out) {
+  if (im < 2 || jm < 2 || km < 2) return fail("domain extents must be at least 2 in every direction");
+  const long sj = ((long)im + 4 + 63) / 64 * 64;
+  const long sk = sj * (jm + 4);
+  const long n3 = sk * (km + 4);
+  if (n3 >= (1L << 31)) return fail("domain too large for 32-bit field indexing");
+  les_domain* D = new les_domain();
+  Grid& g = D->d.g;
+  g.im = im;
+  g.jm = jm;
+  g.km = km;
+  g.sj = (int)sj;
+  g.sk = (int)sk;
+  D->n3 = (int)n3;
+  auto bail = [&](const std::string& m) {
+    domain_free(D);
+    return fail(m);
+  };
+  hipError_t e = hipStreamCreateWithFlags(&D->d.stream, hipStreamNonBlocking);
+  if (e != hipSuccess) return bail(std::string("hipStreamCreate: ") + hipGetErrorString(e));
+  for (int f = 0; f < NFIELDS; f++) {
+    const size_t nb = (kFields[f].kind == K2D ? (size_t)sk : (size_t)n3) * sizeof(float);
+    e = hipMalloc(&D->fields[f], nb);
+    if (e == hipSuccess) e = hipMemset(D->fields[f], 0, nb);
+    if (e != hipSuccess) return bail(std::string("field allocation: ") + hipGetErrorString(e));
+    D->bytes += nb;
+  }
+  for (int q = 1; q <= 9; q++) {
+    const size_t nb = (size_t)n3 * sizeof(float);
+    e = hipMalloc(&D->cov[q], nb);
+    if (e == hipSuccess) e = hipMemset(D->cov[q], 0, nb);
+    if (e != hipSuccess) return bail(std::string("scratch allocation: ") + hipGetErrorString(e));
+    D->bytes += nb;
+  }
+  // metric pack: each array gets n+4 slots (indices -1..n+2)
+  int off = 0;
+  for (int mi = 0; mi < NMETRICS; mi++) {
+    D->moff[mi] = off;
+    off += axis_n(D, kMetrics[mi].axis) + 4;
+  }
+  D->mhost.assign(off, 0.0f);
+  e = hipMalloc(&D->mbuf, off * sizeof(float));
+  if (e != hipSuccess) return bail(std::string("metric allocation: ") + hipGetErrorString(e));
+  Metrics& m = D->d.m;
+  m.dx1 = D->mbuf + D->moff[MDX1];
+  m.dy1 = D->mbuf + D->moff[MDY1];
+  m.dzn = D->mbuf + D->moff[MDZN];
+  m.dzs = D->mbuf + D->moff[MDZS];
+  m.dxs = D->mbuf + D->moff[MDXS];
+  m.dys = D->mbuf + D->moff[MDYS];
+  m.cn2l = D->mbuf + D->moff[MCN2L];
+  m.cn2s = D->mbuf + D->moff[MCN2S];
+  m.cn3l = D->mbuf + D->moff[MCN3L];
+  m.cn3s = D->mbuf + D->moff[MCN3S];
+  m.cn4l = D->mbuf + D->moff[MCN4L];
+  m.cn4s = D->mbuf + D->moff[MCN4S];
+  m.delx1 = D->mbuf + D->moff[MDELX1];
+  m.inflow = nullptr;
+  // reduction scratch
+  const int gx = (im + 63) / 64, gy = (jm + 3) / 4;
+  D->d.npartials = gx * gy > 4096 ? gx * gy : 4096;
+  e = hipMalloc(&D->d.partials, (size_t)D->d.npartials * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc(&D->d.ctrl, sizeof(Ctrl));
+  if (e == hipSuccess) e = hipMemset(D->d.ctrl, 0, sizeof(Ctrl));
+  if (e != hipSuccess) return bail(std::string("control allocation: ") + hipGetErrorString(e));
+  D->d.nmaxp = nmaxp_default;
+  D->d.pjuge = pjuge_default;
+  D->d.omega = omega_default;
+  DevFields& F = D->d.F;
+  F.u = D->fields[FU];
+  F.v = D->fields[FV];
+  F.w = D->fields[FW];
+  F.p = D->fields[FP];
+  F.f = D->fields[FF];
+  F.g = D->fields[FG];
+  F.h = D->fields[FH];
+  F.usum = D->fields[FUSUM];
+  F.vsum = D->fields[FVSUM];
+  F.wsum = D->fields[FWSUM];
+  F.fold = D->fields[FFOLD];
+  F.gold = D->fields[FGOLD];
+  F.hold = D->fields[FHOLD];
+  F.rhs = D->fields[FRHS];
+  F.sm = D->fields[FSM];
+  F.uspd = D->fields[FUSPD];
+  F.vspd = D->fields[FVSPD];
+  F.diu[0] = nullptr;
+  F.cov[0] = nullptr;
+  for (int q = 1; q <= 9; q++) {
+    F.diu[q] = D->fields[FDIU1 + q - 1];
+    F.cov[q] = D->cov[q];
+  }
+  F.bmask = D->fields[FBMASK];
+  F.cmask = D->fields[FCMASK];
+  F.dmask = D->fields[FDMASK];
+  F.cn1 = D->fields[FCN1];
+  D->bytes += (long long)off * 4 + (long long)D->d.npartials * 8 + (long long)sizeof(Ctrl);
+  *out = D;
+  return 0;
+}
+
+int run_stage(les_domain* D, int s, int n, int nmax) {
+  (void)nmax;
+  if (flush_metrics(D)) return 1;
+  switch (s) {
+    case 0: launch_velnw(D->d, D->ro, D->dt); break;
+    case 1: launch_bondv1(D->d, n, D->dt); break;
+    case 2: launch_velfg(D->d, D->vn); break;
+    case 3: launch_feedbf(D->d, D->alpha, D->dt, D->beta); break;
+    case 4: launch_les(D->d); break;
+    case 5: launch_adam(D->d); break;
+    case 6: launch_press(D->d, D->dt); break;
+    default: return fail("unknown stage");
+  }
+  HIP_TRY(hipGetLastError());
+  return 0;
+}
+
+// -------------------------------------------- reference-signature layer
+std::mutex g_mu;
+std::map<std::tuple<int, int, int>, les_domain*> g_domains;
+std::map<std::tuple<int, int, int>, std::vector<float>> g_inflows;
+
+int ref_domain(int im, int jm, int km, les_domain** out) {
+  auto key = std::make_tuple(im, jm, km);
+  auto it = g_domains.find(key);
+  if (it != g_domains.end()) {
+    *out = it->second;
+    return 0;
+  }
+  les_domain* D = nullptr;
+  if (domain_alloc(im, jm, km, &D)) return 1;
+  auto pit = g_inflows.find(key);
+  if (pit != g_inflows.end() && set_inflow(D, pit->second.data())) {
+    domain_free(D);
+    return 1;
+  }
+  g_domains[key] = D;
+  *out = D;
+  return 0;
+}
+
+// Mirror a host array: upload when this field last mirrored another address.
+int bind(les_domain* D, int f, const float* host) {
+  if (!host) return fail(std::string("null array for ") + kFields[f].name);
+  if (D->bound[f] == host) return 0;
+  if (upload_field(D, f, host)) return 1;
+  D->bound[f] = host;
+  return 0;
+}
+
+#define BIND(f, ptr) \
+  if (bind(D, f, ptr)) return 1
+
+#define REF_DOMAIN(im, jm, km)                 \
+  std::lock_guard<std::mutex> _lk(g_mu);       \
+  les_domain* D = nullptr;                     \
+  if (ref_domain(im, jm, km, &D)) return 1
+
+}  // namespace
+
+extern "C" {
+
+const char* les_last_error(void) { return g_err.c_str(); }
+
+int les_velnw(int km, int jm, int im, const float* p, float ro, const float* dxs, float* u, float dt,
+              const float* f, const float* dys, float* v, const float* g, const float* dzs, float* w,
+              const float* h) {
+  REF_DOMAIN(im, jm, km);
+  set_metric(D, MDXS, dxs);
+  set_metric(D, MDYS, dys);
+  set_metric(D, MDZS, dzs);
+  BIND(FP, p);
+  BIND(FU, u);
+  BIND(FF, f);
+  BIND(FV, v);
+  BIND(FG, g);
+  BIND(FW, w);
+  BIND(FH, h);
+  D->ro = ro;
+  D->dt = dt;
+  return run_stage(D, 0, 0, 0);
+}
+
+int les_bondv1(int jm, float* u, const float* z2, const float* dzn, float* v, float* w, int km, int n,
+               int im, float dt, const float* dxs) {
+  (void)z2;
+  REF_DOMAIN(im, jm, km);
+  set_metric(D, MDZN, dzn);
+  set_metric(D, MDXS, dxs);
+  BIND(FU, u);
+  BIND(FV, v);
+  BIND(FW, w);
+  D->dt = dt;
+  return run_stage(D, 1, n, 0);
+}
+
+int les_velfg(int km, int jm, int im, const float* dx1, float* cov1, float* cov2, float* cov3,
+              float* dfu1, float* diu1, float* diu2, const float* dy1, float* diu3, const float* dzn,
+              float vn, float* f, float* cov4, float* cov5, float* cov6, float* dfv1, float* diu4,
+              float* diu5, float* diu6, float* g, float* cov7, float* cov8, float* cov9, float* dfw1,
+              float* diu7, float* diu8, float* diu9, const float* dzs, float* h, float* nou1,
+              const float* u, float* nou5, const float* v, float* nou9, const float* w, float* nou2,
+              float* nou3, float* nou4, float* nou6, float* nou7, float* nou8, float* uspd,
+              float* vspd) {
+  (void)cov1; (void)cov2; (void)cov3; (void)cov4; (void)cov5; (void)cov6; (void)cov7; (void)cov8;
+  (void)cov9; (void)dfu1; (void)dfv1; (void)dfw1; (void)nou1; (void)nou2; (void)nou3; (void)nou4;
+  (void)nou5; (void)nou6; (void)nou7; (void)nou8; (void)nou9;
+  REF_DOMAIN(im, jm, km);
+  set_metric(D, MDX1, dx1);
+  set_metric(D, MDY1, dy1);
+  set_metric(D, MDZN, dzn);
+  set_metric(D, MDZS, dzs);
+  float* diu[9] = {diu1, diu2, diu3, diu4, diu5, diu6, diu7, diu8, diu9};
+  for (int q = 0; q < 9; q++) BIND(FDIU1 + q, diu[q]);
+  BIND(FF, f);
+  BIND(FG, g);
+  BIND(FH, h);
+  BIND(FU, u);
+  BIND(FV, v);
+  BIND(FW, w);
+  BIND(FUSPD, uspd);
+  BIND(FVSPD, vspd);
+  D->vn = vn;
+  return run_stage(D, 2, 0, 0);
+}
+
+int les_feedbf(int km, int jm, int im, float* usum, const float* u, const float* bmask1, float* vsum,
+               const float* v, const float* cmask1, float* wsum, const float* w, const float* dmask1,
+               float alpha, float dt, float beta, float* fx, float* fy, float* fz, float* f, float* g,
+               float* h) {
+  (void)fx; (void)fy; (void)fz;
+  REF_DOMAIN(im, jm, km);
+  BIND(FUSUM, usum);
+  BIND(FU, u);
+  BIND(FBMASK, bmask1);
+  BIND(FVSUM, vsum);
+  BIND(FV, v);
+  BIND(FCMASK, cmask1);
+  BIND(FWSUM, wsum);
+  BIND(FW, w);
+  BIND(FDMASK, dmask1);
+  BIND(FF, f);
+  BIND(FG, g);
+  BIND(FH, h);
+  D->alpha = alpha;
+  D->dt = dt;
+  D->beta = beta;
+  return run_stage(D, 3, 0, 0);
+}
+
+int les_les(int km, float* delx1, const float* dx1, const float* dy1, const float* dzn, int jm, int im,
+            const float* diu1, const float* diu2, const float* diu3, const float* diu4,
+            const float* diu5, const float* diu6, const float* diu7, const float* diu8,
+            const float* diu9, float* sm, float* f, float* g, float* h, const float* uspd,
+            const float* vspd, const float* dxs, const float* dys) {
+  REF_DOMAIN(im, jm, km);
+  set_metric(D, MDX1, dx1);
+  set_metric(D, MDY1, dy1);
+  set_metric(D, MDZN, dzn);
+  set_metric(D, MDXS, dxs);
+  set_metric(D, MDYS, dys);
+  const float* diu[9] = {diu1, diu2, diu3, diu4, diu5, diu6, diu7, diu8, diu9};
+  for (int q = 0; q < 9; q++) BIND(FDIU1 + q, diu[q]);
+  BIND(FSM, sm);
+  BIND(FF, f);
+  BIND(FG, g);
+  BIND(FH, h);
+  BIND(FUSPD, uspd);
+  BIND(FVSPD, vspd);
+  if (flush_metrics(D)) return 1;
+  if (delx1)
+    for (int k = 1; k <= km; k++) delx1[k - 1] = metric_at(D, MDELX1, k);
+  return run_stage(D, 4, 0, 0);
+}
+
+int les_adam(int n, int nmax, float* fold, int im, int jm, int km, float* gold, float* hold,
+             const float* fghold, float* f, float* g, float* h) {
+  (void)fghold;
+  REF_DOMAIN(im, jm, km);
+  BIND(FFOLD, fold);
+  BIND(FGOLD, gold);
+  BIND(FHOLD, hold);
+  BIND(FF, f);
+  BIND(FG, g);
+  BIND(FH, h);
+  return run_stage(D, 5, n, nmax);
+}
+
+int les_press(int km, int jm, int im, float* rhs, const float* u, const float* dx1, const float* v,
+              const float* dy1, const float* w, const float* dzn, float* f, float* g, float* h,
+              float dt, const float* cn1, const float* cn2l, float* p, const float* cn2s,
+              const float* cn3l, const float* cn3s, const float* cn4l, const float* cn4s, int n,
+              int nmax, const float* usum, const float* vsum, const float* wsum) {
+  (void)usum; (void)vsum; (void)wsum;
+  REF_DOMAIN(im, jm, km);
+  set_metric(D, MDX1, dx1);
+  set_metric(D, MDY1, dy1);
+  set_metric(D, MDZN, dzn);
+  set_metric(D, MCN2L, cn2l);
+  set_metric(D, MCN2S, cn2s);
+  set_metric(D, MCN3L, cn3l);
+  set_metric(D, MCN3S, cn3s);
+  set_metric(D, MCN4L, cn4l);
+  set_metric(D, MCN4S, cn4s);
+  BIND(FRHS, rhs);
+  BIND(FU, u);
+  BIND(FV, v);
+  BIND(FW, w);
+  BIND(FF, f);
+  BIND(FG, g);
+  BIND(FH, h);
+  BIND(FCN1, cn1);
+  BIND(FP, p);
+  D->dt = dt;
+  D->d.nmaxp = nmaxp_default;
+  D->d.pjuge = pjuge_default;
+  D->d.omega = omega_default;
+  return run_stage(D, 6, n, nmax);
+}
+
+int les_bondv1_set_inflow(int im, int jm, int km, const float* prof) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  auto key = std::make_tuple(im, jm, km);
+  if (prof)
+    g_inflows[key] = std::vector<float>(prof, prof + (size_t)jm * km);
+  else
+    g_inflows.erase(key);
+  auto it = g_domains.find(key);
+  if (it != g_domains.end()) return set_inflow(it->second, prof);
+  return 0;
+}
+
+int les_press_config(int nmaxp, float pjuge, float omega) {
+  if (nmaxp < 1 || nmaxp > kSorHist) return fail("nmaxp out of range (1..1024)");
+  nmaxp_default = nmaxp;
+  pjuge_default = pjuge;
+  omega_default = omega;
+  return 0;
+}
+
+static int read_stats(les_domain* D, int* iters, double* sor_hist, int hist_len, double* rhsav,
+                      double* pav, int* nonfinite) {
+  Ctrl c;
+  HIP_TRY(hipStreamSynchronize(D->d.stream));
+  HIP_TRY(hipMemcpy(&c, D->d.ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost));
+  if (iters) *iters = c.sor_iters;
+  for (int l = 0; sor_hist && l < hist_len && l < c.sor_iters && l < kSorHist; l++) sor_hist[l] = c.sor_hist[l];
+  if (rhsav) *rhsav = c.rhsav;
+  if (pav) *pav = c.pav;
+  if (nonfinite) *nonfinite = c.nonfinite;
+  return 0;
+}
+
+int les_press_stats(int im, int jm, int km, int* iters, double* sor_hist, int hist_len, double* rhsav,
+                    double* pav, int* nonfinite) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  auto it = g_domains.find(std::make_tuple(im, jm, km));
+  if (it == g_domains.end()) return fail("no domain of these extents");
+  return read_stats(it->second, iters, sor_hist, hist_len, rhsav, pav, nonfinite);
+}
+
+int les_sync_host(const void* host_array) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  for (auto& kv : g_domains)
+    for (int f = 0; f < NFIELDS; f++)
+      if (kv.second->bound[f] == host_array) return download_field(kv.second, f, (float*)host_array);
+  g_err = "array was never handed to the library";
+  return 2;
+}
+
+int les_mark_host_dirty(const void* host_array) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  for (auto& kv : g_domains)
+    for (int f = 0; f < NFIELDS; f++)
+      if (kv.second->bound[f] == host_array) kv.second->bound[f] = nullptr;
+  return 0;
+}
+
+int les_synchronize(void) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  for (auto& kv : g_domains) HIP_TRY(hipStreamSynchronize(kv.second->d.stream));
+  return 0;
+}
+
+int les_release_all(void) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  for (auto& kv : g_domains) {
+    hipStreamSynchronize(kv.second->d.stream);
+    domain_free(kv.second);
+  }
+  g_domains.clear();
+  g_inflows.clear();
+  return 0;
+}
+
+// ------------------------------------------------------------ domain API
+int les_domain_create(const les_domain_desc* s, les_domain** out) {
+  if (!s || !out) return fail("null argument");
+  les_domain* D = nullptr;
+  if (domain_alloc(s->im, s->jm, s->km, &D)) return 1;
+  set_metric(D, MDX1, s->dx1);
+  set_metric(D, MDY1, s->dy1);
+  set_metric(D, MDZN, s->dzn);
+  set_metric(D, MDZS, s->dzs);
+  set_metric(D, MDXS, s->dxs);
+  set_metric(D, MDYS, s->dys);
+  set_metric(D, MCN2L, s->cn2l);
+  set_metric(D, MCN2S, s->cn2s);
+  set_metric(D, MCN3L, s->cn3l);
+  set_metric(D, MCN3S, s->cn3s);
+  set_metric(D, MCN4L, s->cn4l);
+  set_metric(D, MCN4S, s->cn4s);
+  int rc = 0;
+  if (s->cn1) rc |= upload_field(D, FCN1, s->cn1);
+  if (s->bmask1) rc |= upload_field(D, FBMASK, s->bmask1);
+  if (s->cmask1) rc |= upload_field(D, FCMASK, s->cmask1);
+  if (s->dmask1) rc |= upload_field(D, FDMASK, s->dmask1);
+  rc |= set_inflow(D, s->inflow);
+  rc |= flush_metrics(D);
+  if (rc) {
+    std::string e = g_err;
+    domain_free(D);
+    return fail(e);
+  }
+  D->ro = s->ro;
+  D->vn = s->vn;
+  D->alpha = s->alpha;
+  D->beta = s->beta;
+  D->dt = s->dt;
+  if (s->nmaxp < 1 || s->nmaxp > kSorHist) {
+    domain_free(D);
+    return fail("nmaxp out of range (1..1024)");
+  }
+  D->d.nmaxp = s->nmaxp;
+  D->d.pjuge = s->pjuge;
+  D->d.omega = s->omega;
+  *out = D;
+  return 0;
+}
+
+int les_domain_destroy(les_domain* D) {
+  if (!D) return 0;
+  hipStreamSynchronize(D->d.stream);
+  domain_free(D);
+  return 0;
+}
+
+int les_domain_stage(les_domain* D, int stage, int n, int nmax) {
+  if (!D) return fail("null domain");
+  return run_stage(D, stage, n, nmax);
+}
+
+int les_domain_step(les_domain* D, int n, int nmax) {
+  if (!D) return fail("null domain");
+  for (int s = 0; s < 7; s++)
+    if (run_stage(D, s, n, nmax)) return 1;
+  return 0;
+}
+
+int les_domain_run(les_domain* D, int n0, int n1, int nmax) {
+  for (int n = n0; n <= n1; n++)
+    if (les_domain_step(D, n, nmax)) return 1;
+  return 0;
+}
+
+int les_domain_upload(les_domain* D, const char* field, const float* host) {
+  if (!D || !field || !host) return fail("null argument");
+  const int f = field_by_name(field);
+  if (f < 0) return fail(std::string("unknown field ") + field);
+  return upload_field(D, f, host);
+}
+
+int les_domain_download(les_domain* D, const char* field, float* host) {
+  if (!D || !field || !host) return fail("null argument");
+  const int f = field_by_name(field);
+  if (f < 0) return fail(std::string("unknown field ") + field);
+  return download_field(D, f, host);
+}
+
+int les_domain_synchronize(les_domain* D) {
+  if (!D) return fail("null domain");
+  HIP_TRY(hipStreamSynchronize(D->d.stream));
+  return 0;
+}
+
+int les_domain_press_stats(les_domain* D, int* iters, double* sor_hist, int hist_len, double* rhsav,
+                           double* pav, int* nonfinite) {
+  if (!D) return fail("null domain");
+  return read_stats(D, iters, sor_hist, hist_len, rhsav, pav, nonfinite);
+}
+
+long long les_domain_bytes(const les_domain* D) { return D ? D->bytes : 0; }
+
+}  // extern "C"
