@@ -1,0 +1,202 @@
+#!/usr/bin/env python3
+"""Benchmark of the LES step on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
+
+A step is one pass of the hot path velnw -> bondv1 -> velfg -> feedbf ->
+les -> adam -> press over the whole domain, with every field resident in
+device memory (main.f95:260-296).  Default workload: the 300x300x90 plain
+LES domain (dx=dy=4 m, dz 2 m x1.05 capped at 20 m, 30x30 city blocks of
+8x8 cells with 2-cell streets, lognormal heights median 16 m sigma 0.5,
+log-law inflow u*=0.5 m/s z0=0.1 m with +-1% noise, dt=0.05 s, Cs=0.1,
+SOR omega=1, at most 50 iterations, stop at sum(reltmp^2) < 1e-4).
+
+W untimed warmup steps (n = 1..W) are followed by K timed steps
+(n = W+1..W+K) bracketed by device synchronisation.  During the timed steps
+the dominant kernel (the SOR sweep) is bracketed by HIP events on the
+domain's stream; its algorithmic bytes per launch divided by its average
+duration gives the roofline figure.  A second short pass with every launch
+timed gives the per-kernel table.  The CPU baseline runs the reference's
+own Fortran subroutines (flang build under oracle/_ref) -- or, when that
+build is absent, the C restatement -- on one host core for a bounded sample
+of the same steps, starting from the GPU's state after the warmup.
+
+Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import dataclasses
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT / "mpi-les_amd"))
+
+import numpy as np  # noqa: E402
+
+from lesamd import api, setup_gen  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "steps/s + Mcell-updates/s at 300x300x90 FP32, %HBM peak; served p50/p99"
+
+# Algorithmic bytes per interior cell of each stage: every live reference
+# array crossing the stage boundary counted once, FP32 (velnw 40, velfg+vel2
+# 60, feedbf 64, les+boundsm 64, adam 48, press rhs 28, SOR 12 per
+# iteration, pressure-mean removal 12).
+STEP_BYTES_FIXED = 40 + 60 + 64 + 64 + 48 + 28 + 12
+SOR_BYTES_PER_ITER = 12
+# per launch of each kernel class, bytes per interior cell
+CLASS_BYTES = {
+    "velnw": 40, "vel2": 48, "velfg": 12, "feedbf": 64, "les_sm": 40, "les_visc": 24, "adam": 48,
+    "rhs": 28, "sor": SOR_BYTES_PER_ITER / 2, "pav": 4, "padj": 8,
+}
+STATE_FIELDS = ("u", "v", "w", "p", "f", "g", "h", "usum", "vsum", "wsum", "fold", "gold", "hold", "rhs",
+                "sm", "uspd", "vspd") + tuple(f"diu{q}" for q in range(1, 10))
+
+
+def cpu_baseline(setup, state, n0, nsteps, budget_s):
+    """Time the CPU checker on this host for up to nsteps steps (stopping
+    once budget_s is spent), from the GPU state `state` before step n0."""
+    sys.path.insert(0, str(ROOT))
+    from oracle import oracle
+    w = setup.workload
+    kind = "reference" if oracle.ref_available("large") and w.im <= 300 and w.jm <= 300 and w.km <= 105 else "port"
+    if kind == "reference":
+        ref_setup = dataclasses.replace(setup, inflow=None)
+        be = oracle.RefBackend(ref_setup, "large")
+        what = "reference Fortran subroutines (flang -O2, oracle/_ref/large) with its fixed 5 m/s inflow"
+    else:
+        be = oracle.OracleBackend(setup, 32, redmode=0)
+        what = "C restatement (gcc -O2, FP32)"
+    be.load(state)
+    times = []
+    t_all = time.perf_counter()
+    for n in range(n0, n0 + nsteps):
+        t = time.perf_counter()
+        be.step(n, 10 ** 9)
+        times.append(time.perf_counter() - t)
+        if time.perf_counter() - t_all > budget_s:
+            break
+    sps = len(times) / sum(times)
+    return {
+        "value": round(sps, 4),
+        "unit": "steps/s",
+        "cores": 1,
+        "kind": kind,
+        "sample": f"{len(times)} steps (n={n0}..{n0 + len(times) - 1}) of the {w.im}x{w.jm}x{w.km} domain on one "
+                  f"host core, {what}, started from the GPU state after the warmup; "
+                  f"{sum(times):.1f} s of CPU time",
+        "mcell_updates_per_s": round(sps * w.cells / 1e6, 3),
+    }
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--workload", default="plain-300x300x90", choices=sorted(setup_gen.workloads()))
+    ap.add_argument("--cpu-steps", type=int, default=8, help="max CPU-baseline steps")
+    ap.add_argument("--cpu-budget", type=float, default=20.0, help="CPU-baseline time budget [s]")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args(argv)
+    if args.gpus != 1:
+        print("bench.py: this build runs on exactly one GPU (one domain per device); "
+              f"--gpus {args.gpus} is not supported", file=sys.stderr)
+        return 2
+    if int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: multi-rank launch is not supported; run with one GPU", file=sys.stderr)
+        return 2
+
+    w = setup_gen.workloads()[args.workload]
+    setup = setup_gen.build(w)
+    dom = api.Domain(setup)
+    W, K = args.warmup, args.steps
+    nmax = W + K
+    print(f"[bench] {w.name}: {w.cells} cells, {dom.nbytes() / 2**30:.2f} GiB on device; "
+          f"warmup {W}, timed {K}", file=sys.stderr, flush=True)
+    for n in range(1, W + 1):
+        dom.step(n, nmax)
+    dom.synchronize()
+    snapshot = None if args.no_cpu_baseline else {n: dom.download(n) for n in STATE_FIELDS}
+
+    # timed region: only the dominant kernel class (the SOR sweep) is
+    # event-bracketed
+    dom.set_timing(True, ["sor"])
+    dom.timing()
+    dom.synchronize()
+    t0 = time.perf_counter()
+    for n in range(W + 1, W + K + 1):
+        dom.step(n, nmax)
+    dom.synchronize()
+    t1 = time.perf_counter()
+    kt = dom.timing()
+    st = dom.press_stats()
+    # short second pass with every launch timed: the per-kernel table
+    dom.set_timing(True)
+    dom.timing()
+    nprof = 2
+    for n in range(W + K + 1, W + K + 1 + nprof):
+        dom.step(n, nmax + nprof)
+    kt_all = dom.timing()
+    dom.set_timing(False)
+    if st["nonfinite"]:
+        print("bench.py: non-finite SOR residual", file=sys.stderr)
+        return 1
+    elapsed = t1 - t0
+    sps = K / elapsed
+    L = st["iters"]
+    # dominant kernel = the class with the largest total time
+    dom_cls = max((c for c in kt if kt[c][1] > 0), key=lambda c: kt[c][0])
+    dom_ms, dom_n = kt[dom_cls]
+    avg_s = dom_ms / 1e3 / dom_n
+    alg_bytes = CLASS_BYTES.get(dom_cls, 0) * w.cells
+    achieved = alg_bytes / avg_s / 1e9
+    step_bytes = w.cells * (STEP_BYTES_FIXED + SOR_BYTES_PER_ITER * L)
+    per_kernel = {c: {"ms_per_step": round(v[0] / nprof, 4), "launches_per_step": v[1] / nprof,
+                      "us_per_launch": round(1e3 * v[0] / v[1], 2)} for c, v in kt_all.items() if v[1]}
+
+    result = {
+        "metric": METRIC,
+        "value": round(sps, 3),
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": round(1e3 * elapsed / K, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded city, log-law inflow)",
+        "config": {"workload": w.name, "grid": [w.im, w.jm, w.km], "dx": w.dx, "dt": w.dt,
+                   "nmaxp": w.nmaxp, "pjuge": w.pjuge, "seed": w.seed},
+        "mcell_updates_per_s": round(sps * w.cells / 1e6, 2),
+        "sor_iters_last_step": L,
+        "hbm_frac_step": round(step_bytes * sps / 1e9 / HBM_PEAK_GBS, 4),
+        "roofline": {
+            "kernel": dom_cls,
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None,
+            "alg_bytes_per_launch": int(alg_bytes),
+            "avg_launch_us": round(avg_s * 1e6, 3),
+        },
+        "kernels": per_kernel,
+    }
+    dom.close()
+    if snapshot is not None:
+        result["cpu_baseline"] = cpu_baseline(setup, snapshot, W + 1, args.cpu_steps, args.cpu_budget)
+    print(json.dumps(result), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -158,6 +158,18 @@ int les_domain_press_stats(les_domain* d, int* iters, double* sor_hist, int hist
 /* Bytes of device memory the domain holds. */
 long long les_domain_bytes(const les_domain* d);
 
+/* Per-kernel timing: while on (a nonzero bit mask of kernel classes, -1 for
+ * all), every launch of a selected class is bracketed by a pair of HIP
+ * events on the domain's stream.  les_domain_timing fills
+ * ms[c] and count[c] for each kernel class c < les_kernel_classes() with
+ * the launches recorded since the previous read, then clears them.
+ * Synchronises.  No reference counterpart beyond the -DTIMINGS per-stage
+ * clock of main.f95:243-303. */
+int les_domain_set_timing(les_domain* d, int on);
+int les_domain_timing(les_domain* d, double* ms, long long* count);
+int les_kernel_classes(void);
+const char* les_kernel_class_name(int c);
+
 #ifdef __cplusplus
 }
 #endif

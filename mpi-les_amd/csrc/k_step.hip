@@ -42,8 +42,8 @@ __global__ __launch_bounds__(NT) void k_velnw(Grid g, Metrics m, const float* __
 
 void launch_velnw(const DevDomain& d, float ro, float dt) {
   const Grid& g = d.g;
-  hipLaunchKernelGGL(k_velnw, grid2d(g.im, g.jm, g.km), block2d(), 0, d.stream, g, d.m, d.F.p,
-                     d.F.f, d.F.g, d.F.h, d.F.u, d.F.v, d.F.w, ro, dt);
+  TIMED(d, KC_VELNW, hipLaunchKernelGGL(k_velnw, grid2d(g.im, g.jm, g.km), block2d(), 0, d.stream, g, d.m, d.F.p,
+                     d.F.f, d.F.g, d.F.h, d.F.u, d.F.v, d.F.w, ro, dt));
 }
 
 // ----------------------------------------------------------------- bondv1
@@ -158,17 +158,17 @@ __global__ __launch_bounds__(256) void k_bondv1_ghost(Grid g, float* __restrict_
 void launch_bondv1(const DevDomain& d, int n, float dt) {
   const Grid& g = d.g;
   const int ilast = (n == 1) ? g.im : 1;
-  hipLaunchKernelGGL(k_bondv1_inflow, grid2d(ilast + 1, g.jm, g.km), block2d(), 0, d.stream, g, d.m,
-                     d.F.u, d.F.v, d.F.w, ilast, d.ctrl);
+  TIMED(d, KC_BONDV1, hipLaunchKernelGGL(k_bondv1_inflow, grid2d(ilast + 1, g.jm, g.km), block2d(), 0, d.stream, g, d.m,
+                     d.F.u, d.F.v, d.F.w, ilast, d.ctrl));
   const int nplane = g.jm * g.km;
   int nb = (nplane + 255) / 256;
   if (nb > 256) nb = 256;
-  hipLaunchKernelGGL(k_bondv1_uout, dim3(nb), dim3(256), 0, d.stream, g, d.F.u, d.ctrl);
-  hipLaunchKernelGGL(k_bondv1_outflow, dim3((nplane + 255) / 256), dim3(256), 0, d.stream, g, d.m,
-                     d.F.u, d.F.v, d.F.w, (const Ctrl*)d.ctrl, dt);
+  TIMED(d, KC_BONDV1, hipLaunchKernelGGL(k_bondv1_uout, dim3(nb), dim3(256), 0, d.stream, g, d.F.u, d.ctrl));
+  TIMED(d, KC_BONDV1, hipLaunchKernelGGL(k_bondv1_outflow, dim3((nplane + 255) / 256), dim3(256), 0, d.stream, g, d.m,
+                     d.F.u, d.F.v, d.F.w, (const Ctrl*)d.ctrl, dt));
   const int ng = (g.im + 2) * (g.km + g.jm + 3);
-  hipLaunchKernelGGL(k_bondv1_ghost, dim3((ng + 255) / 256), dim3(256), 0, d.stream, g, d.F.u,
-                     d.F.v, d.F.w);
+  TIMED(d, KC_BONDV1, hipLaunchKernelGGL(k_bondv1_ghost, dim3((ng + 255) / 256), dim3(256), 0, d.stream, g, d.F.u,
+                     d.F.v, d.F.w));
 }
 
 // ------------------------------------------------------------ vel2/velfg
@@ -364,14 +364,14 @@ static P10 p10(float* const* a) {
 void launch_velfg(const DevDomain& d, float vn) {
   const Grid& g = d.g;
   const P10 diu = p10(d.F.diu), cov = p10(d.F.cov);
-  hipLaunchKernelGGL(k_vel2, grid2d(g.im, g.jm, g.km + 1), block2d(), 0, d.stream, g, d.m, d.F.u,
-                     d.F.v, d.F.w, diu, cov, d.F.uspd, d.F.vspd, d.wl);
+  TIMED(d, KC_VEL2, hipLaunchKernelGGL(k_vel2, grid2d(g.im, g.jm, g.km + 1), block2d(), 0, d.stream, g, d.m, d.F.u,
+                     d.F.v, d.F.w, diu, cov, d.F.uspd, d.F.vspd, d.wl));
   const int n1 = g.jm * g.km + g.im * g.km;
-  hipLaunchKernelGGL(k_vel2_ghost1, dim3((n1 + 255) / 256), dim3(256), 0, d.stream, g, diu, cov);
+  TIMED(d, KC_VEL2_GHOST, hipLaunchKernelGGL(k_vel2_ghost1, dim3((n1 + 255) / 256), dim3(256), 0, d.stream, g, diu, cov));
   const int n2 = (g.jm + 1) * (g.km + 1) + (g.im + 1) * (g.km + 1);
-  hipLaunchKernelGGL(k_vel2_ghost2, dim3((n2 + 255) / 256), dim3(256), 0, d.stream, g, diu);
-  hipLaunchKernelGGL(k_velfg, grid2d(g.im, g.jm, g.km), block2d(), 0, d.stream, g, d.m, diu, cov,
-                     d.F.f, d.F.g, d.F.h, vn);
+  TIMED(d, KC_VEL2_GHOST, hipLaunchKernelGGL(k_vel2_ghost2, dim3((n2 + 255) / 256), dim3(256), 0, d.stream, g, diu));
+  TIMED(d, KC_VELFG, hipLaunchKernelGGL(k_velfg, grid2d(g.im, g.jm, g.km), block2d(), 0, d.stream, g, d.m, diu, cov,
+                     d.F.f, d.F.g, d.F.h, vn));
 }
 
 // ----------------------------------------------------------------- feedbf
@@ -406,9 +406,9 @@ __global__ __launch_bounds__(NT) void k_feedbf(Grid g, const float* __restrict__
 
 void launch_feedbf(const DevDomain& d, float alpha, float dt, float beta) {
   const Grid& g = d.g;
-  hipLaunchKernelGGL(k_feedbf, grid2d(g.im, g.jm, g.km), block2d(), 0, d.stream, g, d.F.u, d.F.v,
+  TIMED(d, KC_FEEDBF, hipLaunchKernelGGL(k_feedbf, grid2d(g.im, g.jm, g.km), block2d(), 0, d.stream, g, d.F.u, d.F.v,
                      d.F.w, d.F.bmask, d.F.cmask, d.F.dmask, d.F.usum, d.F.vsum, d.F.wsum, d.F.f,
-                     d.F.g, d.F.h, alpha, dt, beta);
+                     d.F.g, d.F.h, alpha, dt, beta));
 }
 
 // -------------------------------------------------------------------- les
@@ -602,13 +602,13 @@ __global__ __launch_bounds__(NT) void k_les_visc(Grid g, Metrics m, P10 diu,
 void launch_les(const DevDomain& d) {
   const Grid& g = d.g;
   const P10 diu = p10(d.F.diu);
-  hipLaunchKernelGGL(k_les_sm, grid2d(g.im, g.jm, g.km), block2d(), 0, d.stream, g, diu,
-                     d.m.delx1 + 1, d.F.sm);
+  TIMED(d, KC_LES_SM, hipLaunchKernelGGL(k_les_sm, grid2d(g.im, g.jm, g.km), block2d(), 0, d.stream, g, diu,
+                     d.m.delx1 + 1, d.F.sm));
   const int ni = g.im + 2, nj = g.jm + 3;
   const int nb = 2 * ni * nj + 3 * ni * g.km + 2 * g.jm * g.km;
-  hipLaunchKernelGGL(k_boundsm, dim3((nb + 255) / 256), dim3(256), 0, d.stream, g, d.F.sm);
-  hipLaunchKernelGGL(k_les_visc, grid2d(g.im, g.jm, g.km), block2d(), 0, d.stream, g, d.m, diu,
-                     d.F.sm, d.F.uspd, d.F.vspd, d.F.f, d.F.g, d.F.h, d.wl);
+  TIMED(d, KC_BOUNDSM, hipLaunchKernelGGL(k_boundsm, dim3((nb + 255) / 256), dim3(256), 0, d.stream, g, d.F.sm));
+  TIMED(d, KC_LES_VISC, hipLaunchKernelGGL(k_les_visc, grid2d(g.im, g.jm, g.km), block2d(), 0, d.stream, g, d.m, diu,
+                     d.F.sm, d.F.uspd, d.F.vspd, d.F.f, d.F.g, d.F.h, d.wl));
 }
 
 // ------------------------------------------------------------------- adam
@@ -632,8 +632,8 @@ __global__ __launch_bounds__(NT) void k_adam(Grid g, float* __restrict__ f, floa
 
 void launch_adam(const DevDomain& d) {
   const Grid& g = d.g;
-  hipLaunchKernelGGL(k_adam, grid2d(g.im, g.jm, g.km), block2d(), 0, d.stream, g, d.F.f, d.F.g,
-                     d.F.h, d.F.fold, d.F.gold, d.F.hold);
+  TIMED(d, KC_ADAM, hipLaunchKernelGGL(k_adam, grid2d(g.im, g.jm, g.km), block2d(), 0, d.stream, g, d.F.f, d.F.g,
+                     d.F.h, d.F.fold, d.F.gold, d.F.hold));
 }
 
 // ------------------------------------------------------ layout conversion
@@ -662,8 +662,8 @@ void launch_unpack(const DevDomain& d, const float* src, float* dst, int li, int
   const long n = (long)ni * nj * nk;
   long nb = (n + 255) / 256;
   if (nb > 65536) nb = 65536;
-  hipLaunchKernelGGL(k_unpack, dim3((unsigned)nb), dim3(256), 0, d.stream, d.g, src, dst, li, lj, lk,
-                     ni, nj, nk);
+  TIMED(d, KC_COPY, hipLaunchKernelGGL(k_unpack, dim3((unsigned)nb), dim3(256), 0, d.stream, d.g, src, dst, li, lj, lk,
+                     ni, nj, nk));
 }
 
 void launch_pack(const DevDomain& d, const float* src, float* dst, int li, int lj, int lk, int ni,
@@ -671,8 +671,8 @@ void launch_pack(const DevDomain& d, const float* src, float* dst, int li, int l
   const long n = (long)ni * nj * nk;
   long nb = (n + 255) / 256;
   if (nb > 65536) nb = 65536;
-  hipLaunchKernelGGL(k_pack, dim3((unsigned)nb), dim3(256), 0, d.stream, d.g, src, dst, li, lj, lk,
-                     ni, nj, nk);
+  TIMED(d, KC_COPY, hipLaunchKernelGGL(k_pack, dim3((unsigned)nb), dim3(256), 0, d.stream, d.g, src, dst, li, lj, lk,
+                     ni, nj, nk));
 }
 
 }  // namespace les

@@ -39,6 +39,9 @@ struct Metrics {
   const float* cn4l;    // cn4l(1:kp)
   const float* cn4s;
   const float* delx1;   // delx1(1:kp), (dx1(0)*dy1(0)*dzn(k))**(1/3), les.f95:38-42
+  const float* cnax;    // 2/(dxs(i-1)*dxs(i)), the three terms of cn1 (init.f95:100-107)
+  const float* cnby;    // 2/(dys(j-1)*dys(j))
+  const float* cncz;    // 2/(dzs(k-1)*dzs(k))
   const float* inflow;  // u_in(j,k) at (j-1)+(k-1)*jm, nullptr = 5 m/s
 };
 #define M1(a, x) ((a)[(x) + 1])

@@ -17,6 +17,13 @@ struct DevFields {
   float* cov[10];                 // cov1..cov9 (index 0 unused)
   float *bmask, *cmask, *dmask;   // IBM masks
   float* cn1;                     // Poisson diagonal (1:ip,1:jp,1:kp)
+  float* ps[2];                   // SOR iterates (alternating)
+};
+
+// Kernel classes for the optional per-launch timing (les_domain_set_timing).
+enum KClass {
+  KC_VELNW, KC_BONDV1, KC_VEL2, KC_VEL2_GHOST, KC_VELFG, KC_FEEDBF, KC_LES_SM, KC_BOUNDSM,
+  KC_LES_VISC, KC_ADAM, KC_RHS, KC_SOR, KC_BOUNDP, KC_PAV, KC_PADJ, KC_COPY, KC_N
 };
 
 struct DevDomain {
@@ -32,7 +39,24 @@ struct DevDomain {
   int nmaxp;         // SOR iteration cap (press.f95:44)
   float pjuge;       // SOR stop threshold on sum(reltmp^2) (press.f95:43)
   float omega;       // SOR relaxation (press.f95:45)
+  void* timer;       // host-side event recorder, nullptr when timing is off
+  int sor_kchunk;    // k levels per SOR block
+  int sor_parts_cap; // SOR partials per iteration slot (two slots follow the reduction partials)
+  int cn1_separable; // cn1 equals 1/(cnax+cnby+cncz) bit for bit: recompute instead of reading it
 };
+
+int sor_parts_needed(const Grid& g, int kchunk);
+int reduction_parts_needed();
+
+// Event pairs around launches, recorded only when d.timer is set.
+void timer_begin(const DevDomain& d, int cls);
+void timer_end(const DevDomain& d, int cls);
+#define TIMED(d, cls, ...)      \
+  do {                          \
+    timer_begin((d), (cls));    \
+    __VA_ARGS__;                \
+    timer_end((d), (cls));      \
+  } while (0)
 
 void launch_velnw(const DevDomain& d, float ro, float dt);
 void launch_bondv1(const DevDomain& d, int n, float dt);

@@ -3,6 +3,7 @@
 // conversion, and the C ABI declared in include/les_amd.h.
 #include <math.h>
 #include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <map>
@@ -79,7 +80,10 @@ int field_by_name(const char* n) {
 }
 
 // 1-D metrics: reference lower bound, length minus n, which n (0 i, 1 j, 2 k)
-enum Metric { MDX1, MDY1, MDZN, MDZS, MDXS, MDYS, MCN2L, MCN2S, MCN3L, MCN3S, MCN4L, MCN4S, MDELX1, NMETRICS };
+enum Metric {
+  MDX1, MDY1, MDZN, MDZS, MDXS, MDYS, MCN2L, MCN2S, MCN3L, MCN3S, MCN4L, MCN4S, MDELX1,
+  MCNAX, MCNBY, MCNCZ, NMETRICS
+};
 struct MetricInfo {
   int lb, elen, axis;
 };
@@ -97,9 +101,49 @@ const MetricInfo kMetrics[NMETRICS] = {
     {1, 0, 2},   // cn4l(kp)
     {1, 0, 2},   // cn4s(kp)
     {1, 0, 2},   // delx1(kp)
+    {1, 0, 0},   // cnax(ip)   derived
+    {1, 0, 1},   // cnby(jp)   derived
+    {1, 0, 2},   // cncz(kp)   derived
 };
 
+// Event pairs recorded around launches while timing is on.
+struct Timer {
+  std::vector<hipEvent_t> ev;  // 2 per pair
+  std::vector<int> cls;        // class of each pair
+  size_t used = 0;             // pairs recorded since the last read
+  unsigned int mask = ~0u;     // classes to record
+  bool open = false;           // a start event is waiting for its end
+};
+
+const char* kClassNames[KC_N] = {"velnw", "bondv1", "vel2", "vel2_ghost", "velfg", "feedbf", "les_sm",
+                                 "boundsm", "les_visc", "adam", "rhs", "sor", "boundp", "pav", "padj",
+                                 "copy"};
+
 }  // namespace
+
+void les::timer_begin(const DevDomain& d, int c) {
+  Timer* T = (Timer*)d.timer;
+  if (!T || !(T->mask & (1u << c))) return;
+  if (2 * T->used + 2 > T->ev.size()) {
+    hipEvent_t a, b;
+    if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return;
+    T->ev.push_back(a);
+    T->ev.push_back(b);
+    T->cls.push_back(0);
+  }
+  T->cls[T->used] = c;
+  hipEventRecord(T->ev[2 * T->used], d.stream);
+  T->open = true;
+}
+
+void les::timer_end(const DevDomain& d, int c) {
+  (void)c;
+  Timer* T = (Timer*)d.timer;
+  if (!T || !T->open) return;
+  hipEventRecord(T->ev[2 * T->used + 1], d.stream);
+  T->used++;
+  T->open = false;
+}
 
 struct les_domain {
   DevDomain d{};
@@ -145,7 +189,32 @@ int ensure_staging(les_domain* D, size_t n) {
   return 0;
 }
 
+void derive(les_domain* D);
+
+// cn1 of init.f95:100-107 is 1/(cnax(i)+cnby(j)+cncz(k)); when the array
+// handed over equals that bit for bit the SOR recomputes it instead of
+// reading 4 bytes per cell per sweep.
+void check_cn1(les_domain* D, const float* host) {
+  derive(D);
+  const Grid& g = D->d.g;
+  const float* ax = D->mhost.data() + D->moff[MCNAX] + 1;
+  const float* by = D->mhost.data() + D->moff[MCNBY] + 1;
+  const float* cz = D->mhost.data() + D->moff[MCNCZ] + 1;
+  size_t q = 0;
+  for (int k = 1; k <= g.km; k++)
+    for (int j = 1; j <= g.jm; j++)
+      for (int i = 1; i <= g.im; i++, q++) {
+        const float c = 1.0f / ((ax[i] + by[j]) + cz[k]);
+        if (memcmp(&c, host + q, sizeof(float)) != 0) {
+          D->d.cn1_separable = 0;
+          return;
+        }
+      }
+  D->d.cn1_separable = 1;
+}
+
 int upload_field(les_domain* D, int f, const float* host) {
+  if (f == FCN1) check_cn1(D, host);
   const Kind k = kFields[f].kind;
   int ni, nj, nk;
   ref_extent(D, k, &ni, &nj, &nk);
@@ -183,6 +252,7 @@ void set_metric(les_domain* D, int mi, const float* host) {
   if (memcmp(dst, host, len * sizeof(float)) != 0) {
     memcpy(dst, host, len * sizeof(float));
     D->mdirty = true;
+    if (mi == MDXS || mi == MDYS || mi == MDZS) D->d.cn1_separable = 0;
   }
 }
 
@@ -199,6 +269,13 @@ void derive(les_domain* D) {
     D->mhost[D->moff[MDELX1] + k + 1] = v;
   }
   D->d.wl = logf(0.5f * metric_at(D, MDZN, 1) / 0.1f);
+  // the three terms of cn1 (init.f95:100-107), FP32 as written there
+  for (int i = 1; i <= g.im; i++)
+    D->mhost[D->moff[MCNAX] + i + 1] = 2.0f / (metric_at(D, MDXS, i - 1) * metric_at(D, MDXS, i));
+  for (int j = 1; j <= g.jm; j++)
+    D->mhost[D->moff[MCNBY] + j + 1] = 2.0f / (metric_at(D, MDYS, j - 1) * metric_at(D, MDYS, j));
+  for (int k = 1; k <= g.km; k++)
+    D->mhost[D->moff[MCNCZ] + k + 1] = 2.0f / (metric_at(D, MDZS, k - 1) * metric_at(D, MDZS, k));
   double area = 0.0;
   for (int k = 1; k <= g.km; k++) {
     const float dz = metric_at(D, MDZN, k);
@@ -238,12 +315,23 @@ int nmaxp_default = 50;
 float pjuge_default = 1.0e-4f;
 float omega_default = 1.0f;
 
+void timer_free(les_domain* D) {
+  Timer* T = (Timer*)D->d.timer;
+  if (!T) return;
+  for (hipEvent_t e : T->ev) hipEventDestroy(e);
+  delete T;
+  D->d.timer = nullptr;
+}
+
 void domain_free(les_domain* D) {
   if (!D) return;
+  timer_free(D);
   for (int f = 0; f < NFIELDS; f++)
     if (D->fields[f]) hipFree(D->fields[f]);
   for (int q = 1; q <= 9; q++)
     if (D->cov[q]) hipFree(D->cov[q]);
+  for (int q = 0; q < 2; q++)
+    if (D->d.F.ps[q]) hipFree(D->d.F.ps[q]);
   if (D->mbuf) hipFree(D->mbuf);
   if (D->inflow_dev) hipFree(D->inflow_dev);
   if (D->staging) hipFree(D->staging);
@@ -287,6 +375,13 @@ int domain_alloc(int im, int jm, int km, les_domain** out) {
     if (e != hipSuccess) return bail(std::string("scratch allocation: ") + hipGetErrorString(e));
     D->bytes += nb;
   }
+  for (int q = 0; q < 2; q++) {
+    const size_t nb = (size_t)n3 * sizeof(float);
+    e = hipMalloc(&D->d.F.ps[q], nb);
+    if (e == hipSuccess) e = hipMemset(D->d.F.ps[q], 0, nb);
+    if (e != hipSuccess) return bail(std::string("SOR buffer allocation: ") + hipGetErrorString(e));
+    D->bytes += nb;
+  }
   // metric pack: each array gets n+4 slots (indices -1..n+2)
   int off = 0;
   for (int mi = 0; mi < NMETRICS; mi++) {
@@ -310,11 +405,27 @@ int domain_alloc(int im, int jm, int km, les_domain** out) {
   m.cn4l = D->mbuf + D->moff[MCN4L];
   m.cn4s = D->mbuf + D->moff[MCN4S];
   m.delx1 = D->mbuf + D->moff[MDELX1];
+  m.cnax = D->mbuf + D->moff[MCNAX];
+  m.cnby = D->mbuf + D->moff[MCNBY];
+  m.cncz = D->mbuf + D->moff[MCNCZ];
   m.inflow = nullptr;
-  // reduction scratch
+  // reduction scratch: reduction partials, then two SOR partial slots
   const int gx = (im + 63) / 64, gy = (jm + 3) / 4;
-  D->d.npartials = gx * gy > 4096 ? gx * gy : 4096;
-  e = hipMalloc(&D->d.partials, (size_t)D->d.npartials * sizeof(double));
+  const int red = reduction_parts_needed();
+  D->d.npartials = gx * gy > red ? gx * gy : red;
+  {
+    // k levels per SOR block: enough blocks to fill the device (~1024)
+    const int tiles = ((im + 63) / 64) * ((jm + 7) / 8);
+    int nkc = (1024 + tiles - 1) / tiles;
+    if (nkc > km) nkc = km;
+    if (nkc < 1) nkc = 1;
+    int kc = (km + nkc - 1) / nkc;
+    const char* env = getenv("LES_SOR_KCHUNK");
+    if (env && atoi(env) > 0) kc = atoi(env) < km ? atoi(env) : km;
+    D->d.sor_kchunk = kc;
+    D->d.sor_parts_cap = sor_parts_needed(D->d.g, kc);
+  }
+  e = hipMalloc(&D->d.partials, ((size_t)D->d.npartials + 2 * (size_t)D->d.sor_parts_cap) * sizeof(double));
   if (e == hipSuccess) e = hipMalloc(&D->d.ctrl, sizeof(Ctrl));
   if (e == hipSuccess) e = hipMemset(D->d.ctrl, 0, sizeof(Ctrl));
   if (e != hipSuccess) return bail(std::string("control allocation: ") + hipGetErrorString(e));
@@ -349,7 +460,8 @@ int domain_alloc(int im, int jm, int km, les_domain** out) {
   F.cmask = D->fields[FCMASK];
   F.dmask = D->fields[FDMASK];
   F.cn1 = D->fields[FCN1];
-  D->bytes += (long long)off * 4 + (long long)D->d.npartials * 8 + (long long)sizeof(Ctrl);
+  D->bytes += (long long)off * 4 + ((long long)D->d.npartials + 2LL * D->d.sor_parts_cap) * 8 +
+              (long long)sizeof(Ctrl);
   *out = D;
   return 0;
 }
@@ -746,5 +858,37 @@ int les_domain_press_stats(les_domain* D, int* iters, double* sor_hist, int hist
 }
 
 long long les_domain_bytes(const les_domain* D) { return D ? D->bytes : 0; }
+
+int les_domain_set_timing(les_domain* D, int on) {
+  if (!D) return fail("null domain");
+  HIP_TRY(hipStreamSynchronize(D->d.stream));
+  if (on && !D->d.timer) D->d.timer = new Timer();
+  if (on) ((Timer*)D->d.timer)->mask = (unsigned int)on;
+  if (!on) timer_free(D);
+  return 0;
+}
+
+int les_domain_timing(les_domain* D, double* ms, long long* count) {
+  if (!D) return fail("null domain");
+  for (int c = 0; c < KC_N; c++) {
+    ms[c] = 0.0;
+    count[c] = 0;
+  }
+  Timer* T = (Timer*)D->d.timer;
+  if (!T) return 0;
+  HIP_TRY(hipStreamSynchronize(D->d.stream));
+  for (size_t q = 0; q < T->used; q++) {
+    float e = 0.0f;
+    HIP_TRY(hipEventElapsedTime(&e, T->ev[2 * q], T->ev[2 * q + 1]));
+    ms[T->cls[q]] += e;
+    count[T->cls[q]] += 1;
+  }
+  T->used = 0;
+  return 0;
+}
+
+int les_kernel_classes(void) { return KC_N; }
+
+const char* les_kernel_class_name(int c) { return (c >= 0 && c < KC_N) ? kClassNames[c] : ""; }
 
 }  // extern "C"

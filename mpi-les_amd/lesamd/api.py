@@ -86,6 +86,10 @@ def lib():
     L.les_domain_press_stats.argtypes = [vp, vp, vp, ci, vp, vp, vp]
     L.les_domain_bytes.argtypes = [vp]
     L.les_domain_bytes.restype = ctypes.c_longlong
+    L.les_domain_set_timing.argtypes = [vp, ci]
+    L.les_domain_timing.argtypes = [vp, vp, vp]
+    L.les_kernel_class_name.restype = ctypes.c_char_p
+    L.les_kernel_class_name.argtypes = [ci]
     _lib = L
     return L
 
@@ -220,6 +224,28 @@ class Domain:
 
     def press_stats(self):
         return _stats(lib().les_domain_press_stats, self._h)
+
+    def set_timing(self, on: bool, classes=None):
+        """Bracket launches with HIP events; classes: names to record (all if None)."""
+        mask = 0
+        if on:
+            if classes is None:
+                mask = -1
+            else:
+                L = lib()
+                names = [L.les_kernel_class_name(c).decode() for c in range(L.les_kernel_classes())]
+                for c in classes:
+                    mask |= 1 << names.index(c)
+        _check(lib().les_domain_set_timing(self._h, mask))
+
+    def timing(self):
+        """{kernel class: (total ms, launches)} since the previous read."""
+        L = lib()
+        n = L.les_kernel_classes()
+        ms = np.zeros(n)
+        cnt = np.zeros(n, dtype=np.int64)
+        _check(L.les_domain_timing(self._h, ms.ctypes.data, cnt.ctypes.data))
+        return {L.les_kernel_class_name(c).decode(): (float(ms[c]), int(cnt[c])) for c in range(n)}
 
     def nbytes(self):
         return lib().les_domain_bytes(self._h)

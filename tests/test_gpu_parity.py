@@ -91,3 +91,42 @@ def test_domain_api_matches_reference_signature_api():
     st = dom.press_stats()
     assert st["iters"] == gpu.press_stats()["iters"]
     dom.close()
+
+
+@pytest.mark.parametrize("kchunk", [1, 2, 5])
+def test_sor_k_chunking(kchunk, monkeypatch):
+    """The fused red-black SOR splits k among blocks that recompute the
+    neighbouring red planes; any split gives the same iterate."""
+    monkeypatch.setenv("LES_SOR_KCHUNK", str(kchunk))
+    s = _mk(dict(im=70, jm=19, km=11, seed=5))
+    print(f"\n[sor chunking] kchunk={kchunk} {s.workload.name}", flush=True)
+    gpu = GpuBackend(s)
+    orc = oracle.OracleBackend(s, 32, redmode=1)
+    for n in range(1, 3):
+        gpu.step(n, 2)
+        orc.step(n, 2)
+    a, b = gpu.fetch(["p", "u", "rhs"]), orc.fetch(["p", "u", "rhs"])
+    for name in ("p", "u", "rhs"):
+        assert rel_l2(a[name], b[name]) <= 1e-6, (name, rel_l2(a[name], b[name]))
+    assert gpu.press_stats()["iters"] == orc.press_stats()["iters"]
+
+
+def test_sor_converges_early_like_reference():
+    """A loose stop threshold makes SOR stop before nmaxp: the iteration
+    count, residual history and iterate must match (the stop test runs one
+    launch behind on the GPU)."""
+    s = _mk(dict(im=24, jm=20, km=16, seed=6, pjuge=50.0))
+    print(f"\n[sor early stop] {s.workload.name}", flush=True)
+    gpu = GpuBackend(s)
+    orc = oracle.OracleBackend(s, 32, redmode=1)
+    seen = set()
+    for n in range(1, 4):
+        gpu.step(n, 3)
+        orc.step(n, 3)
+        sg, so = gpu.press_stats(), orc.press_stats()
+        assert sg["iters"] == so["iters"], (n, sg["iters"], so["iters"])
+        seen.add(so["iters"])
+        np.testing.assert_allclose(sg["sor"], so["sor"], rtol=1e-9)
+        a, b = gpu.fetch(["p"]), orc.fetch(["p"])
+        assert rel_l2(a["p"], b["p"]) <= 1e-6
+    assert min(seen) < 50, seen
