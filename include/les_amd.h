@@ -143,6 +143,10 @@ int les_domain_create(const les_domain_desc* desc, les_domain** out);
 int les_domain_destroy(les_domain* d);
 /* One time step n (main.f95:260-296): the seven stages in order. */
 int les_domain_step(les_domain* d, int n, int nmax);
+/* With on != 0, steps n > 1 replay one captured HIP graph of the whole
+ * step (recaptured when the domain's constants change); results are the
+ * same as eager launches.  Off by default; ignored while timing is on. */
+int les_domain_set_graph(les_domain* d, int on);
 /* Advance steps n0..n1 inclusive. */
 int les_domain_run(les_domain* d, int n0, int n1, int nmax);
 /* Stage s of step n: 0 velnw, 1 bondv1, 2 velfg, 3 feedbf, 4 les, 5 adam, 6 press. */

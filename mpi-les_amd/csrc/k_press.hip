@@ -9,27 +9,25 @@
 // results are run-to-run reproducible.  No kernel waits on another block:
 // each reduction writes per-block partials and a later launch adds them.
 //
-// SOR: one launch per iteration updates the red cells (i+j+k odd, nrd=0 at
-// press.f95:92) and then the black cells of the same iteration.  A block
-// owns a TX x TY column of cells and a range of k levels and marches k
-// upward: at level k it computes the new red values of plane k over its
-// tile plus a one-cell halo (from the previous iterate, held in an LDS ring
-// with a two-cell halo), then the new black values of plane k-1 from those
-// red values.  The previous iterate is read from one buffer and the new one
-// written to the other (the two alternate), so blocks never read what
-// another block writes.  Ghost reads reproduce boundp1/boundp2 exactly:
-// across i=0/i=im+1 and k=0/k=km+1 a cell reads its own pre-update value
-// (the ghost is a copy of it: boundp.f95:16-21,44-58 run after the
-// colour that last changed it), and across j=0/j=jm+1 it reads the
-// periodic image (boundp.f95:63-69): the red sweep sees the previous
-// iterate, the black sweep sees the new red value, or -- when jm is odd
-// and the image has the same colour -- the previous iterate.
+// SOR storage: the red cells (i+j+k odd, updated first: nrd=0 at
+// press.f95:92) and the black cells live in two compact arrays, pair t of a
+// (j,k) row holding i = 2t+1 and i = 2t+2, so a colour sweep streams
+// contiguous memory: the same-row neighbours i-1, i+1 of a cell at t are at
+// t-1+d and t+d of the other colour (d = 0 when the cell is the odd-i one),
+// its j and k neighbours at t.  Each colour has two buffers: iteration l
+// reads the buffers of parity l-1 and writes those of parity l, so a sweep
+// never reads what it writes.  Ghost reads reproduce boundp1/boundp2
+// exactly: across i=0/im+1 and k=0/km+1 a cell reads its own pre-update
+// value (the ghost copy, boundp.f95:16-21,44-58, was taken after the last
+// change of that cell), across j=0/jm+1 the periodic image (boundp.f95:
+// 63-69) -- of the other colour when jm is even (the previous iterate for
+// the red sweep, the new red value for the black sweep), of the same colour
+// when jm is odd (always the previous iterate).
 //
 // The stop test (press.f95:119-121) runs one iteration behind: block 0 of
-// iteration l+1 adds the partials of iteration l and records the result;
-// iteration l+2 then finds the flag and exits.  When iteration l converged,
-// iteration l+1 has written only the other buffer, so iteration l's result
-// is intact and is the one kept.
+// iteration l+1's red sweep adds the partials of iteration l and records
+// the result; later sweeps find the flag and exit.  Iteration l's values
+// stay intact because iteration l+1 writes the other buffers.
 #include "les_kernels.h"
 
 namespace les {
@@ -43,13 +41,17 @@ static inline dim3 blk() { return dim3(BX, BY, 1); }
 // at most this many blocks write partials in one reduction launch
 constexpr int kMaxRedBlocks = 2048;
 
-// k-blocks of a reduction launch: at most `cap` blocks in total unless one
-// k-block alone has more (the partial buffer holds max(cap, gx*gy)).
+// k-blocks of a launch: at most `cap` blocks in total unless one k-block
+// alone has more (partial buffers hold max(cap, gx*gy)).
 static inline int kblocks(int gx, int gy, int nk, int cap) {
   int gz = cap / (gx * gy);
   if (gz > nk) gz = nk;
   if (gz < 1) gz = 1;
   return gz;
+}
+
+__device__ __forceinline__ int cix(const SplitGrid& s, int t, int j, int k) {
+  return t + (j - 1) * s.cp + (k - 1) * s.cplane;
 }
 
 // ---------------------------------------------------------------- rhs
@@ -112,236 +114,211 @@ __global__ __launch_bounds__(256) void k_finalize(const double* __restrict__ par
   }
 }
 
-// rhs = rhs - rhsav (press.f95:81-87), the subtraction done in FP64.
-__global__ __launch_bounds__(NT) void k_rhs_sub(Grid g, float* __restrict__ rhs, const Ctrl* ctrl) {
+// rhs = rhs - rhsav (press.f95:81-87, the subtraction in FP64), and the
+// colour-split copies the SOR reads: rhs, the starting iterate p, and cn1
+// when it is not recomputed.
+__global__ __launch_bounds__(NT) void k_rhs_split(Grid g, SplitGrid s, float* __restrict__ rhs,
+                                                  const float* __restrict__ p,
+                                                  const float* __restrict__ cn1, SplitBufs sb,
+                                                  const Ctrl* ctrl) {
   const int i = 1 + blockIdx.x * BX + threadIdx.x, j = 1 + blockIdx.y * BY + threadIdx.y;
   if (i > g.im || j > g.jm) return;
   const double av = ctrl->rhsav;
+  const int t = (i - 1) >> 1;
   for (int k = 1 + blockIdx.z; k <= g.km; k += gridDim.z) {
     const int c = ix(g, i, j, k);
-    rhs[c] = (float)((double)rhs[c] - av);
+    const int col = (i + j + k) & 1;  // 1 = red
+    const float r = (float)((double)rhs[c] - av);
+    rhs[c] = r;
+    const int q = cix(s, t, j, k);
+    sb.rhs[col][q] = r;
+    sb.p[0][col][q] = p[c];
+    if (sb.cn1[0]) sb.cn1[col][q] = cn1[c];
   }
 }
 
 // ---------------------------------------------------------------- SOR
-constexpr int STX = 64;            // tile width (i)
-constexpr int STY = 8;             // tile height (j)
-constexpr int SNT = 256;           // threads per block
-constexpr int AXW = STX + 4;       // previous-iterate plane with a 2-cell halo
-constexpr int AYW = STY + 4;
-constexpr int MXW = STX + 2;       // merged plane (new red, old black) with a 1-cell halo
-constexpr int MYW = STY + 2;
-
-struct SorBufs {
-  const float* p;     // iterate read by iteration 1 (the field p)
-  float* s0;          // alternating iterates: iteration l writes s[l&1]
-  float* s1;
-  double* part0;      // per-block sum(reltmp^2): iteration l writes part[l&1]
-  double* part1;
-};
-
-__device__ __forceinline__ int wrapj(int j, int jm) {
-  int r = (j - 1) % jm;
-  if (r < 0) r += jm;
-  return r + 1;
+// One colour sweep of iteration `iter` (press.f95:92-104).  COL 1 = red
+// (the first sweep), 0 = black.  Red sweeps of iteration >= 2 run the
+// lagged stop test in block 0.
+// Workgroups are dispatched to the 8 XCDs round-robin (workgroup b runs on
+// XCD b % 8).  Renumber them so each XCD receives a contiguous range of
+// tiles: neighbouring j rows and k planes then share that XCD's L2.
+__device__ __forceinline__ int xcd_contiguous(int b, int nb) {
+  constexpr int kXcd = 8;
+  const int x = b % kXcd, l = b / kXcd;
+  const int base = nb / kXcd, rem = nb % kXcd;
+  return x * base + (x < rem ? x : rem) + l;
 }
 
-template <bool SEP>
-__device__ __forceinline__ float sor_new(const Grid& g, const Metrics& m, const float* __restrict__ cn1,
-                                         const float* __restrict__ rhs, int i, int j, int k, float own,
-                                         float pim, float pip, float pjm, float pjp, float pkm, float pkp,
-                                         float omega, float* reltmp_out) {
-  const int c = ix(g, i, j, k);
-  float cn;
-  if (SEP)
-    cn = 1.0f / ((M1(m.cnax, i) + M1(m.cnby, j)) + M1(m.cncz, k));  // init.f95:100-107
-  else
-    cn = cn1[c];
-  const float reltmp = omega * (cn * (M1(m.cn2l, i) * pip + M1(m.cn2s, i) * pim + M1(m.cn3l, j) * pjp +
-                                      M1(m.cn3s, j) * pjm + M1(m.cn4l, k) * pkp + M1(m.cn4s, k) * pkm -
-                                      rhs[c]) -
-                                own);
-  *reltmp_out = reltmp;
-  return own + reltmp;
-}
-
-template <bool SEP>
-__global__ __launch_bounds__(SNT) void k_sor_iter(Grid g, Metrics m, SorBufs sb,
-                                                  const float* __restrict__ cn1,
-                                                  const float* __restrict__ rhs, Ctrl* ctrl, int iter,
-                                                  int kchunk, int nparts, float omega, float pjuge) {
-  __shared__ float As[4][AYW][AXW];
-  __shared__ float Ms[3][MYW][MXW];
-  __shared__ double lds[SNT / kWave];
+template <int COL, bool SEP>
+__global__ __launch_bounds__(NT) void k_sweep(Grid g, SplitGrid s, Metrics m, SplitBufs sb, Ctrl* ctrl,
+                                              int iter, int nparts, float omega, float pjuge, int gx,
+                                              int gy, int kchunk, int remap) {
+  __shared__ double lds[NT / kWave];
   __shared__ int skip;
-  const int tid = threadIdx.x;
-  // Block 0 of this launch may set the flag while other blocks start; read
-  // it once per block so the whole block takes one decision.  A block that
-  // sees the flag set by this launch's block 0 leaves its part of an
-  // iterate that is discarded anyway.
+  const int tid = threadIdx.y * BX + threadIdx.x;
+  // Block 0 of a red sweep may set the flag while other blocks start; each
+  // block reads it once.  Values a skipped block leaves unwritten belong to
+  // an iterate that is discarded.
   if (tid == 0) skip = (ctrl->sor_done != 0);
   __syncthreads();
   if (skip) return;
-  const int bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
-  if (iter >= 2 && bid == 0) {
-    // stop test of the previous iteration (press.f95:119-121)
-    const double* pp = ((iter - 1) & 1) ? sb.part1 : sb.part0;
-    double s = 0.0;
-    for (int b = tid; b < nparts; b += SNT) s += pp[b];
-    s = block_sum<SNT>(s, tid, lds);
+  const int bid = blockIdx.x;
+  if (COL == 1 && iter >= 2 && bid == 0) {
+    const int pp = (iter - 1) & 1;
+    double sum = 0.0;
+    for (int b = tid; b < nparts; b += NT) sum += sb.part[pp][1][b] + sb.part[pp][0][b];
+    sum = block_sum<NT>(sum, tid, lds);
     if (tid == 0) {
-      if (iter - 1 <= kSorHist) ctrl->sor_hist[iter - 2] = s;
+      if (iter - 1 <= kSorHist) ctrl->sor_hist[iter - 2] = sum;
       ctrl->sor_iters = iter - 1;
-      if (!(s == s) || s > 1.0e300) ctrl->nonfinite = 1;
-      if (s < (double)pjuge || ctrl->nonfinite) ctrl->sor_done = iter - 1;
+      if (!(sum == sum) || sum > 1.0e300) ctrl->nonfinite = 1;
+      if (sum < (double)pjuge || ctrl->nonfinite) ctrl->sor_done = iter - 1;  // press.f95:119-121
     }
   }
-  const float* __restrict__ A = (iter == 1) ? sb.p : (((iter - 1) & 1) ? sb.s1 : sb.s0);
-  float* __restrict__ B = (iter & 1) ? sb.s1 : sb.s0;
-  double* part = (iter & 1) ? sb.part1 : sb.part0;
+  const int prv = (iter - 1) & 1, cur = iter & 1;
+  const float* __restrict__ own_in = sb.p[prv][COL];
+  float* __restrict__ out = sb.p[cur][COL];
+  // other colour: the previous iterate for the red sweep, the new red values for the black sweep
+  const float* __restrict__ oth = (COL == 1) ? sb.p[prv][0] : sb.p[cur][1];
+  const float* __restrict__ rhsc = sb.rhs[COL];
+  const float* __restrict__ cnc = sb.cn1[COL];
   const int im = g.im, jm = g.jm, km = g.km;
-  const int i0 = 1 + blockIdx.x * STX, j0 = 1 + blockIdx.y * STY;
-  const int k0 = 1 + blockIdx.z * kchunk;
-  const int k1 = min(km, k0 + kchunk - 1);
-
-  auto loadA = [&](int k) {
-    if (k < 1 || k > km) return;
-    float(*P)[AXW] = As[k & 3];
-    for (int e = tid; e < AXW * AYW; e += SNT) {
-      const int r = e / AXW, q = e - r * AXW;
-      const int i = i0 - 2 + q;
-      const int j = wrapj(j0 - 2 + r, jm);
-      P[r][q] = (i >= 1 && i <= im) ? A[ix(g, i, j, k)] : 0.0f;
-    }
-  };
-
+  const int lb = remap ? xcd_contiguous(bid, nparts) : bid;
+  const int bx = lb % gx, by = (lb / gx) % gy, bz = lb / (gx * gy);
+  const int t = bx * BX + threadIdx.x, j = 1 + by * BY + threadIdx.y;
+  const int k0 = 1 + bz * kchunk, k1 = min(km, k0 + kchunk - 1);
   double acc = 0.0;
-  loadA(k0 - 2);
-  loadA(k0 - 1);
-  for (int k = k0 - 1; k <= k1 + 1; k++) {
-    loadA(k + 1);
-    __syncthreads();
-    // ---- red cells of plane k over the tile and a one-cell halo
-    if (k >= 1 && k <= km) {
-      const float(*P)[AXW] = As[k & 3];
-      const float(*Pd)[AXW] = As[(k - 1) & 3];
-      const float(*Pu)[AXW] = As[(k + 1) & 3];
-      float(*M)[MXW] = Ms[k % 3];
-      for (int e = tid; e < MXW * MYW; e += SNT) {
-        const int r = e / MXW, q = e - r * MXW;
-        const int i = i0 - 1 + q;
-        const int jj = j0 - 1 + r;
-        const int j = wrapj(jj, jm);
-        const float own = P[r + 1][q + 1];
-        float val = own;
-        if (i >= 1 && i <= im && ((i + j + k) & 1)) {
-          const float pim = (i == 1) ? own : P[r + 1][q];
-          const float pip = (i == im) ? own : P[r + 1][q + 2];
-          const float pkm = (k == 1) ? own : Pd[r + 1][q + 1];
-          const float pkp = (k == km) ? own : Pu[r + 1][q + 1];
-          float rel;
-          val = sor_new<SEP>(g, m, cn1, rhs, i, j, k, own, pim, pip, P[r][q + 1], P[r + 2][q + 1], pkm, pkp,
-                             omega, &rel);
-          if (r >= 1 && r <= STY && q >= 1 && q <= STX && jj <= jm && k >= k0 && k <= k1)
-            acc += (double)rel * (double)rel;
-        }
-        M[r][q] = val;
+  if (j <= jm && t < s.nt && k0 <= km) {
+    const bool jodd = (jm & 1) != 0;
+    const int jlo = (j == 1) ? jm : j - 1, jhi = (j == jm) ? 1 : j + 1;
+    // j neighbours across the periodic boundary have this colour when jm is odd
+    const float* __restrict__ src_jlo = (j == 1 && jodd) ? own_in : oth;
+    const float* __restrict__ src_jhi = (j == jm && jodd) ? own_in : oth;
+    const int djlo = (jlo - j) * s.cp, djhi = (jhi - j) * s.cp;
+    const float c3l = M1(m.cn3l, j), c3s = M1(m.cn3s, j);
+    const float byj = SEP ? M1(m.cnby, j) : 0.0f;
+    // Per level k the cell is (t, j, k) with i = 2t+1+d.  Its other-colour
+    // neighbours: a = same t (i-1 when d=1, i+1 when d=0), b = t+1 or t-1
+    // (the other i side), and, at levels k-1 and k+1, the a of those levels.
+    // Loads for level k+1 are issued before level k is computed.
+    struct Lv {
+      float own, a, b, jl, jh, r, cn;
+    };
+    auto load = [&](int k, Lv& L) {
+      const int d = (COL + 1 + j + k) & 1;
+      const int q = cix(s, t, j, k);
+      int tb = t + (d ? 1 : -1);
+      tb = (tb < 0 || tb >= s.nt) ? t : tb;  // outside the row only where the boundary rule applies
+      L.own = own_in[q];
+      L.a = oth[q];
+      L.b = oth[q - t + tb];
+      L.jl = src_jlo[q + djlo];
+      L.jh = src_jhi[q + djhi];
+      L.r = rhsc[q];
+      L.cn = SEP ? 0.0f : cnc[q];
+    };
+    Lv cur, nxt;
+    float a_prev = (k0 > 1) ? oth[cix(s, t, j, k0 - 1)] : 0.0f;
+    load(k0, cur);
+    for (int k = k0; k <= k1; k++) {
+      float a_next = 0.0f;
+      if (k < k1)
+        load(k + 1, nxt);
+      else if (k < km)
+        a_next = oth[cix(s, t, j, k + 1)];
+      if (k < k1) a_next = nxt.a;
+      const int d = (COL + 1 + j + k) & 1;
+      const int i = 2 * t + 1 + d;
+      if (i <= im) {
+        const float own = cur.own;
+        const float lo = d ? cur.a : cur.b, hi = d ? cur.b : cur.a;  // values at i-1, i+1
+        const float pim = (i == 1) ? own : lo;
+        const float pip = (i == im) ? own : hi;
+        const float pkm = (k == 1) ? own : a_prev;
+        const float pkp = (k == km) ? own : a_next;
+        const float cn = SEP ? 1.0f / ((M1(m.cnax, i) + byj) + M1(m.cncz, k)) : cur.cn;  // init.f95:100-107
+        const float reltmp = omega * (cn * (M1(m.cn2l, i) * pip + M1(m.cn2s, i) * pim + c3l * cur.jh +
+                                            c3s * cur.jl + M1(m.cn4l, k) * pkp + M1(m.cn4s, k) * pkm -
+                                            cur.r) -
+                                      own);
+        out[cix(s, t, j, k)] = own + reltmp;
+        acc += (double)reltmp * (double)reltmp;
       }
-    }
-    __syncthreads();
-    // ---- black cells of plane kb = k-1 of the tile; store the plane
-    const int kb = k - 1;
-    if (kb >= k0 && kb <= k1) {
-      const float(*P)[AXW] = As[kb & 3];
-      const float(*M)[MXW] = Ms[kb % 3];
-      const float(*Md)[MXW] = Ms[(kb + 2) % 3];  // plane kb-1
-      const float(*Mu)[MXW] = Ms[(kb + 1) % 3];  // plane kb+1
-      for (int e = tid; e < STX * STY; e += SNT) {
-        const int r = e / STX, q = e - r * STX;
-        const int i = i0 + q, j = j0 + r;
-        if (i > im || j > jm) continue;
-        float val;
-        if ((i + j + kb) & 1) {
-          val = M[r + 1][q + 1];  // red, already updated
-        } else {
-          const float own = P[r + 2][q + 2];
-          const float pim = (i == 1) ? own : M[r + 1][q];
-          const float pip = (i == im) ? own : M[r + 1][q + 2];
-          const float pkm = (kb == 1) ? own : Md[r + 1][q + 1];
-          const float pkp = (kb == km) ? own : Mu[r + 1][q + 1];
-          float rel;
-          val = sor_new<SEP>(g, m, cn1, rhs, i, j, kb, own, pim, pip, M[r][q + 1], M[r + 2][q + 1], pkm, pkp,
-                             omega, &rel);
-          acc += (double)rel * (double)rel;
-        }
-        B[ix(g, i, j, kb)] = val;
-      }
+      a_prev = cur.a;
+      cur = nxt;
     }
   }
-  const double s = block_sum<SNT>(acc, tid, lds);
-  if (tid == 0) part[bid] = s;
+  const double sum = block_sum<NT>(acc, tid, lds);
+  if (tid == 0) sb.part[cur][COL][bid] = sum;
 }
 
 // After the iteration loop: when no iteration met the stop test, the last
 // one (nmaxp) is summed here.  Records the iteration count and sum.
-__global__ __launch_bounds__(256) void k_sor_final(SorBufs sb, Ctrl* ctrl, int nmaxp, int nparts,
-                                                   float pjuge) {
+__global__ __launch_bounds__(256) void k_sor_final(SplitBufs sb, Ctrl* ctrl, int nmaxp, int nparts) {
   __shared__ double lds[4];
-  if (ctrl->sor_done != 0) return;  // uniform across the block
-  const double* pp = (nmaxp & 1) ? sb.part1 : sb.part0;
-  double s = 0.0;
-  for (int b = threadIdx.x; b < nparts; b += 256) s += pp[b];
-  s = block_sum<256>(s, threadIdx.x, lds);
+  if (ctrl->sor_done != 0) return;  // uniform: written by earlier launches
+  const int pp = nmaxp & 1;
+  double sum = 0.0;
+  for (int b = threadIdx.x; b < nparts; b += 256) sum += sb.part[pp][1][b] + sb.part[pp][0][b];
+  sum = block_sum<256>(sum, threadIdx.x, lds);
   if (threadIdx.x == 0) {
-    if (nmaxp <= kSorHist) ctrl->sor_hist[nmaxp - 1] = s;
+    if (nmaxp <= kSorHist) ctrl->sor_hist[nmaxp - 1] = sum;
     ctrl->sor_iters = nmaxp;
-    if (!(s == s) || s > 1.0e300) ctrl->nonfinite = 1;
+    if (!(sum == sum) || sum > 1.0e300) ctrl->nonfinite = 1;
   }
 }
 
-__device__ __forceinline__ const float* sor_result(const SorBufs& sb, const Ctrl* ctrl) {
-  const int L = ctrl->sor_iters;
-  return (L == 0) ? sb.p : ((L & 1) ? sb.s1 : sb.s0);
-}
-
 // pav partials: p*dx1*dy1*dzn over the SOR result (press.f95:124-133).
-__global__ __launch_bounds__(NT) void k_pav(Grid g, Metrics m, SorBufs sb, double* __restrict__ part,
-                                            const Ctrl* ctrl) {
+__global__ __launch_bounds__(NT) void k_pav(Grid g, SplitGrid s, Metrics m, SplitBufs sb,
+                                            double* __restrict__ part, const Ctrl* ctrl) {
   __shared__ double lds[NT / kWave];
   const int tid = threadIdx.y * BX + threadIdx.x;
   const int bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
-  const float* __restrict__ p = sor_result(sb, ctrl);
+  const int L = ctrl->sor_iters;
+  const float* __restrict__ red = sb.p[L & 1][1];
+  const float* __restrict__ blk = sb.p[L & 1][0];
   const int i = 1 + blockIdx.x * BX + threadIdx.x, j = 1 + blockIdx.y * BY + threadIdx.y;
   double acc = 0.0;
   if (i <= g.im && j <= g.jm) {
     const float dxi = M1(m.dx1, i), dyj = M1(m.dy1, j);
-    for (int k = 1 + blockIdx.z; k <= g.km; k += gridDim.z)
-      acc += (double)(p[ix(g, i, j, k)] * dxi * dyj * M1(m.dzn, k));
+    const int t = (i - 1) >> 1;
+    for (int k = 1 + blockIdx.z; k <= g.km; k += gridDim.z) {
+      const float pv = (((i + j + k) & 1) ? red : blk)[cix(s, t, j, k)];
+      acc += (double)(pv * dxi * dyj * M1(m.dzn, k));
+    }
   }
-  const double s = block_sum<NT>(acc, tid, lds);
-  if (tid == 0) part[bid] = s;
+  const double sum = block_sum<NT>(acc, tid, lds);
+  if (tid == 0) part[bid] = sum;
 }
 
 // p = p - pav on the interior (press.f95:145-150, FP64 subtraction) and the
 // ghost cells of boundp1 then boundp2 (press.f95:155-159): every ghost
 // receives the adjusted value of the interior cell the copies trace it to
-// (i: 0->1, im+1->im; j: 0->jm, jm+1->1; k: 0->1, km+1->km).  Reads the SOR
-// result, writes p; covers i,j,k = 0..im+1, 0..jm+1, 0..km+1.
-__global__ __launch_bounds__(NT) void k_padj(Grid g, SorBufs sb, float* __restrict__ p, const Ctrl* ctrl) {
+// (i: 0->1, im+1->im; j: 0->jm, jm+1->1; k: 0->1, km+1->km).  Covers
+// i,j,k = 0..im+1, 0..jm+1, 0..km+1.
+__global__ __launch_bounds__(NT) void k_padj(Grid g, SplitGrid s, SplitBufs sb, float* __restrict__ p,
+                                             const Ctrl* ctrl) {
   const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
   if (i > g.im + 1 || j > g.jm + 1) return;
-  const float* __restrict__ src = sor_result(sb, ctrl);
+  const int L = ctrl->sor_iters;
+  const float* __restrict__ red = sb.p[L & 1][1];
+  const float* __restrict__ blk = sb.p[L & 1][0];
   const double av = ctrl->pav;
   const int ii = (i == 0) ? 1 : ((i == g.im + 1) ? g.im : i);
   const int jj = (j == 0) ? g.jm : ((j == g.jm + 1) ? 1 : j);
+  const int t = (ii - 1) >> 1;
   for (int k = blockIdx.z; k <= g.km + 1; k += gridDim.z) {
     const int kk = (k == 0) ? 1 : ((k == g.km + 1) ? g.km : k);
-    p[ix(g, i, j, k)] = (float)((double)src[ix(g, ii, jj, kk)] - av);
+    const float v = (((ii + jj + kk) & 1) ? red : blk)[cix(s, t, jj, kk)];
+    p[ix(g, i, j, k)] = (float)((double)v - av);
   }
 }
 
 void launch_press(const DevDomain& d, float dt) {
   const Grid& g = d.g;
+  const SplitGrid& s = d.sg;
   const int gx = (g.im + BX - 1) / BX, gy = (g.jm + BY - 1) / BY;
   const int gz = kblocks(gx, gy, g.km, kMaxRedBlocks);
   const int nred = gx * gy * gz;
@@ -349,42 +326,55 @@ void launch_press(const DevDomain& d, float dt) {
                                       d.F.w, d.F.f, d.F.g, d.F.h, d.F.rhs, d.partials, d.ctrl, dt));
   TIMED(d, KC_RHS, hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, d.stream, d.partials, nred, d.ctrl, 0,
                                       d.area));
-  TIMED(d, KC_RHS, hipLaunchKernelGGL(k_rhs_sub, dim3(gx, gy, g.km), blk(), 0, d.stream, g, d.F.rhs,
-                                      (const Ctrl*)d.ctrl));
-  SorBufs sb;
-  sb.p = d.F.p;
-  sb.s0 = d.F.ps[0];
-  sb.s1 = d.F.ps[1];
-  sb.part0 = d.partials + d.npartials;
-  sb.part1 = d.partials + d.npartials + d.sor_parts_cap;
-  const int tx = (g.im + STX - 1) / STX, ty = (g.jm + STY - 1) / STY;
-  const int nkc = (g.km + d.sor_kchunk - 1) / d.sor_kchunk;
-  const int nparts = tx * ty * nkc;
+  SplitBufs sb = d.sb;
+  if (d.cn1_separable) sb.cn1[0] = sb.cn1[1] = nullptr;
+  TIMED(d, KC_RHS, hipLaunchKernelGGL(k_rhs_split, dim3(gx, gy, g.km), blk(), 0, d.stream, g, s, d.F.rhs,
+                                      (const float*)d.F.p, (const float*)d.F.cn1, sb, (const Ctrl*)d.ctrl));
+  const int tx = (s.nt + BX - 1) / BX;
+  const int kc = sor_kchunk(g, s, d.sor_blocks);
+  const int tz = (g.km + kc - 1) / kc;
+  const int nparts = tx * gy * tz;
+  const dim3 sgrid(nparts);
   for (int l = 1; l <= d.nmaxp; l++) {
-    if (d.cn1_separable)
-      TIMED(d, KC_SOR, hipLaunchKernelGGL(k_sor_iter<true>, dim3(tx, ty, nkc), dim3(SNT), 0, d.stream, g, d.m, sb,
-                                          (const float*)d.F.cn1, (const float*)d.F.rhs, d.ctrl, l, d.sor_kchunk,
-                                          nparts, d.omega, d.pjuge));
-    else
-      TIMED(d, KC_SOR, hipLaunchKernelGGL(k_sor_iter<false>, dim3(tx, ty, nkc), dim3(SNT), 0, d.stream, g, d.m, sb,
-                                          (const float*)d.F.cn1, (const float*)d.F.rhs, d.ctrl, l, d.sor_kchunk,
-                                          nparts, d.omega, d.pjuge));
+    if (d.cn1_separable) {
+      TIMED(d, KC_SOR, hipLaunchKernelGGL((k_sweep<1, true>), sgrid, blk(), 0, d.stream, g, s, d.m, sb, d.ctrl,
+                                          l, nparts, d.omega, d.pjuge, tx, gy, kc, d.xcd_remap));
+      TIMED(d, KC_SOR, hipLaunchKernelGGL((k_sweep<0, true>), sgrid, blk(), 0, d.stream, g, s, d.m, sb, d.ctrl,
+                                          l, nparts, d.omega, d.pjuge, tx, gy, kc, d.xcd_remap));
+    } else {
+      TIMED(d, KC_SOR, hipLaunchKernelGGL((k_sweep<1, false>), sgrid, blk(), 0, d.stream, g, s, d.m, sb, d.ctrl,
+                                          l, nparts, d.omega, d.pjuge, tx, gy, kc, d.xcd_remap));
+      TIMED(d, KC_SOR, hipLaunchKernelGGL((k_sweep<0, false>), sgrid, blk(), 0, d.stream, g, s, d.m, sb, d.ctrl,
+                                          l, nparts, d.omega, d.pjuge, tx, gy, kc, d.xcd_remap));
+    }
   }
-  TIMED(d, KC_PAV, hipLaunchKernelGGL(k_sor_final, dim3(1), dim3(256), 0, d.stream, sb, d.ctrl, d.nmaxp, nparts,
-                                      d.pjuge));
-  TIMED(d, KC_PAV, hipLaunchKernelGGL(k_pav, dim3(gx, gy, gz), blk(), 0, d.stream, g, d.m, sb, d.partials,
+  TIMED(d, KC_PAV, hipLaunchKernelGGL(k_sor_final, dim3(1), dim3(256), 0, d.stream, sb, d.ctrl, d.nmaxp, nparts));
+  TIMED(d, KC_PAV, hipLaunchKernelGGL(k_pav, dim3(gx, gy, gz), blk(), 0, d.stream, g, s, d.m, sb, d.partials,
                                       (const Ctrl*)d.ctrl));
   TIMED(d, KC_PAV, hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, d.stream, d.partials, nred, d.ctrl, 1,
                                       d.area));
   TIMED(d, KC_PADJ, hipLaunchKernelGGL(k_padj, dim3((g.im + 2 + BX - 1) / BX, (g.jm + 2 + BY - 1) / BY, g.km + 2),
-                                       blk(), 0, d.stream, g, sb, d.F.p, (const Ctrl*)d.ctrl));
+                                       blk(), 0, d.stream, g, s, sb, d.F.p, (const Ctrl*)d.ctrl));
 }
 
-int sor_parts_needed(const Grid& g, int kchunk) {
-  const int tx = (g.im + STX - 1) / STX, ty = (g.jm + STY - 1) / STY;
-  return tx * ty * ((g.km + kchunk - 1) / kchunk);
+// k levels per SOR block so that the sweep has about `target` blocks
+int sor_kchunk(const Grid& g, const SplitGrid& s, int target) {
+  const int tx = (s.nt + BX - 1) / BX, gy = (g.jm + BY - 1) / BY;
+  int tz = target / (tx * gy);
+  if (tz > g.km) tz = g.km;
+  if (tz < 1) tz = 1;
+  return (g.km + tz - 1) / tz;
 }
 
-int reduction_parts_needed() { return kMaxRedBlocks; }
+int sor_parts_needed(const Grid& g, const SplitGrid& s, int target) {
+  const int tx = (s.nt + BX - 1) / BX, gy = (g.jm + BY - 1) / BY;
+  const int kc = sor_kchunk(g, s, target);
+  return tx * gy * ((g.km + kc - 1) / kc);
+}
+
+int reduction_parts_needed(const Grid& g) {
+  const int gx = (g.im + BX - 1) / BX, gy = (g.jm + BY - 1) / BY;
+  return gx * gy * kblocks(gx, gy, g.km, kMaxRedBlocks);
+}
 
 }  // namespace les

@@ -17,7 +17,20 @@ struct DevFields {
   float* cov[10];                 // cov1..cov9 (index 0 unused)
   float *bmask, *cmask, *dmask;   // IBM masks
   float* cn1;                     // Poisson diagonal (1:ip,1:jp,1:kp)
-  float* ps[2];                   // SOR iterates (alternating)
+};
+
+// Colour-split SOR storage: pair t of row (j,k) holds i = 2t+1, 2t+2; one
+// array per colour, index t + (j-1)*cp + (k-1)*cplane.
+struct SplitGrid {
+  int nt;      // pairs per row, (im+1)/2
+  int cp;      // row pitch (floats)
+  int cplane;  // plane pitch (floats)
+};
+struct SplitBufs {
+  float* p[2][2];       // [iterate parity][colour: 0 black, 1 red]
+  float* rhs[2];        // [colour]
+  float* cn1[2];        // [colour]; unused while cn1 is recomputed
+  double* part[2][2];   // [iterate parity][colour] per-block sums of reltmp^2
 };
 
 // Kernel classes for the optional per-launch timing (les_domain_set_timing).
@@ -40,13 +53,16 @@ struct DevDomain {
   float pjuge;       // SOR stop threshold on sum(reltmp^2) (press.f95:43)
   float omega;       // SOR relaxation (press.f95:45)
   void* timer;       // host-side event recorder, nullptr when timing is off
-  int sor_kchunk;    // k levels per SOR block
-  int sor_parts_cap; // SOR partials per iteration slot (two slots follow the reduction partials)
   int cn1_separable; // cn1 equals 1/(cnax+cnby+cncz) bit for bit: recompute instead of reading it
+  SplitGrid sg;
+  SplitBufs sb;
+  int sor_blocks;    // target number of blocks of one SOR sweep
+  int xcd_remap;     // give each XCD a contiguous range of SOR tiles
 };
 
-int sor_parts_needed(const Grid& g, int kchunk);
-int reduction_parts_needed();
+int sor_kchunk(const Grid& g, const SplitGrid& s, int target);
+int sor_parts_needed(const Grid& g, const SplitGrid& s, int target);
+int reduction_parts_needed(const Grid& g);
 
 // Event pairs around launches, recorded only when d.timer is set.
 void timer_begin(const DevDomain& d, int cls);

@@ -156,6 +156,11 @@ struct les_domain {
   int moff[NMETRICS] = {};          // offset of each metric in the pack
   bool mdirty = true;
   bool have_inflow = false;
+  float* split = nullptr;           // colour-split SOR arrays
+  int use_graph = 0;                // replay steps n > 1 from a captured graph
+  hipGraphExec_t gexec = nullptr;   // captured step
+  DevDomain gkey{};                 // domain state the graph was captured with
+  float gscal[5] = {};              // ro, vn, alpha, beta, dt at capture
   float* staging = nullptr;         // reference-layout staging buffer
   size_t staging_floats = 0;
   const void* bound[NFIELDS] = {};  // host address each field mirrors (reference-signature layer)
@@ -311,6 +316,11 @@ int set_inflow(les_domain* D, const float* prof) {
   return 0;
 }
 
+int env_int(const char* name, int dflt) {
+  const char* v = getenv(name);
+  return (v && *v) ? atoi(v) : dflt;
+}
+
 int nmaxp_default = 50;
 float pjuge_default = 1.0e-4f;
 float omega_default = 1.0f;
@@ -326,12 +336,12 @@ void timer_free(les_domain* D) {
 void domain_free(les_domain* D) {
   if (!D) return;
   timer_free(D);
+  if (D->gexec) hipGraphExecDestroy(D->gexec);
   for (int f = 0; f < NFIELDS; f++)
     if (D->fields[f]) hipFree(D->fields[f]);
   for (int q = 1; q <= 9; q++)
     if (D->cov[q]) hipFree(D->cov[q]);
-  for (int q = 0; q < 2; q++)
-    if (D->d.F.ps[q]) hipFree(D->d.F.ps[q]);
+  if (D->split) hipFree(D->split);
   if (D->mbuf) hipFree(D->mbuf);
   if (D->inflow_dev) hipFree(D->inflow_dev);
   if (D->staging) hipFree(D->staging);
@@ -375,12 +385,24 @@ int domain_alloc(int im, int jm, int km, les_domain** out) {
     if (e != hipSuccess) return bail(std::string("scratch allocation: ") + hipGetErrorString(e));
     D->bytes += nb;
   }
-  for (int q = 0; q < 2; q++) {
-    const size_t nb = (size_t)n3 * sizeof(float);
-    e = hipMalloc(&D->d.F.ps[q], nb);
-    if (e == hipSuccess) e = hipMemset(D->d.F.ps[q], 0, nb);
+  {
+    // colour-split SOR arrays: 4 iterate buffers, 2 rhs, 2 cn1
+    SplitGrid& sg = D->d.sg;
+    sg.nt = (im + 1) / 2;
+    sg.cp = (sg.nt + 31) / 32 * 32;
+    sg.cplane = sg.cp * jm;
+    const size_t cs = (size_t)sg.cplane * km;
+    const size_t nb = 8 * cs * sizeof(float);
+    e = hipMalloc(&D->split, nb);
+    if (e == hipSuccess) e = hipMemset(D->split, 0, nb);
     if (e != hipSuccess) return bail(std::string("SOR buffer allocation: ") + hipGetErrorString(e));
     D->bytes += nb;
+    SplitBufs& sb = D->d.sb;
+    for (int q = 0; q < 4; q++) sb.p[q >> 1][q & 1] = D->split + q * cs;
+    for (int c = 0; c < 2; c++) {
+      sb.rhs[c] = D->split + (4 + c) * cs;
+      sb.cn1[c] = D->split + (6 + c) * cs;
+    }
   }
   // metric pack: each array gets n+4 slots (indices -1..n+2)
   int off = 0;
@@ -409,23 +431,14 @@ int domain_alloc(int im, int jm, int km, les_domain** out) {
   m.cnby = D->mbuf + D->moff[MCNBY];
   m.cncz = D->mbuf + D->moff[MCNCZ];
   m.inflow = nullptr;
-  // reduction scratch: reduction partials, then two SOR partial slots
-  const int gx = (im + 63) / 64, gy = (jm + 3) / 4;
-  const int red = reduction_parts_needed();
-  D->d.npartials = gx * gy > red ? gx * gy : red;
-  {
-    // k levels per SOR block: enough blocks to fill the device (~1024)
-    const int tiles = ((im + 63) / 64) * ((jm + 7) / 8);
-    int nkc = (1024 + tiles - 1) / tiles;
-    if (nkc > km) nkc = km;
-    if (nkc < 1) nkc = 1;
-    int kc = (km + nkc - 1) / nkc;
-    const char* env = getenv("LES_SOR_KCHUNK");
-    if (env && atoi(env) > 0) kc = atoi(env) < km ? atoi(env) : km;
-    D->d.sor_kchunk = kc;
-    D->d.sor_parts_cap = sor_parts_needed(D->d.g, kc);
-  }
-  e = hipMalloc(&D->d.partials, ((size_t)D->d.npartials + 2 * (size_t)D->d.sor_parts_cap) * sizeof(double));
+  // reduction scratch: reduction partials, then the four SOR partial slots
+  D->d.npartials = reduction_parts_needed(D->d.g);
+  D->d.sor_blocks = env_int("LES_SOR_BLOCKS", 2048);
+  D->d.xcd_remap = env_int("LES_XCD_REMAP", 1);
+  const int nsor = sor_parts_needed(D->d.g, D->d.sg, D->d.sor_blocks);
+  e = hipMalloc(&D->d.partials, ((size_t)D->d.npartials + 4 * (size_t)nsor) * sizeof(double));
+  if (e == hipSuccess)
+    for (int q = 0; q < 4; q++) D->d.sb.part[q >> 1][q & 1] = D->d.partials + D->d.npartials + (size_t)q * nsor;
   if (e == hipSuccess) e = hipMalloc(&D->d.ctrl, sizeof(Ctrl));
   if (e == hipSuccess) e = hipMemset(D->d.ctrl, 0, sizeof(Ctrl));
   if (e != hipSuccess) return bail(std::string("control allocation: ") + hipGetErrorString(e));
@@ -460,8 +473,7 @@ int domain_alloc(int im, int jm, int km, les_domain** out) {
   F.cmask = D->fields[FCMASK];
   F.dmask = D->fields[FDMASK];
   F.cn1 = D->fields[FCN1];
-  D->bytes += (long long)off * 4 + ((long long)D->d.npartials + 2LL * D->d.sor_parts_cap) * 8 +
-              (long long)sizeof(Ctrl);
+  D->bytes += (long long)off * 4 + ((long long)D->d.npartials + 4LL * nsor) * 8 + (long long)sizeof(Ctrl);
   *out = D;
   return 0;
 }
@@ -818,10 +830,53 @@ int les_domain_stage(les_domain* D, int stage, int n, int nmax) {
   return run_stage(D, stage, n, nmax);
 }
 
+// One step from a graph captured on the domain's stream.  Every launch
+// argument of a step n > 1 is fixed by the domain state (pointers, sizes,
+// physical constants, SOR controls), so one capture serves every such step;
+// a change of that state triggers a new capture.  Step 1 differs (initial
+// copy in bondv1) and runs eagerly.
+static int step_graph(les_domain* D, int n, int nmax) {
+  if (flush_metrics(D)) return 1;
+  const float sc[5] = {D->ro, D->vn, D->alpha, D->beta, D->dt};
+  if (D->gexec && (memcmp(&D->gkey, &D->d, sizeof(DevDomain)) != 0 || memcmp(D->gscal, sc, sizeof(sc)) != 0)) {
+    hipGraphExecDestroy(D->gexec);
+    D->gexec = nullptr;
+  }
+  if (!D->gexec) {
+    HIP_TRY(hipStreamBeginCapture(D->d.stream, hipStreamCaptureModeThreadLocal));
+    int rc = 0;
+    for (int s = 0; s < 7 && !rc; s++) rc = run_stage(D, s, n, nmax);
+    hipGraph_t gr = nullptr;
+    hipError_t e = hipStreamEndCapture(D->d.stream, &gr);
+    if (rc) {
+      if (gr) hipGraphDestroy(gr);
+      return 1;
+    }
+    if (e != hipSuccess) return fail(std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    e = hipGraphInstantiate(&D->gexec, gr, nullptr, nullptr, 0);
+    hipGraphDestroy(gr);
+    if (e != hipSuccess) {
+      D->gexec = nullptr;
+      return fail(std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    }
+    memcpy(&D->gkey, &D->d, sizeof(DevDomain));
+    memcpy(D->gscal, sc, sizeof(sc));
+  }
+  HIP_TRY(hipGraphLaunch(D->gexec, D->d.stream));
+  return 0;
+}
+
 int les_domain_step(les_domain* D, int n, int nmax) {
   if (!D) return fail("null domain");
+  if (D->use_graph && n != 1 && !D->d.timer) return step_graph(D, n, nmax);
   for (int s = 0; s < 7; s++)
     if (run_stage(D, s, n, nmax)) return 1;
+  return 0;
+}
+
+int les_domain_set_graph(les_domain* D, int on) {
+  if (!D) return fail("null domain");
+  D->use_graph = on ? 1 : 0;
   return 0;
 }
 

@@ -87,6 +87,7 @@ def lib():
     L.les_domain_bytes.argtypes = [vp]
     L.les_domain_bytes.restype = ctypes.c_longlong
     L.les_domain_set_timing.argtypes = [vp, ci]
+    L.les_domain_set_graph.argtypes = [vp, ci]
     L.les_domain_timing.argtypes = [vp, vp, vp]
     L.les_kernel_class_name.restype = ctypes.c_char_p
     L.les_kernel_class_name.argtypes = [ci]
@@ -224,6 +225,10 @@ class Domain:
 
     def press_stats(self):
         return _stats(lib().les_domain_press_stats, self._h)
+
+    def set_graph(self, on: bool):
+        """Replay steps n > 1 from one captured HIP graph."""
+        _check(lib().les_domain_set_graph(self._h, 1 if on else 0))
 
     def set_timing(self, on: bool, classes=None):
         """Bracket launches with HIP events; classes: names to record (all if None)."""

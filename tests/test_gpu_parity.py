@@ -93,13 +93,14 @@ def test_domain_api_matches_reference_signature_api():
     dom.close()
 
 
-@pytest.mark.parametrize("kchunk", [1, 2, 5])
-def test_sor_k_chunking(kchunk, monkeypatch):
-    """The fused red-black SOR splits k among blocks that recompute the
-    neighbouring red planes; any split gives the same iterate."""
-    monkeypatch.setenv("LES_SOR_KCHUNK", str(kchunk))
+@pytest.mark.parametrize("blocks", [1, 20, 1000])
+def test_sor_k_chunking(blocks, monkeypatch):
+    """Each SOR block walks a range of k levels, carrying neighbour values
+    from level to level; any split of k among blocks (one range, ranges of
+    3, single levels) gives the same iterate."""
+    monkeypatch.setenv("LES_SOR_BLOCKS", str(blocks))
     s = _mk(dict(im=70, jm=19, km=11, seed=5))
-    print(f"\n[sor chunking] kchunk={kchunk} {s.workload.name}", flush=True)
+    print(f"\n[sor chunking] blocks={blocks} {s.workload.name}", flush=True)
     gpu = GpuBackend(s)
     orc = oracle.OracleBackend(s, 32, redmode=1)
     for n in range(1, 3):
@@ -130,3 +131,21 @@ def test_sor_converges_early_like_reference():
         a, b = gpu.fetch(["p"]), orc.fetch(["p"])
         assert rel_l2(a["p"], b["p"]) <= 1e-6
     assert min(seen) < 50, seen
+
+
+def test_graph_replay_matches_eager():
+    """Steps replayed from the captured graph equal eager steps bit for bit,
+    on every replay (checked after each step, with host reads in between)."""
+    from lesamd import api
+    s = _mk(dict(im=40, jm=24, km=18, seed=7))
+    print(f"\n[graph replay] {s.workload.name}", flush=True)
+    eager, graph = api.Domain(s), api.Domain(s)
+    graph.set_graph(True)
+    for n in range(1, 7):
+        eager.step(n, 6)
+        graph.step(n, 6)
+        for name in ("u", "v", "w", "p", "sm", "fold"):
+            assert np.array_equal(eager.download(name), graph.download(name)), (n, name)
+        assert eager.press_stats()["sor"].tolist() == graph.press_stats()["sor"].tolist()
+    eager.close()
+    graph.close()
