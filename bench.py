@@ -29,6 +29,7 @@ import argparse
 import dataclasses
 import json
 import os
+import re
 import sys
 import time
 from pathlib import Path
@@ -94,6 +95,39 @@ def cpu_baseline(setup, state, n0, nsteps, budget_s):
     }
 
 
+def profiled_traffic(kernel_class: str, workload: str):
+    """HBM bytes per launch of the dominant kernel, as measured by the
+    rocprofv3 FETCH_SIZE and WRITE_SIZE counters (scaled by a copy of known
+    size) and recorded in profiles/<kernel>_<workload>.md; None when no such
+    measurement is recorded for this kernel and workload."""
+    name = {"sor": "sor_sweep"}.get(kernel_class, kernel_class)
+    f = ROOT / "profiles" / f"{name}_{workload}.md"
+    if not f.exists():
+        return None
+    m = re.search(r"^traffic_bytes_per_launch:\s*(\d+)", f.read_text(), flags=re.M)
+    return int(m.group(1)) if m else None
+
+
+def copy_calibration(mib: int) -> int:
+    """Ten device-to-device copies of `mib` MiB through the HIP runtime: a
+    kernel of known traffic (read mib, write mib) for scaling the HBM byte
+    counters."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    n = mib << 20
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(n)) == 0
+    assert hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(n)) == 0
+    hip.hipMemset(a, 1, ctypes.c_size_t(n))
+    for _ in range(10):
+        assert hip.hipMemcpy(b, a, ctypes.c_size_t(n), 3) == 0  # device to device
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(json.dumps({"copy_calibration_bytes": n, "copies": 10}))
+    return 0
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -103,7 +137,12 @@ def main(argv=None):
     ap.add_argument("--cpu-steps", type=int, default=8, help="max CPU-baseline steps")
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="CPU-baseline time budget [s]")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--eager", action="store_true", help="launch kernels one by one (no graph replay)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run device-to-device copies of this size (HBM counter calibration)")
     args = ap.parse_args(argv)
+    if args.copy_calibration:
+        return copy_calibration(args.copy_calibration)
     if args.gpus != 1:
         print("bench.py: this build runs on exactly one GPU (one domain per device); "
               f"--gpus {args.gpus} is not supported", file=sys.stderr)
@@ -125,7 +164,7 @@ def main(argv=None):
     snapshot = None if args.no_cpu_baseline else {n: dom.download(n) for n in STATE_FIELDS}
 
     # timed region: steps replayed from the captured graph of the step
-    dom.set_graph(True)
+    dom.set_graph(not args.eager)
     dom.step(W + 1, nmax)  # capture (not timed)
     dom.synchronize()
     t0 = time.perf_counter()
@@ -192,7 +231,7 @@ def main(argv=None):
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": profiled_traffic(dom_cls, w.name),
             "alg_bytes_per_launch": int(alg_bytes),
             "avg_launch_us": round(avg_s * 1e6, 3),
         },
