@@ -41,6 +41,36 @@ def _compile(src: str) -> Path:
     return out
 
 
+FLANG = os.environ.get("FLANG", "/opt/rocm/lib/llvm/bin/flang")
+# the reference's own Fortran flags that matter for the interface: free
+# form, preprocessed, big-endian unformatted records (SConstruct:77)
+FFLAGS = ["-cpp", "-O2", "-ffree-form", "-fPIC", "-fconvert=big-endian"]
+FORTRAN_MODULES = HERE / "fortran" / "les_amd_modules.F90"
+TIME_LOOP_SRC = ROOT / "tests" / "fortran" / "time_loop.F90"
+TIME_LOOP_LIB = ROOT / "tests" / "fortran" / "liblestimeloop.so"
+
+
+def build_fortran(verbose: bool = True) -> Path:
+    """Compiles the drop-in Fortran modules (fortran/les_amd_modules.F90)
+    and the test time loop against them into tests/fortran/liblestimeloop.so,
+    linked to lesamd/liblesamd.so."""
+    mod = OBJ / "fortran"
+    mod.mkdir(parents=True, exist_ok=True)
+    steps = [
+        [FLANG, *FFLAGS, "-module-dir", str(mod), "-c", str(FORTRAN_MODULES), "-o", str(mod / "les_amd_modules.o")],
+        [FLANG, *FFLAGS, "-module-dir", str(mod), "-c", str(TIME_LOOP_SRC), "-o", str(mod / "time_loop.o")],
+        [FLANG, "-shared", "-fPIC", "-o", str(TIME_LOOP_LIB), str(mod / "les_amd_modules.o"), str(mod / "time_loop.o"),
+         f"-L{LIB.parent}", "-llesamd", "-Wl,-rpath,$ORIGIN/../../mpi-les_amd/lesamd"],
+    ]
+    for cmd in steps:
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"flang failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    if verbose:
+        print(f"built {TIME_LOOP_LIB.relative_to(ROOT)}")
+    return TIME_LOOP_LIB
+
+
 def build(verbose: bool = True) -> Path:
     OBJ.mkdir(exist_ok=True)
     with cf.ThreadPoolExecutor(max_workers=len(SOURCES)) as ex:
@@ -58,4 +88,6 @@ def build(verbose: bool = True) -> Path:
 
 if __name__ == "__main__":
     build()
+    if Path(FLANG).exists():
+        build_fortran()
     sys.exit(0)

@@ -1,0 +1,421 @@
+! Drop-in Fortran modules for the MPI-LES time loop (main.f95:260-296).
+!
+! Each module keeps the reference's module name, subroutine name and
+! argument list, so main.f95 compiles unchanged against this file in place of
+! velnw.f95, bondv1.f95, velFG.f95 (+vel2.f95), feedbf.f95, les.f95
+! (+boundsm.f95), adam.f95 and press.f95 (+bondFG.f95, boundp.f95).  Every
+! subroutine forwards its arguments through iso_c_binding to the gfx950
+! library (include/les_amd.h), which keeps a device copy of each array it is
+! handed: the host arrays are read once, on the first call that sees them,
+! and are stale afterwards until les_sync(array) copies the device state back.
+!
+! Array dummies are declared with the reference's bounds, with the extents
+! im,jm,km in place of the compile-time ip,jp,kp (the reference requires
+! ip==im etc., ifdata.f95:238-241).  Scalars go by value.  A nonzero status
+! from the library ends the program with error stop and the library's
+! message, as the reference's own failures (stop, MPI_Abort) end it.
+!
+! File output kept on the Fortran side, as in the reference:
+!   adam   data21 dump of fold,gold,hold,fghold at mod(n,1000)==0 or n==nmax
+!          (adam.f95:20-29)
+!   press  centre-value print and data20 dump of u,v,w,p,usum,vsum,wsum at
+!          mod(n,1000)==0 or n==nmax (press.f95:197-217); the VERBOSE
+!          convergence/profile/CFL prints every 20 steps (press.f95:164-195)
+! The dumps are compiled out with -DNO_IO or -DMPI, as in the reference.
+! Unformatted records are big-endian, as the reference's build makes them
+! (-fconvert=big-endian, SConstruct:77), whatever the host program's own
+! flags: the OPEN statements say so (convert=, supported by flang and
+! gfortran), since a byte-order flag reaches only a Fortran main program.
+
+module les_amd_c
+    use iso_c_binding
+    implicit none
+
+    interface
+        function les_last_error() bind(C, name='les_last_error')
+            import :: c_ptr
+            type(c_ptr) :: les_last_error
+        end function
+
+        integer(c_int) function les_velnw(km,jm,im,p,ro,dxs,u,dt,f,dys,v,g,dzs,w,h) &
+                bind(C, name='les_velnw')
+            import :: c_int, c_float
+            integer(c_int), value :: km, jm, im
+            real(c_float), value :: ro, dt
+            real(c_float) :: p(*), dxs(*), u(*), f(*), dys(*), v(*), g(*), dzs(*), w(*), h(*)
+        end function
+
+        integer(c_int) function les_bondv1(jm,u,z2,dzn,v,w,km,n,im,dt,dxs) bind(C, name='les_bondv1')
+            import :: c_int, c_float
+            integer(c_int), value :: jm, km, n, im
+            real(c_float), value :: dt
+            real(c_float) :: u(*), z2(*), dzn(*), v(*), w(*), dxs(*)
+        end function
+
+        integer(c_int) function les_velfg(km,jm,im,dx1,cov1,cov2,cov3,dfu1,diu1,diu2,dy1,diu3,dzn, &
+                vn,f,cov4,cov5,cov6,dfv1,diu4,diu5,diu6,g,cov7,cov8,cov9,dfw1,diu7,diu8,diu9,dzs,h, &
+                nou1,u,nou5,v,nou9,w,nou2,nou3,nou4,nou6,nou7,nou8,uspd,vspd) bind(C, name='les_velfg')
+            import :: c_int, c_float
+            integer(c_int), value :: km, jm, im
+            real(c_float), value :: vn
+            real(c_float) :: dx1(*), cov1(*), cov2(*), cov3(*), dfu1(*), diu1(*), diu2(*), dy1(*), &
+                diu3(*), dzn(*), f(*), cov4(*), cov5(*), cov6(*), dfv1(*), diu4(*), diu5(*), diu6(*), &
+                g(*), cov7(*), cov8(*), cov9(*), dfw1(*), diu7(*), diu8(*), diu9(*), dzs(*), h(*), &
+                nou1(*), u(*), nou5(*), v(*), nou9(*), w(*), nou2(*), nou3(*), nou4(*), nou6(*), &
+                nou7(*), nou8(*), uspd(*), vspd(*)
+        end function
+
+        integer(c_int) function les_feedbf(km,jm,im,usum,u,bmask1,vsum,v,cmask1,wsum,w,dmask1, &
+                alpha,dt,beta,fx,fy,fz,f,g,h) bind(C, name='les_feedbf')
+            import :: c_int, c_float
+            integer(c_int), value :: km, jm, im
+            real(c_float), value :: alpha, dt, beta
+            real(c_float) :: usum(*), u(*), bmask1(*), vsum(*), v(*), cmask1(*), wsum(*), w(*), &
+                dmask1(*), fx(*), fy(*), fz(*), f(*), g(*), h(*)
+        end function
+
+        integer(c_int) function les_les(km,delx1,dx1,dy1,dzn,jm,im,diu1,diu2,diu3,diu4,diu5,diu6, &
+                diu7,diu8,diu9,sm,f,g,h,uspd,vspd,dxs,dys) bind(C, name='les_les')
+            import :: c_int, c_float
+            integer(c_int), value :: km, jm, im
+            real(c_float) :: delx1(*), dx1(*), dy1(*), dzn(*), diu1(*), diu2(*), diu3(*), diu4(*), &
+                diu5(*), diu6(*), diu7(*), diu8(*), diu9(*), sm(*), f(*), g(*), h(*), uspd(*), &
+                vspd(*), dxs(*), dys(*)
+        end function
+
+        integer(c_int) function les_adam(n,nmax,fold,im,jm,km,gold,hold,fghold,f,g,h) &
+                bind(C, name='les_adam')
+            import :: c_int, c_float
+            integer(c_int), value :: n, nmax, im, jm, km
+            real(c_float) :: fold(*), gold(*), hold(*), fghold(*), f(*), g(*), h(*)
+        end function
+
+        integer(c_int) function les_press(km,jm,im,rhs,u,dx1,v,dy1,w,dzn,f,g,h,dt,cn1,cn2l,p,cn2s, &
+                cn3l,cn3s,cn4l,cn4s,n,nmax,usum,vsum,wsum) bind(C, name='les_press')
+            import :: c_int, c_float
+            integer(c_int), value :: km, jm, im, n, nmax
+            real(c_float), value :: dt
+            real(c_float) :: rhs(*), u(*), dx1(*), v(*), dy1(*), w(*), dzn(*), f(*), g(*), h(*), &
+                cn1(*), cn2l(*), p(*), cn2s(*), cn3l(*), cn3s(*), cn4l(*), cn4s(*), usum(*), &
+                vsum(*), wsum(*)
+        end function
+
+        integer(c_int) function les_press_stats(im,jm,km,iters,sor_hist,hist_len,rhsav,pav,nonfinite) &
+                bind(C, name='les_press_stats')
+            import :: c_int, c_double
+            integer(c_int), value :: im, jm, km, hist_len
+            integer(c_int) :: iters, nonfinite
+            real(c_double) :: sor_hist(*), rhsav, pav
+        end function
+
+        integer(c_int) function les_sync_host_c(a) bind(C, name='les_sync_host')
+            import :: c_int, c_float
+            real(c_float) :: a(*)
+        end function
+
+        integer(c_int) function les_mark_host_dirty_c(a) bind(C, name='les_mark_host_dirty')
+            import :: c_int, c_float
+            real(c_float) :: a(*)
+        end function
+
+        integer(c_int) function les_synchronize() bind(C, name='les_synchronize')
+            import :: c_int
+        end function
+
+        integer(c_int) function les_release_all() bind(C, name='les_release_all')
+            import :: c_int
+        end function
+
+        integer(c_int) function les_bondv1_set_inflow(im,jm,km,prof) bind(C, name='les_bondv1_set_inflow')
+            import :: c_int, c_float
+            integer(c_int), value :: im, jm, km
+            real(c_float) :: prof(*)
+        end function
+
+        integer(c_int) function les_press_config(nmaxp,pjuge,omega) bind(C, name='les_press_config')
+            import :: c_int, c_float
+            integer(c_int), value :: nmaxp
+            real(c_float), value :: pjuge, omega
+        end function
+    end interface
+
+contains
+
+    ! Ends the program when a library call failed, with the library's message.
+    subroutine les_check(st, what)
+        integer(c_int), intent(in) :: st
+        character(*), intent(in) :: what
+        if (st /= 0) then
+            write(0,'(a,a,a,a)') 'les_amd: ', what, ': ', les_error_text()
+            error stop 1
+        end if
+    end subroutine
+
+    function les_error_text() result(s)
+        character(len=:), allocatable :: s
+        character(kind=c_char), pointer :: c(:)
+        type(c_ptr) :: p
+        integer :: n
+        p = les_last_error()
+        if (.not. c_associated(p)) then
+            s = ''
+            return
+        end if
+        call c_f_pointer(p, c, [1024])
+        n = 0
+        do while (n < 1024)
+            if (c(n + 1) == c_null_char) exit
+            n = n + 1
+        end do
+        allocate(character(len=n) :: s)
+        s = transfer(c(1:n), s)
+    end function
+
+    ! Copies the device state of an array handed to the library back into
+    ! it (the host copy is stale between calls); arrays the library has not
+    ! seen are left alone.
+    subroutine les_sync(a)
+        real(c_float) :: a(*)
+        integer(c_int) :: st
+        st = les_sync_host_c(a)
+        if (st /= 2) call les_check(st, 'les_sync')
+    end subroutine
+
+    ! The next library call that receives this array uploads it again
+    ! (after the host program has changed it).
+    subroutine les_mark_dirty(a)
+        real(c_float) :: a(*)
+        call les_check(les_mark_host_dirty_c(a), 'les_mark_dirty')
+    end subroutine
+end module les_amd_c
+
+
+module module_velnw
+contains
+    subroutine velnw(km,jm,im,p,ro,dxs,u,dt,f,dys,v,g,dzs,w,h)
+        use les_amd_c
+        integer, intent(In) :: km, jm, im
+        real(kind=4), intent(In) :: ro, dt
+        real(kind=4), dimension(0:im+2,0:jm+2,0:km+1), intent(In) :: p
+        real(kind=4), dimension(0:im), intent(In) :: dxs
+        real(kind=4), dimension(0:jm), intent(In) :: dys
+        real(kind=4), dimension(-1:km+2), intent(In) :: dzs
+        real(kind=4), dimension(0:im,0:jm,0:km), intent(In) :: f, g, h
+        real(kind=4), dimension(0:im+1,-1:jm+1,0:km+1), intent(InOut) :: u, v
+        real(kind=4), dimension(0:im+1,-1:jm+1,-1:km+1), intent(InOut) :: w
+        call les_check(les_velnw(km,jm,im,p,ro,dxs,u,dt,f,dys,v,g,dzs,w,h), 'velnw')
+    end subroutine velnw
+end module module_velnw
+
+
+module module_bondv1
+contains
+    subroutine bondv1(jm,u,z2,dzn,v,w,km,n,im,dt,dxs)
+        use les_amd_c
+        integer, intent(In) :: jm, km, n, im
+        real(kind=4), intent(In) :: dt
+        real(kind=4), dimension(0:im+1,-1:jm+1,0:km+1), intent(InOut) :: u, v
+        real(kind=4), dimension(0:im+1,-1:jm+1,-1:km+1), intent(InOut) :: w
+        real(kind=4), dimension(0:km+2), intent(In) :: z2
+        real(kind=4), dimension(-1:km+2), intent(In) :: dzn
+        real(kind=4), dimension(0:im), intent(In) :: dxs
+        call les_check(les_bondv1(jm,u,z2,dzn,v,w,km,n,im,dt,dxs), 'bondv1')
+    end subroutine bondv1
+end module module_bondv1
+
+
+module module_velFG
+contains
+    subroutine velfg(km,jm,im,dx1,cov1,cov2,cov3,dfu1,diu1,diu2,dy1,diu3,dzn,vn,f,cov4,cov5,cov6, &
+            dfv1,diu4,diu5,diu6,g,cov7,cov8,cov9,dfw1,diu7,diu8,diu9,dzs,h,nou1,u,nou5,v,nou9,w,nou2, &
+            nou3,nou4,nou6,nou7,nou8,uspd,vspd)
+        use les_amd_c
+        integer, intent(In) :: km, jm, im
+        real(kind=4), intent(In) :: vn
+        real(kind=4), dimension(-1:im+2,0:jm+2,0:km+2), intent(Out) :: cov1, cov5, diu1, diu5, nou1, nou5
+        real(kind=4), dimension(0:im+2,0:jm+2,0:km+2), intent(Out) :: cov2, cov3, cov4, cov6, cov7, &
+            cov8, cov9, diu2, diu3, diu4, diu6, diu7, diu8, diu9, nou2, nou3, nou4, nou6, nou7, nou8, nou9
+        real(kind=4), dimension(0:im,jm,km), intent(Out) :: dfu1
+        real(kind=4), dimension(im,0:jm,km), intent(Out) :: dfv1
+        real(kind=4), dimension(im,jm,km), intent(Out) :: dfw1
+        real(kind=4), dimension(-1:im+1), intent(In) :: dx1
+        real(kind=4), dimension(0:jm+1), intent(In) :: dy1
+        real(kind=4), dimension(-1:km+2), intent(In) :: dzn, dzs
+        real(kind=4), dimension(0:im,0:jm,0:km), intent(Out) :: f, g, h
+        real(kind=4), dimension(0:im+1,-1:jm+1,0:km+1), intent(In) :: u, v
+        real(kind=4), dimension(0:im+1,-1:jm+1,-1:km+1), intent(In) :: w
+        real(kind=4), dimension(0:im+1,0:jm+1), intent(Out) :: uspd, vspd
+        call les_check(les_velfg(km,jm,im,dx1,cov1,cov2,cov3,dfu1,diu1,diu2,dy1,diu3,dzn,vn,f,cov4, &
+            cov5,cov6,dfv1,diu4,diu5,diu6,g,cov7,cov8,cov9,dfw1,diu7,diu8,diu9,dzs,h,nou1,u,nou5,v, &
+            nou9,w,nou2,nou3,nou4,nou6,nou7,nou8,uspd,vspd), 'velfg')
+    end subroutine velfg
+end module module_velFG
+
+
+module module_feedbf
+contains
+    subroutine feedbf(km,jm,im,usum,u,bmask1,vsum,v,cmask1,wsum,w,dmask1,alpha,dt,beta,fx,fy,fz,f,g,h)
+        use les_amd_c
+        integer, intent(In) :: km, jm, im
+        real(kind=4), intent(In) :: alpha, dt, beta
+        real(kind=4), dimension(-1:im+1,0:jm+1,0:km+1), intent(In) :: bmask1
+        real(kind=4), dimension(0:im+1,-1:jm+1,0:km+1), intent(In) :: cmask1
+        real(kind=4), dimension(0:im+1,0:jm+1,0:km+1), intent(In) :: dmask1
+        real(kind=4), dimension(0:im,0:jm,0:km), intent(InOut) :: f, g, h, usum, vsum, wsum
+        real(kind=4), dimension(0:im,0:jm,0:km), intent(Out) :: fx, fy, fz
+        real(kind=4), dimension(0:im+1,-1:jm+1,0:km+1), intent(In) :: u, v
+        real(kind=4), dimension(0:im+1,-1:jm+1,-1:km+1), intent(In) :: w
+        call les_check(les_feedbf(km,jm,im,usum,u,bmask1,vsum,v,cmask1,wsum,w,dmask1,alpha,dt,beta, &
+            fx,fy,fz,f,g,h), 'feedbf')
+    end subroutine feedbf
+end module module_feedbf
+
+
+module module_les
+contains
+    subroutine les(km,delx1,dx1,dy1,dzn,jm,im,diu1,diu2,diu3,diu4,diu5,diu6,diu7,diu8,diu9, &
+            sm,f,g,h,uspd,vspd,dxs,dys)
+        use les_amd_c
+        integer, intent(In) :: km, jm, im
+        real(kind=4), dimension(km), intent(Out) :: delx1
+        real(kind=4), dimension(-1:im+2,0:jm+2,0:km+2), intent(In) :: diu1, diu5
+        real(kind=4), dimension(0:im+2,0:jm+2,0:km+2), intent(In) :: diu2, diu3, diu4, diu6, diu7, &
+            diu8, diu9
+        real(kind=4), dimension(-1:im+1), intent(In) :: dx1
+        real(kind=4), dimension(0:jm+1), intent(In) :: dy1
+        real(kind=4), dimension(-1:km+2), intent(In) :: dzn
+        real(kind=4), dimension(0:im,0:jm,0:km), intent(InOut) :: f, g, h
+        real(kind=4), dimension(-1:im+1,-1:jm+1,0:km+1), intent(Out) :: sm
+        real(kind=4), dimension(0:im+1,0:jm+1), intent(In) :: uspd, vspd
+        real(kind=4), dimension(0:im), intent(In) :: dxs
+        real(kind=4), dimension(0:jm), intent(In) :: dys
+        call les_check(les_les(km,delx1,dx1,dy1,dzn,jm,im,diu1,diu2,diu3,diu4,diu5,diu6,diu7,diu8, &
+            diu9,sm,f,g,h,uspd,vspd,dxs,dys), 'les')
+    end subroutine les
+end module module_les
+
+
+module module_adam
+contains
+    subroutine adam(n,nmax,data21,fold,im,jm,km,gold,hold,fghold,f,g,h)
+        use les_amd_c
+        integer, intent(In) :: n, nmax, im, jm, km
+        character(len=70), intent(In) :: data21
+        real(kind=4), dimension(0:im,0:jm,0:km), intent(InOut) :: f, g, h
+        ! fghold is only read; InOut because the dump refreshes it from the device
+        real(kind=4), dimension(im,jm,km), intent(InOut) :: fghold
+        real(kind=4), dimension(im,jm,km), intent(InOut) :: fold, gold, hold
+        integer :: i, j, k
+#if !defined(NO_IO) && !defined(MPI)
+        ! adam.f95:20-29: the dump precedes the update
+        if (mod(n,1000) == 0 .or. n == nmax) then
+            call les_sync(fold)
+            call les_sync(gold)
+            call les_sync(hold)
+            call les_sync(fghold)
+            open(unit=21, file=data21, form='unformatted', status='unknown', convert='BIG_ENDIAN')
+            write(21) (((fold(i,j,k),i=1,im),j=1,jm),k=1,km)
+            write(21) (((gold(i,j,k),i=1,im),j=1,jm),k=1,km)
+            write(21) (((hold(i,j,k),i=1,im),j=1,jm),k=1,km)
+            write(21) (((fghold(i,j,k),i=1,im),j=1,jm),k=1,km)
+            close(unit=21)
+        end if
+#endif
+        call les_check(les_adam(n,nmax,fold,im,jm,km,gold,hold,fghold,f,g,h), 'adam')
+    end subroutine adam
+end module module_adam
+
+
+module module_press
+contains
+    subroutine press(km,jm,im,rhs,u,dx1,v,dy1,w,dzn,f,g,h,dt,cn1,cn2l,p,cn2s,cn3l,cn3s,cn4l,cn4s, &
+            n,nmax,data20,usum,vsum,wsum)
+        use les_amd_c
+        integer, intent(In) :: km, jm, im, n, nmax
+        character(len=70), intent(In) :: data20
+        real(kind=4), intent(In) :: dt
+        real(kind=4), dimension(im,jm,km), intent(In) :: cn1
+        real(kind=4), dimension(im), intent(In) :: cn2l, cn2s
+        real(kind=4), dimension(jm), intent(In) :: cn3l, cn3s
+        real(kind=4), dimension(km), intent(In) :: cn4l, cn4s
+        real(kind=4), dimension(-1:im+1), intent(In) :: dx1
+        real(kind=4), dimension(0:jm+1), intent(In) :: dy1
+        real(kind=4), dimension(-1:km+2), intent(In) :: dzn
+        real(kind=4), dimension(0:im,0:jm,0:km), intent(InOut) :: f, g, h
+        real(kind=4), dimension(0:im+2,0:jm+2,0:km+1), intent(InOut) :: p
+        real(kind=4), dimension(0:im+1,0:jm+1,0:km+1), intent(Out) :: rhs
+        ! u..wsum are only read; InOut because the dumps refresh them from the device
+        real(kind=4), dimension(0:im+1,-1:jm+1,0:km+1), intent(InOut) :: u, v
+        real(kind=4), dimension(0:im+1,-1:jm+1,-1:km+1), intent(InOut) :: w
+        real(kind=4), dimension(0:im,0:jm,0:km), intent(InOut) :: usum, vsum, wsum
+        integer :: i, j, k, nn
+#ifdef VERBOSE
+        integer(c_int) :: iters, nonfinite, l
+        real(c_double) :: hist(1024), rhsav, pav
+        real(kind=4) :: cflu, cflv, cflw
+#endif
+        call les_check(les_press(km,jm,im,rhs,u,dx1,v,dy1,w,dzn,f,g,h,dt,cn1,cn2l,p,cn2s,cn3l,cn3s, &
+            cn4l,cn4s,n,nmax,usum,vsum,wsum), 'press')
+#if !defined(NO_IO) && defined(VERBOSE)
+        ! press.f95:164-195, every 20 steps
+        ! (the SOR-loop print of press.f95:107-112 is rebuilt from the residual history)
+        if (mod(n-1,10) == 0 .or. mod(n-1,20) == 0) then
+            call les_check(les_press_stats(im,jm,km,iters,hist,1024,rhsav,pav,nonfinite), 'press_stats')
+        end if
+        if (mod(n-1,10) == 0) then
+            do l = 20, min(iters, 1024), 20
+                print *, 'time step, iteration step, conv =', n, l, real(hist(l), 4)
+            end do
+        end if
+        if (mod(n-1,20) == 0) then
+            call les_sync(u)
+            call les_sync(v)
+            call les_sync(w)
+            write(6,*) '=mac= time step, iteration step, conv =', n, iters, real(hist(min(iters, 1024)), 4)
+            do k = 1,km,10
+                write(6,*) 'Inflow=',k,'u=',u(0,jm/2,k),v(0,jm/2,k),w(0,jm/2,k)
+            end do
+            do k = 1,km,10
+                write(6,*) 'Urbanflow=',k,'u=',u(im/2,jm/2,k),v(im/2,jm/2,k),w(im/2,jm/2,k)
+            end do
+            cflu = 0.
+            cflv = 0.
+            cflw = 0.
+            do k = 1,km
+                do j = 1,jm
+                    do i = 1,im
+                        cflu = amax1(cflu,abs(u(i,j,k)*dt/dx1(i)))
+                        cflv = amax1(cflv,abs(v(i,j,k)*dt/dy1(j)))
+                        cflw = amax1(cflw,abs(w(i,j,k)*dt/dzn(k)))
+                    end do
+                end do
+            end do
+            write(6,*) 'Check_CFL,u*dt/dx,v*dt/dy,w*dt/dz=',cflu,cflv,cflw
+        end if
+#endif
+        ! press.f95:197-217
+        if (mod(n,1000) == 0 .or. n == nmax) then
+            nn = n/1000
+            call les_sync(p)
+            call les_sync(u)
+            call les_sync(v)
+            call les_sync(w)
+            print *, 'timestep: ',nn,' pressure at centre: ',p(im/2,jm/2,km/2), &
+                'vel at centre: ', u(im/2,jm/2,km/2),v(im/2,jm/2,km/2),w(im/2,jm/2,km/2)
+#if !defined(NO_IO) && !defined(MPI)
+            call les_sync(usum)
+            call les_sync(vsum)
+            call les_sync(wsum)
+            open(unit=20, file=data20, form='unformatted', status='unknown', convert='BIG_ENDIAN')
+            write(20) (((u(i,j,k),i=1,im),j=1,jm),k=1,km)
+            write(20) (((v(i,j,k),i=1,im),j=1,jm),k=1,km)
+            write(20) (((w(i,j,k),i=1,im),j=1,jm),k=1,km)
+            write(20) (((p(i,j,k),i=1,im),j=1,jm),k=1,km)
+            write(20) (((usum(i,j,k),i=1,im),j=1,jm),k=1,km)
+            write(20) (((vsum(i,j,k),i=1,im),j=1,jm),k=1,km)
+            write(20) (((wsum(i,j,k),i=1,im),j=1,jm),k=1,km)
+            close(unit=20)
+#endif
+        end if
+    end subroutine press
+end module module_press
