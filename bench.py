@@ -50,10 +50,10 @@ METRIC = "steps/s + Mcell-updates/s at 300x300x90 FP32, %HBM peak; served p50/p9
 # iteration, pressure-mean removal 12).
 STEP_BYTES_FIXED = 40 + 60 + 64 + 64 + 48 + 28 + 12
 SOR_BYTES_PER_ITER = 12
-# per launch of each kernel class, bytes per interior cell
+# per launch of each kernel class, bytes per interior cell (sor: per iteration)
 CLASS_BYTES = {
     "velnw": 40, "vel2": 48, "velfg": 12, "feedbf": 64, "les_sm": 40, "les_visc": 24, "adam": 48,
-    "rhs": 28, "sor": SOR_BYTES_PER_ITER / 2, "pav": 4, "padj": 8,
+    "rhs": 28, "sor": SOR_BYTES_PER_ITER, "pav": 4, "padj": 8,
 }
 STATE_FIELDS = ("u", "v", "w", "p", "f", "g", "h", "usum", "vsum", "wsum", "fold", "gold", "hold", "rhs",
                 "sm", "uspd", "vspd") + tuple(f"diu{q}" for q in range(1, 10))
@@ -100,7 +100,10 @@ def profiled_traffic(kernel_class: str, workload: str):
     rocprofv3 FETCH_SIZE and WRITE_SIZE counters (scaled by a copy of known
     size) and recorded in profiles/<kernel>_<workload>.md; None when no such
     measurement is recorded for this kernel and workload."""
-    name = {"sor": "sor_sweep"}.get(kernel_class, kernel_class)
+    if kernel_class == "sor":
+        name = "sor_sweep" if os.environ.get("LES_SOR_FUSED", "1") == "0" else "sor_iter"
+    else:
+        name = kernel_class
     f = ROOT / "profiles" / f"{name}_{workload}.md"
     if not f.exists():
         return None
@@ -201,6 +204,11 @@ def main(argv=None):
     dom_ms, dom_n = kt[dom_cls]
     avg_s = dom_ms / 1e3 / dom_n
     alg_bytes = CLASS_BYTES.get(dom_cls, 0) * w.cells
+    if dom_cls == "sor":
+        # a launch is one colour sweep (half an iteration) or one fused
+        # iteration: price it by the SOR iterations it carries (every timed
+        # step ran L iterations)
+        alg_bytes = SOR_BYTES_PER_ITER * w.cells * (L * K) / dom_n
     achieved = alg_bytes / avg_s / 1e9
     step_bytes = w.cells * (STEP_BYTES_FIXED + SOR_BYTES_PER_ITER * L)
     per_kernel = {c: {"ms_per_step": round(v[0] / nprof, 4), "launches_per_step": v[1] / nprof,

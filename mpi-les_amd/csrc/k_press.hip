@@ -254,6 +254,225 @@ __global__ __launch_bounds__(NT) void k_sweep(Grid g, SplitGrid s, Metrics m, Sp
   if (tid == 0) sb.part[cur][COL][bid] = sum;
 }
 
+// ------------------------------------------------- fused SOR iteration
+// Both colour sweeps of one SOR iteration in one launch, with the same
+// per-cell arithmetic (and so bit-identical iterates) as k_sweep<1> then
+// k_sweep<0>.  A block owns a tile of FTX pairs x FTY rows and a chunk of
+// k levels, and marches k upward with one thread per (pair, row) of the
+// tile plus a two-cell ring.  At each step kr it
+//   1. puts the previous black iterate of level kr in LDS (ring included),
+//   2. computes the new red values of level kr on the tile and a one-cell
+//      ring (the ring values are recomputed by the neighbouring block too,
+//      with identical results) and puts them in LDS,
+//   3. computes the new black values of level kr-1 from the red values of
+//      levels kr-2..kr (in-plane neighbours from LDS, the vertical ones
+//      from registers).
+// LDS planes alternate between two buffers, so one barrier per step
+// suffices.  HBM traffic per iteration is one read of each colour's
+// iterate, one read of rhs and one write of each colour (12 B per cell);
+// the ring re-reads mostly hit L2.  The periodic j ring is indexed modulo
+// jm; when jm is odd the j-neighbours across the periodic boundary have
+// the cell's own colour and keep their previous iterate (as in k_sweep).
+constexpr int FX = 64, FY = 16, FTX = FX - 4, FTY = FY - 4, FNT = FX * FY;
+
+__device__ __forceinline__ int jwrap(int j, int jm) {
+  int r = (j - 1) % jm;
+  if (r < 0) r += jm;
+  return r + 1;
+}
+
+// Operands of one step, streamed three steps ahead of their use.
+struct SorSlot {
+  float ro, rr, cr;  // red: previous iterate, rhs, cn1 (level kr)
+  float wr;          // red: periodic same-colour neighbour (jm odd, rows 1 and jm; level kr)
+  float rb, cb;      // black: rhs, cn1 (level kr-1)
+  float wb;          // black: periodic same-colour neighbour (level kr-1)
+  float bq;          // black previous iterate (level kr+1), for the LDS ring
+};
+
+template <bool SEP>
+__global__ __launch_bounds__(FNT) void k_sor_iter(Grid g, SplitGrid s, Metrics m, SplitBufs sb, Ctrl* ctrl,
+                                                  int iter, int nparts, float omega, float pjuge, int ntt,
+                                                  int ntj, int kchunk, int remap) {
+  __shared__ float ldsB[4][FY][FX];  // previous black iterate, levels mod 4
+  __shared__ float ldsR[2][FY][FX];  // new red values, levels mod 2
+  __shared__ double lds[FNT / kWave];
+  __shared__ int skip;
+  extern __shared__ float lvm[];  // [3][kchunk + 2]: cn4l, cn4s, cncz of levels krs..kre
+  const int lane = threadIdx.x, row = threadIdx.y, tid = row * FX + lane;
+  if (tid == 0) skip = (ctrl->sor_done != 0);
+  __syncthreads();
+  if (skip) return;
+  const int bid = blockIdx.x;
+  if (iter >= 2 && bid == 0) {  // stop test of the previous iteration (press.f95:119-121)
+    const int pp = (iter - 1) & 1;
+    double sum = 0.0;
+    for (int b = tid; b < nparts; b += FNT) sum += sb.part[pp][1][b] + sb.part[pp][0][b];
+    sum = block_sum<FNT>(sum, tid, lds);
+    if (tid == 0) {
+      if (iter - 1 <= kSorHist) ctrl->sor_hist[iter - 2] = sum;
+      ctrl->sor_iters = iter - 1;
+      if (!(sum == sum) || sum > 1.0e300) ctrl->nonfinite = 1;
+      if (sum < (double)pjuge || ctrl->nonfinite) ctrl->sor_done = iter - 1;
+    }
+    __syncthreads();  // lds is reused below
+  }
+  const int prv = (iter - 1) & 1, cur = iter & 1;
+  const float* __restrict__ Ro = sb.p[prv][1];
+  const float* __restrict__ Bo = sb.p[prv][0];
+  float* __restrict__ Rn = sb.p[cur][1];
+  float* __restrict__ Bn = sb.p[cur][0];
+  const float* __restrict__ rR = sb.rhs[1];
+  const float* __restrict__ rB = sb.rhs[0];
+  const float* __restrict__ cR = sb.cn1[1];
+  const float* __restrict__ cB = sb.cn1[0];
+  const int im = g.im, jm = g.jm, km = g.km, nt = s.nt, cp = s.cp, cpl = s.cplane;
+  const int lb = remap ? xcd_contiguous(bid, nparts) : bid;
+  const int bt = lb % ntt, bj = (lb / ntt) % ntj, bk = lb / (ntt * ntj);
+  const int t = bt * FTX - 2 + lane;
+  const int jl = 1 + bj * FTY - 2 + row;  // row of the tile (may lie outside 1..jm)
+  const int j = jwrap(jl, jm);            // the row it stands for
+  const int k0 = 1 + bk * kchunk, k1 = min(km, k0 + kchunk - 1);
+  const bool tin = (t >= 0) && (t < nt);
+  const bool in_red = tin && lane >= 1 && lane <= FX - 2 && row >= 1 && row <= FY - 2;
+  const bool in_tile = tin && lane >= 2 && lane <= FX - 3 && row >= 2 && row <= FY - 3 && jl <= jm;
+  const bool jodd = (jm & 1) != 0;
+  const bool wlo = jodd && j == 1, whi = jodd && j == jm;  // same-colour periodic neighbours
+  const int wofs = (jm - 1) * cp;
+  // lanes outside the pairs of the row read pair 0 of their own row, so the
+  // periodic row shift below stays inside the array for every lane
+  const int q0 = (tin ? t : 0) + (j - 1) * cp;
+  auto lvl = [&](int k) { return q0 + (k - 1) * cpl; };
+  // metrics of the two cells of the pair (i = 2t+1, 2t+2) and of row j
+  float c2l[2] = {0.f, 0.f}, c2s[2] = {0.f, 0.f}, ax[2] = {0.f, 0.f};
+  if (tin) {
+    for (int d = 0; d < 2; d++) {
+      const int i = 2 * t + 1 + d;
+      c2l[d] = M1(m.cn2l, i);
+      c2s[d] = M1(m.cn2s, i);
+      ax[d] = SEP ? M1(m.cnax, i) : 0.0f;
+    }
+  }
+  const float c3l = M1(m.cn3l, j), c3s = M1(m.cn3s, j);
+  const float byj = SEP ? M1(m.cnby, j) : 0.0f;
+  const int krs = max(1, k0 - 1), kre = k1 + 1;
+  const int nl = kchunk + 2;
+  for (int x = tid; x < nl; x += FNT) {
+    const int k = krs + x;
+    if (k <= km) {
+      lvm[x] = M1(m.cn4l, k);
+      lvm[nl + x] = M1(m.cn4s, k);
+      lvm[2 * nl + x] = SEP ? M1(m.cncz, k) : 0.0f;
+    }
+  }
+  // The streams: red operands of level L, black operands of level L-1 and
+  // the black iterate of level L+1, for the step at level L.  Every lane
+  // issues the same loads in every step (levels clamped into 1..km, lanes
+  // outside the domain reading pair 0), so the number of loads in flight is
+  // the same on every path and a use waits only for its own load; values
+  // loaded for clamped levels or idle lanes are never used.
+  const int wsh = wlo ? wofs : (whi ? -wofs : 0);
+  auto fill = [&](SorSlot& sl, int L) {
+    const int qr = lvl(min(L, km));
+    const int qb = lvl(max(1, min(L - 1, km)));
+    sl.ro = Ro[qr];
+    sl.rr = rR[qr];
+    if (!SEP) sl.cr = cR[qr];
+    sl.wr = Ro[qr + wsh];
+    sl.rb = rB[qb];
+    if (!SEP) sl.cb = cB[qb];
+    sl.wb = Bo[qb + wsh];
+    sl.bq = Bo[lvl(min(L + 1, km))];
+  };
+  SorSlot s0 = {}, s1 = {}, s2 = {};
+  fill(s0, krs);
+  fill(s1, krs + 1);
+  fill(s2, krs + 2);
+  // the black iterate of levels krs-1 and krs enters the ring directly
+  if (tin) {
+    if (krs >= 2) ldsB[(krs - 1) & 3][row][lane] = Bo[lvl(krs - 1)];
+    ldsB[krs & 3][row][lane] = Bo[lvl(krs)];
+  }
+  float rm2 = 0.f, rm1 = 0.f;  // new red values of levels kr-2, kr-1 at this (pair, row)
+  double acc = 0.0;
+
+  auto step = [&](int kr, SorSlot& sl) {
+    const bool active = kr <= kre;  // steps past kre only pad the last group of three
+    const bool has_red = active && kr <= km;
+    const int kb = kr - 1;
+    const bool has_blk = active && kb >= k0;
+    if (active && kr + 1 <= km) ldsB[(kr + 1) & 3][row][lane] = sl.bq;
+    __syncthreads();
+    float r0 = 0.f;
+    if (has_red && in_red) {
+      const int d = (j + kr) & 1;  // red: i+j+k odd
+      const int i = 2 * t + 1 + d;
+      if (i <= im) {
+        const float own = sl.ro;
+        const float a = ldsB[kr & 3][row][lane];
+        const float bb = d ? ldsB[kr & 3][row][lane + 1] : ldsB[kr & 3][row][lane - 1];
+        const float lo = d ? a : bb, hi = d ? bb : a;  // i-1, i+1
+        const float pim = (i == 1) ? own : lo;
+        const float pip = (i == im) ? own : hi;
+        const float pjm = wlo ? sl.wr : ldsB[kr & 3][row - 1][lane];
+        const float pjp = whi ? sl.wr : ldsB[kr & 3][row + 1][lane];
+        const float pkm = (kr == 1) ? own : ldsB[(kr - 1) & 3][row][lane];
+        const float pkp = (kr == km) ? own : ldsB[(kr + 1) & 3][row][lane];
+        const int x = kr - krs;
+        const float cn = SEP ? 1.0f / ((ax[d] + byj) + lvm[2 * nl + x]) : sl.cr;  // init.f95:100-107
+        const float reltmp = omega * (cn * (c2l[d] * pip + c2s[d] * pim + c3l * pjp + c3s * pjm +
+                                            lvm[x] * pkp + lvm[nl + x] * pkm - sl.rr) -
+                                      own);
+        r0 = own + reltmp;
+        if (in_tile && kr >= k0 && kr <= k1) {
+          Rn[lvl(kr)] = r0;
+          acc += (double)reltmp * (double)reltmp;
+        }
+      }
+      ldsR[kr & 1][row][lane] = r0;
+    }
+    if (has_blk && in_tile) {
+      const int d = (1 + j + kb) & 1;  // black: i+j+k even
+      const int i = 2 * t + 1 + d;
+      if (i <= im) {
+        const float own = ldsB[kb & 3][row][lane];
+        const float bb = d ? ldsR[kb & 1][row][lane + 1] : ldsR[kb & 1][row][lane - 1];
+        const float lo = d ? rm1 : bb, hi = d ? bb : rm1;
+        const float pim = (i == 1) ? own : lo;
+        const float pip = (i == im) ? own : hi;
+        const float pjm = wlo ? sl.wb : ldsR[kb & 1][row - 1][lane];
+        const float pjp = whi ? sl.wb : ldsR[kb & 1][row + 1][lane];
+        const float pkm = (kb == 1) ? own : rm2;
+        const float pkp = (kb == km) ? own : r0;
+        const int x = kb - krs;
+        const float cn = SEP ? 1.0f / ((ax[d] + byj) + lvm[2 * nl + x]) : sl.cb;
+        const float reltmp = omega * (cn * (c2l[d] * pip + c2s[d] * pim + c3l * pjp + c3s * pjm +
+                                            lvm[x] * pkp + lvm[nl + x] * pkm - sl.rb) -
+                                      own);
+        Bn[lvl(kb)] = own + reltmp;
+        acc += (double)reltmp * (double)reltmp;
+      }
+    }
+    rm2 = rm1;
+    rm1 = r0;
+    fill(sl, kr + 3);
+  };
+  // unrolled by the stream depth, with no conditional step, so that each
+  // slot stays in its own registers from load to use
+  const int ngroups = (kre - krs + 3) / 3;
+  for (int grp = 0; grp < ngroups; grp++) {
+    const int kr = krs + 3 * grp;
+    step(kr, s0);
+    step(kr + 1, s1);
+    step(kr + 2, s2);
+  }
+  const double sum = block_sum<FNT>(acc, tid, lds);
+  if (tid == 0) {
+    sb.part[cur][0][bid] = sum;
+    sb.part[cur][1][bid] = 0.0;
+  }
+}
+
 // After the iteration loop: when no iteration met the stop test, the last
 // one (nmaxp) is summed here.  Records the iteration count and sum.
 __global__ __launch_bounds__(256) void k_sor_final(SplitBufs sb, Ctrl* ctrl, int nmaxp, int nparts) {
@@ -316,6 +535,21 @@ __global__ __launch_bounds__(NT) void k_padj(Grid g, SplitGrid s, SplitBufs sb, 
   }
 }
 
+// After the SOR loop: the closing stop-test sum, the pressure mean and the
+// adjusted p with its ghosts.
+static void launch_press_tail(const DevDomain& d, const SplitBufs& sb, int nparts, int gx, int gy, int gz,
+                              int nred) {
+  const Grid& g = d.g;
+  const SplitGrid& s = d.sg;
+  TIMED(d, KC_PAV, hipLaunchKernelGGL(k_sor_final, dim3(1), dim3(256), 0, d.stream, sb, d.ctrl, d.nmaxp, nparts));
+  TIMED(d, KC_PAV, hipLaunchKernelGGL(k_pav, dim3(gx, gy, gz), blk(), 0, d.stream, g, s, d.m, sb, d.partials,
+                                      (const Ctrl*)d.ctrl));
+  TIMED(d, KC_PAV, hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, d.stream, d.partials, nred, d.ctrl, 1,
+                                      d.area));
+  TIMED(d, KC_PADJ, hipLaunchKernelGGL(k_padj, dim3((g.im + 2 + BX - 1) / BX, (g.jm + 2 + BY - 1) / BY, g.km + 2),
+                                       blk(), 0, d.stream, g, s, sb, d.F.p, (const Ctrl*)d.ctrl));
+}
+
 void launch_press(const DevDomain& d, float dt) {
   const Grid& g = d.g;
   const SplitGrid& s = d.sg;
@@ -330,6 +564,22 @@ void launch_press(const DevDomain& d, float dt) {
   if (d.cn1_separable) sb.cn1[0] = sb.cn1[1] = nullptr;
   TIMED(d, KC_RHS, hipLaunchKernelGGL(k_rhs_split, dim3(gx, gy, g.km), blk(), 0, d.stream, g, s, d.F.rhs,
                                       (const float*)d.F.p, (const float*)d.F.cn1, sb, (const Ctrl*)d.ctrl));
+  if (d.sor_fused) {
+    const int ntt = (s.nt + FTX - 1) / FTX, ntj = (g.jm + FTY - 1) / FTY;
+    const int kc = sor_fused_kchunk(g, s, d.sor_fused_blocks, d.ncu);
+    const int nparts = ntt * ntj * ((g.km + kc - 1) / kc);
+    const size_t lvm_bytes = 3 * (size_t)(kc + 2) * sizeof(float);
+    for (int l = 1; l <= d.nmaxp; l++) {
+      if (d.cn1_separable)
+        TIMED(d, KC_SOR, hipLaunchKernelGGL((k_sor_iter<true>), dim3(nparts), dim3(FX, FY), lvm_bytes, d.stream, g, s, d.m,
+                                            sb, d.ctrl, l, nparts, d.omega, d.pjuge, ntt, ntj, kc, d.xcd_remap));
+      else
+        TIMED(d, KC_SOR, hipLaunchKernelGGL((k_sor_iter<false>), dim3(nparts), dim3(FX, FY), lvm_bytes, d.stream, g, s, d.m,
+                                            sb, d.ctrl, l, nparts, d.omega, d.pjuge, ntt, ntj, kc, d.xcd_remap));
+    }
+    launch_press_tail(d, sb, nparts, gx, gy, gz, nred);
+    return;
+  }
   const int tx = (s.nt + BX - 1) / BX;
   const int kc = sor_kchunk(g, s, d.sor_blocks);
   const int tz = (g.km + kc - 1) / kc;
@@ -348,13 +598,7 @@ void launch_press(const DevDomain& d, float dt) {
                                           l, nparts, d.omega, d.pjuge, tx, gy, kc, d.xcd_remap));
     }
   }
-  TIMED(d, KC_PAV, hipLaunchKernelGGL(k_sor_final, dim3(1), dim3(256), 0, d.stream, sb, d.ctrl, d.nmaxp, nparts));
-  TIMED(d, KC_PAV, hipLaunchKernelGGL(k_pav, dim3(gx, gy, gz), blk(), 0, d.stream, g, s, d.m, sb, d.partials,
-                                      (const Ctrl*)d.ctrl));
-  TIMED(d, KC_PAV, hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, d.stream, d.partials, nred, d.ctrl, 1,
-                                      d.area));
-  TIMED(d, KC_PADJ, hipLaunchKernelGGL(k_padj, dim3((g.im + 2 + BX - 1) / BX, (g.jm + 2 + BY - 1) / BY, g.km + 2),
-                                       blk(), 0, d.stream, g, s, sb, d.F.p, (const Ctrl*)d.ctrl));
+  launch_press_tail(d, sb, nparts, gx, gy, gz, nred);
 }
 
 // k levels per SOR block so that the sweep has about `target` blocks
@@ -370,6 +614,40 @@ int sor_parts_needed(const Grid& g, const SplitGrid& s, int target) {
   const int tx = (s.nt + BX - 1) / BX, gy = (g.jm + BY - 1) / BY;
   const int kc = sor_kchunk(g, s, target);
   return tx * gy * ((g.km + kc - 1) / kc);
+}
+
+// k levels per block of the fused iteration.  target > 0: about `target`
+// blocks.  target <= 0: the split that minimises (rounds of resident blocks)
+// x (steps per block), with two 1024-thread blocks resident per CU.
+int sor_fused_kchunk(const Grid& g, const SplitGrid& s, int target, int ncu) {
+  const int ntt = (s.nt + FTX - 1) / FTX, ntj = (g.jm + FTY - 1) / FTY;
+  const int tiles = ntt * ntj;
+  if (target > 0) {
+    int tz = target / tiles;
+    if (tz > g.km) tz = g.km;
+    if (tz < 1) tz = 1;
+    return (g.km + tz - 1) / tz;
+  }
+  const long slots = 2L * (ncu > 0 ? ncu : 256);
+  long best = -1;
+  int best_kc = g.km;
+  for (int tz = 1; tz <= g.km; tz++) {
+    const int kc = (g.km + tz - 1) / tz;
+    const int nz = (g.km + kc - 1) / kc;
+    const long rounds = ((long)tiles * nz + slots - 1) / slots;
+    const long cost = rounds * (kc + 2);
+    if (best < 0 || cost < best) {
+      best = cost;
+      best_kc = kc;
+    }
+  }
+  return best_kc;
+}
+
+int sor_fused_parts_needed(const Grid& g, const SplitGrid& s, int target, int ncu) {
+  const int ntt = (s.nt + FTX - 1) / FTX, ntj = (g.jm + FTY - 1) / FTY;
+  const int kc = sor_fused_kchunk(g, s, target, ncu);
+  return ntt * ntj * ((g.km + kc - 1) / kc);
 }
 
 int reduction_parts_needed(const Grid& g) {

@@ -58,10 +58,15 @@ struct DevDomain {
   SplitBufs sb;
   int sor_blocks;    // target number of blocks of one SOR sweep
   int xcd_remap;     // give each XCD a contiguous range of SOR tiles
+  int sor_fused;     // one launch per SOR iteration (k_sor_iter) instead of one per colour
+  int sor_fused_blocks;  // target number of blocks of one fused SOR iteration (0: automatic)
+  int ncu;           // compute units of the device
 };
 
 int sor_kchunk(const Grid& g, const SplitGrid& s, int target);
 int sor_parts_needed(const Grid& g, const SplitGrid& s, int target);
+int sor_fused_kchunk(const Grid& g, const SplitGrid& s, int target, int ncu);
+int sor_fused_parts_needed(const Grid& g, const SplitGrid& s, int target, int ncu);
 int reduction_parts_needed(const Grid& g);
 
 // Event pairs around launches, recorded only when d.timer is set.

@@ -6,6 +6,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
 #include <map>
 #include <mutex>
 #include <string>
@@ -435,7 +436,16 @@ int domain_alloc(int im, int jm, int km, les_domain** out) {
   D->d.npartials = reduction_parts_needed(D->d.g);
   D->d.sor_blocks = env_int("LES_SOR_BLOCKS", 2048);
   D->d.xcd_remap = env_int("LES_XCD_REMAP", 1);
-  const int nsor = sor_parts_needed(D->d.g, D->d.sg, D->d.sor_blocks);
+  D->d.sor_fused = env_int("LES_SOR_FUSED", 1);
+  D->d.sor_fused_blocks = env_int("LES_SOR_FUSED_BLOCKS", 0);
+  {
+    int dev = 0, ncu = 0;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess)
+      D->d.ncu = ncu;
+  }
+  const int nsor = std::max(sor_parts_needed(D->d.g, D->d.sg, D->d.sor_blocks),
+                            sor_fused_parts_needed(D->d.g, D->d.sg, D->d.sor_fused_blocks, D->d.ncu));
   e = hipMalloc(&D->d.partials, ((size_t)D->d.npartials + 4 * (size_t)nsor) * sizeof(double));
   if (e == hipSuccess)
     for (int q = 0; q < 4; q++) D->d.sb.part[q >> 1][q & 1] = D->d.partials + D->d.npartials + (size_t)q * nsor;

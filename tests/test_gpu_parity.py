@@ -149,3 +149,39 @@ def test_graph_replay_matches_eager():
         assert eager.press_stats()["sor"].tolist() == graph.press_stats()["sor"].tolist()
     eager.close()
     graph.close()
+
+
+FUSED_CASES = [
+    (dict(im=250, jm=29, km=20, seed=8), 640),   # three pair tiles, three row tiles, odd jm
+    (dict(im=37, jm=12, km=9, seed=9), 640),     # one row tile exactly, odd im
+    (dict(im=20, jm=3, km=7, seed=10), 640),     # jm odd and smaller than the tile ring
+    (dict(im=9, jm=2, km=5, seed=11), 640),      # jm = 2
+    (dict(im=64, jm=26, km=33, seed=12), 1),     # one k chunk of 33 levels
+    (dict(im=64, jm=26, km=33, seed=12), 5000),  # one level per block
+]
+
+
+@pytest.mark.parametrize("case,blocks", FUSED_CASES,
+                         ids=lambda c: f"{c['im']}x{c['jm']}x{c['km']}" if isinstance(c, dict) else f"b{c}")
+def test_fused_sor_iteration_equals_colour_sweeps(case, blocks, monkeypatch):
+    """One launch per SOR iteration (red on the tile and a ring, then black,
+    marching k) gives the iterates of the two colour sweeps bit for bit, for
+    any tiling of the domain, and the same iteration counts and residuals."""
+    from lesamd import api
+    s = _mk(case)
+    print(f"\n[fused sor] {s.workload.name} blocks={blocks}", flush=True)
+    monkeypatch.setenv("LES_SOR_FUSED", "0")
+    two = api.Domain(s)
+    monkeypatch.setenv("LES_SOR_FUSED", "1")
+    monkeypatch.setenv("LES_SOR_FUSED_BLOCKS", str(blocks))
+    one = api.Domain(s)
+    for n in range(1, 4):
+        two.step(n, 3)
+        one.step(n, 3)
+        for name in ("p", "u", "v", "w", "rhs"):
+            assert np.array_equal(two.download(name), one.download(name)), (n, name)
+        a, b = two.press_stats(), one.press_stats()
+        assert a["iters"] == b["iters"]
+        np.testing.assert_allclose(a["sor"], b["sor"], rtol=1e-12)
+    two.close()
+    one.close()
