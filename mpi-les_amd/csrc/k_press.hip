@@ -290,16 +290,42 @@ struct SorSlot {
   float bq;          // black previous iterate (level kr+1), for the LDS ring
 };
 
+// Buffer access with a per-lane byte offset and a uniform (scalar) byte
+// offset: no per-access address arithmetic.  All colour-split arrays live
+// in one allocation, addressed through one descriptor; the array and the
+// level enter through the scalar offset.
+typedef __amdgpu_buffer_rsrc_t Rsrc;
+__device__ __forceinline__ Rsrc brsrc(const float* p, unsigned bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc((void*)p, (short)0, (int)bytes, 0x00020000);
+}
+// soff is the same in every lane (block- and level-derived); readfirstlane
+// states so, keeping it in a scalar register
+__device__ __forceinline__ float bload(Rsrc r, unsigned voff, unsigned soff) {
+  return __builtin_bit_cast(
+      float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, __builtin_amdgcn_readfirstlane(soff), 0));
+}
+__device__ __forceinline__ void bstore(Rsrc r, float v, unsigned voff, unsigned soff) {
+  __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, voff,
+                                        __builtin_amdgcn_readfirstlane(soff), 0);
+}
+__device__ __forceinline__ unsigned boff(const float* base, const float* p) {
+  return (unsigned)((const char*)p - (const char*)base);
+}
+
+// two blocks per CU need at most 64 VGPRs per lane (8 waves per SIMD)
 template <bool SEP>
-__global__ __launch_bounds__(FNT) void k_sor_iter(Grid g, SplitGrid s, Metrics m, SplitBufs sb, Ctrl* ctrl,
-                                                  int iter, int nparts, float omega, float pjuge, int ntt,
-                                                  int ntj, int kchunk, int remap) {
+__global__ __launch_bounds__(FNT) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_sor_iter(
+    Grid g, SplitGrid s, Metrics m, SplitBufs sb, Ctrl* ctrl, int iter, int nparts, float omega, float pjuge,
+    int ntt, int ntj, int kchunk, int remap) {
   __shared__ float ldsB[4][FY][FX];  // previous black iterate, levels mod 4
   __shared__ float ldsR[2][FY][FX];  // new red values, levels mod 2
   __shared__ double lds[FNT / kWave];
   __shared__ int skip;
   extern __shared__ float lvm[];  // [3][kchunk + 2]: cn4l, cn4s, cncz of levels krs..kre
-  const int lane = threadIdx.x, row = threadIdx.y, tid = row * FX + lane;
+  const int lane = threadIdx.x;
+  // a wave is one row of the block (blockDim.x == 64): row-derived values are wave-uniform
+  const int row = __builtin_amdgcn_readfirstlane(threadIdx.y);
+  const int tid = row * FX + lane;
   if (tid == 0) skip = (ctrl->sor_done != 0);
   __syncthreads();
   if (skip) return;
@@ -318,15 +344,13 @@ __global__ __launch_bounds__(FNT) void k_sor_iter(Grid g, SplitGrid s, Metrics m
     __syncthreads();  // lds is reused below
   }
   const int prv = (iter - 1) & 1, cur = iter & 1;
-  const float* __restrict__ Ro = sb.p[prv][1];
-  const float* __restrict__ Bo = sb.p[prv][0];
-  float* __restrict__ Rn = sb.p[cur][1];
-  float* __restrict__ Bn = sb.p[cur][0];
-  const float* __restrict__ rR = sb.rhs[1];
-  const float* __restrict__ rB = sb.rhs[0];
-  const float* __restrict__ cR = sb.cn1[1];
-  const float* __restrict__ cB = sb.cn1[0];
   const int im = g.im, jm = g.jm, km = g.km, nt = s.nt, cp = s.cp, cpl = s.cplane;
+  const float* base = sb.p[0][0];  // the first of the eight split arrays of one allocation
+  const Rsrc S = brsrc(base, 8u * (unsigned)cpl * (unsigned)km * 4u);
+  const unsigned oRo = boff(base, sb.p[prv][1]), oBo = boff(base, sb.p[prv][0]);
+  const unsigned oRn = boff(base, sb.p[cur][1]), oBn = boff(base, sb.p[cur][0]);
+  const unsigned orR = boff(base, sb.rhs[1]), orB = boff(base, sb.rhs[0]);
+  const unsigned ocR = SEP ? 0u : boff(base, sb.cn1[1]), ocB = SEP ? 0u : boff(base, sb.cn1[0]);
   const int lb = remap ? xcd_contiguous(bid, nparts) : bid;
   const int bt = lb % ntt, bj = (lb / ntt) % ntj, bk = lb / (ntt * ntj);
   const int t = bt * FTX - 2 + lane;
@@ -334,23 +358,29 @@ __global__ __launch_bounds__(FNT) void k_sor_iter(Grid g, SplitGrid s, Metrics m
   const int j = jwrap(jl, jm);            // the row it stands for
   const int k0 = 1 + bk * kchunk, k1 = min(km, k0 + kchunk - 1);
   const bool tin = (t >= 0) && (t < nt);
-  const bool in_red = tin && lane >= 1 && lane <= FX - 2 && row >= 1 && row <= FY - 2;
-  const bool in_tile = tin && lane >= 2 && lane <= FX - 3 && row >= 2 && row <= FY - 3 && jl <= jm;
+  const bool row_red = row >= 1 && row <= FY - 2;               // wave-uniform
+  const bool row_tile = row >= 2 && row <= FY - 3 && jl <= jm;  // wave-uniform
+  const bool lane_tile = tin && lane >= 2 && lane <= FX - 3;
   const bool jodd = (jm & 1) != 0;
   const bool wlo = jodd && j == 1, whi = jodd && j == jm;  // same-colour periodic neighbours
   const int wofs = (jm - 1) * cp;
   // lanes outside the pairs of the row read pair 0 of their own row, so the
-  // periodic row shift below stays inside the array for every lane
+  // periodic row shift stays inside the array for every lane
   const int q0 = (tin ? t : 0) + (j - 1) * cp;
-  auto lvl = [&](int k) { return q0 + (k - 1) * cpl; };
+  const unsigned vo = 4u * (unsigned)q0;
+  const unsigned vw = 4u * (unsigned)(q0 + (wlo ? wofs : (whi ? -wofs : 0)));
+  const unsigned cplb = 4u * (unsigned)cpl;
+  auto soff = [&](int k) { return (unsigned)(k - 1) * cplb; };
   // metrics of the two cells of the pair (i = 2t+1, 2t+2) and of row j
-  float c2l[2] = {0.f, 0.f}, c2s[2] = {0.f, 0.f}, ax[2] = {0.f, 0.f};
+  float c2l0 = 0.f, c2l1 = 0.f, c2s0 = 0.f, c2s1 = 0.f, ax0 = 0.f, ax1 = 0.f;
   if (tin) {
-    for (int d = 0; d < 2; d++) {
-      const int i = 2 * t + 1 + d;
-      c2l[d] = M1(m.cn2l, i);
-      c2s[d] = M1(m.cn2s, i);
-      ax[d] = SEP ? M1(m.cnax, i) : 0.0f;
+    c2l0 = M1(m.cn2l, 2 * t + 1);
+    c2l1 = M1(m.cn2l, 2 * t + 2);
+    c2s0 = M1(m.cn2s, 2 * t + 1);
+    c2s1 = M1(m.cn2s, 2 * t + 2);
+    if (SEP) {
+      ax0 = M1(m.cnax, 2 * t + 1);
+      ax1 = M1(m.cnax, 2 * t + 2);
     }
   }
   const float c3l = M1(m.cn3l, j), c3s = M1(m.cn3s, j);
@@ -367,94 +397,96 @@ __global__ __launch_bounds__(FNT) void k_sor_iter(Grid g, SplitGrid s, Metrics m
   }
   // The streams: red operands of level L, black operands of level L-1 and
   // the black iterate of level L+1, for the step at level L.  Every lane
-  // issues the same loads in every step (levels clamped into 1..km, lanes
-  // outside the domain reading pair 0), so the number of loads in flight is
-  // the same on every path and a use waits only for its own load; values
-  // loaded for clamped levels or idle lanes are never used.
-  const int wsh = wlo ? wofs : (whi ? -wofs : 0);
+  // issues the same loads in every step (levels clamped into 1..km), so the
+  // number of loads in flight is the same on every path and a use waits
+  // only for its own load; values of clamped levels or idle lanes are never
+  // used.
   auto fill = [&](SorSlot& sl, int L) {
-    const int qr = lvl(min(L, km));
-    const int qb = lvl(max(1, min(L - 1, km)));
-    sl.ro = Ro[qr];
-    sl.rr = rR[qr];
-    if (!SEP) sl.cr = cR[qr];
-    sl.wr = Ro[qr + wsh];
-    sl.rb = rB[qb];
-    if (!SEP) sl.cb = cB[qb];
-    sl.wb = Bo[qb + wsh];
-    sl.bq = Bo[lvl(min(L + 1, km))];
+    const unsigned sr = soff(min(L, km)), sk = soff(max(1, min(L - 1, km)));
+    sl.ro = bload(S, vo, oRo + sr);
+    sl.rr = bload(S, vo, orR + sr);
+    if (!SEP) sl.cr = bload(S, vo, ocR + sr);
+    sl.wr = bload(S, vw, oRo + sr);
+    sl.rb = bload(S, vo, orB + sk);
+    if (!SEP) sl.cb = bload(S, vo, ocB + sk);
+    sl.wb = bload(S, vw, oBo + sk);
+    sl.bq = bload(S, vo, oBo + soff(min(L + 1, km)));
   };
   SorSlot s0 = {}, s1 = {}, s2 = {};
   fill(s0, krs);
   fill(s1, krs + 1);
   fill(s2, krs + 2);
   // the black iterate of levels krs-1 and krs enters the ring directly
-  if (tin) {
-    if (krs >= 2) ldsB[(krs - 1) & 3][row][lane] = Bo[lvl(krs - 1)];
-    ldsB[krs & 3][row][lane] = Bo[lvl(krs)];
-  }
-  float rm2 = 0.f, rm1 = 0.f;  // new red values of levels kr-2, kr-1 at this (pair, row)
+  if (krs >= 2) ldsB[(krs - 1) & 3][row][lane] = bload(S, vo, oBo + soff(krs - 1));
+  ldsB[krs & 3][row][lane] = bload(S, vo, oBo + soff(krs));
+  float rm2 = 0.f;  // new red value of level kr-2 at this (pair, row)
   double acc = 0.0;
+  const bool first_t = (t == 0);          // i = 1 when d = 0
+  const bool last0 = (2 * t + 1 == im);   // i = im when d = 0
+  const bool last1 = (2 * t + 2 == im);   // i = im when d = 1
+  const bool ok0 = lane_tile && (2 * t + 1 <= im), ok1 = lane_tile && (2 * t + 2 <= im);
 
   auto step = [&](int kr, SorSlot& sl) {
     const bool active = kr <= kre;  // steps past kre only pad the last group of three
-    const bool has_red = active && kr <= km;
     const int kb = kr - 1;
-    const bool has_blk = active && kb >= k0;
     if (active && kr + 1 <= km) ldsB[(kr + 1) & 3][row][lane] = sl.bq;
     __syncthreads();
+    // The red cell of level kr and the black cell of level kr-1 in this
+    // (pair, row) have the same position d in the pair: i = 2t+1+d.  The
+    // other colour's array holds i-1 at pair t-1+d and i+1 at pair t+d.
+    const int d = (j + kr) & 1;  // wave-uniform
+    const bool w = d ? ok1 : ok0;  // this lane owns an interior cell of the tile
+    const bool at_first = d ? false : first_t;
+    const bool at_last = d ? last1 : last0;
+    const float c2l = d ? c2l1 : c2l0, c2s = d ? c2s1 : c2s0, axd = d ? ax1 : ax0;
     float r0 = 0.f;
-    if (has_red && in_red) {
-      const int d = (j + kr) & 1;  // red: i+j+k odd
-      const int i = 2 * t + 1 + d;
-      if (i <= im) {
-        const float own = sl.ro;
-        const float a = ldsB[kr & 3][row][lane];
-        const float bb = d ? ldsB[kr & 3][row][lane + 1] : ldsB[kr & 3][row][lane - 1];
-        const float lo = d ? a : bb, hi = d ? bb : a;  // i-1, i+1
-        const float pim = (i == 1) ? own : lo;
-        const float pip = (i == im) ? own : hi;
-        const float pjm = wlo ? sl.wr : ldsB[kr & 3][row - 1][lane];
-        const float pjp = whi ? sl.wr : ldsB[kr & 3][row + 1][lane];
-        const float pkm = (kr == 1) ? own : ldsB[(kr - 1) & 3][row][lane];
-        const float pkp = (kr == km) ? own : ldsB[(kr + 1) & 3][row][lane];
-        const int x = kr - krs;
-        const float cn = SEP ? 1.0f / ((ax[d] + byj) + lvm[2 * nl + x]) : sl.cr;  // init.f95:100-107
-        const float reltmp = omega * (cn * (c2l[d] * pip + c2s[d] * pim + c3l * pjp + c3s * pjm +
-                                            lvm[x] * pkp + lvm[nl + x] * pkm - sl.rr) -
-                                      own);
-        r0 = own + reltmp;
-        if (in_tile && kr >= k0 && kr <= k1) {
-          Rn[lvl(kr)] = r0;
-          acc += (double)reltmp * (double)reltmp;
-        }
-      }
+    if (active && kr <= km && row_red) {
+      // every lane of the wave computes (lanes outside the ring compute
+      // values nobody reads); the tile's cells are stored
+      const float own = sl.ro;
+      const float* pl = &ldsB[kr & 3][row][lane - 1 + d];
+      const float lo = pl[0], hi = pl[1];
+      const float pim = at_first ? own : lo;
+      const float pip = at_last ? own : hi;
+      const float pjm = wlo ? sl.wr : ldsB[kr & 3][row - 1][lane];
+      const float pjp = whi ? sl.wr : ldsB[kr & 3][row + 1][lane];
+      const float pkm = (kr == 1) ? own : ldsB[(kr - 1) & 3][row][lane];
+      const float pkp = (kr == km) ? own : ldsB[(kr + 1) & 3][row][lane];
+      const int x = kr - krs;
+      const float cn = SEP ? 1.0f / ((axd + byj) + lvm[2 * nl + x]) : sl.cr;  // init.f95:100-107
+      const float reltmp = omega * (cn * (c2l * pip + c2s * pim + c3l * pjp + c3s * pjm + lvm[x] * pkp +
+                                          lvm[nl + x] * pkm - sl.rr) -
+                                    own);
+      r0 = own + reltmp;
       ldsR[kr & 1][row][lane] = r0;
-    }
-    if (has_blk && in_tile) {
-      const int d = (1 + j + kb) & 1;  // black: i+j+k even
-      const int i = 2 * t + 1 + d;
-      if (i <= im) {
-        const float own = ldsB[kb & 3][row][lane];
-        const float bb = d ? ldsR[kb & 1][row][lane + 1] : ldsR[kb & 1][row][lane - 1];
-        const float lo = d ? rm1 : bb, hi = d ? bb : rm1;
-        const float pim = (i == 1) ? own : lo;
-        const float pip = (i == im) ? own : hi;
-        const float pjm = wlo ? sl.wb : ldsR[kb & 1][row - 1][lane];
-        const float pjp = whi ? sl.wb : ldsR[kb & 1][row + 1][lane];
-        const float pkm = (kb == 1) ? own : rm2;
-        const float pkp = (kb == km) ? own : r0;
-        const int x = kb - krs;
-        const float cn = SEP ? 1.0f / ((ax[d] + byj) + lvm[2 * nl + x]) : sl.cb;
-        const float reltmp = omega * (cn * (c2l[d] * pip + c2s[d] * pim + c3l * pjp + c3s * pjm +
-                                            lvm[x] * pkp + lvm[nl + x] * pkm - sl.rb) -
-                                      own);
-        Bn[lvl(kb)] = own + reltmp;
-        acc += (double)reltmp * (double)reltmp;
+      if (kr >= k0 && kr <= k1 && row_tile && w) {
+        bstore(S, r0, vo, oRn + soff(kr));
+        // the FP64 product of two FP32 values is exact, so the fused
+        // multiply-add rounds exactly as multiply then add
+        acc = __builtin_fma((double)reltmp, (double)reltmp, acc);
       }
     }
-    rm2 = rm1;
-    rm1 = r0;
+    if (active && kb >= k0 && row_tile) {
+      const float own = ldsB[kb & 3][row][lane];
+      const float* pl = &ldsR[kb & 1][row][lane - 1 + d];
+      const float lo = pl[0], hi = pl[1];
+      const float pim = at_first ? own : lo;
+      const float pip = at_last ? own : hi;
+      const float pjm = wlo ? sl.wb : ldsR[kb & 1][row - 1][lane];
+      const float pjp = whi ? sl.wb : ldsR[kb & 1][row + 1][lane];
+      const float pkm = (kb == 1) ? own : rm2;
+      const float pkp = (kb == km) ? own : r0;
+      const int x = kb - krs;
+      const float cn = SEP ? 1.0f / ((axd + byj) + lvm[2 * nl + x]) : sl.cb;
+      const float reltmp = omega * (cn * (c2l * pip + c2s * pim + c3l * pjp + c3s * pjm + lvm[x] * pkp +
+                                          lvm[nl + x] * pkm - sl.rb) -
+                                    own);
+      if (w) {
+        bstore(S, own + reltmp, vo, oBn + soff(kb));
+        acc = __builtin_fma((double)reltmp, (double)reltmp, acc);
+      }
+    }
+    rm2 = ldsR[(kr - 1) & 1][row][lane];  // red of level kr-1, for the next step's black
     fill(sl, kr + 3);
   };
   // unrolled by the stream depth, with no conditional step, so that each

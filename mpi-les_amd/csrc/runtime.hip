@@ -200,8 +200,14 @@ void derive(les_domain* D);
 // cn1 of init.f95:100-107 is 1/(cnax(i)+cnby(j)+cncz(k)); when the array
 // handed over equals that bit for bit the SOR recomputes it instead of
 // reading 4 bytes per cell per sweep.
+int env_int(const char* name, int dflt);
+
 void check_cn1(les_domain* D, const float* host) {
   derive(D);
+  if (env_int("LES_SOR_CN1_LOAD", 0)) {  // read cn1 in the SOR even when it could be recomputed
+    D->d.cn1_separable = 0;
+    return;
+  }
   const Grid& g = D->d.g;
   const float* ax = D->mhost.data() + D->moff[MCNAX] + 1;
   const float* by = D->mhost.data() + D->moff[MCNBY] + 1;

@@ -16,13 +16,15 @@ The library must be built (mpi-les_amd/build.py); there is no fallback.
 from __future__ import annotations
 
 import ctypes
+import os
 from pathlib import Path
 
 import numpy as np
 
 from . import layout
 
-LIB_PATH = Path(__file__).with_name("liblesamd.so")
+# LESAMD_LIBRARY selects another build of the library (same ABI)
+LIB_PATH = Path(os.environ.get("LESAMD_LIBRARY") or Path(__file__).with_name("liblesamd.so"))
 _lib = None
 
 INT_ARGS = {"km", "jm", "im", "n", "nmax"}
