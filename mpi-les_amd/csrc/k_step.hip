@@ -374,6 +374,293 @@ void launch_velfg(const DevDomain& d, float vn) {
                      d.F.f, d.F.g, d.F.h, vn));
 }
 
+// ------------------------------------------- fused vel2 + velfg + feedbf
+// One pass of vel2 (vel2.f95:54-356), velfg (velFG.f95:64-110) and feedbf
+// (feedbf.f95:35-61) for the whole-step path.  A thread owns the column
+// (i,j) and marches its k levels.  The products cov1..cov9 (dead after
+// velfg) are never stored; diu1..diu9, which les reads, are stored together
+// with every ghost cell vel2's copies leave (vel2.f95:202-314), each written
+// by the thread that owns its source.  The diu/cov values velfg needs at
+// i+1 and j+1 are recomputed (at i=im and j=jm the ghost copies: the value
+// at im, and the periodic image at j=1), those at k+1 carried from level to
+// level.  Every value is computed by the expression of the unfused kernels,
+// so results are bit-identical to velfg followed by feedbf.
+struct DC {
+  float d, c;  // diu, cov
+};
+
+struct VelCtx {
+  Grid g;
+  Metrics m;
+  const float* __restrict__ u;
+  const float* __restrict__ v;
+  const float* __restrict__ w;
+  float wl;
+  __device__ float U(int i, int j, int k) const { return u[ix(g, i, j, k)]; }
+  __device__ float V(int i, int j, int k) const { return v[ix(g, i, j, k)]; }
+  __device__ float W(int i, int j, int k) const { return w[ix(g, i, j, k)]; }
+  __device__ float dx(int i) const { return M1(m.dx1, i); }
+  __device__ float dy(int j) const { return M1(m.dy1, j); }
+  // vel2.f95:77-86
+  __device__ DC q1(int i, int j, int k) const {
+    const float a = U(i - 1, j, k), b = U(i, j, k);
+    const float nou = (a + b) / 2.0f, d = (-a + b) / dx(i);
+    return {d, nou * d};
+  }
+  __device__ DC q5(int i, int j, int k) const {
+    const float a = V(i, j - 1, k), b = V(i, j, k);
+    const float nou = (a + b) / 2.0f, d = (-a + b) / dy(j);
+    return {d, nou * d};
+  }
+  __device__ DC q9(int i, int j, int k) const {
+    const float a = W(i, j, k - 1), b = W(i, j, k);
+    const float nou = (a + b) / 2.0f, d = (-a + b) / M1(m.dzn, k);
+    return {d, nou * d};
+  }
+  // vel2.f95:103-111
+  __device__ DC q2(int i, int j, int k) const {
+    const float dxi = dx(i), dxp = dx(i + 1);
+    const float nou = (dxp * V(i, j - 1, k) + dxi * V(i + 1, j - 1, k)) / (dxi + dxp);
+    const float d = 2.0f * (-U(i, j - 1, k) + U(i, j, k)) / (dy(j - 1) + dy(j));
+    return {d, nou * d};
+  }
+  // vel2.f95:144-152
+  __device__ DC q4(int i, int j, int k) const {
+    const float dyj = dy(j), dyp = dy(j + 1);
+    const float nou = (dyp * U(i - 1, j, k) + dyj * U(i - 1, j + 1, k)) / (dyj + dyp);
+    const float d = 2.0f * (-V(i - 1, j, k) + V(i, j, k)) / (dx(i - 1) + dx(i));
+    return {d, (nou - 0.0f) * d};
+  }
+  // vel2.f95:119-124 (k >= 2)
+  __device__ DC q3(int i, int j, int k) const {
+    const float dxi = dx(i), dxp = dx(i + 1);
+    const float nou = (dxp * W(i, j, k - 1) + dxi * W(i + 1, j, k - 1)) / (dxi + dxp);
+    const float d = (-U(i, j, k - 1) + U(i, j, k)) / M1(m.dzs, k - 1);
+    return {d, nou * d};
+  }
+  // vel2.f95:154-159 (k >= 2)
+  __device__ DC q6(int i, int j, int k) const {
+    const float dyj = dy(j), dyp = dy(j + 1);
+    const float nou = (dyp * W(i, j, k - 1) + dyj * W(i, j + 1, k - 1)) / (dyj + dyp);
+    const float d = (-V(i, j, k - 1) + V(i, j, k)) / M1(m.dzs, k - 1);
+    return {d, nou * d};
+  }
+  // vel2.f95:179-186 (k <= km-1)
+  __device__ DC q7(int i, int j, int k) const {
+    const float dzk = M1(m.dzn, k), dzp = M1(m.dzn, k + 1);
+    const float nou = (dzp * U(i - 1, j, k) + dzk * U(i - 1, j, k + 1)) / (dzk + dzp);
+    const float d = 2.0f * (-W(i - 1, j, k) + W(i, j, k)) / (dx(i - 1) + dx(i));
+    return {d, (nou - 0.0f) * d};
+  }
+  // vel2.f95:189-197 (k <= km-1)
+  __device__ DC q8(int i, int j, int k) const {
+    const float dzk = M1(m.dzn, k), dzp = M1(m.dzn, k + 1);
+    const float nou = (dzp * V(i, j - 1, k) + dzk * V(i, j - 1, k + 1)) / (dzk + dzp);
+    const float d = 2.0f * (-W(i, j - 1, k) + W(i, j, k)) / (dy(j - 1) + dy(j));
+    return {d, nou * d};
+  }
+  // the wall layer k=1 of diu3/diu6 with |U_h|, |V_h| (vel2.f95:54-65,
+  // 135-141, 170-176) and uspd, vspd (vel2.f95:351-356)
+  __device__ void wall(int i, int j, DC& r3, DC& r6, float& us_out, float& vs_out) const {
+    const float dxi = dx(i), dxp = dx(i + 1), dyj = dy(j), dyp = dy(j + 1);
+    const float uc = U(i, j, 1), vc = V(i, j, 1);
+    float t = (0.5f * (V(i, j - 1, 1) + vc) * dxp + 0.5f * (V(i + 1, j - 1, 1) + V(i + 1, j, 1)) * dxi) / (dxi + dxp);
+    const float us = sqrtf(uc * uc + t * t);
+    t = (0.5f * (U(i - 1, j, 1) + uc) * dyp + 0.5f * (U(i - 1, j + 1, 1) + U(i, j + 1, 1)) * dyj) / (dyj + dyp);
+    const float vs = sqrtf(vc * vc + t * t);
+    const float dz1 = M1(m.dzn, 1);
+    float nou = 0.5f * (dxp * W(i, j, 1) + dxi * W(i + 1, j, 1)) / (dxi + dxp);
+    float d = us * 0.4f / wl / (0.5f * dz1) / 0.4f * uc / us;
+    r3 = {d, nou * d};
+    nou = 0.5f * (dyp * W(i, j, 1) + dyj * W(i, j + 1, 1)) / (dyj + dyp);
+    d = vs * 0.4f / wl / (0.5f * dz1) / 0.4f * vc / vs;
+    r6 = {d, nou * d};
+    us_out = uc / us;
+    vs_out = vc / vs;
+  }
+};
+
+__global__ __launch_bounds__(NT) void k_velfg_feedbf(VelCtx X, P10 diu, float* __restrict__ f,
+                                                     float* __restrict__ gg, float* __restrict__ h,
+                                                     float* __restrict__ usum, float* __restrict__ vsum,
+                                                     float* __restrict__ wsum, const float* __restrict__ bm,
+                                                     const float* __restrict__ cm, const float* __restrict__ dm,
+                                                     float* __restrict__ uspd, float* __restrict__ vspd, float vn,
+                                                     float alpha, float dt, float beta, int kchunk) {
+  const Grid& g = X.g;
+  const Metrics& m = X.m;
+  const int i = 1 + blockIdx.x * BX + threadIdx.x, j = 1 + blockIdx.y * BY + threadIdx.y;
+  if (i > g.im || j > g.jm) return;
+  const int im = g.im, jm = g.jm, km = g.km;
+  const int k0 = 1 + blockIdx.z * kchunk, k1 = min(km, k0 + kchunk - 1);
+  if (k0 > km) return;
+  const int jn = (j == jm) ? 1 : j + 1;  // the row velfg reads at j+1 (periodic ghost at jm)
+  const bool ilast = (i == im);
+  const float dxi = X.dx(i), dxp = X.dx(i + 1);
+  const float dyj = X.dy(j), dyp = X.dy(j + 1);
+  float* const* D = diu.a;
+  auto put = [&](int q, int ii, int jj, int kk, float val) { D[q][ix(g, ii, jj, kk)] = val; };
+  auto cpy = [&](int q, int ii, int jj, int kk, int si, int sj, int sk) {
+    D[q][ix(g, ii, jj, kk)] = D[q][ix(g, si, sj, sk)];
+  };
+  // diu3, diu6 (and diu9) of level k; the march carries them from k to k+1
+  DC z3, z6, z9 = {0.f, 0.f};
+  if (k0 == 1) {
+    float us, vs;
+    X.wall(i, j, z3, z6, us, vs);
+    const int c2 = (i + 1) + (j + 1) * g.sj;
+    uspd[c2] = us;
+    vspd[c2] = vs;
+  } else {
+    z3 = X.q3(i, j, k0);
+    z6 = X.q6(i, j, k0);
+  }
+  z9 = X.q9(i, j, k0);
+  for (int k = k0; k <= k1; k++) {
+    const int c = ix(g, i, j, k);
+    const float dzk = M1(m.dzn, k);
+    const DC a1 = X.q1(i, j, k), a2 = X.q2(i, j, k), a4 = X.q4(i, j, k), a5 = X.q5(i, j, k);
+    const DC a3 = z3, a6 = z6, a9 = z9;
+    const bool hk = (k <= km - 1);
+    DC a7 = {0.f, 0.f}, a8 = {0.f, 0.f};
+    if (hk) {
+      a7 = X.q7(i, j, k);
+      a8 = X.q8(i, j, k);
+    }
+    // next level (k+1 <= km+1 for diu3/diu6, <= km for diu9)
+    const DC n3 = X.q3(i, j, k + 1), n6 = X.q6(i, j, k + 1);
+    const DC n9 = (k + 1 <= km) ? X.q9(i, j, k + 1) : DC{0.f, 0.f};
+    // neighbours at i+1 (the copy of im at i=im) and j+1 (row 1 at j=jm)
+    const DC b1 = ilast ? a1 : X.q1(i + 1, j, k);
+    const DC b4 = ilast ? a4 : X.q4(i + 1, j, k);
+    const DC b2 = X.q2(i, jn, k);
+    const DC b5 = X.q5(i, jn, k);
+    // ---- velFG.f95:64-77
+    float fv, gv;
+    {
+      const float covx1 = (dxp * a1.c + dxi * b1.c) / (dxi + dxp);
+      const float covy1 = (a2.c + b2.c) / 2.0f;
+      const float covz1 = (a3.c + n3.c) / 2.0f;
+      const float covc = covx1 + covy1 + covz1;
+      const float dfu = 2.0f * (-a1.d + b1.d) / (dxi + dxp) + (-a2.d + b2.d) / dyj + (-a3.d + n3.d) / dzk;
+      fv = (-covc + vn * dfu);
+    }
+    // ---- velFG.f95:80-93
+    {
+      const float covx1 = (a4.c + b4.c) / 2.0f;
+      const float covy1 = (dyp * a5.c + dyj * b5.c) / (dyj + dyp);
+      const float covz1 = (a6.c + n6.c) / 2.0f;
+      const float covc = covx1 + covy1 + covz1;
+      const float dfv = (-a4.d + b4.d) / dxi + 2.0f * (-a5.d + b5.d) / (dyj + dyp) + (-a6.d + n6.d) / dzk;
+      gv = (-covc + vn * dfv);
+    }
+    // ---- velFG.f95:97-110 (k <= km-1; h(km) keeps its value)
+    float hv;
+    if (hk) {
+      const DC b7 = ilast ? a7 : X.q7(i + 1, j, k);
+      const DC b8 = X.q8(i, jn, k);
+      const float dzp = M1(m.dzn, k + 1);
+      const float covx1 = (a7.c + b7.c) / 2.0f;
+      const float covy1 = (a8.c + b8.c) / 2.0f;
+      const float covz1 = (dzp * a9.c + dzk * n9.c) / (dzk + dzp);
+      const float covc = covx1 + covy1 + covz1;
+      const float dfw = (-a7.d + b7.d) / dxi + (-a8.d + b8.d) / dyj + (-a9.d + n9.d) / M1(m.dzs, k);
+      hv = (-covc + vn * dfw);
+    } else {
+      hv = h[c];
+    }
+    // ---- feedbf.f95:35-61
+    {
+      const float uc = X.u[c], vc = X.v[c], wc = X.w[c], b = bm[c], cc = cm[c], dd = dm[c];
+      const float su = (usum[c] + uc) * b, sv = (vsum[c] + vc) * cc, sw = (wsum[c] + wc) * dd;
+      usum[c] = su;
+      vsum[c] = sv;
+      wsum[c] = sw;
+      f[c] = fv + (alpha * su * dt + beta * uc * b);
+      gg[c] = gv + (alpha * sv * dt + beta * vc * cc);
+      h[c] = hv + (alpha * sw * dt + beta * wc * dd);
+    }
+    // ---- diu of level k and the ghost cells vel2.f95:202-314 derives from them
+    put(1, i, j, k, a1.d);
+    put(2, i, j, k, a2.d);
+    put(3, i, j, k, a3.d);
+    put(4, i, j, k, a4.d);
+    put(5, i, j, k, a5.d);
+    put(6, i, j, k, a6.d);
+    put(9, i, j, k, a9.d);
+    if (hk) {
+      put(7, i, j, k, a7.d);
+      put(8, i, j, k, a8.d);
+    }
+    if (ilast) {  // i=im+1 copies of 1,4,7 (vel2.f95:208-226) and of 2,3 (vel2.f95:299-306)
+      put(1, im + 1, j, k, a1.d);
+      put(4, im + 1, j, k, a4.d);
+      if (hk) put(7, im + 1, j, k, a7.d);
+      put(2, im + 1, j, k, a2.d);
+      put(3, im + 1, j, k, a3.d);
+    }
+    if (j == jm) {  // j=0 copies of 2,5,8 (vel2.f95:231-286) and of 4,6 (vel2.f95:308-314)
+      put(2, i, 0, k, a2.d);
+      put(5, i, 0, k, a5.d);
+      if (hk) put(8, i, 0, k, a8.d);
+      put(4, i, 0, k, a4.d);
+      put(6, i, 0, k, a6.d);
+      if (ilast) {
+        put(4, im + 1, 0, k, a4.d);               // diu4(im+1,jm) is the i-copy of diu4(im,jm)
+        cpy(6, im + 1, 0, k, im + 1, jm, k);      // diu6(im+1,jm) is never written
+      }
+    }
+    if (j == 1) {  // j=jm+1 copies of 2,5,8
+      put(2, i, jm + 1, k, a2.d);
+      put(5, i, jm + 1, k, a5.d);
+      if (hk) put(8, i, jm + 1, k, a8.d);
+      if (ilast) {
+        put(2, im + 1, jm + 1, k, a2.d);          // diu2(im,jm+1) is the j-copy of diu2(im,1)
+        cpy(3, im + 1, jm + 1, k, im, jm + 1, k); // diu3(im,jm+1) is never written
+      }
+    }
+    z3 = n3;
+    z6 = n6;
+    z9 = n9;
+  }
+  if (k1 == km) {  // level km+1: diu3, diu6 (vel2.f95:119-176) and the copies from it
+    const int k = km + 1;
+    put(3, i, j, k, z3.d);
+    put(6, i, j, k, z6.d);
+    if (ilast) {
+      put(3, im + 1, j, k, z3.d);
+      cpy(2, im + 1, j, k, im, j, k);
+    }
+    if (j == jm) {
+      cpy(4, i, 0, k, i, jm, k);
+      put(6, i, 0, k, z6.d);
+      if (ilast) {
+        cpy(4, im + 1, 0, k, im + 1, jm, k);
+        cpy(6, im + 1, 0, k, im + 1, jm, k);
+      }
+    }
+    if (j == 1 && ilast) {
+      cpy(2, im + 1, jm + 1, k, im, jm + 1, k);
+      cpy(3, im + 1, jm + 1, k, im, jm + 1, k);
+    }
+  }
+}
+
+void launch_velfg_feedbf(const DevDomain& d, float vn, float alpha, float dt, float beta) {
+  const Grid& g = d.g;
+  const int gx = (g.im + BX - 1) / BX, gy = (g.jm + BY - 1) / BY;
+  int nz = 2048 / (gx * gy);
+  if (nz < 1) nz = 1;
+  if (nz > g.km) nz = g.km;
+  int kc = (g.km + nz - 1) / nz;
+  if (d.velfg_kchunk > 0) kc = d.velfg_kchunk < g.km ? d.velfg_kchunk : g.km;
+  nz = (g.km + kc - 1) / kc;
+  VelCtx X{g, d.m, d.F.u, d.F.v, d.F.w, d.wl};
+  TIMED(d, KC_VEL2, hipLaunchKernelGGL(k_velfg_feedbf, dim3(gx, gy, nz), block2d(), 0, d.stream, X, p10(d.F.diu),
+                                        d.F.f, d.F.g, d.F.h, d.F.usum, d.F.vsum, d.F.wsum, d.F.bmask, d.F.cmask,
+                                        d.F.dmask, d.F.uspd, d.F.vspd, vn, alpha, dt, beta, kc));
+}
+
 // ----------------------------------------------------------------- feedbf
 // feedbf.f95:35-61 (fx,fy,fz are dead outputs and are not stored).
 __global__ __launch_bounds__(NT) void k_feedbf(Grid g, const float* __restrict__ u,

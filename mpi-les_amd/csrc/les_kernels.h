@@ -59,6 +59,8 @@ struct DevDomain {
   int sor_blocks;    // target number of blocks of one SOR sweep
   int xcd_remap;     // give each XCD a contiguous range of SOR tiles
   int sor_fused;     // one launch per SOR iteration (k_sor_iter) instead of one per colour
+  int fuse_velfg;    // whole steps run vel2+velfg+feedbf as one kernel
+  int velfg_kchunk;  // k levels per thread of that kernel (0: automatic)
   int sor_fused_blocks;  // target number of blocks of one fused SOR iteration (0: automatic)
   int ncu;           // compute units of the device
 };
@@ -83,6 +85,7 @@ void launch_velnw(const DevDomain& d, float ro, float dt);
 void launch_bondv1(const DevDomain& d, int n, float dt);
 void launch_velfg(const DevDomain& d, float vn);
 void launch_feedbf(const DevDomain& d, float alpha, float dt, float beta);
+void launch_velfg_feedbf(const DevDomain& d, float vn, float alpha, float dt, float beta);
 void launch_les(const DevDomain& d);
 void launch_adam(const DevDomain& d);
 void launch_press(const DevDomain& d, float dt);

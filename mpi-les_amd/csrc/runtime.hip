@@ -443,6 +443,8 @@ int domain_alloc(int im, int jm, int km, les_domain** out) {
   D->d.sor_blocks = env_int("LES_SOR_BLOCKS", 2048);
   D->d.xcd_remap = env_int("LES_XCD_REMAP", 1);
   D->d.sor_fused = env_int("LES_SOR_FUSED", 1);
+  D->d.fuse_velfg = env_int("LES_FUSE_VELFG", 1);
+  D->d.velfg_kchunk = env_int("LES_VELFG_KCHUNK", 0);
   D->d.sor_fused_blocks = env_int("LES_SOR_FUSED_BLOCKS", 0);
   {
     int dev = 0, ncu = 0;
@@ -491,6 +493,24 @@ int domain_alloc(int im, int jm, int km, les_domain** out) {
   F.cn1 = D->fields[FCN1];
   D->bytes += (long long)off * 4 + ((long long)D->d.npartials + 4LL * nsor) * 8 + (long long)sizeof(Ctrl);
   *out = D;
+  return 0;
+}
+
+int run_stage(les_domain* D, int s, int n, int nmax);
+
+// One whole step n (main.f95:260-296); with fuse_velfg the velfg and feedbf
+// stages run as one kernel.
+int run_step(les_domain* D, int n, int nmax) {
+  for (int s = 0; s < 7; s++) {
+    if (s == 2 && D->d.fuse_velfg) {
+      if (flush_metrics(D)) return 1;
+      launch_velfg_feedbf(D->d, D->vn, D->alpha, D->dt, D->beta);
+      HIP_TRY(hipGetLastError());
+      s = 3;
+      continue;
+    }
+    if (run_stage(D, s, n, nmax)) return 1;
+  }
   return 0;
 }
 
@@ -861,7 +881,7 @@ static int step_graph(les_domain* D, int n, int nmax) {
   if (!D->gexec) {
     HIP_TRY(hipStreamBeginCapture(D->d.stream, hipStreamCaptureModeThreadLocal));
     int rc = 0;
-    for (int s = 0; s < 7 && !rc; s++) rc = run_stage(D, s, n, nmax);
+    rc = run_step(D, n, nmax);
     hipGraph_t gr = nullptr;
     hipError_t e = hipStreamEndCapture(D->d.stream, &gr);
     if (rc) {
@@ -885,9 +905,7 @@ static int step_graph(les_domain* D, int n, int nmax) {
 int les_domain_step(les_domain* D, int n, int nmax) {
   if (!D) return fail("null domain");
   if (D->use_graph && n != 1 && !D->d.timer) return step_graph(D, n, nmax);
-  for (int s = 0; s < 7; s++)
-    if (run_stage(D, s, n, nmax)) return 1;
-  return 0;
+  return run_step(D, n, nmax);
 }
 
 int les_domain_set_graph(les_domain* D, int on) {

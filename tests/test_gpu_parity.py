@@ -185,3 +185,37 @@ def test_fused_sor_iteration_equals_colour_sweeps(case, blocks, monkeypatch):
         np.testing.assert_allclose(a["sor"], b["sor"], rtol=1e-12)
     two.close()
     one.close()
+
+
+FUSED_STEP_CASES = [
+    (dict(im=24, jm=20, km=16, seed=13), 0),     # automatic k chunks
+    (dict(im=23, jm=19, km=13, seed=14), 4),     # odd extents, chunks of 4 levels
+    (dict(im=70, jm=9, km=11, seed=15), 11),     # one chunk per column, two i blocks
+    (dict(im=13, jm=2, km=6, seed=16), 1),       # jm = 2, one level per thread
+]
+ALL_FIELDS = ["u", "v", "w", "p", "f", "g", "h", "usum", "vsum", "wsum", "fold", "gold", "hold", "rhs", "sm",
+              "uspd", "vspd"] + [f"diu{q}" for q in range(1, 10)]
+
+
+@pytest.mark.parametrize("case,kchunk", FUSED_STEP_CASES,
+                         ids=lambda c: f"{c['im']}x{c['jm']}x{c['km']}" if isinstance(c, dict) else f"kc{c}")
+def test_fused_velfg_feedbf_equals_stages(case, kchunk, monkeypatch):
+    """The whole-step kernel doing vel2, velfg and feedbf in one pass leaves
+    every array -- diu1..diu9 with all their ghost cells, f, g, h, the
+    feedback integrals, uspd, vspd -- bit-identical to the separate stage
+    kernels, at steps 1 (initial copy) to 3."""
+    from lesamd import api
+    s = _mk(case)
+    print(f"\n[fused velfg] {s.workload.name} kchunk={kchunk}", flush=True)
+    monkeypatch.setenv("LES_FUSE_VELFG", "0")
+    sep = api.Domain(s)
+    monkeypatch.setenv("LES_FUSE_VELFG", "1")
+    monkeypatch.setenv("LES_VELFG_KCHUNK", str(kchunk))
+    one = api.Domain(s)
+    for n in range(1, 4):
+        sep.step(n, 3)
+        one.step(n, 3)
+        for name in ALL_FIELDS:
+            assert np.array_equal(sep.download(name), one.download(name)), (n, name)
+    sep.close()
+    one.close()
