@@ -28,6 +28,8 @@
 // iteration l+1's red sweep adds the partials of iteration l and records
 // the result; later sweeps find the flag and exit.  Iteration l's values
 // stay intact because iteration l+1 writes the other buffers.
+#include <algorithm>
+
 #include "les_kernels.h"
 
 namespace les {
@@ -76,7 +78,9 @@ __global__ __launch_bounds__(NT) void k_rhs(Grid g, Metrics m, const float* __re
   double acc = 0.0;
   if (i <= g.im && j <= g.jm) {
     const float dxi = M1(m.dx1, i), dyj = M1(m.dy1, j);
-    for (int k = 1 + blockIdx.z; k <= g.km; k += gridDim.z) {
+    int k0, k1;
+    kspan(1, g.km, k0, k1);
+    for (int k = k0; k <= k1; k++) {
       const int c = ix(g, i, j, k);
       const float dzk = M1(m.dzn, k);
       const float fc = f[c], gc = gg[c];
@@ -125,7 +129,9 @@ __global__ __launch_bounds__(NT) void k_rhs_split(Grid g, SplitGrid s, float* __
   if (i > g.im || j > g.jm) return;
   const double av = ctrl->rhsav;
   const int t = (i - 1) >> 1;
-  for (int k = 1 + blockIdx.z; k <= g.km; k += gridDim.z) {
+  int k0, k1;
+  kspan(1, g.km, k0, k1);
+  for (int k = k0; k <= k1; k++) {
     const int c = ix(g, i, j, k);
     const int col = (i + j + k) & 1;  // 1 = red
     const float r = (float)((double)rhs[c] - av);
@@ -535,7 +541,9 @@ __global__ __launch_bounds__(NT) void k_pav(Grid g, SplitGrid s, Metrics m, Spli
   if (i <= g.im && j <= g.jm) {
     const float dxi = M1(m.dx1, i), dyj = M1(m.dy1, j);
     const int t = (i - 1) >> 1;
-    for (int k = 1 + blockIdx.z; k <= g.km; k += gridDim.z) {
+    int k0, k1;
+    kspan(1, g.km, k0, k1);
+    for (int k = k0; k <= k1; k++) {
       const float pv = (((i + j + k) & 1) ? red : blk)[cix(s, t, j, k)];
       acc += (double)(pv * dxi * dyj * M1(m.dzn, k));
     }
@@ -560,7 +568,9 @@ __global__ __launch_bounds__(NT) void k_padj(Grid g, SplitGrid s, SplitBufs sb, 
   const int ii = (i == 0) ? 1 : ((i == g.im + 1) ? g.im : i);
   const int jj = (j == 0) ? g.jm : ((j == g.jm + 1) ? 1 : j);
   const int t = (ii - 1) >> 1;
-  for (int k = blockIdx.z; k <= g.km + 1; k += gridDim.z) {
+  int k0, k1;
+  kspan(0, g.km + 1, k0, k1);
+  for (int k = k0; k <= k1; k++) {
     const int kk = (k == 0) ? 1 : ((k == g.km + 1) ? g.km : k);
     const float v = (((ii + jj + kk) & 1) ? red : blk)[cix(s, t, jj, kk)];
     p[ix(g, i, j, k)] = (float)((double)v - av);
@@ -578,7 +588,9 @@ static void launch_press_tail(const DevDomain& d, const SplitBufs& sb, int npart
                                       (const Ctrl*)d.ctrl));
   TIMED(d, KC_PAV, hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, d.stream, d.partials, nred, d.ctrl, 1,
                                       d.area));
-  TIMED(d, KC_PADJ, hipLaunchKernelGGL(k_padj, dim3((g.im + 2 + BX - 1) / BX, (g.jm + 2 + BY - 1) / BY, g.km + 2),
+  const int px = (g.im + 2 + BX - 1) / BX, py = (g.jm + 2 + BY - 1) / BY;
+  const int pz = g.km + 2;
+  TIMED(d, KC_PADJ, hipLaunchKernelGGL(k_padj, dim3(px, py, pz),
                                        blk(), 0, d.stream, g, s, sb, d.F.p, (const Ctrl*)d.ctrl));
 }
 
@@ -594,7 +606,8 @@ void launch_press(const DevDomain& d, float dt) {
                                       d.area));
   SplitBufs sb = d.sb;
   if (d.cn1_separable) sb.cn1[0] = sb.cn1[1] = nullptr;
-  TIMED(d, KC_RHS, hipLaunchKernelGGL(k_rhs_split, dim3(gx, gy, g.km), blk(), 0, d.stream, g, s, d.F.rhs,
+  const int sz = g.km;
+  TIMED(d, KC_RHS, hipLaunchKernelGGL(k_rhs_split, dim3(gx, gy, sz), blk(), 0, d.stream, g, s, d.F.rhs,
                                       (const float*)d.F.p, (const float*)d.F.cn1, sb, (const Ctrl*)d.ctrl));
   if (d.sor_fused) {
     const int ntt = (s.nt + FTX - 1) / FTX, ntj = (g.jm + FTY - 1) / FTY;

@@ -14,6 +14,11 @@ static inline dim3 block2d() { return dim3(BX, BY, 1); }
 static inline dim3 grid2d(int ni, int nj, int nk) {
   return dim3((unsigned)((ni + BX - 1) / BX), (unsigned)((nj + BY - 1) / BY), (unsigned)nk);
 }
+// blocks along k of an (i,j)-tiled stencil launch: one level per block
+// (runs of consecutive levels per thread measured slower here -- the
+// level-k+1 neighbours are found in L2/MALL either way, and one level per
+// block gives the most independent waves)
+static inline int kz(const Grid& g, int extra = 0) { return g.km + extra; }
 
 // ------------------------------------------------------------------ velnw
 // velnw.f95:27-59: u,v for k=1..km, w for k=1..km-1.
@@ -26,7 +31,9 @@ __global__ __launch_bounds__(NT) void k_velnw(Grid g, Metrics m, const float* __
   const int i = 1 + blockIdx.x * BX + threadIdx.x, j = 1 + blockIdx.y * BY + threadIdx.y;
   if (i > g.im || j > g.jm) return;
   const float rdx = M1(m.dxs, i), rdy = M1(m.dys, j);
-  for (int k = 1 + blockIdx.z; k <= g.km; k += gridDim.z) {
+  int k0, k1;
+  kspan(1, g.km, k0, k1);
+  for (int k = k0; k <= k1; k++) {
     const int c = ix(g, i, j, k);
     const float pc = p[c];
     float pz = (-pc + p[c + 1]) / ro / rdx;
@@ -42,7 +49,7 @@ __global__ __launch_bounds__(NT) void k_velnw(Grid g, Metrics m, const float* __
 
 void launch_velnw(const DevDomain& d, float ro, float dt) {
   const Grid& g = d.g;
-  TIMED(d, KC_VELNW, hipLaunchKernelGGL(k_velnw, grid2d(g.im, g.jm, g.km), block2d(), 0, d.stream, g, d.m, d.F.p,
+  TIMED(d, KC_VELNW, hipLaunchKernelGGL(k_velnw, grid2d(g.im, g.jm, kz(g)), block2d(), 0, d.stream, g, d.m, d.F.p,
                      d.F.f, d.F.g, d.F.h, d.F.u, d.F.v, d.F.w, ro, dt));
 }
 
@@ -190,7 +197,9 @@ __global__ __launch_bounds__(NT) void k_vel2(Grid g, Metrics m, const float* __r
   const float dxi = M1(m.dx1, i), dxp = M1(m.dx1, i + 1), dxm = M1(m.dx1, i - 1);
   const float dyj = M1(m.dy1, j), dyp = M1(m.dy1, j + 1), dym = M1(m.dy1, j - 1);
   const int sj = g.sj, sk = g.sk;
-  for (int k = 1 + blockIdx.z; k <= g.km + 1; k += gridDim.z) {
+  int k0, k1;
+  kspan(1, g.km + 1, k0, k1);
+  for (int k = k0; k <= k1; k++) {
     const int c = ix(g, i, j, k);
     if (k <= g.km) {
       float nou = (u[c - 1] + u[c]) / 2.0f;  // vel2.f95:77-86
@@ -317,7 +326,9 @@ __global__ __launch_bounds__(NT) void k_velfg(Grid g, Metrics m, P10 diu, P10 co
   const float dxi = M1(m.dx1, i), dxp = M1(m.dx1, i + 1);
   const float dyj = M1(m.dy1, j), dyp = M1(m.dy1, j + 1);
   const int sj = g.sj, sk = g.sk;
-  for (int k = 1 + blockIdx.z; k <= g.km; k += gridDim.z) {
+  int k0, k1;
+  kspan(1, g.km, k0, k1);
+  for (int k = k0; k <= k1; k++) {
     const int c = ix(g, i, j, k);
     const float dzk = M1(m.dzn, k);
     {
@@ -364,13 +375,13 @@ static P10 p10(float* const* a) {
 void launch_velfg(const DevDomain& d, float vn) {
   const Grid& g = d.g;
   const P10 diu = p10(d.F.diu), cov = p10(d.F.cov);
-  TIMED(d, KC_VEL2, hipLaunchKernelGGL(k_vel2, grid2d(g.im, g.jm, g.km + 1), block2d(), 0, d.stream, g, d.m, d.F.u,
+  TIMED(d, KC_VEL2, hipLaunchKernelGGL(k_vel2, grid2d(g.im, g.jm, kz(g, 1)), block2d(), 0, d.stream, g, d.m, d.F.u,
                      d.F.v, d.F.w, diu, cov, d.F.uspd, d.F.vspd, d.wl));
   const int n1 = g.jm * g.km + g.im * g.km;
   TIMED(d, KC_VEL2_GHOST, hipLaunchKernelGGL(k_vel2_ghost1, dim3((n1 + 255) / 256), dim3(256), 0, d.stream, g, diu, cov));
   const int n2 = (g.jm + 1) * (g.km + 1) + (g.im + 1) * (g.km + 1);
   TIMED(d, KC_VEL2_GHOST, hipLaunchKernelGGL(k_vel2_ghost2, dim3((n2 + 255) / 256), dim3(256), 0, d.stream, g, diu));
-  TIMED(d, KC_VELFG, hipLaunchKernelGGL(k_velfg, grid2d(g.im, g.jm, g.km), block2d(), 0, d.stream, g, d.m, diu, cov,
+  TIMED(d, KC_VELFG, hipLaunchKernelGGL(k_velfg, grid2d(g.im, g.jm, kz(g)), block2d(), 0, d.stream, g, d.m, diu, cov,
                      d.F.f, d.F.g, d.F.h, vn));
 }
 
@@ -675,7 +686,9 @@ __global__ __launch_bounds__(NT) void k_feedbf(Grid g, const float* __restrict__
                                                float beta) {
   const int i = 1 + blockIdx.x * BX + threadIdx.x, j = 1 + blockIdx.y * BY + threadIdx.y;
   if (i > g.im || j > g.jm) return;
-  for (int k = 1 + blockIdx.z; k <= g.km; k += gridDim.z) {
+  int k0, k1;
+  kspan(1, g.km, k0, k1);
+  for (int k = k0; k <= k1; k++) {
     const int c = ix(g, i, j, k);
     const float uc = u[c], vc = v[c], wc = w[c], b = bm[c], cc = cm[c], dd = dm[c];
     const float su = (us[c] + uc) * b, sv = (vs[c] + vc) * cc, sw = (ws[c] + wc) * dd;
@@ -693,7 +706,7 @@ __global__ __launch_bounds__(NT) void k_feedbf(Grid g, const float* __restrict__
 
 void launch_feedbf(const DevDomain& d, float alpha, float dt, float beta) {
   const Grid& g = d.g;
-  TIMED(d, KC_FEEDBF, hipLaunchKernelGGL(k_feedbf, grid2d(g.im, g.jm, g.km), block2d(), 0, d.stream, g, d.F.u, d.F.v,
+  TIMED(d, KC_FEEDBF, hipLaunchKernelGGL(k_feedbf, grid2d(g.im, g.jm, kz(g)), block2d(), 0, d.stream, g, d.F.u, d.F.v,
                      d.F.w, d.F.bmask, d.F.cmask, d.F.dmask, d.F.usum, d.F.vsum, d.F.wsum, d.F.f,
                      d.F.g, d.F.h, alpha, dt, beta));
 }
@@ -711,7 +724,9 @@ __global__ __launch_bounds__(NT) void k_les_sm(Grid g, P10 diu, const float* __r
   const float* d6 = diu.a[6];
   const float* d7 = diu.a[7];
   const float* d8 = diu.a[8];
-  for (int k = 1 + blockIdx.z; k <= g.km; k += gridDim.z) {
+  int k0, k1;
+  kspan(1, g.km, k0, k1);
+  for (int k = k0; k <= k1; k++) {
     const int c = ix(g, i, j, k);
     const float dudxx1 = diu.a[1][c];
     const float dudyx1 = (d2[c - 1] + d2[c - 1 + sj] + d2[c] + d2[c + sj]) * 0.25f;
@@ -786,12 +801,16 @@ __global__ __launch_bounds__(256) void k_boundsm(Grid g, float* __restrict__ sm)
 
 // Viscous terms added to f (k=2..km, wall k=1), g (same) and h (k=1..km),
 // les.f95:83-242.
+// ADAM: the whole-step path also applies adam (adam.f95:30-44) to the new
+// f,g,h, which is what k_adam after this kernel computes.
+template <bool ADAM>
 __global__ __launch_bounds__(NT) void k_les_visc(Grid g, Metrics m, P10 diu,
                                                  const float* __restrict__ sm,
                                                  const float* __restrict__ uspd,
                                                  const float* __restrict__ vspd,
                                                  float* __restrict__ f, float* __restrict__ gg,
-                                                 float* __restrict__ h, float wl) {
+                                                 float* __restrict__ h, float* __restrict__ fo,
+                                                 float* __restrict__ go, float* __restrict__ ho, float wl) {
   const int i = 1 + blockIdx.x * BX + threadIdx.x, j = 1 + blockIdx.y * BY + threadIdx.y;
   if (i > g.im || j > g.jm) return;
   const int sj = g.sj, sk = g.sk;
@@ -811,9 +830,12 @@ __global__ __launch_bounds__(NT) void k_les_visc(Grid g, Metrics m, P10 diu,
 #define SMXM(c) ((dxi * sm[(c) - 1] + dxm * sm[(c)]) / (dxm + dxi))     // x-interp at i-1/2
 #define SMY(c) ((dyp * sm[(c)] + dyj * sm[(c) + sj]) / (dyj + dyp))     // y-interp at j+1/2
 #define SMYM(c) ((dyj * sm[(c) - sj] + dym * sm[(c)]) / (dym + dyj))    // y-interp at j-1/2
-  for (int k = 1 + blockIdx.z; k <= g.km; k += gridDim.z) {
+  int k0, k1;
+  kspan(1, g.km, k0, k1);
+  for (int k = k0; k <= k1; k++) {
     const int c = ix(g, i, j, k);
     const float dzk = M1(m.dzn, k), dzp = M1(m.dzn, k + 1), dzm = M1(m.dzn, k - 1);
+    float fv, gv, hv;
     // ---- f (les.f95:83-114; wall les.f95:118-139)
     {
       const float evsx2 = sm[c + 1];
@@ -837,7 +859,7 @@ __global__ __launch_bounds__(NT) void k_les_visc(Grid g, Metrics m, P10 diu,
         vfu = (visux2 - visux1) / dxs + (visuy2 - visuy1) / dyj + (visuz2 - visuz1) / dzk +
               (visuy2 - visuy1) / dyj + (visuz2 - visuz1) / dzk;
       }
-      f[c] = (f[c] + vfu);
+      fv = (f[c] + vfu);
     }
     // ---- g (les.f95:143-174; wall les.f95:178-201)
     {
@@ -860,7 +882,7 @@ __global__ __launch_bounds__(NT) void k_les_visc(Grid g, Metrics m, P10 diu,
         visvz1 = t * t * vspd[(i + 1) + (j + 1) * sj];
       }
       const float vfv = (visvx2 - visvx1) / dxi + (visvy2 - visvy1) / dys + (visvz2 - visvz1) / dzk;
-      gg[c] = (gg[c] + vfv);
+      gv = (gg[c] + vfv);
     }
     // ---- h (les.f95:206-242)
     {
@@ -877,7 +899,19 @@ __global__ __launch_bounds__(NT) void k_les_visc(Grid g, Metrics m, P10 diu,
       const float viswz2 = (evsz2) * 2.0f * d9[c + sk];
       const float viswz1 = (evsz1) * 2.0f * d9[c];
       const float vfw = (viswx2 - viswx1) / dxi + (viswy2 - viswy1) / dyj + (viswz2 - viswz1) / dzk;
-      h[c] = (h[c] + vfw);
+      hv = (h[c] + vfw);
+    }
+    if (ADAM) {
+      f[c] = 1.5f * fv - 0.5f * fo[c];
+      gg[c] = 1.5f * gv - 0.5f * go[c];
+      h[c] = 1.5f * hv - 0.5f * ho[c];
+      fo[c] = fv;
+      go[c] = gv;
+      ho[c] = hv;
+    } else {
+      f[c] = fv;
+      gg[c] = gv;
+      h[c] = hv;
     }
   }
 #undef SMX
@@ -889,13 +923,27 @@ __global__ __launch_bounds__(NT) void k_les_visc(Grid g, Metrics m, P10 diu,
 void launch_les(const DevDomain& d) {
   const Grid& g = d.g;
   const P10 diu = p10(d.F.diu);
-  TIMED(d, KC_LES_SM, hipLaunchKernelGGL(k_les_sm, grid2d(g.im, g.jm, g.km), block2d(), 0, d.stream, g, diu,
+  TIMED(d, KC_LES_SM, hipLaunchKernelGGL(k_les_sm, grid2d(g.im, g.jm, kz(g)), block2d(), 0, d.stream, g, diu,
                      d.m.delx1 + 1, d.F.sm));
   const int ni = g.im + 2, nj = g.jm + 3;
   const int nb = 2 * ni * nj + 3 * ni * g.km + 2 * g.jm * g.km;
   TIMED(d, KC_BOUNDSM, hipLaunchKernelGGL(k_boundsm, dim3((nb + 255) / 256), dim3(256), 0, d.stream, g, d.F.sm));
-  TIMED(d, KC_LES_VISC, hipLaunchKernelGGL(k_les_visc, grid2d(g.im, g.jm, g.km), block2d(), 0, d.stream, g, d.m, diu,
-                     d.F.sm, d.F.uspd, d.F.vspd, d.F.f, d.F.g, d.F.h, d.wl));
+  TIMED(d, KC_LES_VISC, hipLaunchKernelGGL(k_les_visc<false>, grid2d(g.im, g.jm, kz(g)), block2d(), 0, d.stream, g,
+                     d.m, diu, d.F.sm, d.F.uspd, d.F.vspd, d.F.f, d.F.g, d.F.h, nullptr, nullptr, nullptr, d.wl));
+}
+
+// les followed by adam (whole-step path)
+void launch_les_adam(const DevDomain& d) {
+  const Grid& g = d.g;
+  const P10 diu = p10(d.F.diu);
+  TIMED(d, KC_LES_SM, hipLaunchKernelGGL(k_les_sm, grid2d(g.im, g.jm, kz(g)), block2d(), 0, d.stream, g, diu,
+                     d.m.delx1 + 1, d.F.sm));
+  const int ni = g.im + 2, nj = g.jm + 3;
+  const int nb = 2 * ni * nj + 3 * ni * g.km + 2 * g.jm * g.km;
+  TIMED(d, KC_BOUNDSM, hipLaunchKernelGGL(k_boundsm, dim3((nb + 255) / 256), dim3(256), 0, d.stream, g, d.F.sm));
+  TIMED(d, KC_LES_VISC, hipLaunchKernelGGL(k_les_visc<true>, grid2d(g.im, g.jm, kz(g)), block2d(), 0, d.stream, g,
+                     d.m, diu, d.F.sm, d.F.uspd, d.F.vspd, d.F.f, d.F.g, d.F.h, d.F.fold, d.F.gold, d.F.hold,
+                     d.wl));
 }
 
 // ------------------------------------------------------------------- adam
@@ -905,7 +953,9 @@ __global__ __launch_bounds__(NT) void k_adam(Grid g, float* __restrict__ f, floa
                                              float* __restrict__ go, float* __restrict__ ho) {
   const int i = 1 + blockIdx.x * BX + threadIdx.x, j = 1 + blockIdx.y * BY + threadIdx.y;
   if (i > g.im || j > g.jm) return;
-  for (int k = 1 + blockIdx.z; k <= g.km; k += gridDim.z) {
+  int k0, k1;
+  kspan(1, g.km, k0, k1);
+  for (int k = k0; k <= k1; k++) {
     const int c = ix(g, i, j, k);
     const float fd = f[c], gd = gg[c], hd = h[c];
     f[c] = 1.5f * fd - 0.5f * fo[c];
@@ -919,7 +969,7 @@ __global__ __launch_bounds__(NT) void k_adam(Grid g, float* __restrict__ f, floa
 
 void launch_adam(const DevDomain& d) {
   const Grid& g = d.g;
-  TIMED(d, KC_ADAM, hipLaunchKernelGGL(k_adam, grid2d(g.im, g.jm, g.km), block2d(), 0, d.stream, g, d.F.f, d.F.g,
+  TIMED(d, KC_ADAM, hipLaunchKernelGGL(k_adam, grid2d(g.im, g.jm, kz(g)), block2d(), 0, d.stream, g, d.F.f, d.F.g,
                      d.F.h, d.F.fold, d.F.gold, d.F.hold));
 }
 

@@ -64,6 +64,17 @@ struct Ctrl {
 
 constexpr int kWave = 64;
 
+// The levels lo..hi are split into gridDim.z contiguous runs; block z gets
+// [k0, k1] (empty when k0 > k1).  A thread walking consecutive levels finds
+// the level-(k-1) and level-(k+1) neighbours of level k in its own CU's
+// caches.
+__device__ __forceinline__ void kspan(int lo, int hi, int& k0, int& k1) {
+  const int n = hi - lo + 1;
+  const int c = (n + (int)gridDim.z - 1) / (int)gridDim.z;
+  k0 = lo + (int)blockIdx.z * c;
+  k1 = (k0 + c - 1 < hi) ? k0 + c - 1 : hi;
+}
+
 // Deterministic sum over a block of NT threads (tid = linear thread id):
 // wavefront butterfly, then wave totals through LDS added by thread 0 in
 // wave order.  The result is valid in thread 0.

@@ -61,6 +61,7 @@ struct DevDomain {
   int sor_fused;     // one launch per SOR iteration (k_sor_iter) instead of one per colour
   int fuse_velfg;    // whole steps run vel2+velfg+feedbf as one kernel
   int velfg_kchunk;  // k levels per thread of that kernel (0: automatic)
+  int fuse_les;      // whole steps apply adam inside the les viscous kernel
   int sor_fused_blocks;  // target number of blocks of one fused SOR iteration (0: automatic)
   int ncu;           // compute units of the device
 };
@@ -86,6 +87,7 @@ void launch_bondv1(const DevDomain& d, int n, float dt);
 void launch_velfg(const DevDomain& d, float vn);
 void launch_feedbf(const DevDomain& d, float alpha, float dt, float beta);
 void launch_velfg_feedbf(const DevDomain& d, float vn, float alpha, float dt, float beta);
+void launch_les_adam(const DevDomain& d);
 void launch_les(const DevDomain& d);
 void launch_adam(const DevDomain& d);
 void launch_press(const DevDomain& d, float dt);

@@ -445,6 +445,7 @@ int domain_alloc(int im, int jm, int km, les_domain** out) {
   D->d.sor_fused = env_int("LES_SOR_FUSED", 1);
   D->d.fuse_velfg = env_int("LES_FUSE_VELFG", 1);
   D->d.velfg_kchunk = env_int("LES_VELFG_KCHUNK", 0);
+  D->d.fuse_les = env_int("LES_FUSE_LES", 1);
   D->d.sor_fused_blocks = env_int("LES_SOR_FUSED_BLOCKS", 0);
   {
     int dev = 0, ncu = 0;
@@ -499,7 +500,7 @@ int domain_alloc(int im, int jm, int km, les_domain** out) {
 int run_stage(les_domain* D, int s, int n, int nmax);
 
 // One whole step n (main.f95:260-296); with fuse_velfg the velfg and feedbf
-// stages run as one kernel.
+// stages run as one kernel, and with fuse_les adam is applied inside les.
 int run_step(les_domain* D, int n, int nmax) {
   for (int s = 0; s < 7; s++) {
     if (s == 2 && D->d.fuse_velfg) {
@@ -507,6 +508,13 @@ int run_step(les_domain* D, int n, int nmax) {
       launch_velfg_feedbf(D->d, D->vn, D->alpha, D->dt, D->beta);
       HIP_TRY(hipGetLastError());
       s = 3;
+      continue;
+    }
+    if (s == 4 && D->d.fuse_les) {
+      if (flush_metrics(D)) return 1;
+      launch_les_adam(D->d);
+      HIP_TRY(hipGetLastError());
+      s = 5;
       continue;
     }
     if (run_stage(D, s, n, nmax)) return 1;

@@ -190,8 +190,9 @@ def test_fused_sor_iteration_equals_colour_sweeps(case, blocks, monkeypatch):
 FUSED_STEP_CASES = [
     (dict(im=24, jm=20, km=16, seed=13), 0),     # automatic k chunks
     (dict(im=23, jm=19, km=13, seed=14), 4),     # odd extents, chunks of 4 levels
-    (dict(im=70, jm=9, km=11, seed=15), 11),     # one chunk per column, two i blocks
+    (dict(im=70, jm=9, km=11, seed=15), 11),     # two blocks in i, one chunk per column
     (dict(im=13, jm=2, km=6, seed=16), 1),       # jm = 2, one level per thread
+    (dict(im=130, jm=31, km=9, seed=17), 3),     # three blocks in i
 ]
 ALL_FIELDS = ["u", "v", "w", "p", "f", "g", "h", "usum", "vsum", "wsum", "fold", "gold", "hold", "rhs", "sm",
               "uspd", "vspd"] + [f"diu{q}" for q in range(1, 10)]
@@ -199,17 +200,20 @@ ALL_FIELDS = ["u", "v", "w", "p", "f", "g", "h", "usum", "vsum", "wsum", "fold",
 
 @pytest.mark.parametrize("case,kchunk", FUSED_STEP_CASES,
                          ids=lambda c: f"{c['im']}x{c['jm']}x{c['km']}" if isinstance(c, dict) else f"kc{c}")
-def test_fused_velfg_feedbf_equals_stages(case, kchunk, monkeypatch):
-    """The whole-step kernel doing vel2, velfg and feedbf in one pass leaves
-    every array -- diu1..diu9 with all their ghost cells, f, g, h, the
-    feedback integrals, uspd, vspd -- bit-identical to the separate stage
-    kernels, at steps 1 (initial copy) to 3."""
+def test_fused_step_kernels_equal_stages(case, kchunk, monkeypatch):
+    """The whole-step kernels -- vel2+velfg+feedbf in one pass, adam inside
+    the les viscous kernel -- leave every array (diu1..diu9 and sm with all their ghost
+    cells, f, g, h and their previous values, the feedback integrals, uspd,
+    vspd, p) bit-identical to the separate stage kernels, at steps 1
+    (initial copy) to 3, for any split of k among threads and blocks."""
     from lesamd import api
     s = _mk(case)
-    print(f"\n[fused velfg] {s.workload.name} kchunk={kchunk}", flush=True)
+    print(f"\n[fused step kernels] {s.workload.name} kchunk={kchunk}", flush=True)
     monkeypatch.setenv("LES_FUSE_VELFG", "0")
+    monkeypatch.setenv("LES_FUSE_LES", "0")
     sep = api.Domain(s)
     monkeypatch.setenv("LES_FUSE_VELFG", "1")
+    monkeypatch.setenv("LES_FUSE_LES", "1")
     monkeypatch.setenv("LES_VELFG_KCHUNK", str(kchunk))
     one = api.Domain(s)
     for n in range(1, 4):
