@@ -75,13 +75,26 @@ def cpu_baseline(setup, state, n0, nsteps, budget_s):
         what = "C restatement (gcc -O2, FP32)"
     be.load(state)
     times = []
-    t_all = time.perf_counter()
-    for n in range(n0, n0 + nsteps):
-        t = time.perf_counter()
-        be.step(n, 10 ** 9)
-        times.append(time.perf_counter() - t)
-        if time.perf_counter() - t_all > budget_s:
-            break
+    # the reference prints progress lines on stdout: keep them out of the
+    # one-line JSON output
+    sys.stdout.flush()
+    saved = os.dup(1)
+    devnull = os.open(os.devnull, os.O_WRONLY)
+    os.dup2(devnull, 1)
+    try:
+        t_all = time.perf_counter()
+        for n in range(n0, n0 + nsteps):
+            t = time.perf_counter()
+            be.step(n, 10 ** 9)
+            times.append(time.perf_counter() - t)
+            if time.perf_counter() - t_all > budget_s:
+                break
+        if hasattr(be, "flush"):
+            be.flush()
+    finally:
+        os.dup2(saved, 1)
+        os.close(saved)
+        os.close(devnull)
     sps = len(times) / sum(times)
     return {
         "value": round(sps, 4),
@@ -109,6 +122,28 @@ def profiled_traffic(kernel_class: str, workload: str):
         return None
     m = re.search(r"^traffic_bytes_per_launch:\s*(\d+)", f.read_text(), flags=re.M)
     return int(m.group(1)) if m else None
+
+
+def serve_measurement(nreq: int, nsteps: int) -> dict:
+    """Mixed-size requests (nx, ny in 96..512, nz in 64..128, seeded urban
+    set-ups) all arriving at once: served concurrently (one stream and step
+    graph per domain, admission by device memory) and, for comparison, one
+    after another.  Reports aggregate cell updates per second of both and the
+    time-to-result percentiles of the concurrent run."""
+    from lesamd import serving
+    reqs = serving.sample_requests(nreq, nsteps, seed=2024)
+    srv = serving.Server(budget_bytes=200 << 30, burst=4)
+    print(f"[bench] serving {nreq} requests x {nsteps} steps", file=sys.stderr, flush=True)
+    seq = serving.summarize(srv.run(reqs, sequential=True))
+    con = serving.summarize(srv.run(reqs))
+    return {
+        "requests": nreq, "steps_each": nsteps, "arrivals": "all at t=0",
+        "mcell_updates_per_s": round(con["mcell_updates_per_s"], 1),
+        "sequential_mcell_updates_per_s": round(seq["mcell_updates_per_s"], 1),
+        "speedup_vs_sequential": round(con["mcell_updates_per_s"] / seq["mcell_updates_per_s"], 3),
+        "p50_s": round(con["p50_s"], 4), "p99_s": round(con["p99_s"], 4),
+        "sequential_p50_s": round(seq["p50_s"], 4), "sequential_p99_s": round(seq["p99_s"], 4),
+    }
 
 
 def copy_calibration(mib: int) -> int:
@@ -141,6 +176,9 @@ def main(argv=None):
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="CPU-baseline time budget [s]")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--eager", action="store_true", help="launch kernels one by one (no graph replay)")
+    ap.add_argument("--serve-requests", type=int, default=24,
+                    help="mixed-size requests of the serving measurement (0: skip it)")
+    ap.add_argument("--serve-steps", type=int, default=40, help="time steps per served request")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run device-to-device copies of this size (HBM counter calibration)")
     args = ap.parse_args(argv)
@@ -246,6 +284,8 @@ def main(argv=None):
         "kernels": per_kernel,
     }
     dom.close()
+    if args.serve_requests > 0:
+        result["served"] = serve_measurement(args.serve_requests, args.serve_steps)
     if snapshot is not None:
         result["cpu_baseline"] = cpu_baseline(setup, snapshot, W + 1, args.cpu_steps, args.cpu_budget)
     print(json.dumps(result), flush=True)

@@ -129,7 +129,14 @@ typedef struct {
    * cn2l,cn2s(1:im) cn3l,cn3s(1:jm) cn4l,cn4s(1:km) */
   const float *dx1, *dy1, *dzn, *dzs, *dxs, *dys;
   const float *cn2l, *cn2s, *cn3l, *cn3s, *cn4l, *cn4s;
-  const float* cn1;      /* cn1(1:im,1:jm,1:km) */
+  /* cn1(1:im,1:jm,1:km); NULL selects the separable definition of
+   * init.f95:100-107, 1/(2/(dxs dxs) + 2/(dys dys) + 2/(dz1 dz2)), which the
+   * SOR evaluates on the fly (nothing is uploaded). */
+  const float* cn1;
+  /* Masks in reference layout, or all three NULL together with zbm and z2:
+   * the masks are then built on the device by the rule of
+   * feedbfm.f95:87-116 (1 inside buildings, zbm(i,j) > z2(k)+0.5*dzn(k),
+   * 0 elsewhere and on every ghost cell). */
   const float* bmask1;   /* (-1:im+1,0:jm+1,0:km+1) */
   const float* cmask1;   /* (0:im+1,-1:jm+1,0:km+1) */
   const float* dmask1;   /* (0:im+1,0:jm+1,0:km+1) */
@@ -137,6 +144,8 @@ typedef struct {
   float ro, vn, alpha, beta, dt;   /* set.f95:89-95 */
   int nmaxp;
   float pjuge, omega;
+  const float* zbm;      /* building heights zbm(1:im,1:jm), i fastest (feedbfm.f95:52-77) */
+  const float* z2;       /* z2(0:km+2) (grid.f95:154-160) */
 } les_domain_desc;
 
 int les_domain_create(const les_domain_desc* desc, les_domain** out);
@@ -157,6 +166,12 @@ int les_domain_stage(les_domain* d, int stage, int n, int nmax);
 int les_domain_upload(les_domain* d, const char* field, const float* host);
 int les_domain_download(les_domain* d, const char* field, float* host);
 int les_domain_synchronize(les_domain* d);
+/* 1 when every call issued on the domain has completed, 0 while work is
+ * pending, -1 on error (does not wait).  No reference counterpart: used by
+ * hosts that serve several domains at once. */
+int les_domain_query(les_domain* d);
+/* Bytes of device memory a domain of these extents would hold. */
+long long les_domain_bytes_needed(int im, int jm, int km);
 int les_domain_press_stats(les_domain* d, int* iters, double* sor_hist, int hist_len, double* rhsav,
                            double* pav, int* nonfinite);
 /* Bytes of device memory the domain holds. */

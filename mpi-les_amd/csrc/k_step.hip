@@ -973,6 +973,29 @@ void launch_adam(const DevDomain& d) {
                      d.F.h, d.F.fold, d.F.gold, d.F.hold));
 }
 
+// ---------------------------------------------------------- building masks
+// feedbfm.f95:87-116 on the device: amask1=0 where zbm(i,j) > z2(k)+0.5*dzn(k)
+// (top[k-1] holds that sum, evaluated in FP32 on the host), and
+// b/c/dmask1 = 1 exactly there; ghost cells stay 0.
+__global__ __launch_bounds__(NT) void k_masks(Grid g, const float* __restrict__ zbm, const float* __restrict__ top,
+                                              float* __restrict__ bm, float* __restrict__ cm,
+                                              float* __restrict__ dm) {
+  const int i = 1 + blockIdx.x * BX + threadIdx.x, j = 1 + blockIdx.y * BY + threadIdx.y;
+  const int k = 1 + blockIdx.z;
+  if (i > g.im || j > g.jm) return;
+  const float v = (zbm[(i - 1) + (j - 1) * g.im] > top[k - 1]) ? 1.0f : 0.0f;
+  const int c = ix(g, i, j, k);
+  bm[c] = v;
+  cm[c] = v;
+  dm[c] = v;
+}
+
+void launch_masks(const DevDomain& d, const float* zbm, const float* top) {
+  const Grid& g = d.g;
+  hipLaunchKernelGGL(k_masks, grid2d(g.im, g.jm, g.km), block2d(), 0, d.stream, g, zbm, top, d.F.bmask, d.F.cmask,
+                     d.F.dmask);
+}
+
 // ------------------------------------------------------ layout conversion
 __global__ __launch_bounds__(256) void k_unpack(Grid g, const float* __restrict__ src,
                                                 float* __restrict__ dst, int li, int lj, int lk,

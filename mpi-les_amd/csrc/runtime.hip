@@ -385,13 +385,9 @@ int domain_alloc(int im, int jm, int km, les_domain** out) {
     if (e != hipSuccess) return bail(std::string("field allocation: ") + hipGetErrorString(e));
     D->bytes += nb;
   }
-  for (int q = 1; q <= 9; q++) {
-    const size_t nb = (size_t)n3 * sizeof(float);
-    e = hipMalloc(&D->cov[q], nb);
-    if (e == hipSuccess) e = hipMemset(D->cov[q], 0, nb);
-    if (e != hipSuccess) return bail(std::string("scratch allocation: ") + hipGetErrorString(e));
-    D->bytes += nb;
-  }
+  // cov1..cov9 are scratch of the stage-by-stage velfg only; they are
+  // allocated on its first use (ensure_cov), the whole-step path never
+  // stores them
   {
     // colour-split SOR arrays: 4 iterate buffers, 2 rhs, 2 cn1
     SplitGrid& sg = D->d.sg;
@@ -499,6 +495,26 @@ int domain_alloc(int im, int jm, int km, les_domain** out) {
 
 int run_stage(les_domain* D, int s, int n, int nmax);
 
+// The products cov1..cov9 of the stage kernels of velfg, allocated once.
+int ensure_cov(les_domain* D) {
+  if (D->cov[1]) return 0;
+  const size_t nb = (size_t)D->n3 * sizeof(float);
+  for (int q = 1; q <= 9; q++) {
+    hipError_t e = hipMalloc(&D->cov[q], nb);
+    if (e == hipSuccess) e = hipMemset(D->cov[q], 0, nb);
+    if (e != hipSuccess) {
+      for (int r = 1; r <= q; r++) {
+        if (D->cov[r]) hipFree(D->cov[r]);
+        D->cov[r] = nullptr;
+      }
+      return fail(std::string("scratch allocation: ") + hipGetErrorString(e));
+    }
+    D->d.F.cov[q] = D->cov[q];
+    D->bytes += nb;
+  }
+  return 0;
+}
+
 // One whole step n (main.f95:260-296); with fuse_velfg the velfg and feedbf
 // stages run as one kernel, and with fuse_les adam is applied inside les.
 int run_step(les_domain* D, int n, int nmax) {
@@ -528,7 +544,10 @@ int run_stage(les_domain* D, int s, int n, int nmax) {
   switch (s) {
     case 0: launch_velnw(D->d, D->ro, D->dt); break;
     case 1: launch_bondv1(D->d, n, D->dt); break;
-    case 2: launch_velfg(D->d, D->vn); break;
+    case 2:
+      if (ensure_cov(D)) return 1;
+      launch_velfg(D->d, D->vn);
+      break;
     case 3: launch_feedbf(D->d, D->alpha, D->dt, D->beta); break;
     case 4: launch_les(D->d); break;
     case 5: launch_adam(D->d); break;
@@ -818,6 +837,28 @@ int les_release_all(void) {
 }
 
 // ------------------------------------------------------------ domain API
+// Masks from the height map on the device (feedbfm.f95:87-116); the level
+// tops z2(k)+0.5*dzn(k) are summed here in FP32 as the reference does.
+int build_masks(les_domain* D, const float* zbm, const float* z2, const float* dzn) {
+  const Grid& g = D->d.g;
+  std::vector<float> top(g.km);
+  for (int k = 1; k <= g.km; k++) top[k - 1] = z2[k] + 0.5f * dzn[k + 1];  // z2(0:), dzn(-1:)
+  float* buf = nullptr;
+  const size_t nz = (size_t)g.im * g.jm;
+  HIP_TRY(hipMalloc(&buf, (nz + g.km) * sizeof(float)));
+  hipError_t e = hipMemcpyAsync(buf, zbm, nz * sizeof(float), hipMemcpyHostToDevice, D->d.stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(buf + nz, top.data(), g.km * sizeof(float), hipMemcpyHostToDevice, D->d.stream);
+  if (e == hipSuccess) {
+    launch_masks(D->d, buf, buf + nz);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(D->d.stream);
+  hipFree(buf);
+  if (e != hipSuccess) return fail(std::string("mask generation: ") + hipGetErrorString(e));
+  return 0;
+}
+
 int les_domain_create(const les_domain_desc* s, les_domain** out) {
   if (!s || !out) return fail("null argument");
   les_domain* D = nullptr;
@@ -835,12 +876,22 @@ int les_domain_create(const les_domain_desc* s, les_domain** out) {
   set_metric(D, MCN4L, s->cn4l);
   set_metric(D, MCN4S, s->cn4s);
   int rc = 0;
-  if (s->cn1) rc |= upload_field(D, FCN1, s->cn1);
+  if (s->cn1) {
+    rc |= upload_field(D, FCN1, s->cn1);
+  } else if (env_int("LES_SOR_CN1_LOAD", 0)) {
+    rc |= fail("LES_SOR_CN1_LOAD needs cn1 in the domain description");
+  } else {
+    derive(D);
+    D->d.cn1_separable = 1;  // init.f95:100-107 by definition
+  }
+  const bool gen_masks = !s->bmask1 && !s->cmask1 && !s->dmask1;
+  if (gen_masks && !(s->zbm && s->z2)) rc |= fail("masks: give bmask1/cmask1/dmask1, or zbm and z2");
   if (s->bmask1) rc |= upload_field(D, FBMASK, s->bmask1);
   if (s->cmask1) rc |= upload_field(D, FCMASK, s->cmask1);
   if (s->dmask1) rc |= upload_field(D, FDMASK, s->dmask1);
   rc |= set_inflow(D, s->inflow);
   rc |= flush_metrics(D);
+  if (!rc && gen_masks) rc |= build_masks(D, s->zbm, s->z2, s->dzn);
   if (rc) {
     std::string e = g_err;
     domain_free(D);
@@ -860,6 +911,28 @@ int les_domain_create(const les_domain_desc* s, les_domain** out) {
   D->d.omega = s->omega;
   *out = D;
   return 0;
+}
+
+int les_domain_query(les_domain* D) {
+  if (!D) {
+    fail("null domain");
+    return -1;
+  }
+  const hipError_t e = hipStreamQuery(D->d.stream);
+  if (e == hipSuccess) return 1;
+  if (e == hipErrorNotReady) return 0;
+  fail(std::string("hipStreamQuery: ") + hipGetErrorString(e));
+  return -1;
+}
+
+long long les_domain_bytes_needed(int im, int jm, int km) {
+  if (im < 2 || jm < 2 || km < 2) return -1;
+  const long long sj = ((long long)im + 4 + 63) / 64 * 64, sk = sj * (jm + 4), n3 = sk * (km + 4);
+  const long long nt = (im + 1) / 2, cp = (nt + 31) / 32 * 32, cs = cp * jm * km;
+  long long b = 0;
+  for (int f = 0; f < NFIELDS; f++) b += (kFields[f].kind == K2D ? sk : n3) * 4;
+  b += 8 * cs * 4;   // colour-split SOR arrays
+  return b + (1LL << 20);  // metrics, partial sums, control block (bounded by 1 MiB here)
 }
 
 int les_domain_destroy(les_domain* D) {
@@ -887,6 +960,8 @@ static int step_graph(les_domain* D, int n, int nmax) {
     D->gexec = nullptr;
   }
   if (!D->gexec) {
+    // allocations cannot be captured: make the stage scratch exist first
+    if (!D->d.fuse_velfg && ensure_cov(D)) return 1;
     HIP_TRY(hipStreamBeginCapture(D->d.stream, hipStreamCaptureModeThreadLocal));
     int rc = 0;
     rc = run_step(D, n, nmax);

@@ -54,7 +54,8 @@ class DomainDesc(ctypes.Structure):
         (n, ctypes.c_void_p) for n in ("dx1", "dy1", "dzn", "dzs", "dxs", "dys", "cn2l", "cn2s", "cn3l",
                                        "cn3s", "cn4l", "cn4s", "cn1", "bmask1", "cmask1", "dmask1", "inflow")
     ] + [(n, ctypes.c_float) for n in ("ro", "vn", "alpha", "beta", "dt")] + [
-        ("nmaxp", ctypes.c_int), ("pjuge", ctypes.c_float), ("omega", ctypes.c_float)]
+        ("nmaxp", ctypes.c_int), ("pjuge", ctypes.c_float), ("omega", ctypes.c_float),
+        ("zbm", ctypes.c_void_p), ("z2", ctypes.c_void_p)]
 
 
 def lib():
@@ -85,6 +86,9 @@ def lib():
     L.les_domain_upload.argtypes = [vp, ctypes.c_char_p, vp]
     L.les_domain_download.argtypes = [vp, ctypes.c_char_p, vp]
     L.les_domain_synchronize.argtypes = [vp]
+    L.les_domain_query.argtypes = [vp]
+    L.les_domain_bytes_needed.argtypes = [ci, ci, ci]
+    L.les_domain_bytes_needed.restype = ctypes.c_longlong
     L.les_domain_press_stats.argtypes = [vp, vp, vp, ci, vp, vp, vp]
     L.les_domain_bytes.argtypes = [vp]
     L.les_domain_bytes.restype = ctypes.c_longlong
@@ -187,10 +191,16 @@ class Domain:
         d.im, d.jm, d.km = w.im, w.jm, w.km
         for n in ("dx1", "dy1", "dzn", "dzs", "dxs", "dys"):
             setattr(d, n, arr(m[n]))
-        for n in ("cn2l", "cn2s", "cn3l", "cn3s", "cn4l", "cn4s", "cn1"):
+        for n in ("cn2l", "cn2s", "cn3l", "cn3s", "cn4l", "cn4s"):
             setattr(d, n, arr(c[n]))
-        for n in ("bmask1", "cmask1", "dmask1"):
-            setattr(d, n, arr(setup.masks[n]))
+        # without a dense cn1 the library uses its separable definition
+        d.cn1 = arr(c["cn1"]) if "cn1" in c else None
+        if setup.masks is not None:
+            for n in ("bmask1", "cmask1", "dmask1"):
+                setattr(d, n, arr(setup.masks[n]))
+        else:  # masks built on the device from the height map
+            d.zbm = arr(setup.zbm)
+            d.z2 = arr(m["z2"])
         d.inflow = arr(setup.inflow) if setup.inflow is not None else None
         d.ro, d.vn, d.alpha, d.beta, d.dt = ro, float(VN if vn is None else vn), alpha, beta, w.dt
         d.nmaxp, d.pjuge, d.omega = w.nmaxp, w.pjuge, w.omega
@@ -224,6 +234,13 @@ class Domain:
 
     def synchronize(self):
         _check(lib().les_domain_synchronize(self._h))
+
+    def done(self) -> bool:
+        """True when every call issued on the domain has completed (no wait)."""
+        r = lib().les_domain_query(self._h)
+        if r < 0:
+            raise LesError(lib().les_last_error().decode())
+        return r == 1
 
     def press_stats(self):
         return _stats(lib().les_domain_press_stats, self._h)
@@ -267,3 +284,8 @@ class Domain:
             self.close()
         except Exception:
             pass
+
+
+def bytes_needed(im: int, jm: int, km: int) -> int:
+    """Device memory a domain of these extents holds (for admission control)."""
+    return int(lib().les_domain_bytes_needed(im, jm, km))

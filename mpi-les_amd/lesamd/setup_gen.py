@@ -146,7 +146,7 @@ def grid_metrics(w: Workload) -> dict:
     return m
 
 
-def poisson_coefficients(m: dict, im: int, jm: int, km: int) -> dict:
+def poisson_coefficients(m: dict, im: int, jm: int, km: int, with_cn1: bool = True) -> dict:
     """cn1..cn4 (init.f95:82-107), FP32 in the reference's order."""
     dxs, dys, dzs = m["dxs"], m["dys"], m["dzs"]
     c = {n: layout.new_metric(n, im, jm, km) for n in ("cn2l", "cn2s", "cn3l", "cn3s", "cn4l", "cn4s")}
@@ -167,8 +167,12 @@ def poisson_coefficients(m: dict, im: int, jm: int, km: int) -> dict:
         c["cn4l"][k - 1] = two / (dz2 * (dz1 + dz2))
         cz[k - 1] = two / (dz1 * dz2)
     # cn1 = 1/(2/(dxs dxs) + 2/(dys dys) + 2/(dz1 dz2)), summed left to right
-    s = (ax[:, None, None] + by[None, :, None]) + cz[None, None, :]
-    c["cn1"] = np.asfortranarray((one / s).astype(f32))
+    if with_cn1:
+        s = np.empty((im, jm, km), dtype=f32, order="F")
+        np.add(ax[:, None, None], by[None, :, None], out=s)
+        s += cz[None, None, :]
+        np.divide(one, s, out=s)
+        c["cn1"] = s
     c["_cn1_terms"] = (ax, by, cz)
     return c
 
@@ -208,10 +212,9 @@ def building_masks(zbm: np.ndarray, m: dict, im: int, jm: int, km: int) -> dict:
     z2 = m["z2"]
     dzn = m["dzn"]
     top = np.array([z2[k] + f32(0.5) * dzn[k + 1] for k in range(1, km + 1)], dtype=f32)
-    solid = zbm[:, :, None] > top[None, None, :]           # (im, jm, km)
-    a = np.where(solid, f32(0.0), f32(1.0)).astype(f32)
-    b = np.where(solid, f32(1.0), f32(0.0)).astype(f32)
-    out["amask1"][layout.ref_slice("amask1", (1, 1, 1), (im, jm, km))] = a
+    solid = np.asfortranarray(zbm[:, :, None] > top[None, None, :])    # (im, jm, km)
+    b = solid.astype(f32)
+    out["amask1"][layout.ref_slice("amask1", (1, 1, 1), (im, jm, km))] = f32(1.0) - b
     for n in ("bmask1", "cmask1", "dmask1"):
         out[n][layout.ref_slice(n, (1, 1, 1), (im, jm, km))] = b
     return out
@@ -248,11 +251,14 @@ class DomainSetup:
         return {n: layout.new_field(n, w.im, w.jm, w.km, dtype) for n in layout.STEP_FIELDS}
 
 
-def build(w: Workload) -> DomainSetup:
+def build(w: Workload, dense: bool = True) -> DomainSetup:
+    """The set-up arrays of a workload.  dense=False leaves out the 3-D
+    arrays (cn1 and the masks): the library then evaluates cn1 from the 1-D
+    coefficients and builds the masks on the device from zbm."""
     m = grid_metrics(w)
-    c = poisson_coefficients(m, w.im, w.jm, w.km)
+    c = poisson_coefficients(m, w.im, w.jm, w.km, with_cn1=dense)
     zb = city_heights(w)
-    masks = building_masks(zb, m, w.im, w.jm, w.km)
+    masks = building_masks(zb, m, w.im, w.jm, w.km) if dense else None
     prof = inflow_profile(w, m)
     return DomainSetup(w, m, c, masks, zb, prof)
 

@@ -222,6 +222,11 @@ class RefBackend:
     def arrays(self):
         return self.dom.arrays
 
+    def flush(self):
+        """Flush the reference's unit-6 output (its progress prints)."""
+        if hasattr(self.lib, "ref_flush"):
+            self.lib.ref_flush()
+
     def call(self, stage, **extra):
         self.fn[stage](*self.dom.args(stage, **extra))
 

@@ -168,4 +168,10 @@ contains
         call boundp2(jm, im, p, km)
     end subroutine
 
+    ! The reference subroutines print progress to unit 6 (vel2.f95:66-68 and
+    ! others); a caller that silences stdout flushes it before restoring it.
+    subroutine ref_flush() bind(C, name='ref_flush')
+        flush(6)
+    end subroutine
+
 end module ref_harness

@@ -1,0 +1,108 @@
+"""The serving scheduler (admission by device memory, concurrent stepping,
+completion by polling) on stand-in domains: no GPU needed."""
+import itertools
+
+import pytest
+
+from lesamd import serving
+
+
+class FakeClock:
+    def __init__(self):
+        self.t = 0.0
+
+    def __call__(self):
+        self.t += 1e-4
+        return self.t
+
+
+class FakeDomain:
+    live = 0
+    peak = 0
+
+    def __init__(self, req, log):
+        self.req, self.log, self.steps, self.polls = req, log, [], 0
+        FakeDomain.live += 1
+        FakeDomain.peak = max(FakeDomain.peak, FakeDomain.live)
+
+    def step(self, n, nmax):
+        assert nmax == self.req.nsteps
+        self.steps.append(n)
+
+    def done(self):
+        self.polls += 1
+        return self.polls >= 2
+
+    def close(self):
+        FakeDomain.live -= 1
+        self.log.append((self.req.rid, tuple(self.steps)))
+
+
+def make(budget, reqs, sizes, burst=3, max_active=64):
+    log = []
+    srv = serving.Server(budget, burst=burst, max_active=max_active,
+                         domain_factory=lambda r: FakeDomain(r, log), bytes_of=lambda r: sizes[r.rid],
+                         clock=FakeClock())
+    return srv, log
+
+
+def _reqs(n, nsteps=7):
+    return [serving.Request(r, serving.setup_gen.Workload(f"r{r}", 8, 8, 4), nsteps) for r in range(n)]
+
+
+def test_every_request_runs_every_step_in_order():
+    FakeDomain.live = FakeDomain.peak = 0
+    reqs = _reqs(5)
+    srv, log = make(10 ** 9, reqs, {r.rid: 100 for r in reqs})
+    res = srv.run(reqs)
+    assert [r.rid for r in res] == list(range(5))
+    assert sorted(log) == [(r, tuple(range(1, 8))) for r in range(5)]
+    assert all(r.finished >= r.admitted >= r.arrival for r in res)
+    assert FakeDomain.peak == 5
+
+
+def test_admission_never_exceeds_the_memory_budget():
+    FakeDomain.live = FakeDomain.peak = 0
+    reqs = _reqs(6)
+    sizes = {0: 60, 1: 50, 2: 40, 3: 30, 4: 70, 5: 10}
+    srv, log = make(100, reqs, sizes)
+    srv.run(reqs)
+    # first come first served: 0 (60) then 1 would make 110 > 100, so only 0 runs alone
+    assert FakeDomain.peak <= 3
+    assert len(log) == 6
+
+
+def test_request_larger_than_budget_is_refused():
+    reqs = _reqs(1)
+    srv, _ = make(10, reqs, {0: 11})
+    with pytest.raises(ValueError):
+        srv.run(reqs)
+
+
+def test_sequential_mode_runs_one_domain_at_a_time():
+    FakeDomain.live = FakeDomain.peak = 0
+    reqs = _reqs(4)
+    srv, log = make(10 ** 9, reqs, {r.rid: 1 for r in reqs})
+    srv.run(reqs, sequential=True)
+    assert FakeDomain.peak == 1
+    assert [r for r, _ in log] == [0, 1, 2, 3]
+
+
+def test_sampled_requests_follow_the_mixed_size_distribution():
+    reqs = serving.sample_requests(200, 10, rate=5.0, seed=3)
+    dims = {(r.workload.im, r.workload.jm, r.workload.km) for r in reqs}
+    assert all(96 <= im <= 512 and im % 32 == 0 and km in (64, 96, 128) for im, _, km in dims)
+    arr = [r.arrival for r in reqs]
+    assert arr == sorted(arr) and arr[0] == 0.0
+    gaps = [b - a for a, b in itertools.pairwise(arr)] if hasattr(itertools, "pairwise") else \
+        [arr[i + 1] - arr[i] for i in range(len(arr) - 1)]
+    assert 0.1 < sum(gaps) / len(gaps) < 0.3   # mean inter-arrival 1/rate
+    again = serving.sample_requests(200, 10, rate=5.0, seed=3)
+    assert [r.workload for r in again] == [r.workload for r in reqs]
+
+
+def test_summary_percentiles():
+    res = [serving.Result(i, 1000, 10, 0.0, 0.0, float(i + 1)) for i in range(100)]
+    s = serving.summarize(res)
+    assert s["requests"] == 100 and abs(s["p50_s"] - 50.5) < 1e-9 and s["p99_s"] > 99
+    assert abs(s["mcell_updates_per_s"] - 100 * 1000 * 10 / 100 / 1e6) < 1e-12
