@@ -166,6 +166,15 @@ int les_domain_stage(les_domain* d, int stage, int n, int nmax);
 int les_domain_upload(les_domain* d, const char* field, const float* host);
 int les_domain_download(les_domain* d, const char* field, float* host);
 int les_domain_synchronize(les_domain* d);
+/* Pressure solver of the domain's press stage.  mode 0 (default): the
+ * reference's SOR (press.f95:89-122).  mode 1: FP64 defect correction --
+ * `outer` times, the defect d = (rhs - rhsav) - B(p) of the SOR's equation
+ * is formed in FP64 and `inner` FP32 SOR iterations (from zero, no stop
+ * test) solve B(e) = d, then p += e in FP64; p = p - pav in FP64 at the
+ * end.  In mode 1 les_domain_press_stats reports `outer` as the iteration
+ * count and sum(d^2) of each outer step as the history.  No reference
+ * counterpart: a precision variant (1 <= outer <= 64, 1 <= inner <= 1024). */
+int les_domain_set_press_mode(les_domain* d, int mode, int outer, int inner);
 /* 1 when every call issued on the domain has completed, 0 while work is
  * pending, -1 on error (does not wait).  No reference counterpart: used by
  * hosts that serve several domains at once. */

@@ -113,8 +113,10 @@ __global__ __launch_bounds__(256) void k_finalize(const double* __restrict__ par
   if (threadIdx.x == 0) {
     if (what == 0)
       ctrl->rhsav = s / denom;
-    else
+    else if (what == 1)
       ctrl->pav = s / denom;
+    else
+      ctrl->rhsav64 = s / denom;
   }
 }
 
@@ -160,7 +162,8 @@ __device__ __forceinline__ int xcd_contiguous(int b, int nb) {
 template <int COL, bool SEP>
 __global__ __launch_bounds__(NT) void k_sweep(Grid g, SplitGrid s, Metrics m, SplitBufs sb, Ctrl* ctrl,
                                               int iter, int nparts, float omega, float pjuge, int gx,
-                                              int gy, int kchunk, int remap) {
+                                              int gy, int kchunk, int remap,
+                                              const float* __restrict__ pent) {
   __shared__ double lds[NT / kWave];
   __shared__ int skip;
   const int tid = threadIdx.y * BX + threadIdx.x;
@@ -240,13 +243,24 @@ __global__ __launch_bounds__(NT) void k_sweep(Grid g, SplitGrid s, Metrics m, Sp
       if (i <= im) {
         const float own = cur.own;
         const float lo = d ? cur.a : cur.b, hi = d ? cur.b : cur.a;  // values at i-1, i+1
-        const float pim = (i == 1) ? own : lo;
-        const float pip = (i == im) ? own : hi;
-        const float pkm = (k == 1) ? own : a_prev;
-        const float pkp = (k == km) ? own : a_next;
+        float pim = (i == 1) ? own : lo;
+        float pip = (i == im) ? own : hi;
+        float pkm = (k == 1) ? own : a_prev;
+        float pkp = (k == km) ? own : a_next;
+        float pjm = cur.jl, pjp = cur.jh;
+        if (pent) {  // iteration 1: the ghost values p had on entry to press
+          if (COL == 1) {
+            if (i == 1) pim = pent[ix(g, 0, j, k)];
+            if (i == im) pip = pent[ix(g, im + 1, j, k)];
+            if (j == 1) pjm = pent[ix(g, i, 0, k)];
+            if (j == jm) pjp = pent[ix(g, i, jm + 1, k)];
+          }
+          if (k == 1) pkm = pent[ix(g, i, j, 0)];
+          if (k == km) pkp = pent[ix(g, i, j, km + 1)];
+        }
         const float cn = SEP ? 1.0f / ((M1(m.cnax, i) + byj) + M1(m.cncz, k)) : cur.cn;  // init.f95:100-107
-        const float reltmp = omega * (cn * (M1(m.cn2l, i) * pip + M1(m.cn2s, i) * pim + c3l * cur.jh +
-                                            c3s * cur.jl + M1(m.cn4l, k) * pkp + M1(m.cn4s, k) * pkm -
+        const float reltmp = omega * (cn * (M1(m.cn2l, i) * pip + M1(m.cn2s, i) * pim + c3l * pjp +
+                                            c3s * pjm + M1(m.cn4l, k) * pkp + M1(m.cn4s, k) * pkm -
                                             cur.r) -
                                       own);
         out[cix(s, t, j, k)] = own + reltmp;
@@ -319,10 +333,12 @@ __device__ __forceinline__ unsigned boff(const float* base, const float* p) {
 }
 
 // two blocks per CU need at most 64 VGPRs per lane (8 waves per SIMD)
-template <bool SEP>
+// ENTRY: iteration 1, which reads the ghost values p had on entry to press
+// (pent); the other iterations are compiled without that code.
+template <bool SEP, bool ENTRY>
 __global__ __launch_bounds__(FNT) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_sor_iter(
     Grid g, SplitGrid s, Metrics m, SplitBufs sb, Ctrl* ctrl, int iter, int nparts, float omega, float pjuge,
-    int ntt, int ntj, int kchunk, int remap) {
+    int ntt, int ntj, int kchunk, int remap, const float* __restrict__ pent) {
   __shared__ float ldsB[4][FY][FX];  // previous black iterate, levels mod 4
   __shared__ float ldsR[2][FY][FX];  // new red values, levels mod 2
   __shared__ double lds[FNT / kWave];
@@ -452,12 +468,21 @@ __global__ __launch_bounds__(FNT) __attribute__((amdgpu_waves_per_eu(8, 8))) voi
       const float own = sl.ro;
       const float* pl = &ldsB[kr & 3][row][lane - 1 + d];
       const float lo = pl[0], hi = pl[1];
-      const float pim = at_first ? own : lo;
-      const float pip = at_last ? own : hi;
-      const float pjm = wlo ? sl.wr : ldsB[kr & 3][row - 1][lane];
-      const float pjp = whi ? sl.wr : ldsB[kr & 3][row + 1][lane];
-      const float pkm = (kr == 1) ? own : ldsB[(kr - 1) & 3][row][lane];
-      const float pkp = (kr == km) ? own : ldsB[(kr + 1) & 3][row][lane];
+      float pim = at_first ? own : lo;
+      float pip = at_last ? own : hi;
+      float pjm = wlo ? sl.wr : ldsB[kr & 3][row - 1][lane];
+      float pjp = whi ? sl.wr : ldsB[kr & 3][row + 1][lane];
+      float pkm = (kr == 1) ? own : ldsB[(kr - 1) & 3][row][lane];
+      float pkp = (kr == km) ? own : ldsB[(kr + 1) & 3][row][lane];
+      if (ENTRY && tin) {  // iteration 1: the ghost values p had on entry to press
+        const int i = 2 * t + 1 + d;
+        if (at_first) pim = pent[ix(g, 0, j, kr)];
+        if (at_last) pip = pent[ix(g, im + 1, j, kr)];
+        if (j == 1) pjm = pent[ix(g, i, 0, kr)];
+        if (j == jm) pjp = pent[ix(g, i, jm + 1, kr)];
+        if (kr == 1) pkm = pent[ix(g, i, j, 0)];
+        if (kr == km) pkp = pent[ix(g, i, j, km + 1)];
+      }
       const int x = kr - krs;
       const float cn = SEP ? 1.0f / ((axd + byj) + lvm[2 * nl + x]) : sl.cr;  // init.f95:100-107
       const float reltmp = omega * (cn * (c2l * pip + c2s * pim + c3l * pjp + c3s * pjm + lvm[x] * pkp +
@@ -480,8 +505,13 @@ __global__ __launch_bounds__(FNT) __attribute__((amdgpu_waves_per_eu(8, 8))) voi
       const float pip = at_last ? own : hi;
       const float pjm = wlo ? sl.wb : ldsR[kb & 1][row - 1][lane];
       const float pjp = whi ? sl.wb : ldsR[kb & 1][row + 1][lane];
-      const float pkm = (kb == 1) ? own : rm2;
-      const float pkp = (kb == km) ? own : r0;
+      float pkm = (kb == 1) ? own : rm2;
+      float pkp = (kb == km) ? own : r0;
+      if (ENTRY && tin) {  // iteration 1: boundp2 refreshes the k ghosts only after the black sweep
+        const int i = 2 * t + 1 + d;
+        if (kb == 1) pkm = pent[ix(g, i, j, 0)];
+        if (kb == km) pkp = pent[ix(g, i, j, km + 1)];
+      }
       const int x = kb - krs;
       const float cn = SEP ? 1.0f / ((axd + byj) + lvm[2 * nl + x]) : sl.cb;
       const float reltmp = omega * (cn * (c2l * pip + c2s * pim + c3l * pjp + c3s * pjm + lvm[x] * pkp +
@@ -594,6 +624,212 @@ static void launch_press_tail(const DevDomain& d, const SplitBufs& sb, int npart
                                        blk(), 0, d.stream, g, s, sb, d.F.p, (const Ctrl*)d.ctrl));
 }
 
+// ------------------------------------------- FP64 defect correction
+// The precision variant of press (press mode 1): the pressure equation of
+// the SOR, B(p) = rhs with B(p) = cn2l p(i+1) + cn2s p(i-1) + cn3l p(j+1) +
+// cn3s p(j-1) + cn4l p(k+1) + cn4s p(k-1) - p/cn1 (press.f95:95-98 at its
+// fixed point; Neumann in i and k, periodic in j as boundp1/boundp2 make
+// it), is solved by defect correction: the iterate P64 and the right-hand
+// side R64 = rhs - rhsav are kept in FP64; each outer step forms the defect
+// d = R64 - B(P64) in FP64, solves B(e) = d approximately with FP32 SOR
+// iterations from e = 0 (k_sor_iter, unchanged), and adds e to P64.  The
+// defect norms are recorded per outer step; p = P64 - pav at the end.
+
+// R64 = div(F) + div(u)/dt evaluated in FP64 (press.f95:49-60 with the
+// ghosts k_rhs has just written), block partials of dV*R64 for its mean;
+// P64 = p, the previous pressure, as the starting iterate.
+__global__ __launch_bounds__(NT) void k_dc_init(Grid g, Metrics m, const float* __restrict__ u,
+                                                const float* __restrict__ v, const float* __restrict__ w,
+                                                const float* __restrict__ f, const float* __restrict__ gg,
+                                                const float* __restrict__ h, const float* __restrict__ p,
+                                                double* __restrict__ R64, double* __restrict__ P64,
+                                                double* __restrict__ part, double dt) {
+  __shared__ double lds[NT / kWave];
+  const int tid = threadIdx.y * BX + threadIdx.x;
+  const int bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+  const int i = 1 + blockIdx.x * BX + threadIdx.x, j = 1 + blockIdx.y * BY + threadIdx.y;
+  double acc = 0.0;
+  if (i <= g.im && j <= g.jm) {
+    const double dxi = M1(m.dx1, i), dyj = M1(m.dy1, j);
+    const int sj = g.sj, sk = g.sk;
+    int k0, k1;
+    kspan(1, g.km, k0, k1);
+    for (int k = k0; k <= k1; k++) {
+      const int c = ix(g, i, j, k);
+      const double dzk = M1(m.dzn, k);
+      const double r = (-(double)u[c - 1] + u[c]) / dxi + (-(double)v[c - sj] + v[c]) / dyj +
+                       (-(double)w[c - sk] + w[c]) / dzk;
+      const double rr = ((double)f[c] - f[c - 1]) / dxi + ((double)gg[c] - gg[c - sj]) / dyj +
+                        ((double)h[c] - h[c - sk]) / dzk + r / dt;
+      R64[c] = rr;
+      P64[c] = (double)p[c];
+      acc += dxi * dyj * dzk * rr;
+    }
+  }
+  const double sum = block_sum<NT>(acc, tid, lds);
+  if (tid == 0) part[bid] = sum;
+}
+
+// d = (R64 - rhsav64) - B(P64) in FP64: its FP32 copy into the split rhs
+// the SOR reads, e = 0 in the split iterate of parity 0, block partials of
+// d^2; block 0 resets the SOR control for the inner solve.  (The defect is
+// not projected: K inner iterations from e = 0 are then, in exact
+// arithmetic, K further iterations of the reference's SOR.)
+template <bool SEP>
+__global__ __launch_bounds__(NT) void k_dc_defect(Grid g, SplitGrid s, Metrics m, const float* __restrict__ cn1f,
+                                                  const double* __restrict__ R64,
+                                                  const double* __restrict__ P64, SplitBufs sb,
+                                                  double* __restrict__ part, Ctrl* ctrl) {
+  __shared__ double lds[NT / kWave];
+  const int tid = threadIdx.y * BX + threadIdx.x;
+  const int bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+  const double rav = ctrl->rhsav64;
+  __syncthreads();  // every thread has read the control block before block 0 writes it
+  if (bid == 0 && tid == 0) {
+    ctrl->sor_done = 0;
+    ctrl->nonfinite = 0;
+  }
+  const int i = 1 + blockIdx.x * BX + threadIdx.x, j = 1 + blockIdx.y * BY + threadIdx.y;
+  double acc = 0.0;
+  if (i <= g.im && j <= g.jm) {
+    const int sj = g.sj, sk = g.sk;
+    const int dip = (i == g.im) ? 0 : 1, dim_ = (i == 1) ? 0 : -1;
+    const int cjp = (j == g.jm) ? (1 - j) * sj : sj, cjm = (j == 1) ? (g.jm - 1) * sj : -sj;
+    const double c2l = M1(m.cn2l, i), c2s = M1(m.cn2s, i), c3l = M1(m.cn3l, j), c3s = M1(m.cn3s, j);
+    const int t = (i - 1) >> 1;
+    int k0, k1;
+    kspan(1, g.km, k0, k1);
+    for (int k = k0; k <= k1; k++) {
+      const int c = ix(g, i, j, k);
+      const int dkp = (k == g.km) ? 0 : sk, dkm = (k == 1) ? 0 : -sk;
+      float cn;
+      if (SEP)
+        cn = 1.0f / ((M1(m.cnax, i) + M1(m.cnby, j)) + M1(m.cncz, k));  // init.f95:100-107
+      else
+        cn = cn1f[c];
+      const double pc = P64[c];
+      const double b = c2l * P64[c + dip] + c2s * P64[c + dim_] + c3l * P64[c + cjp] + c3s * P64[c + cjm] +
+                       (double)M1(m.cn4l, k) * P64[c + dkp] + (double)M1(m.cn4s, k) * P64[c + dkm] -
+                       pc / (double)cn;
+      const double dd = (R64[c] - rav) - b;
+      const int col = (i + j + k) & 1;
+      const int q = cix(s, t, j, k);
+      sb.rhs[col][q] = (float)dd;
+      sb.p[0][col][q] = 0.0f;
+      acc += dd * dd;
+    }
+  }
+  const double sum = block_sum<NT>(acc, tid, lds);
+  if (tid == 0) part[bid] = sum;
+}
+
+// dc_hist[step] = sum d^2 of the outer step
+__global__ __launch_bounds__(256) void k_dc_norm(const double* __restrict__ part, int n, double* __restrict__ hist,
+                                                 int step) {
+  __shared__ double lds[4];
+  double s = 0.0;
+  for (int b = threadIdx.x; b < n; b += 256) s += part[b];
+  s = block_sum<256>(s, threadIdx.x, lds);
+  if (threadIdx.x == 0) hist[step] = s;
+}
+
+// P64 += e (the inner solve's last iterate, parity `par`)
+__global__ __launch_bounds__(NT) void k_dc_correct(Grid g, SplitGrid s, SplitBufs sb, double* __restrict__ P64,
+                                                   int par) {
+  const int i = 1 + blockIdx.x * BX + threadIdx.x, j = 1 + blockIdx.y * BY + threadIdx.y;
+  if (i > g.im || j > g.jm) return;
+  const int t = (i - 1) >> 1;
+  int k0, k1;
+  kspan(1, g.km, k0, k1);
+  for (int k = k0; k <= k1; k++) {
+    const int col = (i + j + k) & 1;
+    P64[ix(g, i, j, k)] += (double)sb.p[par][col][cix(s, t, j, k)];
+  }
+}
+
+// pav partials of P64 (press.f95:124-133 in FP64)
+__global__ __launch_bounds__(NT) void k_dc_pav(Grid g, Metrics m, const double* __restrict__ P64,
+                                               double* __restrict__ part) {
+  __shared__ double lds[NT / kWave];
+  const int tid = threadIdx.y * BX + threadIdx.x;
+  const int bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+  const int i = 1 + blockIdx.x * BX + threadIdx.x, j = 1 + blockIdx.y * BY + threadIdx.y;
+  double acc = 0.0;
+  if (i <= g.im && j <= g.jm) {
+    const double dxy = (double)M1(m.dx1, i) * (double)M1(m.dy1, j);
+    int k0, k1;
+    kspan(1, g.km, k0, k1);
+    for (int k = k0; k <= k1; k++) acc += P64[ix(g, i, j, k)] * dxy * (double)M1(m.dzn, k);
+  }
+  const double sum = block_sum<NT>(acc, tid, lds);
+  if (tid == 0) part[bid] = sum;
+}
+
+// p = P64 - pav with the ghost cells of boundp1/boundp2 (as k_padj), and
+// the defect history into the statistics (sor_iters = outer steps)
+__global__ __launch_bounds__(NT) void k_dc_padj(Grid g, const double* __restrict__ P64, float* __restrict__ p,
+                                                Ctrl* ctrl, const double* __restrict__ hist, int nouter) {
+  const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+  if (blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0 && threadIdx.x == 0 && threadIdx.y == 0) {
+    for (int q = 0; q < nouter; q++) ctrl->sor_hist[q] = hist[q];
+    ctrl->sor_iters = nouter;
+    ctrl->sor_done = 0;
+  }
+  if (i > g.im + 1 || j > g.jm + 1) return;
+  const double av = ctrl->pav;
+  const int ii = (i == 0) ? 1 : ((i == g.im + 1) ? g.im : i);
+  const int jj = (j == 0) ? g.jm : ((j == g.jm + 1) ? 1 : j);
+  int k0, k1;
+  kspan(0, g.km + 1, k0, k1);
+  for (int k = k0; k <= k1; k++) {
+    const int kk = (k == 0) ? 1 : ((k == g.km + 1) ? g.km : k);
+    p[ix(g, i, j, k)] = (float)(P64[ix(g, ii, jj, kk)] - av);
+  }
+}
+
+static void launch_press_dc(const DevDomain& d, const SplitBufs& sb, int gx, int gy, int gz, int nred) {
+  const Grid& g = d.g;
+  const SplitGrid& s = d.sg;
+  const int ntt = (s.nt + FTX - 1) / FTX, ntj = (g.jm + FTY - 1) / FTY;
+  const int kc = sor_fused_kchunk(g, s, d.sor_fused_blocks, d.ncu);
+  const int nparts = ntt * ntj * ((g.km + kc - 1) / kc);
+  const size_t lvm_bytes = 3 * (size_t)(kc + 2) * sizeof(float);
+  double* P64 = d.dc64;
+  double* R64 = d.dc64 + d.n3;
+  double* hist = d.dc64 + 2 * (size_t)d.n3;
+  for (int o = 0; o < d.dc_outer; o++) {
+    if (d.cn1_separable)
+      TIMED(d, KC_RHS, hipLaunchKernelGGL((k_dc_defect<true>), dim3(gx, gy, gz), blk(), 0, d.stream, g, s, d.m,
+                                          (const float*)d.F.cn1, (const double*)R64, (const double*)P64, sb,
+                                          d.partials, d.ctrl));
+    else
+      TIMED(d, KC_RHS, hipLaunchKernelGGL((k_dc_defect<false>), dim3(gx, gy, gz), blk(), 0, d.stream, g, s, d.m,
+                                          (const float*)d.F.cn1, (const double*)R64, (const double*)P64, sb,
+                                          d.partials, d.ctrl));
+    TIMED(d, KC_RHS, hipLaunchKernelGGL(k_dc_norm, dim3(1), dim3(256), 0, d.stream, (const double*)d.partials, nred,
+                                        hist, o));
+    for (int l = 1; l <= d.dc_inner; l++) {
+      if (d.cn1_separable)
+        TIMED(d, KC_SOR, hipLaunchKernelGGL((k_sor_iter<true, false>), dim3(nparts), dim3(FX, FY), lvm_bytes, d.stream,
+                                            g, s, d.m, sb, d.ctrl, l, nparts, d.omega, 0.0f, ntt, ntj, kc,
+                                            d.xcd_remap, nullptr));
+      else
+        TIMED(d, KC_SOR, hipLaunchKernelGGL((k_sor_iter<false, false>), dim3(nparts), dim3(FX, FY), lvm_bytes,
+                                            d.stream, g, s, d.m, sb, d.ctrl, l, nparts, d.omega, 0.0f, ntt, ntj, kc,
+                                            d.xcd_remap, nullptr));
+    }
+    TIMED(d, KC_PAV, hipLaunchKernelGGL(k_dc_correct, dim3(gx, gy, g.km), blk(), 0, d.stream, g, s, sb, P64,
+                                        d.dc_inner & 1));
+  }
+  TIMED(d, KC_PAV, hipLaunchKernelGGL(k_dc_pav, dim3(gx, gy, gz), blk(), 0, d.stream, g, d.m, (const double*)P64,
+                                      d.partials));
+  TIMED(d, KC_PAV, hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, d.stream, d.partials, nred, d.ctrl, 1,
+                                      d.area));
+  const int px = (g.im + 2 + BX - 1) / BX, py = (g.jm + 2 + BY - 1) / BY;
+  TIMED(d, KC_PADJ, hipLaunchKernelGGL(k_dc_padj, dim3(px, py, g.km + 2), blk(), 0, d.stream, g, (const double*)P64,
+                                       d.F.p, d.ctrl, (const double*)hist, d.dc_outer));
+}
+
 void launch_press(const DevDomain& d, float dt) {
   const Grid& g = d.g;
   const SplitGrid& s = d.sg;
@@ -606,21 +842,47 @@ void launch_press(const DevDomain& d, float dt) {
                                       d.area));
   SplitBufs sb = d.sb;
   if (d.cn1_separable) sb.cn1[0] = sb.cn1[1] = nullptr;
+  if (d.press_mode == 1) {
+    TIMED(d, KC_RHS, hipLaunchKernelGGL(k_dc_init, dim3(gx, gy, gz), blk(), 0, d.stream, g, d.m,
+                                        (const float*)d.F.u, (const float*)d.F.v, (const float*)d.F.w,
+                                        (const float*)d.F.f, (const float*)d.F.g, (const float*)d.F.h,
+                                        (const float*)d.F.p, d.dc64 + d.n3, d.dc64, d.partials, (double)dt));
+    TIMED(d, KC_RHS, hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, d.stream, d.partials, nred, d.ctrl, 2,
+                                        d.area));
+  }
   const int sz = g.km;
   TIMED(d, KC_RHS, hipLaunchKernelGGL(k_rhs_split, dim3(gx, gy, sz), blk(), 0, d.stream, g, s, d.F.rhs,
                                       (const float*)d.F.p, (const float*)d.F.cn1, sb, (const Ctrl*)d.ctrl));
+  if (d.press_mode == 1) {
+    launch_press_dc(d, sb, gx, gy, gz, nred);
+    return;
+  }
   if (d.sor_fused) {
     const int ntt = (s.nt + FTX - 1) / FTX, ntj = (g.jm + FTY - 1) / FTY;
     const int kc = sor_fused_kchunk(g, s, d.sor_fused_blocks, d.ncu);
     const int nparts = ntt * ntj * ((g.km + kc - 1) / kc);
     const size_t lvm_bytes = 3 * (size_t)(kc + 2) * sizeof(float);
+    const float* pent = d.F.p;
     for (int l = 1; l <= d.nmaxp; l++) {
-      if (d.cn1_separable)
-        TIMED(d, KC_SOR, hipLaunchKernelGGL((k_sor_iter<true>), dim3(nparts), dim3(FX, FY), lvm_bytes, d.stream, g, s, d.m,
-                                            sb, d.ctrl, l, nparts, d.omega, d.pjuge, ntt, ntj, kc, d.xcd_remap));
-      else
-        TIMED(d, KC_SOR, hipLaunchKernelGGL((k_sor_iter<false>), dim3(nparts), dim3(FX, FY), lvm_bytes, d.stream, g, s, d.m,
-                                            sb, d.ctrl, l, nparts, d.omega, d.pjuge, ntt, ntj, kc, d.xcd_remap));
+      if (d.cn1_separable) {
+        if (l == 1)
+          TIMED(d, KC_SOR, hipLaunchKernelGGL((k_sor_iter<true, true>), dim3(nparts), dim3(FX, FY), lvm_bytes, d.stream,
+                                              g, s, d.m, sb, d.ctrl, l, nparts, d.omega, d.pjuge, ntt, ntj, kc,
+                                              d.xcd_remap, pent));
+        else
+          TIMED(d, KC_SOR, hipLaunchKernelGGL((k_sor_iter<true, false>), dim3(nparts), dim3(FX, FY), lvm_bytes,
+                                              d.stream, g, s, d.m, sb, d.ctrl, l, nparts, d.omega, d.pjuge, ntt, ntj,
+                                              kc, d.xcd_remap, nullptr));
+      } else {
+        if (l == 1)
+          TIMED(d, KC_SOR, hipLaunchKernelGGL((k_sor_iter<false, true>), dim3(nparts), dim3(FX, FY), lvm_bytes,
+                                              d.stream, g, s, d.m, sb, d.ctrl, l, nparts, d.omega, d.pjuge, ntt, ntj,
+                                              kc, d.xcd_remap, pent));
+        else
+          TIMED(d, KC_SOR, hipLaunchKernelGGL((k_sor_iter<false, false>), dim3(nparts), dim3(FX, FY), lvm_bytes,
+                                              d.stream, g, s, d.m, sb, d.ctrl, l, nparts, d.omega, d.pjuge, ntt, ntj,
+                                              kc, d.xcd_remap, nullptr));
+      }
     }
     launch_press_tail(d, sb, nparts, gx, gy, gz, nred);
     return;
@@ -633,14 +895,18 @@ void launch_press(const DevDomain& d, float dt) {
   for (int l = 1; l <= d.nmaxp; l++) {
     if (d.cn1_separable) {
       TIMED(d, KC_SOR, hipLaunchKernelGGL((k_sweep<1, true>), sgrid, blk(), 0, d.stream, g, s, d.m, sb, d.ctrl,
-                                          l, nparts, d.omega, d.pjuge, tx, gy, kc, d.xcd_remap));
+                                          l, nparts, d.omega, d.pjuge, tx, gy, kc, d.xcd_remap,
+                                          l == 1 ? (const float*)d.F.p : nullptr));
       TIMED(d, KC_SOR, hipLaunchKernelGGL((k_sweep<0, true>), sgrid, blk(), 0, d.stream, g, s, d.m, sb, d.ctrl,
-                                          l, nparts, d.omega, d.pjuge, tx, gy, kc, d.xcd_remap));
+                                          l, nparts, d.omega, d.pjuge, tx, gy, kc, d.xcd_remap,
+                                          l == 1 ? (const float*)d.F.p : nullptr));
     } else {
       TIMED(d, KC_SOR, hipLaunchKernelGGL((k_sweep<1, false>), sgrid, blk(), 0, d.stream, g, s, d.m, sb, d.ctrl,
-                                          l, nparts, d.omega, d.pjuge, tx, gy, kc, d.xcd_remap));
+                                          l, nparts, d.omega, d.pjuge, tx, gy, kc, d.xcd_remap,
+                                          l == 1 ? (const float*)d.F.p : nullptr));
       TIMED(d, KC_SOR, hipLaunchKernelGGL((k_sweep<0, false>), sgrid, blk(), 0, d.stream, g, s, d.m, sb, d.ctrl,
-                                          l, nparts, d.omega, d.pjuge, tx, gy, kc, d.xcd_remap));
+                                          l, nparts, d.omega, d.pjuge, tx, gy, kc, d.xcd_remap,
+                                          l == 1 ? (const float*)d.F.p : nullptr));
     }
   }
   launch_press_tail(d, sb, nparts, gx, gy, gz, nred);

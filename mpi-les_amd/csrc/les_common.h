@@ -47,6 +47,7 @@ struct Metrics {
 #define M1(a, x) ((a)[(x) + 1])
 
 constexpr int kSorHist = 1024;
+constexpr int kDcMaxOuter = 64;  // defect-correction steps per press call, at most
 
 // Per-domain device scalars of the reductions and the SOR loop.
 struct Ctrl {
@@ -59,6 +60,7 @@ struct Ctrl {
   double rhsav;            // volume-weighted mean of rhs
   double pav;              // volume-weighted mean of p
   double sor_first;        // sum of reltmp^2 of the current iteration's first colour
+  double rhsav64;          // volume-weighted mean of the FP64 rhs (press mode 1)
   double sor_hist[kSorHist];
 };
 

@@ -354,6 +354,7 @@ void domain_free(les_domain* D) {
   if (D->staging) hipFree(D->staging);
   if (D->d.ctrl) hipFree(D->d.ctrl);
   if (D->d.partials) hipFree(D->d.partials);
+  if (D->d.dc64) hipFree(D->d.dc64);
   if (D->d.stream) hipStreamDestroy(D->d.stream);
   delete D;
 }
@@ -372,6 +373,7 @@ int domain_alloc(int im, int jm, int km, les_domain** out) {
   g.sj = (int)sj;
   g.sk = (int)sk;
   D->n3 = (int)n3;
+  D->d.n3 = (int)n3;
   auto bail = [&](const std::string& m) {
     domain_free(D);
     return fail(m);
@@ -910,6 +912,32 @@ int les_domain_create(const les_domain_desc* s, les_domain** out) {
   D->d.pjuge = s->pjuge;
   D->d.omega = s->omega;
   *out = D;
+  return 0;
+}
+
+int les_domain_set_press_mode(les_domain* D, int mode, int outer, int inner) {
+  if (!D) return fail("null domain");
+  if (mode == 0) {
+    D->d.press_mode = 0;
+    return 0;
+  }
+  if (mode != 1) return fail("press mode must be 0 (reference SOR) or 1 (FP64 defect correction)");
+  if (outer < 1 || outer > kDcMaxOuter) return fail("defect-correction steps out of range (1..64)");
+  if (inner < 1 || inner > kSorHist) return fail("inner SOR iterations out of range (1..1024)");
+  if (!D->d.dc64) {
+    const size_t nb = (2 * (size_t)D->n3 + kDcMaxOuter) * sizeof(double);
+    hipError_t e = hipMalloc(&D->d.dc64, nb);
+    if (e == hipSuccess) e = hipMemset(D->d.dc64, 0, nb);
+    if (e != hipSuccess) {
+      if (D->d.dc64) hipFree(D->d.dc64);
+      D->d.dc64 = nullptr;
+      return fail(std::string("defect-correction buffers: ") + hipGetErrorString(e));
+    }
+    D->bytes += (long long)nb;
+  }
+  D->d.press_mode = 1;
+  D->d.dc_outer = outer;
+  D->d.dc_inner = inner;
   return 0;
 }
 

@@ -87,6 +87,7 @@ def lib():
     L.les_domain_download.argtypes = [vp, ctypes.c_char_p, vp]
     L.les_domain_synchronize.argtypes = [vp]
     L.les_domain_query.argtypes = [vp]
+    L.les_domain_set_press_mode.argtypes = [vp, ci, ci, ci]
     L.les_domain_bytes_needed.argtypes = [ci, ci, ci]
     L.les_domain_bytes_needed.restype = ctypes.c_longlong
     L.les_domain_press_stats.argtypes = [vp, vp, vp, ci, vp, vp, vp]
@@ -234,6 +235,11 @@ class Domain:
 
     def synchronize(self):
         _check(lib().les_domain_synchronize(self._h))
+
+    def set_press_mode(self, mode: int, outer: int = 4, inner: int = 25):
+        """0: the reference's SOR; 1: FP64 defect correction (`outer` steps
+        of `inner` FP32 SOR iterations)."""
+        _check(lib().les_domain_set_press_mode(self._h, mode, outer, inner))
 
     def done(self) -> bool:
         """True when every call issued on the domain has completed (no wait)."""

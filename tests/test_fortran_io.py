@@ -41,3 +41,28 @@ def test_dropin_module_library_exports_time_loop():
         pytest.skip("tests/fortran/liblestimeloop.so not built (flang absent)")
     lib = ctypes.CDLL(str(fortran_loop.LIB))
     assert hasattr(lib, "les_test_time_loop")
+
+
+def test_restart_pair_layout_and_round_trip(tmp_path):
+    """data30/data31 as anime.f95:326-595 writes them: an 8-byte (n, time)
+    record, then one interior record per field, big-endian."""
+    from lesamd import layout, restart
+    im, jm, km = 6, 5, 4
+    rng = np.random.default_rng(1)
+    fields = {}
+    for nm in restart.DATA30 + restart.DATA31:
+        a = layout.new_field(nm, im, jm, km)
+        a[...] = rng.standard_normal(a.shape).astype(np.float32)
+        fields[nm] = a
+    p30, p31 = restart.write_restart(tmp_path, 48000, 2399.95, fields, im, jm, km)
+    assert p30.name == "data30048000.dat" and p31.name == "data31048000.dat"
+    recs = fortran_io.read_records(p30)
+    assert len(recs) == 8 and len(recs[0]) == 8 and len(recs[1]) == 4 * im * jm * km
+    assert recs[0][:4] == (48000).to_bytes(4, "big")
+    assert len(fortran_io.read_records(p31)) == 6
+    n, t, got = restart.read_restart(p30, p31, im, jm, km)
+    assert n == 48000 and abs(t - 2399.95) < 1e-3
+    for nm in got:
+        assert np.array_equal(got[nm], layout.interior(nm, fields[nm], im, jm, km)), nm
+    with pytest.raises(ValueError):
+        restart.read_restart(p31, p30, im, jm, km)
