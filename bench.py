@@ -146,6 +146,44 @@ def serve_measurement(nreq: int, nsteps: int) -> dict:
     }
 
 
+def precision_comparison(K: int, W: int) -> int:
+    """The precision-512x512x128 workload (dx=dy=2 m, 35 % cover, u*=0.6,
+    z0=0.3, dt=0.02; SOR at most 100 iterations, stop at 1e-6) stepped with
+    the reference's FP32 SOR and with the FP64 defect correction doing the
+    same number of SOR iterations (4 outer steps of 25): time per step from
+    graph replay, and the SOR statistics of the last step."""
+    w = setup_gen.workloads()["precision-512x512x128"]
+    out = {"workload": w.name, "grid": [w.im, w.jm, w.km], "steps": K, "warmup": W}
+    for mode, tag in ((0, "fp32_sor"), (1, "fp64_defect_correction")):
+        dom = api.Domain(setup_gen.build(w, dense=False))
+        if mode:
+            dom.set_press_mode(1, outer=4, inner=25)
+        nmax = W + K + 1
+        for n in range(1, W + 1):
+            dom.step(n, nmax)
+        dom.set_graph(True)
+        dom.step(W + 1, nmax)
+        dom.synchronize()
+        t0 = time.perf_counter()
+        for n in range(W + 2, W + K + 2):
+            dom.step(n, nmax)
+        dom.synchronize()
+        dt_ = (time.perf_counter() - t0) / K
+        st = dom.press_stats()
+        rec = {"ms_per_step": round(1e3 * dt_, 3), "mcell_updates_per_s": round(w.cells / dt_ / 1e6, 1)}
+        if mode:
+            h = st["sor"]
+            rec.update({"sor_iterations": 4 * 25, "defect_sumsq_per_outer_step": [float(x) for x in h],
+                        "defect_reduction": float(h[-1] / h[0])})
+        else:
+            rec.update({"sor_iterations": st["iters"], "last_sum_reltmp_sq": float(st["sor"][-1])})
+        out[tag] = rec
+        dom.close()
+    out["time_ratio"] = round(out["fp64_defect_correction"]["ms_per_step"] / out["fp32_sor"]["ms_per_step"], 3)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def copy_calibration(mib: int) -> int:
     """Ten device-to-device copies of `mib` MiB through the HIP runtime: a
     kernel of known traffic (read mib, write mib) for scaling the HBM byte
@@ -179,11 +217,16 @@ def main(argv=None):
     ap.add_argument("--serve-requests", type=int, default=24,
                     help="mixed-size requests of the serving measurement (0: skip it)")
     ap.add_argument("--serve-steps", type=int, default=40, help="time steps per served request")
+    ap.add_argument("--precision", action="store_true",
+                    help="only compare the reference SOR with the FP64 defect-correction press on the "
+                         "precision-512x512x128 workload (prints its own JSON line)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run device-to-device copies of this size (HBM counter calibration)")
     args = ap.parse_args(argv)
     if args.copy_calibration:
         return copy_calibration(args.copy_calibration)
+    if args.precision:
+        return precision_comparison(args.steps if args.steps != 100 else 20, args.warmup if args.warmup != 10 else 3)
     if args.gpus != 1:
         print("bench.py: this build runs on exactly one GPU (one domain per device); "
               f"--gpus {args.gpus} is not supported", file=sys.stderr)
