@@ -179,6 +179,12 @@ int les_domain_set_press_mode(les_domain* d, int mode, int outer, int inner);
  * pending, -1 on error (does not wait).  No reference counterpart: used by
  * hosts that serve several domains at once. */
 int les_domain_query(les_domain* d);
+/* Self-test of the device reciprocal the kernels use in place of the
+ * correctly rounded division 1.0f/x: counts the x with bit patterns in
+ * [lo_bits, hi_bits) where the two differ, and the first such pattern
+ * (0xffffffff when none).  Synchronises.  No reference counterpart. */
+int les_selftest_reciprocal(unsigned lo_bits, unsigned hi_bits, unsigned long long* misses,
+                            unsigned* first_bits);
 /* Bytes of device memory a domain of these extents would hold. */
 long long les_domain_bytes_needed(int im, int jm, int km);
 int les_domain_press_stats(les_domain* d, int* iters, double* sor_hist, int hist_len, double* rhsav,

@@ -258,7 +258,7 @@ __global__ __launch_bounds__(NT) void k_sweep(Grid g, SplitGrid s, Metrics m, Sp
           if (k == 1) pkm = pent[ix(g, i, j, 0)];
           if (k == km) pkp = pent[ix(g, i, j, km + 1)];
         }
-        const float cn = SEP ? 1.0f / ((M1(m.cnax, i) + byj) + M1(m.cncz, k)) : cur.cn;  // init.f95:100-107
+        const float cn = SEP ? rcp_rn((M1(m.cnax, i) + byj) + M1(m.cncz, k)) : cur.cn;  // init.f95:100-107
         const float reltmp = omega * (cn * (M1(m.cn2l, i) * pip + M1(m.cn2s, i) * pim + c3l * pjp +
                                             c3s * pjm + M1(m.cn4l, k) * pkp + M1(m.cn4s, k) * pkm -
                                             cur.r) -
@@ -484,7 +484,7 @@ __global__ __launch_bounds__(FNT) __attribute__((amdgpu_waves_per_eu(8, 8))) voi
         if (kr == km) pkp = pent[ix(g, i, j, km + 1)];
       }
       const int x = kr - krs;
-      const float cn = SEP ? 1.0f / ((axd + byj) + lvm[2 * nl + x]) : sl.cr;  // init.f95:100-107
+      const float cn = SEP ? rcp_rn((axd + byj) + lvm[2 * nl + x]) : sl.cr;  // init.f95:100-107
       const float reltmp = omega * (cn * (c2l * pip + c2s * pim + c3l * pjp + c3s * pjm + lvm[x] * pkp +
                                           lvm[nl + x] * pkm - sl.rr) -
                                     own);
@@ -513,7 +513,7 @@ __global__ __launch_bounds__(FNT) __attribute__((amdgpu_waves_per_eu(8, 8))) voi
         if (kb == km) pkp = pent[ix(g, i, j, km + 1)];
       }
       const int x = kb - krs;
-      const float cn = SEP ? 1.0f / ((axd + byj) + lvm[2 * nl + x]) : sl.cb;
+      const float cn = SEP ? rcp_rn((axd + byj) + lvm[2 * nl + x]) : sl.cb;
       const float reltmp = omega * (cn * (c2l * pip + c2s * pim + c3l * pjp + c3s * pjm + lvm[x] * pkp +
                                           lvm[nl + x] * pkm - sl.rb) -
                                     own);
@@ -704,7 +704,7 @@ __global__ __launch_bounds__(NT) void k_dc_defect(Grid g, SplitGrid s, Metrics m
       const int dkp = (k == g.km) ? 0 : sk, dkm = (k == 1) ? 0 : -sk;
       float cn;
       if (SEP)
-        cn = 1.0f / ((M1(m.cnax, i) + M1(m.cnby, j)) + M1(m.cncz, k));  // init.f95:100-107
+        cn = rcp_rn((M1(m.cnax, i) + M1(m.cnby, j)) + M1(m.cncz, k));  // init.f95:100-107
       else
         cn = cn1f[c];
       const double pc = P64[c];

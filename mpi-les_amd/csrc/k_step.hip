@@ -996,6 +996,50 @@ void launch_masks(const DevDomain& d, const float* zbm, const float* top) {
                      d.F.dmask);
 }
 
+// ------------------------------------------------------------- self-test
+// Counts the x with bit patterns in [lo, hi) whose rcp_rn(x) differs from
+// the correctly rounded 1.0f/x (one vector atomic per block with a miss).
+__global__ __launch_bounds__(256) void k_rcp_check(unsigned lo, unsigned hi, unsigned* __restrict__ miss,
+                                                  unsigned* __restrict__ first) {
+  __shared__ unsigned cnt;
+  if (threadIdx.x == 0) cnt = 0;
+  __syncthreads();
+  const unsigned long long stride = (unsigned long long)gridDim.x * 256;
+  unsigned c = 0, f = 0xffffffffu;
+  for (unsigned long long b = lo + (unsigned long long)blockIdx.x * 256 + threadIdx.x; b < hi; b += stride) {
+    const float x = __uint_as_float((unsigned)b);
+    const float a = rcp_rn(x), e = 1.0f / x;
+    if (__float_as_uint(a) != __float_as_uint(e)) {
+      c++;
+      if ((unsigned)b < f) f = (unsigned)b;
+    }
+  }
+  if (c) {
+    atomicAdd(&cnt, c);
+    atomicMin(first, f);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0 && cnt) atomicAdd(miss, cnt);
+}
+
+int selftest_reciprocal(unsigned lo, unsigned hi, unsigned long long* misses, unsigned* first_bits) {
+  unsigned* buf = nullptr;
+  if (hipMalloc(&buf, 2 * sizeof(unsigned)) != hipSuccess) return 1;
+  const unsigned init[2] = {0u, 0xffffffffu};
+  hipError_t e = hipMemcpy(buf, init, sizeof(init), hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_rcp_check, dim3(8192), dim3(256), 0, 0, lo, hi, buf, buf + 1);
+    e = hipGetLastError();
+  }
+  unsigned out[2] = {0u, 0u};
+  if (e == hipSuccess) e = hipMemcpy(out, buf, sizeof(out), hipMemcpyDeviceToHost);
+  hipFree(buf);
+  if (e != hipSuccess) return 1;
+  *misses = out[0];
+  *first_bits = out[1];
+  return 0;
+}
+
 // ------------------------------------------------------ layout conversion
 __global__ __launch_bounds__(256) void k_unpack(Grid g, const float* __restrict__ src,
                                                 float* __restrict__ dst, int li, int lj, int lk,

@@ -66,6 +66,18 @@ struct Ctrl {
 
 constexpr int kWave = 64;
 
+// 1/x correctly rounded for positive normal x whose reciprocal is normal:
+// the hardware reciprocal (within 1 ulp) refined by one Newton step with
+// fused multiply-adds (the residual 1 - x*r is exact in FMA).  It equals
+// the correctly rounded division 1.0f/x bit for bit over the whole range
+// [2^-125, 2^125); les_selftest_reciprocal checks every such x on the
+// device.
+__device__ __forceinline__ float rcp_rn(float x) {
+  const float r = __builtin_amdgcn_rcpf(x);
+  const float e = __builtin_fmaf(-x, r, 1.0f);
+  return __builtin_fmaf(r, e, r);
+}
+
 // The levels lo..hi are split into gridDim.z contiguous runs; block z gets
 // [k0, k1] (empty when k0 > k1).  A thread walking consecutive levels finds
 // the level-(k-1) and level-(k+1) neighbours of level k in its own CU's

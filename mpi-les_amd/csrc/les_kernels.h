@@ -93,6 +93,7 @@ void launch_velfg(const DevDomain& d, float vn);
 void launch_feedbf(const DevDomain& d, float alpha, float dt, float beta);
 void launch_velfg_feedbf(const DevDomain& d, float vn, float alpha, float dt, float beta);
 void launch_les_adam(const DevDomain& d);
+int selftest_reciprocal(unsigned lo, unsigned hi, unsigned long long* misses, unsigned* first_bits);
 void launch_masks(const DevDomain& d, const float* zbm, const float* top);
 void launch_les(const DevDomain& d);
 void launch_adam(const DevDomain& d);

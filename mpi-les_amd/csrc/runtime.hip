@@ -2,6 +2,7 @@
 // host arrays handed to the reference-signature entry points, layout
 // conversion, and the C ABI declared in include/les_amd.h.
 #include <math.h>
+#include <cmath>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -202,9 +203,35 @@ void derive(les_domain* D);
 // reading 4 bytes per cell per sweep.
 int env_int(const char* name, int dflt);
 
+// The kernels recompute cn1 with rcp_rn, which equals the correctly rounded
+// 1.0f/x for x in [2^-125, 2^125) (checked exhaustively by
+// les_selftest_reciprocal).  Rounded FP32 addition is monotone, so bounding
+// the per-axis minima and maxima bounds every (cnax+cnby)+cncz.
+bool cn_rcp_range_ok(const les_domain* D) {
+  const Grid& g = D->d.g;
+  const float* ax = D->mhost.data() + D->moff[MCNAX] + 1;
+  const float* by = D->mhost.data() + D->moff[MCNBY] + 1;
+  const float* cz = D->mhost.data() + D->moff[MCNCZ] + 1;
+  auto lim = [](const float* a, int n, float* lo, float* hi) {
+    *lo = a[1];
+    *hi = a[1];
+    for (int x = 1; x <= n; x++) {
+      if (!(a[x] >= 0.0f) || !std::isfinite(a[x])) return false;  // NaN, negative or infinite
+      *lo = std::min(*lo, a[x]);
+      *hi = std::max(*hi, a[x]);
+    }
+    return true;
+  };
+  float l0, h0, l1, h1, l2, h2;
+  if (!lim(ax, g.im, &l0, &h0) || !lim(by, g.jm, &l1, &h1) || !lim(cz, g.km, &l2, &h2)) return false;
+  const float lo = (l0 + l1) + l2, hi = (h0 + h1) + h2;
+  return lo >= 0x1p-125f && hi < 0x1p125f;
+}
+
 void check_cn1(les_domain* D, const float* host) {
   derive(D);
-  if (env_int("LES_SOR_CN1_LOAD", 0)) {  // read cn1 in the SOR even when it could be recomputed
+  if (env_int("LES_SOR_CN1_LOAD", 0) || !cn_rcp_range_ok(D)) {
+    // read cn1 in the SOR: asked for, or its recomputation would not be exact
     D->d.cn1_separable = 0;
     return;
   }
@@ -342,6 +369,7 @@ void timer_free(les_domain* D) {
 
 void domain_free(les_domain* D) {
   if (!D) return;
+  if (D->d.stream) hipStreamSynchronize(D->d.stream);  // nothing of this domain may still run
   timer_free(D);
   if (D->gexec) hipGraphExecDestroy(D->gexec);
   for (int f = 0; f < NFIELDS; f++)
@@ -885,6 +913,8 @@ int les_domain_create(const les_domain_desc* s, les_domain** out) {
   } else {
     derive(D);
     D->d.cn1_separable = 1;  // init.f95:100-107 by definition
+    if (!cn_rcp_range_ok(D))
+      rc |= fail("cn1 terms out of the exact-reciprocal range [2^-125, 2^125): pass cn1 explicitly");
   }
   const bool gen_masks = !s->bmask1 && !s->cmask1 && !s->dmask1;
   if (gen_masks && !(s->zbm && s->z2)) rc |= fail("masks: give bmask1/cmask1/dmask1, or zbm and z2");
@@ -941,6 +971,14 @@ int les_domain_set_press_mode(les_domain* D, int mode, int outer, int inner) {
   return 0;
 }
 
+int les_selftest_reciprocal(unsigned lo_bits, unsigned hi_bits, unsigned long long* misses,
+                            unsigned* first_bits) {
+  if (!misses || !first_bits) return fail("null argument");
+  if (lo_bits >= hi_bits) return fail("empty bit range");
+  if (selftest_reciprocal(lo_bits, hi_bits, misses, first_bits)) return fail("reciprocal self-test failed to run");
+  return 0;
+}
+
 int les_domain_query(les_domain* D) {
   if (!D) {
     fail("null domain");
@@ -984,6 +1022,7 @@ static int step_graph(les_domain* D, int n, int nmax) {
   if (flush_metrics(D)) return 1;
   const float sc[5] = {D->ro, D->vn, D->alpha, D->beta, D->dt};
   if (D->gexec && (memcmp(&D->gkey, &D->d, sizeof(DevDomain)) != 0 || memcmp(D->gscal, sc, sizeof(sc)) != 0)) {
+    HIP_TRY(hipStreamSynchronize(D->d.stream));  // the previous replay may still be running
     hipGraphExecDestroy(D->gexec);
     D->gexec = nullptr;
   }
