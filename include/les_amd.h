@@ -97,6 +97,30 @@ int les_press(int km, int jm, int im, float* rhs, const float* u, const float* d
  * reference's inflow is a literal. */
 int les_bondv1_set_inflow(int im, int jm, int km, const float* prof);
 
+/* The inflow planes of bondv1_data24 (bondv1_data24.f95:6, which reads
+ * them from its data24_out<n+48000>.dat records, :43-68, and sets
+ * u,v,w(i=0..1,j,k) from them, :79-87; the rest of that subroutine is
+ * bondv1's).  u_in, v_in, w_in are (j,k), j=1..jm fastest then k=1..km;
+ * v_in/w_in NULL mean 0, u_in NULL restores the 5 m/s of bondv1.  The next
+ * les_bondv1 call on a domain of these extents uses them: setting the
+ * planes each step and calling les_bondv1 is bondv1_data24. */
+int les_bondv1_set_inflow_uvw(int im, int jm, int km, const float* u_in, const float* v_in,
+                              const float* w_in);
+
+/* aveflow.f95:5-6  subroutine aveflow(n,n1,km,jm,im,aveu,avev,avew,avep,avel,aveuu,avevv,
+ * aveww,avesm,avesmsm,uwfx,avesu,...,amask1): the time statistics.
+ * Arguments the statistics never touch (avesu..avesww, uwfxs, time,
+ * amask1) and the file names data10..data14 are not passed.  The sums live
+ * on the device; at n == nmax the final means, deviations and u-w flux are
+ * copied into the host arrays, from which the caller writes data10/data11
+ * (aveflow.f95:207-345).  Before that the host arrays are stale (les_sync_host
+ * fetches them).  Layouts: aveu, avev (1:im,1:jm,0:km); avew
+ * (1:im+1,1:jm,0:km+2); the rest (1:im,1:jm,1:km). */
+int les_aveflow(int n, int n1, int km, int jm, int im, float* aveu, float* avev, float* avew, float* avep,
+                float* avel, float* aveuu, float* avevv, float* aveww, float* avesm, float* avesmsm,
+                float* uwfx, const float* u, const float* v, const float* w, const float* p,
+                const float* sm, int nmax);
+
 /* SOR controls fixed by parameters in the reference (press.f95:43-45):
  * nmaxp = 50, pjuge = 1e-4, omega = 1.  Applies to domains created after
  * the call and to reference-signature domains. */
@@ -162,10 +186,20 @@ int les_domain_run(les_domain* d, int n0, int n1, int nmax);
 int les_domain_stage(les_domain* d, int stage, int n, int nmax);
 /* Field by reference name ("u","v","w","p","f","g","h","usum","vsum","wsum",
  * "fold","gold","hold","rhs","sm","uspd","vspd","diu1".."diu9","cn1",
- * "bmask1","cmask1","dmask1"), host array in reference layout. */
+ * "bmask1","cmask1","dmask1", and the aveflow statistics "aveu","avev",
+ * "avew","avep","avel","aveuu","avevv","aveww","avesm","avesmsm","uwfx"),
+ * host array in reference layout. */
 int les_domain_upload(les_domain* d, const char* field, const float* host);
 int les_domain_download(les_domain* d, const char* field, float* host);
 int les_domain_synchronize(les_domain* d);
+/* aveflow (aveflow.f95:80-203) on the domain's device state after step n:
+ * zero at n == n1, accumulate for n >= n1, finalise at n == nmax.  Results
+ * by les_domain_download("aveu", ...).  Does not synchronise. */
+int les_domain_aveflow(les_domain* d, int n, int n1, int nmax);
+/* Inflow planes for the domain's next steps, as les_bondv1_set_inflow_uvw.
+ * The copy is ordered on the domain's stream (a step already enqueued keeps
+ * the previous planes); the host arrays may be reused on return. */
+int les_domain_set_inflow(les_domain* d, const float* u_in, const float* v_in, const float* w_in);
 /* Pressure solver of the domain's press stage.  mode 0 (default): the
  * reference's SOR (press.f95:89-122).  mode 1: FP64 defect correction --
  * `outer` times, the defect d = (rhs - rhsav) - B(p) of the SOR's equation

@@ -55,7 +55,8 @@ void launch_velnw(const DevDomain& d, float ro, float dt) {
 
 // ----------------------------------------------------------------- bondv1
 // Inflow planes i=0,1 (bondv1.f95:33-54) and, at n==1, the copy of column
-// i=1 over i=2..im (bondv1.f95:61-88): u=u_in(j,k), v=w=0 for i=0..ilast.
+// i=1 over i=2..im (bondv1.f95:61-88): u=u_in(j,k), v=v_in, w=w_in (zero
+// unless set) for i=0..ilast.
 // Also clears the outflow max/min accumulators.
 __global__ __launch_bounds__(NT) void k_bondv1_inflow(Grid g, Metrics m, float* __restrict__ u,
                                                       float* __restrict__ v, float* __restrict__ w,
@@ -67,11 +68,11 @@ __global__ __launch_bounds__(NT) void k_bondv1_inflow(Grid g, Metrics m, float* 
   const int i = blockIdx.x * BX + threadIdx.x, j = 1 + blockIdx.y * BY + threadIdx.y;
   const int k = 1 + blockIdx.z;
   if (i > ilast || j > g.jm) return;
-  const float uin = m.inflow ? m.inflow[(j - 1) + (k - 1) * g.jm] : 5.0f;
+  const int q = (j - 1) + (k - 1) * g.jm;
   const int c = ix(g, i, j, k);
-  u[c] = uin;
-  v[c] = 0.0f;
-  w[c] = 0.0f;
+  u[c] = m.inflow ? m.inflow[q] : 5.0f;
+  v[c] = m.inflow_v ? m.inflow_v[q] : 0.0f;  // bondv1_data24.f95:79-87 sets v and w too
+  w[c] = m.inflow_w ? m.inflow_w[q] : 0.0f;
 }
 
 // aaa = max(0, max u(im,j,k)), bbb = min(0, min u(im,j,k)) (bondv1.f95:98-114).

@@ -42,7 +42,9 @@ struct Metrics {
   const float* cnax;    // 2/(dxs(i-1)*dxs(i)), the three terms of cn1 (init.f95:100-107)
   const float* cnby;    // 2/(dys(j-1)*dys(j))
   const float* cncz;    // 2/(dzs(k-1)*dzs(k))
-  const float* inflow;  // u_in(j,k) at (j-1)+(k-1)*jm, nullptr = 5 m/s
+  const float* inflow;    // u_in(j,k) at (j-1)+(k-1)*jm, nullptr = 5 m/s
+  const float* inflow_v;  // v_in(j,k), nullptr = 0 (bondv1_data24.f95:79-87 reads all three)
+  const float* inflow_w;  // w_in(j,k), nullptr = 0
 };
 #define M1(a, x) ((a)[(x) + 1])
 

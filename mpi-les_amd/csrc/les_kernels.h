@@ -99,6 +99,15 @@ void launch_les(const DevDomain& d);
 void launch_adam(const DevDomain& d);
 void launch_press(const DevDomain& d, float dt);
 
+// The time statistics of aveflow (aveflow.f95:8-23), device layout.
+struct AveBufs {
+  float *aveu, *avev, *avew, *avep, *avel;
+  float *aveuu, *avevv, *aveww, *avesm, *avesmsm, *uwfx;
+};
+// One aveflow call (aveflow.f95:82-197): zero at n == n1, accumulate for
+// n >= n1, turn sums into means, deviations and the u-w flux at n == nmax.
+void launch_aveflow(const DevDomain& d, const AveBufs& a, int n, int n1, int nmax);
+
 // layout conversion between a packed reference-layout staging buffer and
 // the device layout: the reference array covers [lo, lo+n) per axis
 void launch_unpack(const DevDomain& d, const float* src, float* dst, int li, int lj, int lk, int ni,

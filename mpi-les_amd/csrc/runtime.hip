@@ -36,7 +36,7 @@ int fail(const std::string& msg) {
   } while (0)
 
 // ------------------------------------------------------------ field table
-enum Kind { KU, KW, KP, KF, KI, KR, KS, KD1, KD, KB, K2D };
+enum Kind { KU, KW, KP, KF, KI, KR, KS, KD1, KD, KB, K2D, KAU, KAW };
 
 struct KindInfo {
   int li, lj, lk;    // reference lower bounds
@@ -55,13 +55,18 @@ const KindInfo kKinds[] = {
     {0, 0, 0, 3, 3, 3},     // KD  other diu      (0:ip+2,0:jp+2,0:kp+2)
     {-1, 0, 0, 3, 2, 2},    // KB  bmask1         (-1:ip+1,0:jp+1,0:kp+1)
     {0, 0, -1, 2, 2, -1},   // K2D uspd, vspd     (0:ip+1,0:jp+1), stored in plane k=-1
+    {1, 1, 0, 0, 0, 1},     // KAU aveu, avev     (ip,jp,0:kp)        aveflow.f95:18,20
+    {1, 1, 0, 1, 0, 3},     // KAW avew           (ip+1,jp,0:kp+2)    aveflow.f95:22
 };
 
 enum Field {
   FU, FV, FW, FP, FF, FG, FH, FUSUM, FVSUM, FWSUM, FFOLD, FGOLD, FHOLD, FRHS, FSM, FUSPD, FVSPD,
   FDIU1, FDIU2, FDIU3, FDIU4, FDIU5, FDIU6, FDIU7, FDIU8, FDIU9, FBMASK, FCMASK, FDMASK, FCN1,
+  // the time statistics of aveflow (aveflow.f95:8-23), allocated on first use
+  FAVEU, FAVEV, FAVEW, FAVEP, FAVEL, FAVEUU, FAVEVV, FAVEWW, FAVESM, FAVESMSM, FUWFX,
   NFIELDS
 };
+constexpr int kLazyFirst = FAVEU;  // fields from here on are allocated when first used
 struct FieldInfo {
   const char* name;
   Kind kind;
@@ -73,6 +78,9 @@ const FieldInfo kFields[NFIELDS] = {
     {"uspd", K2D},   {"vspd", K2D},   {"diu1", KD1},   {"diu2", KD},    {"diu3", KD},
     {"diu4", KD},    {"diu5", KD1},   {"diu6", KD},    {"diu7", KD},    {"diu8", KD},
     {"diu9", KD},    {"bmask1", KB},  {"cmask1", KU},  {"dmask1", KR},  {"cn1", KI},
+    {"aveu", KAU},   {"avev", KAU},   {"avew", KAW},   {"avep", KI},    {"avel", KI},
+    {"aveuu", KI},   {"avevv", KI},   {"aveww", KI},   {"avesm", KI},   {"avesmsm", KI},
+    {"uwfx", KI},
 };
 
 int field_by_name(const char* n) {
@@ -252,7 +260,25 @@ void check_cn1(les_domain* D, const float* host) {
   D->d.cn1_separable = 1;
 }
 
+// Fields allocated on first use start at zero, as the reference's static
+// arrays do.
+int ensure_field(les_domain* D, int f) {
+  if (D->fields[f]) return 0;
+  const size_t nb = (size_t)D->n3 * sizeof(float);
+  float* a = nullptr;
+  HIP_TRY(hipMalloc(&a, nb));
+  hipError_t e = hipMemsetAsync(a, 0, nb, D->d.stream);
+  if (e != hipSuccess) {
+    hipFree(a);
+    return fail(std::string("field allocation: ") + hipGetErrorString(e));
+  }
+  D->fields[f] = a;
+  D->bytes += nb;
+  return 0;
+}
+
 int upload_field(les_domain* D, int f, const float* host) {
+  if (ensure_field(D, f)) return 1;
   if (f == FCN1) check_cn1(D, host);
   const Kind k = kFields[f].kind;
   int ni, nj, nk;
@@ -268,6 +294,7 @@ int upload_field(les_domain* D, int f, const float* host) {
 }
 
 int download_field(les_domain* D, int f, float* host) {
+  if (ensure_field(D, f)) return 1;
   const Kind k = kFields[f].kind;
   int ni, nj, nk;
   ref_extent(D, k, &ni, &nj, &nk);
@@ -336,17 +363,34 @@ int flush_metrics(les_domain* D) {
   return 0;
 }
 
-int set_inflow(les_domain* D, const float* prof) {
+// Inflow planes u_in, v_in, w_in (j,k), j fastest: u_in alone replaces
+// bondv1's literal (bondv1.f95:33-54, v = w = 0); all three are the planes
+// bondv1_data24 reads from its files (bondv1_data24.f95:62-87).  The copy is
+// ordered on the domain's stream, so a step already enqueued still sees the
+// previous planes.
+int set_inflow(les_domain* D, const float* u_in, const float* v_in = nullptr, const float* w_in = nullptr) {
   const Grid& g = D->d.g;
-  if (!prof) {
+  Metrics& m = D->d.m;
+  if (!u_in) {
+    if (v_in || w_in) return fail("inflow: v and w planes need a u plane");
     D->have_inflow = false;
-    D->d.m.inflow = nullptr;
+    m.inflow = m.inflow_v = m.inflow_w = nullptr;
     return 0;
   }
-  if (!D->inflow_dev) HIP_TRY(hipMalloc(&D->inflow_dev, (size_t)g.jm * g.km * sizeof(float)));
-  HIP_TRY(hipMemcpy(D->inflow_dev, prof, (size_t)g.jm * g.km * sizeof(float), hipMemcpyHostToDevice));
+  const size_t plane = (size_t)g.jm * g.km;
+  if (!D->inflow_dev) {
+    HIP_TRY(hipMalloc(&D->inflow_dev, 3 * plane * sizeof(float)));
+    D->bytes += 3 * plane * sizeof(float);
+  }
+  const float* src[3] = {u_in, v_in, w_in};
+  for (int c = 0; c < 3; c++)
+    if (src[c])
+      HIP_TRY(hipMemcpyAsync(D->inflow_dev + c * plane, src[c], plane * sizeof(float), hipMemcpyHostToDevice,
+                             D->d.stream));
   D->have_inflow = true;
-  D->d.m.inflow = D->inflow_dev;
+  m.inflow = D->inflow_dev;
+  m.inflow_v = v_in ? D->inflow_dev + plane : nullptr;
+  m.inflow_w = w_in ? D->inflow_dev + 2 * plane : nullptr;
   return 0;
 }
 
@@ -408,7 +452,7 @@ int domain_alloc(int im, int jm, int km, les_domain** out) {
   };
   hipError_t e = hipStreamCreateWithFlags(&D->d.stream, hipStreamNonBlocking);
   if (e != hipSuccess) return bail(std::string("hipStreamCreate: ") + hipGetErrorString(e));
-  for (int f = 0; f < NFIELDS; f++) {
+  for (int f = 0; f < kLazyFirst; f++) {
     const size_t nb = (kFields[f].kind == K2D ? (size_t)sk : (size_t)n3) * sizeof(float);
     e = hipMalloc(&D->fields[f], nb);
     if (e == hipSuccess) e = hipMemset(D->fields[f], 0, nb);
@@ -588,10 +632,36 @@ int run_stage(les_domain* D, int s, int n, int nmax) {
   return 0;
 }
 
+// One aveflow call on the device statistics (allocated, zero, on first use).
+int run_aveflow(les_domain* D, int n, int n1, int nmax) {
+  for (int f = FAVEU; f <= FUWFX; f++)
+    if (ensure_field(D, f)) return 1;
+  AveBufs a;
+  a.aveu = D->fields[FAVEU];
+  a.avev = D->fields[FAVEV];
+  a.avew = D->fields[FAVEW];
+  a.avep = D->fields[FAVEP];
+  a.avel = D->fields[FAVEL];
+  a.aveuu = D->fields[FAVEUU];
+  a.avevv = D->fields[FAVEVV];
+  a.aveww = D->fields[FAVEWW];
+  a.avesm = D->fields[FAVESM];
+  a.avesmsm = D->fields[FAVESMSM];
+  a.uwfx = D->fields[FUWFX];
+  launch_aveflow(D->d, a, n, n1, nmax);
+  HIP_TRY(hipGetLastError());
+  return 0;
+}
+
 // -------------------------------------------- reference-signature layer
 std::mutex g_mu;
 std::map<std::tuple<int, int, int>, les_domain*> g_domains;
-std::map<std::tuple<int, int, int>, std::vector<float>> g_inflows;
+struct InflowPlanes {
+  std::vector<float> u, v, w;  // v, w empty: zero
+};
+std::map<std::tuple<int, int, int>, InflowPlanes> g_inflows;
+
+const float* opt(const std::vector<float>& a) { return a.empty() ? nullptr : a.data(); }
 
 int ref_domain(int im, int jm, int km, les_domain** out) {
   auto key = std::make_tuple(im, jm, km);
@@ -603,7 +673,7 @@ int ref_domain(int im, int jm, int km, les_domain** out) {
   les_domain* D = nullptr;
   if (domain_alloc(im, jm, km, &D)) return 1;
   auto pit = g_inflows.find(key);
-  if (pit != g_inflows.end() && set_inflow(D, pit->second.data())) {
+  if (pit != g_inflows.end() && set_inflow(D, opt(pit->second.u), opt(pit->second.v), opt(pit->second.w))) {
     domain_free(D);
     return 1;
   }
@@ -791,16 +861,43 @@ int les_press(int km, int jm, int im, float* rhs, const float* u, const float* d
   return run_stage(D, 6, n, nmax);
 }
 
-int les_bondv1_set_inflow(int im, int jm, int km, const float* prof) {
+int les_aveflow(int n, int n1, int km, int jm, int im, float* aveu, float* avev, float* avew, float* avep,
+                float* avel, float* aveuu, float* avevv, float* aveww, float* avesm, float* avesmsm,
+                float* uwfx, const float* u, const float* v, const float* w, const float* p,
+                const float* sm, int nmax) {
+  REF_DOMAIN(im, jm, km);
+  float* st[11] = {aveu, avev, avew, avep, avel, aveuu, avevv, aveww, avesm, avesmsm, uwfx};
+  for (int q = 0; q < 11; q++) BIND(FAVEU + q, st[q]);
+  BIND(FU, u);
+  BIND(FV, v);
+  BIND(FW, w);
+  BIND(FP, p);
+  BIND(FSM, sm);
+  if (run_aveflow(D, n, n1, nmax)) return 1;
+  if (n == nmax)  // the caller writes data10/data11 from these (aveflow.f95:207-345)
+    for (int q = 0; q < 11; q++)
+      if (download_field(D, FAVEU + q, st[q])) return 1;
+  return 0;
+}
+
+int les_bondv1_set_inflow_uvw(int im, int jm, int km, const float* u_in, const float* v_in,
+                              const float* w_in) {
+  if (!u_in && (v_in || w_in)) return fail("inflow: v and w planes need a u plane");
   std::lock_guard<std::mutex> lk(g_mu);
   auto key = std::make_tuple(im, jm, km);
-  if (prof)
-    g_inflows[key] = std::vector<float>(prof, prof + (size_t)jm * km);
+  const size_t n = (size_t)jm * km;
+  auto copy = [n](const float* a) { return a ? std::vector<float>(a, a + n) : std::vector<float>(); };
+  if (u_in)
+    g_inflows[key] = InflowPlanes{copy(u_in), copy(v_in), copy(w_in)};
   else
     g_inflows.erase(key);
   auto it = g_domains.find(key);
-  if (it != g_domains.end()) return set_inflow(it->second, prof);
+  if (it != g_domains.end()) return set_inflow(it->second, u_in, v_in, w_in);
   return 0;
+}
+
+int les_bondv1_set_inflow(int im, int jm, int km, const float* prof) {
+  return les_bondv1_set_inflow_uvw(im, jm, km, prof, nullptr, nullptr);
 }
 
 int les_press_config(int nmaxp, float pjuge, float omega) {
@@ -1056,6 +1153,16 @@ int les_domain_step(les_domain* D, int n, int nmax) {
   if (!D) return fail("null domain");
   if (D->use_graph && n != 1 && !D->d.timer) return step_graph(D, n, nmax);
   return run_step(D, n, nmax);
+}
+
+int les_domain_aveflow(les_domain* D, int n, int n1, int nmax) {
+  if (!D) return fail("null domain");
+  return run_aveflow(D, n, n1, nmax);
+}
+
+int les_domain_set_inflow(les_domain* D, const float* u_in, const float* v_in, const float* w_in) {
+  if (!D) return fail("null domain");
+  return set_inflow(D, u_in, v_in, w_in);
 }
 
 int les_domain_set_graph(les_domain* D, int on) {

@@ -3,7 +3,8 @@
 ! Each module keeps the reference's module name, subroutine name and
 ! argument list, so main.f95 compiles unchanged against this file in place of
 ! velnw.f95, bondv1.f95, velFG.f95 (+vel2.f95), feedbf.f95, les.f95
-! (+boundsm.f95), adam.f95 and press.f95 (+bondFG.f95, boundp.f95).  Every
+! (+boundsm.f95), adam.f95 and press.f95 (+bondFG.f95, boundp.f95), and of
+! the step's neighbours bondv1_data24.f95 and aveflow.f95.  Every
 ! subroutine forwards its arguments through iso_c_binding to the gfx950
 ! library (include/les_amd.h), which keeps a device copy of each array it is
 ! handed: the host arrays are read once, on the first call that sees them,
@@ -22,6 +23,11 @@
 !          mod(n,1000)==0 or n==nmax (press.f95:197-217); the VERBOSE
 !          convergence/profile/CFL prints every 20 steps (press.f95:164-195)
 ! The dumps are compiled out with -DNO_IO or -DMPI, as in the reference.
+!   aveflow        data10 (aveu, avew, avev) and data11 (aveuu, aveww, avevv,
+!                  uwfx) at n==nmax (aveflow.f95:207-345); the sums stay on
+!                  the device until then
+!   bondv1_data24  reads the inflow records data24_out<n+48000>.dat
+!                  (bondv1_data24.f95:43-59) on the host each step
 ! Unformatted records are big-endian, as the reference's build makes them
 ! (-fconvert=big-endian, SConstruct:77), whatever the host program's own
 ! flags: the OPEN statements say so (convert=, supported by flang and
@@ -132,6 +138,21 @@ module les_amd_c
             real(c_float) :: prof(*)
         end function
 
+        integer(c_int) function les_bondv1_set_inflow_uvw(im,jm,km,u_in,v_in,w_in) &
+                bind(C, name='les_bondv1_set_inflow_uvw')
+            import :: c_int, c_float
+            integer(c_int), value :: im, jm, km
+            real(c_float) :: u_in(*), v_in(*), w_in(*)
+        end function
+
+        integer(c_int) function les_aveflow(n,n1,km,jm,im,aveu,avev,avew,avep,avel,aveuu,avevv,aveww, &
+                avesm,avesmsm,uwfx,u,v,w,p,sm,nmax) bind(C, name='les_aveflow')
+            import :: c_int, c_float
+            integer(c_int), value :: n, n1, km, jm, im, nmax
+            real(c_float) :: aveu(*), avev(*), avew(*), avep(*), avel(*), aveuu(*), avevv(*), aveww(*), &
+                avesm(*), avesmsm(*), uwfx(*), u(*), v(*), w(*), p(*), sm(*)
+        end function
+
         integer(c_int) function les_press_config(nmaxp,pjuge,omega) bind(C, name='les_press_config')
             import :: c_int, c_float
             integer(c_int), value :: nmaxp
@@ -222,6 +243,77 @@ contains
         call les_check(les_bondv1(jm,u,z2,dzn,v,w,km,n,im,dt,dxs), 'bondv1')
     end subroutine bondv1
 end module module_bondv1
+
+
+! bondv1_data24.f95:6: bondv1 with its inflow planes read from the records
+! of data24_out<n+48000>.dat, unformatted direct access, 1200 values per
+! record (recl=4*1200): km records of u, then km of w, then km of v
+! (bondv1_data24.f95:43-59); the planes are u,w,v(400+j,k) for j=1..jm
+! (:62-68).  The reference opens a fixed absolute path; here the directory
+! is data24_dir, which the host program may set, and otherwise comes from
+! the environment variable LES_DATA24_DIR (empty: the working directory).
+module module_bondv1_data24
+    implicit none
+    character(len=512), save :: data24_dir = ''
+    logical, save :: data24_dir_known = .false.
+contains
+    subroutine bondv1_data24(jm,u,z2,dzn,v,w,km,n,im,dt,dxs)
+        use les_amd_c
+        integer, intent(In) :: jm, km, n, im
+        real(kind=4), intent(In) :: dt
+        real(kind=4), dimension(0:im+1,-1:jm+1,0:km+1), intent(InOut) :: u, v
+        real(kind=4), dimension(0:im+1,-1:jm+1,-1:km+1), intent(InOut) :: w
+        real(kind=4), dimension(km+2), intent(In) :: z2
+        real(kind=4), dimension(-1:km+2), intent(In) :: dzn
+        real(kind=4), dimension(0:im), intent(In) :: dxs
+        real(kind=4), allocatable :: ub(:,:), vb(:,:), wb(:,:), ua(:,:), va(:,:), wa(:,:)
+        character(len=600) :: filename
+        integer :: irec, j, k, unit, ios
+        if (jm > 800) then
+            write(0,'(a)') 'les_amd: bondv1_data24: jm > 800 does not fit the 1200-value data24 records'
+            error stop 1
+        end if
+        if (.not. data24_dir_known) then
+            if (len_trim(data24_dir) == 0) call get_environment_variable('LES_DATA24_DIR', data24_dir)
+            data24_dir_known = .true.
+        end if
+        if (len_trim(data24_dir) > 0) then
+            write(filename,'(a,"/data24_out",i6.6,".dat")') trim(data24_dir), n+48000
+        else
+            write(filename,'("data24_out",i6.6,".dat")') n+48000
+        end if
+        allocate(ub(1200,km), vb(1200,km), wb(1200,km), ua(jm,km), va(jm,km), wa(jm,km))
+        open(newunit=unit, file=trim(filename), form='unformatted', status='old', access='direct', &
+             recl=4*1200, convert='BIG_ENDIAN', iostat=ios)
+        if (ios /= 0) then
+            write(0,'(a,a)') 'les_amd: bondv1_data24: cannot open ', trim(filename)
+            error stop 1
+        end if
+        irec = 1
+        do k = 1, km
+            read(unit, rec=irec) (ub(j,k), j=1,1200)
+            irec = irec + 1
+        end do
+        do k = 1, km
+            read(unit, rec=irec) (wb(j,k), j=1,1200)
+            irec = irec + 1
+        end do
+        do k = 1, km
+            read(unit, rec=irec) (vb(j,k), j=1,1200)
+            irec = irec + 1
+        end do
+        close(unit)
+        do k = 1, km
+            do j = 1, jm
+                ua(j,k) = ub(400+j,k)
+                va(j,k) = vb(400+j,k)
+                wa(j,k) = wb(400+j,k)
+            end do
+        end do
+        call les_check(les_bondv1_set_inflow_uvw(im,jm,km,ua,va,wa), 'bondv1_data24')
+        call les_check(les_bondv1(jm,u,z2,dzn,v,w,km,n,im,dt,dxs), 'bondv1_data24')
+    end subroutine bondv1_data24
+end module module_bondv1_data24
 
 
 module module_velFG
@@ -419,3 +511,49 @@ contains
         end if
     end subroutine press
 end module module_press
+
+
+! aveflow.f95:5-6: time statistics of u, v, w, p, sm from step n1 on.  The
+! sums live on the device (les_aveflow); at n == nmax the library returns the
+! means, deviations and u-w flux in the host arrays and data10/data11 are
+! written as aveflow.f95:207-345 writes them.  The statistics arrays are
+! InOut here: the reference declares them intent(Out) but accumulates into
+! them across calls.  avesu..avesww, uwfxs, time, amask1 and data13/data14
+! are not used by the reference's statistics either.
+module module_aveflow
+contains
+    subroutine aveflow(n,n1,km,jm,im,aveu,avev,avew,avep,avel,aveuu,avevv,aveww,avesm,avesmsm, &
+            uwfx,avesu,avesv,avesw,avesuu,avesvv,avesww,u,v,w,p,sm,nmax,uwfxs,data10,time,data11, &
+            data13,data14,amask1)
+        use les_amd_c
+        integer, intent(In) :: n, n1, km, jm, im, nmax
+        real(kind=4), dimension(im,jm,km), intent(InOut) :: avel, avep, avesm, avesmsm, aveuu, avevv, &
+            aveww, uwfx
+        real(kind=4), dimension(im,km), intent(InOut) :: avesu, avesuu, avesv, avesvv, avesw, avesww, uwfxs
+        real(kind=4), dimension(im,jm,0:km), intent(InOut) :: aveu, avev
+        real(kind=4), dimension(im+1,jm,0:km+2), intent(InOut) :: avew
+        character(len=70), intent(In) :: data10, data11, data13, data14
+        real(kind=4), intent(In) :: time
+        real(kind=4), dimension(0:im+2,0:jm+2,0:km+1), intent(In) :: p
+        real(kind=4), dimension(-1:im+1,-1:jm+1,0:km+1), intent(In) :: sm
+        real(kind=4), dimension(0:im+1,-1:jm+1,0:km+1), intent(In) :: u, v
+        real(kind=4), dimension(0:im+1,-1:jm+1,-1:km+1), intent(In) :: w
+        real(kind=4), dimension(0:im+1,0:jm+1,0:km+1), intent(In) :: amask1
+        integer :: i, j, k
+        call les_check(les_aveflow(n,n1,km,jm,im,aveu,avev,avew,avep,avel,aveuu,avevv,aveww,avesm, &
+                                   avesmsm,uwfx,u,v,w,p,sm,nmax), 'aveflow')
+        if (n == nmax) then
+            open(unit=10, file=data10, form='unformatted', status='unknown', convert='BIG_ENDIAN')
+            write(10) (((aveu(i,j,k),i=1,im),j=1,jm),k=1,km)
+            write(10) (((avew(i,j,k),i=1,im),j=1,jm),k=1,km)
+            write(10) (((avev(i,j,k),i=1,im),j=1,jm),k=1,km)
+            close(unit=10)
+            open(unit=11, file=data11, form='unformatted', status='unknown', convert='BIG_ENDIAN')
+            write(11) (((aveuu(i,j,k),i=1,im),j=1,jm),k=1,km)
+            write(11) (((aveww(i,j,k),i=1,im),j=1,jm),k=1,km)
+            write(11) (((avevv(i,j,k),i=1,im),j=1,jm),k=1,km)
+            write(11) (((uwfx(i,j,k),i=1,im),j=1,jm),k=1,km)
+            close(unit=11)
+        end if
+    end subroutine aveflow
+end module module_aveflow

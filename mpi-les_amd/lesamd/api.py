@@ -27,7 +27,7 @@ from . import layout
 LIB_PATH = Path(os.environ.get("LESAMD_LIBRARY") or Path(__file__).with_name("liblesamd.so"))
 _lib = None
 
-INT_ARGS = {"km", "jm", "im", "n", "nmax"}
+INT_ARGS = {"km", "jm", "im", "n", "nmax", "n1"}
 REAL_ARGS = {"ro", "dt", "vn", "alpha", "beta"}
 
 SIGNATURES = {
@@ -40,6 +40,8 @@ SIGNATURES = {
     "les": "km delx1 dx1 dy1 dzn jm im diu1 diu2 diu3 diu4 diu5 diu6 diu7 diu8 diu9 sm f g h uspd vspd dxs dys",
     "adam": "n nmax fold im jm km gold hold fghold f g h",
     "press": "km jm im rhs u dx1 v dy1 w dzn f g h dt cn1 cn2l p cn2s cn3l cn3s cn4l cn4s n nmax usum vsum wsum",
+    # aveflow.f95:5-6 without the arguments its statistics never touch
+    "aveflow": "n n1 km jm im aveu avev avew avep avel aveuu avevv aveww avesm avesmsm uwfx u v w p sm nmax",
 }
 SIGNATURES = {k: v.split() for k, v in SIGNATURES.items()}
 STAGES = ("velnw", "bondv1", "velfg", "feedbf", "les", "adam", "press")
@@ -74,6 +76,9 @@ def lib():
         f.restype = ctypes.c_int
     vp, ci, cf, cd = ctypes.c_void_p, ctypes.c_int, ctypes.c_float, ctypes.c_double
     L.les_bondv1_set_inflow.argtypes = [ci, ci, ci, vp]
+    L.les_bondv1_set_inflow_uvw.argtypes = [ci, ci, ci, vp, vp, vp]
+    L.les_domain_aveflow.argtypes = [vp, ci, ci, ci]
+    L.les_domain_set_inflow.argtypes = [vp, vp, vp, vp]
     L.les_press_config.argtypes = [ci, cf, cf]
     L.les_press_stats.argtypes = [ci, ci, ci, vp, vp, ci, vp, vp, vp]
     L.les_sync_host.argtypes = [vp]
@@ -149,6 +154,17 @@ def set_inflow(im, jm, km, prof):
     p = None if prof is None else np.asfortranarray(prof, dtype=np.float32)
     _check(lib().les_bondv1_set_inflow(im, jm, km, None if p is None else p.ctypes.data))
     return p
+
+
+def set_inflow_uvw(im, jm, km, u_in, v_in=None, w_in=None):
+    """Inflow planes (j,k) of bondv1_data24 for the next bondv1 calls on a
+    domain of these extents; returns the arrays handed over."""
+    planes = [None if a is None else np.asfortranarray(a, dtype=np.float32) for a in (u_in, v_in, w_in)]
+    for a in planes:
+        if a is not None and a.shape != (jm, km):
+            raise LesError(f"inflow plane of shape {a.shape}, expected {(jm, km)}")
+    _check(lib().les_bondv1_set_inflow_uvw(im, jm, km, *[None if a is None else a.ctypes.data for a in planes]))
+    return planes
 
 
 def press_config(nmaxp=50, pjuge=1e-4, omega=1.0):
@@ -235,6 +251,20 @@ class Domain:
 
     def synchronize(self):
         _check(lib().les_domain_synchronize(self._h))
+
+    def aveflow(self, n: int, n1: int, nmax: int):
+        """aveflow after step n on the device (zero at n1, accumulate from
+        n1, finalise at nmax); fetch with download("aveu") etc."""
+        _check(lib().les_domain_aveflow(self._h, n, n1, nmax))
+
+    def set_inflow(self, u_in, v_in=None, w_in=None):
+        """Inflow planes (jm, km) for the next steps (bondv1_data24's u, v, w)."""
+        w = self.workload
+        planes = [None if a is None else np.asfortranarray(a, dtype=np.float32) for a in (u_in, v_in, w_in)]
+        for a in planes:
+            if a is not None and a.shape != (w.jm, w.km):
+                raise LesError(f"inflow plane of shape {a.shape}, expected {(w.jm, w.km)}")
+        _check(lib().les_domain_set_inflow(self._h, *[None if a is None else a.ctypes.data for a in planes]))
 
     def set_press_mode(self, mode: int, outer: int = 4, inner: int = 25):
         """0: the reference's SOR; 1: FP64 defect correction (`outer` steps

@@ -27,6 +27,8 @@ KINDS = {
     "2D": ((0, 0), lambda ip, jp, kp: (ip + 2, jp + 2)),              # uspd, vspd
     "DU": ((0, 1, 1), lambda ip, jp, kp: (ip + 1, jp, kp)),           # dfu1
     "DV": ((1, 0, 1), lambda ip, jp, kp: (ip, jp + 1, kp)),           # dfv1
+    "AU": ((1, 1, 0), lambda ip, jp, kp: (ip, jp, kp + 1)),           # aveu, avev (aveflow.f95:18,20)
+    "AW": ((1, 1, 0), lambda ip, jp, kp: (ip + 1, jp, kp + 3)),       # avew (aveflow.f95:22)
 }
 
 FIELD_KIND = {
@@ -36,7 +38,12 @@ FIELD_KIND = {
     "fold": "I", "gold": "I", "hold": "I", "fghold": "I", "cn1": "I", "dfw1": "I",
     "rhs": "R", "amask1": "R", "dmask1": "R", "bmask1": "B", "cmask1": "U",
     "sm": "S", "uspd": "2D", "vspd": "2D", "dfu1": "DU", "dfv1": "DV",
+    "aveu": "AU", "avev": "AU", "avew": "AW",
 }
+# the time statistics of aveflow (aveflow.f95:8-23), in its argument order
+AVE_FIELDS = ("aveu", "avev", "avew", "avep", "avel", "aveuu", "avevv", "aveww", "avesm", "avesmsm", "uwfx")
+for _n in AVE_FIELDS[3:]:
+    FIELD_KIND[_n] = "I"
 for _n in range(1, 10):
     _k = "D1" if _n in (1, 5) else "D"
     for _p in ("diu", "cov", "nou"):

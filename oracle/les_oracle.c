@@ -9,10 +9,16 @@
  * (a+b)+c and a*b/c as (a*b)/c; x**2 is x*x and x**0.5 is sqrt(x) in the
  * flang build of the reference).
  *
- * Parity status: every function here is checked against the reference's
- * own subroutines built with flang (oracle/_ref) on seeded inputs, stage by
- * stage and over whole steps, by tests/test_oracle_vs_ref.py and the
- * committed fixtures under tests/golden/.
+ * Parity status: every hot-path function here is checked against the
+ * reference's own subroutines built with flang (oracle/_ref) on seeded
+ * inputs, stage by stage and over whole steps, by tests/test_oracle_vs_ref.py
+ * and the committed fixtures under tests/golden/.  aveflow and
+ * bondv1_data24 cannot be built from the reference without MPI (they call
+ * distributeaveu/distributebondu of communication_helper_real.f95); they
+ * are pinned instead by closed forms (means, population standard
+ * deviations and the u-w covariance of fields with known time series, and
+ * the inflow planes reducing to bondv1 with the same profile) in
+ * tests/test_oracle_outputs.py.
  */
 #include <math.h>
 #include <stddef.h>
@@ -95,6 +101,9 @@ void ORC(bondv1_set_inflow)(const real* prof, int jm, int km) {
  * is 5.0 for k=1..78 and u(i,j,77)=5.0 above, i.e. 5.0 at every level; a
  * profile set with bondv1_set_inflow replaces those values (the configured
  * log-law inflow), nothing else changes. */
+static void bondv1_after_inflow(int jm, real* u, real* v, real* w, int km, int n, int im, real dt,
+                                const real* dxs);
+
 void ORC(bondv1)(int jm, real* u, const real* z2, const real* dzn, real* v, real* w, int km, int n,
                  int im, real dt, const real* dxs) {
   lay LU = layU(im, jm), LW = layW(im, jm);
@@ -110,6 +119,35 @@ void ORC(bondv1)(int jm, real* u, const real* z2, const real* dzn, real* v, real
         AT(v, LU, i, j, k) = C(0.0);
         AT(w, LW, i, j, k) = C(0.0);
       }
+  bondv1_after_inflow(jm, u, v, w, km, n, im, dt, dxs);
+}
+
+/* bondv1_data24.f95:6-260 (one process: isMaster and isTopRow hold,
+ * jpmax = jp).  The file records (bondv1_data24.f95:43-59) are read by the
+ * caller; ua/va/wa(j,k), j fastest, hold ub/vb/wb(400+j,k) for j=1..jm
+ * (bondv1_data24.f95:62-68).  Inflow planes i=0,1 for k=1..km, j=1..jm
+ * (:79-87), then the rest of bondv1 unchanged (:124-258 repeat
+ * bondv1.f95:61-195). */
+void ORC(bondv1_data24)(int jm, real* u, const real* z2, const real* dzn, real* v, real* w, int km,
+                        int n, int im, real dt, const real* dxs, const real* ua, const real* va,
+                        const real* wa) {
+  lay LU = layU(im, jm), LW = layW(im, jm);
+  (void)z2;
+  (void)dzn;
+  for (int k = 1; k <= km; k++)
+    for (int j = 1; j <= jm; j++)
+      for (int i = 0; i <= 1; i++) {
+        const long q = (j - 1) + (long)(k - 1) * jm;
+        AT(u, LU, i, j, k) = ua[q];
+        AT(v, LU, i, j, k) = va[q];
+        AT(w, LW, i, j, k) = wa[q];
+      }
+  bondv1_after_inflow(jm, u, v, w, km, n, im, dt, dxs);
+}
+
+static void bondv1_after_inflow(int jm, real* u, real* v, real* w, int km, int n, int im, real dt,
+                                const real* dxs) {
+  lay LU = layU(im, jm), LW = layW(im, jm);
   if (n == 1) { /* bondv1.f95:61-88, ICAL == 0 */
     for (int k = 1; k <= km; k++)
       for (int j = 1; j <= jm; j++)
@@ -851,4 +889,99 @@ void ORC(press)(int km, int jm, int im, real* rhs, const real* u, const real* dx
   }
   ORC(boundp1)(km, jm, p, im); /* press.f95:155 */
   ORC(boundp2)(jm, im, p, km); /* press.f95:159 */
+}
+
+/* --------------------------------------------------------------- aveflow */
+/* aveflow.f95:5-354, the statistics part (one process, ipmax=ip, jpmax=jp;
+ * the data10/data11 writes at :207-345 are the caller's).  Layouts:
+ * aveu, avev (ip,jp,0:kp); avew (ip+1,jp,0:kp+2); every other statistic
+ * (ip,jp,kp).  avel is zeroed at n==n1 and never accumulated (:86). */
+void ORC(aveflow)(int n, int n1, int km, int jm, int im, real* aveu, real* avev, real* avew,
+                  real* avep, real* avel, real* aveuu, real* avevv, real* aveww, real* avesm,
+                  real* avesmsm, real* uwfx, const real* u, const real* v, const real* w,
+                  const real* p, const real* sm, int nmax) {
+  lay LU = layU(im, jm), LW = layW(im, jm), LP = layP(im, jm), LS = layS(im, jm), LI = layI(im, jm);
+  lay LA = {1, 1, 0, im, jm};      /* aveu, avev (ip,jp,0:kp) */
+  lay LAW = {1, 1, 0, im + 1, jm}; /* avew (ip+1,jp,0:kp+2) */
+  if (n == n1) { /* :80-113 */
+    for (int k = 1; k <= km; k++)
+      for (int j = 1; j <= jm; j++)
+        for (int i = 1; i <= im; i++) {
+          AT(avev, LA, i, j, k) = C(0.0);
+          AT(avep, LI, i, j, k) = C(0.0);
+          AT(avel, LI, i, j, k) = C(0.0);
+          AT(aveuu, LI, i, j, k) = C(0.0);
+          AT(avevv, LI, i, j, k) = C(0.0);
+          AT(aveww, LI, i, j, k) = C(0.0);
+          AT(avesm, LI, i, j, k) = C(0.0);
+          AT(avesmsm, LI, i, j, k) = C(0.0);
+          AT(uwfx, LI, i, j, k) = C(0.0);
+        }
+    for (int k = 0; k <= km; k++)
+      for (int j = 1; j <= jm; j++)
+        for (int i = 1; i <= im; i++) AT(aveu, LA, i, j, k) = C(0.0);
+    for (int k = 0; k <= km; k++)
+      for (int j = 1; j <= jm; j++)
+        for (int i = 1; i <= im + 1; i++) AT(avew, LAW, i, j, k) = C(0.0);
+  }
+  if (n >= n1) { /* :116-149 */
+    for (int k = 1; k <= km; k++)
+      for (int j = 1; j <= jm; j++)
+        for (int i = 1; i <= im; i++) {
+          const real uc = AT(u, LU, i, j, k), vc = AT(v, LU, i, j, k), wc = AT(w, LW, i, j, k);
+          const real s = AT(sm, LS, i, j, k);
+          AT(avev, LA, i, j, k) = AT(avev, LA, i, j, k) + vc;
+          AT(avep, LI, i, j, k) = AT(avep, LI, i, j, k) + AT(p, LP, i, j, k);
+          AT(aveuu, LI, i, j, k) = AT(aveuu, LI, i, j, k) + uc * uc;
+          AT(avevv, LI, i, j, k) = AT(avevv, LI, i, j, k) + vc * vc;
+          AT(aveww, LI, i, j, k) = AT(aveww, LI, i, j, k) + wc * wc;
+          AT(avesm, LI, i, j, k) = AT(avesm, LI, i, j, k) + s;
+          AT(avesmsm, LI, i, j, k) = AT(avesmsm, LI, i, j, k) + s * s;
+          AT(uwfx, LI, i, j, k) = AT(uwfx, LI, i, j, k) +
+                                  C(0.5) * (AT(u, LU, i, j, k - 1) + uc) * C(0.5) *
+                                      (AT(w, LW, i, j, k - 1) + AT(w, LW, i + 1, j, k - 1));
+        }
+    for (int k = 0; k <= km; k++)
+      for (int j = 1; j <= jm; j++)
+        for (int i = 1; i <= im; i++) AT(aveu, LA, i, j, k) = AT(aveu, LA, i, j, k) + AT(u, LU, i, j, k);
+    for (int k = 0; k <= km; k++)
+      for (int j = 1; j <= jm; j++)
+        for (int i = 1; i <= im + 1; i++)
+          AT(avew, LAW, i, j, k) = AT(avew, LAW, i, j, k) + AT(w, LW, i, j, k);
+  }
+  if (n == nmax) { /* :151-203 */
+    const real c = (real)(nmax - n1 + 1); /* float(nmax-n1+1) */
+    for (int k = 1; k <= km; k++)
+      for (int j = 1; j <= jm; j++)
+        for (int i = 1; i <= im; i++) {
+          AT(avev, LA, i, j, k) = AT(avev, LA, i, j, k) / c;
+          AT(avep, LI, i, j, k) = AT(avep, LI, i, j, k) / c;
+          AT(avel, LI, i, j, k) = AT(avel, LI, i, j, k) / c;
+          AT(aveuu, LI, i, j, k) = AT(aveuu, LI, i, j, k) / c;
+          AT(avevv, LI, i, j, k) = AT(avevv, LI, i, j, k) / c;
+          AT(aveww, LI, i, j, k) = AT(aveww, LI, i, j, k) / c;
+          AT(avesm, LI, i, j, k) = AT(avesm, LI, i, j, k) / c;
+          AT(avesmsm, LI, i, j, k) = AT(avesmsm, LI, i, j, k) / c;
+        }
+    for (int k = 0; k <= km; k++)
+      for (int j = 1; j <= jm; j++)
+        for (int i = 1; i <= im; i++) AT(aveu, LA, i, j, k) = AT(aveu, LA, i, j, k) / c;
+    for (int k = 0; k <= km; k++)
+      for (int j = 1; j <= jm; j++)
+        for (int i = 1; i <= im + 1; i++) AT(avew, LAW, i, j, k) = AT(avew, LAW, i, j, k) / c;
+    for (int k = 1; k <= km; k++)
+      for (int j = 1; j <= jm; j++)
+        for (int i = 1; i <= im; i++)
+          AT(uwfx, LI, i, j, k) = AT(uwfx, LI, i, j, k) / c -
+                                  C(0.5) * (AT(aveu, LA, i, j, k - 1) + AT(aveu, LA, i, j, k)) * C(0.5) *
+                                      (AT(avew, LAW, i, j, k - 1) + AT(avew, LAW, i + 1, j, k - 1));
+    for (int k = 1; k <= km; k++)
+      for (int j = 1; j <= jm; j++)
+        for (int i = 1; i <= im; i++) {
+          const real au = AT(aveu, LA, i, j, k), av = AT(avev, LA, i, j, k), aw = AT(avew, LAW, i, j, k);
+          AT(aveuu, LI, i, j, k) = SQRT(fabs(AT(aveuu, LI, i, j, k) - au * au));
+          AT(avevv, LI, i, j, k) = SQRT(fabs(AT(avevv, LI, i, j, k) - av * av));
+          AT(aveww, LI, i, j, k) = SQRT(fabs(AT(aveww, LI, i, j, k) - aw * aw));
+        }
+  }
 }

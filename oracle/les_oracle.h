@@ -29,6 +29,13 @@
   void P##bondv1(int jm, real* u, const real* z2, const real* dzn, real* v, real* w, int km, int n,  \
                  int im, real dt, const real* dxs);                                                 \
   void P##bondv1_set_inflow(const real* prof, int jm, int km);                                      \
+  void P##bondv1_data24(int jm, real* u, const real* z2, const real* dzn, real* v, real* w, int km,  \
+                        int n, int im, real dt, const real* dxs, const real* ua, const real* va,     \
+                        const real* wa);                                                            \
+  void P##aveflow(int n, int n1, int km, int jm, int im, real* aveu, real* avev, real* avew,        \
+                  real* avep, real* avel, real* aveuu, real* avevv, real* aveww, real* avesm,       \
+                  real* avesmsm, real* uwfx, const real* u, const real* v, const real* w,           \
+                  const real* p, const real* sm, int nmax);                                         \
   void P##velfg(int km, int jm, int im, const real* dx1, real* cov1, real* cov2, real* cov3,        \
                 real* dfu1, real* diu1, real* diu2, const real* dy1, real* diu3, const real* dzn,    \
                 real vn, real* f, real* cov4, real* cov5, real* cov6, real* dfv1, real* diu4,        \

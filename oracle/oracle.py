@@ -25,7 +25,7 @@ ROOT = HERE.parent
 sys.path.insert(0, str(ROOT / "mpi-les_amd"))
 from lesamd import layout  # noqa: E402
 
-INT_ARGS = {"km", "jm", "im", "n", "nmax"}
+INT_ARGS = {"km", "jm", "im", "n", "nmax", "n1"}
 REAL_ARGS = {"ro", "dt", "vn", "alpha", "beta"}
 
 # Argument lists of the reference subroutines (velnw.f95:5, bondv1.f95:5,
@@ -48,6 +48,14 @@ SIGNATURES = {
     "boundsm": "km jm sm im",
 }
 SIGNATURES = {k: v.split() for k, v in SIGNATURES.items()}
+# Restated only (the reference's versions need MPI): bondv1_data24.f95:6 with
+# the planes its file records hold as ua, va, wa, and aveflow.f95:5-6 minus
+# the arguments its statistics never touch.
+ORACLE_ONLY = {
+    "bondv1_data24": "jm u z2 dzn v w km n im dt dxs ua va wa".split(),
+    "aveflow": ("n n1 km jm im aveu avev avew avep avel aveuu avevv aveww avesm avesmsm uwfx "
+                "u v w p sm nmax").split(),
+}
 STAGES = ("velnw", "bondv1", "velfg", "feedbf", "les", "adam", "press")
 
 
@@ -122,7 +130,7 @@ class _Domain:
 
     def args(self, stage, **extra):
         vals = []
-        for a in SIGNATURES[stage]:
+        for a in SIGNATURES[stage] if stage in SIGNATURES else ORACLE_ONLY[stage]:
             if a in extra:
                 vals.append(extra[a])
             elif a in self.scalars:
@@ -144,7 +152,7 @@ class OracleBackend:
         self.dom = _Domain(setup, np.float32 if prec == 32 else np.float64)
         self.setup = setup
         self.fn = {}
-        for s, names in SIGNATURES.items():
+        for s, names in list(SIGNATURES.items()) + list(ORACLE_ONLY.items()):
             f = getattr(self.lib, self.prefix + s)
             f.argtypes = _argtypes(names, real_ct)
             f.restype = None
@@ -177,6 +185,12 @@ class OracleBackend:
             self.call(s, **kw)
             if s == stop_after:
                 return
+
+    def bondv1_data24(self, n, ua, va, wa):
+        """bondv1 with inflow planes (jm, km) for u, v, w (bondv1_data24.f95)."""
+        planes = [np.asfortranarray(a, dtype=self.dom.dtype) for a in (ua, va, wa)]
+        self.call("bondv1_data24", n=n, ua=planes[0].ctypes.data, va=planes[1].ctypes.data,
+                  wa=planes[2].ctypes.data)
 
     def press_stats(self, hist_len=4096):
         it = ctypes.c_int()
@@ -279,3 +293,25 @@ class RefBackend:
         for n in ("u", "v", "w", "p"):
             out[n] = layout.extract(n, a[n], w.im, w.jm, w.km)
         return out
+
+
+def oracle_aveflow(n: int, n1: int, nmax: int, stats: dict, fields: dict, im: int, jm: int, km: int,
+                   prec: int = 32):
+    """aveflow (aveflow.f95:80-203) of the restatement on host arrays in
+    reference layout: `stats` (layout.AVE_FIELDS, updated in place) and
+    `fields` u, v, w, p, sm (read)."""
+    lib = ctypes.CDLL(str(oracle_lib_path()))
+    f = getattr(lib, ("orc32_" if prec == 32 else "orc64_") + "aveflow")
+    f.argtypes = _argtypes(ORACLE_ONLY["aveflow"], ctypes.c_float)
+    f.restype = None
+    dt = np.float32 if prec == 32 else np.float64
+    for name, a in list(stats.items()) + list(fields.items()):
+        if a.dtype != dt or not a.flags.f_contiguous:
+            raise ValueError(f"{name}: need a Fortran-ordered {dt.__name__} array")
+    vals = []
+    for a in ORACLE_ONLY["aveflow"]:
+        if a in INT_ARGS:
+            vals.append(dict(n=n, n1=n1, km=km, jm=jm, im=im, nmax=nmax)[a])
+        else:
+            vals.append((stats[a] if a in stats else fields[a]).ctypes.data)
+    f(*vals)

@@ -1,5 +1,6 @@
-! Test driver: the time loop of main.f95:247-296 (velnw, bondv1, velfg,
-! feedbf, les, adam, press with the reference's argument lists), compiled
+! Test driver: the time loop of main.f95:247-322 (velnw, bondv1 or
+! bondv1_data24, velfg, feedbf, les, adam, press, and aveflow, with the
+! reference's argument lists), compiled
 ! against the drop-in modules of mpi-les_amd/fortran/les_amd_modules.F90,
 ! behind one bind(C) entry point so a test can hand it seeded arrays.
 !
@@ -7,7 +8,10 @@
 ! nou, dfu1/dfv1/dfw1, fx/fy/fz) are local here.  On return the state arrays
 ! hold the device state (les_sync) and every host array is forgotten by the
 ! library (les_release_all), so the locals can be freed.  data20/data21 are
-! written (n1 == nmax) into files named by the caller.
+! written (n1 == nmax) into files named by the caller.  With jdata24 /= 0
+! the inflow comes from data24 records in the directory dir24
+! (main.f95:264-268); with nave1 > 0 aveflow runs after every step, averaging
+! from step nave1, and writes data10/data11 at n == nmax (main.f95:320-322).
 module les_time_loop_test
     use iso_c_binding
     implicit none
@@ -15,7 +19,9 @@ contains
     subroutine time_loop(im, jm, km, n0, n1, nmax, ro, dt, vn, alpha, beta, &
             dx1, dy1, dzn, dzs, dxs, dys, z2, cn1, cn2l, cn2s, cn3l, cn3s, cn4l, cn4s, &
             bmask1, cmask1, dmask1, p, u, v, w, f, g, h, usum, vsum, wsum, fold, gold, hold, fghold, &
-            rhs, sm, uspd, vspd, delx1, c_data20, c_data21) bind(C, name='les_test_time_loop')
+            rhs, sm, uspd, vspd, delx1, c_data20, c_data21, jdata24, c_dir24, nave1, aveu, avev, avew, &
+            avep, avel, aveuu, avevv, aveww, avesm, avesmsm, uwfx, c_data10, c_data11) &
+            bind(C, name='les_test_time_loop')
         use les_amd_c
         use module_velnw
         use module_bondv1
@@ -24,7 +30,9 @@ contains
         use module_les
         use module_adam
         use module_press
-        integer(c_int), value :: im, jm, km, n0, n1, nmax
+        use module_bondv1_data24
+        use module_aveflow
+        integer(c_int), value :: im, jm, km, n0, n1, nmax, jdata24, nave1
         real(c_float), value :: ro, dt, vn, alpha, beta
         real(c_float), dimension(-1:im+1) :: dx1
         real(c_float), dimension(0:jm+1) :: dy1
@@ -44,7 +52,15 @@ contains
         real(c_float), dimension(0:im,0:jm,0:km) :: f, g, h, usum, vsum, wsum
         real(c_float), dimension(-1:im+1,-1:jm+1,0:km+1) :: sm
         real(c_float), dimension(0:im+1,0:jm+1) :: uspd, vspd
-        character(kind=c_char), dimension(70) :: c_data20, c_data21
+        character(kind=c_char), dimension(70) :: c_data20, c_data21, c_data10, c_data11
+        character(kind=c_char), dimension(512) :: c_dir24
+        real(c_float), dimension(im,jm,0:km) :: aveu, avev
+        real(c_float), dimension(im+1,jm,0:km+2) :: avew
+        real(c_float), dimension(im,jm,km) :: avep, avel, aveuu, avevv, aveww, avesm, avesmsm, uwfx
+        real(c_float), allocatable, dimension(:,:) :: avesu, avesv, avesw, avesuu, avesvv, avesww, uwfxs
+        real(c_float), allocatable, dimension(:,:,:) :: amask1
+        character(len=70) :: data10, data11, data13, data14
+        real(c_float) :: time
 
         real(c_float), allocatable, dimension(:,:,:) :: diu1, diu2, diu3, diu4, diu5, diu6, diu7, &
             diu8, diu9, cov1, cov2, cov3, cov4, cov5, cov6, cov7, cov8, cov9, nou1, nou2, nou3, nou4, &
@@ -55,7 +71,19 @@ contains
         do i = 1, 70
             data20(i:i) = c_data20(i)
             data21(i:i) = c_data21(i)
+            data10(i:i) = c_data10(i)
+            data11(i:i) = c_data11(i)
         end do
+        data13 = ''
+        data14 = ''
+        do i = 1, 512
+            data24_dir(i:i) = c_dir24(i)
+        end do
+        data24_dir_known = .true.
+        allocate(avesu(im,km), avesv(im,km), avesw(im,km), avesuu(im,km), avesvv(im,km), avesww(im,km), &
+                 uwfxs(im,km), amask1(0:im+1,0:jm+1,0:km+1))
+        amask1 = 0.
+        time = 0.
         allocate(diu1(-1:im+2,0:jm+2,0:km+2), diu5(-1:im+2,0:jm+2,0:km+2), &
                  cov1(-1:im+2,0:jm+2,0:km+2), cov5(-1:im+2,0:jm+2,0:km+2), &
                  nou1(-1:im+2,0:jm+2,0:km+2), nou5(-1:im+2,0:jm+2,0:km+2))
@@ -74,7 +102,11 @@ contains
 
         do n = n0, n1
             call velnw(km,jm,im,p,ro,dxs,u,dt,f,dys,v,g,dzs,w,h)
-            call bondv1(jm,u,z2,dzn,v,w,km,n,im,dt,dxs)
+            if (jdata24 == 0) then
+                call bondv1(jm,u,z2,dzn,v,w,km,n,im,dt,dxs)
+            else
+                call bondv1_data24(jm,u,z2,dzn,v,w,km,n,im,dt,dxs)
+            end if
             call velfg(km,jm,im,dx1,cov1,cov2,cov3,dfu1,diu1,diu2,dy1,diu3,dzn, &
                        vn,f,cov4,cov5,cov6,dfv1,diu4,diu5,diu6,g,cov7,cov8,cov9, &
                        dfw1,diu7,diu8,diu9,dzs,h,nou1,u,nou5,v,nou9,w,nou2,nou3, &
@@ -86,6 +118,11 @@ contains
             call adam(n,nmax,data21,fold,im,jm,km,gold,hold,fghold,f,g,h)
             call press(km,jm,im,rhs,u,dx1,v,dy1,w,dzn,f,g,h,dt,cn1,cn2l,p,cn2s, &
                        cn3l,cn3s,cn4l,cn4s,n, nmax,data20,usum,vsum,wsum)
+            if (nave1 > 0) then
+                call aveflow(n,nave1,km,jm,im,aveu,avev,avew,avep,avel,aveuu,avevv,aveww, &
+                             avesm,avesmsm,uwfx,avesu,avesv,avesw,avesuu,avesvv, &
+                             avesww,u,v,w,p,sm,nmax,uwfxs,data10,time,data11,data13,data14,amask1)
+            end if
         end do
 
         call les_sync(p); call les_sync(u); call les_sync(v); call les_sync(w)
