@@ -305,9 +305,10 @@ def oracle_aveflow(n: int, n1: int, nmax: int, stats: dict, fields: dict, im: in
     f.argtypes = _argtypes(ORACLE_ONLY["aveflow"], ctypes.c_float)
     f.restype = None
     dt = np.float32 if prec == 32 else np.float64
-    for name, a in list(stats.items()) + list(fields.items()):
+    for name, a in stats.items():  # updated in place
         if a.dtype != dt or not a.flags.f_contiguous:
             raise ValueError(f"{name}: need a Fortran-ordered {dt.__name__} array")
+    fields = {name: np.asfortranarray(a, dtype=dt) for name, a in fields.items()}
     vals = []
     for a in ORACLE_ONLY["aveflow"]:
         if a in INT_ARGS:
