@@ -29,6 +29,7 @@
 // the result; later sweeps find the flag and exit.  Iteration l's values
 // stay intact because iteration l+1 writes the other buffers.
 #include <algorithm>
+#include <type_traits>
 
 #include "les_kernels.h"
 
@@ -541,6 +542,311 @@ __global__ __launch_bounds__(FNT) __attribute__((amdgpu_waves_per_eu(8, 8))) voi
   }
 }
 
+// ------------------------------- fused SOR iteration, two rows per thread
+// The same tile (FTX pairs x FTY rows, two-cell ring), k march and per-cell
+// arithmetic as k_sor_iter, so the iterates are bit-identical, with half the
+// threads: wave w owns tile rows 2w and 2w+1.  For even jm the first of the
+// two rows is always odd, so the pair positions of a thread's cells at step
+// kr follow from the parity of kr alone: the step loop is compiled for both
+// start parities and every position-dependent choice is made at compile
+// time.  (Odd jm, whose periodic rows break that, uses k_sor_iter.)  A
+// thread keeps what it can of its two cells in registers:
+//  * the previous black iterate of its cells at levels kr-1, kr, kr+1 (a
+//    six-level ring filled four steps ahead), which gives the red update's
+//    k neighbours, the black update's own value and, across the thread's
+//    two rows, one j neighbour of each red update;
+//  * the new red values of levels kr-2..kr, for the black update's k
+//    neighbours and one of its j neighbours.
+// LDS carries only the in-plane neighbours owned by other threads (the
+// previous black iterate of level kr, three planes; the new red values of
+// level kr-1, two planes) and the per-level metrics.  The step loop is
+// unrolled by six, the period of every ring, so ring slots and LDS planes
+// are fixed at compile time.  Every lane issues the same loads and stores on
+// every step: stores of cells the thread does not own go to the padding
+// pair of their row (cp > nt), those of levels outside its chunk to the
+// scratch plane at the start of the allocation; loads of levels outside
+// 1..km read neighbouring planes of the same allocation and are never used.
+constexpr int GW = FY / 2, GNT = FX * GW;
+
+template <bool SEP>
+__global__ __launch_bounds__(GNT) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_sor_iter2(
+    Grid g, SplitGrid s, Metrics m, SplitBufs sb, Ctrl* ctrl, int iter, int nparts, float omega, float pjuge,
+    int ntt, int ntj, int kchunk, int remap) {
+  __shared__ float ldsB[3][FY][FX];  // previous black iterate of level kr (planes by level mod 3)
+  __shared__ float ldsR[2][FY][FX];  // new red values (planes by level mod 2)
+  __shared__ double lds[GNT / kWave];
+  __shared__ int skip;
+  extern __shared__ float4 lvm4[];  // per level from krs: cn4l, cn4s, cncz
+  const int lane = threadIdx.x;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.y);
+  const int tid = w * FX + lane;
+  if (tid == 0) skip = (ctrl->sor_done != 0);
+  __syncthreads();
+  if (skip) return;
+  const int bid = blockIdx.x;
+  if (iter >= 2 && bid == 0) {  // stop test of the previous iteration (press.f95:119-121)
+    const int pp = (iter - 1) & 1;
+    double sum = 0.0;
+    for (int b = tid; b < nparts; b += GNT) sum += sb.part[pp][1][b] + sb.part[pp][0][b];
+    sum = block_sum<GNT>(sum, tid, lds);
+    if (tid == 0) {
+      if (iter - 1 <= kSorHist) ctrl->sor_hist[iter - 2] = sum;
+      ctrl->sor_iters = iter - 1;
+      if (!(sum == sum) || sum > 1.0e300) ctrl->nonfinite = 1;
+      if (sum < (double)pjuge || ctrl->nonfinite) ctrl->sor_done = iter - 1;
+    }
+  }
+  const int prv = (iter - 1) & 1, cur = iter & 1;
+  const int im = g.im, jm = g.jm, km = g.km, nt = s.nt, cp = s.cp;
+  const unsigned cplb = 4u * (unsigned)s.cplane;
+  const float* base = sb.base;
+  const Rsrc S = brsrc(base, (unsigned)sb.nplanes * cplb);
+  // byte offset of level 0 of each array: level L is at o + L * cplb
+  const unsigned oRo = boff(base, sb.p[prv][1]) - cplb, oBo = boff(base, sb.p[prv][0]) - cplb;
+  const unsigned oRn = boff(base, sb.p[cur][1]) - cplb, oBn = boff(base, sb.p[cur][0]) - cplb;
+  const unsigned orR = boff(base, sb.rhs[1]) - cplb, orB = boff(base, sb.rhs[0]) - cplb;
+  const unsigned ocR = SEP ? 0u : boff(base, sb.cn1[1]) - cplb, ocB = SEP ? 0u : boff(base, sb.cn1[0]) - cplb;
+  const int lb = remap ? xcd_contiguous(bid, nparts) : bid;
+  const int bt = lb % ntt, bj = (lb / ntt) % ntj, bk = lb / (ntt * ntj);
+  const int t = bt * FTX - 2 + lane;
+  const bool tin = (t >= 0) && (t < nt);
+  const int rA = 2 * w, rB = rA + 1;
+  const int jlA = 1 + bj * FTY - 2 + rA, jlB = jlA + 1;  // tile rows (may lie outside 1..jm)
+  const int jA = jwrap(jlA, jm), jB = jwrap(jlB, jm);     // the rows they stand for (jA odd)
+  const int k0 = 1 + bk * kchunk, k1 = min(km, k0 + kchunk - 1);
+  const int krs = max(1, k0 - 1), kre = k1 + 1;
+  const int tq = tin ? t : 0;
+  const unsigned voA = 4u * (unsigned)(tq + (jA - 1) * cp), voB = 4u * (unsigned)(tq + (jB - 1) * cp);
+  const unsigned trA = 4u * (unsigned)(nt + (jA - 1) * cp), trB = 4u * (unsigned)(nt + (jB - 1) * cp);
+  const bool lane_tile = tin && lane >= 2 && lane <= FX - 3;
+  const bool tileA = rA >= 2 && rA <= FY - 3 && jlA <= jm, tileB = rB >= 2 && rB <= FY - 3 && jlB <= jm;
+  // per position p of the pair (i = 2t+1+p)
+  float c2l[2] = {0.f, 0.f}, c2s[2] = {0.f, 0.f}, axp[2] = {0.f, 0.f};
+  if (tin) {
+    for (int p = 0; p < 2; p++) {
+      c2l[p] = M1(m.cn2l, 2 * t + 1 + p);
+      c2s[p] = M1(m.cn2s, 2 * t + 1 + p);
+      if (SEP) axp[p] = M1(m.cnax, 2 * t + 1 + p);
+    }
+  }
+  const float c3lA = M1(m.cn3l, jA), c3sA = M1(m.cn3s, jA), c3lB = M1(m.cn3l, jB), c3sB = M1(m.cn3s, jB);
+  const float byA = SEP ? M1(m.cnby, jA) : 0.f, byB = SEP ? M1(m.cnby, jB) : 0.f;
+  float sA[2], sB[2];
+  unsigned svA[2], svB[2];
+  bool okA[2], okB[2], last[2];
+  for (int p = 0; p < 2; p++) {
+    sA[p] = axp[p] + byA;
+    sB[p] = axp[p] + byB;
+    okA[p] = lane_tile && tileA && (2 * t + 1 + p <= im);
+    okB[p] = lane_tile && tileB && (2 * t + 1 + p <= im);
+    svA[p] = okA[p] ? voA : trA;
+    svB[p] = okB[p] ? voB : trB;
+    last[p] = (2 * t + 1 + p == im);
+  }
+  const bool first = (t == 0);  // i = 1 (position 0)
+  constexpr int PL = FY * FX;    // floats per LDS plane
+  const int ownA = rA * FX + lane;
+  // the other colour's i-1 neighbour of position p is at own-1+p: bases kept
+  // inside the plane (only outer-ring lanes of rows 0 and FY-1 are clamped)
+  const int bsA = min(max(ownA - 1, 0), PL - 3), bsB = min(max(ownA + FX - 1, 0), PL - 3);
+  const int upA = (rA == 0 ? rA : rA - 1) * FX + lane;       // row A-1 (row 0 reads itself, unused)
+  const int dnB = (rB == FY - 1 ? rB : rB + 1) * FX + lane;  // row B+1
+  const int nsteps = kre - krs + 1;
+  for (int x = tid; x < nsteps; x += GNT) {
+    const int k = krs + x;
+    float4 v = {0.f, 0.f, 0.f, 0.f};
+    if (k <= km) {
+      v.x = M1(m.cn4l, k);
+      v.y = M1(m.cn4s, k);
+      v.z = SEP ? M1(m.cncz, k) : 0.f;
+    }
+    lvm4[x] = v;
+  }
+  // streams: red operands of level L and black operands of level L-1,
+  // three steps ahead (slots by step mod 3); the previous black iterate by
+  // level mod 6, four levels ahead of the step
+  struct Sl {
+    float roA, rrA, rbA, roB, rrB, rbB;
+    float crA, cbA, crB, cbB;  // !SEP
+  };
+  const int kcl = km + 1;
+  auto fill = [&](Sl& sl, int L) {
+    const unsigned lr = (unsigned)min(L, kcl) * cplb, lk = (unsigned)min(L - 1, kcl) * cplb;
+    sl.roA = bload(S, voA, oRo + lr);
+    sl.roB = bload(S, voB, oRo + lr);
+    sl.rrA = bload(S, voA, orR + lr);
+    sl.rrB = bload(S, voB, orR + lr);
+    sl.rbA = bload(S, voA, orB + lk);
+    sl.rbB = bload(S, voB, orB + lk);
+    if (!SEP) {
+      sl.crA = bload(S, voA, ocR + lr);
+      sl.crB = bload(S, voB, ocR + lr);
+      sl.cbA = bload(S, voA, ocB + lk);
+      sl.cbB = bload(S, voB, ocB + lk);
+    }
+  };
+  Sl sl[3];
+  float bqA[6], bqB[6];  // previous black iterate, level L at (L - krs + 1) mod 6
+  auto loadbq = [&](float& a, float& b, int L) {
+    const unsigned o = oBo + (unsigned)min(L, kcl) * cplb;
+    a = bload(S, voA, o);
+    b = bload(S, voB, o);
+  };
+  fill(sl[0], krs);
+  fill(sl[1], krs + 1);
+  fill(sl[2], krs + 2);
+  for (int q = 0; q < 5; q++) loadbq(bqA[q], bqB[q], krs - 1 + q);  // levels krs-1 .. krs+3
+  float* const LB = &ldsB[0][0][0];
+  float* const LR = &ldsR[0][0][0];
+  LB[ownA] = bqA[1];  // level krs into plane 0
+  LB[ownA + FX] = bqB[1];
+  float rdA[3] = {0.f, 0.f, 0.f}, rdB[3] = {0.f, 0.f, 0.f};  // new red values, level L at (L - krs) mod 3
+  float mx[2] = {0.f, 0.f}, my[2] = {0.f, 0.f}, mz[2] = {0.f, 0.f};  // metrics of the step's level, by parity
+  double acc = 0.0;
+  __syncthreads();  // lvm4 and plane 0 are complete
+
+  // D0: position of row A's cells at the first step (row B's is the other)
+  auto march = [&](auto D0C) {
+    constexpr int D0 = decltype(D0C)::value;
+    auto step = [&](auto UC, int kr) {
+      constexpr int U = decltype(UC)::value;
+      constexpr int P = U & 1;
+      constexpr int pa = D0 ^ P, pb = pa ^ 1;
+      const int kb = kr - 1;
+      // 1. previous black iterate of level kr+1 into its plane
+      LB[((U + 1) % 3) * PL + ownA] = bqA[(U + 2) % 6];
+      LB[((U + 1) % 3) * PL + ownA + FX] = bqB[(U + 2) % 6];
+      {
+        const float4 v = lvm4[kr - krs];
+        mx[P] = v.x;
+        my[P] = v.y;
+        mz[P] = v.z;
+      }
+      __syncthreads();
+      const bool kinR = (unsigned)(kr - k0) <= (unsigned)(k1 - k0);
+      const bool kinB = (unsigned)(kb - k0) <= (unsigned)(k1 - k0);
+      const unsigned soR = kinR ? oRn + (unsigned)kr * cplb : 0u;
+      const unsigned soB = kinB ? oBn + (unsigned)kb * cplb : 0u;
+      const bool kfirst = kr == 1, klast = kr == km, bfirst = kb == 1, blast = kb == km;
+      Sl& q = sl[U % 3];
+      const float* pb_ = LB + (U % 3) * PL;
+      // 2. red, level kr (press.f95:93-104), rows A and B
+      float redA, redB;
+      {
+        const float own = q.roA;
+        const float lo = pb_[bsA + pa], hi = pb_[bsA + pa + 1];
+        const float pim = (pa == 0 && first) ? own : lo;
+        const float pip = last[pa] ? own : hi;
+        const float pjm = pb_[upA];
+        const float pjp = bqB[(U + 1) % 6];  // row B's previous black iterate, level kr
+        const float pkm = kfirst ? own : bqA[U % 6];
+        const float pkp = klast ? own : bqA[(U + 2) % 6];
+        const float cn = SEP ? rcp_rn(sA[pa] + mz[P]) : q.crA;  // init.f95:100-107
+        const float reltmp = omega * (cn * (c2l[pa] * pip + c2s[pa] * pim + c3lA * pjp + c3sA * pjm +
+                                            mx[P] * pkp + my[P] * pkm - q.rrA) -
+                                      own);
+        redA = own + reltmp;
+        bstore(S, redA, svA[pa], soR);
+        const float z = (okA[pa] && kinR) ? reltmp : 0.0f;
+        acc = __builtin_fma((double)z, (double)z, acc);  // exact square, as k_sor_iter
+      }
+      {
+        const float own = q.roB;
+        const float lo = pb_[bsB + pb], hi = pb_[bsB + pb + 1];
+        const float pim = (pb == 0 && first) ? own : lo;
+        const float pip = last[pb] ? own : hi;
+        const float pjm = bqA[(U + 1) % 6];  // row A's previous black iterate, level kr
+        const float pjp = pb_[dnB];
+        const float pkm = kfirst ? own : bqB[U % 6];
+        const float pkp = klast ? own : bqB[(U + 2) % 6];
+        const float cn = SEP ? rcp_rn(sB[pb] + mz[P]) : q.crB;
+        const float reltmp = omega * (cn * (c2l[pb] * pip + c2s[pb] * pim + c3lB * pjp + c3sB * pjm +
+                                            mx[P] * pkp + my[P] * pkm - q.rrB) -
+                                      own);
+        redB = own + reltmp;
+        bstore(S, redB, svB[pb], soR);
+        const float z = (okB[pb] && kinR) ? reltmp : 0.0f;
+        acc = __builtin_fma((double)z, (double)z, acc);
+      }
+      LR[P * PL + ownA] = redA;
+      LR[P * PL + ownA + FX] = redB;
+      rdA[U % 3] = redA;
+      rdB[U % 3] = redB;
+      // 3. black, level kb = kr-1 (press.f95:93-104), from the red values of
+      // levels kb-1..kb+1; its in-plane red neighbours were written last step
+      const float* pr = LR + (P ^ 1) * PL;
+      {
+        const float own = bqA[U % 6];
+        const float lo = pr[bsA + pa], hi = pr[bsA + pa + 1];
+        const float pim = (pa == 0 && first) ? own : lo;
+        const float pip = last[pa] ? own : hi;
+        const float pjm = pr[upA];
+        const float pjp = rdB[(U + 2) % 3];  // row B's red value, level kb
+        const float pkm = bfirst ? own : rdA[(U + 1) % 3];
+        const float pkp = blast ? own : redA;
+        const float cn = SEP ? rcp_rn(sA[pa] + mz[P ^ 1]) : q.cbA;
+        const float reltmp = omega * (cn * (c2l[pa] * pip + c2s[pa] * pim + c3lA * pjp + c3sA * pjm +
+                                            mx[P ^ 1] * pkp + my[P ^ 1] * pkm - q.rbA) -
+                                      own);
+        bstore(S, own + reltmp, svA[pa], soB);
+        const float z = (okA[pa] && kinB) ? reltmp : 0.0f;
+        acc = __builtin_fma((double)z, (double)z, acc);
+      }
+      {
+        const float own = bqB[U % 6];
+        const float lo = pr[bsB + pb], hi = pr[bsB + pb + 1];
+        const float pim = (pb == 0 && first) ? own : lo;
+        const float pip = last[pb] ? own : hi;
+        const float pjm = rdA[(U + 2) % 3];  // row A's red value, level kb
+        const float pjp = pr[dnB];
+        const float pkm = bfirst ? own : rdB[(U + 1) % 3];
+        const float pkp = blast ? own : redB;
+        const float cn = SEP ? rcp_rn(sB[pb] + mz[P ^ 1]) : q.cbB;
+        const float reltmp = omega * (cn * (c2l[pb] * pip + c2s[pb] * pim + c3lB * pjp + c3sB * pjm +
+                                            mx[P ^ 1] * pkp + my[P ^ 1] * pkm - q.rbB) -
+                                      own);
+        bstore(S, own + reltmp, svB[pb], soB);
+        const float z = (okB[pb] && kinB) ? reltmp : 0.0f;
+        acc = __builtin_fma((double)z, (double)z, acc);
+      }
+      // 4. operands of later steps
+      fill(q, kr + 3);
+      loadbq(bqA[(U + 5) % 6], bqB[(U + 5) % 6], kr + 4);
+    };
+    using I0 = std::integral_constant<int, 0>;
+    using I1 = std::integral_constant<int, 1>;
+    using I2 = std::integral_constant<int, 2>;
+    using I3 = std::integral_constant<int, 3>;
+    using I4 = std::integral_constant<int, 4>;
+    using I5 = std::integral_constant<int, 5>;
+    int kr = krs;
+    for (; kr + 5 <= kre; kr += 6) {
+      step(I0{}, kr);
+      step(I1{}, kr + 1);
+      step(I2{}, kr + 2);
+      step(I3{}, kr + 3);
+      step(I4{}, kr + 4);
+      step(I5{}, kr + 5);
+    }
+    const int rest = kre - kr + 1;  // 0..5 steps left, block-uniform
+    if (rest > 0) step(I0{}, kr);
+    if (rest > 1) step(I1{}, kr + 1);
+    if (rest > 2) step(I2{}, kr + 2);
+    if (rest > 3) step(I3{}, kr + 3);
+    if (rest > 4) step(I4{}, kr + 4);
+  };
+  // row A (jA odd) at level krs: position (jA + krs) & 1
+  if (((1 + krs) & 1) == 0)
+    march(std::integral_constant<int, 0>{});
+  else
+    march(std::integral_constant<int, 1>{});
+  const double sum = block_sum<GNT>(acc, tid, lds);
+  if (tid == 0) {
+    sb.part[cur][0][bid] = sum;
+    sb.part[cur][1][bid] = 0.0;
+  }
+}
+
 // After the iteration loop: when no iteration met the stop test, the last
 // one (nmaxp) is summed here.  Records the iteration count and sum.
 __global__ __launch_bounds__(256) void k_sor_final(SplitBufs sb, Ctrl* ctrl, int nmaxp, int nparts) {
@@ -787,13 +1093,40 @@ __global__ __launch_bounds__(NT) void k_dc_padj(Grid g, const double* __restrict
   }
 }
 
+// One fused SOR iteration l (l >= 2, or any iteration without the entry
+// ghosts): the two-rows-per-thread kernel, unless sor_rows == 1 or jm is odd.
+static void launch_sor_iter(const DevDomain& d, const SplitBufs& sb, int l, float pjuge, int nparts, int ntt, int ntj,
+                            int kc) {
+  const Grid& g = d.g;
+  const SplitGrid& s = d.sg;
+  if (d.sor_rows == 1 || (g.jm & 1) != 0) {
+    const size_t lvm_bytes = 3 * (size_t)(kc + 2) * sizeof(float);
+    if (d.cn1_separable)
+      TIMED(d, KC_SOR, hipLaunchKernelGGL((k_sor_iter<true, false>), dim3(nparts), dim3(FX, FY), lvm_bytes, d.stream,
+                                          g, s, d.m, sb, d.ctrl, l, nparts, d.omega, pjuge, ntt, ntj, kc, d.xcd_remap,
+                                          nullptr));
+    else
+      TIMED(d, KC_SOR, hipLaunchKernelGGL((k_sor_iter<false, false>), dim3(nparts), dim3(FX, FY), lvm_bytes,
+                                          d.stream, g, s, d.m, sb, d.ctrl, l, nparts, d.omega, pjuge, ntt, ntj, kc,
+                                          d.xcd_remap, nullptr));
+    return;
+  }
+  const size_t lvm_bytes = (size_t)(kc + 2) * sizeof(float4);
+  const dim3 grid(nparts), block(FX, GW);
+  if (d.cn1_separable)
+    TIMED(d, KC_SOR, hipLaunchKernelGGL((k_sor_iter2<true>), grid, block, lvm_bytes, d.stream, g, s, d.m, sb, d.ctrl,
+                                        l, nparts, d.omega, pjuge, ntt, ntj, kc, d.xcd_remap));
+  else
+    TIMED(d, KC_SOR, hipLaunchKernelGGL((k_sor_iter2<false>), grid, block, lvm_bytes, d.stream, g, s, d.m, sb, d.ctrl,
+                                        l, nparts, d.omega, pjuge, ntt, ntj, kc, d.xcd_remap));
+}
+
 static void launch_press_dc(const DevDomain& d, const SplitBufs& sb, int gx, int gy, int gz, int nred) {
   const Grid& g = d.g;
   const SplitGrid& s = d.sg;
   const int ntt = (s.nt + FTX - 1) / FTX, ntj = (g.jm + FTY - 1) / FTY;
   const int kc = sor_fused_kchunk(g, s, d.sor_fused_blocks, d.ncu);
   const int nparts = ntt * ntj * ((g.km + kc - 1) / kc);
-  const size_t lvm_bytes = 3 * (size_t)(kc + 2) * sizeof(float);
   double* P64 = d.dc64;
   double* R64 = d.dc64 + d.n3;
   double* hist = d.dc64 + 2 * (size_t)d.n3;
@@ -808,16 +1141,7 @@ static void launch_press_dc(const DevDomain& d, const SplitBufs& sb, int gx, int
                                           d.partials, d.ctrl));
     TIMED(d, KC_RHS, hipLaunchKernelGGL(k_dc_norm, dim3(1), dim3(256), 0, d.stream, (const double*)d.partials, nred,
                                         hist, o));
-    for (int l = 1; l <= d.dc_inner; l++) {
-      if (d.cn1_separable)
-        TIMED(d, KC_SOR, hipLaunchKernelGGL((k_sor_iter<true, false>), dim3(nparts), dim3(FX, FY), lvm_bytes, d.stream,
-                                            g, s, d.m, sb, d.ctrl, l, nparts, d.omega, 0.0f, ntt, ntj, kc,
-                                            d.xcd_remap, nullptr));
-      else
-        TIMED(d, KC_SOR, hipLaunchKernelGGL((k_sor_iter<false, false>), dim3(nparts), dim3(FX, FY), lvm_bytes,
-                                            d.stream, g, s, d.m, sb, d.ctrl, l, nparts, d.omega, 0.0f, ntt, ntj, kc,
-                                            d.xcd_remap, nullptr));
-    }
+    for (int l = 1; l <= d.dc_inner; l++) launch_sor_iter(d, sb, l, 0.0f, nparts, ntt, ntj, kc);
     TIMED(d, KC_PAV, hipLaunchKernelGGL(k_dc_correct, dim3(gx, gy, g.km), blk(), 0, d.stream, g, s, sb, P64,
                                         d.dc_inner & 1));
   }
@@ -863,27 +1187,16 @@ void launch_press(const DevDomain& d, float dt) {
     const int nparts = ntt * ntj * ((g.km + kc - 1) / kc);
     const size_t lvm_bytes = 3 * (size_t)(kc + 2) * sizeof(float);
     const float* pent = d.F.p;
-    for (int l = 1; l <= d.nmaxp; l++) {
-      if (d.cn1_separable) {
-        if (l == 1)
-          TIMED(d, KC_SOR, hipLaunchKernelGGL((k_sor_iter<true, true>), dim3(nparts), dim3(FX, FY), lvm_bytes, d.stream,
-                                              g, s, d.m, sb, d.ctrl, l, nparts, d.omega, d.pjuge, ntt, ntj, kc,
-                                              d.xcd_remap, pent));
-        else
-          TIMED(d, KC_SOR, hipLaunchKernelGGL((k_sor_iter<true, false>), dim3(nparts), dim3(FX, FY), lvm_bytes,
-                                              d.stream, g, s, d.m, sb, d.ctrl, l, nparts, d.omega, d.pjuge, ntt, ntj,
-                                              kc, d.xcd_remap, nullptr));
-      } else {
-        if (l == 1)
-          TIMED(d, KC_SOR, hipLaunchKernelGGL((k_sor_iter<false, true>), dim3(nparts), dim3(FX, FY), lvm_bytes,
-                                              d.stream, g, s, d.m, sb, d.ctrl, l, nparts, d.omega, d.pjuge, ntt, ntj,
-                                              kc, d.xcd_remap, pent));
-        else
-          TIMED(d, KC_SOR, hipLaunchKernelGGL((k_sor_iter<false, false>), dim3(nparts), dim3(FX, FY), lvm_bytes,
-                                              d.stream, g, s, d.m, sb, d.ctrl, l, nparts, d.omega, d.pjuge, ntt, ntj,
-                                              kc, d.xcd_remap, nullptr));
-      }
-    }
+    // iteration 1 reads the ghost values p had on entry (k_sor_iter<., true>)
+    if (d.cn1_separable)
+      TIMED(d, KC_SOR, hipLaunchKernelGGL((k_sor_iter<true, true>), dim3(nparts), dim3(FX, FY), lvm_bytes, d.stream, g,
+                                          s, d.m, sb, d.ctrl, 1, nparts, d.omega, d.pjuge, ntt, ntj, kc, d.xcd_remap,
+                                          pent));
+    else
+      TIMED(d, KC_SOR, hipLaunchKernelGGL((k_sor_iter<false, true>), dim3(nparts), dim3(FX, FY), lvm_bytes, d.stream,
+                                          g, s, d.m, sb, d.ctrl, 1, nparts, d.omega, d.pjuge, ntt, ntj, kc,
+                                          d.xcd_remap, pent));
+    for (int l = 2; l <= d.nmaxp; l++) launch_sor_iter(d, sb, l, d.pjuge, nparts, ntt, ntj, kc);
     launch_press_tail(d, sb, nparts, gx, gy, gz, nred);
     return;
   }

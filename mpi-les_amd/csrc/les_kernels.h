@@ -31,6 +31,8 @@ struct SplitBufs {
   float* rhs[2];        // [colour]
   float* cn1[2];        // [colour]; unused while cn1 is recomputed
   double* part[2][2];   // [iterate parity][colour] per-block sums of reltmp^2
+  float* base;          // start of the allocation: a scratch plane, the eight arrays, a scratch plane
+  int nplanes;          // planes in the allocation (8 km + 2)
 };
 
 // Kernel classes for the optional per-launch timing (les_domain_set_timing).
@@ -69,6 +71,7 @@ struct DevDomain {
   double* dc64;      // P64, R64 (n3 each) and the defect history (kDcMaxOuter)
   int sor_fused_blocks;  // target number of blocks of one fused SOR iteration (0: automatic)
   int ncu;           // compute units of the device
+  int sor_rows;      // tile rows per thread of the fused SOR iteration (1: k_sor_iter, 2: k_sor_iter2)
 };
 
 int sor_kchunk(const Grid& g, const SplitGrid& s, int target);

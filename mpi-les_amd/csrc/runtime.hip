@@ -463,23 +463,31 @@ int domain_alloc(int im, int jm, int km, les_domain** out) {
   // allocated on its first use (ensure_cov), the whole-step path never
   // stores them
   {
-    // colour-split SOR arrays: 4 iterate buffers, 2 rhs, 2 cn1
+    // colour-split SOR arrays: 4 iterate buffers, 2 rhs, 2 cn1, after one
+    // scratch plane and followed by another.  Rows are padded to at least
+    // one pair beyond the last (cp > nt): the SOR kernel sends the stores of
+    // lanes that own no cell there, and the planes at either end take the
+    // loads and stores of levels outside 1..km.  Nothing reads either.
     SplitGrid& sg = D->d.sg;
     sg.nt = (im + 1) / 2;
-    sg.cp = (sg.nt + 31) / 32 * 32;
+    sg.cp = (sg.nt + 1 + 31) / 32 * 32;
     sg.cplane = sg.cp * jm;
     const size_t cs = (size_t)sg.cplane * km;
-    const size_t nb = 8 * cs * sizeof(float);
+    const size_t nplanes = 8 * (size_t)km + 2;
+    const size_t nb = nplanes * sg.cplane * sizeof(float);
     e = hipMalloc(&D->split, nb);
     if (e == hipSuccess) e = hipMemset(D->split, 0, nb);
     if (e != hipSuccess) return bail(std::string("SOR buffer allocation: ") + hipGetErrorString(e));
     D->bytes += nb;
     SplitBufs& sb = D->d.sb;
-    for (int q = 0; q < 4; q++) sb.p[q >> 1][q & 1] = D->split + q * cs;
+    float* arr0 = D->split + sg.cplane;  // level 1 of the first array
+    for (int q = 0; q < 4; q++) sb.p[q >> 1][q & 1] = arr0 + q * cs;
     for (int c = 0; c < 2; c++) {
-      sb.rhs[c] = D->split + (4 + c) * cs;
-      sb.cn1[c] = D->split + (6 + c) * cs;
+      sb.rhs[c] = arr0 + (4 + c) * cs;
+      sb.cn1[c] = arr0 + (6 + c) * cs;
     }
+    sb.base = D->split;
+    sb.nplanes = (int)nplanes;
   }
   // metric pack: each array gets n+4 slots (indices -1..n+2)
   int off = 0;
@@ -517,6 +525,7 @@ int domain_alloc(int im, int jm, int km, les_domain** out) {
   D->d.velfg_kchunk = env_int("LES_VELFG_KCHUNK", 0);
   D->d.fuse_les = env_int("LES_FUSE_LES", 1);
   D->d.sor_fused_blocks = env_int("LES_SOR_FUSED_BLOCKS", 0);
+  D->d.sor_rows = env_int("LES_SOR_ROWS", 2) == 1 ? 1 : 2;
   {
     int dev = 0, ncu = 0;
     if (hipGetDevice(&dev) == hipSuccess &&
@@ -1091,10 +1100,10 @@ int les_domain_query(les_domain* D) {
 long long les_domain_bytes_needed(int im, int jm, int km) {
   if (im < 2 || jm < 2 || km < 2) return -1;
   const long long sj = ((long long)im + 4 + 63) / 64 * 64, sk = sj * (jm + 4), n3 = sk * (km + 4);
-  const long long nt = (im + 1) / 2, cp = (nt + 31) / 32 * 32, cs = cp * jm * km;
+  const long long nt = (im + 1) / 2, cp = (nt + 1 + 31) / 32 * 32, cs = cp * jm * km;
   long long b = 0;
-  for (int f = 0; f < NFIELDS; f++) b += (kFields[f].kind == K2D ? sk : n3) * 4;
-  b += 8 * cs * 4;   // colour-split SOR arrays
+  for (int f = 0; f < kLazyFirst; f++) b += (kFields[f].kind == K2D ? sk : n3) * 4;
+  b += (8 * cs + 2 * cp * jm) * 4;   // colour-split SOR arrays and their two scratch planes
   return b + (1LL << 20);  // metrics, partial sums, control block (bounded by 1 MiB here)
 }
 

@@ -158,22 +158,29 @@ FUSED_CASES = [
     (dict(im=9, jm=2, km=5, seed=11), 640),      # jm = 2
     (dict(im=64, jm=26, km=33, seed=12), 1),     # one k chunk of 33 levels
     (dict(im=64, jm=26, km=33, seed=12), 5000),  # one level per block
+    (dict(im=250, jm=30, km=20, seed=18), 640),  # three pair tiles, three row tiles, even jm
+    (dict(im=121, jm=14, km=17, seed=19), 0),    # automatic k chunks, odd im, even jm
+    (dict(im=300, jm=24, km=90, seed=20), 0),    # the 300-wide rows of the bench domain
 ]
 
 
+@pytest.mark.parametrize("rows", [1, 2])
 @pytest.mark.parametrize("case,blocks", FUSED_CASES,
                          ids=lambda c: f"{c['im']}x{c['jm']}x{c['km']}" if isinstance(c, dict) else f"b{c}")
-def test_fused_sor_iteration_equals_colour_sweeps(case, blocks, monkeypatch):
+def test_fused_sor_iteration_equals_colour_sweeps(case, blocks, rows, monkeypatch):
     """One launch per SOR iteration (red on the tile and a ring, then black,
     marching k) gives the iterates of the two colour sweeps bit for bit, for
-    any tiling of the domain, and the same iteration counts and residuals."""
+    any tiling of the domain, and the same iteration counts and residuals;
+    with one tile row per thread (k_sor_iter) and with two (k_sor_iter2,
+    even jm; odd jm keeps k_sor_iter)."""
     from lesamd import api
     s = _mk(case)
-    print(f"\n[fused sor] {s.workload.name} blocks={blocks}", flush=True)
+    print(f"\n[fused sor] {s.workload.name} blocks={blocks} rows={rows}", flush=True)
     monkeypatch.setenv("LES_SOR_FUSED", "0")
     two = api.Domain(s)
     monkeypatch.setenv("LES_SOR_FUSED", "1")
     monkeypatch.setenv("LES_SOR_FUSED_BLOCKS", str(blocks))
+    monkeypatch.setenv("LES_SOR_ROWS", str(rows))
     one = api.Domain(s)
     for n in range(1, 4):
         two.step(n, 3)
