@@ -114,7 +114,10 @@ def profiled_traffic(kernel_class: str, workload: str):
     size) and recorded in profiles/<kernel>_<workload>.md; None when no such
     measurement is recorded for this kernel and workload."""
     if kernel_class == "sor":
-        name = "sor_sweep" if os.environ.get("LES_SOR_FUSED", "1") == "0" else "sor_iter"
+        if os.environ.get("LES_SOR_FUSED", "1") == "0":
+            name = "sor_sweep"
+        else:
+            name = "sor_iter" if os.environ.get("LES_SOR_ROWS", "2") == "1" else "sor_iter2"
     else:
         name = kernel_class
     f = ROOT / "profiles" / f"{name}_{workload}.md"
@@ -257,10 +260,12 @@ def main(argv=None):
     dom.synchronize()
     t1 = time.perf_counter()
     st = dom.press_stats()
-    # the dominant kernel (the SOR sweep) bracketed by HIP events on the
+    # the dominant kernel (the SOR iteration) bracketed by HIP events on the
     # domain's stream over K more steps launched eagerly (events cannot
-    # bracket single kernels inside a graph); kernel durations are the
-    # same either way, only the gaps between launches differ
+    # bracket single kernels inside a graph).  One event pair per launch:
+    # the pair adds its own marker packets (a few us) to the kernel's
+    # duration.  (One pair per run of back-to-back launches, runs=True,
+    # measures more: the eager dispatch gaps between the launches.)
     dom.set_timing(True, ["sor"])
     dom.timing()
     for n in range(W + K + 2, W + 2 * K + 2):

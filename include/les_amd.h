@@ -235,6 +235,11 @@ long long les_domain_bytes(const les_domain* d);
  * clock of main.f95:243-303. */
 int les_domain_set_timing(les_domain* d, int on);
 int les_domain_timing(les_domain* d, double* ms, long long* count);
+/* With on != 0, back-to-back launches of one selected class (no other
+ * launch between them) share one event pair, so the time of a run of n
+ * launches is n kernel durations plus the n-1 gaps between them, without
+ * an event per launch.  Needs timing on.  No reference counterpart. */
+int les_domain_set_timing_runs(les_domain* d, int on);
 int les_kernel_classes(void);
 const char* les_kernel_class_name(int c);
 

@@ -669,9 +669,11 @@ __global__ __launch_bounds__(GNT) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
     float roA, rrA, rbA, roB, rrB, rbB;
     float crA, cbA, crB, cbB;  // !SEP
   };
-  const int kcl = km + 1;
+  // Levels past the chunk's last use are clamped to it: the streams run
+  // ahead of the march, and reloading a line just read costs no HBM traffic.
+  const int lbq = min(kre + 1, km + 1);  // last previous-black level used (red of level kre)
   auto fill = [&](Sl& sl, int L) {
-    const unsigned lr = (unsigned)min(L, kcl) * cplb, lk = (unsigned)min(L - 1, kcl) * cplb;
+    const unsigned lr = (unsigned)min(L, kre) * cplb, lk = (unsigned)min(L - 1, kre - 1) * cplb;
     sl.roA = bload(S, voA, oRo + lr);
     sl.roB = bload(S, voB, oRo + lr);
     sl.rrA = bload(S, voA, orR + lr);
@@ -688,7 +690,7 @@ __global__ __launch_bounds__(GNT) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
   Sl sl[3];
   float bqA[6], bqB[6];  // previous black iterate, level L at (L - krs + 1) mod 6
   auto loadbq = [&](float& a, float& b, int L) {
-    const unsigned o = oBo + (unsigned)min(L, kcl) * cplb;
+    const unsigned o = oBo + (unsigned)min(L, lbq) * cplb;
     a = bload(S, voA, o);
     b = bload(S, voB, o);
   };

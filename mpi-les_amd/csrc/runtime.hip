@@ -120,9 +120,13 @@ const MetricInfo kMetrics[NMETRICS] = {
 struct Timer {
   std::vector<hipEvent_t> ev;  // 2 per pair
   std::vector<int> cls;        // class of each pair
+  std::vector<int> n;          // launches each pair brackets
   size_t used = 0;             // pairs recorded since the last read
   unsigned int mask = ~0u;     // classes to record
   bool open = false;           // a start event is waiting for its end
+  bool runs = false;           // one pair per run of back-to-back launches of one class
+  bool chain = false;          // the last launch was recorded and ended a pair
+  bool extend = false;         // the open launch extends the last pair
 };
 
 const char* kClassNames[KC_N] = {"velnw", "bondv1", "vel2", "vel2_ghost", "velfg", "feedbf", "les_sm",
@@ -133,13 +137,23 @@ const char* kClassNames[KC_N] = {"velnw", "bondv1", "vel2", "vel2_ghost", "velfg
 
 void les::timer_begin(const DevDomain& d, int c) {
   Timer* T = (Timer*)d.timer;
-  if (!T || !(T->mask & (1u << c))) return;
+  if (!T) return;
+  if (!(T->mask & (1u << c))) {  // an unrecorded launch ends a run
+    T->chain = false;
+    return;
+  }
+  if (T->runs && T->chain && T->used > 0 && T->cls[T->used - 1] == c) {
+    T->extend = true;  // the previous launch was the same class: move its end event
+    T->open = true;
+    return;
+  }
   if (2 * T->used + 2 > T->ev.size()) {
     hipEvent_t a, b;
     if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return;
     T->ev.push_back(a);
     T->ev.push_back(b);
     T->cls.push_back(0);
+    T->n.push_back(0);
   }
   T->cls[T->used] = c;
   hipEventRecord(T->ev[2 * T->used], d.stream);
@@ -150,9 +164,17 @@ void les::timer_end(const DevDomain& d, int c) {
   (void)c;
   Timer* T = (Timer*)d.timer;
   if (!T || !T->open) return;
-  hipEventRecord(T->ev[2 * T->used + 1], d.stream);
-  T->used++;
+  if (T->extend) {
+    hipEventRecord(T->ev[2 * (T->used - 1) + 1], d.stream);
+    T->n[T->used - 1]++;
+    T->extend = false;
+  } else {
+    hipEventRecord(T->ev[2 * T->used + 1], d.stream);
+    T->n[T->used] = 1;
+    T->used++;
+  }
   T->open = false;
+  T->chain = true;
 }
 
 struct les_domain {
@@ -1236,9 +1258,20 @@ int les_domain_timing(les_domain* D, double* ms, long long* count) {
     float e = 0.0f;
     HIP_TRY(hipEventElapsedTime(&e, T->ev[2 * q], T->ev[2 * q + 1]));
     ms[T->cls[q]] += e;
-    count[T->cls[q]] += 1;
+    count[T->cls[q]] += T->n[q];
   }
   T->used = 0;
+  T->chain = false;
+  return 0;
+}
+
+int les_domain_set_timing_runs(les_domain* D, int on) {
+  if (!D) return fail("null domain");
+  Timer* T = (Timer*)D->d.timer;
+  if (!T) return fail("timing is off");
+  HIP_TRY(hipStreamSynchronize(D->d.stream));
+  T->runs = on != 0;
+  T->chain = false;
   return 0;
 }
 

@@ -99,6 +99,7 @@ def lib():
     L.les_domain_bytes.argtypes = [vp]
     L.les_domain_bytes.restype = ctypes.c_longlong
     L.les_domain_set_timing.argtypes = [vp, ci]
+    L.les_domain_set_timing_runs.argtypes = [vp, ci]
     L.les_domain_set_graph.argtypes = [vp, ci]
     L.les_domain_timing.argtypes = [vp, vp, vp]
     L.les_kernel_class_name.restype = ctypes.c_char_p
@@ -285,8 +286,9 @@ class Domain:
         """Replay steps n > 1 from one captured HIP graph."""
         _check(lib().les_domain_set_graph(self._h, 1 if on else 0))
 
-    def set_timing(self, on: bool, classes=None):
-        """Bracket launches with HIP events; classes: names to record (all if None)."""
+    def set_timing(self, on: bool, classes=None, runs: bool = False):
+        """Bracket launches with HIP events; classes: names to record (all if
+        None); runs: one event pair per run of back-to-back launches of a class."""
         mask = 0
         if on:
             if classes is None:
@@ -297,6 +299,8 @@ class Domain:
                 for c in classes:
                     mask |= 1 << names.index(c)
         _check(lib().les_domain_set_timing(self._h, mask))
+        if on:
+            _check(lib().les_domain_set_timing_runs(self._h, 1 if runs else 0))
 
     def timing(self):
         """{kernel class: (total ms, launches)} since the previous read."""
