@@ -217,6 +217,7 @@ def main(argv=None):
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="CPU-baseline time budget [s]")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--eager", action="store_true", help="launch kernels one by one (no graph replay)")
+    ap.add_argument("--no-tune", action="store_true", help="keep the default SOR launch shape (no autotuning)")
     ap.add_argument("--serve-requests", type=int, default=24,
                     help="mixed-size requests of the serving measurement (0: skip it)")
     ap.add_argument("--serve-steps", type=int, default=40, help="time steps per served request")
@@ -248,6 +249,9 @@ def main(argv=None):
     for n in range(1, W + 1):
         dom.step(n, nmax)
     dom.synchronize()
+    # the SOR launch shape for this grid shape (les_domain_autotune), picked
+    # before the timed steps as a served domain gets it at admission
+    tuned = None if args.no_tune else dom.autotune(reps=10)
     snapshot = None if args.no_cpu_baseline else {n: dom.download(n) for n in STATE_FIELDS}
 
     # timed region: steps replayed from the captured graph of the step
@@ -331,6 +335,10 @@ def main(argv=None):
         },
         "kernels": per_kernel,
     }
+    if tuned is not None:
+        result["sor_launch"] = {"rows_per_thread": tuned["rows"], "k_levels_per_block": tuned["kchunk"],
+                                "tuned_us_per_iter": round(tuned["us_per_iter"], 2),
+                                "default_us_per_iter": round(tuned["us_per_iter_default"], 2)}
     dom.close()
     if args.serve_requests > 0:
         result["served"] = serve_measurement(args.serve_requests, args.serve_steps)

@@ -1127,7 +1127,7 @@ static void launch_press_dc(const DevDomain& d, const SplitBufs& sb, int gx, int
   const Grid& g = d.g;
   const SplitGrid& s = d.sg;
   const int ntt = (s.nt + FTX - 1) / FTX, ntj = (g.jm + FTY - 1) / FTY;
-  const int kc = sor_fused_kchunk(g, s, d.sor_fused_blocks, d.ncu);
+  const int kc = d.sor_kchunk > 0 ? std::min(d.sor_kchunk, g.km) : sor_fused_kchunk(g, s, d.sor_fused_blocks, d.ncu);
   const int nparts = ntt * ntj * ((g.km + kc - 1) / kc);
   double* P64 = d.dc64;
   double* R64 = d.dc64 + d.n3;
@@ -1185,7 +1185,7 @@ void launch_press(const DevDomain& d, float dt) {
   }
   if (d.sor_fused) {
     const int ntt = (s.nt + FTX - 1) / FTX, ntj = (g.jm + FTY - 1) / FTY;
-    const int kc = sor_fused_kchunk(g, s, d.sor_fused_blocks, d.ncu);
+    const int kc = d.sor_kchunk > 0 ? std::min(d.sor_kchunk, g.km) : sor_fused_kchunk(g, s, d.sor_fused_blocks, d.ncu);
     const int nparts = ntt * ntj * ((g.km + kc - 1) / kc);
     const size_t lvm_bytes = 3 * (size_t)(kc + 2) * sizeof(float);
     const float* pent = d.F.p;
@@ -1274,6 +1274,28 @@ int sor_fused_parts_needed(const Grid& g, const SplitGrid& s, int target, int nc
   const int ntt = (s.nt + FTX - 1) / FTX, ntj = (g.jm + FTY - 1) / FTY;
   const int kc = sor_fused_kchunk(g, s, target, ncu);
   return ntt * ntj * ((g.km + kc - 1) / kc);
+}
+
+int sor_fused_tiles(const Grid& g, const SplitGrid& s) {
+  return ((s.nt + FTX - 1) / FTX) * ((g.jm + FTY - 1) / FTY);
+}
+
+// `iters` fused SOR iterations (numbered 2.., never stopping) with the given
+// rows per thread and k chunk, on whatever the split arrays hold: a timing
+// trial for the autotuner.  Writes only the split iterates, the SOR partial
+// slots and the SOR fields of the control block.
+void launch_sor_trial(const DevDomain& d0, int rows, int kc, int iters) {
+  DevDomain d = d0;
+  d.sor_rows = rows;
+  d.timer = nullptr;
+  const Grid& g = d.g;
+  const SplitGrid& s = d.sg;
+  SplitBufs sb = d.sb;
+  if (d.cn1_separable) sb.cn1[0] = sb.cn1[1] = nullptr;
+  const int ntt = (s.nt + FTX - 1) / FTX, ntj = (g.jm + FTY - 1) / FTY;
+  kc = std::max(1, std::min(kc, g.km));
+  const int nparts = ntt * ntj * ((g.km + kc - 1) / kc);
+  for (int l = 2; l < 2 + iters; l++) launch_sor_iter(d, sb, l, 0.0f, nparts, ntt, ntj, kc);
 }
 
 int reduction_parts_needed(const Grid& g) {

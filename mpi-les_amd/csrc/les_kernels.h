@@ -72,12 +72,15 @@ struct DevDomain {
   int sor_fused_blocks;  // target number of blocks of one fused SOR iteration (0: automatic)
   int ncu;           // compute units of the device
   int sor_rows;      // tile rows per thread of the fused SOR iteration (1: k_sor_iter, 2: k_sor_iter2)
+  int sor_kchunk;    // k levels per block of the fused SOR iteration (0: from sor_fused_blocks)
 };
 
 int sor_kchunk(const Grid& g, const SplitGrid& s, int target);
 int sor_parts_needed(const Grid& g, const SplitGrid& s, int target);
 int sor_fused_kchunk(const Grid& g, const SplitGrid& s, int target, int ncu);
 int sor_fused_parts_needed(const Grid& g, const SplitGrid& s, int target, int ncu);
+int sor_fused_tiles(const Grid& g, const SplitGrid& s);
+void launch_sor_trial(const DevDomain& d, int rows, int kc, int iters);
 int reduction_parts_needed(const Grid& g);
 
 // Event pairs around launches, recorded only when d.timer is set.

@@ -548,14 +548,17 @@ int domain_alloc(int im, int jm, int km, les_domain** out) {
   D->d.fuse_les = env_int("LES_FUSE_LES", 1);
   D->d.sor_fused_blocks = env_int("LES_SOR_FUSED_BLOCKS", 0);
   D->d.sor_rows = env_int("LES_SOR_ROWS", 2) == 1 ? 1 : 2;
+  D->d.sor_kchunk = std::max(0, env_int("LES_SOR_KCHUNK", 0));
   {
     int dev = 0, ncu = 0;
     if (hipGetDevice(&dev) == hipSuccess &&
         hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess)
       D->d.ncu = ncu;
   }
+  // SOR partial slots for any k chunk of the fused iteration (down to one
+  // level per block) and for the colour sweeps
   const int nsor = std::max(sor_parts_needed(D->d.g, D->d.sg, D->d.sor_blocks),
-                            sor_fused_parts_needed(D->d.g, D->d.sg, D->d.sor_fused_blocks, D->d.ncu));
+                            sor_fused_tiles(D->d.g, D->d.sg) * D->d.g.km);
   e = hipMalloc(&D->d.partials, ((size_t)D->d.npartials + 4 * (size_t)nsor) * sizeof(double));
   if (e == hipSuccess)
     for (int q = 0; q < 4; q++) D->d.sb.part[q >> 1][q & 1] = D->d.partials + D->d.npartials + (size_t)q * nsor;
@@ -681,6 +684,99 @@ int run_aveflow(les_domain* D, int n, int n1, int nmax) {
   a.uwfx = D->fields[FUWFX];
   launch_aveflow(D->d, a, n, n1, nmax);
   HIP_TRY(hipGetLastError());
+  return 0;
+}
+
+// ------------------------------------------------------------ autotuner
+// The fused SOR iteration's launch shape (tile rows per thread, k levels per
+// block) changes only the speed, never the iterates (every tiling gives the
+// colour sweeps' values bit for bit).  The tuner times short runs of each
+// candidate on the domain's own split arrays, which hold only scratch
+// between press calls, keeps the fastest and remembers it per grid shape.
+struct TuneKey {
+  int im, jm, km, sep, ncu;
+  bool operator<(const TuneKey& o) const {
+    return std::tie(im, jm, km, sep, ncu) < std::tie(o.im, o.jm, o.km, o.sep, o.ncu);
+  }
+};
+struct TuneResult {
+  int rows = 2, kchunk = 0;
+  double us_best = 0.0, us_default = 0.0;
+};
+std::mutex g_tune_mu;
+std::map<TuneKey, TuneResult> g_tuned;
+
+int tune_sor(les_domain* D, int reps, TuneResult* out) {
+  const Grid& g = D->d.g;
+  HIP_TRY(hipStreamSynchronize(D->d.stream));
+  Ctrl saved;
+  HIP_TRY(hipMemcpy(&saved, D->d.ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost));
+  const int tiles = sor_fused_tiles(g, D->d.sg);
+  const int ncu = D->d.ncu > 0 ? D->d.ncu : 256;
+  const int kdef = sor_fused_kchunk(g, D->d.sg, D->d.sor_fused_blocks, D->d.ncu);
+  // candidate k chunks: every distinct split of km into nz chunks with
+  // between a quarter and eight times as many blocks as compute units
+  std::vector<int> kcs{kdef};
+  for (int nz = 1; nz <= g.km; nz++) {
+    const int kc = (g.km + nz - 1) / nz;
+    const long blocks = (long)tiles * ((g.km + kc - 1) / kc);
+    if (blocks * 4 < ncu || blocks > 8L * ncu) continue;
+    if (std::find(kcs.begin(), kcs.end(), kc) == kcs.end()) kcs.push_back(kc);
+  }
+  std::vector<int> rowsets{1};
+  if ((g.jm & 1) == 0) rowsets.insert(rowsets.begin(), 2);
+  const int rdef = ((g.jm & 1) == 0 && D->d.sor_rows == 2) ? 2 : 1;  // the shape used without tuning
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  HIP_TRY(hipEventCreate(&e0));
+  if (hipEventCreate(&e1) != hipSuccess) {
+    hipEventDestroy(e0);
+    return fail("hipEventCreate");
+  }
+  int rc = 0;
+  double best = -1.0;
+  TuneResult r;
+  r.rows = D->d.sor_rows;
+  r.kchunk = 0;
+  for (int rows : rowsets) {
+    for (int kc : kcs) {
+      Ctrl c = saved;
+      c.sor_done = 0;
+      c.nonfinite = 0;
+      hipError_t e = hipMemcpyAsync(D->d.ctrl, &c, sizeof(Ctrl), hipMemcpyHostToDevice, D->d.stream);
+      if (e == hipSuccess) {
+        launch_sor_trial(D->d, rows, kc, 2);  // warm-up
+        e = hipEventRecord(e0, D->d.stream);
+      }
+      if (e == hipSuccess) {
+        launch_sor_trial(D->d, rows, kc, reps);
+        e = hipEventRecord(e1, D->d.stream);
+      }
+      if (e == hipSuccess) e = hipEventSynchronize(e1);
+      float ms = 0.0f;
+      if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+      if (e == hipSuccess) e = hipGetLastError();
+      if (e != hipSuccess) {
+        rc = fail(std::string("autotune: ") + hipGetErrorString(e));
+        break;
+      }
+      const double us = 1e3 * ms / reps;
+      if (rows == rdef && kc == kdef) r.us_default = us;
+      if (best < 0.0 || us < best) {
+        best = us;
+        r.rows = rows;
+        r.kchunk = kc;
+      }
+    }
+    if (rc) break;
+  }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  // the last press call's statistics, as they were
+  if (hipMemcpy(D->d.ctrl, &saved, sizeof(Ctrl), hipMemcpyHostToDevice) != hipSuccess && !rc)
+    rc = fail("autotune: restoring the control block");
+  if (rc) return rc;
+  r.us_best = best;
+  *out = r;
   return 0;
 }
 
@@ -1194,6 +1290,27 @@ int les_domain_aveflow(les_domain* D, int n, int n1, int nmax) {
 int les_domain_set_inflow(les_domain* D, const float* u_in, const float* v_in, const float* w_in) {
   if (!D) return fail("null domain");
   return set_inflow(D, u_in, v_in, w_in);
+}
+
+int les_domain_autotune(les_domain* D, int reps, int* rows, int* kchunk, double* us_best, double* us_default) {
+  if (!D) return fail("null domain");
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  const Grid& g = D->d.g;
+  const TuneKey key{g.im, g.jm, g.km, D->d.cn1_separable, D->d.ncu};
+  auto it = g_tuned.find(key);
+  if (it == g_tuned.end() || reps > 0) {
+    TuneResult t;
+    if (tune_sor(D, reps > 0 ? reps : 10, &t)) return 1;
+    it = g_tuned.insert_or_assign(key, t).first;
+  }
+  const TuneResult& r = it->second;
+  D->d.sor_rows = r.rows;
+  D->d.sor_kchunk = r.kchunk;
+  if (rows) *rows = r.rows;
+  if (kchunk) *kchunk = r.kchunk;
+  if (us_best) *us_best = r.us_best;
+  if (us_default) *us_default = r.us_default;
+  return 0;
 }
 
 int les_domain_set_graph(les_domain* D, int on) {

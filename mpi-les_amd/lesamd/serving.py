@@ -106,6 +106,7 @@ class Server:
             def factory(req):
                 d = api.Domain(setup_gen.build(req.workload, dense=False))
                 d.set_graph(True)
+                d.autotune()  # SOR launch shape for this grid shape (timed once per shape)
                 return d
 
             domain_factory = domain_factory or factory

@@ -1,0 +1,50 @@
+"""The per-shape autotuner of the fused SOR iteration's launch shape
+(les_domain_autotune): it picks a valid shape, leaves the domain's state
+and last press statistics untouched, is remembered per grid shape, and the
+tuned domain steps bit for bit like an untuned one (every tiling gives the
+colour sweeps' iterates)."""
+import numpy as np
+import pytest
+
+from lesamd import api, setup_gen
+
+pytestmark = pytest.mark.gpu
+FIELDS = ("u", "v", "w", "p", "rhs", "sm", "f", "g", "h")
+
+
+@pytest.mark.parametrize("case", [dict(im=90, jm=40, km=24, seed=31), dict(im=61, jm=27, km=18, seed=32)],
+                         ids=lambda c: f"{c['im']}x{c['jm']}x{c['km']}")
+def test_autotune_keeps_state_and_results(case):
+    s = setup_gen.build(setup_gen.small_workload(**case))
+    w = s.workload
+    print(f"\n[autotune] {w.im}x{w.jm}x{w.km}", flush=True)
+    api.release_all()
+    a, b = api.Domain(s), api.Domain(s)
+    for n in (1, 2):
+        a.step(n, 4)
+        b.step(n, 4)
+    before = {f: a.download(f) for f in FIELDS}
+    st_before = a.press_stats()
+    r = a.autotune(reps=4)
+    assert r["rows"] in ((1, 2) if w.jm % 2 == 0 else (1,)), r
+    assert 1 <= r["kchunk"] <= w.km, r
+    assert r["us_per_iter"] > 0.0 and r["us_per_iter_default"] > 0.0, r
+    assert r["us_per_iter"] <= r["us_per_iter_default"], r
+    for f in FIELDS:
+        assert np.array_equal(a.download(f), before[f]), f
+    st_after = a.press_stats()
+    assert st_after["iters"] == st_before["iters"]
+    assert st_after["sor"].tolist() == st_before["sor"].tolist()
+    # remembered for the shape
+    assert b.autotune()["kchunk"] == r["kchunk"] and b.autotune()["rows"] == r["rows"]
+    c = api.Domain(s)  # an untuned twin stepping alongside
+    for n in (1, 2):
+        c.step(n, 4)
+    for n in (3, 4):
+        a.step(n, 4)
+        c.step(n, 4)
+    for f in FIELDS:
+        assert np.array_equal(a.download(f), c.download(f)), f
+    assert a.press_stats()["sor"].tolist() == c.press_stats()["sor"].tolist()
+    for d in (a, b, c):
+        d.close()
