@@ -336,7 +336,8 @@ def main(argv=None):
         "kernels": per_kernel,
     }
     if tuned is not None:
-        result["sor_launch"] = {"rows_per_thread": tuned["rows"], "k_levels_per_block": tuned["kchunk"],
+        result["sor_launch"] = {"rows_per_thread": tuned["rows"], "tile_rows": tuned["tile_rows"],
+                                "k_levels_per_block": tuned["kchunk"],
                                 "tuned_us_per_iter": round(tuned["us_per_iter"], 2),
                                 "default_us_per_iter": round(tuned["us_per_iter_default"], 2)}
     dom.close()

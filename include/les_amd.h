@@ -179,8 +179,9 @@ int les_domain_step(les_domain* d, int n, int nmax);
 /* With on != 0, steps n > 1 replay one captured HIP graph of the whole
  * step (recaptured when the domain's constants change); results are the
  * same as eager launches.  Off by default; ignored while timing is on. */
-/* Picks the launch shape of the fused SOR iteration (tile rows per thread,
- * k levels per block) for this grid shape: times `reps` iterations of each
+/* Picks the launch shape of the fused SOR iteration (tile rows per thread: 1
+ * or 2; rows per tile with the ring: 16 or 32; k levels per block) for this
+ * grid shape: times `reps` iterations of each
  * candidate on the domain's SOR scratch arrays and keeps the fastest.  The
  * choice is remembered per (im, jm, km) for the process; later calls with
  * reps <= 0 reuse it without timing.  Results do not depend on the shape.
@@ -188,7 +189,8 @@ int les_domain_step(les_domain* d, int n, int nmax);
  * Reports the choice and the microseconds per iteration measured, when the
  * shape was timed, for it and for the untuned default.  No reference
  * counterpart: the per-shape autotuner of the serving path. */
-int les_domain_autotune(les_domain* d, int reps, int* rows, int* kchunk, double* us_best, double* us_default);
+int les_domain_autotune(les_domain* d, int reps, int* rows, int* tile_rows, int* kchunk, double* us_best,
+                        double* us_default);
 int les_domain_set_graph(les_domain* d, int on);
 /* Advance steps n0..n1 inclusive. */
 int les_domain_run(les_domain* d, int n0, int n1, int nmax);

@@ -339,7 +339,7 @@ __device__ __forceinline__ unsigned boff(const float* base, const float* p) {
 template <bool SEP, bool ENTRY>
 __global__ __launch_bounds__(FNT) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_sor_iter(
     Grid g, SplitGrid s, Metrics m, SplitBufs sb, Ctrl* ctrl, int iter, int nparts, float omega, float pjuge,
-    int ntt, int ntj, int kchunk, int remap, const float* __restrict__ pent) {
+    int ntt, int ntj, int kchunk, int remap, const float* __restrict__ pent, int nprev) {
   __shared__ float ldsB[4][FY][FX];  // previous black iterate, levels mod 4
   __shared__ float ldsR[2][FY][FX];  // new red values, levels mod 2
   __shared__ double lds[FNT / kWave];
@@ -356,7 +356,7 @@ __global__ __launch_bounds__(FNT) __attribute__((amdgpu_waves_per_eu(8, 8))) voi
   if (iter >= 2 && bid == 0) {  // stop test of the previous iteration (press.f95:119-121)
     const int pp = (iter - 1) & 1;
     double sum = 0.0;
-    for (int b = tid; b < nparts; b += FNT) sum += sb.part[pp][1][b] + sb.part[pp][0][b];
+    for (int b = tid; b < nprev; b += FNT) sum += sb.part[pp][1][b] + sb.part[pp][0][b];
     sum = block_sum<FNT>(sum, tid, lds);
     if (tid == 0) {
       if (iter - 1 <= kSorHist) ctrl->sor_hist[iter - 2] = sum;
@@ -566,14 +566,16 @@ __global__ __launch_bounds__(FNT) __attribute__((amdgpu_waves_per_eu(8, 8))) voi
 // pair of their row (cp > nt), those of levels outside its chunk to the
 // scratch plane at the start of the allocation; loads of levels outside
 // 1..km read neighbouring planes of the same allocation and are never used.
-constexpr int GW = FY / 2, GNT = FX * GW;
 
-template <bool SEP>
-__global__ __launch_bounds__(GNT) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_sor_iter2(
-    Grid g, SplitGrid s, Metrics m, SplitBufs sb, Ctrl* ctrl, int iter, int nparts, float omega, float pjuge,
-    int ntt, int ntj, int kchunk, int remap) {
-  __shared__ float ldsB[3][FY][FX];  // previous black iterate of level kr (planes by level mod 3)
-  __shared__ float ldsR[2][FY][FX];  // new red values (planes by level mod 2)
+// TY: tile rows including the ring (16 or 32); the block has TY/2 waves.
+// nprev: partial-sum slots written by the previous iteration's launch.
+template <bool SEP, int TY>
+__global__ __launch_bounds__(FX * TY / 2) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_sor_iter2(
+    Grid g, SplitGrid s, Metrics m, SplitBufs sb, Ctrl* ctrl, int iter, int nparts, int nprev, float omega,
+    float pjuge, int ntt, int ntj, int kchunk, int remap) {
+  constexpr int GNT = FX * TY / 2, FTYT = TY - 4;
+  __shared__ float ldsB[3][TY][FX];  // previous black iterate of level kr (planes by level mod 3)
+  __shared__ float ldsR[2][TY][FX];  // new red values (planes by level mod 2)
   __shared__ double lds[GNT / kWave];
   __shared__ int skip;
   extern __shared__ float4 lvm4[];  // per level from krs: cn4l, cn4s, cncz
@@ -587,7 +589,7 @@ __global__ __launch_bounds__(GNT) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
   if (iter >= 2 && bid == 0) {  // stop test of the previous iteration (press.f95:119-121)
     const int pp = (iter - 1) & 1;
     double sum = 0.0;
-    for (int b = tid; b < nparts; b += GNT) sum += sb.part[pp][1][b] + sb.part[pp][0][b];
+    for (int b = tid; b < nprev; b += GNT) sum += sb.part[pp][1][b] + sb.part[pp][0][b];
     sum = block_sum<GNT>(sum, tid, lds);
     if (tid == 0) {
       if (iter - 1 <= kSorHist) ctrl->sor_hist[iter - 2] = sum;
@@ -611,7 +613,7 @@ __global__ __launch_bounds__(GNT) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
   const int t = bt * FTX - 2 + lane;
   const bool tin = (t >= 0) && (t < nt);
   const int rA = 2 * w, rB = rA + 1;
-  const int jlA = 1 + bj * FTY - 2 + rA, jlB = jlA + 1;  // tile rows (may lie outside 1..jm)
+  const int jlA = 1 + bj * FTYT - 2 + rA, jlB = jlA + 1;  // tile rows (may lie outside 1..jm)
   const int jA = jwrap(jlA, jm), jB = jwrap(jlB, jm);     // the rows they stand for (jA odd)
   const int k0 = 1 + bk * kchunk, k1 = min(km, k0 + kchunk - 1);
   const int krs = max(1, k0 - 1), kre = k1 + 1;
@@ -619,7 +621,7 @@ __global__ __launch_bounds__(GNT) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
   const unsigned voA = 4u * (unsigned)(tq + (jA - 1) * cp), voB = 4u * (unsigned)(tq + (jB - 1) * cp);
   const unsigned trA = 4u * (unsigned)(nt + (jA - 1) * cp), trB = 4u * (unsigned)(nt + (jB - 1) * cp);
   const bool lane_tile = tin && lane >= 2 && lane <= FX - 3;
-  const bool tileA = rA >= 2 && rA <= FY - 3 && jlA <= jm, tileB = rB >= 2 && rB <= FY - 3 && jlB <= jm;
+  const bool tileA = rA >= 2 && rA <= TY - 3 && jlA <= jm, tileB = rB >= 2 && rB <= TY - 3 && jlB <= jm;
   // per position p of the pair (i = 2t+1+p)
   float c2l[2] = {0.f, 0.f}, c2s[2] = {0.f, 0.f}, axp[2] = {0.f, 0.f};
   if (tin) {
@@ -644,13 +646,13 @@ __global__ __launch_bounds__(GNT) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
     last[p] = (2 * t + 1 + p == im);
   }
   const bool first = (t == 0);  // i = 1 (position 0)
-  constexpr int PL = FY * FX;    // floats per LDS plane
+  constexpr int PL = TY * FX;    // floats per LDS plane
   const int ownA = rA * FX + lane;
   // the other colour's i-1 neighbour of position p is at own-1+p: bases kept
-  // inside the plane (only outer-ring lanes of rows 0 and FY-1 are clamped)
+  // inside the plane (only outer-ring lanes of rows 0 and TY-1 are clamped)
   const int bsA = min(max(ownA - 1, 0), PL - 3), bsB = min(max(ownA + FX - 1, 0), PL - 3);
   const int upA = (rA == 0 ? rA : rA - 1) * FX + lane;       // row A-1 (row 0 reads itself, unused)
-  const int dnB = (rB == FY - 1 ? rB : rB + 1) * FX + lane;  // row B+1
+  const int dnB = (rB == TY - 1 ? rB : rB + 1) * FX + lane;  // row B+1
   const int nsteps = kre - krs + 1;
   for (int x = tid; x < nsteps; x += GNT) {
     const int k = krs + x;
@@ -1095,40 +1097,79 @@ __global__ __launch_bounds__(NT) void k_dc_padj(Grid g, const double* __restrict
   }
 }
 
-// One fused SOR iteration l (l >= 2, or any iteration without the entry
-// ghosts): the two-rows-per-thread kernel, unless sor_rows == 1 or jm is odd.
-static void launch_sor_iter(const DevDomain& d, const SplitBufs& sb, int l, float pjuge, int nparts, int ntt, int ntj,
-                            int kc) {
+// Launch shape of the fused SOR iteration: kernel (rows per thread), tile
+// rows and k levels per block; every shape gives the same iterates.
+struct SorShape {
+  int rows, ty, kc, ntt, ntj, nparts;
+};
+
+static SorShape sor_shape(const Grid& g, const SplitGrid& s, int rows, int ty, int kc) {
+  SorShape h;
+  h.rows = ((g.jm & 1) != 0) ? 1 : rows;  // odd jm: one row per thread
+  h.ty = (h.rows == 1) ? FY : ty;          // k_sor_iter has 16-row tiles
+  h.kc = std::max(1, std::min(kc, g.km));
+  h.ntt = (s.nt + FTX - 1) / FTX;
+  h.ntj = (g.jm + (h.ty - 4) - 1) / (h.ty - 4);
+  h.nparts = h.ntt * h.ntj * ((g.km + h.kc - 1) / h.kc);
+  return h;
+}
+
+// The domain's shape: its settings, or the automatic k chunk.
+static SorShape sor_shape(const DevDomain& d) {
+  const int kc = d.sor_kchunk > 0 ? d.sor_kchunk : sor_fused_kchunk(d.g, d.sg, d.sor_fused_blocks, d.ncu);
+  return sor_shape(d.g, d.sg, d.sor_rows, d.sor_tile_rows, kc);
+}
+
+// One fused SOR iteration l with shape h; nprev is the previous launch's
+// number of partial-sum slots; pent != nullptr for iteration 1 of press,
+// which reads the entry ghosts of p (k_sor_iter only).
+static void launch_sor_iter(const DevDomain& d, const SplitBufs& sb, int l, float pjuge, const SorShape& h, int nprev,
+                            const float* pent = nullptr) {
   const Grid& g = d.g;
   const SplitGrid& s = d.sg;
-  if (d.sor_rows == 1 || (g.jm & 1) != 0) {
-    const size_t lvm_bytes = 3 * (size_t)(kc + 2) * sizeof(float);
-    if (d.cn1_separable)
-      TIMED(d, KC_SOR, hipLaunchKernelGGL((k_sor_iter<true, false>), dim3(nparts), dim3(FX, FY), lvm_bytes, d.stream,
-                                          g, s, d.m, sb, d.ctrl, l, nparts, d.omega, pjuge, ntt, ntj, kc, d.xcd_remap,
-                                          nullptr));
-    else
-      TIMED(d, KC_SOR, hipLaunchKernelGGL((k_sor_iter<false, false>), dim3(nparts), dim3(FX, FY), lvm_bytes,
-                                          d.stream, g, s, d.m, sb, d.ctrl, l, nparts, d.omega, pjuge, ntt, ntj, kc,
-                                          d.xcd_remap, nullptr));
+  if (h.rows == 1 || pent) {
+    const size_t lvm_bytes = 3 * (size_t)(h.kc + 2) * sizeof(float);
+#define SOR1(SEP, ENT)                                                                                           \
+  TIMED(d, KC_SOR, hipLaunchKernelGGL((k_sor_iter<SEP, ENT>), dim3(h.nparts), dim3(FX, FY), lvm_bytes, d.stream, g, \
+                                      s, d.m, sb, d.ctrl, l, h.nparts, d.omega, pjuge, h.ntt, h.ntj, h.kc,          \
+                                      d.xcd_remap, pent, nprev))
+    if (d.cn1_separable) {
+      if (pent)
+        SOR1(true, true);
+      else
+        SOR1(true, false);
+    } else {
+      if (pent)
+        SOR1(false, true);
+      else
+        SOR1(false, false);
+    }
+#undef SOR1
     return;
   }
-  const size_t lvm_bytes = (size_t)(kc + 2) * sizeof(float4);
-  const dim3 grid(nparts), block(FX, GW);
-  if (d.cn1_separable)
-    TIMED(d, KC_SOR, hipLaunchKernelGGL((k_sor_iter2<true>), grid, block, lvm_bytes, d.stream, g, s, d.m, sb, d.ctrl,
-                                        l, nparts, d.omega, pjuge, ntt, ntj, kc, d.xcd_remap));
-  else
-    TIMED(d, KC_SOR, hipLaunchKernelGGL((k_sor_iter2<false>), grid, block, lvm_bytes, d.stream, g, s, d.m, sb, d.ctrl,
-                                        l, nparts, d.omega, pjuge, ntt, ntj, kc, d.xcd_remap));
+  const size_t lvm_bytes = (size_t)(h.kc + 2) * sizeof(float4);
+#define SOR2(SEP, TY)                                                                                        \
+  TIMED(d, KC_SOR, hipLaunchKernelGGL((k_sor_iter2<SEP, TY>), dim3(h.nparts), dim3(FX, TY / 2), lvm_bytes,     \
+                                      d.stream, g, s, d.m, sb, d.ctrl, l, h.nparts, nprev, d.omega, pjuge, h.ntt, \
+                                      h.ntj, h.kc, d.xcd_remap))
+  if (d.cn1_separable) {
+    if (h.ty == 32)
+      SOR2(true, 32);
+    else
+      SOR2(true, 16);
+  } else {
+    if (h.ty == 32)
+      SOR2(false, 32);
+    else
+      SOR2(false, 16);
+  }
+#undef SOR2
 }
 
 static void launch_press_dc(const DevDomain& d, const SplitBufs& sb, int gx, int gy, int gz, int nred) {
   const Grid& g = d.g;
   const SplitGrid& s = d.sg;
-  const int ntt = (s.nt + FTX - 1) / FTX, ntj = (g.jm + FTY - 1) / FTY;
-  const int kc = d.sor_kchunk > 0 ? std::min(d.sor_kchunk, g.km) : sor_fused_kchunk(g, s, d.sor_fused_blocks, d.ncu);
-  const int nparts = ntt * ntj * ((g.km + kc - 1) / kc);
+  const SorShape h = sor_shape(d);
   double* P64 = d.dc64;
   double* R64 = d.dc64 + d.n3;
   double* hist = d.dc64 + 2 * (size_t)d.n3;
@@ -1143,7 +1184,7 @@ static void launch_press_dc(const DevDomain& d, const SplitBufs& sb, int gx, int
                                           d.partials, d.ctrl));
     TIMED(d, KC_RHS, hipLaunchKernelGGL(k_dc_norm, dim3(1), dim3(256), 0, d.stream, (const double*)d.partials, nred,
                                         hist, o));
-    for (int l = 1; l <= d.dc_inner; l++) launch_sor_iter(d, sb, l, 0.0f, nparts, ntt, ntj, kc);
+    for (int l = 1; l <= d.dc_inner; l++) launch_sor_iter(d, sb, l, 0.0f, h, h.nparts);
     TIMED(d, KC_PAV, hipLaunchKernelGGL(k_dc_correct, dim3(gx, gy, g.km), blk(), 0, d.stream, g, s, sb, P64,
                                         d.dc_inner & 1));
   }
@@ -1184,22 +1225,12 @@ void launch_press(const DevDomain& d, float dt) {
     return;
   }
   if (d.sor_fused) {
-    const int ntt = (s.nt + FTX - 1) / FTX, ntj = (g.jm + FTY - 1) / FTY;
-    const int kc = d.sor_kchunk > 0 ? std::min(d.sor_kchunk, g.km) : sor_fused_kchunk(g, s, d.sor_fused_blocks, d.ncu);
-    const int nparts = ntt * ntj * ((g.km + kc - 1) / kc);
-    const size_t lvm_bytes = 3 * (size_t)(kc + 2) * sizeof(float);
-    const float* pent = d.F.p;
-    // iteration 1 reads the ghost values p had on entry (k_sor_iter<., true>)
-    if (d.cn1_separable)
-      TIMED(d, KC_SOR, hipLaunchKernelGGL((k_sor_iter<true, true>), dim3(nparts), dim3(FX, FY), lvm_bytes, d.stream, g,
-                                          s, d.m, sb, d.ctrl, 1, nparts, d.omega, d.pjuge, ntt, ntj, kc, d.xcd_remap,
-                                          pent));
-    else
-      TIMED(d, KC_SOR, hipLaunchKernelGGL((k_sor_iter<false, true>), dim3(nparts), dim3(FX, FY), lvm_bytes, d.stream,
-                                          g, s, d.m, sb, d.ctrl, 1, nparts, d.omega, d.pjuge, ntt, ntj, kc,
-                                          d.xcd_remap, pent));
-    for (int l = 2; l <= d.nmaxp; l++) launch_sor_iter(d, sb, l, d.pjuge, nparts, ntt, ntj, kc);
-    launch_press_tail(d, sb, nparts, gx, gy, gz, nred);
+    const SorShape h = sor_shape(d);
+    // iteration 1 reads the ghost values p had on entry (k_sor_iter, 16-row tiles)
+    const SorShape h1 = sor_shape(g, s, 1, FY, h.kc);
+    launch_sor_iter(d, sb, 1, d.pjuge, h1, h1.nparts, d.F.p);
+    for (int l = 2; l <= d.nmaxp; l++) launch_sor_iter(d, sb, l, d.pjuge, h, l == 2 ? h1.nparts : h.nparts);
+    launch_press_tail(d, sb, d.nmaxp == 1 ? h1.nparts : h.nparts, gx, gy, gz, nred);
     return;
   }
   const int tx = (s.nt + BX - 1) / BX;
@@ -1281,21 +1312,16 @@ int sor_fused_tiles(const Grid& g, const SplitGrid& s) {
 }
 
 // `iters` fused SOR iterations (numbered 2.., never stopping) with the given
-// rows per thread and k chunk, on whatever the split arrays hold: a timing
-// trial for the autotuner.  Writes only the split iterates, the SOR partial
-// slots and the SOR fields of the control block.
-void launch_sor_trial(const DevDomain& d0, int rows, int kc, int iters) {
+// rows per thread, tile rows and k chunk, on whatever the split arrays hold:
+// a timing trial for the autotuner.  Writes only the split iterates, the SOR
+// partial slots and the SOR fields of the control block.
+void launch_sor_trial(const DevDomain& d0, int rows, int ty, int kc, int iters) {
   DevDomain d = d0;
-  d.sor_rows = rows;
   d.timer = nullptr;
-  const Grid& g = d.g;
-  const SplitGrid& s = d.sg;
   SplitBufs sb = d.sb;
   if (d.cn1_separable) sb.cn1[0] = sb.cn1[1] = nullptr;
-  const int ntt = (s.nt + FTX - 1) / FTX, ntj = (g.jm + FTY - 1) / FTY;
-  kc = std::max(1, std::min(kc, g.km));
-  const int nparts = ntt * ntj * ((g.km + kc - 1) / kc);
-  for (int l = 2; l < 2 + iters; l++) launch_sor_iter(d, sb, l, 0.0f, nparts, ntt, ntj, kc);
+  const SorShape h = sor_shape(d.g, d.sg, rows, ty, kc);
+  for (int l = 2; l < 2 + iters; l++) launch_sor_iter(d, sb, l, 0.0f, h, h.nparts);
 }
 
 int reduction_parts_needed(const Grid& g) {

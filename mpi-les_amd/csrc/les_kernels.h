@@ -73,6 +73,7 @@ struct DevDomain {
   int ncu;           // compute units of the device
   int sor_rows;      // tile rows per thread of the fused SOR iteration (1: k_sor_iter, 2: k_sor_iter2)
   int sor_kchunk;    // k levels per block of the fused SOR iteration (0: from sor_fused_blocks)
+  int sor_tile_rows; // tile rows (with the ring) of k_sor_iter2: 16 or 32
 };
 
 int sor_kchunk(const Grid& g, const SplitGrid& s, int target);
@@ -80,7 +81,7 @@ int sor_parts_needed(const Grid& g, const SplitGrid& s, int target);
 int sor_fused_kchunk(const Grid& g, const SplitGrid& s, int target, int ncu);
 int sor_fused_parts_needed(const Grid& g, const SplitGrid& s, int target, int ncu);
 int sor_fused_tiles(const Grid& g, const SplitGrid& s);
-void launch_sor_trial(const DevDomain& d, int rows, int kc, int iters);
+void launch_sor_trial(const DevDomain& d, int rows, int ty, int kc, int iters);
 int reduction_parts_needed(const Grid& g);
 
 // Event pairs around launches, recorded only when d.timer is set.

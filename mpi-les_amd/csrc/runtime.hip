@@ -549,6 +549,7 @@ int domain_alloc(int im, int jm, int km, les_domain** out) {
   D->d.sor_fused_blocks = env_int("LES_SOR_FUSED_BLOCKS", 0);
   D->d.sor_rows = env_int("LES_SOR_ROWS", 2) == 1 ? 1 : 2;
   D->d.sor_kchunk = std::max(0, env_int("LES_SOR_KCHUNK", 0));
+  D->d.sor_tile_rows = env_int("LES_SOR_TILE_ROWS", 16) == 32 ? 32 : 16;
   {
     int dev = 0, ncu = 0;
     if (hipGetDevice(&dev) == hipSuccess &&
@@ -700,7 +701,7 @@ struct TuneKey {
   }
 };
 struct TuneResult {
-  int rows = 2, kchunk = 0;
+  int rows = 2, tile_rows = 16, kchunk = 0;
   double us_best = 0.0, us_default = 0.0;
 };
 std::mutex g_tune_mu;
@@ -723,9 +724,13 @@ int tune_sor(les_domain* D, int reps, TuneResult* out) {
     if (blocks * 4 < ncu || blocks > 8L * ncu) continue;
     if (std::find(kcs.begin(), kcs.end(), kc) == kcs.end()) kcs.push_back(kc);
   }
-  std::vector<int> rowsets{1};
-  if ((g.jm & 1) == 0) rowsets.insert(rowsets.begin(), 2);
-  const int rdef = ((g.jm & 1) == 0 && D->d.sor_rows == 2) ? 2 : 1;  // the shape used without tuning
+  // (rows per thread, tile rows): k_sor_iter2 with 16- or 32-row tiles
+  // (even jm only), k_sor_iter
+  std::vector<std::pair<int, int>> kernels{{1, 16}};
+  if ((g.jm & 1) == 0) kernels.insert(kernels.begin(), {{2, 16}, {2, 32}});
+  // the shape used without tuning
+  const std::pair<int, int> kdef2 = ((g.jm & 1) == 0 && D->d.sor_rows == 2) ? std::make_pair(2, D->d.sor_tile_rows)
+                                                                            : std::make_pair(1, 16);
   hipEvent_t e0 = nullptr, e1 = nullptr;
   HIP_TRY(hipEventCreate(&e0));
   if (hipEventCreate(&e1) != hipSuccess) {
@@ -736,19 +741,21 @@ int tune_sor(les_domain* D, int reps, TuneResult* out) {
   double best = -1.0;
   TuneResult r;
   r.rows = D->d.sor_rows;
+  r.tile_rows = D->d.sor_tile_rows;
   r.kchunk = 0;
-  for (int rows : rowsets) {
+  for (const auto& kt : kernels) {
+    const int rows = kt.first, ty = kt.second;
     for (int kc : kcs) {
       Ctrl c = saved;
       c.sor_done = 0;
       c.nonfinite = 0;
       hipError_t e = hipMemcpyAsync(D->d.ctrl, &c, sizeof(Ctrl), hipMemcpyHostToDevice, D->d.stream);
       if (e == hipSuccess) {
-        launch_sor_trial(D->d, rows, kc, 2);  // warm-up
+        launch_sor_trial(D->d, rows, ty, kc, 2);  // warm-up
         e = hipEventRecord(e0, D->d.stream);
       }
       if (e == hipSuccess) {
-        launch_sor_trial(D->d, rows, kc, reps);
+        launch_sor_trial(D->d, rows, ty, kc, reps);
         e = hipEventRecord(e1, D->d.stream);
       }
       if (e == hipSuccess) e = hipEventSynchronize(e1);
@@ -760,10 +767,11 @@ int tune_sor(les_domain* D, int reps, TuneResult* out) {
         break;
       }
       const double us = 1e3 * ms / reps;
-      if (rows == rdef && kc == kdef) r.us_default = us;
+      if (kt == kdef2 && kc == kdef) r.us_default = us;
       if (best < 0.0 || us < best) {
         best = us;
         r.rows = rows;
+        r.tile_rows = ty;
         r.kchunk = kc;
       }
     }
@@ -1292,7 +1300,8 @@ int les_domain_set_inflow(les_domain* D, const float* u_in, const float* v_in, c
   return set_inflow(D, u_in, v_in, w_in);
 }
 
-int les_domain_autotune(les_domain* D, int reps, int* rows, int* kchunk, double* us_best, double* us_default) {
+int les_domain_autotune(les_domain* D, int reps, int* rows, int* tile_rows, int* kchunk, double* us_best,
+                        double* us_default) {
   if (!D) return fail("null domain");
   std::lock_guard<std::mutex> lk(g_tune_mu);
   const Grid& g = D->d.g;
@@ -1305,8 +1314,10 @@ int les_domain_autotune(les_domain* D, int reps, int* rows, int* kchunk, double*
   }
   const TuneResult& r = it->second;
   D->d.sor_rows = r.rows;
+  D->d.sor_tile_rows = r.tile_rows;
   D->d.sor_kchunk = r.kchunk;
   if (rows) *rows = r.rows;
+  if (tile_rows) *tile_rows = r.rows == 2 ? r.tile_rows : 16;
   if (kchunk) *kchunk = r.kchunk;
   if (us_best) *us_best = r.us_best;
   if (us_default) *us_default = r.us_default;

@@ -101,7 +101,7 @@ def lib():
     L.les_domain_set_timing.argtypes = [vp, ci]
     L.les_domain_set_timing_runs.argtypes = [vp, ci]
     L.les_domain_set_graph.argtypes = [vp, ci]
-    L.les_domain_autotune.argtypes = [vp, ci, vp, vp, vp, vp]
+    L.les_domain_autotune.argtypes = [vp, ci, vp, vp, vp, vp, vp]
     L.les_domain_timing.argtypes = [vp, vp, vp]
     L.les_kernel_class_name.restype = ctypes.c_char_p
     L.les_kernel_class_name.argtypes = [ci]
@@ -286,11 +286,12 @@ class Domain:
     def autotune(self, reps: int = 0) -> dict:
         """Pick the SOR launch shape for this grid shape (cached per shape;
         reps > 0 times it again with that many iterations per candidate)."""
-        r, kc = ctypes.c_int(), ctypes.c_int()
+        r, ty, kc = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
         ub, ud = ctypes.c_double(), ctypes.c_double()
-        _check(lib().les_domain_autotune(self._h, reps, ctypes.byref(r), ctypes.byref(kc), ctypes.byref(ub),
-                                         ctypes.byref(ud)))
-        return dict(rows=r.value, kchunk=kc.value, us_per_iter=ub.value, us_per_iter_default=ud.value)
+        _check(lib().les_domain_autotune(self._h, reps, ctypes.byref(r), ctypes.byref(ty), ctypes.byref(kc),
+                                         ctypes.byref(ub), ctypes.byref(ud)))
+        return dict(rows=r.value, tile_rows=ty.value, kchunk=kc.value, us_per_iter=ub.value,
+                    us_per_iter_default=ud.value)
 
     def set_graph(self, on: bool):
         """Replay steps n > 1 from one captured HIP graph."""

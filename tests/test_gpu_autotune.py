@@ -27,6 +27,7 @@ def test_autotune_keeps_state_and_results(case):
     st_before = a.press_stats()
     r = a.autotune(reps=4)
     assert r["rows"] in ((1, 2) if w.jm % 2 == 0 else (1,)), r
+    assert r["tile_rows"] in ((16, 32) if r["rows"] == 2 else (16,)), r
     assert 1 <= r["kchunk"] <= w.km, r
     assert r["us_per_iter"] > 0.0 and r["us_per_iter_default"] > 0.0, r
     assert r["us_per_iter"] <= r["us_per_iter_default"], r
@@ -36,7 +37,8 @@ def test_autotune_keeps_state_and_results(case):
     assert st_after["iters"] == st_before["iters"]
     assert st_after["sor"].tolist() == st_before["sor"].tolist()
     # remembered for the shape
-    assert b.autotune()["kchunk"] == r["kchunk"] and b.autotune()["rows"] == r["rows"]
+    rb = b.autotune()
+    assert (rb["kchunk"], rb["rows"], rb["tile_rows"]) == (r["kchunk"], r["rows"], r["tile_rows"])
     c = api.Domain(s)  # an untuned twin stepping alongside
     for n in (1, 2):
         c.step(n, 4)

@@ -13,10 +13,8 @@ arithmetic on the CPU."""
 import numpy as np
 import pytest
 
-FX, FY = 64, 16
-FTX, FTY = FX - 4, FY - 4
-GW = FY // 2
-PL = FY * FX
+FX = 64
+FTX = FX - 4
 f32 = np.float32
 
 
@@ -34,8 +32,12 @@ class Err(Exception):
     pass
 
 
-def run(im, jm, km, kc, seed=0, omega=1.0, remap=1):
+def run(im, jm, km, kc, ty=16, seed=0, omega=1.0, remap=1):
     assert jm % 2 == 0
+    FY = ty  # tile rows with the ring
+    FTY = FY - 4
+    GW = FY // 2
+    PL = FY * FX
     rng = np.random.default_rng(seed)
     nt = (im + 1) // 2
     cp = (nt + 1 + 31) // 32 * 32
@@ -364,11 +366,12 @@ def run(im, jm, km, kc, seed=0, omega=1.0, remap=1):
 
 
 CASES = [(20, 14, 9, 9), (20, 14, 9, 3), (21, 14, 9, 4), (130, 26, 12, 5), (7, 2, 2, 1), (9, 4, 3, 2),
-         (125, 30, 20, 7), (64, 16, 13, 13), (3, 2, 5, 5), (121, 12, 8, 6)]
+         (125, 30, 20, 7), (64, 16, 13, 13), (3, 2, 5, 5), (121, 12, 8, 6), (70, 60, 11, 4)]
 
 
+@pytest.mark.parametrize("ty", [16, 32])
 @pytest.mark.parametrize("im,jm,km,kc", CASES)
-def test_sor_iter2_tiling_emulation(im, jm, km, kc):
-    bad, rel, _ = run(im, jm, km, kc)
+def test_sor_iter2_tiling_emulation(im, jm, km, kc, ty):
+    bad, rel, _ = run(im, jm, km, kc, ty)
     assert bad == 0
     assert rel < 1e-12  # the stop-test sum: same terms, another order
