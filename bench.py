@@ -127,25 +127,29 @@ def profiled_traffic(kernel_class: str, workload: str):
     return int(m.group(1)) if m else None
 
 
-def serve_measurement(nreq: int, nsteps: int) -> dict:
+def serve_measurement(nreq: int, nsteps: int, sizes=None) -> dict:
     """Mixed-size requests (nx, ny in 96..512, nz in 64..128, seeded urban
-    set-ups) all arriving at once: served concurrently (one stream and step
-    graph per domain, admission by device memory) and, for comparison, one
-    after another.  Reports aggregate cell updates per second of both and the
-    time-to-result percentiles of the concurrent run."""
+    set-ups; or the given sizes) all arriving at once: served concurrently
+    (one stream and step graph per domain, admission by device memory) and,
+    for comparison, one after another.  Reports aggregate cell updates per
+    second of both and the time-to-result percentiles of the concurrent run."""
     from lesamd import serving
-    reqs = serving.sample_requests(nreq, nsteps, seed=2024)
+    reqs = serving.sample_requests(nreq, nsteps, seed=2024, sizes=sizes)
     srv = serving.Server(budget_bytes=200 << 30, burst=4)
     print(f"[bench] serving {nreq} requests x {nsteps} steps", file=sys.stderr, flush=True)
     seq = serving.summarize(srv.run(reqs, sequential=True))
     con = serving.summarize(srv.run(reqs))
+    bat = serving.summarize(srv.run_batched(reqs))
     return {
         "requests": nreq, "steps_each": nsteps, "arrivals": "all at t=0",
-        "mcell_updates_per_s": round(con["mcell_updates_per_s"], 1),
+        "mode": "continuous batching (one batched SOR launch per iteration for all active domains)",
+        "mcell_updates_per_s": round(bat["mcell_updates_per_s"], 1),
+        "p50_s": round(bat["p50_s"], 4), "p99_s": round(bat["p99_s"], 4),
+        "speedup_vs_sequential": round(bat["mcell_updates_per_s"] / seq["mcell_updates_per_s"], 3),
         "sequential_mcell_updates_per_s": round(seq["mcell_updates_per_s"], 1),
-        "speedup_vs_sequential": round(con["mcell_updates_per_s"] / seq["mcell_updates_per_s"], 3),
-        "p50_s": round(con["p50_s"], 4), "p99_s": round(con["p99_s"], 4),
         "sequential_p50_s": round(seq["p50_s"], 4), "sequential_p99_s": round(seq["p99_s"], 4),
+        "streams_mcell_updates_per_s": round(con["mcell_updates_per_s"], 1),
+        "streams_p50_s": round(con["p50_s"], 4), "streams_p99_s": round(con["p99_s"], 4),
     }
 
 
@@ -343,6 +347,10 @@ def main(argv=None):
     dom.close()
     if args.serve_requests > 0:
         result["served"] = serve_measurement(args.serve_requests, args.serve_steps)
+        # the same service on small domains only (nx, ny in 96..160, nz 64),
+        # where one domain cannot fill the GPU
+        small = [(a, b, 64) for a in (96, 128, 160) for b in (96, 128, 160)]
+        result["served_small"] = serve_measurement(args.serve_requests, args.serve_steps, sizes=small)
     if snapshot is not None:
         result["cpu_baseline"] = cpu_baseline(setup, snapshot, W + 1, args.cpu_steps, args.cpu_budget)
     print(json.dumps(result), flush=True)

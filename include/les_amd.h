@@ -238,6 +238,29 @@ int les_domain_press_stats(les_domain* d, int* iters, double* sor_hist, int hist
 /* Bytes of device memory the domain holds. */
 long long les_domain_bytes(const les_domain* d);
 
+/* ------------------------------------------------------------ batches
+ * Several domains advanced together, one step of each per les_batch_step.
+ * Each domain's stages run on its own stream; the SOR iterations 2..nmaxp
+ * of every domain whose SOR can be batched (fused reference SOR, separable
+ * cn1, even jm) run as one launch per iteration for all of them, between
+ * events that join the streams, and each domain then finishes its press on
+ * its own stream.  With the graph on (default), steps where every n > 1
+ * replay one graph captured across all the streams (recaptured when a
+ * domain's state changes).  Results are those of stepping each domain
+ * alone.  While in a batch a domain must not be stepped on its own; the
+ * domain API may read it (les_domain_download waits for its stream).  No
+ * reference counterpart: the batched step of the serving path. */
+typedef struct les_batch les_batch;
+int les_batch_create(les_domain* const* domains, int n, les_batch** out);
+/* Waits for the batch's work and frees the batch (not the domains). */
+int les_batch_destroy(les_batch* b);
+int les_batch_set_graph(les_batch* b, int on);
+/* One step of every domain: n[q], nmax[q] for domain q (as les_domain_step). */
+int les_batch_step(les_batch* b, const int* n, const int* nmax);
+/* 1 when every step issued has completed, 0 while work is pending, -1 on error. */
+int les_batch_query(les_batch* b);
+int les_batch_synchronize(les_batch* b);
+
 /* Per-kernel timing: while on (a nonzero bit mask of kernel classes, -1 for
  * all), every launch of a selected class is bracketed by a pair of HIP
  * events on the domain's stream.  les_domain_timing fills

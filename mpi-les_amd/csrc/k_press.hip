@@ -569,10 +569,12 @@ __global__ __launch_bounds__(FNT) __attribute__((amdgpu_waves_per_eu(8, 8))) voi
 
 // TY: tile rows including the ring (16 or 32); the block has TY/2 waves.
 // nprev: partial-sum slots written by the previous iteration's launch.
+// One block's share of the iteration: bid is the block's index within the
+// launch of this domain.
 template <bool SEP, int TY>
-__global__ __launch_bounds__(FX * TY / 2) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_sor_iter2(
-    Grid g, SplitGrid s, Metrics m, SplitBufs sb, Ctrl* ctrl, int iter, int nparts, int nprev, float omega,
-    float pjuge, int ntt, int ntj, int kchunk, int remap) {
+__device__ __forceinline__ void sor_iter2_block(Grid g, SplitGrid s, Metrics m, const SplitBufs& sb, Ctrl* ctrl, int iter,
+                                                int nparts, int nprev, float omega, float pjuge, int ntt, int ntj,
+                                                int kchunk, int remap, int bid) {
   constexpr int GNT = FX * TY / 2, FTYT = TY - 4;
   __shared__ float ldsB[3][TY][FX];  // previous black iterate of level kr (planes by level mod 3)
   __shared__ float ldsR[2][TY][FX];  // new red values (planes by level mod 2)
@@ -585,7 +587,6 @@ __global__ __launch_bounds__(FX * TY / 2) __attribute__((amdgpu_waves_per_eu(4, 
   if (tid == 0) skip = (ctrl->sor_done != 0);
   __syncthreads();
   if (skip) return;
-  const int bid = blockIdx.x;
   if (iter >= 2 && bid == 0) {  // stop test of the previous iteration (press.f95:119-121)
     const int pp = (iter - 1) & 1;
     double sum = 0.0;
@@ -849,6 +850,36 @@ __global__ __launch_bounds__(FX * TY / 2) __attribute__((amdgpu_waves_per_eu(4, 
     sb.part[cur][0][bid] = sum;
     sb.part[cur][1][bid] = 0.0;
   }
+}
+
+template <bool SEP, int TY>
+__global__ __launch_bounds__(FX * TY / 2) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_sor_iter2(
+    Grid g, SplitGrid s, Metrics m, SplitBufs sb, Ctrl* ctrl, int iter, int nparts, int nprev, float omega,
+    float pjuge, int ntt, int ntj, int kchunk, int remap) {
+  sor_iter2_block<SEP, TY>(g, s, m, sb, ctrl, iter, nparts, nprev, omega, pjuge, ntt, ntj, kchunk, remap,
+                           (int)blockIdx.x);
+}
+
+// The same iteration for several domains in one launch: each domain owns a
+// range of blocks (starting at a multiple of 8, so a block's XCD is the same
+// as in a launch of its own), domains past their own nmaxp return at once.
+// Only k_sor_iter2<true, 16> shapes are batched.
+__global__ __launch_bounds__(FX * FY / 2) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_sor_iter2_batch(
+    const SorBatchEntry* __restrict__ tab, int ntab, int iter, int remap) {
+  int e = 0;
+  for (int q = 1; q < ntab; q++)
+    if ((int)blockIdx.x >= tab[q].block0) e = q;
+  const SorBatchEntry& E = tab[e];
+  const int bid = (int)blockIdx.x - E.block0;
+  if (bid >= E.nparts || iter > E.nmaxp) return;  // padding block, or this domain has finished its iterations
+  sor_iter2_block<true, FY>(E.g, E.s, E.m, E.sb, E.ctrl, iter, E.nparts, iter == 2 ? E.nparts1 : E.nparts, E.omega,
+                            E.pjuge, E.ntt, E.ntj, E.kchunk, remap, bid);
+}
+
+void launch_sor_batch(const SorBatchEntry* tab, int ntab, int nblocks, int iter, int kcmax, int remap,
+                      hipStream_t stream) {
+  hipLaunchKernelGGL(k_sor_iter2_batch, dim3(nblocks), dim3(FX, FY / 2), (size_t)(kcmax + 2) * sizeof(float4), stream,
+                     tab, ntab, iter, remap);
 }
 
 // After the iteration loop: when no iteration met the stop test, the last
@@ -1197,18 +1228,15 @@ static void launch_press_dc(const DevDomain& d, const SplitBufs& sb, int gx, int
                                        d.F.p, d.ctrl, (const double*)hist, d.dc_outer));
 }
 
-void launch_press(const DevDomain& d, float dt) {
+// The right-hand side (with bondfg), its mean, and the split arrays the SOR
+// works on (press.f95:47-87).
+static void press_head(const DevDomain& d, float dt, const SplitBufs& sb, int gx, int gy, int gz, int nred) {
   const Grid& g = d.g;
   const SplitGrid& s = d.sg;
-  const int gx = (g.im + BX - 1) / BX, gy = (g.jm + BY - 1) / BY;
-  const int gz = kblocks(gx, gy, g.km, kMaxRedBlocks);
-  const int nred = gx * gy * gz;
   TIMED(d, KC_RHS, hipLaunchKernelGGL(k_rhs, dim3(gx, gy, gz), blk(), 0, d.stream, g, d.m, d.F.u, d.F.v,
                                       d.F.w, d.F.f, d.F.g, d.F.h, d.F.rhs, d.partials, d.ctrl, dt));
   TIMED(d, KC_RHS, hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, d.stream, d.partials, nred, d.ctrl, 0,
                                       d.area));
-  SplitBufs sb = d.sb;
-  if (d.cn1_separable) sb.cn1[0] = sb.cn1[1] = nullptr;
   if (d.press_mode == 1) {
     TIMED(d, KC_RHS, hipLaunchKernelGGL(k_dc_init, dim3(gx, gy, gz), blk(), 0, d.stream, g, d.m,
                                         (const float*)d.F.u, (const float*)d.F.v, (const float*)d.F.w,
@@ -1217,9 +1245,66 @@ void launch_press(const DevDomain& d, float dt) {
     TIMED(d, KC_RHS, hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, d.stream, d.partials, nred, d.ctrl, 2,
                                         d.area));
   }
-  const int sz = g.km;
-  TIMED(d, KC_RHS, hipLaunchKernelGGL(k_rhs_split, dim3(gx, gy, sz), blk(), 0, d.stream, g, s, d.F.rhs,
+  TIMED(d, KC_RHS, hipLaunchKernelGGL(k_rhs_split, dim3(gx, gy, g.km), blk(), 0, d.stream, g, s, d.F.rhs,
                                       (const float*)d.F.p, (const float*)d.F.cn1, sb, (const Ctrl*)d.ctrl));
+}
+
+bool sor_batchable(const DevDomain& d) {
+  return d.sor_fused && d.press_mode == 0 && d.cn1_separable && (d.g.jm & 1) == 0;
+}
+
+static void batch_shapes(const DevDomain& d, SorShape* h, SorShape* h1) {
+  const int kc = d.sor_kchunk > 0 ? d.sor_kchunk : sor_fused_kchunk(d.g, d.sg, d.sor_fused_blocks, d.ncu);
+  *h = sor_shape(d.g, d.sg, 2, FY, kc);
+  *h1 = sor_shape(d.g, d.sg, 1, FY, h->kc);
+}
+
+void sor_batch_entry(const DevDomain& d, SorBatchEntry* e) {
+  SorShape h, h1;
+  batch_shapes(d, &h, &h1);
+  e->g = d.g;
+  e->s = d.sg;
+  e->m = d.m;
+  e->sb = d.sb;
+  e->sb.cn1[0] = e->sb.cn1[1] = nullptr;  // separable cn1 (sor_batchable)
+  e->ctrl = d.ctrl;
+  e->nparts = h.nparts;
+  e->nparts1 = h1.nparts;
+  e->ntt = h.ntt;
+  e->ntj = h.ntj;
+  e->kchunk = h.kc;
+  e->nmaxp = d.nmaxp;
+  e->block0 = 0;
+  e->omega = d.omega;
+  e->pjuge = d.pjuge;
+}
+
+void launch_press_part(const DevDomain& d, float dt, int part) {
+  const Grid& g = d.g;
+  const int gx = (g.im + BX - 1) / BX, gy = (g.jm + BY - 1) / BY;
+  const int gz = kblocks(gx, gy, g.km, kMaxRedBlocks);
+  const int nred = gx * gy * gz;
+  SplitBufs sb = d.sb;
+  sb.cn1[0] = sb.cn1[1] = nullptr;  // separable cn1 (sor_batchable)
+  SorShape h, h1;
+  batch_shapes(d, &h, &h1);
+  if (part == 0) {
+    press_head(d, dt, sb, gx, gy, gz, nred);
+    launch_sor_iter(d, sb, 1, d.pjuge, h1, h1.nparts, d.F.p);
+  } else {
+    launch_press_tail(d, sb, d.nmaxp == 1 ? h1.nparts : h.nparts, gx, gy, gz, nred);
+  }
+}
+
+void launch_press(const DevDomain& d, float dt) {
+  const Grid& g = d.g;
+  const SplitGrid& s = d.sg;
+  const int gx = (g.im + BX - 1) / BX, gy = (g.jm + BY - 1) / BY;
+  const int gz = kblocks(gx, gy, g.km, kMaxRedBlocks);
+  const int nred = gx * gy * gz;
+  SplitBufs sb = d.sb;
+  if (d.cn1_separable) sb.cn1[0] = sb.cn1[1] = nullptr;
+  press_head(d, dt, sb, gx, gy, gz, nred);
   if (d.press_mode == 1) {
     launch_press_dc(d, sb, gx, gy, gz, nred);
     return;

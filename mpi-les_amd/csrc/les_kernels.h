@@ -81,6 +81,33 @@ int sor_parts_needed(const Grid& g, const SplitGrid& s, int target);
 int sor_fused_kchunk(const Grid& g, const SplitGrid& s, int target, int ncu);
 int sor_fused_parts_needed(const Grid& g, const SplitGrid& s, int target, int ncu);
 int sor_fused_tiles(const Grid& g, const SplitGrid& s);
+
+// One domain of a batched SOR iteration (k_sor_iter2_batch): its arguments
+// of k_sor_iter2<true, 16> and its range of blocks in the launch.
+struct SorBatchEntry {
+  Grid g;
+  SplitGrid s;
+  Metrics m;
+  SplitBufs sb;  // with the cn1 pointers cleared (separable cn1)
+  Ctrl* ctrl;
+  int nparts;    // blocks of this domain
+  int nparts1;   // blocks of its iteration 1 (the slots iteration 2's stop test reads)
+  int ntt, ntj, kchunk, nmaxp;
+  int block0;    // first block of this domain in the launch (a multiple of 8)
+  float omega, pjuge;
+};
+void launch_sor_batch(const SorBatchEntry* tab, int ntab, int nblocks, int iter, int kcmax, int remap,
+                      hipStream_t stream);
+// The press stage in two parts around iterations 2..nmaxp of the SOR, for
+// domains whose iterations run batched: part 0 is the right-hand side and
+// iteration 1, part 1 the stop test of the last iteration and the mean
+// removal.
+void launch_press_part(const DevDomain& d, float dt, int part);
+// The domain's entry in a batched SOR launch (block0 left to the caller).
+void sor_batch_entry(const DevDomain& d, SorBatchEntry* e);
+// Whether a domain's SOR iterations can run in a batch (fused reference SOR,
+// separable cn1, even jm).
+bool sor_batchable(const DevDomain& d);
 void launch_sor_trial(const DevDomain& d, int rows, int ty, int kc, int iters);
 int reduction_parts_needed(const Grid& g);
 

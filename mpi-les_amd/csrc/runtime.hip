@@ -647,6 +647,34 @@ int run_step(les_domain* D, int n, int nmax) {
   return 0;
 }
 
+// Stages velnw..adam of a step (as run_step), then press: whole when the
+// domain's SOR cannot be batched, else its first part (the right-hand side
+// and SOR iteration 1).  Returns whether the press was split.
+int run_step_head(les_domain* D, int n, int nmax, bool* split) {
+  for (int s = 0; s < 6; s++) {
+    if (s == 2 && D->d.fuse_velfg) {
+      if (flush_metrics(D)) return 1;
+      launch_velfg_feedbf(D->d, D->vn, D->alpha, D->dt, D->beta);
+      HIP_TRY(hipGetLastError());
+      s = 3;
+      continue;
+    }
+    if (s == 4 && D->d.fuse_les) {
+      if (flush_metrics(D)) return 1;
+      launch_les_adam(D->d);
+      HIP_TRY(hipGetLastError());
+      s = 5;
+      continue;
+    }
+    if (run_stage(D, s, n, nmax)) return 1;
+  }
+  *split = sor_batchable(D->d);
+  if (!*split) return run_stage(D, 6, n, nmax);
+  launch_press_part(D->d, D->dt, 0);
+  HIP_TRY(hipGetLastError());
+  return 0;
+}
+
 int run_stage(les_domain* D, int s, int n, int nmax) {
   (void)nmax;
   if (flush_metrics(D)) return 1;
@@ -1324,6 +1352,14 @@ int les_domain_autotune(les_domain* D, int reps, int* rows, int* tile_rows, int*
   return 0;
 }
 
+// ------------------------------------------------------------ batches
+// Several domains advanced together.  Each domain's stages run on its own
+// stream; the SOR iterations 2..nmaxp of every domain whose SOR can be
+// batched run as one launch per iteration (k_sor_iter2_batch) on the
+// batch's stream, between events that join the streams; each domain then
+// finishes its press on its own stream.  Steps with every n > 1 replay one
+// graph captured across all the streams.
+
 int les_domain_set_graph(les_domain* D, int on) {
   if (!D) return fail("null domain");
   D->use_graph = on ? 1 : 0;
@@ -1406,5 +1442,217 @@ int les_domain_set_timing_runs(les_domain* D, int on) {
 int les_kernel_classes(void) { return KC_N; }
 
 const char* les_kernel_class_name(int c) { return (c >= 0 && c < KC_N) ? kClassNames[c] : ""; }
+
+}  // extern "C"
+
+struct les_batch {
+  std::vector<les_domain*> doms;
+  hipStream_t stream = nullptr;
+  hipEvent_t fork = nullptr, sor_done = nullptr;
+  std::vector<hipEvent_t> head_done, tail_done;
+  SorBatchEntry* dtab = nullptr;  // device copy of the SOR table
+  int ntab = 0, nblocks = 0, kcmax = 0, lmax = 0, remap = 1;
+  std::vector<DevDomain> key;     // domain states the table and graph were made for
+  std::vector<float> skey;        // and their scalars
+  hipGraphExec_t gexec = nullptr;
+  int use_graph = 1;
+};
+
+namespace {
+
+void batch_free(les_batch* B) {
+  if (B->stream) hipStreamSynchronize(B->stream);
+  for (les_domain* D : B->doms)
+    if (D->d.stream) hipStreamSynchronize(D->d.stream);
+  if (B->gexec) hipGraphExecDestroy(B->gexec);
+  if (B->dtab) hipFree(B->dtab);
+  for (hipEvent_t e : B->head_done) hipEventDestroy(e);
+  for (hipEvent_t e : B->tail_done) hipEventDestroy(e);
+  if (B->fork) hipEventDestroy(B->fork);
+  if (B->sor_done) hipEventDestroy(B->sor_done);
+  if (B->stream) hipStreamDestroy(B->stream);
+  delete B;
+}
+
+std::vector<float> batch_scalars(const les_batch* B) {
+  std::vector<float> v;
+  for (const les_domain* D : B->doms) v.insert(v.end(), {D->ro, D->vn, D->alpha, D->beta, D->dt});
+  return v;
+}
+
+bool batch_key_same(const les_batch* B) {
+  if (B->key.size() != B->doms.size()) return false;
+  for (size_t q = 0; q < B->doms.size(); q++)
+    if (memcmp(&B->key[q], &B->doms[q]->d, sizeof(DevDomain)) != 0) return false;
+  return B->skey == batch_scalars(B);
+}
+
+// The SOR table for the domains' current states (synchronous upload).
+int batch_prepare(les_batch* B) {
+  std::vector<SorBatchEntry> tab;
+  int block = 0;
+  B->kcmax = 0;
+  B->lmax = 0;
+  for (les_domain* D : B->doms) {
+    if (flush_metrics(D)) return 1;
+    if (!sor_batchable(D->d)) continue;
+    SorBatchEntry e;
+    sor_batch_entry(D->d, &e);
+    e.block0 = block;
+    block += (e.nparts + 7) / 8 * 8;
+    B->kcmax = std::max(B->kcmax, e.kchunk);
+    B->lmax = std::max(B->lmax, e.nmaxp);
+    tab.push_back(e);
+  }
+  B->ntab = (int)tab.size();
+  B->nblocks = block;
+  if (B->dtab) {
+    HIP_TRY(hipStreamSynchronize(B->stream));
+    HIP_TRY(hipFree(B->dtab));
+    B->dtab = nullptr;
+  }
+  if (!tab.empty()) {
+    HIP_TRY(hipMalloc(&B->dtab, tab.size() * sizeof(SorBatchEntry)));
+    HIP_TRY(hipMemcpy(B->dtab, tab.data(), tab.size() * sizeof(SorBatchEntry), hipMemcpyHostToDevice));
+  }
+  B->key.clear();
+  for (les_domain* D : B->doms) B->key.push_back(D->d);
+  B->skey = batch_scalars(B);
+  return 0;
+}
+
+// One step of every domain, enqueued on the streams (also under capture).
+int batch_enqueue(les_batch* B, const int* ns, const int* nmaxs) {
+  const size_t nd = B->doms.size();
+  HIP_TRY(hipEventRecord(B->fork, B->stream));
+  std::vector<char> split(nd, 0);
+  for (size_t q = 0; q < nd; q++) {
+    les_domain* D = B->doms[q];
+    HIP_TRY(hipStreamWaitEvent(D->d.stream, B->fork, 0));
+    bool sp = false;
+    if (run_step_head(D, ns[q], nmaxs[q], &sp)) return 1;
+    split[q] = sp;
+    HIP_TRY(hipEventRecord(B->head_done[q], D->d.stream));
+    HIP_TRY(hipStreamWaitEvent(B->stream, B->head_done[q], 0));
+  }
+  for (int l = 2; l <= B->lmax && B->ntab > 0; l++)
+    launch_sor_batch(B->dtab, B->ntab, B->nblocks, l, B->kcmax, B->remap, B->stream);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipEventRecord(B->sor_done, B->stream));
+  for (size_t q = 0; q < nd; q++) {
+    les_domain* D = B->doms[q];
+    if (split[q]) {
+      HIP_TRY(hipStreamWaitEvent(D->d.stream, B->sor_done, 0));
+      launch_press_part(D->d, D->dt, 1);
+      HIP_TRY(hipGetLastError());
+    }
+    HIP_TRY(hipEventRecord(B->tail_done[q], D->d.stream));
+    HIP_TRY(hipStreamWaitEvent(B->stream, B->tail_done[q], 0));
+  }
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int les_batch_create(les_domain* const* doms, int n, les_batch** out) {
+  if (!doms || n < 1 || !out) return fail("les_batch_create: need at least one domain");
+  for (int q = 0; q < n; q++) {
+    if (!doms[q]) return fail("les_batch_create: null domain");
+    for (int r = 0; r < q; r++)
+      if (doms[r] == doms[q]) return fail("les_batch_create: a domain appears twice");
+    if (doms[q]->d.timer) return fail("les_batch_create: a domain has timing on");
+  }
+  les_batch* B = new les_batch();
+  B->doms.assign(doms, doms + n);
+  B->remap = doms[0]->d.xcd_remap;
+  hipError_t e = hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&B->fork, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&B->sor_done, hipEventDisableTiming);
+  for (int q = 0; q < n && e == hipSuccess; q++) {
+    hipEvent_t a = nullptr, b = nullptr;
+    e = hipEventCreateWithFlags(&a, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&b, hipEventDisableTiming);
+    if (a) B->head_done.push_back(a);
+    if (b) B->tail_done.push_back(b);
+  }
+  if (e != hipSuccess) {
+    batch_free(B);
+    return fail(std::string("les_batch_create: ") + hipGetErrorString(e));
+  }
+  *out = B;
+  return 0;
+}
+
+int les_batch_destroy(les_batch* B) {
+  if (B) batch_free(B);
+  return 0;
+}
+
+int les_batch_set_graph(les_batch* B, int on) {
+  if (!B) return fail("null batch");
+  B->use_graph = on ? 1 : 0;
+  return 0;
+}
+
+int les_batch_step(les_batch* B, const int* ns, const int* nmaxs) {
+  if (!B || !ns || !nmaxs) return fail("null argument");
+  if (!batch_key_same(B)) {
+    if (B->gexec) {
+      HIP_TRY(hipStreamSynchronize(B->stream));
+      HIP_TRY(hipGraphExecDestroy(B->gexec));
+      B->gexec = nullptr;
+    }
+    if (batch_prepare(B)) return 1;
+  }
+  bool replay = B->use_graph != 0;
+  for (size_t q = 0; q < B->doms.size(); q++) replay = replay && ns[q] != 1;
+  if (!replay) return batch_enqueue(B, ns, nmaxs);
+  if (!B->gexec) {
+    for (les_domain* D : B->doms)
+      if (!D->d.fuse_velfg && ensure_cov(D)) return 1;  // allocations cannot be captured
+    HIP_TRY(hipStreamBeginCapture(B->stream, hipStreamCaptureModeThreadLocal));
+    const int rc = batch_enqueue(B, ns, nmaxs);
+    hipGraph_t gr = nullptr;
+    const hipError_t e = hipStreamEndCapture(B->stream, &gr);
+    if (rc) {
+      if (gr) hipGraphDestroy(gr);
+      return 1;
+    }
+    if (e != hipSuccess) return fail(std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    const hipError_t ei = hipGraphInstantiate(&B->gexec, gr, nullptr, nullptr, 0);
+    hipGraphDestroy(gr);
+    if (ei != hipSuccess) {
+      B->gexec = nullptr;
+      return fail(std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
+    }
+  }
+  // a graph runs on the batch stream only: order it after the domains'
+  // streams and their later work after it
+  for (size_t q = 0; q < B->doms.size(); q++) {
+    HIP_TRY(hipEventRecord(B->head_done[q], B->doms[q]->d.stream));
+    HIP_TRY(hipStreamWaitEvent(B->stream, B->head_done[q], 0));
+  }
+  HIP_TRY(hipGraphLaunch(B->gexec, B->stream));
+  HIP_TRY(hipEventRecord(B->sor_done, B->stream));
+  for (les_domain* D : B->doms) HIP_TRY(hipStreamWaitEvent(D->d.stream, B->sor_done, 0));
+  return 0;
+}
+
+int les_batch_query(les_batch* B) {
+  if (!B) return fail("null batch");
+  const hipError_t e = hipStreamQuery(B->stream);
+  if (e == hipSuccess) return 1;
+  if (e == hipErrorNotReady) return 0;
+  fail(std::string("hipStreamQuery: ") + hipGetErrorString(e));
+  return -1;
+}
+
+int les_batch_synchronize(les_batch* B) {
+  if (!B) return fail("null batch");
+  HIP_TRY(hipStreamSynchronize(B->stream));
+  return 0;
+}
 
 }  // extern "C"

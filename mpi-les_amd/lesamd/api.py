@@ -103,6 +103,12 @@ def lib():
     L.les_domain_set_graph.argtypes = [vp, ci]
     L.les_domain_autotune.argtypes = [vp, ci, vp, vp, vp, vp, vp]
     L.les_domain_timing.argtypes = [vp, vp, vp]
+    L.les_batch_create.argtypes = [vp, ci, ctypes.POINTER(vp)]
+    L.les_batch_destroy.argtypes = [vp]
+    L.les_batch_set_graph.argtypes = [vp, ci]
+    L.les_batch_step.argtypes = [vp, vp, vp]
+    L.les_batch_query.argtypes = [vp]
+    L.les_batch_synchronize.argtypes = [vp]
     L.les_kernel_class_name.restype = ctypes.c_char_p
     L.les_kernel_class_name.argtypes = [ci]
     _lib = L
@@ -328,6 +334,46 @@ class Domain:
     def close(self):
         if self._h:
             lib().les_domain_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Domains advanced together (les_batch_*): one step of each per call,
+    the SOR iterations of all of them batched into one launch per iteration."""
+
+    def __init__(self, domains, graph: bool = True):
+        self.domains = list(domains)
+        hs = (ctypes.c_void_p * len(self.domains))(*[d.handle.value for d in self.domains])
+        h = ctypes.c_void_p()
+        _check(lib().les_batch_create(hs, len(self.domains), ctypes.byref(h)))
+        self._h = h
+        _check(lib().les_batch_set_graph(h, 1 if graph else 0))
+
+    def step(self, ns, nmaxs):
+        n = np.ascontiguousarray(ns, dtype=np.int32)
+        m = np.ascontiguousarray(nmaxs, dtype=np.int32)
+        if n.size != len(self.domains) or m.size != len(self.domains):
+            raise LesError("one n and one nmax per domain")
+        _check(lib().les_batch_step(self._h, n.ctypes.data, m.ctypes.data))
+
+    def done(self) -> bool:
+        r = lib().les_batch_query(self._h)
+        if r < 0:
+            raise LesError(lib().les_last_error().decode())
+        return r == 1
+
+    def synchronize(self):
+        _check(lib().les_batch_synchronize(self._h))
+
+    def close(self):
+        if self._h:
+            lib().les_batch_destroy(self._h)
             self._h = None
 
     def __del__(self):

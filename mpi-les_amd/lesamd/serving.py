@@ -95,7 +95,7 @@ class Server:
     """
 
     def __init__(self, budget_bytes: int, burst: int = 4, max_active: int = 64, domain_factory=None,
-                 bytes_of=None, clock=time.perf_counter):
+                 bytes_of=None, clock=time.perf_counter, batch_factory=None):
         self.budget = int(budget_bytes)
         self.burst = int(burst)
         self.max_active = int(max_active)
@@ -114,6 +114,72 @@ class Server:
                                                                 req.workload.km))
         self.factory = domain_factory
         self.bytes_of = bytes_of
+        if batch_factory is None:
+            def batch_factory(doms):
+                from . import api
+                return api.Batch(doms)
+        # batch_factory(domains) -> object with step(ns, nmaxs), done(), close()
+        self.batch_factory = batch_factory
+
+    def run_batched(self, requests) -> list[Result]:
+        """Continuous batching: the admitted domains advance in lockstep,
+        one batched step (one launch per SOR iteration for all of them) at a
+        time; requests join at step boundaries as memory frees up, and leave
+        when their last step has completed."""
+        pending = deque(sorted(requests, key=lambda r: (r.arrival, r.rid)))
+        for r in pending:
+            if self.bytes_of(r) > self.budget:
+                raise ValueError(f"request {r.rid} needs more device memory than the whole budget")
+        active: list[_Active] = []
+        batch = None
+        used = 0
+        results = []
+        t0 = self.clock()
+        while pending or active:
+            now = self.clock() - t0
+            joined = False
+            while pending and pending[0].arrival <= now and len(active) < self.max_active:
+                need = self.bytes_of(pending[0])
+                if used + need > self.budget:
+                    break
+                req = pending.popleft()
+                active.append(_Active(req, self.factory(req), self.clock() - t0))
+                used += need
+                joined = True
+            if not active:
+                wait = pending[0].arrival - (self.clock() - t0)
+                if wait > 0:
+                    time.sleep(min(wait, 1e-3))
+                continue
+            if joined or batch is None:
+                if batch is not None:
+                    batch.close()
+                batch = self.batch_factory([a.dom for a in active])
+            k = min(self.burst, min(a.req.nsteps - a.issued for a in active))
+            for _ in range(k):
+                batch.step([a.issued + 1 for a in active], [a.req.nsteps for a in active])
+                for a in active:
+                    a.issued += 1
+            if any(a.issued >= a.req.nsteps for a in active):
+                while not batch.done():
+                    time.sleep(20e-6)
+                still = []
+                for a in active:
+                    if a.issued >= a.req.nsteps:
+                        results.append(Result(a.req.rid, a.req.cells, a.req.nsteps, a.req.arrival, a.admitted,
+                                              self.clock() - t0))
+                        used -= self.bytes_of(a.req)
+                    else:
+                        still.append(a)
+                batch.close()
+                batch = None
+                for a in active:
+                    if a.issued >= a.req.nsteps:
+                        a.dom.close()
+                active = still
+        if batch is not None:
+            batch.close()
+        return sorted(results, key=lambda r: r.rid)
 
     def run(self, requests, sequential: bool = False) -> list[Result]:
         """Serve the requests (arrival times relative to now); with

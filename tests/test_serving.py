@@ -106,3 +106,39 @@ def test_summary_percentiles():
     s = serving.summarize(res)
     assert s["requests"] == 100 and abs(s["p50_s"] - 50.5) < 1e-9 and s["p99_s"] > 99
     assert abs(s["mcell_updates_per_s"] - 100 * 1000 * 10 / 100 / 1e6) < 1e-12
+
+
+class FakeBatch:
+    def __init__(self, doms, blog):
+        self.doms, self.blog, self.polls = doms, blog, 0
+        blog.append(("open", tuple(d.req.rid for d in doms)))
+
+    def step(self, ns, nmaxs):
+        assert len(ns) == len(self.doms) == len(nmaxs)
+        for d, n, m in zip(self.doms, ns, nmaxs):
+            d.step(n, m)
+
+    def done(self):
+        self.polls += 1
+        return self.polls >= 2
+
+    def close(self):
+        self.blog.append(("close", tuple(d.req.rid for d in self.doms)))
+
+
+def test_batched_serving_steps_every_request_in_order_within_the_budget():
+    FakeDomain.live = FakeDomain.peak = 0
+    reqs = [serving.Request(r, serving.setup_gen.Workload(f"r{r}", 8, 8, 4), 5 + r) for r in range(6)]
+    sizes = {0: 60, 1: 30, 2: 40, 3: 30, 4: 70, 5: 10}
+    log, blog = [], []
+    srv = serving.Server(100, burst=3, domain_factory=lambda r: FakeDomain(r, log),
+                         bytes_of=lambda r: sizes[r.rid], clock=FakeClock(),
+                         batch_factory=lambda doms: FakeBatch(doms, blog))
+    res = srv.run_batched(reqs)
+    assert [r.rid for r in res] == list(range(6))
+    assert sorted(log) == [(r, tuple(range(1, 6 + r))) for r in range(6)]
+    # every batch holds admitted requests within the budget, and each is closed
+    opens = [b for kind, b in blog if kind == "open"]
+    assert all(sum(sizes[r] for r in b) <= 100 for b in opens)
+    assert len(opens) == sum(1 for kind, _ in blog if kind == "close")
+    assert FakeDomain.live == 0
