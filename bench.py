@@ -244,7 +244,9 @@ def main(argv=None):
         return 2
 
     w = setup_gen.workloads()[args.workload]
-    setup = setup_gen.build(w)
+    # the CPU baseline needs the host set-up arrays (masks, cn1); without it
+    # the light set-up (masks built on the device, cn1 recomputed) suffices
+    setup = setup_gen.build(w, dense=not args.no_cpu_baseline)
     dom = api.Domain(setup)
     W, K = args.warmup, args.steps
     nmax = W + 2 * K + 4
