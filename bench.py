@@ -142,7 +142,7 @@ def serve_measurement(nreq: int, nsteps: int, sizes=None) -> dict:
     bat = serving.summarize(srv.run_batched(reqs))
     return {
         "requests": nreq, "steps_each": nsteps, "arrivals": "all at t=0",
-        "mode": "continuous batching (one batched SOR launch per iteration for all active domains)",
+        "mode": "continuous batching (each stage one launch for all active domains, SOR one launch per iteration)",
         "mcell_updates_per_s": round(bat["mcell_updates_per_s"], 1),
         "p50_s": round(bat["p50_s"], 4), "p99_s": round(bat["p99_s"], 4),
         "speedup_vs_sequential": round(bat["mcell_updates_per_s"] / seq["mcell_updates_per_s"], 3),

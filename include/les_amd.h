@@ -240,16 +240,22 @@ long long les_domain_bytes(const les_domain* d);
 
 /* ------------------------------------------------------------ batches
  * Several domains advanced together, one step of each per les_batch_step.
- * Each domain's stages run on its own stream; the SOR iterations 2..nmaxp
- * of every domain whose SOR can be batched (fused reference SOR, separable
- * cn1, even jm) run as one launch per iteration for all of them, between
- * events that join the streams, and each domain then finishes its press on
- * its own stream.  With the graph on (default), steps where every n > 1
- * replay one graph captured across all the streams (recaptured when a
- * domain's state changes).  Results are those of stepping each domain
- * alone.  While in a batch a domain must not be stepped on its own; the
- * domain API may read it (les_domain_download waits for its stream).  No
- * reference counterpart: the batched step of the serving path. */
+ * Domains with a batchable whole step (fused reference SOR, separable cn1,
+ * even jm, velfg+feedbf and les+adam fused) run every stage as one launch
+ * for all of them: each block runs its own domain's stage on that domain's
+ * grid, and the SOR iterations 2..nmaxp run as one launch per iteration
+ * with one k chunk for the batch (LES_BATCH_SOR_KCHUNK and
+ * LES_BATCH_SOR_TILE_ROWS override it).  A step in which such a domain has
+ * n == 1 runs its stages on its own stream and only the SOR iterations
+ * batched.  Other domains step whole on their own streams, joined by
+ * events.  With the graph on (default), steps where every n > 1 replay one
+ * graph (recaptured when a domain's state changes).  Fields and SOR
+ * iteration counts are those of stepping each domain alone, bit for bit;
+ * the FP64 residual history agrees to rounding (the batch's k chunk groups
+ * the sum differently).  While in a batch a domain must not be stepped on
+ * its own; the domain API may read it (les_domain_download waits for its
+ * stream).  No reference counterpart: the batched step of the serving
+ * path. */
 typedef struct les_batch les_batch;
 int les_batch_create(les_domain* const* domains, int n, les_batch** out);
 /* Waits for the batch's work and frees the batch (not the domains). */

@@ -62,25 +62,25 @@ __device__ __forceinline__ int cix(const SplitGrid& s, int t, int j, int k) {
 // g(i,0)=g(i,jm), h(k=0)=h(km)=0; the ghosts are also stored so the arrays
 // end as the reference leaves them.  rhs = div(F) + div(u)/dt
 // (press.f95:49-60); block partials of dx1*dy1*dzn*rhs (press.f95:63-72).
-__global__ __launch_bounds__(NT) void k_rhs(Grid g, Metrics m, const float* __restrict__ u,
-                                            const float* __restrict__ v, const float* __restrict__ w,
-                                            float* __restrict__ f, float* __restrict__ gg,
-                                            float* __restrict__ h, float* __restrict__ rhs,
-                                            double* __restrict__ part, Ctrl* ctrl, float dt) {
+__device__ __forceinline__ void rhs_body(const VBlk& vb, const Grid& g, const Metrics& m,
+                                         const float* __restrict__ u, const float* __restrict__ v,
+                                         const float* __restrict__ w, float* __restrict__ f, float* __restrict__ gg,
+                                         float* __restrict__ h, float* __restrict__ rhs, double* __restrict__ part,
+                                         Ctrl* ctrl, float dt) {
   __shared__ double lds[NT / kWave];
   const int tid = threadIdx.y * BX + threadIdx.x;
-  const int bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+  const int bid = vlinear(vb);
   if (bid == 0 && tid == 0) {
     ctrl->sor_done = 0;
     ctrl->sor_iters = 0;
     ctrl->nonfinite = 0;
   }
-  const int i = 1 + blockIdx.x * BX + threadIdx.x, j = 1 + blockIdx.y * BY + threadIdx.y;
+  const int i = 1 + vb.x * BX + threadIdx.x, j = 1 + vb.y * BY + threadIdx.y;
   double acc = 0.0;
   if (i <= g.im && j <= g.jm) {
     const float dxi = M1(m.dx1, i), dyj = M1(m.dy1, j);
     int k0, k1;
-    kspan(1, g.km, k0, k1);
+    kspan(vb, 1, g.km, k0, k1);
     for (int k = k0; k <= k1; k++) {
       const int c = ix(g, i, j, k);
       const float dzk = M1(m.dzn, k);
@@ -102,11 +102,18 @@ __global__ __launch_bounds__(NT) void k_rhs(Grid g, Metrics m, const float* __re
   const double s = block_sum<NT>(acc, tid, lds);
   if (tid == 0) part[bid] = s;
 }
+__global__ __launch_bounds__(NT) void k_rhs(Grid g, Metrics m, const float* __restrict__ u,
+                                            const float* __restrict__ v, const float* __restrict__ w,
+                                            float* __restrict__ f, float* __restrict__ gg,
+                                            float* __restrict__ h, float* __restrict__ rhs,
+                                            double* __restrict__ part, Ctrl* ctrl, float dt) {
+  rhs_body(hw_blk(), g, m, u, v, w, f, gg, h, rhs, part, ctrl, dt);
+}
 
 // One block: total of n partials in a fixed order; what=0 -> rhsav,
 // what=1 -> pav (each divided by the volume).
-__global__ __launch_bounds__(256) void k_finalize(const double* __restrict__ part, int n, Ctrl* ctrl,
-                                                  int what, double denom) {
+__device__ __forceinline__ void finalize_body(const double* __restrict__ part, int n, Ctrl* ctrl, int what,
+                                              double denom) {
   __shared__ double lds[4];
   double s = 0.0;
   for (int b = threadIdx.x; b < n; b += 256) s += part[b];
@@ -120,20 +127,24 @@ __global__ __launch_bounds__(256) void k_finalize(const double* __restrict__ par
       ctrl->rhsav64 = s / denom;
   }
 }
+__global__ __launch_bounds__(256) void k_finalize(const double* __restrict__ part, int n, Ctrl* ctrl,
+                                                  int what, double denom) {
+  finalize_body(part, n, ctrl, what, denom);
+}
 
 // rhs = rhs - rhsav (press.f95:81-87, the subtraction in FP64), and the
 // colour-split copies the SOR reads: rhs, the starting iterate p, and cn1
 // when it is not recomputed.
-__global__ __launch_bounds__(NT) void k_rhs_split(Grid g, SplitGrid s, float* __restrict__ rhs,
-                                                  const float* __restrict__ p,
-                                                  const float* __restrict__ cn1, SplitBufs sb,
-                                                  const Ctrl* ctrl) {
-  const int i = 1 + blockIdx.x * BX + threadIdx.x, j = 1 + blockIdx.y * BY + threadIdx.y;
+__device__ __forceinline__ void rhs_split_body(const VBlk& vb, const Grid& g, const SplitGrid& s,
+                                               float* __restrict__ rhs, const float* __restrict__ p,
+                                               const float* __restrict__ cn1, const SplitBufs& sb,
+                                               const Ctrl* ctrl) {
+  const int i = 1 + vb.x * BX + threadIdx.x, j = 1 + vb.y * BY + threadIdx.y;
   if (i > g.im || j > g.jm) return;
   const double av = ctrl->rhsav;
   const int t = (i - 1) >> 1;
   int k0, k1;
-  kspan(1, g.km, k0, k1);
+  kspan(vb, 1, g.km, k0, k1);
   for (int k = k0; k <= k1; k++) {
     const int c = ix(g, i, j, k);
     const int col = (i + j + k) & 1;  // 1 = red
@@ -144,6 +155,11 @@ __global__ __launch_bounds__(NT) void k_rhs_split(Grid g, SplitGrid s, float* __
     sb.p[0][col][q] = p[c];
     if (sb.cn1[0]) sb.cn1[col][q] = cn1[c];
   }
+}
+__global__ __launch_bounds__(NT) void k_rhs_split(Grid g, SplitGrid s, float* __restrict__ rhs,
+                                                  const float* __restrict__ p, const float* __restrict__ cn1,
+                                                  SplitBufs sb, const Ctrl* ctrl) {
+  rhs_split_body(hw_blk(), g, s, rhs, p, cn1, sb, ctrl);
 }
 
 // ---------------------------------------------------------------- SOR
@@ -336,10 +352,12 @@ __device__ __forceinline__ unsigned boff(const float* base, const float* p) {
 // two blocks per CU need at most 64 VGPRs per lane (8 waves per SIMD)
 // ENTRY: iteration 1, which reads the ghost values p had on entry to press
 // (pent); the other iterations are compiled without that code.
+// bid: the block's index within the launch of this domain.
 template <bool SEP, bool ENTRY>
-__global__ __launch_bounds__(FNT) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_sor_iter(
-    Grid g, SplitGrid s, Metrics m, SplitBufs sb, Ctrl* ctrl, int iter, int nparts, float omega, float pjuge,
-    int ntt, int ntj, int kchunk, int remap, const float* __restrict__ pent, int nprev) {
+__device__ __forceinline__ void sor_iter_block(const Grid& g, const SplitGrid& s, const Metrics& m,
+                                               const SplitBufs& sb, Ctrl* ctrl, int iter, int nparts, float omega,
+                                               float pjuge, int ntt, int ntj, int kchunk, int remap,
+                                               const float* __restrict__ pent, int nprev, int bid) {
   __shared__ float ldsB[4][FY][FX];  // previous black iterate, levels mod 4
   __shared__ float ldsR[2][FY][FX];  // new red values, levels mod 2
   __shared__ double lds[FNT / kWave];
@@ -352,7 +370,6 @@ __global__ __launch_bounds__(FNT) __attribute__((amdgpu_waves_per_eu(8, 8))) voi
   if (tid == 0) skip = (ctrl->sor_done != 0);
   __syncthreads();
   if (skip) return;
-  const int bid = blockIdx.x;
   if (iter >= 2 && bid == 0) {  // stop test of the previous iteration (press.f95:119-121)
     const int pp = (iter - 1) & 1;
     double sum = 0.0;
@@ -540,6 +557,13 @@ __global__ __launch_bounds__(FNT) __attribute__((amdgpu_waves_per_eu(8, 8))) voi
     sb.part[cur][0][bid] = sum;
     sb.part[cur][1][bid] = 0.0;
   }
+}
+template <bool SEP, bool ENTRY>
+__global__ __launch_bounds__(FNT) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_sor_iter(
+    Grid g, SplitGrid s, Metrics m, SplitBufs sb, Ctrl* ctrl, int iter, int nparts, float omega, float pjuge,
+    int ntt, int ntj, int kchunk, int remap, const float* __restrict__ pent, int nprev) {
+  sor_iter_block<SEP, ENTRY>(g, s, m, sb, ctrl, iter, nparts, omega, pjuge, ntt, ntj, kchunk, remap, pent, nprev,
+                             (int)blockIdx.x);
 }
 
 // ------------------------------- fused SOR iteration, two rows per thread
@@ -863,28 +887,34 @@ __global__ __launch_bounds__(FX * TY / 2) __attribute__((amdgpu_waves_per_eu(4, 
 // The same iteration for several domains in one launch: each domain owns a
 // range of blocks (starting at a multiple of 8, so a block's XCD is the same
 // as in a launch of its own), domains past their own nmaxp return at once.
-// Only k_sor_iter2<true, 16> shapes are batched.
-__global__ __launch_bounds__(FX * FY / 2) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_sor_iter2_batch(
-    const SorBatchEntry* __restrict__ tab, int ntab, int iter, int remap) {
+// Only k_sor_iter2<true, TY> shapes are batched, with one TY for the batch.
+template <int TY>
+__global__ __launch_bounds__(FX * TY / 2) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_sor_iter2_batch(
+    const SorBatchEntry* __restrict__ tab, const int* __restrict__ block0, int ntab, int iter, int remap) {
   int e = 0;
-  for (int q = 1; q < ntab; q++)
-    if ((int)blockIdx.x >= tab[q].block0) e = q;
+  for (int q = 1; q < ntab; q++)  // block0[q] == tab[q].block0, packed
+    if ((int)blockIdx.x >= block0[q]) e = q;
   const SorBatchEntry& E = tab[e];
   const int bid = (int)blockIdx.x - E.block0;
   if (bid >= E.nparts || iter > E.nmaxp) return;  // padding block, or this domain has finished its iterations
-  sor_iter2_block<true, FY>(E.g, E.s, E.m, E.sb, E.ctrl, iter, E.nparts, iter == 2 ? E.nparts1 : E.nparts, E.omega,
+  sor_iter2_block<true, TY>(E.g, E.s, E.m, E.sb, E.ctrl, iter, E.nparts, iter == 2 ? E.nparts1 : E.nparts, E.omega,
                             E.pjuge, E.ntt, E.ntj, E.kchunk, remap, bid);
 }
 
-void launch_sor_batch(const SorBatchEntry* tab, int ntab, int nblocks, int iter, int kcmax, int remap,
-                      hipStream_t stream) {
-  hipLaunchKernelGGL(k_sor_iter2_batch, dim3(nblocks), dim3(FX, FY / 2), (size_t)(kcmax + 2) * sizeof(float4), stream,
-                     tab, ntab, iter, remap);
+void launch_sor_batch(const SorBatchEntry* tab, const int* block0, int ntab, int nblocks, int iter, int kcmax,
+                      int ty, int remap, hipStream_t stream) {
+  const size_t lds = (size_t)(kcmax + 2) * sizeof(float4);
+  if (ty == 32)
+    hipLaunchKernelGGL(k_sor_iter2_batch<32>, dim3(nblocks), dim3(FX, 16), lds, stream, tab, block0, ntab, iter,
+                       remap);
+  else
+    hipLaunchKernelGGL(k_sor_iter2_batch<FY>, dim3(nblocks), dim3(FX, FY / 2), lds, stream, tab, block0, ntab, iter,
+                       remap);
 }
 
 // After the iteration loop: when no iteration met the stop test, the last
 // one (nmaxp) is summed here.  Records the iteration count and sum.
-__global__ __launch_bounds__(256) void k_sor_final(SplitBufs sb, Ctrl* ctrl, int nmaxp, int nparts) {
+__device__ __forceinline__ void sor_final_body(const SplitBufs& sb, Ctrl* ctrl, int nmaxp, int nparts) {
   __shared__ double lds[4];
   if (ctrl->sor_done != 0) return;  // uniform: written by earlier launches
   const int pp = nmaxp & 1;
@@ -897,23 +927,26 @@ __global__ __launch_bounds__(256) void k_sor_final(SplitBufs sb, Ctrl* ctrl, int
     if (!(sum == sum) || sum > 1.0e300) ctrl->nonfinite = 1;
   }
 }
+__global__ __launch_bounds__(256) void k_sor_final(SplitBufs sb, Ctrl* ctrl, int nmaxp, int nparts) {
+  sor_final_body(sb, ctrl, nmaxp, nparts);
+}
 
 // pav partials: p*dx1*dy1*dzn over the SOR result (press.f95:124-133).
-__global__ __launch_bounds__(NT) void k_pav(Grid g, SplitGrid s, Metrics m, SplitBufs sb,
-                                            double* __restrict__ part, const Ctrl* ctrl) {
+__device__ __forceinline__ void pav_body(const VBlk& vb, const Grid& g, const SplitGrid& s, const Metrics& m,
+                                         const SplitBufs& sb, double* __restrict__ part, const Ctrl* ctrl) {
   __shared__ double lds[NT / kWave];
   const int tid = threadIdx.y * BX + threadIdx.x;
-  const int bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+  const int bid = vlinear(vb);
   const int L = ctrl->sor_iters;
   const float* __restrict__ red = sb.p[L & 1][1];
   const float* __restrict__ blk = sb.p[L & 1][0];
-  const int i = 1 + blockIdx.x * BX + threadIdx.x, j = 1 + blockIdx.y * BY + threadIdx.y;
+  const int i = 1 + vb.x * BX + threadIdx.x, j = 1 + vb.y * BY + threadIdx.y;
   double acc = 0.0;
   if (i <= g.im && j <= g.jm) {
     const float dxi = M1(m.dx1, i), dyj = M1(m.dy1, j);
     const int t = (i - 1) >> 1;
     int k0, k1;
-    kspan(1, g.km, k0, k1);
+    kspan(vb, 1, g.km, k0, k1);
     for (int k = k0; k <= k1; k++) {
       const float pv = (((i + j + k) & 1) ? red : blk)[cix(s, t, j, k)];
       acc += (double)(pv * dxi * dyj * M1(m.dzn, k));
@@ -922,15 +955,19 @@ __global__ __launch_bounds__(NT) void k_pav(Grid g, SplitGrid s, Metrics m, Spli
   const double sum = block_sum<NT>(acc, tid, lds);
   if (tid == 0) part[bid] = sum;
 }
+__global__ __launch_bounds__(NT) void k_pav(Grid g, SplitGrid s, Metrics m, SplitBufs sb,
+                                            double* __restrict__ part, const Ctrl* ctrl) {
+  pav_body(hw_blk(), g, s, m, sb, part, ctrl);
+}
 
 // p = p - pav on the interior (press.f95:145-150, FP64 subtraction) and the
 // ghost cells of boundp1 then boundp2 (press.f95:155-159): every ghost
 // receives the adjusted value of the interior cell the copies trace it to
 // (i: 0->1, im+1->im; j: 0->jm, jm+1->1; k: 0->1, km+1->km).  Covers
 // i,j,k = 0..im+1, 0..jm+1, 0..km+1.
-__global__ __launch_bounds__(NT) void k_padj(Grid g, SplitGrid s, SplitBufs sb, float* __restrict__ p,
-                                             const Ctrl* ctrl) {
-  const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+__device__ __forceinline__ void padj_body(const VBlk& vb, const Grid& g, const SplitGrid& s, const SplitBufs& sb,
+                                          float* __restrict__ p, const Ctrl* ctrl) {
+  const int i = vb.x * BX + threadIdx.x, j = vb.y * BY + threadIdx.y;
   if (i > g.im + 1 || j > g.jm + 1) return;
   const int L = ctrl->sor_iters;
   const float* __restrict__ red = sb.p[L & 1][1];
@@ -940,12 +977,16 @@ __global__ __launch_bounds__(NT) void k_padj(Grid g, SplitGrid s, SplitBufs sb, 
   const int jj = (j == 0) ? g.jm : ((j == g.jm + 1) ? 1 : j);
   const int t = (ii - 1) >> 1;
   int k0, k1;
-  kspan(0, g.km + 1, k0, k1);
+  kspan(vb, 0, g.km + 1, k0, k1);
   for (int k = k0; k <= k1; k++) {
     const int kk = (k == 0) ? 1 : ((k == g.km + 1) ? g.km : k);
     const float v = (((ii + jj + kk) & 1) ? red : blk)[cix(s, t, jj, kk)];
     p[ix(g, i, j, k)] = (float)((double)v - av);
   }
+}
+__global__ __launch_bounds__(NT) void k_padj(Grid g, SplitGrid s, SplitBufs sb, float* __restrict__ p,
+                                             const Ctrl* ctrl) {
+  padj_body(hw_blk(), g, s, sb, p, ctrl);
 }
 
 // After the SOR loop: the closing stop-test sum, the pressure mean and the
@@ -1253,15 +1294,45 @@ bool sor_batchable(const DevDomain& d) {
   return d.sor_fused && d.press_mode == 0 && d.cn1_separable && (d.g.jm & 1) == 0;
 }
 
-static void batch_shapes(const DevDomain& d, SorShape* h, SorShape* h1) {
-  const int kc = d.sor_kchunk > 0 ? d.sor_kchunk : sor_fused_kchunk(d.g, d.sg, d.sor_fused_blocks, d.ncu);
-  *h = sor_shape(d.g, d.sg, 2, FY, kc);
+// Shapes of a batched domain's SOR: iterations 2.. (h: tile rows ty, k
+// chunk kc; kc 0: the domain's own) and iteration 1 (h1, k_sor_iter).
+static void batch_shapes(const DevDomain& d, int kc, int ty, SorShape* h, SorShape* h1) {
+  if (kc <= 0) kc = d.sor_kchunk > 0 ? d.sor_kchunk : sor_fused_kchunk(d.g, d.sg, d.sor_fused_blocks, d.ncu);
+  *h = sor_shape(d.g, d.sg, 2, ty == 32 ? 32 : FY, kc);
   *h1 = sor_shape(d.g, d.sg, 1, FY, h->kc);
 }
 
-void sor_batch_entry(const DevDomain& d, SorBatchEntry* e) {
+// One k chunk for all batched domains: the one that minimises (rounds of
+// resident blocks) x (steps per block) over the whole batch, with two
+// k_sor_iter2<true, 16> blocks (one <true, 32>) resident per CU.  A domain
+// alone picks its chunk to fill the device by itself (a few levels for a
+// small domain); in a batch the other domains fill it, and longer chunks
+// waste fewer steps on the chunk ends.
+int sor_batch_kchunk(const DevDomain* const* ds, int n, int ty, int ncu) {
+  int kmax = 1;
+  for (int q = 0; q < n; q++) kmax = std::max(kmax, ds[q]->g.km);
+  const long slots = (ty == 32 ? 1L : 2L) * (ncu > 0 ? ncu : 256);
+  long best = -1;
+  int best_kc = kmax;
+  for (int kc = 1; kc <= kmax; kc++) {
+    long blocks = 0, steps = 0;
+    for (int q = 0; q < n; q++) {
+      const SorShape h = sor_shape(ds[q]->g, ds[q]->sg, 2, ty == 32 ? 32 : FY, kc);
+      blocks += h.nparts;
+      steps = std::max(steps, (long)h.kc + 2);
+    }
+    const long cost = (blocks + slots - 1) / slots * steps;
+    if (best < 0 || cost < best) {
+      best = cost;
+      best_kc = kc;
+    }
+  }
+  return best_kc;
+}
+
+void sor_batch_entry(const DevDomain& d, int kc, int ty, SorBatchEntry* e) {
   SorShape h, h1;
-  batch_shapes(d, &h, &h1);
+  batch_shapes(d, kc, ty, &h, &h1);
   e->g = d.g;
   e->s = d.sg;
   e->m = d.m;
@@ -1279,7 +1350,7 @@ void sor_batch_entry(const DevDomain& d, SorBatchEntry* e) {
   e->pjuge = d.pjuge;
 }
 
-void launch_press_part(const DevDomain& d, float dt, int part) {
+void launch_press_part(const DevDomain& d, float dt, int part, int kc, int ty) {
   const Grid& g = d.g;
   const int gx = (g.im + BX - 1) / BX, gy = (g.jm + BY - 1) / BY;
   const int gz = kblocks(gx, gy, g.km, kMaxRedBlocks);
@@ -1287,12 +1358,106 @@ void launch_press_part(const DevDomain& d, float dt, int part) {
   SplitBufs sb = d.sb;
   sb.cn1[0] = sb.cn1[1] = nullptr;  // separable cn1 (sor_batchable)
   SorShape h, h1;
-  batch_shapes(d, &h, &h1);
+  batch_shapes(d, kc, ty, &h, &h1);
   if (part == 0) {
     press_head(d, dt, sb, gx, gy, gz, nred);
     launch_sor_iter(d, sb, 1, d.pjuge, h1, h1.nparts, d.F.p);
   } else {
     launch_press_tail(d, sb, d.nmaxp == 1 ? h1.nparts : h.nparts, gx, gy, gz, nred);
+  }
+}
+
+// ------------------------------------------------ batched press stages
+// The press launches of launch_press_part for all domains of a batched step,
+// one launch per stage (as the step stages in k_step.hip): each block runs
+// the stage kernel's body on its own domain with that domain's grid, block
+// index and arguments, so every value and every partial sum is the one of
+// the domain's own launch.
+#define BATCH_BLOCK(stage)                                          \
+  int e;                                                            \
+  VBlk vb;                                                          \
+  if (!batch_block(sp + (stage) * nd, nd, e, vb)) return;           \
+  const BatchDom& B = bd[e];                                        \
+  const DevDomain& d = B.d
+
+__global__ __launch_bounds__(NT) void kb_rhs(const BatchDom* __restrict__ bd, const StageSpan* __restrict__ sp,
+                                             int nd) {
+  BATCH_BLOCK(BS_RHS);
+  rhs_body(vb, d.g, d.m, d.F.u, d.F.v, d.F.w, d.F.f, d.F.g, d.F.h, d.F.rhs, d.partials, d.ctrl, B.dt);
+}
+// one block per domain
+__global__ __launch_bounds__(256) void kb_finalize(const BatchDom* __restrict__ bd, int what) {
+  const BatchDom& B = bd[blockIdx.x];
+  finalize_body(B.d.partials, B.nred, B.d.ctrl, what, B.d.area);
+}
+__global__ __launch_bounds__(NT) void kb_rhs_split(const BatchDom* __restrict__ bd,
+                                                   const StageSpan* __restrict__ sp, int nd) {
+  BATCH_BLOCK(BS_SPLIT);
+  rhs_split_body(vb, d.g, d.sg, d.F.rhs, d.F.p, d.F.cn1, B.sb, d.ctrl);
+}
+__global__ __launch_bounds__(FNT) __attribute__((amdgpu_waves_per_eu(8, 8))) void kb_sor_entry(
+    const BatchDom* __restrict__ bd, const StageSpan* __restrict__ sp, int nd) {
+  BATCH_BLOCK(BS_SOR1);
+  sor_iter_block<true, true>(d.g, d.sg, d.m, B.sb, d.ctrl, 1, B.nparts1, d.omega, d.pjuge, B.ntt1, B.ntj1, B.kc1,
+                             d.xcd_remap, d.F.p, B.nparts1, vb.x);
+}
+// one block per domain
+__global__ __launch_bounds__(256) void kb_sor_final(const BatchDom* __restrict__ bd) {
+  const BatchDom& B = bd[blockIdx.x];
+  sor_final_body(B.sb, B.d.ctrl, B.d.nmaxp, B.nparts_last);
+}
+__global__ __launch_bounds__(NT) void kb_pav(const BatchDom* __restrict__ bd, const StageSpan* __restrict__ sp,
+                                             int nd) {
+  BATCH_BLOCK(BS_PAV);
+  pav_body(vb, d.g, d.sg, d.m, B.sb, d.partials, d.ctrl);
+}
+__global__ __launch_bounds__(NT) void kb_padj(const BatchDom* __restrict__ bd, const StageSpan* __restrict__ sp,
+                                              int nd) {
+  BATCH_BLOCK(BS_PADJ);
+  padj_body(vb, d.g, d.sg, B.sb, d.F.p, d.ctrl);
+}
+#undef BATCH_BLOCK
+
+void press_batch_spans(const DevDomain& d, int kc, int ty, int nzb, BatchDom* e, StageSpan* sp) {
+  const Grid& g = d.g;
+  const int gx = (g.im + BX - 1) / BX, gy = (g.jm + BY - 1) / BY;
+  const int gz = kblocks(gx, gy, g.km, kMaxRedBlocks);
+  e->nred = gx * gy * gz;
+  e->sb = d.sb;
+  e->sb.cn1[0] = e->sb.cn1[1] = nullptr;  // separable cn1 (sor_batchable)
+  SorShape h, h1;
+  batch_shapes(d, kc, ty, &h, &h1);
+  e->ntt1 = h1.ntt;
+  e->ntj1 = h1.ntj;
+  e->kc1 = h1.kc;
+  e->nparts1 = h1.nparts;
+  e->nparts_last = d.nmaxp == 1 ? h1.nparts : h.nparts;
+  const int px = (g.im + 2 + BX - 1) / BX, py = (g.jm + 2 + BY - 1) / BY;
+  // the reductions keep their own grids (their partial sums are the domain's
+  // own); the copies may run longer k runs per block
+  sp[BS_RHS] = {0, gx, gy, gz};
+  sp[BS_SPLIT] = {0, gx, gy, nzb > 0 ? std::min(nzb, g.km) : g.km};
+  sp[BS_SOR1] = {0, h1.nparts, 1, 1};
+  sp[BS_PAV] = {0, gx, gy, gz};
+  sp[BS_PADJ] = {0, px, py, nzb > 0 ? std::min(nzb, g.km + 2) : g.km + 2};
+}
+
+void launch_press_batch(const StageBatch& b, int part, hipStream_t stream) {
+  const BatchDom* bd = b.doms;
+  const StageSpan* sp = b.spans;
+  const int nd = b.nd;
+  auto nb = [&](int s) { return dim3((unsigned)b.nblocks[s]); };
+  if (part == 0) {
+    hipLaunchKernelGGL(kb_rhs, nb(BS_RHS), blk(), 0, stream, bd, sp, nd);
+    hipLaunchKernelGGL(kb_finalize, dim3((unsigned)nd), dim3(256), 0, stream, bd, 0);
+    hipLaunchKernelGGL(kb_rhs_split, nb(BS_SPLIT), blk(), 0, stream, bd, sp, nd);
+    hipLaunchKernelGGL(kb_sor_entry, nb(BS_SOR1), dim3(FX, FY), 3 * (size_t)(b.kc1max + 2) * sizeof(float), stream,
+                       bd, sp, nd);
+  } else {
+    hipLaunchKernelGGL(kb_sor_final, dim3((unsigned)nd), dim3(256), 0, stream, bd);
+    hipLaunchKernelGGL(kb_pav, nb(BS_PAV), blk(), 0, stream, bd, sp, nd);
+    hipLaunchKernelGGL(kb_finalize, dim3((unsigned)nd), dim3(256), 0, stream, bd, 1);
+    hipLaunchKernelGGL(kb_padj, nb(BS_PADJ), blk(), 0, stream, bd, sp, nd);
   }
 }
 

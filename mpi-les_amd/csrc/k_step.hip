@@ -2,6 +2,8 @@
 // adam) and the layout-conversion kernels.  Expressions are evaluated in the
 // order of the cited Fortran lines and the library is built without FMA
 // contraction, so each FP32 result equals the reference's.
+#include <algorithm>
+
 #include "les_kernels.h"
 
 namespace les {
@@ -22,17 +24,16 @@ static inline int kz(const Grid& g, int extra = 0) { return g.km + extra; }
 
 // ------------------------------------------------------------------ velnw
 // velnw.f95:27-59: u,v for k=1..km, w for k=1..km-1.
-__global__ __launch_bounds__(NT) void k_velnw(Grid g, Metrics m, const float* __restrict__ p,
-                                              const float* __restrict__ f,
-                                              const float* __restrict__ gg,
-                                              const float* __restrict__ h, float* __restrict__ u,
-                                              float* __restrict__ v, float* __restrict__ w,
-                                              float ro, float dt) {
-  const int i = 1 + blockIdx.x * BX + threadIdx.x, j = 1 + blockIdx.y * BY + threadIdx.y;
+__device__ __forceinline__ void velnw_body(const VBlk& vb, const Grid& g, const Metrics& m,
+                                           const float* __restrict__ p, const float* __restrict__ f,
+                                           const float* __restrict__ gg, const float* __restrict__ h,
+                                           float* __restrict__ u, float* __restrict__ v, float* __restrict__ w,
+                                           float ro, float dt) {
+  const int i = 1 + vb.x * BX + threadIdx.x, j = 1 + vb.y * BY + threadIdx.y;
   if (i > g.im || j > g.jm) return;
   const float rdx = M1(m.dxs, i), rdy = M1(m.dys, j);
   int k0, k1;
-  kspan(1, g.km, k0, k1);
+  kspan(vb, 1, g.km, k0, k1);
   for (int k = k0; k <= k1; k++) {
     const int c = ix(g, i, j, k);
     const float pc = p[c];
@@ -47,6 +48,13 @@ __global__ __launch_bounds__(NT) void k_velnw(Grid g, Metrics m, const float* __
   }
 }
 
+__global__ __launch_bounds__(NT) void k_velnw(Grid g, Metrics m, const float* __restrict__ p,
+                                              const float* __restrict__ f, const float* __restrict__ gg,
+                                              const float* __restrict__ h, float* __restrict__ u,
+                                              float* __restrict__ v, float* __restrict__ w, float ro, float dt) {
+  velnw_body(hw_blk(), g, m, p, f, gg, h, u, v, w, ro, dt);
+}
+
 void launch_velnw(const DevDomain& d, float ro, float dt) {
   const Grid& g = d.g;
   TIMED(d, KC_VELNW, hipLaunchKernelGGL(k_velnw, grid2d(g.im, g.jm, kz(g)), block2d(), 0, d.stream, g, d.m, d.F.p,
@@ -58,15 +66,15 @@ void launch_velnw(const DevDomain& d, float ro, float dt) {
 // i=1 over i=2..im (bondv1.f95:61-88): u=u_in(j,k), v=v_in, w=w_in (zero
 // unless set) for i=0..ilast.
 // Also clears the outflow max/min accumulators.
-__global__ __launch_bounds__(NT) void k_bondv1_inflow(Grid g, Metrics m, float* __restrict__ u,
-                                                      float* __restrict__ v, float* __restrict__ w,
-                                                      int ilast, Ctrl* ctrl) {
-  if (blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0 && threadIdx.x == 0 && threadIdx.y == 0) {
+__device__ __forceinline__ void bondv1_inflow_body(const VBlk& vb, const Grid& g, const Metrics& m,
+                                                   float* __restrict__ u, float* __restrict__ v,
+                                                   float* __restrict__ w, int ilast, Ctrl* ctrl) {
+  if (vb.x == 0 && vb.y == 0 && vb.z == 0 && threadIdx.x == 0 && threadIdx.y == 0) {
     ctrl->uout_max = 0u;
     ctrl->uout_nmin = 0u;
   }
-  const int i = blockIdx.x * BX + threadIdx.x, j = 1 + blockIdx.y * BY + threadIdx.y;
-  const int k = 1 + blockIdx.z;
+  const int i = vb.x * BX + threadIdx.x, j = 1 + vb.y * BY + threadIdx.y;
+  const int k = 1 + vb.z;
   if (i > ilast || j > g.jm) return;
   const int q = (j - 1) + (k - 1) * g.jm;
   const int c = ix(g, i, j, k);
@@ -74,14 +82,20 @@ __global__ __launch_bounds__(NT) void k_bondv1_inflow(Grid g, Metrics m, float* 
   v[c] = m.inflow_v ? m.inflow_v[q] : 0.0f;  // bondv1_data24.f95:79-87 sets v and w too
   w[c] = m.inflow_w ? m.inflow_w[q] : 0.0f;
 }
+__global__ __launch_bounds__(NT) void k_bondv1_inflow(Grid g, Metrics m, float* __restrict__ u,
+                                                      float* __restrict__ v, float* __restrict__ w,
+                                                      int ilast, Ctrl* ctrl) {
+  bondv1_inflow_body(hw_blk(), g, m, u, v, w, ilast, ctrl);
+}
 
 // aaa = max(0, max u(im,j,k)), bbb = min(0, min u(im,j,k)) (bondv1.f95:98-114).
 // Max/min are exact in any order; non-negative floats order like their bits.
-__global__ __launch_bounds__(256) void k_bondv1_uout(Grid g, const float* __restrict__ u, Ctrl* ctrl) {
+__device__ __forceinline__ void bondv1_uout_body(const VBlk& vb, const Grid& g, const float* __restrict__ u,
+                                                 Ctrl* ctrl) {
   __shared__ float smax[4], smin[4];
   float mx = 0.0f, mn = 0.0f;
   const int n = g.jm * g.km;
-  for (int t = blockIdx.x * 256 + threadIdx.x; t < n; t += gridDim.x * 256) {
+  for (int t = vb.x * 256 + threadIdx.x; t < n; t += vb.nx * 256) {
     const int j = 1 + t % g.jm, k = 1 + t / g.jm;
     const float x = u[ix(g, g.im, j, k)];
     mx = (x > mx) ? x : mx;
@@ -106,12 +120,15 @@ __global__ __launch_bounds__(256) void k_bondv1_uout(Grid g, const float* __rest
     atomicMax(&ctrl->uout_nmin, __float_as_uint(0.0f - mn));
   }
 }
+__global__ __launch_bounds__(256) void k_bondv1_uout(Grid g, const float* __restrict__ u, Ctrl* ctrl) {
+  bondv1_uout_body(hw_blk(), g, u, ctrl);
+}
 
 // Advective outflow at i=im (u) and i=im+1 (v,w), bondv1.f95:116-134.
-__global__ __launch_bounds__(256) void k_bondv1_outflow(Grid g, Metrics m, float* __restrict__ u,
-                                                        float* __restrict__ v, float* __restrict__ w,
-                                                        const Ctrl* ctrl, float dt) {
-  const int t = blockIdx.x * 256 + threadIdx.x;
+__device__ __forceinline__ void bondv1_outflow_body(const VBlk& vb, const Grid& g, const Metrics& m,
+                                                    float* __restrict__ u, float* __restrict__ v,
+                                                    float* __restrict__ w, const Ctrl* ctrl, float dt) {
+  const int t = vb.x * 256 + threadIdx.x;
   if (t >= g.jm * g.km) return;
   const int j = 1 + t % g.jm, k = 1 + t / g.jm;
   const float aaa = __uint_as_float(ctrl->uout_max);
@@ -124,6 +141,11 @@ __global__ __launch_bounds__(256) void k_bondv1_outflow(Grid g, Metrics m, float
   v[c] = v[c] - dt * uout * (v[c] - v[c - 1]) / dxs;
   w[c] = w[c] - dt * uout * (w[c] - w[c - 1]) / dxs;
 }
+__global__ __launch_bounds__(256) void k_bondv1_outflow(Grid g, Metrics m, float* __restrict__ u,
+                                                        float* __restrict__ v, float* __restrict__ w,
+                                                        const Ctrl* ctrl, float dt) {
+  bondv1_outflow_body(hw_blk(), g, m, u, v, w, ctrl, dt);
+}
 
 // Periodic j ghosts (bondv1.f95:137-156) followed by the ground/top rows
 // (bondv1.f95:176-195), written as their final values:
@@ -131,11 +153,11 @@ __global__ __launch_bounds__(256) void k_bondv1_outflow(Grid g, Metrics m, float
 //   part B: k=0 and k=km+1 rows of u,v for j=0..jm+1 (periodic image of
 //           j folded in), w(k=0)=w(k=km)=0 for j=-1..jm+1.
 // Every value read is an interior j row, none of which is written here.
-__global__ __launch_bounds__(256) void k_bondv1_ghost(Grid g, float* __restrict__ u,
-                                                      float* __restrict__ v, float* __restrict__ w) {
+__device__ __forceinline__ void bondv1_ghost_body(const VBlk& vb, const Grid& g, float* __restrict__ u,
+                                                  float* __restrict__ v, float* __restrict__ w) {
   const int ni = g.im + 2;
   const int nA = ni * g.km, nB = ni * (g.jm + 3);
-  int t = blockIdx.x * 256 + threadIdx.x;
+  int t = vb.x * 256 + threadIdx.x;
   if (t < nA) {
     const int i = t % ni, k = 1 + t / ni;
     const int c0 = ix(g, i, 0, k), cjm = ix(g, i, g.jm, k), c1 = ix(g, i, 1, k), cj1 = ix(g, i, g.jm + 1, k);
@@ -162,20 +184,32 @@ __global__ __launch_bounds__(256) void k_bondv1_ghost(Grid g, float* __restrict_
   w[ix(g, i, j, 0)] = 0.0f;
   w[ix(g, i, j, g.km)] = 0.0f;
 }
+__global__ __launch_bounds__(256) void k_bondv1_ghost(Grid g, float* __restrict__ u, float* __restrict__ v,
+                                                      float* __restrict__ w) {
+  bondv1_ghost_body(hw_blk(), g, u, v, w);
+}
+
+// Grids of the four bondv1 launches (one-dimensional but the first).
+static dim3 bondv1_grid(const Grid& g, int ilast, int part) {
+  const int nplane = g.jm * g.km;
+  switch (part) {
+    case 0: return grid2d(ilast + 1, g.jm, g.km);
+    case 1: return dim3((unsigned)std::min((nplane + 255) / 256, 256));
+    case 2: return dim3((unsigned)((nplane + 255) / 256));
+    default: return dim3((unsigned)(((g.im + 2) * (g.km + g.jm + 3) + 255) / 256));
+  }
+}
 
 void launch_bondv1(const DevDomain& d, int n, float dt) {
   const Grid& g = d.g;
   const int ilast = (n == 1) ? g.im : 1;
-  TIMED(d, KC_BONDV1, hipLaunchKernelGGL(k_bondv1_inflow, grid2d(ilast + 1, g.jm, g.km), block2d(), 0, d.stream, g, d.m,
+  TIMED(d, KC_BONDV1, hipLaunchKernelGGL(k_bondv1_inflow, bondv1_grid(g, ilast, 0), block2d(), 0, d.stream, g, d.m,
                      d.F.u, d.F.v, d.F.w, ilast, d.ctrl));
-  const int nplane = g.jm * g.km;
-  int nb = (nplane + 255) / 256;
-  if (nb > 256) nb = 256;
-  TIMED(d, KC_BONDV1, hipLaunchKernelGGL(k_bondv1_uout, dim3(nb), dim3(256), 0, d.stream, g, d.F.u, d.ctrl));
-  TIMED(d, KC_BONDV1, hipLaunchKernelGGL(k_bondv1_outflow, dim3((nplane + 255) / 256), dim3(256), 0, d.stream, g, d.m,
+  TIMED(d, KC_BONDV1, hipLaunchKernelGGL(k_bondv1_uout, bondv1_grid(g, ilast, 1), dim3(256), 0, d.stream, g, d.F.u,
+                                         d.ctrl));
+  TIMED(d, KC_BONDV1, hipLaunchKernelGGL(k_bondv1_outflow, bondv1_grid(g, ilast, 2), dim3(256), 0, d.stream, g, d.m,
                      d.F.u, d.F.v, d.F.w, (const Ctrl*)d.ctrl, dt));
-  const int ng = (g.im + 2) * (g.km + g.jm + 3);
-  TIMED(d, KC_BONDV1, hipLaunchKernelGGL(k_bondv1_ghost, dim3((ng + 255) / 256), dim3(256), 0, d.stream, g, d.F.u,
+  TIMED(d, KC_BONDV1, hipLaunchKernelGGL(k_bondv1_ghost, bondv1_grid(g, ilast, 3), dim3(256), 0, d.stream, g, d.F.u,
                      d.F.v, d.F.w));
 }
 
@@ -367,7 +401,7 @@ __global__ __launch_bounds__(NT) void k_velfg(Grid g, Metrics m, P10 diu, P10 co
   }
 }
 
-static P10 p10(float* const* a) {
+__host__ __device__ inline P10 p10(float* const* a) {
   P10 r;
   for (int q = 0; q < 10; q++) r.a[q] = a[q];
   return r;
@@ -492,19 +526,17 @@ struct VelCtx {
   }
 };
 
-__global__ __launch_bounds__(NT) void k_velfg_feedbf(VelCtx X, P10 diu, float* __restrict__ f,
-                                                     float* __restrict__ gg, float* __restrict__ h,
-                                                     float* __restrict__ usum, float* __restrict__ vsum,
-                                                     float* __restrict__ wsum, const float* __restrict__ bm,
-                                                     const float* __restrict__ cm, const float* __restrict__ dm,
-                                                     float* __restrict__ uspd, float* __restrict__ vspd, float vn,
-                                                     float alpha, float dt, float beta, int kchunk) {
+__device__ __forceinline__ void velfg_feedbf_body(
+    const VBlk& vb, const VelCtx& X, const P10& diu, float* __restrict__ f, float* __restrict__ gg,
+    float* __restrict__ h, float* __restrict__ usum, float* __restrict__ vsum, float* __restrict__ wsum,
+    const float* __restrict__ bm, const float* __restrict__ cm, const float* __restrict__ dm,
+    float* __restrict__ uspd, float* __restrict__ vspd, float vn, float alpha, float dt, float beta, int kchunk) {
   const Grid& g = X.g;
   const Metrics& m = X.m;
-  const int i = 1 + blockIdx.x * BX + threadIdx.x, j = 1 + blockIdx.y * BY + threadIdx.y;
+  const int i = 1 + vb.x * BX + threadIdx.x, j = 1 + vb.y * BY + threadIdx.y;
   if (i > g.im || j > g.jm) return;
   const int im = g.im, jm = g.jm, km = g.km;
-  const int k0 = 1 + blockIdx.z * kchunk, k1 = min(km, k0 + kchunk - 1);
+  const int k0 = 1 + vb.z * kchunk, k1 = min(km, k0 + kchunk - 1);
   if (k0 > km) return;
   const int jn = (j == jm) ? 1 : j + 1;  // the row velfg reads at j+1 (periodic ghost at jm)
   const bool ilast = (i == im);
@@ -657,8 +689,19 @@ __global__ __launch_bounds__(NT) void k_velfg_feedbf(VelCtx X, P10 diu, float* _
     }
   }
 }
+__global__ __launch_bounds__(NT) void k_velfg_feedbf(VelCtx X, P10 diu, float* __restrict__ f,
+                                                     float* __restrict__ gg, float* __restrict__ h,
+                                                     float* __restrict__ usum, float* __restrict__ vsum,
+                                                     float* __restrict__ wsum, const float* __restrict__ bm,
+                                                     const float* __restrict__ cm, const float* __restrict__ dm,
+                                                     float* __restrict__ uspd, float* __restrict__ vspd, float vn,
+                                                     float alpha, float dt, float beta, int kchunk) {
+  velfg_feedbf_body(hw_blk(), X, diu, f, gg, h, usum, vsum, wsum, bm, cm, dm, uspd, vspd, vn, alpha, dt, beta,
+                    kchunk);
+}
 
-void launch_velfg_feedbf(const DevDomain& d, float vn, float alpha, float dt, float beta) {
+// Grid of k_velfg_feedbf and its k levels per thread.
+static dim3 velfg_grid(const DevDomain& d, int* kchunk) {
   const Grid& g = d.g;
   const int gx = (g.im + BX - 1) / BX, gy = (g.jm + BY - 1) / BY;
   // about 36 blocks per compute unit: short k runs measured fastest (at
@@ -669,8 +712,15 @@ void launch_velfg_feedbf(const DevDomain& d, float vn, float alpha, float dt, fl
   int kc = (g.km + nz - 1) / nz;
   if (d.velfg_kchunk > 0) kc = d.velfg_kchunk < g.km ? d.velfg_kchunk : g.km;
   nz = (g.km + kc - 1) / kc;
-  VelCtx X{g, d.m, d.F.u, d.F.v, d.F.w, d.wl};
-  TIMED(d, KC_VEL2, hipLaunchKernelGGL(k_velfg_feedbf, dim3(gx, gy, nz), block2d(), 0, d.stream, X, p10(d.F.diu),
+  *kchunk = kc;
+  return dim3((unsigned)gx, (unsigned)gy, (unsigned)nz);
+}
+
+void launch_velfg_feedbf(const DevDomain& d, float vn, float alpha, float dt, float beta) {
+  int kc = 0;
+  const dim3 grid = velfg_grid(d, &kc);
+  VelCtx X{d.g, d.m, d.F.u, d.F.v, d.F.w, d.wl};
+  TIMED(d, KC_VEL2, hipLaunchKernelGGL(k_velfg_feedbf, grid, block2d(), 0, d.stream, X, p10(d.F.diu),
                                         d.F.f, d.F.g, d.F.h, d.F.usum, d.F.vsum, d.F.wsum, d.F.bmask, d.F.cmask,
                                         d.F.dmask, d.F.uspd, d.F.vspd, vn, alpha, dt, beta, kc));
 }
@@ -716,9 +766,9 @@ void launch_feedbf(const DevDomain& d, float alpha, float dt, float beta) {
 
 // -------------------------------------------------------------------- les
 // Eddy viscosity sm (les.f95:45-65); delx1 arrives precomputed per level.
-__global__ __launch_bounds__(NT) void k_les_sm(Grid g, P10 diu, const float* __restrict__ delx1,
-                                               float* __restrict__ sm) {
-  const int i = 1 + blockIdx.x * BX + threadIdx.x, j = 1 + blockIdx.y * BY + threadIdx.y;
+__device__ __forceinline__ void les_sm_body(const VBlk& vb, const Grid& g, const P10& diu,
+                                            const float* __restrict__ delx1, float* __restrict__ sm) {
+  const int i = 1 + vb.x * BX + threadIdx.x, j = 1 + vb.y * BY + threadIdx.y;
   if (i > g.im || j > g.jm) return;
   const int sj = g.sj, sk = g.sk;
   const float* d2 = diu.a[2];
@@ -728,7 +778,7 @@ __global__ __launch_bounds__(NT) void k_les_sm(Grid g, P10 diu, const float* __r
   const float* d7 = diu.a[7];
   const float* d8 = diu.a[8];
   int k0, k1;
-  kspan(1, g.km, k0, k1);
+  kspan(vb, 1, g.km, k0, k1);
   for (int k = k0; k <= k1; k++) {
     const int c = ix(g, i, j, k);
     const float dudxx1 = diu.a[1][c];
@@ -747,6 +797,10 @@ __global__ __launch_bounds__(NT) void k_les_sm(Grid g, P10 diu, const float* __r
     sm[c] = a * a * sqrtf(s);
   }
 }
+__global__ __launch_bounds__(NT) void k_les_sm(Grid g, P10 diu, const float* __restrict__ delx1,
+                                               float* __restrict__ sm) {
+  les_sm_body(hw_blk(), g, diu, delx1, sm);
+}
 
 // boundsm.f95:18-64 as one pass: the serial build runs the i-Neumann
 // copies (j=-1..jm+1, k=0..km+1), then the j-Neumann copies (i=0..im+1,
@@ -759,13 +813,13 @@ __device__ __forceinline__ bool sm_written(const Grid& g, int i, int j, int k) {
   return (ib && jin && kin) || (jb && iin && kin) || (kb && jin && iin);
 }
 
-__global__ __launch_bounds__(256) void k_boundsm(Grid g, float* __restrict__ sm) {
+__device__ __forceinline__ void boundsm_body(const VBlk& vb, const Grid& g, float* __restrict__ sm) {
   // enumerate the box i=0..im+1, j=-1..jm+1, k=0..km+1 face by face:
   // k faces (2 planes), then j faces (rows j=-1,0,jm+1 for k=1..km), then
   // i faces (i=0,im+1 for j=1..jm, k=1..km)
   const int ni = g.im + 2, nj = g.jm + 3;
   const int nK = 2 * ni * nj, nJ = 3 * ni * g.km, nI = 2 * g.jm * g.km;
-  int t = blockIdx.x * 256 + threadIdx.x;
+  int t = vb.x * 256 + threadIdx.x;
   int i, j, k;
   if (t < nK) {
     i = t % ni;
@@ -801,20 +855,19 @@ __global__ __launch_bounds__(256) void k_boundsm(Grid g, float* __restrict__ sm)
   const float s = sm[ix(g, ii, jj, kk)];
   sm[ix(g, i, j, k)] = (sgn < 0.0f) ? -s : s;
 }
+__global__ __launch_bounds__(256) void k_boundsm(Grid g, float* __restrict__ sm) { boundsm_body(hw_blk(), g, sm); }
 
 // Viscous terms added to f (k=2..km, wall k=1), g (same) and h (k=1..km),
 // les.f95:83-242.
 // ADAM: the whole-step path also applies adam (adam.f95:30-44) to the new
 // f,g,h, which is what k_adam after this kernel computes.
 template <bool ADAM>
-__global__ __launch_bounds__(NT) void k_les_visc(Grid g, Metrics m, P10 diu,
-                                                 const float* __restrict__ sm,
-                                                 const float* __restrict__ uspd,
-                                                 const float* __restrict__ vspd,
-                                                 float* __restrict__ f, float* __restrict__ gg,
-                                                 float* __restrict__ h, float* __restrict__ fo,
-                                                 float* __restrict__ go, float* __restrict__ ho, float wl) {
-  const int i = 1 + blockIdx.x * BX + threadIdx.x, j = 1 + blockIdx.y * BY + threadIdx.y;
+__device__ __forceinline__ void les_visc_body(const VBlk& vb, const Grid& g, const Metrics& m, const P10& diu,
+                                              const float* __restrict__ sm, const float* __restrict__ uspd,
+                                              const float* __restrict__ vspd, float* __restrict__ f,
+                                              float* __restrict__ gg, float* __restrict__ h, float* __restrict__ fo,
+                                              float* __restrict__ go, float* __restrict__ ho, float wl) {
+  const int i = 1 + vb.x * BX + threadIdx.x, j = 1 + vb.y * BY + threadIdx.y;
   if (i > g.im || j > g.jm) return;
   const int sj = g.sj, sk = g.sk;
   const float dxi = M1(m.dx1, i), dxp = M1(m.dx1, i + 1), dxm = M1(m.dx1, i - 1);
@@ -834,7 +887,7 @@ __global__ __launch_bounds__(NT) void k_les_visc(Grid g, Metrics m, P10 diu,
 #define SMY(c) ((dyp * sm[(c)] + dyj * sm[(c) + sj]) / (dyj + dyp))     // y-interp at j+1/2
 #define SMYM(c) ((dyj * sm[(c) - sj] + dym * sm[(c)]) / (dym + dyj))    // y-interp at j-1/2
   int k0, k1;
-  kspan(1, g.km, k0, k1);
+  kspan(vb, 1, g.km, k0, k1);
   for (int k = k0; k <= k1; k++) {
     const int c = ix(g, i, j, k);
     const float dzk = M1(m.dzn, k), dzp = M1(m.dzn, k + 1), dzm = M1(m.dzn, k - 1);
@@ -922,15 +975,27 @@ __global__ __launch_bounds__(NT) void k_les_visc(Grid g, Metrics m, P10 diu,
 #undef SMY
 #undef SMYM
 }
+template <bool ADAM>
+__global__ __launch_bounds__(NT) void k_les_visc(Grid g, Metrics m, P10 diu, const float* __restrict__ sm,
+                                                 const float* __restrict__ uspd, const float* __restrict__ vspd,
+                                                 float* __restrict__ f, float* __restrict__ gg,
+                                                 float* __restrict__ h, float* __restrict__ fo,
+                                                 float* __restrict__ go, float* __restrict__ ho, float wl) {
+  les_visc_body<ADAM>(hw_blk(), g, m, diu, sm, uspd, vspd, f, gg, h, fo, go, ho, wl);
+}
+
+static int boundsm_blocks(const Grid& g) {
+  const int ni = g.im + 2, nj = g.jm + 3;
+  return (2 * ni * nj + 3 * ni * g.km + 2 * g.jm * g.km + 255) / 256;
+}
 
 void launch_les(const DevDomain& d) {
   const Grid& g = d.g;
   const P10 diu = p10(d.F.diu);
   TIMED(d, KC_LES_SM, hipLaunchKernelGGL(k_les_sm, grid2d(g.im, g.jm, kz(g)), block2d(), 0, d.stream, g, diu,
                      d.m.delx1 + 1, d.F.sm));
-  const int ni = g.im + 2, nj = g.jm + 3;
-  const int nb = 2 * ni * nj + 3 * ni * g.km + 2 * g.jm * g.km;
-  TIMED(d, KC_BOUNDSM, hipLaunchKernelGGL(k_boundsm, dim3((nb + 255) / 256), dim3(256), 0, d.stream, g, d.F.sm));
+  TIMED(d, KC_BOUNDSM, hipLaunchKernelGGL(k_boundsm, dim3((unsigned)boundsm_blocks(g)), dim3(256), 0, d.stream, g,
+                                          d.F.sm));
   TIMED(d, KC_LES_VISC, hipLaunchKernelGGL(k_les_visc<false>, grid2d(g.im, g.jm, kz(g)), block2d(), 0, d.stream, g,
                      d.m, diu, d.F.sm, d.F.uspd, d.F.vspd, d.F.f, d.F.g, d.F.h, nullptr, nullptr, nullptr, d.wl));
 }
@@ -941,9 +1006,8 @@ void launch_les_adam(const DevDomain& d) {
   const P10 diu = p10(d.F.diu);
   TIMED(d, KC_LES_SM, hipLaunchKernelGGL(k_les_sm, grid2d(g.im, g.jm, kz(g)), block2d(), 0, d.stream, g, diu,
                      d.m.delx1 + 1, d.F.sm));
-  const int ni = g.im + 2, nj = g.jm + 3;
-  const int nb = 2 * ni * nj + 3 * ni * g.km + 2 * g.jm * g.km;
-  TIMED(d, KC_BOUNDSM, hipLaunchKernelGGL(k_boundsm, dim3((nb + 255) / 256), dim3(256), 0, d.stream, g, d.F.sm));
+  TIMED(d, KC_BOUNDSM, hipLaunchKernelGGL(k_boundsm, dim3((unsigned)boundsm_blocks(g)), dim3(256), 0, d.stream, g,
+                                          d.F.sm));
   TIMED(d, KC_LES_VISC, hipLaunchKernelGGL(k_les_visc<true>, grid2d(g.im, g.jm, kz(g)), block2d(), 0, d.stream, g,
                      d.m, diu, d.F.sm, d.F.uspd, d.F.vspd, d.F.f, d.F.g, d.F.h, d.F.fold, d.F.gold, d.F.hold,
                      d.wl));
@@ -1080,6 +1144,109 @@ void launch_pack(const DevDomain& d, const float* src, float* dst, int li, int l
   if (nb > 65536) nb = 65536;
   TIMED(d, KC_COPY, hipLaunchKernelGGL(k_pack, dim3((unsigned)nb), dim3(256), 0, d.stream, d.g, src, dst, li, lj, lk,
                      ni, nj, nk));
+}
+
+// ------------------------------------------------ batched step stages
+// One launch per stage for all domains of a batched step (steps n > 1,
+// velfg+feedbf and les+adam fused).  Each block finds its domain and its
+// position in that domain's own grid, then runs the stage kernel's body with
+// that domain's arguments: the work of every block, and so every value, is
+// the same as in the domain's own launch.
+#define BATCH_BLOCK(stage)                                          \
+  int e;                                                            \
+  VBlk vb;                                                          \
+  if (!batch_block(sp + (stage) * nd, nd, e, vb)) return;           \
+  const BatchDom& B = bd[e];                                        \
+  const DevDomain& d = B.d
+
+__global__ __launch_bounds__(NT) void kb_velnw(const BatchDom* __restrict__ bd, const StageSpan* __restrict__ sp,
+                                               int nd) {
+  BATCH_BLOCK(BS_VELNW);
+  velnw_body(vb, d.g, d.m, d.F.p, d.F.f, d.F.g, d.F.h, d.F.u, d.F.v, d.F.w, B.ro, B.dt);
+}
+__global__ __launch_bounds__(NT) void kb_bondv1_inflow(const BatchDom* __restrict__ bd,
+                                                       const StageSpan* __restrict__ sp, int nd) {
+  BATCH_BLOCK(BS_INFLOW);
+  bondv1_inflow_body(vb, d.g, d.m, d.F.u, d.F.v, d.F.w, B.ilast, d.ctrl);
+}
+__global__ __launch_bounds__(256) void kb_bondv1_uout(const BatchDom* __restrict__ bd,
+                                                      const StageSpan* __restrict__ sp, int nd) {
+  BATCH_BLOCK(BS_UOUT);
+  bondv1_uout_body(vb, d.g, d.F.u, d.ctrl);
+}
+__global__ __launch_bounds__(256) void kb_bondv1_outflow(const BatchDom* __restrict__ bd,
+                                                         const StageSpan* __restrict__ sp, int nd) {
+  BATCH_BLOCK(BS_OUTFLOW);
+  bondv1_outflow_body(vb, d.g, d.m, d.F.u, d.F.v, d.F.w, d.ctrl, B.dt);
+}
+__global__ __launch_bounds__(256) void kb_bondv1_ghost(const BatchDom* __restrict__ bd,
+                                                       const StageSpan* __restrict__ sp, int nd) {
+  BATCH_BLOCK(BS_GHOST);
+  bondv1_ghost_body(vb, d.g, d.F.u, d.F.v, d.F.w);
+}
+__global__ __launch_bounds__(NT) void kb_velfg_feedbf(const BatchDom* __restrict__ bd,
+                                                      const StageSpan* __restrict__ sp, int nd) {
+  BATCH_BLOCK(BS_VELFG);
+  const VelCtx X{d.g, d.m, d.F.u, d.F.v, d.F.w, d.wl};
+  velfg_feedbf_body(vb, X, p10(d.F.diu), d.F.f, d.F.g, d.F.h, d.F.usum, d.F.vsum, d.F.wsum, d.F.bmask, d.F.cmask,
+                    d.F.dmask, d.F.uspd, d.F.vspd, B.vn, B.alpha, B.dt, B.beta, B.velfg_kc);
+}
+__global__ __launch_bounds__(NT) void kb_les_sm(const BatchDom* __restrict__ bd, const StageSpan* __restrict__ sp,
+                                                int nd) {
+  BATCH_BLOCK(BS_LES_SM);
+  les_sm_body(vb, d.g, p10(d.F.diu), d.m.delx1 + 1, d.F.sm);
+}
+__global__ __launch_bounds__(256) void kb_boundsm(const BatchDom* __restrict__ bd, const StageSpan* __restrict__ sp,
+                                                  int nd) {
+  BATCH_BLOCK(BS_BOUNDSM);
+  boundsm_body(vb, d.g, d.F.sm);
+}
+__global__ __launch_bounds__(NT) void kb_les_visc_adam(const BatchDom* __restrict__ bd,
+                                                       const StageSpan* __restrict__ sp, int nd) {
+  BATCH_BLOCK(BS_LES_VISC);
+  les_visc_body<true>(vb, d.g, d.m, p10(d.F.diu), d.F.sm, d.F.uspd, d.F.vspd, d.F.f, d.F.g, d.F.h, d.F.fold,
+                      d.F.gold, d.F.hold, d.wl);
+}
+#undef BATCH_BLOCK
+
+static StageSpan span_of(dim3 g) { return {0, (int)g.x, (int)g.y, (int)g.z}; }
+
+void step_batch_spans(const DevDomain& d, int nzb, BatchDom* e, StageSpan* sp) {
+  const Grid& g = d.g;
+  // the stencils keep one level per block (measured: longer runs made
+  // les_sm and les_visc slower in a batch as alone); velfg takes nzb
+  const int nz = nzb > 0 ? std::min(nzb, g.km) : kz(g);
+  e->ilast = 1;
+  sp[BS_VELNW] = span_of(grid2d(g.im, g.jm, kz(g)));
+  sp[BS_INFLOW] = span_of(bondv1_grid(g, 1, 0));
+  sp[BS_UOUT] = span_of(bondv1_grid(g, 1, 1));
+  sp[BS_OUTFLOW] = span_of(bondv1_grid(g, 1, 2));
+  sp[BS_GHOST] = span_of(bondv1_grid(g, 1, 3));
+  if (nzb > 0) {
+    e->velfg_kc = (g.km + nz - 1) / nz;
+    sp[BS_VELFG] = span_of(grid2d(g.im, g.jm, (g.km + e->velfg_kc - 1) / e->velfg_kc));
+  } else {
+    sp[BS_VELFG] = span_of(velfg_grid(d, &e->velfg_kc));
+  }
+  sp[BS_LES_SM] = span_of(grid2d(g.im, g.jm, kz(g)));
+  sp[BS_BOUNDSM] = span_of(dim3((unsigned)boundsm_blocks(g)));
+  sp[BS_LES_VISC] = span_of(grid2d(g.im, g.jm, kz(g)));
+}
+
+void launch_step_batch(const StageBatch& b, hipStream_t stream) {
+  const BatchDom* bd = b.doms;
+  const StageSpan* sp = b.spans;
+  const int nd = b.nd;
+  auto nb = [&](int s) { return dim3((unsigned)b.nblocks[s]); };
+  hipLaunchKernelGGL(kb_velnw, nb(BS_VELNW), block2d(), 0, stream, bd, sp, nd);
+  hipLaunchKernelGGL(kb_bondv1_inflow, nb(BS_INFLOW), block2d(), 0, stream, bd, sp, nd);
+  hipLaunchKernelGGL(kb_bondv1_uout, nb(BS_UOUT), dim3(256), 0, stream, bd, sp, nd);
+  hipLaunchKernelGGL(kb_bondv1_outflow, nb(BS_OUTFLOW), dim3(256), 0, stream, bd, sp, nd);
+  hipLaunchKernelGGL(kb_bondv1_ghost, nb(BS_GHOST), dim3(256), 0, stream, bd, sp, nd);
+  hipLaunchKernelGGL(kb_velfg_feedbf, nb(BS_VELFG), block2d(), 0, stream, bd, sp, nd);
+  hipLaunchKernelGGL(kb_les_sm, nb(BS_LES_SM), block2d(), 0, stream, bd, sp, nd);
+  hipLaunchKernelGGL(kb_boundsm, nb(BS_BOUNDSM), dim3(256), 0, stream, bd, sp, nd);
+  hipLaunchKernelGGL(kb_les_visc_adam, nb(BS_LES_VISC), block2d(), 0, stream, bd, sp, nd);
 }
 
 }  // namespace les

@@ -91,6 +91,24 @@ __device__ __forceinline__ void kspan(int lo, int hi, int& k0, int& k1) {
   k1 = (k0 + c - 1 < hi) ? k0 + c - 1 : hi;
 }
 
+// A block's position (x, y, z) in its domain's grid (nx, ny, nz).  The stage
+// kernels of a whole step take it in place of blockIdx/gridDim, so that one
+// launch can run the grids of several domains (each domain keeps the grid it
+// has in a launch of its own, hence the same per-block work and partials).
+struct VBlk {
+  int x, y, z, nx, ny, nz;
+};
+__device__ __forceinline__ VBlk hw_blk() {
+  return {(int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z, (int)gridDim.x, (int)gridDim.y, (int)gridDim.z};
+}
+__device__ __forceinline__ int vlinear(const VBlk& b) { return b.x + b.nx * (b.y + b.ny * b.z); }
+__device__ __forceinline__ void kspan(const VBlk& b, int lo, int hi, int& k0, int& k1) {
+  const int n = hi - lo + 1;
+  const int c = (n + b.nz - 1) / b.nz;
+  k0 = lo + b.z * c;
+  k1 = (k0 + c - 1 < hi) ? k0 + c - 1 : hi;
+}
+
 // Deterministic sum over a block of NT threads (tid = linear thread id):
 // wavefront butterfly, then wave totals through LDS added by thread 0 in
 // wave order.  The result is valid in thread 0.

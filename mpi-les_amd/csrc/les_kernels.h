@@ -96,18 +96,84 @@ struct SorBatchEntry {
   int block0;    // first block of this domain in the launch (a multiple of 8)
   float omega, pjuge;
 };
-void launch_sor_batch(const SorBatchEntry* tab, int ntab, int nblocks, int iter, int kcmax, int remap,
-                      hipStream_t stream);
+// block0: the entries' block0 values, packed (the block-to-domain lookup);
+// ty: tile rows of the batch (16 or 32).
+void launch_sor_batch(const SorBatchEntry* tab, const int* block0, int ntab, int nblocks, int iter, int kcmax,
+                      int ty, int remap, hipStream_t stream);
 // The press stage in two parts around iterations 2..nmaxp of the SOR, for
 // domains whose iterations run batched: part 0 is the right-hand side and
 // iteration 1, part 1 the stop test of the last iteration and the mean
-// removal.
-void launch_press_part(const DevDomain& d, float dt, int part);
-// The domain's entry in a batched SOR launch (block0 left to the caller).
-void sor_batch_entry(const DevDomain& d, SorBatchEntry* e);
+// removal.  kc, ty: the SOR k chunk (0: the domain's own) and tile rows of
+// the batch.
+void launch_press_part(const DevDomain& d, float dt, int part, int kc, int ty);
+// The domain's entry in a batched SOR launch with k chunk kc and tile rows ty
+// (block0 left to the caller).
+void sor_batch_entry(const DevDomain& d, int kc, int ty, SorBatchEntry* e);
+// The k chunk of a batch of domains' SOR launches with tile rows ty.
+int sor_batch_kchunk(const DevDomain* const* ds, int n, int ty, int ncu);
 // Whether a domain's SOR iterations can run in a batch (fused reference SOR,
 // separable cn1, even jm).
 bool sor_batchable(const DevDomain& d);
+
+// The launches of a whole step (velfg+feedbf and les+adam fused, steps
+// n > 1) that a batched step makes once for all its domains, in step order.
+// One-block launches (the reduction finalizers, the SOR's closing sum) run
+// one block per domain and need no span.
+enum BatchStage {
+  BS_VELNW, BS_INFLOW, BS_UOUT, BS_OUTFLOW, BS_GHOST, BS_VELFG, BS_LES_SM, BS_BOUNDSM, BS_LES_VISC,
+  BS_RHS, BS_SPLIT, BS_SOR1, BS_PAV, BS_PADJ, BS_N
+};
+// A domain's grid in one stage and its first block in the batched launch (a
+// multiple of 8, so every block runs on the XCD it has in a launch of its own).
+struct StageSpan {
+  int b0, gx, gy, gz;
+};
+// A domain of a batched step: its description and the scalars and shapes its
+// own launches would use.
+struct BatchDom {
+  DevDomain d;
+  SplitBufs sb;    // the split arrays with the cn1 pointers cleared (separable cn1)
+  float ro, dt, vn, alpha, beta;
+  int nred;        // partial-sum slots of the rhs and pav reductions
+  int velfg_kc;    // k levels per thread of k_velfg_feedbf
+  int ilast;       // last column bondv1 sets from the inflow (1 for n > 1)
+  int ntt1, ntj1, kc1, nparts1;  // shape of SOR iteration 1 (k_sor_iter<true, true>)
+  int nparts_last;               // partial-sum slots of the launch of iteration nmaxp
+};
+struct StageBatch {
+  const BatchDom* doms;    // device, nd entries
+  const StageSpan* spans;  // device, [BS_N][nd]
+  int nd;
+  int nblocks[BS_N];
+  int kc1max;              // largest iteration-1 k chunk (dynamic LDS of that launch)
+  int remap;
+};
+// This block's domain e and position in that domain's grid, in a batched
+// launch over the spans sp[0..nd) (b0 ascending); false for the padding
+// blocks after a domain's grid.  Block-uniform.
+__device__ __forceinline__ bool batch_block(const StageSpan* __restrict__ sp, int nd, int& e, VBlk& vb) {
+  const int b = (int)blockIdx.x;
+  int q = 0;
+  for (int r = 1; r < nd; r++)
+    if (b >= sp[r].b0) q = r;
+  const StageSpan s = sp[q];
+  const int l = b - s.b0;
+  if (l >= s.gx * s.gy * s.gz) return false;
+  vb = {l % s.gx, (l / s.gx) % s.gy, l / (s.gx * s.gy), s.gx, s.gy, s.gz};
+  e = q;
+  return true;
+}
+// A domain's shapes and grids (b0 left to the caller): the step stages, then
+// the press stages.  nzb > 0: at most nzb blocks along k for velfg and the
+// copies into and out of the split arrays (longer k runs per block; no
+// value depends on the split); kc: the SOR k chunk of the batch.
+void step_batch_spans(const DevDomain& d, int nzb, BatchDom* e, StageSpan* sp);
+void press_batch_spans(const DevDomain& d, int kc, int ty, int nzb, BatchDom* e, StageSpan* sp);
+// velnw .. les+adam of every domain, one launch per stage.
+void launch_step_batch(const StageBatch& b, hipStream_t stream);
+// part 0: rhs, its mean, the split arrays and SOR iteration 1; part 1 (after
+// iterations 2..nmaxp): the closing stop-test sum, pav and the adjusted p.
+void launch_press_batch(const StageBatch& b, int part, hipStream_t stream);
 void launch_sor_trial(const DevDomain& d, int rows, int ty, int kc, int iters);
 int reduction_parts_needed(const Grid& g);
 

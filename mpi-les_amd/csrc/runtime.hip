@@ -8,6 +8,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <array>
 #include <map>
 #include <mutex>
 #include <string>
@@ -649,8 +650,9 @@ int run_step(les_domain* D, int n, int nmax) {
 
 // Stages velnw..adam of a step (as run_step), then press: whole when the
 // domain's SOR cannot be batched, else its first part (the right-hand side
-// and SOR iteration 1).  Returns whether the press was split.
-int run_step_head(les_domain* D, int n, int nmax, bool* split) {
+// and SOR iteration 1, with the batch's SOR k chunk kc and tile rows ty).
+// Returns whether the press was split.
+int run_step_head(les_domain* D, int n, int nmax, int kc, int ty, bool* split) {
   for (int s = 0; s < 6; s++) {
     if (s == 2 && D->d.fuse_velfg) {
       if (flush_metrics(D)) return 1;
@@ -670,7 +672,7 @@ int run_step_head(les_domain* D, int n, int nmax, bool* split) {
   }
   *split = sor_batchable(D->d);
   if (!*split) return run_stage(D, 6, n, nmax);
-  launch_press_part(D->d, D->dt, 0);
+  launch_press_part(D->d, D->dt, 0, kc, ty);
   HIP_TRY(hipGetLastError());
   return 0;
 }
@@ -1450,8 +1452,15 @@ struct les_batch {
   hipStream_t stream = nullptr;
   hipEvent_t fork = nullptr, sor_done = nullptr;
   std::vector<hipEvent_t> head_done, tail_done;
-  SorBatchEntry* dtab = nullptr;  // device copy of the SOR table
+  SorBatchEntry* dtab = nullptr;  // device copy of the SOR table, then its block0 values
+  const int* dblock0 = nullptr;
   int ntab = 0, nblocks = 0, kcmax = 0, lmax = 0, remap = 1;
+  int sor_kc = 0, sor_ty = 16;    // SOR k chunk and tile rows of the batched domains
+  // Domains whose whole step runs as batched stage launches (fused velfg and
+  // les, batchable SOR); the others step on their own streams.
+  std::vector<char> full;
+  void* stage_mem = nullptr;      // device: BatchDom[nfull], then StageSpan[BS_N][nfull]
+  StageBatch stages{};            // stages.nd == 0: no batched stages
   std::vector<DevDomain> key;     // domain states the table and graph were made for
   std::vector<float> skey;        // and their scalars
   hipGraphExec_t gexec = nullptr;
@@ -1466,6 +1475,7 @@ void batch_free(les_batch* B) {
     if (D->d.stream) hipStreamSynchronize(D->d.stream);
   if (B->gexec) hipGraphExecDestroy(B->gexec);
   if (B->dtab) hipFree(B->dtab);
+  if (B->stage_mem) hipFree(B->stage_mem);
   for (hipEvent_t e : B->head_done) hipEventDestroy(e);
   for (hipEvent_t e : B->tail_done) hipEventDestroy(e);
   if (B->fork) hipEventDestroy(B->fork);
@@ -1487,33 +1497,102 @@ bool batch_key_same(const les_batch* B) {
   return B->skey == batch_scalars(B);
 }
 
-// The SOR table for the domains' current states (synchronous upload).
+// The SOR table and the stage tables for the domains' current states
+// (synchronous uploads).
 int batch_prepare(les_batch* B) {
   std::vector<SorBatchEntry> tab;
+  std::vector<BatchDom> bdom;
+  std::vector<std::array<StageSpan, BS_N>> spans;
   int block = 0;
   B->kcmax = 0;
   B->lmax = 0;
-  for (les_domain* D : B->doms) {
+  B->full.assign(B->doms.size(), 0);
+  std::vector<const DevDomain*> fd;
+  long tiles = 0;  // (i,j) tiles of the stage kernels over the batched domains
+  int ncu = 256;
+  for (size_t q = 0; q < B->doms.size(); q++) {
+    les_domain* D = B->doms[q];
     if (flush_metrics(D)) return 1;
-    if (!sor_batchable(D->d)) continue;
+    if (!sor_batchable(D->d) || !D->d.fuse_velfg || !D->d.fuse_les) continue;
+    B->full[q] = 1;
+    fd.push_back(&D->d);
+    tiles += (long)((D->d.g.im + 63) / 64) * ((D->d.g.jm + 3) / 4);
+    if (D->d.ncu > 0) ncu = D->d.ncu;
+  }
+  // one SOR k chunk for the batch (LES_BATCH_SOR_KCHUNK overrides), and k
+  // splits of the copy-like stages giving about 36 blocks per CU in all
+  B->sor_ty = env_int("LES_BATCH_SOR_TILE_ROWS", 16) == 32 ? 32 : 16;
+  B->sor_kc = fd.empty() ? 0 : env_int("LES_BATCH_SOR_KCHUNK", 0);
+  if (!fd.empty() && B->sor_kc <= 0) B->sor_kc = sor_batch_kchunk(fd.data(), (int)fd.size(), B->sor_ty, ncu);
+  const int nzb = tiles > 0 ? (int)std::max(1L, 36L * ncu / tiles) : 0;
+  for (size_t q = 0; q < B->doms.size(); q++) {
+    if (!B->full[q]) continue;
+    les_domain* D = B->doms[q];
     SorBatchEntry e;
-    sor_batch_entry(D->d, &e);
+    sor_batch_entry(D->d, B->sor_kc, B->sor_ty, &e);
     e.block0 = block;
     block += (e.nparts + 7) / 8 * 8;
     B->kcmax = std::max(B->kcmax, e.kchunk);
     B->lmax = std::max(B->lmax, e.nmaxp);
     tab.push_back(e);
+    BatchDom bd{};
+    std::array<StageSpan, BS_N> sp{};
+    bd.d = D->d;
+    bd.ro = D->ro;
+    bd.dt = D->dt;
+    bd.vn = D->vn;
+    bd.alpha = D->alpha;
+    bd.beta = D->beta;
+    step_batch_spans(D->d, nzb, &bd, sp.data());
+    press_batch_spans(D->d, B->sor_kc, B->sor_ty, nzb, &bd, sp.data());
+    bdom.push_back(bd);
+    spans.push_back(sp);
   }
   B->ntab = (int)tab.size();
   B->nblocks = block;
-  if (B->dtab) {
+  if (B->dtab || B->stage_mem) {
     HIP_TRY(hipStreamSynchronize(B->stream));
-    HIP_TRY(hipFree(B->dtab));
+    if (B->dtab) HIP_TRY(hipFree(B->dtab));
+    if (B->stage_mem) HIP_TRY(hipFree(B->stage_mem));
     B->dtab = nullptr;
+    B->dblock0 = nullptr;
+    B->stage_mem = nullptr;
   }
+  B->stages = StageBatch{};
   if (!tab.empty()) {
-    HIP_TRY(hipMalloc(&B->dtab, tab.size() * sizeof(SorBatchEntry)));
-    HIP_TRY(hipMemcpy(B->dtab, tab.data(), tab.size() * sizeof(SorBatchEntry), hipMemcpyHostToDevice));
+    // the table, then its block0 values packed for the block-to-domain lookup
+    const size_t ntb = tab.size() * sizeof(SorBatchEntry);
+    std::vector<int> b0s;
+    for (const SorBatchEntry& x : tab) b0s.push_back(x.block0);
+    HIP_TRY(hipMalloc(&B->dtab, ntb + b0s.size() * sizeof(int)));
+    HIP_TRY(hipMemcpy(B->dtab, tab.data(), ntb, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy((char*)B->dtab + ntb, b0s.data(), b0s.size() * sizeof(int), hipMemcpyHostToDevice));
+    B->dblock0 = (const int*)((const char*)B->dtab + ntb);
+    // stage spans by stage, then domain; each domain's range of blocks starts
+    // at a multiple of 8
+    const int nd = (int)bdom.size();
+    std::vector<StageSpan> flat((size_t)BS_N * nd);
+    StageBatch& S = B->stages;
+    for (int s = 0; s < BS_N; s++) {
+      int b0 = 0;
+      for (int q = 0; q < nd; q++) {
+        StageSpan x = spans[q][s];
+        x.b0 = b0;
+        b0 += (x.gx * x.gy * x.gz + 7) / 8 * 8;
+        flat[(size_t)s * nd + q] = x;
+      }
+      S.nblocks[s] = b0;
+    }
+    for (const BatchDom& x : bdom) S.kc1max = std::max(S.kc1max, x.kc1);
+    const size_t nbd = bdom.size() * sizeof(BatchDom);
+    HIP_TRY(hipMalloc(&B->stage_mem, nbd + flat.size() * sizeof(StageSpan)));
+    HIP_TRY(hipMemcpy(B->stage_mem, bdom.data(), nbd, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy((char*)B->stage_mem + nbd, flat.data(), flat.size() * sizeof(StageSpan),
+                      hipMemcpyHostToDevice));
+    S.doms = (const BatchDom*)B->stage_mem;
+    S.spans = (const StageSpan*)((const char*)B->stage_mem + nbd);
+    S.nd = nd;
+    S.remap = B->remap;
   }
   B->key.clear();
   for (les_domain* D : B->doms) B->key.push_back(D->d);
@@ -1521,29 +1600,61 @@ int batch_prepare(les_batch* B) {
   return 0;
 }
 
-// One step of every domain, enqueued on the streams (also under capture).
-int batch_enqueue(les_batch* B, const int* ns, const int* nmaxs) {
+// One step of every domain, enqueued on the streams.  Under capture the
+// domains batched by stage never enter the graph's streams: the graph runs
+// on the batch stream, and les_batch_step orders it against their streams.
+//  * Staged (every full domain at n > 1): the full domains' stages run as one
+//    launch per stage on the batch stream, their SOR iterations 2..nmaxp as
+//    one launch per iteration; the other domains step whole on their streams.
+//  * Otherwise (a first step): every full domain runs its stages and SOR
+//    iteration 1 on its own stream, the batched SOR iterations, then its own
+//    press tail.
+int batch_enqueue(les_batch* B, const int* ns, const int* nmaxs, bool capturing) {
   const size_t nd = B->doms.size();
+  bool staged = B->stages.nd > 0;
+  for (size_t q = 0; q < nd; q++)
+    if (B->full[q] && ns[q] == 1) staged = false;
   HIP_TRY(hipEventRecord(B->fork, B->stream));
   std::vector<char> split(nd, 0);
   for (size_t q = 0; q < nd; q++) {
     les_domain* D = B->doms[q];
+    if (staged && B->full[q]) {
+      if (flush_metrics(D)) return 1;
+      if (!capturing) {  // after the work already on the domain's stream
+        HIP_TRY(hipEventRecord(B->head_done[q], D->d.stream));
+        HIP_TRY(hipStreamWaitEvent(B->stream, B->head_done[q], 0));
+      }
+      continue;
+    }
     HIP_TRY(hipStreamWaitEvent(D->d.stream, B->fork, 0));
-    bool sp = false;
-    if (run_step_head(D, ns[q], nmaxs[q], &sp)) return 1;
-    split[q] = sp;
+    if (B->full[q]) {
+      bool sp = false;
+      if (run_step_head(D, ns[q], nmaxs[q], B->sor_kc, B->sor_ty, &sp)) return 1;
+      split[q] = sp;
+    } else if (run_step(D, ns[q], nmaxs[q])) {
+      return 1;
+    }
     HIP_TRY(hipEventRecord(B->head_done[q], D->d.stream));
     HIP_TRY(hipStreamWaitEvent(B->stream, B->head_done[q], 0));
   }
+  if (staged) {
+    launch_step_batch(B->stages, B->stream);
+    launch_press_batch(B->stages, 0, B->stream);
+  }
   for (int l = 2; l <= B->lmax && B->ntab > 0; l++)
-    launch_sor_batch(B->dtab, B->ntab, B->nblocks, l, B->kcmax, B->remap, B->stream);
+    launch_sor_batch(B->dtab, B->dblock0, B->ntab, B->nblocks, l, B->kcmax, B->sor_ty, B->remap, B->stream);
+  if (staged) launch_press_batch(B->stages, 1, B->stream);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipEventRecord(B->sor_done, B->stream));
   for (size_t q = 0; q < nd; q++) {
     les_domain* D = B->doms[q];
+    if (staged && B->full[q]) {
+      if (!capturing) HIP_TRY(hipStreamWaitEvent(D->d.stream, B->sor_done, 0));
+      continue;
+    }
     if (split[q]) {
       HIP_TRY(hipStreamWaitEvent(D->d.stream, B->sor_done, 0));
-      launch_press_part(D->d, D->dt, 1);
+      launch_press_part(D->d, D->dt, 1, B->sor_kc, B->sor_ty);
       HIP_TRY(hipGetLastError());
     }
     HIP_TRY(hipEventRecord(B->tail_done[q], D->d.stream));
@@ -1606,14 +1717,16 @@ int les_batch_step(les_batch* B, const int* ns, const int* nmaxs) {
     }
     if (batch_prepare(B)) return 1;
   }
+  for (les_domain* D : B->doms)
+    if (flush_metrics(D)) return 1;  // synchronous: not under capture
   bool replay = B->use_graph != 0;
   for (size_t q = 0; q < B->doms.size(); q++) replay = replay && ns[q] != 1;
-  if (!replay) return batch_enqueue(B, ns, nmaxs);
+  if (!replay) return batch_enqueue(B, ns, nmaxs, false);
   if (!B->gexec) {
     for (les_domain* D : B->doms)
       if (!D->d.fuse_velfg && ensure_cov(D)) return 1;  // allocations cannot be captured
     HIP_TRY(hipStreamBeginCapture(B->stream, hipStreamCaptureModeThreadLocal));
-    const int rc = batch_enqueue(B, ns, nmaxs);
+    const int rc = batch_enqueue(B, ns, nmaxs, true);
     hipGraph_t gr = nullptr;
     const hipError_t e = hipStreamEndCapture(B->stream, &gr);
     if (rc) {

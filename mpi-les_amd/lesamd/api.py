@@ -345,7 +345,8 @@ class Domain:
 
 class Batch:
     """Domains advanced together (les_batch_*): one step of each per call,
-    the SOR iterations of all of them batched into one launch per iteration."""
+    every stage one launch for all of them and the SOR iterations one launch
+    per iteration."""
 
     def __init__(self, domains, graph: bool = True):
         self.domains = list(domains)

@@ -123,9 +123,9 @@ class Server:
 
     def run_batched(self, requests) -> list[Result]:
         """Continuous batching: the admitted domains advance in lockstep,
-        one batched step (one launch per SOR iteration for all of them) at a
-        time; requests join at step boundaries as memory frees up, and leave
-        when their last step has completed."""
+        one batched step (one launch per stage and per SOR iteration for all
+        of them) at a time; requests join at step boundaries as memory frees
+        up, and leave when their last step has completed."""
         pending = deque(sorted(requests, key=lambda r: (r.arrival, r.rid)))
         for r in pending:
             if self.bytes_of(r) > self.budget:
