@@ -191,6 +191,13 @@ int les_domain_step(les_domain* d, int n, int nmax);
  * counterpart: the per-shape autotuner of the serving path. */
 int les_domain_autotune(les_domain* d, int reps, int* rows, int* tile_rows, int* kchunk, double* us_best,
                         double* us_default);
+/* Microseconds per fused SOR iteration with the domain's current launch
+ * shape: `iters` back-to-back iterations (numbered 2.., never stopping) on
+ * the domain's SOR scratch arrays between one pair of HIP events on its
+ * stream.  Leaves the fields and the last press statistics unchanged;
+ * synchronises.  No reference counterpart: the measurement of the dominant
+ * kernel. */
+int les_domain_time_sor(les_domain* d, int iters, double* us_per_iter);
 int les_domain_set_graph(les_domain* d, int on);
 /* Advance steps n0..n1 inclusive. */
 int les_domain_run(les_domain* d, int n0, int n1, int nmax);

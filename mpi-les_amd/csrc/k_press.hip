@@ -1201,10 +1201,10 @@ static void launch_sor_iter(const DevDomain& d, const SplitBufs& sb, int l, floa
   const SplitGrid& s = d.sg;
   if (h.rows == 1 || pent) {
     const size_t lvm_bytes = 3 * (size_t)(h.kc + 2) * sizeof(float);
-#define SOR1(SEP, ENT)                                                                                           \
-  TIMED(d, KC_SOR, hipLaunchKernelGGL((k_sor_iter<SEP, ENT>), dim3(h.nparts), dim3(FX, FY), lvm_bytes, d.stream, g, \
-                                      s, d.m, sb, d.ctrl, l, h.nparts, d.omega, pjuge, h.ntt, h.ntj, h.kc,          \
-                                      d.xcd_remap, pent, nprev))
+#define SOR1(SEP, ENT)                                                                                          \
+  TIMED(d, ENT ? KC_SOR1 : KC_SOR,                                                                              \
+        hipLaunchKernelGGL((k_sor_iter<SEP, ENT>), dim3(h.nparts), dim3(FX, FY), lvm_bytes, d.stream, g, s, d.m, \
+                           sb, d.ctrl, l, h.nparts, d.omega, pjuge, h.ntt, h.ntj, h.kc, d.xcd_remap, pent, nprev))
     if (d.cn1_separable) {
       if (pent)
         SOR1(true, true);

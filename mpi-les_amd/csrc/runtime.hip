@@ -132,7 +132,7 @@ struct Timer {
 
 const char* kClassNames[KC_N] = {"velnw", "bondv1", "vel2", "vel2_ghost", "velfg", "feedbf", "les_sm",
                                  "boundsm", "les_visc", "adam", "rhs", "sor", "boundp", "pav", "padj",
-                                 "copy"};
+                                 "copy", "sor1"};
 
 }  // namespace
 
@@ -737,6 +737,33 @@ struct TuneResult {
 std::mutex g_tune_mu;
 std::map<TuneKey, TuneResult> g_tuned;
 
+// Microseconds per iteration of `reps` back-to-back fused SOR iterations
+// (numbered 2.., never stopping) with the given shape, between one pair of
+// events on the domain's stream, after two untimed ones.  The caller saves
+// and restores the control block.
+int time_sor_trial(les_domain* D, const Ctrl& base, int rows, int ty, int kc, int reps, hipEvent_t e0,
+                   hipEvent_t e1, double* us) {
+  Ctrl c = base;
+  c.sor_done = 0;
+  c.nonfinite = 0;
+  hipError_t e = hipMemcpyAsync(D->d.ctrl, &c, sizeof(Ctrl), hipMemcpyHostToDevice, D->d.stream);
+  if (e == hipSuccess) {
+    launch_sor_trial(D->d, rows, ty, kc, 2);  // warm-up
+    e = hipEventRecord(e0, D->d.stream);
+  }
+  if (e == hipSuccess) {
+    launch_sor_trial(D->d, rows, ty, kc, reps);
+    e = hipEventRecord(e1, D->d.stream);
+  }
+  if (e == hipSuccess) e = hipEventSynchronize(e1);
+  float ms = 0.0f;
+  if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+  if (e == hipSuccess) e = hipGetLastError();
+  if (e != hipSuccess) return fail(std::string("SOR trial: ") + hipGetErrorString(e));
+  *us = 1e3 * ms / reps;
+  return 0;
+}
+
 int tune_sor(les_domain* D, int reps, TuneResult* out) {
   const Grid& g = D->d.g;
   HIP_TRY(hipStreamSynchronize(D->d.stream));
@@ -776,27 +803,8 @@ int tune_sor(les_domain* D, int reps, TuneResult* out) {
   for (const auto& kt : kernels) {
     const int rows = kt.first, ty = kt.second;
     for (int kc : kcs) {
-      Ctrl c = saved;
-      c.sor_done = 0;
-      c.nonfinite = 0;
-      hipError_t e = hipMemcpyAsync(D->d.ctrl, &c, sizeof(Ctrl), hipMemcpyHostToDevice, D->d.stream);
-      if (e == hipSuccess) {
-        launch_sor_trial(D->d, rows, ty, kc, 2);  // warm-up
-        e = hipEventRecord(e0, D->d.stream);
-      }
-      if (e == hipSuccess) {
-        launch_sor_trial(D->d, rows, ty, kc, reps);
-        e = hipEventRecord(e1, D->d.stream);
-      }
-      if (e == hipSuccess) e = hipEventSynchronize(e1);
-      float ms = 0.0f;
-      if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
-      if (e == hipSuccess) e = hipGetLastError();
-      if (e != hipSuccess) {
-        rc = fail(std::string("autotune: ") + hipGetErrorString(e));
-        break;
-      }
-      const double us = 1e3 * ms / reps;
+      double us = 0.0;
+      if ((rc = time_sor_trial(D, saved, rows, ty, kc, reps, e0, e1, &us)) != 0) break;
       if (kt == kdef2 && kc == kdef) r.us_default = us;
       if (best < 0.0 || us < best) {
         best = us;
@@ -1354,13 +1362,35 @@ int les_domain_autotune(les_domain* D, int reps, int* rows, int* tile_rows, int*
   return 0;
 }
 
+int les_domain_time_sor(les_domain* D, int iters, double* us_per_iter) {
+  if (!D || !us_per_iter) return fail("null argument");
+  if (iters < 1) return fail("les_domain_time_sor: iters < 1");
+  if (!D->d.sor_fused) return fail("les_domain_time_sor: the fused SOR iteration is off");
+  HIP_TRY(hipStreamSynchronize(D->d.stream));
+  Ctrl saved;
+  HIP_TRY(hipMemcpy(&saved, D->d.ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost));
+  const int kc = D->d.sor_kchunk > 0 ? D->d.sor_kchunk
+                                     : sor_fused_kchunk(D->d.g, D->d.sg, D->d.sor_fused_blocks, D->d.ncu);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  HIP_TRY(hipEventCreate(&e0));
+  if (hipEventCreate(&e1) != hipSuccess) {
+    hipEventDestroy(e0);
+    return fail("hipEventCreate");
+  }
+  int rc = time_sor_trial(D, saved, D->d.sor_rows, D->d.sor_tile_rows, kc, iters, e0, e1, us_per_iter);
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  if (hipMemcpy(D->d.ctrl, &saved, sizeof(Ctrl), hipMemcpyHostToDevice) != hipSuccess && !rc)
+    rc = fail("les_domain_time_sor: restoring the control block");
+  return rc;
+}
+
 // ------------------------------------------------------------ batches
-// Several domains advanced together.  Each domain's stages run on its own
-// stream; the SOR iterations 2..nmaxp of every domain whose SOR can be
-// batched run as one launch per iteration (k_sor_iter2_batch) on the
-// batch's stream, between events that join the streams; each domain then
-// finishes its press on its own stream.  Steps with every n > 1 replay one
-// graph captured across all the streams.
+// Several domains advanced together.  Domains whose whole step can be
+// batched run every stage as one launch for all of them and the SOR
+// iterations 2..nmaxp as one launch per iteration (k_sor_iter2_batch), on
+// the batch's stream; the others step whole on their own streams, joined by
+// events.  Steps with every n > 1 replay one graph.
 
 int les_domain_set_graph(les_domain* D, int on) {
   if (!D) return fail("null domain");

@@ -102,6 +102,7 @@ def lib():
     L.les_domain_set_timing_runs.argtypes = [vp, ci]
     L.les_domain_set_graph.argtypes = [vp, ci]
     L.les_domain_autotune.argtypes = [vp, ci, vp, vp, vp, vp, vp]
+    L.les_domain_time_sor.argtypes = [vp, ci, vp]
     L.les_domain_timing.argtypes = [vp, vp, vp]
     L.les_batch_create.argtypes = [vp, ci, ctypes.POINTER(vp)]
     L.les_batch_destroy.argtypes = [vp]
@@ -298,6 +299,13 @@ class Domain:
                                          ctypes.byref(ub), ctypes.byref(ud)))
         return dict(rows=r.value, tile_rows=ty.value, kchunk=kc.value, us_per_iter=ub.value,
                     us_per_iter_default=ud.value)
+
+    def time_sor(self, iters: int = 200) -> float:
+        """Microseconds per fused SOR iteration of the current launch shape:
+        `iters` back-to-back launches between one pair of HIP events."""
+        us = ctypes.c_double()
+        _check(lib().les_domain_time_sor(self._h, iters, ctypes.byref(us)))
+        return us.value
 
     def set_graph(self, on: bool):
         """Replay steps n > 1 from one captured HIP graph."""

@@ -36,6 +36,11 @@ def test_autotune_keeps_state_and_results(case):
     st_after = a.press_stats()
     assert st_after["iters"] == st_before["iters"]
     assert st_after["sor"].tolist() == st_before["sor"].tolist()
+    # the timed run of the chosen shape leaves the state as well
+    assert a.time_sor(20) > 0.0
+    for f in FIELDS:
+        assert np.array_equal(a.download(f), before[f]), f
+    assert a.press_stats()["sor"].tolist() == st_before["sor"].tolist()
     # remembered for the shape
     rb = b.autotune()
     assert (rb["kchunk"], rb["rows"], rb["tile_rows"]) == (r["kchunk"], r["rows"], r["tile_rows"])
