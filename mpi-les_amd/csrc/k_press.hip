@@ -107,7 +107,7 @@ __global__ __launch_bounds__(NT) void k_rhs(Grid g, Metrics m, const float* __re
                                             float* __restrict__ f, float* __restrict__ gg,
                                             float* __restrict__ h, float* __restrict__ rhs,
                                             double* __restrict__ part, Ctrl* ctrl, float dt) {
-  rhs_body(hw_blk(), g, m, u, v, w, f, gg, h, rhs, part, ctrl, dt);
+  rhs_body(hw_blk<false>(), g, m, u, v, w, f, gg, h, rhs, part, ctrl, dt);
 }
 
 // One block: total of n partials in a fixed order; what=0 -> rhsav,
@@ -159,7 +159,7 @@ __device__ __forceinline__ void rhs_split_body(const VBlk& vb, const Grid& g, co
 __global__ __launch_bounds__(NT) void k_rhs_split(Grid g, SplitGrid s, float* __restrict__ rhs,
                                                   const float* __restrict__ p, const float* __restrict__ cn1,
                                                   SplitBufs sb, const Ctrl* ctrl) {
-  rhs_split_body(hw_blk(), g, s, rhs, p, cn1, sb, ctrl);
+  rhs_split_body(hw_blk<false>(), g, s, rhs, p, cn1, sb, ctrl);
 }
 
 // ---------------------------------------------------------------- SOR
@@ -957,7 +957,7 @@ __device__ __forceinline__ void pav_body(const VBlk& vb, const Grid& g, const Sp
 }
 __global__ __launch_bounds__(NT) void k_pav(Grid g, SplitGrid s, Metrics m, SplitBufs sb,
                                             double* __restrict__ part, const Ctrl* ctrl) {
-  pav_body(hw_blk(), g, s, m, sb, part, ctrl);
+  pav_body(hw_blk<false>(), g, s, m, sb, part, ctrl);
 }
 
 // p = p - pav on the interior (press.f95:145-150, FP64 subtraction) and the
@@ -986,7 +986,7 @@ __device__ __forceinline__ void padj_body(const VBlk& vb, const Grid& g, const S
 }
 __global__ __launch_bounds__(NT) void k_padj(Grid g, SplitGrid s, SplitBufs sb, float* __restrict__ p,
                                              const Ctrl* ctrl) {
-  padj_body(hw_blk(), g, s, sb, p, ctrl);
+  padj_body(hw_blk<false>(), g, s, sb, p, ctrl);
 }
 
 // After the SOR loop: the closing stop-test sum, the pressure mean and the
@@ -1373,16 +1373,16 @@ void launch_press_part(const DevDomain& d, float dt, int part, int kc, int ty) {
 // the stage kernel's body on its own domain with that domain's grid, block
 // index and arguments, so every value and every partial sum is the one of
 // the domain's own launch.
-#define BATCH_BLOCK(stage)                                          \
+#define BATCH_BLOCK(stage, xcd)                                     \
   int e;                                                            \
   VBlk vb;                                                          \
-  if (!batch_block(sp + (stage) * nd, nd, e, vb)) return;           \
+  if (!batch_block<xcd>(sp + (stage) * nd, nd, e, vb)) return;      \
   const BatchDom& B = bd[e];                                        \
   const DevDomain& d = B.d
 
 __global__ __launch_bounds__(NT) void kb_rhs(const BatchDom* __restrict__ bd, const StageSpan* __restrict__ sp,
                                              int nd) {
-  BATCH_BLOCK(BS_RHS);
+  BATCH_BLOCK(BS_RHS, false);
   rhs_body(vb, d.g, d.m, d.F.u, d.F.v, d.F.w, d.F.f, d.F.g, d.F.h, d.F.rhs, d.partials, d.ctrl, B.dt);
 }
 // one block per domain
@@ -1392,12 +1392,12 @@ __global__ __launch_bounds__(256) void kb_finalize(const BatchDom* __restrict__ 
 }
 __global__ __launch_bounds__(NT) void kb_rhs_split(const BatchDom* __restrict__ bd,
                                                    const StageSpan* __restrict__ sp, int nd) {
-  BATCH_BLOCK(BS_SPLIT);
+  BATCH_BLOCK(BS_SPLIT, false);
   rhs_split_body(vb, d.g, d.sg, d.F.rhs, d.F.p, d.F.cn1, B.sb, d.ctrl);
 }
 __global__ __launch_bounds__(FNT) __attribute__((amdgpu_waves_per_eu(8, 8))) void kb_sor_entry(
     const BatchDom* __restrict__ bd, const StageSpan* __restrict__ sp, int nd) {
-  BATCH_BLOCK(BS_SOR1);
+  BATCH_BLOCK(BS_SOR1, false);
   sor_iter_block<true, true>(d.g, d.sg, d.m, B.sb, d.ctrl, 1, B.nparts1, d.omega, d.pjuge, B.ntt1, B.ntj1, B.kc1,
                              d.xcd_remap, d.F.p, B.nparts1, vb.x);
 }
@@ -1408,12 +1408,12 @@ __global__ __launch_bounds__(256) void kb_sor_final(const BatchDom* __restrict__
 }
 __global__ __launch_bounds__(NT) void kb_pav(const BatchDom* __restrict__ bd, const StageSpan* __restrict__ sp,
                                              int nd) {
-  BATCH_BLOCK(BS_PAV);
+  BATCH_BLOCK(BS_PAV, false);
   pav_body(vb, d.g, d.sg, d.m, B.sb, d.partials, d.ctrl);
 }
 __global__ __launch_bounds__(NT) void kb_padj(const BatchDom* __restrict__ bd, const StageSpan* __restrict__ sp,
                                               int nd) {
-  BATCH_BLOCK(BS_PADJ);
+  BATCH_BLOCK(BS_PADJ, false);
   padj_body(vb, d.g, d.sg, B.sb, d.F.p, d.ctrl);
 }
 #undef BATCH_BLOCK

@@ -52,7 +52,7 @@ __global__ __launch_bounds__(NT) void k_velnw(Grid g, Metrics m, const float* __
                                               const float* __restrict__ f, const float* __restrict__ gg,
                                               const float* __restrict__ h, float* __restrict__ u,
                                               float* __restrict__ v, float* __restrict__ w, float ro, float dt) {
-  velnw_body(hw_blk(), g, m, p, f, gg, h, u, v, w, ro, dt);
+  velnw_body(hw_blk<true>(), g, m, p, f, gg, h, u, v, w, ro, dt);
 }
 
 void launch_velnw(const DevDomain& d, float ro, float dt) {
@@ -85,7 +85,7 @@ __device__ __forceinline__ void bondv1_inflow_body(const VBlk& vb, const Grid& g
 __global__ __launch_bounds__(NT) void k_bondv1_inflow(Grid g, Metrics m, float* __restrict__ u,
                                                       float* __restrict__ v, float* __restrict__ w,
                                                       int ilast, Ctrl* ctrl) {
-  bondv1_inflow_body(hw_blk(), g, m, u, v, w, ilast, ctrl);
+  bondv1_inflow_body(hw_blk<false>(), g, m, u, v, w, ilast, ctrl);
 }
 
 // aaa = max(0, max u(im,j,k)), bbb = min(0, min u(im,j,k)) (bondv1.f95:98-114).
@@ -121,7 +121,7 @@ __device__ __forceinline__ void bondv1_uout_body(const VBlk& vb, const Grid& g, 
   }
 }
 __global__ __launch_bounds__(256) void k_bondv1_uout(Grid g, const float* __restrict__ u, Ctrl* ctrl) {
-  bondv1_uout_body(hw_blk(), g, u, ctrl);
+  bondv1_uout_body(hw_blk<false>(), g, u, ctrl);
 }
 
 // Advective outflow at i=im (u) and i=im+1 (v,w), bondv1.f95:116-134.
@@ -144,7 +144,7 @@ __device__ __forceinline__ void bondv1_outflow_body(const VBlk& vb, const Grid& 
 __global__ __launch_bounds__(256) void k_bondv1_outflow(Grid g, Metrics m, float* __restrict__ u,
                                                         float* __restrict__ v, float* __restrict__ w,
                                                         const Ctrl* ctrl, float dt) {
-  bondv1_outflow_body(hw_blk(), g, m, u, v, w, ctrl, dt);
+  bondv1_outflow_body(hw_blk<false>(), g, m, u, v, w, ctrl, dt);
 }
 
 // Periodic j ghosts (bondv1.f95:137-156) followed by the ground/top rows
@@ -186,7 +186,7 @@ __device__ __forceinline__ void bondv1_ghost_body(const VBlk& vb, const Grid& g,
 }
 __global__ __launch_bounds__(256) void k_bondv1_ghost(Grid g, float* __restrict__ u, float* __restrict__ v,
                                                       float* __restrict__ w) {
-  bondv1_ghost_body(hw_blk(), g, u, v, w);
+  bondv1_ghost_body(hw_blk<false>(), g, u, v, w);
 }
 
 // Grids of the four bondv1 launches (one-dimensional but the first).
@@ -696,7 +696,7 @@ __global__ __launch_bounds__(NT) void k_velfg_feedbf(VelCtx X, P10 diu, float* _
                                                      const float* __restrict__ cm, const float* __restrict__ dm,
                                                      float* __restrict__ uspd, float* __restrict__ vspd, float vn,
                                                      float alpha, float dt, float beta, int kchunk) {
-  velfg_feedbf_body(hw_blk(), X, diu, f, gg, h, usum, vsum, wsum, bm, cm, dm, uspd, vspd, vn, alpha, dt, beta,
+  velfg_feedbf_body(hw_blk<false>(), X, diu, f, gg, h, usum, vsum, wsum, bm, cm, dm, uspd, vspd, vn, alpha, dt, beta,
                     kchunk);
 }
 
@@ -799,7 +799,7 @@ __device__ __forceinline__ void les_sm_body(const VBlk& vb, const Grid& g, const
 }
 __global__ __launch_bounds__(NT) void k_les_sm(Grid g, P10 diu, const float* __restrict__ delx1,
                                                float* __restrict__ sm) {
-  les_sm_body(hw_blk(), g, diu, delx1, sm);
+  les_sm_body(hw_blk<true>(), g, diu, delx1, sm);
 }
 
 // boundsm.f95:18-64 as one pass: the serial build runs the i-Neumann
@@ -855,7 +855,7 @@ __device__ __forceinline__ void boundsm_body(const VBlk& vb, const Grid& g, floa
   const float s = sm[ix(g, ii, jj, kk)];
   sm[ix(g, i, j, k)] = (sgn < 0.0f) ? -s : s;
 }
-__global__ __launch_bounds__(256) void k_boundsm(Grid g, float* __restrict__ sm) { boundsm_body(hw_blk(), g, sm); }
+__global__ __launch_bounds__(256) void k_boundsm(Grid g, float* __restrict__ sm) { boundsm_body(hw_blk<false>(), g, sm); }
 
 // Viscous terms added to f (k=2..km, wall k=1), g (same) and h (k=1..km),
 // les.f95:83-242.
@@ -981,7 +981,7 @@ __global__ __launch_bounds__(NT) void k_les_visc(Grid g, Metrics m, P10 diu, con
                                                  float* __restrict__ f, float* __restrict__ gg,
                                                  float* __restrict__ h, float* __restrict__ fo,
                                                  float* __restrict__ go, float* __restrict__ ho, float wl) {
-  les_visc_body<ADAM>(hw_blk(), g, m, diu, sm, uspd, vspd, f, gg, h, fo, go, ho, wl);
+  les_visc_body<ADAM>(hw_blk<true>(), g, m, diu, sm, uspd, vspd, f, gg, h, fo, go, ho, wl);
 }
 
 static int boundsm_blocks(const Grid& g) {
@@ -1152,58 +1152,58 @@ void launch_pack(const DevDomain& d, const float* src, float* dst, int li, int l
 // position in that domain's own grid, then runs the stage kernel's body with
 // that domain's arguments: the work of every block, and so every value, is
 // the same as in the domain's own launch.
-#define BATCH_BLOCK(stage)                                          \
+#define BATCH_BLOCK(stage, xcd)                                     \
   int e;                                                            \
   VBlk vb;                                                          \
-  if (!batch_block(sp + (stage) * nd, nd, e, vb)) return;           \
+  if (!batch_block<xcd>(sp + (stage) * nd, nd, e, vb)) return;      \
   const BatchDom& B = bd[e];                                        \
   const DevDomain& d = B.d
 
 __global__ __launch_bounds__(NT) void kb_velnw(const BatchDom* __restrict__ bd, const StageSpan* __restrict__ sp,
                                                int nd) {
-  BATCH_BLOCK(BS_VELNW);
+  BATCH_BLOCK(BS_VELNW, true);
   velnw_body(vb, d.g, d.m, d.F.p, d.F.f, d.F.g, d.F.h, d.F.u, d.F.v, d.F.w, B.ro, B.dt);
 }
 __global__ __launch_bounds__(NT) void kb_bondv1_inflow(const BatchDom* __restrict__ bd,
                                                        const StageSpan* __restrict__ sp, int nd) {
-  BATCH_BLOCK(BS_INFLOW);
+  BATCH_BLOCK(BS_INFLOW, false);
   bondv1_inflow_body(vb, d.g, d.m, d.F.u, d.F.v, d.F.w, B.ilast, d.ctrl);
 }
 __global__ __launch_bounds__(256) void kb_bondv1_uout(const BatchDom* __restrict__ bd,
                                                       const StageSpan* __restrict__ sp, int nd) {
-  BATCH_BLOCK(BS_UOUT);
+  BATCH_BLOCK(BS_UOUT, false);
   bondv1_uout_body(vb, d.g, d.F.u, d.ctrl);
 }
 __global__ __launch_bounds__(256) void kb_bondv1_outflow(const BatchDom* __restrict__ bd,
                                                          const StageSpan* __restrict__ sp, int nd) {
-  BATCH_BLOCK(BS_OUTFLOW);
+  BATCH_BLOCK(BS_OUTFLOW, false);
   bondv1_outflow_body(vb, d.g, d.m, d.F.u, d.F.v, d.F.w, d.ctrl, B.dt);
 }
 __global__ __launch_bounds__(256) void kb_bondv1_ghost(const BatchDom* __restrict__ bd,
                                                        const StageSpan* __restrict__ sp, int nd) {
-  BATCH_BLOCK(BS_GHOST);
+  BATCH_BLOCK(BS_GHOST, false);
   bondv1_ghost_body(vb, d.g, d.F.u, d.F.v, d.F.w);
 }
 __global__ __launch_bounds__(NT) void kb_velfg_feedbf(const BatchDom* __restrict__ bd,
                                                       const StageSpan* __restrict__ sp, int nd) {
-  BATCH_BLOCK(BS_VELFG);
+  BATCH_BLOCK(BS_VELFG, false);
   const VelCtx X{d.g, d.m, d.F.u, d.F.v, d.F.w, d.wl};
   velfg_feedbf_body(vb, X, p10(d.F.diu), d.F.f, d.F.g, d.F.h, d.F.usum, d.F.vsum, d.F.wsum, d.F.bmask, d.F.cmask,
                     d.F.dmask, d.F.uspd, d.F.vspd, B.vn, B.alpha, B.dt, B.beta, B.velfg_kc);
 }
 __global__ __launch_bounds__(NT) void kb_les_sm(const BatchDom* __restrict__ bd, const StageSpan* __restrict__ sp,
                                                 int nd) {
-  BATCH_BLOCK(BS_LES_SM);
+  BATCH_BLOCK(BS_LES_SM, true);
   les_sm_body(vb, d.g, p10(d.F.diu), d.m.delx1 + 1, d.F.sm);
 }
 __global__ __launch_bounds__(256) void kb_boundsm(const BatchDom* __restrict__ bd, const StageSpan* __restrict__ sp,
                                                   int nd) {
-  BATCH_BLOCK(BS_BOUNDSM);
+  BATCH_BLOCK(BS_BOUNDSM, false);
   boundsm_body(vb, d.g, d.F.sm);
 }
 __global__ __launch_bounds__(NT) void kb_les_visc_adam(const BatchDom* __restrict__ bd,
                                                        const StageSpan* __restrict__ sp, int nd) {
-  BATCH_BLOCK(BS_LES_VISC);
+  BATCH_BLOCK(BS_LES_VISC, true);
   les_visc_body<true>(vb, d.g, d.m, p10(d.F.diu), d.F.sm, d.F.uspd, d.F.vspd, d.F.f, d.F.g, d.F.h, d.F.fold,
                       d.F.gold, d.F.hold, d.wl);
 }

@@ -152,15 +152,23 @@ struct StageBatch {
 };
 // This block's domain e and position in that domain's grid, in a batched
 // launch over the spans sp[0..nd) (b0 ascending); false for the padding
-// blocks after a domain's grid.  Block-uniform.
+// blocks after a domain's grid.  Block-uniform.  XCD: the positions are
+// renumbered as hw_blk<true> does (b0 is a multiple of 8, so block l of a
+// domain runs on XCD l % 8, as in a launch of its own).
+template <bool XCD>
 __device__ __forceinline__ bool batch_block(const StageSpan* __restrict__ sp, int nd, int& e, VBlk& vb) {
   const int b = (int)blockIdx.x;
   int q = 0;
   for (int r = 1; r < nd; r++)
     if (b >= sp[r].b0) q = r;
   const StageSpan s = sp[q];
-  const int l = b - s.b0;
-  if (l >= s.gx * s.gy * s.gz) return false;
+  const int n = s.gx * s.gy * s.gz;
+  int l = b - s.b0;
+  if (l >= n) return false;
+  if (XCD) {
+    const int x8 = l % 8, base = n / 8, rem = n % 8;
+    l = x8 * base + (x8 < rem ? x8 : rem) + l / 8;
+  }
   vb = {l % s.gx, (l / s.gx) % s.gy, l / (s.gx * s.gy), s.gx, s.gy, s.gz};
   e = q;
   return true;
