@@ -345,6 +345,21 @@ __device__ __forceinline__ void bstore(Rsrc r, float v, unsigned voff, unsigned 
   __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, voff,
                                         __builtin_amdgcn_readfirstlane(soff), 0);
 }
+// acc + x*x in FP64 (the square is exact), x taken as 0 where keep is false.
+// The select stays on the FP32 value: the empty asm operand (no instruction)
+// keeps the compiler from moving it after the conversion, where it takes a
+// select per 32-bit half.
+// omega * y, the relaxation of the SOR update (press.f95:93-104).  1 * y is
+// y for every y, so with omega == 1 (OM1) the multiplication is dropped.
+template <bool OM1>
+__device__ __forceinline__ float relax(float omega, float y) {
+  return OM1 ? y : omega * y;
+}
+__device__ __forceinline__ double acc_sq(double acc, float x, bool keep) {
+  float z = keep ? x : 0.0f;
+  asm("" : "+v"(z));
+  return __builtin_fma((double)z, (double)z, acc);
+}
 __device__ __forceinline__ unsigned boff(const float* base, const float* p) {
   return (unsigned)((const char*)p - (const char*)base);
 }
@@ -595,7 +610,7 @@ __global__ __launch_bounds__(FNT) __attribute__((amdgpu_waves_per_eu(8, 8))) voi
 // nprev: partial-sum slots written by the previous iteration's launch.
 // One block's share of the iteration: bid is the block's index within the
 // launch of this domain.
-template <bool SEP, int TY>
+template <bool SEP, int TY, bool OM1>
 __device__ __forceinline__ void sor_iter2_block(Grid g, SplitGrid s, Metrics m, const SplitBufs& sb, Ctrl* ctrl, int iter,
                                                 int nparts, int nprev, float omega, float pjuge, int ntt, int ntj,
                                                 int kchunk, int remap, int bid) {
@@ -771,13 +786,12 @@ __device__ __forceinline__ void sor_iter2_block(Grid g, SplitGrid s, Metrics m, 
         const float pkm = kfirst ? own : bqA[U % 6];
         const float pkp = klast ? own : bqA[(U + 2) % 6];
         const float cn = SEP ? rcp_rn(sA[pa] + mz[P]) : q.crA;  // init.f95:100-107
-        const float reltmp = omega * (cn * (c2l[pa] * pip + c2s[pa] * pim + c3lA * pjp + c3sA * pjm +
+        const float reltmp = relax<OM1>(omega, cn * (c2l[pa] * pip + c2s[pa] * pim + c3lA * pjp + c3sA * pjm +
                                             mx[P] * pkp + my[P] * pkm - q.rrA) -
                                       own);
         redA = own + reltmp;
         bstore(S, redA, svA[pa], soR);
-        const float z = (okA[pa] && kinR) ? reltmp : 0.0f;
-        acc = __builtin_fma((double)z, (double)z, acc);  // exact square, as k_sor_iter
+        acc = acc_sq(acc, reltmp, okA[pa] && kinR);  // exact square, as k_sor_iter
       }
       {
         const float own = q.roB;
@@ -789,13 +803,12 @@ __device__ __forceinline__ void sor_iter2_block(Grid g, SplitGrid s, Metrics m, 
         const float pkm = kfirst ? own : bqB[U % 6];
         const float pkp = klast ? own : bqB[(U + 2) % 6];
         const float cn = SEP ? rcp_rn(sB[pb] + mz[P]) : q.crB;
-        const float reltmp = omega * (cn * (c2l[pb] * pip + c2s[pb] * pim + c3lB * pjp + c3sB * pjm +
+        const float reltmp = relax<OM1>(omega, cn * (c2l[pb] * pip + c2s[pb] * pim + c3lB * pjp + c3sB * pjm +
                                             mx[P] * pkp + my[P] * pkm - q.rrB) -
                                       own);
         redB = own + reltmp;
         bstore(S, redB, svB[pb], soR);
-        const float z = (okB[pb] && kinR) ? reltmp : 0.0f;
-        acc = __builtin_fma((double)z, (double)z, acc);
+        acc = acc_sq(acc, reltmp, okB[pb] && kinR);
       }
       LR[P * PL + ownA] = redA;
       LR[P * PL + ownA + FX] = redB;
@@ -814,12 +827,11 @@ __device__ __forceinline__ void sor_iter2_block(Grid g, SplitGrid s, Metrics m, 
         const float pkm = bfirst ? own : rdA[(U + 1) % 3];
         const float pkp = blast ? own : redA;
         const float cn = SEP ? rcp_rn(sA[pa] + mz[P ^ 1]) : q.cbA;
-        const float reltmp = omega * (cn * (c2l[pa] * pip + c2s[pa] * pim + c3lA * pjp + c3sA * pjm +
+        const float reltmp = relax<OM1>(omega, cn * (c2l[pa] * pip + c2s[pa] * pim + c3lA * pjp + c3sA * pjm +
                                             mx[P ^ 1] * pkp + my[P ^ 1] * pkm - q.rbA) -
                                       own);
         bstore(S, own + reltmp, svA[pa], soB);
-        const float z = (okA[pa] && kinB) ? reltmp : 0.0f;
-        acc = __builtin_fma((double)z, (double)z, acc);
+        acc = acc_sq(acc, reltmp, okA[pa] && kinB);
       }
       {
         const float own = bqB[U % 6];
@@ -831,12 +843,11 @@ __device__ __forceinline__ void sor_iter2_block(Grid g, SplitGrid s, Metrics m, 
         const float pkm = bfirst ? own : rdB[(U + 1) % 3];
         const float pkp = blast ? own : redB;
         const float cn = SEP ? rcp_rn(sB[pb] + mz[P ^ 1]) : q.cbB;
-        const float reltmp = omega * (cn * (c2l[pb] * pip + c2s[pb] * pim + c3lB * pjp + c3sB * pjm +
+        const float reltmp = relax<OM1>(omega, cn * (c2l[pb] * pip + c2s[pb] * pim + c3lB * pjp + c3sB * pjm +
                                             mx[P ^ 1] * pkp + my[P ^ 1] * pkm - q.rbB) -
                                       own);
         bstore(S, own + reltmp, svB[pb], soB);
-        const float z = (okB[pb] && kinB) ? reltmp : 0.0f;
-        acc = __builtin_fma((double)z, (double)z, acc);
+        acc = acc_sq(acc, reltmp, okB[pb] && kinB);
       }
       // 4. operands of later steps
       fill(q, kr + 3);
@@ -876,19 +887,20 @@ __device__ __forceinline__ void sor_iter2_block(Grid g, SplitGrid s, Metrics m, 
   }
 }
 
-template <bool SEP, int TY>
+template <bool SEP, int TY, bool OM1>
 __global__ __launch_bounds__(FX * TY / 2) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_sor_iter2(
     Grid g, SplitGrid s, Metrics m, SplitBufs sb, Ctrl* ctrl, int iter, int nparts, int nprev, float omega,
     float pjuge, int ntt, int ntj, int kchunk, int remap) {
-  sor_iter2_block<SEP, TY>(g, s, m, sb, ctrl, iter, nparts, nprev, omega, pjuge, ntt, ntj, kchunk, remap,
-                           (int)blockIdx.x);
+  sor_iter2_block<SEP, TY, OM1>(g, s, m, sb, ctrl, iter, nparts, nprev, omega, pjuge, ntt, ntj, kchunk, remap,
+                                (int)blockIdx.x);
 }
 
 // The same iteration for several domains in one launch: each domain owns a
 // range of blocks (starting at a multiple of 8, so a block's XCD is the same
 // as in a launch of its own), domains past their own nmaxp return at once.
-// Only k_sor_iter2<true, TY> shapes are batched, with one TY for the batch.
-template <int TY>
+// Only k_sor_iter2<true, TY, OM1> shapes are batched, with one TY for the
+// batch (OM1: every domain has omega == 1).
+template <int TY, bool OM1>
 __global__ __launch_bounds__(FX * TY / 2) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_sor_iter2_batch(
     const SorBatchEntry* __restrict__ tab, const int* __restrict__ block0, int ntab, int iter, int remap) {
   int e = 0;
@@ -897,19 +909,28 @@ __global__ __launch_bounds__(FX * TY / 2) __attribute__((amdgpu_waves_per_eu(4, 
   const SorBatchEntry& E = tab[e];
   const int bid = (int)blockIdx.x - E.block0;
   if (bid >= E.nparts || iter > E.nmaxp) return;  // padding block, or this domain has finished its iterations
-  sor_iter2_block<true, TY>(E.g, E.s, E.m, E.sb, E.ctrl, iter, E.nparts, iter == 2 ? E.nparts1 : E.nparts, E.omega,
-                            E.pjuge, E.ntt, E.ntj, E.kchunk, remap, bid);
+  sor_iter2_block<true, TY, OM1>(E.g, E.s, E.m, E.sb, E.ctrl, iter, E.nparts, iter == 2 ? E.nparts1 : E.nparts,
+                                 E.omega, E.pjuge, E.ntt, E.ntj, E.kchunk, remap, bid);
 }
 
 void launch_sor_batch(const SorBatchEntry* tab, const int* block0, int ntab, int nblocks, int iter, int kcmax,
-                      int ty, int remap, hipStream_t stream) {
+                      int ty, bool om1, int remap, hipStream_t stream) {
   const size_t lds = (size_t)(kcmax + 2) * sizeof(float4);
-  if (ty == 32)
-    hipLaunchKernelGGL(k_sor_iter2_batch<32>, dim3(nblocks), dim3(FX, 16), lds, stream, tab, block0, ntab, iter,
-                       remap);
-  else
-    hipLaunchKernelGGL(k_sor_iter2_batch<FY>, dim3(nblocks), dim3(FX, FY / 2), lds, stream, tab, block0, ntab, iter,
-                       remap);
+#define SORB(TY, OM1)                                                                                          \
+  hipLaunchKernelGGL((k_sor_iter2_batch<TY, OM1>), dim3(nblocks), dim3(FX, TY / 2), lds, stream, tab, block0, ntab, \
+                     iter, remap)
+  if (ty == 32) {
+    if (om1)
+      SORB(32, true);
+    else
+      SORB(32, false);
+  } else {
+    if (om1)
+      SORB(FY, true);
+    else
+      SORB(FY, false);
+  }
+#undef SORB
 }
 
 // After the iteration loop: when no iteration met the stop test, the last
@@ -1221,9 +1242,16 @@ static void launch_sor_iter(const DevDomain& d, const SplitBufs& sb, int l, floa
   }
   const size_t lvm_bytes = (size_t)(h.kc + 2) * sizeof(float4);
 #define SOR2(SEP, TY)                                                                                        \
-  TIMED(d, KC_SOR, hipLaunchKernelGGL((k_sor_iter2<SEP, TY>), dim3(h.nparts), dim3(FX, TY / 2), lvm_bytes,     \
-                                      d.stream, g, s, d.m, sb, d.ctrl, l, h.nparts, nprev, d.omega, pjuge, h.ntt, \
-                                      h.ntj, h.kc, d.xcd_remap))
+  do {                                                                                                       \
+    if (d.omega == 1.0f)                                                                                     \
+      TIMED(d, KC_SOR, hipLaunchKernelGGL((k_sor_iter2<SEP, TY, true>), dim3(h.nparts), dim3(FX, TY / 2),      \
+                                          lvm_bytes, d.stream, g, s, d.m, sb, d.ctrl, l, h.nparts, nprev,      \
+                                          d.omega, pjuge, h.ntt, h.ntj, h.kc, d.xcd_remap));                   \
+    else                                                                                                     \
+      TIMED(d, KC_SOR, hipLaunchKernelGGL((k_sor_iter2<SEP, TY, false>), dim3(h.nparts), dim3(FX, TY / 2),     \
+                                          lvm_bytes, d.stream, g, s, d.m, sb, d.ctrl, l, h.nparts, nprev,      \
+                                          d.omega, pjuge, h.ntt, h.ntj, h.kc, d.xcd_remap));                   \
+  } while (0)
   if (d.cn1_separable) {
     if (h.ty == 32)
       SOR2(true, 32);

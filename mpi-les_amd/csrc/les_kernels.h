@@ -99,9 +99,9 @@ struct SorBatchEntry {
   float omega, pjuge;
 };
 // block0: the entries' block0 values, packed (the block-to-domain lookup);
-// ty: tile rows of the batch (16 or 32).
+// ty: tile rows of the batch (16 or 32); om1: every entry has omega == 1.
 void launch_sor_batch(const SorBatchEntry* tab, const int* block0, int ntab, int nblocks, int iter, int kcmax,
-                      int ty, int remap, hipStream_t stream);
+                      int ty, bool om1, int remap, hipStream_t stream);
 // The press stage in two parts around iterations 2..nmaxp of the SOR, for
 // domains whose iterations run batched: part 0 is the right-hand side and
 // iteration 1, part 1 the stop test of the last iteration and the mean

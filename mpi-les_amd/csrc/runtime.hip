@@ -1486,6 +1486,7 @@ struct les_batch {
   const int* dblock0 = nullptr;
   int ntab = 0, nblocks = 0, kcmax = 0, lmax = 0, remap = 1;
   int sor_kc = 0, sor_ty = 16;    // SOR k chunk and tile rows of the batched domains
+  bool om1 = false;               // every batched domain has omega == 1
   // Domains whose whole step runs as batched stage launches (fused velfg and
   // les, batchable SOR); the others step on their own streams.
   std::vector<char> full;
@@ -1536,6 +1537,7 @@ int batch_prepare(les_batch* B) {
   int block = 0;
   B->kcmax = 0;
   B->lmax = 0;
+  B->om1 = true;
   B->full.assign(B->doms.size(), 0);
   std::vector<const DevDomain*> fd;
   long tiles = 0;  // (i,j) tiles of the stage kernels over the batched domains
@@ -1565,6 +1567,7 @@ int batch_prepare(les_batch* B) {
     B->kcmax = std::max(B->kcmax, e.kchunk);
     B->lmax = std::max(B->lmax, e.nmaxp);
     tab.push_back(e);
+    B->om1 = B->om1 && D->d.omega == 1.0f;
     BatchDom bd{};
     std::array<StageSpan, BS_N> sp{};
     bd.d = D->d;
@@ -1672,7 +1675,7 @@ int batch_enqueue(les_batch* B, const int* ns, const int* nmaxs, bool capturing)
     launch_press_batch(B->stages, 0, B->stream);
   }
   for (int l = 2; l <= B->lmax && B->ntab > 0; l++)
-    launch_sor_batch(B->dtab, B->dblock0, B->ntab, B->nblocks, l, B->kcmax, B->sor_ty, B->remap, B->stream);
+    launch_sor_batch(B->dtab, B->dblock0, B->ntab, B->nblocks, l, B->kcmax, B->sor_ty, B->om1, B->remap, B->stream);
   if (staged) launch_press_batch(B->stages, 1, B->stream);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipEventRecord(B->sor_done, B->stream));
