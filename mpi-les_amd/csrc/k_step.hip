@@ -779,16 +779,32 @@ __device__ __forceinline__ void les_sm_body(const VBlk& vb, const Grid& g, const
   const float* d8 = diu.a[8];
   int k0, k1;
   kspan(vb, 1, g.km, k0, k1);
-  for (int k = k0; k <= k1; k++) {
-    const int c = ix(g, i, j, k);
+  if (k0 > k1) return;
+  // A thread walks its run of levels keeping what level k+1 shares with
+  // level k: diu3 and diu6 at k+1 become the next step's level-k values,
+  // diu7 and diu8 at k become its level-(k-1) values.
+  int c = ix(g, i, j, k0);
+  float h3m = d3[c - 1], h3c = d3[c], h6m = d6[c - sj], h6c = d6[c];  // diu3, diu6 at level k
+  float p7c = d7[c - sk], p7p = d7[c + 1 - sk], p8c = d8[c - sk], p8j = d8[c + sj - sk];  // diu7, diu8 at k-1
+  for (int k = k0; k <= k1; k++, c += sk) {
+    const float l3m = h3m, l3c = h3c, l6m = h6m, l6c = h6c;
+    h3m = d3[c - 1 + sk];
+    h3c = d3[c + sk];
+    h6m = d6[c - sj + sk];
+    h6c = d6[c + sk];
+    const float c7c = d7[c], c7p = d7[c + 1], c8c = d8[c], c8j = d8[c + sj];
     const float dudxx1 = diu.a[1][c];
     const float dudyx1 = (d2[c - 1] + d2[c - 1 + sj] + d2[c] + d2[c + sj]) * 0.25f;
-    const float dudzx1 = (d3[c - 1] + d3[c - 1 + sk] + d3[c] + d3[c + sk]) * 0.25f;
+    const float dudzx1 = (l3m + h3m + l3c + h3c) * 0.25f;
     const float dvdxx1 = (d4[c] + d4[c - sj] + d4[c + 1] + d4[c + 1 - sj]) * 0.25f;
     const float dvdyx1 = diu.a[5][c];
-    const float dvdzx1 = (d6[c - sj] + d6[c - sj + sk] + d6[c] + d6[c + sk]) * 0.25f;
-    const float dwdxx1 = (d7[c] + d7[c - sk] + d7[c + 1] + d7[c + 1 - sk]) * 0.25f;
-    const float dwdyx1 = (d8[c] + d8[c - sk] + d8[c + sj] + d8[c + sj - sk]) * 0.25f;
+    const float dvdzx1 = (l6m + h6m + l6c + h6c) * 0.25f;
+    const float dwdxx1 = (c7c + p7c + c7p + p7p) * 0.25f;
+    const float dwdyx1 = (c8c + p8c + c8j + p8j) * 0.25f;
+    p7c = c7c;
+    p7p = c7p;
+    p8c = c8c;
+    p8j = c8j;
     const float dwdzx1 = diu.a[9][c];
     const float a = 0.1f * delx1[k];
     const float s = 2.0f * (dudxx1 * dudxx1 + dvdyx1 * dvdyx1 + dwdzx1 * dwdzx1) +
@@ -800,6 +816,20 @@ __device__ __forceinline__ void les_sm_body(const VBlk& vb, const Grid& g, const
 __global__ __launch_bounds__(NT) void k_les_sm(Grid g, P10 diu, const float* __restrict__ delx1,
                                                float* __restrict__ sm) {
   les_sm_body(hw_blk<true>(), g, diu, delx1, sm);
+}
+// Grid of k_les_sm: runs of d.sm_kchunk levels per thread, or automatic
+// (about 36 blocks per compute unit, as velfg).
+static dim3 les_sm_grid(const DevDomain& d) {
+  const Grid& g = d.g;
+  const int gx = (g.im + BX - 1) / BX, gy = (g.jm + BY - 1) / BY;
+  int kc = d.sm_kchunk;
+  if (kc <= 0) {
+    int nz = 36 * (d.ncu > 0 ? d.ncu : 256) / (gx * gy);
+    nz = std::max(1, std::min(nz, g.km));
+    kc = (g.km + nz - 1) / nz;
+  }
+  kc = std::min(kc, g.km);
+  return grid2d(g.im, g.jm, (g.km + kc - 1) / kc);
 }
 
 // boundsm.f95:18-64 as one pass: the serial build runs the i-Neumann
@@ -992,7 +1022,7 @@ static int boundsm_blocks(const Grid& g) {
 void launch_les(const DevDomain& d) {
   const Grid& g = d.g;
   const P10 diu = p10(d.F.diu);
-  TIMED(d, KC_LES_SM, hipLaunchKernelGGL(k_les_sm, grid2d(g.im, g.jm, kz(g)), block2d(), 0, d.stream, g, diu,
+  TIMED(d, KC_LES_SM, hipLaunchKernelGGL(k_les_sm, les_sm_grid(d), block2d(), 0, d.stream, g, diu,
                      d.m.delx1 + 1, d.F.sm));
   TIMED(d, KC_BOUNDSM, hipLaunchKernelGGL(k_boundsm, dim3((unsigned)boundsm_blocks(g)), dim3(256), 0, d.stream, g,
                                           d.F.sm));
@@ -1004,7 +1034,7 @@ void launch_les(const DevDomain& d) {
 void launch_les_adam(const DevDomain& d) {
   const Grid& g = d.g;
   const P10 diu = p10(d.F.diu);
-  TIMED(d, KC_LES_SM, hipLaunchKernelGGL(k_les_sm, grid2d(g.im, g.jm, kz(g)), block2d(), 0, d.stream, g, diu,
+  TIMED(d, KC_LES_SM, hipLaunchKernelGGL(k_les_sm, les_sm_grid(d), block2d(), 0, d.stream, g, diu,
                      d.m.delx1 + 1, d.F.sm));
   TIMED(d, KC_BOUNDSM, hipLaunchKernelGGL(k_boundsm, dim3((unsigned)boundsm_blocks(g)), dim3(256), 0, d.stream, g,
                                           d.F.sm));
@@ -1228,7 +1258,7 @@ void step_batch_spans(const DevDomain& d, int nzb, BatchDom* e, StageSpan* sp) {
   } else {
     sp[BS_VELFG] = span_of(velfg_grid(d, &e->velfg_kc));
   }
-  sp[BS_LES_SM] = span_of(grid2d(g.im, g.jm, kz(g)));
+  sp[BS_LES_SM] = span_of(les_sm_grid(d));
   sp[BS_BOUNDSM] = span_of(dim3((unsigned)boundsm_blocks(g)));
   sp[BS_LES_VISC] = span_of(grid2d(g.im, g.jm, kz(g)));
 }
