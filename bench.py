@@ -355,7 +355,7 @@ def main(argv=None):
     }
     if tuned is not None:
         result["sor_launch"] = {"rows_per_thread": tuned["rows"], "tile_rows": tuned["tile_rows"],
-                                "k_levels_per_block": tuned["kchunk"],
+                                "tile_pairs": tuned["tile_width"], "k_levels_per_block": tuned["kchunk"],
                                 "tuned_us_per_iter": round(tuned["us_per_iter"], 2),
                                 "default_us_per_iter": round(tuned["us_per_iter_default"], 2)}
     dom.close()

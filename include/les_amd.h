@@ -180,8 +180,8 @@ int les_domain_step(les_domain* d, int n, int nmax);
  * step (recaptured when the domain's constants change); results are the
  * same as eager launches.  Off by default; ignored while timing is on. */
 /* Picks the launch shape of the fused SOR iteration (tile rows per thread: 1
- * or 2; rows per tile with the ring: 16 or 32; k levels per block) for this
- * grid shape: times `reps` iterations of each
+ * or 2; tile rows x pairs with the ring: 16 x 64, 32 x 64 or 32 x 32; k
+ * levels per block) for this grid shape: times `reps` iterations of each
  * candidate on the domain's SOR scratch arrays and keeps the fastest.  The
  * choice is remembered per (im, jm, km) for the process; later calls with
  * reps <= 0 reuse it without timing.  Results do not depend on the shape.
@@ -189,8 +189,8 @@ int les_domain_step(les_domain* d, int n, int nmax);
  * Reports the choice and the microseconds per iteration measured, when the
  * shape was timed, for it and for the untuned default.  No reference
  * counterpart: the per-shape autotuner of the serving path. */
-int les_domain_autotune(les_domain* d, int reps, int* rows, int* tile_rows, int* kchunk, double* us_best,
-                        double* us_default);
+int les_domain_autotune(les_domain* d, int reps, int* rows, int* tile_rows, int* tile_width, int* kchunk,
+                        double* us_best, double* us_default);
 /* Microseconds per fused SOR iteration with the domain's current launch
  * shape: `iters` back-to-back iterations (numbered 2.., never stopping) on
  * the domain's SOR scratch arrays between one pair of HIP events on its

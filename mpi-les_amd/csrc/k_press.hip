@@ -606,23 +606,28 @@ __global__ __launch_bounds__(FNT) __attribute__((amdgpu_waves_per_eu(8, 8))) voi
 // scratch plane at the start of the allocation; loads of levels outside
 // 1..km read neighbouring planes of the same allocation and are never used.
 
-// TY: tile rows including the ring (16 or 32); the block has TY/2 waves.
-// nprev: partial-sum slots written by the previous iteration's launch.
-// One block's share of the iteration: bid is the block's index within the
-// launch of this domain.
-template <bool SEP, int TY, bool OM1>
+// TY: tile rows including the ring (16 or 32); W: tile width in pairs
+// including the ring (64, or 32 with 32 rows: a wave then holds two half-wave
+// rows, which fits narrow domains better).  The block has TY * W / 128
+// waves.  nprev: partial-sum slots written by the previous iteration's
+// launch.  One block's share of the iteration: bid is the block's index
+// within the launch of this domain.
+template <bool SEP, int TY, bool OM1, int W>
 __device__ __forceinline__ void sor_iter2_block(Grid g, SplitGrid s, Metrics m, const SplitBufs& sb, Ctrl* ctrl, int iter,
                                                 int nparts, int nprev, float omega, float pjuge, int ntt, int ntj,
                                                 int kchunk, int remap, int bid) {
-  constexpr int GNT = FX * TY / 2, FTYT = TY - 4;
-  __shared__ float ldsB[3][TY][FX];  // previous black iterate of level kr (planes by level mod 3)
-  __shared__ float ldsR[2][TY][FX];  // new red values (planes by level mod 2)
+  static_assert(W == FX || (W == FX / 2 && TY == 32), "tile shapes: 16 or 32 rows x 64 pairs, 32 x 32");
+  constexpr int GNT = W * TY / 2, FTYT = TY - 4, FTXW = W - 4;
+  __shared__ float ldsB[3][TY][W];  // previous black iterate of level kr (planes by level mod 3)
+  __shared__ float ldsR[2][TY][W];  // new red values (planes by level mod 2)
   __shared__ double lds[GNT / kWave];
   __shared__ int skip;
   extern __shared__ float4 lvm4[];  // per level from krs: cn4l, cn4s, cncz
-  const int lane = threadIdx.x;
-  const int w = __builtin_amdgcn_readfirstlane(threadIdx.y);
-  const int tid = w * FX + lane;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.y);  // blockDim.x is one wave (FX)
+  const int tid = w * FX + (int)threadIdx.x;
+  // lane = pair of the tile row; vw = the thread's pair of rows (wave-uniform when W == FX)
+  const int lane = (W == FX) ? (int)threadIdx.x : (int)threadIdx.x % W;
+  const int vw = (W == FX) ? w : w * (FX / W) + (int)threadIdx.x / W;
   if (tid == 0) skip = (ctrl->sor_done != 0);
   __syncthreads();
   if (skip) return;
@@ -650,9 +655,9 @@ __device__ __forceinline__ void sor_iter2_block(Grid g, SplitGrid s, Metrics m, 
   const unsigned ocR = SEP ? 0u : boff(base, sb.cn1[1]) - cplb, ocB = SEP ? 0u : boff(base, sb.cn1[0]) - cplb;
   const int lb = remap ? xcd_contiguous(bid, nparts) : bid;
   const int bt = lb % ntt, bj = (lb / ntt) % ntj, bk = lb / (ntt * ntj);
-  const int t = bt * FTX - 2 + lane;
+  const int t = bt * FTXW - 2 + lane;
   const bool tin = (t >= 0) && (t < nt);
-  const int rA = 2 * w, rB = rA + 1;
+  const int rA = 2 * vw, rB = rA + 1;
   const int jlA = 1 + bj * FTYT - 2 + rA, jlB = jlA + 1;  // tile rows (may lie outside 1..jm)
   const int jA = jwrap(jlA, jm), jB = jwrap(jlB, jm);     // the rows they stand for (jA odd)
   const int k0 = 1 + bk * kchunk, k1 = min(km, k0 + kchunk - 1);
@@ -660,7 +665,7 @@ __device__ __forceinline__ void sor_iter2_block(Grid g, SplitGrid s, Metrics m, 
   const int tq = tin ? t : 0;
   const unsigned voA = 4u * (unsigned)(tq + (jA - 1) * cp), voB = 4u * (unsigned)(tq + (jB - 1) * cp);
   const unsigned trA = 4u * (unsigned)(nt + (jA - 1) * cp), trB = 4u * (unsigned)(nt + (jB - 1) * cp);
-  const bool lane_tile = tin && lane >= 2 && lane <= FX - 3;
+  const bool lane_tile = tin && lane >= 2 && lane <= W - 3;
   const bool tileA = rA >= 2 && rA <= TY - 3 && jlA <= jm, tileB = rB >= 2 && rB <= TY - 3 && jlB <= jm;
   // per position p of the pair (i = 2t+1+p)
   float c2l[2] = {0.f, 0.f}, c2s[2] = {0.f, 0.f}, axp[2] = {0.f, 0.f};
@@ -686,13 +691,13 @@ __device__ __forceinline__ void sor_iter2_block(Grid g, SplitGrid s, Metrics m, 
     last[p] = (2 * t + 1 + p == im);
   }
   const bool first = (t == 0);  // i = 1 (position 0)
-  constexpr int PL = TY * FX;    // floats per LDS plane
-  const int ownA = rA * FX + lane;
+  constexpr int PL = TY * W;     // floats per LDS plane
+  const int ownA = rA * W + lane;
   // the other colour's i-1 neighbour of position p is at own-1+p: bases kept
   // inside the plane (only outer-ring lanes of rows 0 and TY-1 are clamped)
-  const int bsA = min(max(ownA - 1, 0), PL - 3), bsB = min(max(ownA + FX - 1, 0), PL - 3);
-  const int upA = (rA == 0 ? rA : rA - 1) * FX + lane;       // row A-1 (row 0 reads itself, unused)
-  const int dnB = (rB == TY - 1 ? rB : rB + 1) * FX + lane;  // row B+1
+  const int bsA = min(max(ownA - 1, 0), PL - 3), bsB = min(max(ownA + W - 1, 0), PL - 3);
+  const int upA = (rA == 0 ? rA : rA - 1) * W + lane;       // row A-1 (row 0 reads itself, unused)
+  const int dnB = (rB == TY - 1 ? rB : rB + 1) * W + lane;  // row B+1
   const int nsteps = kre - krs + 1;
   for (int x = tid; x < nsteps; x += GNT) {
     const int k = krs + x;
@@ -743,7 +748,7 @@ __device__ __forceinline__ void sor_iter2_block(Grid g, SplitGrid s, Metrics m, 
   float* const LB = &ldsB[0][0][0];
   float* const LR = &ldsR[0][0][0];
   LB[ownA] = bqA[1];  // level krs into plane 0
-  LB[ownA + FX] = bqB[1];
+  LB[ownA + W] = bqB[1];
   float rdA[3] = {0.f, 0.f, 0.f}, rdB[3] = {0.f, 0.f, 0.f};  // new red values, level L at (L - krs) mod 3
   float mx[2] = {0.f, 0.f}, my[2] = {0.f, 0.f}, mz[2] = {0.f, 0.f};  // metrics of the step's level, by parity
   double acc = 0.0;
@@ -759,7 +764,7 @@ __device__ __forceinline__ void sor_iter2_block(Grid g, SplitGrid s, Metrics m, 
       const int kb = kr - 1;
       // 1. previous black iterate of level kr+1 into its plane
       LB[((U + 1) % 3) * PL + ownA] = bqA[(U + 2) % 6];
-      LB[((U + 1) % 3) * PL + ownA + FX] = bqB[(U + 2) % 6];
+      LB[((U + 1) % 3) * PL + ownA + W] = bqB[(U + 2) % 6];
       {
         const float4 v = lvm4[kr - krs];
         mx[P] = v.x;
@@ -811,7 +816,7 @@ __device__ __forceinline__ void sor_iter2_block(Grid g, SplitGrid s, Metrics m, 
         acc = acc_sq(acc, reltmp, okB[pb] && kinR);
       }
       LR[P * PL + ownA] = redA;
-      LR[P * PL + ownA + FX] = redB;
+      LR[P * PL + ownA + W] = redB;
       rdA[U % 3] = redA;
       rdB[U % 3] = redB;
       // 3. black, level kb = kr-1 (press.f95:93-104), from the red values of
@@ -887,12 +892,12 @@ __device__ __forceinline__ void sor_iter2_block(Grid g, SplitGrid s, Metrics m, 
   }
 }
 
-template <bool SEP, int TY, bool OM1>
-__global__ __launch_bounds__(FX * TY / 2) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_sor_iter2(
+template <bool SEP, int TY, bool OM1, int W>
+__global__ __launch_bounds__(W * TY / 2) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_sor_iter2(
     Grid g, SplitGrid s, Metrics m, SplitBufs sb, Ctrl* ctrl, int iter, int nparts, int nprev, float omega,
     float pjuge, int ntt, int ntj, int kchunk, int remap) {
-  sor_iter2_block<SEP, TY, OM1>(g, s, m, sb, ctrl, iter, nparts, nprev, omega, pjuge, ntt, ntj, kchunk, remap,
-                                (int)blockIdx.x);
+  sor_iter2_block<SEP, TY, OM1, W>(g, s, m, sb, ctrl, iter, nparts, nprev, omega, pjuge, ntt, ntj, kchunk, remap,
+                                   (int)blockIdx.x);
 }
 
 // The same iteration for several domains in one launch: each domain owns a
@@ -900,8 +905,8 @@ __global__ __launch_bounds__(FX * TY / 2) __attribute__((amdgpu_waves_per_eu(4, 
 // as in a launch of its own), domains past their own nmaxp return at once.
 // Only k_sor_iter2<true, TY, OM1> shapes are batched, with one TY for the
 // batch (OM1: every domain has omega == 1).
-template <int TY, bool OM1>
-__global__ __launch_bounds__(FX * TY / 2) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_sor_iter2_batch(
+template <int TY, bool OM1, int W>
+__global__ __launch_bounds__(W * TY / 2) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_sor_iter2_batch(
     const SorBatchEntry* __restrict__ tab, const int* __restrict__ block0, int ntab, int iter, int remap) {
   int e = 0;
   for (int q = 1; q < ntab; q++)  // block0[q] == tab[q].block0, packed
@@ -909,27 +914,28 @@ __global__ __launch_bounds__(FX * TY / 2) __attribute__((amdgpu_waves_per_eu(4, 
   const SorBatchEntry& E = tab[e];
   const int bid = (int)blockIdx.x - E.block0;
   if (bid >= E.nparts || iter > E.nmaxp) return;  // padding block, or this domain has finished its iterations
-  sor_iter2_block<true, TY, OM1>(E.g, E.s, E.m, E.sb, E.ctrl, iter, E.nparts, iter == 2 ? E.nparts1 : E.nparts,
-                                 E.omega, E.pjuge, E.ntt, E.ntj, E.kchunk, remap, bid);
+  sor_iter2_block<true, TY, OM1, W>(E.g, E.s, E.m, E.sb, E.ctrl, iter, E.nparts, iter == 2 ? E.nparts1 : E.nparts,
+                                    E.omega, E.pjuge, E.ntt, E.ntj, E.kchunk, remap, bid);
 }
 
 void launch_sor_batch(const SorBatchEntry* tab, const int* block0, int ntab, int nblocks, int iter, int kcmax,
-                      int ty, bool om1, int remap, hipStream_t stream) {
+                      int ty, int tw, bool om1, int remap, hipStream_t stream) {
   const size_t lds = (size_t)(kcmax + 2) * sizeof(float4);
-#define SORB(TY, OM1)                                                                                          \
-  hipLaunchKernelGGL((k_sor_iter2_batch<TY, OM1>), dim3(nblocks), dim3(FX, TY / 2), lds, stream, tab, block0, ntab, \
-                     iter, remap)
-  if (ty == 32) {
-    if (om1)
-      SORB(32, true);
-    else
-      SORB(32, false);
-  } else {
-    if (om1)
-      SORB(FY, true);
-    else
-      SORB(FY, false);
-  }
+#define SORB(TY, W)                                                                                            \
+  do {                                                                                                         \
+    if (om1)                                                                                                   \
+      hipLaunchKernelGGL((k_sor_iter2_batch<TY, true, W>), dim3(nblocks), dim3(FX, TY * W / (2 * FX)), lds,    \
+                         stream, tab, block0, ntab, iter, remap);                                              \
+    else                                                                                                       \
+      hipLaunchKernelGGL((k_sor_iter2_batch<TY, false, W>), dim3(nblocks), dim3(FX, TY * W / (2 * FX)), lds,   \
+                         stream, tab, block0, ntab, iter, remap);                                              \
+  } while (0)
+  if (ty == 32 && tw == FX / 2)
+    SORB(32, FX / 2);
+  else if (ty == 32)
+    SORB(32, FX);
+  else
+    SORB(FY, FX);
 #undef SORB
 }
 
@@ -1193,15 +1199,16 @@ __global__ __launch_bounds__(NT) void k_dc_padj(Grid g, const double* __restrict
 // Launch shape of the fused SOR iteration: kernel (rows per thread), tile
 // rows and k levels per block; every shape gives the same iterates.
 struct SorShape {
-  int rows, ty, kc, ntt, ntj, nparts;
+  int rows, ty, tw, kc, ntt, ntj, nparts;  // tw: tile width in pairs with the ring (64, or 32 with 32 rows)
 };
 
-static SorShape sor_shape(const Grid& g, const SplitGrid& s, int rows, int ty, int kc) {
+static SorShape sor_shape(const Grid& g, const SplitGrid& s, int rows, int ty, int kc, int tw = FX) {
   SorShape h;
   h.rows = ((g.jm & 1) != 0) ? 1 : rows;  // odd jm: one row per thread
   h.ty = (h.rows == 1) ? FY : ty;          // k_sor_iter has 16-row tiles
+  h.tw = (h.rows == 2 && h.ty == 32 && tw == FX / 2) ? FX / 2 : FX;
   h.kc = std::max(1, std::min(kc, g.km));
-  h.ntt = (s.nt + FTX - 1) / FTX;
+  h.ntt = (s.nt + (h.tw - 4) - 1) / (h.tw - 4);
   h.ntj = (g.jm + (h.ty - 4) - 1) / (h.ty - 4);
   h.nparts = h.ntt * h.ntj * ((g.km + h.kc - 1) / h.kc);
   return h;
@@ -1210,7 +1217,7 @@ static SorShape sor_shape(const Grid& g, const SplitGrid& s, int rows, int ty, i
 // The domain's shape: its settings, or the automatic k chunk.
 static SorShape sor_shape(const DevDomain& d) {
   const int kc = d.sor_kchunk > 0 ? d.sor_kchunk : sor_fused_kchunk(d.g, d.sg, d.sor_fused_blocks, d.ncu);
-  return sor_shape(d.g, d.sg, d.sor_rows, d.sor_tile_rows, kc);
+  return sor_shape(d.g, d.sg, d.sor_rows, d.sor_tile_rows, kc, d.sor_tile_width);
 }
 
 // One fused SOR iteration l with shape h; nprev is the previous launch's
@@ -1241,27 +1248,33 @@ static void launch_sor_iter(const DevDomain& d, const SplitBufs& sb, int l, floa
     return;
   }
   const size_t lvm_bytes = (size_t)(h.kc + 2) * sizeof(float4);
-#define SOR2(SEP, TY)                                                                                        \
+#define SOR2(SEP, TY, W)                                                                                     \
   do {                                                                                                       \
     if (d.omega == 1.0f)                                                                                     \
-      TIMED(d, KC_SOR, hipLaunchKernelGGL((k_sor_iter2<SEP, TY, true>), dim3(h.nparts), dim3(FX, TY / 2),      \
-                                          lvm_bytes, d.stream, g, s, d.m, sb, d.ctrl, l, h.nparts, nprev,      \
-                                          d.omega, pjuge, h.ntt, h.ntj, h.kc, d.xcd_remap));                   \
+      TIMED(d, KC_SOR, hipLaunchKernelGGL((k_sor_iter2<SEP, TY, true, W>), dim3(h.nparts),                   \
+                                          dim3(FX, TY * W / (2 * FX)), lvm_bytes, d.stream, g, s, d.m, sb,   \
+                                          d.ctrl, l, h.nparts, nprev, d.omega, pjuge, h.ntt, h.ntj, h.kc,    \
+                                          d.xcd_remap));                                                     \
     else                                                                                                     \
-      TIMED(d, KC_SOR, hipLaunchKernelGGL((k_sor_iter2<SEP, TY, false>), dim3(h.nparts), dim3(FX, TY / 2),     \
-                                          lvm_bytes, d.stream, g, s, d.m, sb, d.ctrl, l, h.nparts, nprev,      \
-                                          d.omega, pjuge, h.ntt, h.ntj, h.kc, d.xcd_remap));                   \
+      TIMED(d, KC_SOR, hipLaunchKernelGGL((k_sor_iter2<SEP, TY, false, W>), dim3(h.nparts),                  \
+                                          dim3(FX, TY * W / (2 * FX)), lvm_bytes, d.stream, g, s, d.m, sb,   \
+                                          d.ctrl, l, h.nparts, nprev, d.omega, pjuge, h.ntt, h.ntj, h.kc,    \
+                                          d.xcd_remap));                                                     \
   } while (0)
   if (d.cn1_separable) {
-    if (h.ty == 32)
-      SOR2(true, 32);
+    if (h.tw == FX / 2)
+      SOR2(true, 32, FX / 2);
+    else if (h.ty == 32)
+      SOR2(true, 32, FX);
     else
-      SOR2(true, 16);
+      SOR2(true, 16, FX);
   } else {
-    if (h.ty == 32)
-      SOR2(false, 32);
+    if (h.tw == FX / 2)
+      SOR2(false, 32, FX / 2);
+    else if (h.ty == 32)
+      SOR2(false, 32, FX);
     else
-      SOR2(false, 16);
+      SOR2(false, 16, FX);
   }
 #undef SOR2
 }
@@ -1324,9 +1337,9 @@ bool sor_batchable(const DevDomain& d) {
 
 // Shapes of a batched domain's SOR: iterations 2.. (h: tile rows ty, k
 // chunk kc; kc 0: the domain's own) and iteration 1 (h1, k_sor_iter).
-static void batch_shapes(const DevDomain& d, int kc, int ty, SorShape* h, SorShape* h1) {
+static void batch_shapes(const DevDomain& d, int kc, int ty, int tw, SorShape* h, SorShape* h1) {
   if (kc <= 0) kc = d.sor_kchunk > 0 ? d.sor_kchunk : sor_fused_kchunk(d.g, d.sg, d.sor_fused_blocks, d.ncu);
-  *h = sor_shape(d.g, d.sg, 2, ty == 32 ? 32 : FY, kc);
+  *h = sor_shape(d.g, d.sg, 2, ty == 32 ? 32 : FY, kc, tw);
   *h1 = sor_shape(d.g, d.sg, 1, FY, h->kc);
 }
 
@@ -1336,16 +1349,16 @@ static void batch_shapes(const DevDomain& d, int kc, int ty, SorShape* h, SorSha
 // alone picks its chunk to fill the device by itself (a few levels for a
 // small domain); in a batch the other domains fill it, and longer chunks
 // waste fewer steps on the chunk ends.
-int sor_batch_kchunk(const DevDomain* const* ds, int n, int ty, int ncu) {
+int sor_batch_kchunk(const DevDomain* const* ds, int n, int ty, int tw, int ncu) {
   int kmax = 1;
   for (int q = 0; q < n; q++) kmax = std::max(kmax, ds[q]->g.km);
-  const long slots = (ty == 32 ? 1L : 2L) * (ncu > 0 ? ncu : 256);
+  const long slots = (ty == 32 && tw == FX ? 1L : 2L) * (ncu > 0 ? ncu : 256);
   long best = -1;
   int best_kc = kmax;
   for (int kc = 1; kc <= kmax; kc++) {
     long blocks = 0, steps = 0;
     for (int q = 0; q < n; q++) {
-      const SorShape h = sor_shape(ds[q]->g, ds[q]->sg, 2, ty == 32 ? 32 : FY, kc);
+      const SorShape h = sor_shape(ds[q]->g, ds[q]->sg, 2, ty == 32 ? 32 : FY, kc, tw);
       blocks += h.nparts;
       steps = std::max(steps, (long)h.kc + 2);
     }
@@ -1358,9 +1371,9 @@ int sor_batch_kchunk(const DevDomain* const* ds, int n, int ty, int ncu) {
   return best_kc;
 }
 
-void sor_batch_entry(const DevDomain& d, int kc, int ty, SorBatchEntry* e) {
+void sor_batch_entry(const DevDomain& d, int kc, int ty, int tw, SorBatchEntry* e) {
   SorShape h, h1;
-  batch_shapes(d, kc, ty, &h, &h1);
+  batch_shapes(d, kc, ty, tw, &h, &h1);
   e->g = d.g;
   e->s = d.sg;
   e->m = d.m;
@@ -1378,7 +1391,7 @@ void sor_batch_entry(const DevDomain& d, int kc, int ty, SorBatchEntry* e) {
   e->pjuge = d.pjuge;
 }
 
-void launch_press_part(const DevDomain& d, float dt, int part, int kc, int ty) {
+void launch_press_part(const DevDomain& d, float dt, int part, int kc, int ty, int tw) {
   const Grid& g = d.g;
   const int gx = (g.im + BX - 1) / BX, gy = (g.jm + BY - 1) / BY;
   const int gz = kblocks(gx, gy, g.km, kMaxRedBlocks);
@@ -1386,7 +1399,7 @@ void launch_press_part(const DevDomain& d, float dt, int part, int kc, int ty) {
   SplitBufs sb = d.sb;
   sb.cn1[0] = sb.cn1[1] = nullptr;  // separable cn1 (sor_batchable)
   SorShape h, h1;
-  batch_shapes(d, kc, ty, &h, &h1);
+  batch_shapes(d, kc, ty, tw, &h, &h1);
   if (part == 0) {
     press_head(d, dt, sb, gx, gy, gz, nred);
     launch_sor_iter(d, sb, 1, d.pjuge, h1, h1.nparts, d.F.p);
@@ -1446,7 +1459,7 @@ __global__ __launch_bounds__(NT) void kb_padj(const BatchDom* __restrict__ bd, c
 }
 #undef BATCH_BLOCK
 
-void press_batch_spans(const DevDomain& d, int kc, int ty, int nzb, BatchDom* e, StageSpan* sp) {
+void press_batch_spans(const DevDomain& d, int kc, int ty, int tw, int nzb, BatchDom* e, StageSpan* sp) {
   const Grid& g = d.g;
   const int gx = (g.im + BX - 1) / BX, gy = (g.jm + BY - 1) / BY;
   const int gz = kblocks(gx, gy, g.km, kMaxRedBlocks);
@@ -1454,7 +1467,7 @@ void press_batch_spans(const DevDomain& d, int kc, int ty, int nzb, BatchDom* e,
   e->sb = d.sb;
   e->sb.cn1[0] = e->sb.cn1[1] = nullptr;  // separable cn1 (sor_batchable)
   SorShape h, h1;
-  batch_shapes(d, kc, ty, &h, &h1);
+  batch_shapes(d, kc, ty, tw, &h, &h1);
   e->ntt1 = h1.ntt;
   e->ntj1 = h1.ntj;
   e->kc1 = h1.kc;
@@ -1593,12 +1606,12 @@ int sor_fused_tiles(const Grid& g, const SplitGrid& s) {
 // rows per thread, tile rows and k chunk, on whatever the split arrays hold:
 // a timing trial for the autotuner.  Writes only the split iterates, the SOR
 // partial slots and the SOR fields of the control block.
-void launch_sor_trial(const DevDomain& d0, int rows, int ty, int kc, int iters) {
+void launch_sor_trial(const DevDomain& d0, int rows, int ty, int tw, int kc, int iters) {
   DevDomain d = d0;
   d.timer = nullptr;
   SplitBufs sb = d.sb;
   if (d.cn1_separable) sb.cn1[0] = sb.cn1[1] = nullptr;
-  const SorShape h = sor_shape(d.g, d.sg, rows, ty, kc);
+  const SorShape h = sor_shape(d.g, d.sg, rows, ty, kc, tw);
   for (int l = 2; l < 2 + iters; l++) launch_sor_iter(d, sb, l, 0.0f, h, h.nparts);
 }
 

@@ -77,6 +77,7 @@ struct DevDomain {
   int sor_rows;      // tile rows per thread of the fused SOR iteration (1: k_sor_iter, 2: k_sor_iter2)
   int sor_kchunk;    // k levels per block of the fused SOR iteration (0: from sor_fused_blocks)
   int sor_tile_rows; // tile rows (with the ring) of k_sor_iter2: 16 or 32
+  int sor_tile_width;  // tile width in pairs (with the ring) of k_sor_iter2: 64, or 32 with 32 rows
 };
 
 int sor_kchunk(const Grid& g, const SplitGrid& s, int target);
@@ -100,20 +101,22 @@ struct SorBatchEntry {
   float omega, pjuge;
 };
 // block0: the entries' block0 values, packed (the block-to-domain lookup);
-// ty: tile rows of the batch (16 or 32); om1: every entry has omega == 1.
+// ty, tw: tile rows and width of the batch (16 x 64, 32 x 64 or 32 x 32);
+// om1: every entry has omega == 1.
 void launch_sor_batch(const SorBatchEntry* tab, const int* block0, int ntab, int nblocks, int iter, int kcmax,
-                      int ty, bool om1, int remap, hipStream_t stream);
+                      int ty, int tw, bool om1, int remap, hipStream_t stream);
 // The press stage in two parts around iterations 2..nmaxp of the SOR, for
 // domains whose iterations run batched: part 0 is the right-hand side and
 // iteration 1, part 1 the stop test of the last iteration and the mean
-// removal.  kc, ty: the SOR k chunk (0: the domain's own) and tile rows of
-// the batch.
-void launch_press_part(const DevDomain& d, float dt, int part, int kc, int ty);
-// The domain's entry in a batched SOR launch with k chunk kc and tile rows ty
-// (block0 left to the caller).
-void sor_batch_entry(const DevDomain& d, int kc, int ty, SorBatchEntry* e);
-// The k chunk of a batch of domains' SOR launches with tile rows ty.
-int sor_batch_kchunk(const DevDomain* const* ds, int n, int ty, int ncu);
+// removal.  kc, ty, tw: the SOR k chunk (0: the domain's own), tile rows
+// and tile width of the batch.
+void launch_press_part(const DevDomain& d, float dt, int part, int kc, int ty, int tw);
+// The domain's entry in a batched SOR launch with k chunk kc, tile rows ty
+// and tile width tw (block0 left to the caller).
+void sor_batch_entry(const DevDomain& d, int kc, int ty, int tw, SorBatchEntry* e);
+// The k chunk of a batch of domains' SOR launches with tile rows ty and
+// tile width tw.
+int sor_batch_kchunk(const DevDomain* const* ds, int n, int ty, int tw, int ncu);
 // Whether a domain's SOR iterations can run in a batch (fused reference SOR,
 // separable cn1, even jm).
 bool sor_batchable(const DevDomain& d);
@@ -179,13 +182,13 @@ __device__ __forceinline__ bool batch_block(const StageSpan* __restrict__ sp, in
 // copies into and out of the split arrays (longer k runs per block; no
 // value depends on the split); kc: the SOR k chunk of the batch.
 void step_batch_spans(const DevDomain& d, int nzb, BatchDom* e, StageSpan* sp);
-void press_batch_spans(const DevDomain& d, int kc, int ty, int nzb, BatchDom* e, StageSpan* sp);
+void press_batch_spans(const DevDomain& d, int kc, int ty, int tw, int nzb, BatchDom* e, StageSpan* sp);
 // velnw .. les+adam of every domain, one launch per stage.
 void launch_step_batch(const StageBatch& b, hipStream_t stream);
 // part 0: rhs, its mean, the split arrays and SOR iteration 1; part 1 (after
 // iterations 2..nmaxp): the closing stop-test sum, pav and the adjusted p.
 void launch_press_batch(const StageBatch& b, int part, hipStream_t stream);
-void launch_sor_trial(const DevDomain& d, int rows, int ty, int kc, int iters);
+void launch_sor_trial(const DevDomain& d, int rows, int ty, int tw, int kc, int iters);
 int reduction_parts_needed(const Grid& g);
 
 // Event pairs around launches, recorded only when d.timer is set.

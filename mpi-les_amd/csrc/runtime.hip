@@ -552,6 +552,7 @@ int domain_alloc(int im, int jm, int km, les_domain** out) {
   D->d.sor_rows = env_int("LES_SOR_ROWS", 2) == 1 ? 1 : 2;
   D->d.sor_kchunk = std::max(0, env_int("LES_SOR_KCHUNK", 0));
   D->d.sor_tile_rows = env_int("LES_SOR_TILE_ROWS", 16) == 32 ? 32 : 16;
+  D->d.sor_tile_width = (D->d.sor_tile_rows == 32 && env_int("LES_SOR_TILE_WIDTH", 64) == 32) ? 32 : 64;
   {
     int dev = 0, ncu = 0;
     if (hipGetDevice(&dev) == hipSuccess &&
@@ -653,7 +654,7 @@ int run_step(les_domain* D, int n, int nmax) {
 // domain's SOR cannot be batched, else its first part (the right-hand side
 // and SOR iteration 1, with the batch's SOR k chunk kc and tile rows ty).
 // Returns whether the press was split.
-int run_step_head(les_domain* D, int n, int nmax, int kc, int ty, bool* split) {
+int run_step_head(les_domain* D, int n, int nmax, int kc, int ty, int tw, bool* split) {
   for (int s = 0; s < 6; s++) {
     if (s == 2 && D->d.fuse_velfg) {
       if (flush_metrics(D)) return 1;
@@ -673,7 +674,7 @@ int run_step_head(les_domain* D, int n, int nmax, int kc, int ty, bool* split) {
   }
   *split = sor_batchable(D->d);
   if (!*split) return run_stage(D, 6, n, nmax);
-  launch_press_part(D->d, D->dt, 0, kc, ty);
+  launch_press_part(D->d, D->dt, 0, kc, ty, tw);
   HIP_TRY(hipGetLastError());
   return 0;
 }
@@ -732,7 +733,7 @@ struct TuneKey {
   }
 };
 struct TuneResult {
-  int rows = 2, tile_rows = 16, kchunk = 0;
+  int rows = 2, tile_rows = 16, tile_width = 64, kchunk = 0;
   double us_best = 0.0, us_default = 0.0;
 };
 std::mutex g_tune_mu;
@@ -742,18 +743,18 @@ std::map<TuneKey, TuneResult> g_tuned;
 // (numbered 2.., never stopping) with the given shape, between one pair of
 // events on the domain's stream, after two untimed ones.  The caller saves
 // and restores the control block.
-int time_sor_trial(les_domain* D, const Ctrl& base, int rows, int ty, int kc, int reps, hipEvent_t e0,
+int time_sor_trial(les_domain* D, const Ctrl& base, int rows, int ty, int tw, int kc, int reps, hipEvent_t e0,
                    hipEvent_t e1, double* us) {
   Ctrl c = base;
   c.sor_done = 0;
   c.nonfinite = 0;
   hipError_t e = hipMemcpyAsync(D->d.ctrl, &c, sizeof(Ctrl), hipMemcpyHostToDevice, D->d.stream);
   if (e == hipSuccess) {
-    launch_sor_trial(D->d, rows, ty, kc, 2);  // warm-up
+    launch_sor_trial(D->d, rows, ty, tw, kc, 2);  // warm-up
     e = hipEventRecord(e0, D->d.stream);
   }
   if (e == hipSuccess) {
-    launch_sor_trial(D->d, rows, ty, kc, reps);
+    launch_sor_trial(D->d, rows, ty, tw, kc, reps);
     e = hipEventRecord(e1, D->d.stream);
   }
   if (e == hipSuccess) e = hipEventSynchronize(e1);
@@ -782,13 +783,15 @@ int tune_sor(les_domain* D, int reps, TuneResult* out) {
     if (blocks * 4 < ncu || blocks > 8L * ncu) continue;
     if (std::find(kcs.begin(), kcs.end(), kc) == kcs.end()) kcs.push_back(kc);
   }
-  // (rows per thread, tile rows): k_sor_iter2 with 16- or 32-row tiles
-  // (even jm only), k_sor_iter
-  std::vector<std::pair<int, int>> kernels{{1, 16}};
-  if ((g.jm & 1) == 0) kernels.insert(kernels.begin(), {{2, 16}, {2, 32}});
+  // (rows per thread, tile rows, tile width): k_sor_iter2 with 16 x 64,
+  // 32 x 64 or 32 x 32 tiles (even jm only), k_sor_iter
+  std::vector<std::tuple<int, int, int>> kernels{{1, 16, 64}};
+  if ((g.jm & 1) == 0) kernels.insert(kernels.begin(), {{2, 16, 64}, {2, 32, 64}, {2, 32, 32}});
   // the shape used without tuning
-  const std::pair<int, int> kdef2 = ((g.jm & 1) == 0 && D->d.sor_rows == 2) ? std::make_pair(2, D->d.sor_tile_rows)
-                                                                            : std::make_pair(1, 16);
+  const std::tuple<int, int, int> kdef2 =
+      ((g.jm & 1) == 0 && D->d.sor_rows == 2)
+          ? std::make_tuple(2, D->d.sor_tile_rows, D->d.sor_tile_rows == 32 ? D->d.sor_tile_width : 64)
+          : std::make_tuple(1, 16, 64);
   hipEvent_t e0 = nullptr, e1 = nullptr;
   HIP_TRY(hipEventCreate(&e0));
   if (hipEventCreate(&e1) != hipSuccess) {
@@ -800,17 +803,19 @@ int tune_sor(les_domain* D, int reps, TuneResult* out) {
   TuneResult r;
   r.rows = D->d.sor_rows;
   r.tile_rows = D->d.sor_tile_rows;
+  r.tile_width = D->d.sor_tile_width;
   r.kchunk = 0;
   for (const auto& kt : kernels) {
-    const int rows = kt.first, ty = kt.second;
+    const int rows = std::get<0>(kt), ty = std::get<1>(kt), tw = std::get<2>(kt);
     for (int kc : kcs) {
       double us = 0.0;
-      if ((rc = time_sor_trial(D, saved, rows, ty, kc, reps, e0, e1, &us)) != 0) break;
+      if ((rc = time_sor_trial(D, saved, rows, ty, tw, kc, reps, e0, e1, &us)) != 0) break;
       if (kt == kdef2 && kc == kdef) r.us_default = us;
       if (best < 0.0 || us < best) {
         best = us;
         r.rows = rows;
         r.tile_rows = ty;
+        r.tile_width = tw;
         r.kchunk = kc;
       }
     }
@@ -1339,8 +1344,8 @@ int les_domain_set_inflow(les_domain* D, const float* u_in, const float* v_in, c
   return set_inflow(D, u_in, v_in, w_in);
 }
 
-int les_domain_autotune(les_domain* D, int reps, int* rows, int* tile_rows, int* kchunk, double* us_best,
-                        double* us_default) {
+int les_domain_autotune(les_domain* D, int reps, int* rows, int* tile_rows, int* tile_width, int* kchunk,
+                        double* us_best, double* us_default) {
   if (!D) return fail("null domain");
   std::lock_guard<std::mutex> lk(g_tune_mu);
   const Grid& g = D->d.g;
@@ -1354,9 +1359,11 @@ int les_domain_autotune(les_domain* D, int reps, int* rows, int* tile_rows, int*
   const TuneResult& r = it->second;
   D->d.sor_rows = r.rows;
   D->d.sor_tile_rows = r.tile_rows;
+  D->d.sor_tile_width = r.tile_width;
   D->d.sor_kchunk = r.kchunk;
   if (rows) *rows = r.rows;
   if (tile_rows) *tile_rows = r.rows == 2 ? r.tile_rows : 16;
+  if (tile_width) *tile_width = r.rows == 2 ? r.tile_width : 64;
   if (kchunk) *kchunk = r.kchunk;
   if (us_best) *us_best = r.us_best;
   if (us_default) *us_default = r.us_default;
@@ -1378,7 +1385,8 @@ int les_domain_time_sor(les_domain* D, int iters, double* us_per_iter) {
     hipEventDestroy(e0);
     return fail("hipEventCreate");
   }
-  int rc = time_sor_trial(D, saved, D->d.sor_rows, D->d.sor_tile_rows, kc, iters, e0, e1, us_per_iter);
+  int rc = time_sor_trial(D, saved, D->d.sor_rows, D->d.sor_tile_rows, D->d.sor_tile_width, kc, iters, e0, e1,
+                          us_per_iter);
   hipEventDestroy(e0);
   hipEventDestroy(e1);
   if (hipMemcpy(D->d.ctrl, &saved, sizeof(Ctrl), hipMemcpyHostToDevice) != hipSuccess && !rc)
@@ -1486,7 +1494,7 @@ struct les_batch {
   SorBatchEntry* dtab = nullptr;  // device copy of the SOR table, then its block0 values
   const int* dblock0 = nullptr;
   int ntab = 0, nblocks = 0, kcmax = 0, lmax = 0, remap = 1;
-  int sor_kc = 0, sor_ty = 16;    // SOR k chunk and tile rows of the batched domains
+  int sor_kc = 0, sor_ty = 16, sor_tw = 64;  // SOR k chunk, tile rows and width of the batched domains
   bool om1 = false;               // every batched domain has omega == 1
   // Domains whose whole step runs as batched stage launches (fused velfg and
   // les, batchable SOR); the others step on their own streams.
@@ -1555,14 +1563,15 @@ int batch_prepare(les_batch* B) {
   // one SOR k chunk for the batch (LES_BATCH_SOR_KCHUNK overrides), and k
   // splits of the copy-like stages giving about 36 blocks per CU in all
   B->sor_ty = env_int("LES_BATCH_SOR_TILE_ROWS", 16) == 32 ? 32 : 16;
+  B->sor_tw = (B->sor_ty == 32 && env_int("LES_BATCH_SOR_TILE_WIDTH", 64) == 32) ? 32 : 64;
   B->sor_kc = fd.empty() ? 0 : env_int("LES_BATCH_SOR_KCHUNK", 0);
-  if (!fd.empty() && B->sor_kc <= 0) B->sor_kc = sor_batch_kchunk(fd.data(), (int)fd.size(), B->sor_ty, ncu);
+  if (!fd.empty() && B->sor_kc <= 0) B->sor_kc = sor_batch_kchunk(fd.data(), (int)fd.size(), B->sor_ty, B->sor_tw, ncu);
   const int nzb = tiles > 0 ? (int)std::max(1L, 36L * ncu / tiles) : 0;
   for (size_t q = 0; q < B->doms.size(); q++) {
     if (!B->full[q]) continue;
     les_domain* D = B->doms[q];
     SorBatchEntry e;
-    sor_batch_entry(D->d, B->sor_kc, B->sor_ty, &e);
+    sor_batch_entry(D->d, B->sor_kc, B->sor_ty, B->sor_tw, &e);
     e.block0 = block;
     block += (e.nparts + 7) / 8 * 8;
     B->kcmax = std::max(B->kcmax, e.kchunk);
@@ -1578,7 +1587,7 @@ int batch_prepare(les_batch* B) {
     bd.alpha = D->alpha;
     bd.beta = D->beta;
     step_batch_spans(D->d, nzb, &bd, sp.data());
-    press_batch_spans(D->d, B->sor_kc, B->sor_ty, nzb, &bd, sp.data());
+    press_batch_spans(D->d, B->sor_kc, B->sor_ty, B->sor_tw, nzb, &bd, sp.data());
     bdom.push_back(bd);
     spans.push_back(sp);
   }
@@ -1663,7 +1672,7 @@ int batch_enqueue(les_batch* B, const int* ns, const int* nmaxs, bool capturing)
     HIP_TRY(hipStreamWaitEvent(D->d.stream, B->fork, 0));
     if (B->full[q]) {
       bool sp = false;
-      if (run_step_head(D, ns[q], nmaxs[q], B->sor_kc, B->sor_ty, &sp)) return 1;
+      if (run_step_head(D, ns[q], nmaxs[q], B->sor_kc, B->sor_ty, B->sor_tw, &sp)) return 1;
       split[q] = sp;
     } else if (run_step(D, ns[q], nmaxs[q])) {
       return 1;
@@ -1676,7 +1685,8 @@ int batch_enqueue(les_batch* B, const int* ns, const int* nmaxs, bool capturing)
     launch_press_batch(B->stages, 0, B->stream);
   }
   for (int l = 2; l <= B->lmax && B->ntab > 0; l++)
-    launch_sor_batch(B->dtab, B->dblock0, B->ntab, B->nblocks, l, B->kcmax, B->sor_ty, B->om1, B->remap, B->stream);
+    launch_sor_batch(B->dtab, B->dblock0, B->ntab, B->nblocks, l, B->kcmax, B->sor_ty, B->sor_tw, B->om1, B->remap,
+                     B->stream);
   if (staged) launch_press_batch(B->stages, 1, B->stream);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipEventRecord(B->sor_done, B->stream));
@@ -1688,7 +1698,7 @@ int batch_enqueue(les_batch* B, const int* ns, const int* nmaxs, bool capturing)
     }
     if (split[q]) {
       HIP_TRY(hipStreamWaitEvent(D->d.stream, B->sor_done, 0));
-      launch_press_part(D->d, D->dt, 1, B->sor_kc, B->sor_ty);
+      launch_press_part(D->d, D->dt, 1, B->sor_kc, B->sor_ty, B->sor_tw);
       HIP_TRY(hipGetLastError());
     }
     HIP_TRY(hipEventRecord(B->tail_done[q], D->d.stream));

@@ -101,7 +101,7 @@ def lib():
     L.les_domain_set_timing.argtypes = [vp, ci]
     L.les_domain_set_timing_runs.argtypes = [vp, ci]
     L.les_domain_set_graph.argtypes = [vp, ci]
-    L.les_domain_autotune.argtypes = [vp, ci, vp, vp, vp, vp, vp]
+    L.les_domain_autotune.argtypes = [vp, ci, vp, vp, vp, vp, vp, vp]
     L.les_domain_time_sor.argtypes = [vp, ci, vp]
     L.les_domain_timing.argtypes = [vp, vp, vp]
     L.les_batch_create.argtypes = [vp, ci, ctypes.POINTER(vp)]
@@ -293,11 +293,11 @@ class Domain:
     def autotune(self, reps: int = 0) -> dict:
         """Pick the SOR launch shape for this grid shape (cached per shape;
         reps > 0 times it again with that many iterations per candidate)."""
-        r, ty, kc = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        r, ty, tw, kc = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
         ub, ud = ctypes.c_double(), ctypes.c_double()
-        _check(lib().les_domain_autotune(self._h, reps, ctypes.byref(r), ctypes.byref(ty), ctypes.byref(kc),
-                                         ctypes.byref(ub), ctypes.byref(ud)))
-        return dict(rows=r.value, tile_rows=ty.value, kchunk=kc.value, us_per_iter=ub.value,
+        _check(lib().les_domain_autotune(self._h, reps, ctypes.byref(r), ctypes.byref(ty), ctypes.byref(tw),
+                                         ctypes.byref(kc), ctypes.byref(ub), ctypes.byref(ud)))
+        return dict(rows=r.value, tile_rows=ty.value, tile_width=tw.value, kchunk=kc.value, us_per_iter=ub.value,
                     us_per_iter_default=ud.value)
 
     def time_sor(self, iters: int = 200) -> float:

@@ -28,6 +28,7 @@ def test_autotune_keeps_state_and_results(case):
     r = a.autotune(reps=4)
     assert r["rows"] in ((1, 2) if w.jm % 2 == 0 else (1,)), r
     assert r["tile_rows"] in ((16, 32) if r["rows"] == 2 else (16,)), r
+    assert r["tile_width"] in ((64, 32) if r["tile_rows"] == 32 else (64,)), r
     assert 1 <= r["kchunk"] <= w.km, r
     assert r["us_per_iter"] > 0.0 and r["us_per_iter_default"] > 0.0, r
     assert r["us_per_iter"] <= r["us_per_iter_default"], r
@@ -43,7 +44,8 @@ def test_autotune_keeps_state_and_results(case):
     assert a.press_stats()["sor"].tolist() == st_before["sor"].tolist()
     # remembered for the shape
     rb = b.autotune()
-    assert (rb["kchunk"], rb["rows"], rb["tile_rows"]) == (r["kchunk"], r["rows"], r["tile_rows"])
+    assert (rb["kchunk"], rb["rows"], rb["tile_rows"], rb["tile_width"]) == (r["kchunk"], r["rows"], r["tile_rows"],
+                                                                            r["tile_width"])
     c = api.Domain(s)  # an untuned twin stepping alongside
     for n in (1, 2):
         c.step(n, 4)

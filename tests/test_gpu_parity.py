@@ -164,24 +164,27 @@ FUSED_CASES = [
 ]
 
 
-@pytest.mark.parametrize("rows,tile_rows", [(1, 16), (2, 16), (2, 32)], ids=["r1", "r2t16", "r2t32"])
+@pytest.mark.parametrize("rows,tile_rows,tile_width", [(1, 16, 64), (2, 16, 64), (2, 32, 64), (2, 32, 32)],
+                         ids=["r1", "r2t16", "r2t32", "r2t32w32"])
 @pytest.mark.parametrize("case,blocks", FUSED_CASES,
                          ids=lambda c: f"{c['im']}x{c['jm']}x{c['km']}" if isinstance(c, dict) else f"b{c}")
-def test_fused_sor_iteration_equals_colour_sweeps(case, blocks, rows, tile_rows, monkeypatch):
+def test_fused_sor_iteration_equals_colour_sweeps(case, blocks, rows, tile_rows, tile_width, monkeypatch):
     """One launch per SOR iteration (red on the tile and a ring, then black,
     marching k) gives the iterates of the two colour sweeps bit for bit, for
     any tiling of the domain, and the same iteration counts and residuals;
     with one tile row per thread (k_sor_iter) and with two (k_sor_iter2,
-    even jm, tiles of 16 or 32 rows; odd jm keeps k_sor_iter)."""
+    even jm, tiles of 16 x 64, 32 x 64 or 32 x 32 rows x pairs; odd jm keeps
+    k_sor_iter)."""
     from lesamd import api
     s = _mk(case)
-    print(f"\n[fused sor] {s.workload.name} blocks={blocks} rows={rows} tile_rows={tile_rows}", flush=True)
+    print(f"\n[fused sor] {s.workload.name} blocks={blocks} rows={rows} tile={tile_rows}x{tile_width}", flush=True)
     monkeypatch.setenv("LES_SOR_FUSED", "0")
     two = api.Domain(s)
     monkeypatch.setenv("LES_SOR_FUSED", "1")
     monkeypatch.setenv("LES_SOR_FUSED_BLOCKS", str(blocks))
     monkeypatch.setenv("LES_SOR_ROWS", str(rows))
     monkeypatch.setenv("LES_SOR_TILE_ROWS", str(tile_rows))
+    monkeypatch.setenv("LES_SOR_TILE_WIDTH", str(tile_width))
     one = api.Domain(s)
     for n in range(1, 4):
         two.step(n, 3)

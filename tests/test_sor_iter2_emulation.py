@@ -4,7 +4,9 @@ LDS planes, register rings, store redirection and per-cell FP32 arithmetic,
 statement by statement, checked against a plain red-black iteration with the
 ghost rules of press.f95:89-122 / boundp.f95:7-78 (red from the previous
 black iterate, then black from the new red), bit for bit, for tilings with
-partial tiles, tiny jm and short k chunks.  It also checks that every buffer
+partial tiles, tiny jm and short k chunks, with 16 x 64, 32 x 64 and
+32 x 32 tiles (rows x pairs with the ring; in the last, a wave holds two
+half-wave rows).  It also checks that every buffer
 and LDS index the kernel forms stays inside its allocation, that every
 interior cell is stored exactly once by its owner, and that the redirected
 stores land only in the row padding or the scratch plane.  The GPU tests
@@ -13,8 +15,7 @@ arithmetic on the CPU."""
 import numpy as np
 import pytest
 
-FX = 64
-FTX = FX - 4
+FX = 64  # threads per wave (blockDim.x)
 f32 = np.float32
 
 
@@ -32,12 +33,15 @@ class Err(Exception):
     pass
 
 
-def run(im, jm, km, kc, ty=16, seed=0, omega=1.0, remap=1):
+def run(im, jm, km, kc, ty=16, seed=0, omega=1.0, remap=1, tw=FX):
     assert jm % 2 == 0
+    assert tw == FX or (tw == FX // 2 and ty == 32)
     FY = ty  # tile rows with the ring
     FTY = FY - 4
-    GW = FY // 2
-    PL = FY * FX
+    W = tw  # tile width in pairs with the ring
+    FTXW = W - 4
+    NW = FY * W // (2 * FX)  # waves of the block
+    PL = FY * W
     rng = np.random.default_rng(seed)
     nt = (im + 1) // 2
     cp = (nt + 1 + 31) // 32 * 32
@@ -143,7 +147,7 @@ def run(im, jm, km, kc, ty=16, seed=0, omega=1.0, remap=1):
                     ref_sum += float(r) * float(r)
 
     # ---------------- emulation of the kernel
-    ntt, ntj = (nt + FTX - 1) // FTX, (jm + FTY - 1) // FTY
+    ntt, ntj = (nt + FTXW - 1) // FTXW, (jm + FTY - 1) // FTY
     nz = (km + kc - 1) // kc
     nparts = ntt * ntj * nz
     cplb = 4 * cpl
@@ -170,12 +174,15 @@ def run(im, jm, km, kc, ty=16, seed=0, omega=1.0, remap=1):
             written.setdefault(int(q), []).append((bid, float(v), bool(rm)))
             out[q] = v
 
-    lane = np.arange(FX)[None, :].repeat(GW, 0)
-    wv = np.arange(GW)[:, None].repeat(FX, 1)
+    # thread (wave w, x = threadIdx.x): lane = pair of the tile row, wv = its pair of rows
+    tx = np.arange(FX)[None, :].repeat(NW, 0)
+    ww = np.arange(NW)[:, None].repeat(FX, 1)
+    lane = tx % W
+    wv = ww * (FX // W) + tx // W
     for bid in range(nparts):
         lb = xcd(bid, nparts) if remap else bid
         bt, bj, bk = lb % ntt, (lb // ntt) % ntj, lb // (ntt * ntj)
-        t = bt * FTX - 2 + lane
+        t = bt * FTXW - 2 + lane
         tin = (t >= 0) & (t < nt)
         rA = 2 * wv
         rB = rA + 1
@@ -193,7 +200,7 @@ def run(im, jm, km, kc, ty=16, seed=0, omega=1.0, remap=1):
         voB = 4 * (tq + (jB - 1) * cp)
         trA = 4 * (nt + (jA - 1) * cp)
         trB = 4 * (nt + (jB - 1) * cp)
-        lane_tile = tin & (lane >= 2) & (lane <= FX - 3)
+        lane_tile = tin & (lane >= 2) & (lane <= W - 3)
         tileA = (rA >= 2) & (rA <= FY - 3) & (jlA <= jm)
         tileB = (rB >= 2) & (rB <= FY - 3) & (jlB <= jm)
         c2lp = [np.where(tin, c2l[np.clip(2 * t + 1 + p, -1, im + 2) + 1], 0).astype(f32) for p in (0, 1)]
@@ -209,11 +216,11 @@ def run(im, jm, km, kc, ty=16, seed=0, omega=1.0, remap=1):
         svB = [np.where(okB[p], voB, trB) for p in (0, 1)]
         last = [(2 * t + 1 + p == im) for p in (0, 1)]
         first = (t == 0)
-        ownA = rA * FX + lane
+        ownA = rA * W + lane
         bsA = np.clip(ownA - 1, 0, PL - 3)
-        bsB = np.clip(ownA + FX - 1, 0, PL - 3)
-        upA = np.where(rA == 0, rA, rA - 1) * FX + lane
-        dnB = np.where(rB == FY - 1, rB, rB + 1) * FX + lane
+        bsB = np.clip(ownA + W - 1, 0, PL - 3)
+        upA = np.where(rA == 0, rA, rA - 1) * W + lane
+        dnB = np.where(rB == FY - 1, rB, rB + 1) * W + lane
         nsteps = kre - krs + 1
         lvm = np.zeros((nsteps, 3), f32)
         for x in range(nsteps):
@@ -240,7 +247,7 @@ def run(im, jm, km, kc, ty=16, seed=0, omega=1.0, remap=1):
         LB = np.full(3 * PL, np.nan, f32)
         LR = np.full(2 * PL, np.nan, f32)
         LB[ownA] = bqA[1]
-        LB[ownA + FX] = bqB[1]
+        LB[ownA + W] = bqB[1]
         rdA = [np.zeros_like(voA, dtype=f32)] * 3
         rdB = [np.zeros_like(voA, dtype=f32)] * 3
         mx = [f32(0), f32(0)]
@@ -278,7 +285,7 @@ def run(im, jm, km, kc, ty=16, seed=0, omega=1.0, remap=1):
             pb = pa ^ 1
             kb = kr - 1
             LB[((U + 1) % 3) * PL + ownA] = bqA[(U + 2) % 6]
-            LB[((U + 1) % 3) * PL + ownA + FX] = bqB[(U + 2) % 6]
+            LB[((U + 1) % 3) * PL + ownA + W] = bqB[(U + 2) % 6]
             if not (0 <= kr - krs < nsteps):
                 raise Err("lvm index")
             mx[P], my[P], mz[P] = lvm[kr - krs]
@@ -302,7 +309,7 @@ def run(im, jm, km, kc, ty=16, seed=0, omega=1.0, remap=1):
             store(redB, svB[pb], soR, bid, okB[pb] & kinR)
             acc += np.where(okB[pb] & kinR, rB_.astype(float) ** 2, 0)
             LR[P * PL + ownA] = redA
-            LR[P * PL + ownA + FX] = redB
+            LR[P * PL + ownA + W] = redB
             rdA = list(rdA); rdB = list(rdB)
             rdA[U % 3] = redA
             rdB[U % 3] = redB
@@ -369,9 +376,9 @@ CASES = [(20, 14, 9, 9), (20, 14, 9, 3), (21, 14, 9, 4), (130, 26, 12, 5), (7, 2
          (125, 30, 20, 7), (64, 16, 13, 13), (3, 2, 5, 5), (121, 12, 8, 6), (70, 60, 11, 4)]
 
 
-@pytest.mark.parametrize("ty", [16, 32])
+@pytest.mark.parametrize("ty,tw", [(16, FX), (32, FX), (32, FX // 2)])
 @pytest.mark.parametrize("im,jm,km,kc", CASES)
-def test_sor_iter2_tiling_emulation(im, jm, km, kc, ty):
-    bad, rel, _ = run(im, jm, km, kc, ty)
+def test_sor_iter2_tiling_emulation(im, jm, km, kc, ty, tw):
+    bad, rel, _ = run(im, jm, km, kc, ty, tw=tw)
     assert bad == 0
     assert rel < 1e-12  # the stop-test sum: same terms, another order
