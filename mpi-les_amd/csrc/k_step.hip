@@ -817,12 +817,12 @@ __global__ __launch_bounds__(NT) void k_les_sm(Grid g, P10 diu, const float* __r
                                                float* __restrict__ sm) {
   les_sm_body(hw_blk<true>(), g, diu, delx1, sm);
 }
-// Grid of k_les_sm: runs of d.sm_kchunk levels per thread, or automatic
-// (about 36 blocks per compute unit, as velfg).
-static dim3 les_sm_grid(const DevDomain& d) {
+// Grid of a stencil kernel whose threads walk runs of kchunk levels
+// (0: automatic, about 36 blocks per compute unit, as velfg).
+static dim3 krun_grid(const DevDomain& d, int kchunk) {
   const Grid& g = d.g;
   const int gx = (g.im + BX - 1) / BX, gy = (g.jm + BY - 1) / BY;
-  int kc = d.sm_kchunk;
+  int kc = kchunk;
   if (kc <= 0) {
     int nz = 36 * (d.ncu > 0 ? d.ncu : 256) / (gx * gy);
     nz = std::max(1, std::min(nz, g.km));
@@ -918,26 +918,43 @@ __device__ __forceinline__ void les_visc_body(const VBlk& vb, const Grid& g, con
 #define SMYM(c) ((dyj * sm[(c) - sj] + dym * sm[(c)]) / (dym + dyj))    // y-interp at j-1/2
   int k0, k1;
   kspan(vb, 1, g.km, k0, k1);
-  for (int k = k0; k <= k1; k++) {
-    const int c = ix(g, i, j, k);
-    const float dzk = M1(m.dzn, k), dzp = M1(m.dzn, k + 1), dzm = M1(m.dzn, k - 1);
+  if (k0 > k1) return;
+  // A thread walks its run of levels.  What level k shares with level k-1
+  // is carried in registers, each the same expression on the same operands
+  // as its recomputation would be: the sm interpolations at k (computed as
+  // the k+1 ones of the level below) and the k-1/2 face products visuz1,
+  // visvz1, viswz1 (= visuz2, visvz2, viswz2 of the level below, whose evsz2
+  // is this level's evsz1).
+  int c = ix(g, i, j, k0);
+  float cX0 = SMX(c), cXM0 = SMXM(c), cY0 = SMY(c), cYM0 = SMYM(c);  // SMX, SMXM, SMY, SMYM at level k
+  float cUz1 = 0.f, cVz1 = 0.f;                                      // visuz1, visvz1 (k >= 2)
+  float cWz1 = (sm[c]) * 2.0f * d9[c];                                // viswz1
+  if (k0 >= 2) {
+    const float dzk = M1(m.dzn, k0), dzm = M1(m.dzn, k0 - 1);
+    const float evsz1 = (dzk * SMX(c - sk) + dzm * cX0) / (dzm + dzk);
+    cUz1 = (evsz1) * (d3[c] + d7[c + 1 - sk]);
+    cVz1 = (evsz1) * (d6[c] + d8[c + sj - sk]);
+  }
+  for (int k = k0; k <= k1; k++, c += sk) {
+    const float dzk = M1(m.dzn, k), dzp = M1(m.dzn, k + 1);
+    const float Xp = SMX(c + sk), XMp = SMXM(c + sk), Yp = SMY(c + sk), YMp = SMYM(c + sk);  // level k+1
+    const float evsz2 = (dzp * cX0 + dzk * Xp) / (dzk + dzp);
     float fv, gv, hv;
+    float visuz2, visvz2, viswz2;
     // ---- f (les.f95:83-114; wall les.f95:118-139)
     {
       const float evsx2 = sm[c + 1];
       const float evsx1 = sm[c];
-      const float evsy2 = (dyp * SMX(c) + dyj * SMX(c + sj)) / (dyj + dyp);
-      const float evsy1 = (dyp * SMX(c - sj) + dyj * SMX(c)) / (dyj + dyp);
-      const float evsz2 = (dzp * SMX(c) + dzk * SMX(c + sk)) / (dzk + dzp);
+      const float evsy2 = (dyp * cX0 + dyj * SMX(c + sj)) / (dyj + dyp);
+      const float evsy1 = (dyp * SMX(c - sj) + dyj * cX0) / (dyj + dyp);
       const float visux2 = (evsx2) * 2.0f * d1[c + 1];
       const float visux1 = (evsx1) * 2.0f * d1[c];
       const float visuy2 = (evsy2) * (d2[c + sj] + d4[c + 1]);
       const float visuy1 = (evsy1) * (d2[c] + d4[c + 1 - sj]);
-      const float visuz2 = (evsz2) * (d3[c + sk] + d7[c + 1]);
+      visuz2 = (evsz2) * (d3[c + sk] + d7[c + 1]);
       float vfu;
       if (k >= 2) {
-        const float evsz1 = (dzk * SMX(c - sk) + dzm * SMX(c)) / (dzm + dzk);
-        const float visuz1 = (evsz1) * (d3[c] + d7[c + 1 - sk]);
+        const float visuz1 = cUz1;
         vfu = (visux2 - visux1) / dxs + (visuy2 - visuy1) / dyj + (visuz2 - visuz1) / dzk;
       } else {
         const float t = 0.4f * uspd[(i + 1) + (j + 1) * sj] / wl;
@@ -951,18 +968,16 @@ __device__ __forceinline__ void les_visc_body(const VBlk& vb, const Grid& g, con
     {
       const float evsy2 = sm[c + sj];
       const float evsy1 = sm[c];
-      const float evsx2 = (dyp * SMX(c) + dyj * SMX(c + sj)) / (dyj + dyp);
-      const float evsx1 = (dyp * SMXM(c) + dyj * SMXM(c + sj)) / (dyj + dyp);
-      const float evsz2 = (dzp * SMX(c) + dzk * SMX(c + sk)) / (dzk + dzp);
+      const float evsx2 = (dyp * cX0 + dyj * SMX(c + sj)) / (dyj + dyp);
+      const float evsx1 = (dyp * cXM0 + dyj * SMXM(c + sj)) / (dyj + dyp);
       const float visvx2 = (evsx2) * (d2[c + sj] + d4[c + 1]);
       const float visvx1 = (evsx1) * (d2[c - 1 + sj] + d4[c]);
       const float visvy2 = (evsy2) * 2.0f * d5[c + sj];
       const float visvy1 = (evsy1) * 2.0f * d5[c];
-      const float visvz2 = (evsz2) * (d6[c + sk] + d8[c + sj]);
+      visvz2 = (evsz2) * (d6[c + sk] + d8[c + sj]);
       float visvz1;
       if (k >= 2) {
-        const float evsz1 = (dzk * SMX(c - sk) + dzm * SMX(c)) / (dzm + dzk);
-        visvz1 = (evsz1) * (d6[c] + d8[c + sj - sk]);
+        visvz1 = cVz1;
       } else {
         const float t = 0.4f * vspd[(i + 1) + (j + 1) * sj] / wl;
         visvz1 = t * t * vspd[(i + 1) + (j + 1) * sj];
@@ -972,21 +987,28 @@ __device__ __forceinline__ void les_visc_body(const VBlk& vb, const Grid& g, con
     }
     // ---- h (les.f95:206-242)
     {
-      const float evsz2 = sm[c + sk];
-      const float evsz1 = sm[c];
-      const float evsx2 = (dzp * SMX(c) + dzk * SMX(c + sk)) / (dzk + dzp);
-      const float evsx1 = (dzp * SMXM(c) + dzk * SMXM(c + sk)) / (dzk + dzp);
-      const float evsy2 = (dzp * SMY(c) + dzk * SMY(c + sk)) / (dzk + dzp);
-      const float evsy1 = (dzp * SMYM(c) + dzk * SMYM(c + sk)) / (dzk + dzp);
+      const float hevsz2 = sm[c + sk];
+      const float evsx2 = evsz2;  // (dzp * SMX(c) + dzk * SMX(c + sk)) / (dzk + dzp)
+      const float evsx1 = (dzp * cXM0 + dzk * XMp) / (dzk + dzp);
+      const float evsy2 = (dzp * cY0 + dzk * Yp) / (dzk + dzp);
+      const float evsy1 = (dzp * cYM0 + dzk * YMp) / (dzk + dzp);
       const float viswx2 = (evsx2) * (d3[c + sk] + d7[c + 1]);
       const float viswx1 = (evsx1) * (d3[c - 1 + sk] + d7[c]);
       const float viswy2 = (evsy2) * (d6[c + sk] + d8[c + sj]);
       const float viswy1 = (evsy1) * (d6[c - sj + sk] + d8[c]);
-      const float viswz2 = (evsz2) * 2.0f * d9[c + sk];
-      const float viswz1 = (evsz1) * 2.0f * d9[c];
+      viswz2 = (hevsz2) * 2.0f * d9[c + sk];
+      const float viswz1 = cWz1;
       const float vfw = (viswx2 - viswx1) / dxi + (viswy2 - viswy1) / dyj + (viswz2 - viswz1) / dzk;
       hv = (h[c] + vfw);
     }
+    // carry to level k+1
+    cX0 = Xp;
+    cXM0 = XMp;
+    cY0 = Yp;
+    cYM0 = YMp;
+    cUz1 = visuz2;
+    cVz1 = visvz2;
+    cWz1 = viswz2;
     if (ADAM) {
       f[c] = 1.5f * fv - 0.5f * fo[c];
       gg[c] = 1.5f * gv - 0.5f * go[c];
@@ -1022,11 +1044,11 @@ static int boundsm_blocks(const Grid& g) {
 void launch_les(const DevDomain& d) {
   const Grid& g = d.g;
   const P10 diu = p10(d.F.diu);
-  TIMED(d, KC_LES_SM, hipLaunchKernelGGL(k_les_sm, les_sm_grid(d), block2d(), 0, d.stream, g, diu,
+  TIMED(d, KC_LES_SM, hipLaunchKernelGGL(k_les_sm, krun_grid(d, d.sm_kchunk), block2d(), 0, d.stream, g, diu,
                      d.m.delx1 + 1, d.F.sm));
   TIMED(d, KC_BOUNDSM, hipLaunchKernelGGL(k_boundsm, dim3((unsigned)boundsm_blocks(g)), dim3(256), 0, d.stream, g,
                                           d.F.sm));
-  TIMED(d, KC_LES_VISC, hipLaunchKernelGGL(k_les_visc<false>, grid2d(g.im, g.jm, kz(g)), block2d(), 0, d.stream, g,
+  TIMED(d, KC_LES_VISC, hipLaunchKernelGGL(k_les_visc<false>, krun_grid(d, d.visc_kchunk), block2d(), 0, d.stream, g,
                      d.m, diu, d.F.sm, d.F.uspd, d.F.vspd, d.F.f, d.F.g, d.F.h, nullptr, nullptr, nullptr, d.wl));
 }
 
@@ -1034,11 +1056,11 @@ void launch_les(const DevDomain& d) {
 void launch_les_adam(const DevDomain& d) {
   const Grid& g = d.g;
   const P10 diu = p10(d.F.diu);
-  TIMED(d, KC_LES_SM, hipLaunchKernelGGL(k_les_sm, les_sm_grid(d), block2d(), 0, d.stream, g, diu,
+  TIMED(d, KC_LES_SM, hipLaunchKernelGGL(k_les_sm, krun_grid(d, d.sm_kchunk), block2d(), 0, d.stream, g, diu,
                      d.m.delx1 + 1, d.F.sm));
   TIMED(d, KC_BOUNDSM, hipLaunchKernelGGL(k_boundsm, dim3((unsigned)boundsm_blocks(g)), dim3(256), 0, d.stream, g,
                                           d.F.sm));
-  TIMED(d, KC_LES_VISC, hipLaunchKernelGGL(k_les_visc<true>, grid2d(g.im, g.jm, kz(g)), block2d(), 0, d.stream, g,
+  TIMED(d, KC_LES_VISC, hipLaunchKernelGGL(k_les_visc<true>, krun_grid(d, d.visc_kchunk), block2d(), 0, d.stream, g,
                      d.m, diu, d.F.sm, d.F.uspd, d.F.vspd, d.F.f, d.F.g, d.F.h, d.F.fold, d.F.gold, d.F.hold,
                      d.wl));
 }
@@ -1258,9 +1280,9 @@ void step_batch_spans(const DevDomain& d, int nzb, BatchDom* e, StageSpan* sp) {
   } else {
     sp[BS_VELFG] = span_of(velfg_grid(d, &e->velfg_kc));
   }
-  sp[BS_LES_SM] = span_of(les_sm_grid(d));
+  sp[BS_LES_SM] = span_of(krun_grid(d, d.sm_kchunk));
   sp[BS_BOUNDSM] = span_of(dim3((unsigned)boundsm_blocks(g)));
-  sp[BS_LES_VISC] = span_of(grid2d(g.im, g.jm, kz(g)));
+  sp[BS_LES_VISC] = span_of(krun_grid(d, d.visc_kchunk));
 }
 
 void launch_step_batch(const StageBatch& b, hipStream_t stream) {

@@ -66,6 +66,7 @@ struct DevDomain {
   int fuse_velfg;    // whole steps run vel2+velfg+feedbf as one kernel
   int velfg_kchunk;  // k levels per thread of that kernel (0: automatic)
   int sm_kchunk;     // k levels per thread of k_les_sm (0: automatic)
+  int visc_kchunk;   // k levels per thread of k_les_visc (0: automatic)
   int fuse_les;      // whole steps apply adam inside the les viscous kernel
   int press_mode;    // 0: the reference's SOR; 1: FP64 defect correction around FP32 SOR
   int dc_outer;      // defect-correction steps per press call

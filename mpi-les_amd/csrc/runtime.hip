@@ -547,6 +547,7 @@ int domain_alloc(int im, int jm, int km, les_domain** out) {
   D->d.fuse_velfg = env_int("LES_FUSE_VELFG", 1);
   D->d.velfg_kchunk = env_int("LES_VELFG_KCHUNK", 0);
   D->d.sm_kchunk = std::max(0, env_int("LES_SM_KCHUNK", 0));
+  D->d.visc_kchunk = std::max(0, env_int("LES_VISC_KCHUNK", 0));
   D->d.fuse_les = env_int("LES_FUSE_LES", 1);
   D->d.sor_fused_blocks = env_int("LES_SOR_FUSED_BLOCKS", 0);
   D->d.sor_rows = env_int("LES_SOR_ROWS", 2) == 1 ? 1 : 2;
