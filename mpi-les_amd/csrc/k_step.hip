@@ -73,6 +73,18 @@ __device__ __forceinline__ void bondv1_inflow_body(const VBlk& vb, const Grid& g
     ctrl->uout_max = 0u;
     ctrl->uout_nmin = 0u;
   }
+  if (ilast == 1) {  // steps n > 1: the planes i = 0, 1, one thread per (j, k) (grid of bondv1_grid)
+    const int j = 1 + vb.x * BX + threadIdx.x, k = 1 + vb.y * BY + threadIdx.y;
+    if (j > g.jm || k > g.km) return;
+    const int q = (j - 1) + (k - 1) * g.jm;
+    for (int i = 0; i <= 1; i++) {
+      const int c = ix(g, i, j, k);
+      u[c] = m.inflow ? m.inflow[q] : 5.0f;
+      v[c] = m.inflow_v ? m.inflow_v[q] : 0.0f;
+      w[c] = m.inflow_w ? m.inflow_w[q] : 0.0f;
+    }
+    return;
+  }
   const int i = vb.x * BX + threadIdx.x, j = 1 + vb.y * BY + threadIdx.y;
   const int k = 1 + vb.z;
   if (i > ilast || j > g.jm) return;
@@ -189,11 +201,12 @@ __global__ __launch_bounds__(256) void k_bondv1_ghost(Grid g, float* __restrict_
   bondv1_ghost_body(hw_blk<false>(), g, u, v, w);
 }
 
-// Grids of the four bondv1 launches (one-dimensional but the first).
+// Grids of the four bondv1 launches (one-dimensional but the first, which
+// covers (i, j, k) at n == 1 and (j, k) for the two inflow planes after).
 static dim3 bondv1_grid(const Grid& g, int ilast, int part) {
   const int nplane = g.jm * g.km;
   switch (part) {
-    case 0: return grid2d(ilast + 1, g.jm, g.km);
+    case 0: return ilast == 1 ? grid2d(g.jm, g.km, 1) : grid2d(ilast + 1, g.jm, g.km);
     case 1: return dim3((unsigned)std::min((nplane + 255) / 256, 256));
     case 2: return dim3((unsigned)((nplane + 255) / 256));
     default: return dim3((unsigned)(((g.im + 2) * (g.km + g.jm + 3) + 255) / 256));

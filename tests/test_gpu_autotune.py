@@ -2,7 +2,10 @@
 (les_domain_autotune): it picks a valid shape, leaves the domain's state
 and last press statistics untouched, is remembered per grid shape, and the
 tuned domain steps bit for bit like an untuned one (every tiling gives the
-colour sweeps' iterates)."""
+colour sweeps' iterates) with the same SOR iteration counts.  The residual
+history sum(reltmp^2) is summed in FP64 per SOR block, so another tiling
+adds the same exact squares in another grouping: it agrees to rounding
+(relative 1e-12), not bit for bit."""
 import numpy as np
 import pytest
 
@@ -54,6 +57,8 @@ def test_autotune_keeps_state_and_results(case):
         c.step(n, 4)
     for f in FIELDS:
         assert np.array_equal(a.download(f), c.download(f)), f
-    assert a.press_stats()["sor"].tolist() == c.press_stats()["sor"].tolist()
+    sa, sc = a.press_stats(), c.press_stats()
+    assert sa["iters"] == sc["iters"]
+    np.testing.assert_allclose(sa["sor"], sc["sor"], rtol=1e-12, atol=0)
     for d in (a, b, c):
         d.close()
