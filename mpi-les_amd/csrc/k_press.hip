@@ -736,8 +736,13 @@ __device__ __forceinline__ void sor_iter2_block(Grid g, SplitGrid s, Metrics m, 
   };
   Sl sl[3];
   float bqA[6], bqB[6];  // previous black iterate, level L at (L - krs + 1) mod 6
+  // Level km+1 is the ghost boundp2 fills from level km (press.f95:107,
+  // boundp.f95:28-78), which for the red update at km is the cell itself:
+  // that slot of the ring takes the red iterate of level km, and the red
+  // update reads its k+1 neighbour from the ring without a test.
   auto loadbq = [&](float& a, float& b, int L) {
-    const unsigned o = oBo + (unsigned)min(L, lbq) * cplb;
+    const int Lc = min(L, lbq);
+    const unsigned o = (Lc == km + 1) ? oRo + (unsigned)km * cplb : oBo + (unsigned)Lc * cplb;
     a = bload(S, voA, o);
     b = bload(S, voB, o);
   };
@@ -750,6 +755,15 @@ __device__ __forceinline__ void sor_iter2_block(Grid g, SplitGrid s, Metrics m, 
   LB[ownA] = bqA[1];  // level krs into plane 0
   LB[ownA + W] = bqB[1];
   float rdA[3] = {0.f, 0.f, 0.f}, rdB[3] = {0.f, 0.f, 0.f};  // new red values, level L at (L - krs) mod 3
+  if (krs == 1) {
+    // level 0 is the ghost boundp2 fills from level 1: the k-1 neighbour of
+    // the red update at level 1 is the red cell itself, that of the black
+    // update at level 1 the black cell itself
+    bqA[0] = sl[0].roA;
+    bqB[0] = sl[0].roB;
+    rdA[2] = bqA[1];
+    rdB[2] = bqB[1];
+  }
   float mx[2] = {0.f, 0.f}, my[2] = {0.f, 0.f}, mz[2] = {0.f, 0.f};  // metrics of the step's level, by parity
   double acc = 0.0;
   __syncthreads();  // lvm4 and plane 0 are complete
@@ -776,7 +790,7 @@ __device__ __forceinline__ void sor_iter2_block(Grid g, SplitGrid s, Metrics m, 
       const bool kinB = (unsigned)(kb - k0) <= (unsigned)(k1 - k0);
       const unsigned soR = kinR ? oRn + (unsigned)kr * cplb : 0u;
       const unsigned soB = kinB ? oBn + (unsigned)kb * cplb : 0u;
-      const bool kfirst = kr == 1, klast = kr == km, bfirst = kb == 1, blast = kb == km;
+      const bool blast = kb == km;  // levels 0 and km+1 of the rings hold the ghost values
       Sl& q = sl[U % 3];
       const float* pb_ = LB + (U % 3) * PL;
       // 2. red, level kr (press.f95:93-104), rows A and B
@@ -788,8 +802,8 @@ __device__ __forceinline__ void sor_iter2_block(Grid g, SplitGrid s, Metrics m, 
         const float pip = last[pa] ? own : hi;
         const float pjm = pb_[upA];
         const float pjp = bqB[(U + 1) % 6];  // row B's previous black iterate, level kr
-        const float pkm = kfirst ? own : bqA[U % 6];
-        const float pkp = klast ? own : bqA[(U + 2) % 6];
+        const float pkm = bqA[U % 6];
+        const float pkp = bqA[(U + 2) % 6];
         const float cn = SEP ? rcp_rn(sA[pa] + mz[P]) : q.crA;  // init.f95:100-107
         const float reltmp = relax<OM1>(omega, cn * (c2l[pa] * pip + c2s[pa] * pim + c3lA * pjp + c3sA * pjm +
                                             mx[P] * pkp + my[P] * pkm - q.rrA) -
@@ -805,8 +819,8 @@ __device__ __forceinline__ void sor_iter2_block(Grid g, SplitGrid s, Metrics m, 
         const float pip = last[pb] ? own : hi;
         const float pjm = bqA[(U + 1) % 6];  // row A's previous black iterate, level kr
         const float pjp = pb_[dnB];
-        const float pkm = kfirst ? own : bqB[U % 6];
-        const float pkp = klast ? own : bqB[(U + 2) % 6];
+        const float pkm = bqB[U % 6];
+        const float pkp = bqB[(U + 2) % 6];
         const float cn = SEP ? rcp_rn(sB[pb] + mz[P]) : q.crB;
         const float reltmp = relax<OM1>(omega, cn * (c2l[pb] * pip + c2s[pb] * pim + c3lB * pjp + c3sB * pjm +
                                             mx[P] * pkp + my[P] * pkm - q.rrB) -
@@ -829,7 +843,7 @@ __device__ __forceinline__ void sor_iter2_block(Grid g, SplitGrid s, Metrics m, 
         const float pip = last[pa] ? own : hi;
         const float pjm = pr[upA];
         const float pjp = rdB[(U + 2) % 3];  // row B's red value, level kb
-        const float pkm = bfirst ? own : rdA[(U + 1) % 3];
+        const float pkm = rdA[(U + 1) % 3];
         const float pkp = blast ? own : redA;
         const float cn = SEP ? rcp_rn(sA[pa] + mz[P ^ 1]) : q.cbA;
         const float reltmp = relax<OM1>(omega, cn * (c2l[pa] * pip + c2s[pa] * pim + c3lA * pjp + c3sA * pjm +
@@ -845,7 +859,7 @@ __device__ __forceinline__ void sor_iter2_block(Grid g, SplitGrid s, Metrics m, 
         const float pip = last[pb] ? own : hi;
         const float pjm = rdA[(U + 2) % 3];  // row A's red value, level kb
         const float pjp = pr[dnB];
-        const float pkm = bfirst ? own : rdB[(U + 1) % 3];
+        const float pkm = rdB[(U + 1) % 3];
         const float pkp = blast ? own : redB;
         const float cn = SEP ? rcp_rn(sB[pb] + mz[P ^ 1]) : q.cbB;
         const float reltmp = relax<OM1>(omega, cn * (c2l[pb] * pip + c2s[pb] * pim + c3lB * pjp + c3sB * pjm +

@@ -236,7 +236,9 @@ def run(im, jm, km, kc, ty=16, seed=0, omega=1.0, remap=1, tw=FX):
                         rrB=load(voB, orR + lr), rbA=load(voA, orB + lk), rbB=load(voB, orB + lk))
 
         def loadbq(L):
-            o = oBo + min(L, lbq) * cplb
+            # level km+1 (the boundp2 ghost of level km) reads the red iterate of level km
+            Lc = min(L, lbq)
+            o = oRo + km * cplb if Lc == km + 1 else oBo + Lc * cplb
             return load(voA, o), load(voB, o)
 
         sl = [fill(krs), fill(krs + 1), fill(krs + 2)]
@@ -250,6 +252,10 @@ def run(im, jm, km, kc, ty=16, seed=0, omega=1.0, remap=1, tw=FX):
         LB[ownA + W] = bqB[1]
         rdA = [np.zeros_like(voA, dtype=f32)] * 3
         rdB = [np.zeros_like(voA, dtype=f32)] * 3
+        if krs == 1:  # level 0 (the boundp2 ghost of level 1): the cells themselves
+            bqA[0], bqB[0] = sl[0]["roA"], sl[0]["roB"]
+            rdA = list(rdA); rdB = list(rdB)
+            rdA[2], rdB[2] = bqA[1], bqB[1]
         mx = [f32(0), f32(0)]
         my = [f32(0), f32(0)]
         mz = [f32(0), f32(0)]
@@ -299,13 +305,13 @@ def run(im, jm, km, kc, ty=16, seed=0, omega=1.0, remap=1, tw=FX):
             lo = lds_read(LB, pbase + bsA + pa, 3 * PL); hi = lds_read(LB, pbase + bsA + pa + 1, 3 * PL)
             redA, rA_ = cellupd(q["roA"], lo, hi, lds_read(LB, pbase + upA, 3 * PL), bqB[(U + 1) % 6],
                                 bqA[U % 6], bqA[(U + 2) % 6], pa, c3lA, c3sA, sA[pa], mx[P], my[P], mz[P],
-                                q["rrA"], first, last[pa], kf, kl)
+                                q["rrA"], first, last[pa], False, False)
             store(redA, svA[pa], soR, bid, okA[pa] & kinR)
             acc += np.where(okA[pa] & kinR, rA_.astype(float) ** 2, 0)
             lo = lds_read(LB, pbase + bsB + pb, 3 * PL); hi = lds_read(LB, pbase + bsB + pb + 1, 3 * PL)
             redB, rB_ = cellupd(q["roB"], lo, hi, bqA[(U + 1) % 6], lds_read(LB, pbase + dnB, 3 * PL),
                                 bqB[U % 6], bqB[(U + 2) % 6], pb, c3lB, c3sB, sB[pb], mx[P], my[P], mz[P],
-                                q["rrB"], first, last[pb], kf, kl)
+                                q["rrB"], first, last[pb], False, False)
             store(redB, svB[pb], soR, bid, okB[pb] & kinR)
             acc += np.where(okB[pb] & kinR, rB_.astype(float) ** 2, 0)
             LR[P * PL + ownA] = redA
@@ -318,14 +324,14 @@ def run(im, jm, km, kc, ty=16, seed=0, omega=1.0, remap=1, tw=FX):
             ownb = bqA[U % 6]
             bA, bA_ = cellupd(ownb, lo, hi, lds_read(LR, rbase + upA, 2 * PL), rdB[(U + 2) % 3],
                               rdA[(U + 1) % 3], redA, pa, c3lA, c3sA, sA[pa], mx[P ^ 1], my[P ^ 1], mz[P ^ 1],
-                              q["rbA"], first, last[pa], bf, bl)
+                              q["rbA"], first, last[pa], False, bl)
             store(bA, svA[pa], soB, bid, okA[pa] & kinB)
             acc += np.where(okA[pa] & kinB, bA_.astype(float) ** 2, 0)
             lo = lds_read(LR, rbase + bsB + pb, 2 * PL); hi = lds_read(LR, rbase + bsB + pb + 1, 2 * PL)
             ownb = bqB[U % 6]
             bB, bB_ = cellupd(ownb, lo, hi, rdA[(U + 2) % 3], lds_read(LR, rbase + dnB, 2 * PL),
                               rdB[(U + 1) % 3], redB, pb, c3lB, c3sB, sB[pb], mx[P ^ 1], my[P ^ 1], mz[P ^ 1],
-                              q["rbB"], first, last[pb], bf, bl)
+                              q["rbB"], first, last[pb], False, bl)
             store(bB, svB[pb], soB, bid, okB[pb] & kinB)
             acc += np.where(okB[pb] & kinB, bB_.astype(float) ** 2, 0)
             sl[U % 3] = fill(kr + 3)
