@@ -1278,8 +1278,9 @@ static StageSpan span_of(dim3 g) { return {0, (int)g.x, (int)g.y, (int)g.z}; }
 
 void step_batch_spans(const DevDomain& d, int nzb, BatchDom* e, StageSpan* sp) {
   const Grid& g = d.g;
-  // the stencils keep one level per block (measured: longer runs made
-  // les_sm and les_visc slower in a batch as alone); velfg takes nzb
+  // velfg, les_sm and les_visc (whose threads walk runs of levels, carrying
+  // what neighbouring levels share) take the batch's k split nzb; velnw keeps
+  // one level per block
   const int nz = nzb > 0 ? std::min(nzb, g.km) : kz(g);
   e->ilast = 1;
   sp[BS_VELNW] = span_of(grid2d(g.im, g.jm, kz(g)));
@@ -1293,9 +1294,9 @@ void step_batch_spans(const DevDomain& d, int nzb, BatchDom* e, StageSpan* sp) {
   } else {
     sp[BS_VELFG] = span_of(velfg_grid(d, &e->velfg_kc));
   }
-  sp[BS_LES_SM] = span_of(krun_grid(d, d.sm_kchunk));
+  sp[BS_LES_SM] = span_of(nzb > 0 ? grid2d(g.im, g.jm, nz) : krun_grid(d, d.sm_kchunk));
   sp[BS_BOUNDSM] = span_of(dim3((unsigned)boundsm_blocks(g)));
-  sp[BS_LES_VISC] = span_of(krun_grid(d, d.visc_kchunk));
+  sp[BS_LES_VISC] = span_of(nzb > 0 ? grid2d(g.im, g.jm, nz) : krun_grid(d, d.visc_kchunk));
 }
 
 void launch_step_batch(const StageBatch& b, hipStream_t stream) {
