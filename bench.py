@@ -128,22 +128,32 @@ def profiled_traffic(kernel_class: str, workload: str):
     return int(m.group(1)) if m else None
 
 
+SERVE_BATCH_CELLS = 2 << 20  # domains above this many cells fill the GPU alone and are not batched
+
+
 def serve_measurement(nreq: int, nsteps: int, sizes=None) -> dict:
     """Mixed-size requests (nx, ny in 96..512, nz in 64..128, seeded urban
-    set-ups; or the given sizes) all arriving at once: served concurrently
-    (one stream and step graph per domain, admission by device memory) and,
-    for comparison, one after another.  Reports aggregate cell updates per
-    second of both and the time-to-result percentiles of the concurrent run."""
+    set-ups; or the given sizes) all arriving at once: served with continuous
+    batching (the headline figures: domains of at most SERVE_BATCH_CELLS
+    cells stepped as one batch, larger ones on their own streams alongside),
+    concurrently on streams (one stream and step graph per domain) and one
+    after another, admission by device memory in every mode, domains built
+    and released on a worker thread.  Every grid shape's SOR launch shape is
+    tuned before the timed runs, so that no mode pays for it.  Reports the
+    aggregate cell updates per second and the time-to-result percentiles of
+    each."""
     from lesamd import serving
     reqs = serving.sample_requests(nreq, nsteps, seed=2024, sizes=sizes)
     srv = serving.Server(budget_bytes=200 << 30, burst=4)
     print(f"[bench] serving {nreq} requests x {nsteps} steps", file=sys.stderr, flush=True)
+    srv.warm(reqs)
     seq = serving.summarize(srv.run(reqs, sequential=True))
     con = serving.summarize(srv.run(reqs))
-    bat = serving.summarize(srv.run_batched(reqs))
+    bat = serving.summarize(srv.run_batched(reqs, max_batch_cells=SERVE_BATCH_CELLS))
     return {
         "requests": nreq, "steps_each": nsteps, "arrivals": "all at t=0",
-        "mode": "continuous batching (each stage one launch for all active domains, SOR one launch per iteration)",
+        "mode": f"continuous batching of the domains of at most {SERVE_BATCH_CELLS} cells (each stage one launch "
+                "for all of them, SOR one launch per iteration), larger domains on their own streams",
         "mcell_updates_per_s": round(bat["mcell_updates_per_s"], 1),
         "p50_s": round(bat["p50_s"], 4), "p99_s": round(bat["p99_s"], 4),
         "speedup_vs_sequential": round(bat["mcell_updates_per_s"] / seq["mcell_updates_per_s"], 3),
@@ -152,6 +162,36 @@ def serve_measurement(nreq: int, nsteps: int, sizes=None) -> dict:
         "streams_mcell_updates_per_s": round(con["mcell_updates_per_s"], 1),
         "streams_p50_s": round(con["p50_s"], 4), "streams_p99_s": round(con["p99_s"], 4),
     }
+
+
+def poisson_sweep(rates: list[float], nreq: int, nsteps: int) -> int:
+    """The mixed-size request stream of the serving measurement arriving as a
+    Poisson process, swept over arrival rates (requests per second): at each
+    rate the same seeded requests are served with continuous batching (as in
+    the serving measurement) and one after another, after every grid shape
+    has been tuned.  Reports per rate and mode the aggregate cell updates per
+    second over the run (first arrival to last completion), the latency
+    percentiles and the mean wait from arrival to admission; admission
+    saturates where the wait and p99 climb while the throughput flattens."""
+    from lesamd import serving
+    srv = serving.Server(budget_bytes=200 << 30, burst=4)
+    out = {"requests": nreq, "steps_each": nsteps, "batch_cells_max": SERVE_BATCH_CELLS, "sweep": []}
+    for rate in rates:
+        reqs = serving.sample_requests(nreq, nsteps, rate=rate, seed=2024)
+        srv.warm(reqs)
+        rec = {"rate_per_s": rate, "offered_mcell_updates_per_s":
+               round(sum(r.cells * r.nsteps for r in reqs) / max(reqs[-1].arrival, 1e-9) / 1e6, 1)}
+        for tag, fn in (("batched", lambda: srv.run_batched(reqs, max_batch_cells=SERVE_BATCH_CELLS)),
+                        ("sequential", lambda: srv.run(reqs, sequential=True))):
+            print(f"[bench] poisson rate {rate}/s {tag}", file=sys.stderr, flush=True)
+            res = fn()
+            s = serving.summarize(res)
+            rec[tag] = {"mcell_updates_per_s": round(s["mcell_updates_per_s"], 1),
+                        "p50_s": round(s["p50_s"], 4), "p99_s": round(s["p99_s"], 4),
+                        "mean_wait_s": round(float(np.mean([r.admitted - r.arrival for r in res])), 4)}
+        out["sweep"].append(rec)
+    print(json.dumps(out), flush=True)
+    return 0
 
 
 def precision_comparison(K: int, W: int) -> int:
@@ -231,11 +271,10 @@ def main(argv=None):
                          "precision-512x512x128 workload (prints its own JSON line)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run device-to-device copies of this size (HBM counter calibration)")
+    ap.add_argument("--serve-poisson", default="", metavar="RATES",
+                    help="only serve the mixed request stream with Poisson arrivals at these comma-separated "
+                         "rates (requests per second), batched and sequential (prints its own JSON line)")
     args = ap.parse_args(argv)
-    if args.copy_calibration:
-        return copy_calibration(args.copy_calibration)
-    if args.precision:
-        return precision_comparison(args.steps if args.steps != 100 else 20, args.warmup if args.warmup != 10 else 3)
     if args.gpus != 1:
         print("bench.py: this build runs on exactly one GPU (one domain per device); "
               f"--gpus {args.gpus} is not supported", file=sys.stderr)
@@ -243,6 +282,13 @@ def main(argv=None):
     if int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: multi-rank launch is not supported; run with one GPU", file=sys.stderr)
         return 2
+    if args.serve_poisson:
+        return poisson_sweep([float(x) for x in args.serve_poisson.split(",")], args.serve_requests,
+                             args.serve_steps)
+    if args.copy_calibration:
+        return copy_calibration(args.copy_calibration)
+    if args.precision:
+        return precision_comparison(args.steps if args.steps != 100 else 20, args.warmup if args.warmup != 10 else 3)
 
     w = setup_gen.workloads()[args.workload]
     # the CPU baseline needs the host set-up arrays (masks, cn1); without it

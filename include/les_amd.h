@@ -140,7 +140,8 @@ int les_sync_host(const void* host_array);
 int les_mark_host_dirty(const void* host_array);
 /* Waits for all work of every reference-signature domain. */
 int les_synchronize(void);
-/* Frees every reference-signature domain and forgets all host arrays. */
+/* Frees every reference-signature domain, forgets all host arrays, and
+ * returns to the HIP runtime the device memory kept for reuse (below). */
 int les_release_all(void);
 
 /* --------------------------------------------------------- domain API */
@@ -173,6 +174,13 @@ typedef struct {
 } les_domain_desc;
 
 int les_domain_create(const les_domain_desc* desc, les_domain** out);
+/* Waits for the domain's stream and releases it.  Its device memory is kept
+ * for the next domains and batches (best fit, at most an eighth to spare)
+ * instead of being freed: hipFree would wait for every stream of the device.
+ * Kept memory goes back to the runtime when an allocation finds the device
+ * full, and in les_release_all.  A domain must outlive the batches holding
+ * it.  Creation, destruction and stepping of different domains may run on
+ * different threads. */
 int les_domain_destroy(les_domain* d);
 /* One time step n (main.f95:260-296): the seven stages in order. */
 int les_domain_step(les_domain* d, int n, int nmax);

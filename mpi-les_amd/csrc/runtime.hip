@@ -36,6 +36,101 @@ int fail(const std::string& msg) {
       return fail(std::string(#expr) + ": " + hipGetErrorString(_e));              \
   } while (0)
 
+// ------------------------------------------------------------ device memory
+// Blocks released by destroyed domains and batches are kept for reuse rather
+// than returned with hipFree: hipFree waits for every stream of the device,
+// so a server creating and destroying domains while others step would drain
+// the GPU at each release, and from another thread that wait breaks a graph
+// capture under way.  A request takes the smallest kept block of its device
+// that holds it with at most an eighth to spare, else new memory; when the
+// device is out of memory the kept blocks are returned and the allocation
+// is tried once more.  les_release_all returns them as well.  g_capture_mu
+// orders the calls that may wait for the whole device (hipMalloc, hipFree,
+// stream creation and destruction) against graph captures.
+std::mutex g_pool_mu;
+std::multimap<std::pair<int, size_t>, void*> g_pool_kept;  // (device, bytes) -> block
+std::map<void*, std::pair<int, size_t>> g_pool_live;
+std::recursive_mutex g_capture_mu;
+
+void pool_trim() {
+  std::lock_guard<std::recursive_mutex> ck(g_capture_mu);
+  std::lock_guard<std::mutex> lk(g_pool_mu);
+  for (auto& kv : g_pool_kept) hipFree(kv.second);
+  g_pool_kept.clear();
+}
+
+hipError_t dev_alloc(void** p, size_t nb) {
+  *p = nullptr;
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  nb = std::max<size_t>(nb, 256);
+  {
+    std::lock_guard<std::mutex> lk(g_pool_mu);
+    auto it = g_pool_kept.lower_bound({dev, nb});
+    if (it != g_pool_kept.end() && it->first.first == dev && it->first.second - nb <= nb / 8) {
+      *p = it->second;
+      g_pool_live[*p] = it->first;
+      g_pool_kept.erase(it);
+      return hipSuccess;
+    }
+  }
+  {
+    std::lock_guard<std::recursive_mutex> ck(g_capture_mu);
+    e = hipMalloc(p, nb);
+  }
+  if (e == hipErrorOutOfMemory) {
+    (void)hipGetLastError();
+    pool_trim();
+    std::lock_guard<std::recursive_mutex> ck(g_capture_mu);
+    e = hipMalloc(p, nb);
+  }
+  if (e == hipSuccess) {
+    std::lock_guard<std::mutex> lk(g_pool_mu);
+    g_pool_live[*p] = {dev, nb};
+  } else {
+    *p = nullptr;
+  }
+  return e;
+}
+
+template <class T>
+hipError_t dev_alloc(T** p, size_t nb) {
+  void* v = nullptr;
+  const hipError_t e = dev_alloc(&v, nb);
+  *p = (T*)v;
+  return e;
+}
+
+// The block must no longer be used by any stream: its next owner may write
+// it at once.
+void dev_release(void* p) {
+  if (!p) return;
+  std::lock_guard<std::mutex> lk(g_pool_mu);
+  auto it = g_pool_live.find(p);
+  if (it == g_pool_live.end()) return;
+  g_pool_kept.emplace(it->second, p);
+  g_pool_live.erase(it);
+}
+
+hipError_t stream_create(hipStream_t* s) {
+  std::lock_guard<std::recursive_mutex> ck(g_capture_mu);
+  return hipStreamCreateWithFlags(s, hipStreamNonBlocking);
+}
+
+void stream_destroy(hipStream_t s) {
+  if (!s) return;
+  std::lock_guard<std::recursive_mutex> ck(g_capture_mu);
+  hipStreamDestroy(s);
+}
+
+// A blocking copy ordered on stream s: the null stream's implicit
+// synchronisation would reach graph captures under way on other threads.
+hipError_t copy_sync(void* dst, const void* src, size_t nb, hipMemcpyKind kind, hipStream_t s) {
+  const hipError_t e = hipMemcpyAsync(dst, src, nb, kind, s);
+  return e == hipSuccess ? hipStreamSynchronize(s) : e;
+}
+
 // ------------------------------------------------------------ field table
 enum Kind { KU, KW, KP, KF, KI, KR, KS, KD1, KD, KB, K2D, KAU, KAW };
 
@@ -220,9 +315,9 @@ size_t ref_floats(const les_domain* D, Kind k) {
 
 int ensure_staging(les_domain* D, size_t n) {
   if (D->staging_floats >= n) return 0;
-  if (D->staging) hipFree(D->staging);
+  dev_release(D->staging);
   D->staging = nullptr;
-  HIP_TRY(hipMalloc(&D->staging, n * sizeof(float)));
+  HIP_TRY(dev_alloc(&D->staging, n * sizeof(float)));
   D->staging_floats = n;
   return 0;
 }
@@ -289,10 +384,10 @@ int ensure_field(les_domain* D, int f) {
   if (D->fields[f]) return 0;
   const size_t nb = (size_t)D->n3 * sizeof(float);
   float* a = nullptr;
-  HIP_TRY(hipMalloc(&a, nb));
+  HIP_TRY(dev_alloc(&a, nb));
   hipError_t e = hipMemsetAsync(a, 0, nb, D->d.stream);
   if (e != hipSuccess) {
-    hipFree(a);
+    dev_release(a);
     return fail(std::string("field allocation: ") + hipGetErrorString(e));
   }
   D->fields[f] = a;
@@ -402,7 +497,7 @@ int set_inflow(les_domain* D, const float* u_in, const float* v_in = nullptr, co
   }
   const size_t plane = (size_t)g.jm * g.km;
   if (!D->inflow_dev) {
-    HIP_TRY(hipMalloc(&D->inflow_dev, 3 * plane * sizeof(float)));
+    HIP_TRY(dev_alloc(&D->inflow_dev, 3 * plane * sizeof(float)));
     D->bytes += 3 * plane * sizeof(float);
   }
   const float* src[3] = {u_in, v_in, w_in};
@@ -439,18 +534,16 @@ void domain_free(les_domain* D) {
   if (D->d.stream) hipStreamSynchronize(D->d.stream);  // nothing of this domain may still run
   timer_free(D);
   if (D->gexec) hipGraphExecDestroy(D->gexec);
-  for (int f = 0; f < NFIELDS; f++)
-    if (D->fields[f]) hipFree(D->fields[f]);
-  for (int q = 1; q <= 9; q++)
-    if (D->cov[q]) hipFree(D->cov[q]);
-  if (D->split) hipFree(D->split);
-  if (D->mbuf) hipFree(D->mbuf);
-  if (D->inflow_dev) hipFree(D->inflow_dev);
-  if (D->staging) hipFree(D->staging);
-  if (D->d.ctrl) hipFree(D->d.ctrl);
-  if (D->d.partials) hipFree(D->d.partials);
-  if (D->d.dc64) hipFree(D->d.dc64);
-  if (D->d.stream) hipStreamDestroy(D->d.stream);
+  for (int f = 0; f < NFIELDS; f++) dev_release(D->fields[f]);
+  for (int q = 1; q <= 9; q++) dev_release(D->cov[q]);
+  dev_release(D->split);
+  dev_release(D->mbuf);
+  dev_release(D->inflow_dev);
+  dev_release(D->staging);
+  dev_release(D->d.ctrl);
+  dev_release(D->d.partials);
+  dev_release(D->d.dc64);
+  stream_destroy(D->d.stream);
   delete D;
 }
 
@@ -473,12 +566,12 @@ int domain_alloc(int im, int jm, int km, les_domain** out) {
     domain_free(D);
     return fail(m);
   };
-  hipError_t e = hipStreamCreateWithFlags(&D->d.stream, hipStreamNonBlocking);
+  hipError_t e = stream_create(&D->d.stream);
   if (e != hipSuccess) return bail(std::string("hipStreamCreate: ") + hipGetErrorString(e));
   for (int f = 0; f < kLazyFirst; f++) {
     const size_t nb = (kFields[f].kind == K2D ? (size_t)sk : (size_t)n3) * sizeof(float);
-    e = hipMalloc(&D->fields[f], nb);
-    if (e == hipSuccess) e = hipMemset(D->fields[f], 0, nb);
+    e = dev_alloc(&D->fields[f], nb);
+    if (e == hipSuccess) e = hipMemsetAsync(D->fields[f], 0, nb, D->d.stream);
     if (e != hipSuccess) return bail(std::string("field allocation: ") + hipGetErrorString(e));
     D->bytes += nb;
   }
@@ -498,8 +591,8 @@ int domain_alloc(int im, int jm, int km, les_domain** out) {
     const size_t cs = (size_t)sg.cplane * km;
     const size_t nplanes = 8 * (size_t)km + 2;
     const size_t nb = nplanes * sg.cplane * sizeof(float);
-    e = hipMalloc(&D->split, nb);
-    if (e == hipSuccess) e = hipMemset(D->split, 0, nb);
+    e = dev_alloc(&D->split, nb);
+    if (e == hipSuccess) e = hipMemsetAsync(D->split, 0, nb, D->d.stream);
     if (e != hipSuccess) return bail(std::string("SOR buffer allocation: ") + hipGetErrorString(e));
     D->bytes += nb;
     SplitBufs& sb = D->d.sb;
@@ -519,7 +612,7 @@ int domain_alloc(int im, int jm, int km, les_domain** out) {
     off += axis_n(D, kMetrics[mi].axis) + 4;
   }
   D->mhost.assign(off, 0.0f);
-  e = hipMalloc(&D->mbuf, off * sizeof(float));
+  e = dev_alloc(&D->mbuf, off * sizeof(float));
   if (e != hipSuccess) return bail(std::string("metric allocation: ") + hipGetErrorString(e));
   Metrics& m = D->d.m;
   m.dx1 = D->mbuf + D->moff[MDX1];
@@ -564,11 +657,11 @@ int domain_alloc(int im, int jm, int km, les_domain** out) {
   // level per block) and for the colour sweeps
   const int nsor = std::max(sor_parts_needed(D->d.g, D->d.sg, D->d.sor_blocks),
                             sor_fused_tiles(D->d.g, D->d.sg) * D->d.g.km);
-  e = hipMalloc(&D->d.partials, ((size_t)D->d.npartials + 4 * (size_t)nsor) * sizeof(double));
+  e = dev_alloc(&D->d.partials, ((size_t)D->d.npartials + 4 * (size_t)nsor) * sizeof(double));
   if (e == hipSuccess)
     for (int q = 0; q < 4; q++) D->d.sb.part[q >> 1][q & 1] = D->d.partials + D->d.npartials + (size_t)q * nsor;
-  if (e == hipSuccess) e = hipMalloc(&D->d.ctrl, sizeof(Ctrl));
-  if (e == hipSuccess) e = hipMemset(D->d.ctrl, 0, sizeof(Ctrl));
+  if (e == hipSuccess) e = dev_alloc(&D->d.ctrl, sizeof(Ctrl));
+  if (e == hipSuccess) e = hipMemsetAsync(D->d.ctrl, 0, sizeof(Ctrl), D->d.stream);
   if (e != hipSuccess) return bail(std::string("control allocation: ") + hipGetErrorString(e));
   D->d.nmaxp = nmaxp_default;
   D->d.pjuge = pjuge_default;
@@ -613,11 +706,11 @@ int ensure_cov(les_domain* D) {
   if (D->cov[1]) return 0;
   const size_t nb = (size_t)D->n3 * sizeof(float);
   for (int q = 1; q <= 9; q++) {
-    hipError_t e = hipMalloc(&D->cov[q], nb);
-    if (e == hipSuccess) e = hipMemset(D->cov[q], 0, nb);
+    hipError_t e = dev_alloc(&D->cov[q], nb);
+    if (e == hipSuccess) e = hipMemsetAsync(D->cov[q], 0, nb, D->d.stream);
     if (e != hipSuccess) {
       for (int r = 1; r <= q; r++) {
-        if (D->cov[r]) hipFree(D->cov[r]);
+        dev_release(D->cov[r]);
         D->cov[r] = nullptr;
       }
       return fail(std::string("scratch allocation: ") + hipGetErrorString(e));
@@ -771,7 +864,7 @@ int tune_sor(les_domain* D, int reps, TuneResult* out) {
   const Grid& g = D->d.g;
   HIP_TRY(hipStreamSynchronize(D->d.stream));
   Ctrl saved;
-  HIP_TRY(hipMemcpy(&saved, D->d.ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost));
+  HIP_TRY(copy_sync(&saved, D->d.ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, D->d.stream));
   const int tiles = sor_fused_tiles(g, D->d.sg);
   const int ncu = D->d.ncu > 0 ? D->d.ncu : 256;
   const int kdef = sor_fused_kchunk(g, D->d.sg, D->d.sor_fused_blocks, D->d.ncu);
@@ -825,7 +918,7 @@ int tune_sor(les_domain* D, int reps, TuneResult* out) {
   hipEventDestroy(e0);
   hipEventDestroy(e1);
   // the last press call's statistics, as they were
-  if (hipMemcpy(D->d.ctrl, &saved, sizeof(Ctrl), hipMemcpyHostToDevice) != hipSuccess && !rc)
+  if (copy_sync(D->d.ctrl, &saved, sizeof(Ctrl), hipMemcpyHostToDevice, D->d.stream) != hipSuccess && !rc)
     rc = fail("autotune: restoring the control block");
   if (rc) return rc;
   r.us_best = best;
@@ -1092,7 +1185,7 @@ static int read_stats(les_domain* D, int* iters, double* sor_hist, int hist_len,
                       double* pav, int* nonfinite) {
   Ctrl c;
   HIP_TRY(hipStreamSynchronize(D->d.stream));
-  HIP_TRY(hipMemcpy(&c, D->d.ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost));
+  HIP_TRY(copy_sync(&c, D->d.ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, D->d.stream));
   if (iters) *iters = c.sor_iters;
   for (int l = 0; sor_hist && l < hist_len && l < c.sor_iters && l < kSorHist; l++) sor_hist[l] = c.sor_hist[l];
   if (rhsav) *rhsav = c.rhsav;
@@ -1140,6 +1233,7 @@ int les_release_all(void) {
   }
   g_domains.clear();
   g_inflows.clear();
+  pool_trim();
   return 0;
 }
 
@@ -1152,7 +1246,7 @@ int build_masks(les_domain* D, const float* zbm, const float* z2, const float* d
   for (int k = 1; k <= g.km; k++) top[k - 1] = z2[k] + 0.5f * dzn[k + 1];  // z2(0:), dzn(-1:)
   float* buf = nullptr;
   const size_t nz = (size_t)g.im * g.jm;
-  HIP_TRY(hipMalloc(&buf, (nz + g.km) * sizeof(float)));
+  HIP_TRY(dev_alloc(&buf, (nz + g.km) * sizeof(float)));
   hipError_t e = hipMemcpyAsync(buf, zbm, nz * sizeof(float), hipMemcpyHostToDevice, D->d.stream);
   if (e == hipSuccess)
     e = hipMemcpyAsync(buf + nz, top.data(), g.km * sizeof(float), hipMemcpyHostToDevice, D->d.stream);
@@ -1161,7 +1255,7 @@ int build_masks(les_domain* D, const float* zbm, const float* z2, const float* d
     e = hipGetLastError();
   }
   if (e == hipSuccess) e = hipStreamSynchronize(D->d.stream);
-  hipFree(buf);
+  dev_release(buf);
   if (e != hipSuccess) return fail(std::string("mask generation: ") + hipGetErrorString(e));
   return 0;
 }
@@ -1233,10 +1327,10 @@ int les_domain_set_press_mode(les_domain* D, int mode, int outer, int inner) {
   if (inner < 1 || inner > kSorHist) return fail("inner SOR iterations out of range (1..1024)");
   if (!D->d.dc64) {
     const size_t nb = (2 * (size_t)D->n3 + kDcMaxOuter) * sizeof(double);
-    hipError_t e = hipMalloc(&D->d.dc64, nb);
-    if (e == hipSuccess) e = hipMemset(D->d.dc64, 0, nb);
+    hipError_t e = dev_alloc(&D->d.dc64, nb);
+    if (e == hipSuccess) e = hipMemsetAsync(D->d.dc64, 0, nb, D->d.stream);
     if (e != hipSuccess) {
-      if (D->d.dc64) hipFree(D->d.dc64);
+      dev_release(D->d.dc64);
       D->d.dc64 = nullptr;
       return fail(std::string("defect-correction buffers: ") + hipGetErrorString(e));
     }
@@ -1306,6 +1400,7 @@ static int step_graph(les_domain* D, int n, int nmax) {
   if (!D->gexec) {
     // allocations cannot be captured: make the stage scratch exist first
     if (!D->d.fuse_velfg && ensure_cov(D)) return 1;
+    std::lock_guard<std::recursive_mutex> ck(g_capture_mu);
     HIP_TRY(hipStreamBeginCapture(D->d.stream, hipStreamCaptureModeThreadLocal));
     int rc = 0;
     rc = run_step(D, n, nmax);
@@ -1377,7 +1472,7 @@ int les_domain_time_sor(les_domain* D, int iters, double* us_per_iter) {
   if (!D->d.sor_fused) return fail("les_domain_time_sor: the fused SOR iteration is off");
   HIP_TRY(hipStreamSynchronize(D->d.stream));
   Ctrl saved;
-  HIP_TRY(hipMemcpy(&saved, D->d.ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost));
+  HIP_TRY(copy_sync(&saved, D->d.ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, D->d.stream));
   const int kc = D->d.sor_kchunk > 0 ? D->d.sor_kchunk
                                      : sor_fused_kchunk(D->d.g, D->d.sg, D->d.sor_fused_blocks, D->d.ncu);
   hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -1390,7 +1485,7 @@ int les_domain_time_sor(les_domain* D, int iters, double* us_per_iter) {
                           us_per_iter);
   hipEventDestroy(e0);
   hipEventDestroy(e1);
-  if (hipMemcpy(D->d.ctrl, &saved, sizeof(Ctrl), hipMemcpyHostToDevice) != hipSuccess && !rc)
+  if (copy_sync(D->d.ctrl, &saved, sizeof(Ctrl), hipMemcpyHostToDevice, D->d.stream) != hipSuccess && !rc)
     rc = fail("les_domain_time_sor: restoring the control block");
   return rc;
 }
@@ -1515,13 +1610,13 @@ void batch_free(les_batch* B) {
   for (les_domain* D : B->doms)
     if (D->d.stream) hipStreamSynchronize(D->d.stream);
   if (B->gexec) hipGraphExecDestroy(B->gexec);
-  if (B->dtab) hipFree(B->dtab);
-  if (B->stage_mem) hipFree(B->stage_mem);
+  dev_release(B->dtab);
+  dev_release(B->stage_mem);
   for (hipEvent_t e : B->head_done) hipEventDestroy(e);
   for (hipEvent_t e : B->tail_done) hipEventDestroy(e);
   if (B->fork) hipEventDestroy(B->fork);
   if (B->sor_done) hipEventDestroy(B->sor_done);
-  if (B->stream) hipStreamDestroy(B->stream);
+  stream_destroy(B->stream);
   delete B;
 }
 
@@ -1596,8 +1691,8 @@ int batch_prepare(les_batch* B) {
   B->nblocks = block;
   if (B->dtab || B->stage_mem) {
     HIP_TRY(hipStreamSynchronize(B->stream));
-    if (B->dtab) HIP_TRY(hipFree(B->dtab));
-    if (B->stage_mem) HIP_TRY(hipFree(B->stage_mem));
+    dev_release(B->dtab);
+    dev_release(B->stage_mem);
     B->dtab = nullptr;
     B->dblock0 = nullptr;
     B->stage_mem = nullptr;
@@ -1608,9 +1703,9 @@ int batch_prepare(les_batch* B) {
     const size_t ntb = tab.size() * sizeof(SorBatchEntry);
     std::vector<int> b0s;
     for (const SorBatchEntry& x : tab) b0s.push_back(x.block0);
-    HIP_TRY(hipMalloc(&B->dtab, ntb + b0s.size() * sizeof(int)));
-    HIP_TRY(hipMemcpy(B->dtab, tab.data(), ntb, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy((char*)B->dtab + ntb, b0s.data(), b0s.size() * sizeof(int), hipMemcpyHostToDevice));
+    HIP_TRY(dev_alloc(&B->dtab, ntb + b0s.size() * sizeof(int)));
+    HIP_TRY(copy_sync(B->dtab, tab.data(), ntb, hipMemcpyHostToDevice, B->stream));
+    HIP_TRY(copy_sync((char*)B->dtab + ntb, b0s.data(), b0s.size() * sizeof(int), hipMemcpyHostToDevice, B->stream));
     B->dblock0 = (const int*)((const char*)B->dtab + ntb);
     // stage spans by stage, then domain; each domain's range of blocks starts
     // at a multiple of 8
@@ -1629,10 +1724,10 @@ int batch_prepare(les_batch* B) {
     }
     for (const BatchDom& x : bdom) S.kc1max = std::max(S.kc1max, x.kc1);
     const size_t nbd = bdom.size() * sizeof(BatchDom);
-    HIP_TRY(hipMalloc(&B->stage_mem, nbd + flat.size() * sizeof(StageSpan)));
-    HIP_TRY(hipMemcpy(B->stage_mem, bdom.data(), nbd, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy((char*)B->stage_mem + nbd, flat.data(), flat.size() * sizeof(StageSpan),
-                      hipMemcpyHostToDevice));
+    HIP_TRY(dev_alloc(&B->stage_mem, nbd + flat.size() * sizeof(StageSpan)));
+    HIP_TRY(copy_sync(B->stage_mem, bdom.data(), nbd, hipMemcpyHostToDevice, B->stream));
+    HIP_TRY(copy_sync((char*)B->stage_mem + nbd, flat.data(), flat.size() * sizeof(StageSpan),
+                      hipMemcpyHostToDevice, B->stream));
     S.doms = (const BatchDom*)B->stage_mem;
     S.spans = (const StageSpan*)((const char*)B->stage_mem + nbd);
     S.nd = nd;
@@ -1723,7 +1818,7 @@ int les_batch_create(les_domain* const* doms, int n, les_batch** out) {
   les_batch* B = new les_batch();
   B->doms.assign(doms, doms + n);
   B->remap = doms[0]->d.xcd_remap;
-  hipError_t e = hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking);
+  hipError_t e = stream_create(&B->stream);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&B->fork, hipEventDisableTiming);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&B->sor_done, hipEventDisableTiming);
   for (int q = 0; q < n && e == hipSuccess; q++) {
@@ -1770,6 +1865,7 @@ int les_batch_step(les_batch* B, const int* ns, const int* nmaxs) {
   if (!B->gexec) {
     for (les_domain* D : B->doms)
       if (!D->d.fuse_velfg && ensure_cov(D)) return 1;  // allocations cannot be captured
+    std::lock_guard<std::recursive_mutex> ck(g_capture_mu);
     HIP_TRY(hipStreamBeginCapture(B->stream, hipStreamCaptureModeThreadLocal));
     const int rc = batch_enqueue(B, ns, nmaxs, true);
     hipGraph_t gr = nullptr;

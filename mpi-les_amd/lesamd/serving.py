@@ -2,13 +2,17 @@
 
 Requests (a domain and a number of time steps) arrive over time; the server
 admits a request when the device memory it needs fits in the budget, keeps
-every admitted domain resident, and advances all of them concurrently: each
-domain has its own stream and replays its captured step graph there, so the
-steps of different domains overlap on the GPU.  A request's time to result
-runs from its arrival to the completion of its last step on the device.
+every admitted domain resident, and advances all of them concurrently:
+either each domain on its own stream, replaying its captured step graph
+there (run), or the smaller domains in lockstep as one batch, every stage one
+launch for all of them, with the larger ones on their own streams alongside
+(run_batched).  Domains are built and released on a worker thread, so that
+the steps of the others keep being issued meanwhile.  A request's time to
+result runs from its arrival to the completion of its last step on the
+device.
 
-This is the host side only; each step is the library's whole-step path
-(les_domain_step), unchanged.
+This is the host side only; each step is the library's whole-step or
+batched-step path (les_domain_step, les_batch_step), unchanged.
 """
 from __future__ import annotations
 
@@ -16,6 +20,7 @@ import dataclasses
 import math
 import time
 from collections import deque
+from concurrent.futures import ThreadPoolExecutor
 
 import numpy as np
 
@@ -85,21 +90,110 @@ class _Active:
         self.issued = 0
 
 
+class _Lifecycle:
+    """Admission and the domains' life cycle for one run: first come, first
+    served, a request is admitted once it has arrived and the device memory
+    its domain holds fits the budget.  With a worker thread the domain is
+    built there and, when its request is done, closed there: creating and
+    releasing device memory and uploading the set-up wait for the device, and
+    must not hold up the issue of the other domains' steps.  The memory stays
+    counted from admission until the close has returned."""
+
+    def __init__(self, srv: "Server", requests, cap: int, t0: float):
+        self.srv, self.cap, self.t0 = srv, cap, t0
+        self.pending = deque(sorted(requests, key=lambda r: (r.arrival, r.rid)))
+        for r in self.pending:
+            if srv.bytes_of(r) > srv.budget:
+                raise ValueError(f"request {r.rid} needs more device memory than the whole budget")
+        self.pool = ThreadPoolExecutor(max_workers=srv.workers) if srv.workers > 0 else None
+        self.creating: deque = deque()
+        self.closing: list = []
+        self.used = 0
+
+    def now(self) -> float:
+        return self.srv.clock() - self.t0
+
+    def admit(self, nactive: int) -> list[_Active]:
+        """Start the domains of the requests now admissible; return those
+        whose domain is ready (in admission order)."""
+        ready = []
+        now = self.now()
+        while (self.pending and self.pending[0].arrival <= now
+               and nactive + len(self.creating) + len(ready) < self.cap):
+            need = self.srv.bytes_of(self.pending[0])
+            if self.used + need > self.srv.budget:
+                break
+            req = self.pending.popleft()
+            self.used += need
+            if self.pool is None:
+                ready.append(_Active(req, self.srv.factory(req), self.now()))
+            else:
+                self.creating.append((req, self.pool.submit(self.srv.factory, req)))
+        while self.creating and self.creating[0][1].done():
+            req, fut = self.creating.popleft()
+            ready.append(_Active(req, fut.result(), self.now()))
+        return ready
+
+    def release(self, a: _Active) -> None:
+        need = self.srv.bytes_of(a.req)
+        if self.pool is None:
+            a.dom.close()
+            self.used -= need
+        else:
+            self.closing.append((self.pool.submit(a.dom.close), need))
+
+    def reap(self) -> bool:
+        """Account the closes that have returned; True if any had."""
+        keep, reaped = [], False
+        for fut, need in self.closing:
+            if fut.done():
+                fut.result()
+                self.used -= need
+                reaped = True
+            else:
+                keep.append((fut, need))
+        self.closing = keep
+        return reaped
+
+    def waiting(self) -> bool:
+        return bool(self.pending or self.creating)
+
+    def idle_wait(self, busy: bool) -> None:
+        """Nothing was issued: poll again shortly, or sleep until the next
+        arrival when nothing is in flight."""
+        if busy or self.creating or self.closing:
+            time.sleep(20e-6)
+        elif self.pending:
+            wait = self.pending[0].arrival - self.now()
+            if wait > 0:
+                time.sleep(min(wait, 1e-3))
+
+    def finish(self) -> None:
+        if self.pool is not None:
+            self.pool.shutdown(wait=True)
+            for fut, need in self.closing:
+                fut.result()
+                self.used -= need
+            self.closing = []
+
+
 class Server:
     """Admission by device memory, concurrent stepping, completion by polling.
 
     domain_factory(request) -> object with step(n, nmax), done(), close();
     bytes_of(request) -> device bytes the request's domain holds.  The
     defaults create library domains from the light set-up (masks and cn1
-    derived on the device).
+    derived on the device).  workers > 0 builds and closes the domains on
+    that many worker threads (0: on the serving thread itself).
     """
 
     def __init__(self, budget_bytes: int, burst: int = 4, max_active: int = 64, domain_factory=None,
-                 bytes_of=None, clock=time.perf_counter, batch_factory=None):
+                 bytes_of=None, clock=time.perf_counter, batch_factory=None, workers: int = 1):
         self.budget = int(budget_bytes)
         self.burst = int(burst)
         self.max_active = int(max_active)
         self.clock = clock
+        self.workers = int(workers)
         if domain_factory is None or bytes_of is None:
             from . import api
 
@@ -121,112 +215,124 @@ class Server:
         # batch_factory(domains) -> object with step(ns, nmaxs), done(), close()
         self.batch_factory = batch_factory
 
-    def run_batched(self, requests) -> list[Result]:
-        """Continuous batching: the admitted domains advance in lockstep,
-        one batched step (one launch per stage and per SOR iteration for all
-        of them) at a time; requests join at step boundaries as memory frees
-        up, and leave when their last step has completed."""
-        pending = deque(sorted(requests, key=lambda r: (r.arrival, r.rid)))
-        for r in pending:
-            if self.bytes_of(r) > self.budget:
-                raise ValueError(f"request {r.rid} needs more device memory than the whole budget")
-        active: list[_Active] = []
+    def warm(self, requests) -> None:
+        """Create (and close) one domain of every grid shape among the
+        requests, untimed: the default factory's per-shape SOR tuning is timed
+        once per shape and process, so after this every serving mode starts
+        from the same tuned state."""
+        seen = set()
+        for r in requests:
+            w = r.workload
+            if (w.im, w.jm, w.km) not in seen:
+                seen.add((w.im, w.jm, w.km))
+                self.factory(r).close()
+
+    def run_batched(self, requests, max_batch_cells: int | None = None) -> list[Result]:
+        """Continuous batching: the admitted domains of at most
+        max_batch_cells cells (all of them when None) advance in lockstep, one
+        batched step (one launch per stage and per SOR iteration for all of
+        them) at a time, on the batch's stream; larger domains, which fill the
+        GPU by themselves, replay their own step graphs on their own streams
+        alongside.  Requests join at step boundaries once their domain is
+        ready, and leave when their last step has completed."""
+        lc = _Lifecycle(self, requests, self.max_active, self.clock())
+        batched: list[_Active] = []
+        alone: list[_Active] = []
+        joining: list[_Active] = []
         batch = None
-        used = 0
         results = []
-        t0 = self.clock()
-        while pending or active:
-            now = self.clock() - t0
-            joined = False
-            while pending and pending[0].arrival <= now and len(active) < self.max_active:
-                need = self.bytes_of(pending[0])
-                if used + need > self.budget:
-                    break
-                req = pending.popleft()
-                active.append(_Active(req, self.factory(req), self.clock() - t0))
-                used += need
-                joined = True
-            if not active:
-                wait = pending[0].arrival - (self.clock() - t0)
-                if wait > 0:
-                    time.sleep(min(wait, 1e-3))
-                continue
-            if joined or batch is None:
-                if batch is not None:
-                    batch.close()
-                batch = self.batch_factory([a.dom for a in active])
-            k = min(self.burst, min(a.req.nsteps - a.issued for a in active))
-            for _ in range(k):
-                batch.step([a.issued + 1 for a in active], [a.req.nsteps for a in active])
-                for a in active:
-                    a.issued += 1
-            if any(a.issued >= a.req.nsteps for a in active):
-                while not batch.done():
-                    time.sleep(20e-6)
+
+        def done_with(a):
+            results.append(Result(a.req.rid, a.req.cells, a.req.nsteps, a.req.arrival, a.admitted, lc.now()))
+            lc.release(a)
+
+        try:
+            while lc.waiting() or batched or alone or joining:
+                progressed = lc.reap()
+                for a in lc.admit(len(batched) + len(alone) + len(joining)):
+                    if max_batch_cells is None or a.req.cells <= max_batch_cells:
+                        joining.append(a)
+                    else:
+                        alone.append(a)
+                # the batch: members leave once the batch's stream has drained,
+                # new members join between batched steps
+                if batched and any(a.issued >= a.req.nsteps for a in batched):
+                    if batch is None or batch.done():
+                        if batch is not None:
+                            batch.close()
+                            batch = None
+                        for a in batched:
+                            if a.issued >= a.req.nsteps:
+                                done_with(a)
+                        batched = [a for a in batched if a.issued < a.req.nsteps]
+                        progressed = True
+                elif batched or joining:
+                    if joining or batch is None:
+                        if batch is not None:
+                            batch.close()
+                        batched += joining
+                        joining = []
+                        batch = self.batch_factory([a.dom for a in batched])
+                    k = min(self.burst, min(a.req.nsteps - a.issued for a in batched))
+                    for _ in range(k):
+                        batch.step([a.issued + 1 for a in batched], [a.req.nsteps for a in batched])
+                        for a in batched:
+                            a.issued += 1
+                    progressed = True
+                # the domains on their own streams
                 still = []
-                for a in active:
-                    if a.issued >= a.req.nsteps:
-                        results.append(Result(a.req.rid, a.req.cells, a.req.nsteps, a.req.arrival, a.admitted,
-                                              self.clock() - t0))
-                        used -= self.bytes_of(a.req)
+                for a in alone:
+                    if a.issued < a.req.nsteps:
+                        n1 = min(a.req.nsteps, a.issued + self.burst)
+                        for n in range(a.issued + 1, n1 + 1):
+                            a.dom.step(n, a.req.nsteps)
+                        a.issued = n1
+                        progressed = True
+                    if a.issued >= a.req.nsteps and a.dom.done():
+                        done_with(a)
+                        progressed = True
                     else:
                         still.append(a)
+                alone = still
+                if not progressed:
+                    lc.idle_wait(bool(batched or alone))
+        finally:
+            if batch is not None:
                 batch.close()
-                batch = None
-                for a in active:
-                    if a.issued >= a.req.nsteps:
-                        a.dom.close()
-                active = still
-        if batch is not None:
-            batch.close()
+            lc.finish()
         return sorted(results, key=lambda r: r.rid)
 
     def run(self, requests, sequential: bool = False) -> list[Result]:
         """Serve the requests (arrival times relative to now); with
         sequential=True one domain at a time, each run to completion."""
-        pending = deque(sorted(requests, key=lambda r: (r.arrival, r.rid)))
-        for r in pending:
-            if self.bytes_of(r) > self.budget:
-                raise ValueError(f"request {r.rid} needs more device memory than the whole budget")
+        lc = _Lifecycle(self, requests, 1 if sequential else self.max_active, self.clock())
         active: list[_Active] = []
-        used = 0
         results = []
-        t0 = self.clock()
-        cap = 1 if sequential else self.max_active
-        while pending or active:
-            now = self.clock() - t0
-            # admission: first come, first served, while the memory fits
-            while pending and pending[0].arrival <= now and len(active) < cap:
-                need = self.bytes_of(pending[0])
-                if used + need > self.budget:
-                    break
-                req = pending.popleft()
-                active.append(_Active(req, self.factory(req), self.clock() - t0))
-                used += need
-            progressed = False
-            for a in active:
-                if a.issued < a.req.nsteps:
-                    n1 = min(a.req.nsteps, a.issued + (a.req.nsteps if sequential else self.burst))
-                    for n in range(a.issued + 1, n1 + 1):
-                        a.dom.step(n, a.req.nsteps)
-                    a.issued = n1
-                    progressed = True
-            still = []
-            for a in active:
-                if a.issued >= a.req.nsteps and a.dom.done():
-                    results.append(Result(a.req.rid, a.req.cells, a.req.nsteps, a.req.arrival, a.admitted,
-                                          self.clock() - t0))
-                    a.dom.close()
-                    used -= self.bytes_of(a.req)
-                else:
-                    still.append(a)
-            active = still
-            if not progressed and active:
-                time.sleep(20e-6)
-            elif not active and pending:
-                wait = pending[0].arrival - (self.clock() - t0)
-                if wait > 0:
-                    time.sleep(min(wait, 1e-3))
+        try:
+            while lc.waiting() or active:
+                progressed = lc.reap()
+                active += lc.admit(len(active))
+                for a in active:
+                    if a.issued < a.req.nsteps:
+                        n1 = min(a.req.nsteps, a.issued + (a.req.nsteps if sequential else self.burst))
+                        for n in range(a.issued + 1, n1 + 1):
+                            a.dom.step(n, a.req.nsteps)
+                        a.issued = n1
+                        progressed = True
+                still = []
+                for a in active:
+                    if a.issued >= a.req.nsteps and a.dom.done():
+                        results.append(Result(a.req.rid, a.req.cells, a.req.nsteps, a.req.arrival, a.admitted,
+                                              lc.now()))
+                        lc.release(a)
+                        progressed = True
+                    else:
+                        still.append(a)
+                active = still
+                if not progressed:
+                    lc.idle_wait(bool(active))
+        finally:
+            lc.finish()
         return sorted(results, key=lambda r: r.rid)
 
 

@@ -38,11 +38,11 @@ class FakeDomain:
         self.log.append((self.req.rid, tuple(self.steps)))
 
 
-def make(budget, reqs, sizes, burst=3, max_active=64):
+def make(budget, reqs, sizes, burst=3, max_active=64, workers=1):
     log = []
     srv = serving.Server(budget, burst=burst, max_active=max_active,
                          domain_factory=lambda r: FakeDomain(r, log), bytes_of=lambda r: sizes[r.rid],
-                         clock=FakeClock())
+                         clock=FakeClock(), workers=workers)
     return srv, log
 
 
@@ -50,10 +50,11 @@ def _reqs(n, nsteps=7):
     return [serving.Request(r, serving.setup_gen.Workload(f"r{r}", 8, 8, 4), nsteps) for r in range(n)]
 
 
-def test_every_request_runs_every_step_in_order():
+@pytest.mark.parametrize("workers", [0, 1])
+def test_every_request_runs_every_step_in_order(workers):
     FakeDomain.live = FakeDomain.peak = 0
     reqs = _reqs(5)
-    srv, log = make(10 ** 9, reqs, {r.rid: 100 for r in reqs})
+    srv, log = make(10 ** 9, reqs, {r.rid: 100 for r in reqs}, workers=workers)
     res = srv.run(reqs)
     assert [r.rid for r in res] == list(range(5))
     assert sorted(log) == [(r, tuple(range(1, 8))) for r in range(5)]
@@ -61,11 +62,12 @@ def test_every_request_runs_every_step_in_order():
     assert FakeDomain.peak == 5
 
 
-def test_admission_never_exceeds_the_memory_budget():
+@pytest.mark.parametrize("workers", [0, 1])
+def test_admission_never_exceeds_the_memory_budget(workers):
     FakeDomain.live = FakeDomain.peak = 0
     reqs = _reqs(6)
     sizes = {0: 60, 1: 50, 2: 40, 3: 30, 4: 70, 5: 10}
-    srv, log = make(100, reqs, sizes)
+    srv, log = make(100, reqs, sizes, workers=workers)
     srv.run(reqs)
     # first come first served: 0 (60) then 1 would make 110 > 100, so only 0 runs alone
     assert FakeDomain.peak <= 3
@@ -79,10 +81,11 @@ def test_request_larger_than_budget_is_refused():
         srv.run(reqs)
 
 
-def test_sequential_mode_runs_one_domain_at_a_time():
+@pytest.mark.parametrize("workers", [0, 1])
+def test_sequential_mode_runs_one_domain_at_a_time(workers):
     FakeDomain.live = FakeDomain.peak = 0
     reqs = _reqs(4)
-    srv, log = make(10 ** 9, reqs, {r.rid: 1 for r in reqs})
+    srv, log = make(10 ** 9, reqs, {r.rid: 1 for r in reqs}, workers=workers)
     srv.run(reqs, sequential=True)
     assert FakeDomain.peak == 1
     assert [r for r, _ in log] == [0, 1, 2, 3]
@@ -126,14 +129,15 @@ class FakeBatch:
         self.blog.append(("close", tuple(d.req.rid for d in self.doms)))
 
 
-def test_batched_serving_steps_every_request_in_order_within_the_budget():
+@pytest.mark.parametrize("workers", [0, 1])
+def test_batched_serving_steps_every_request_in_order_within_the_budget(workers):
     FakeDomain.live = FakeDomain.peak = 0
     reqs = [serving.Request(r, serving.setup_gen.Workload(f"r{r}", 8, 8, 4), 5 + r) for r in range(6)]
     sizes = {0: 60, 1: 30, 2: 40, 3: 30, 4: 70, 5: 10}
     log, blog = [], []
     srv = serving.Server(100, burst=3, domain_factory=lambda r: FakeDomain(r, log),
                          bytes_of=lambda r: sizes[r.rid], clock=FakeClock(),
-                         batch_factory=lambda doms: FakeBatch(doms, blog))
+                         batch_factory=lambda doms: FakeBatch(doms, blog), workers=workers)
     res = srv.run_batched(reqs)
     assert [r.rid for r in res] == list(range(6))
     assert sorted(log) == [(r, tuple(range(1, 6 + r))) for r in range(6)]
@@ -142,3 +146,22 @@ def test_batched_serving_steps_every_request_in_order_within_the_budget():
     assert all(sum(sizes[r] for r in b) <= 100 for b in opens)
     assert len(opens) == sum(1 for kind, _ in blog if kind == "close")
     assert FakeDomain.live == 0
+
+
+@pytest.mark.parametrize("workers", [0, 1])
+def test_batched_serving_steps_large_domains_on_their_own(workers):
+    FakeDomain.live = FakeDomain.peak = 0
+    # requests 1, 3, 5 have 512 cells, the others 256
+    reqs = [serving.Request(r, serving.setup_gen.Workload(f"r{r}", 8, 8, 4 * (1 + r % 2)), 4 + r) for r in range(6)]
+    log, blog = [], []
+    srv = serving.Server(1000, burst=2, domain_factory=lambda r: FakeDomain(r, log),
+                         bytes_of=lambda r: 10, clock=FakeClock(),
+                         batch_factory=lambda doms: FakeBatch(doms, blog), workers=workers)
+    res = srv.run_batched(reqs, max_batch_cells=256)
+    assert [r.rid for r in res] == list(range(6))
+    assert sorted(log) == [(r, tuple(range(1, 5 + r))) for r in range(6)]
+    opens = [b for kind, b in blog if kind == "open"]
+    assert opens and all(r % 2 == 0 for b in opens for r in b)
+    assert len(opens) == sum(1 for kind, _ in blog if kind == "close")
+    assert FakeDomain.live == 0
+    assert all(r.finished >= r.admitted >= r.arrival for r in res)
