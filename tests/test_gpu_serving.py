@@ -4,7 +4,9 @@
   cn1 evaluated from its separable definition (init.f95:100-107) give the
   same masks and the same steps, bit for bit, as the host-built arrays;
 * a domain served together with others on concurrent streams produces the
-  same fields, bit for bit, as the same domain run alone.
+  same fields, bit for bit, as the same domain run alone;
+* a domain on device memory kept from a destroyed one steps bit for bit as
+  on fresh memory.
 """
 import numpy as np
 import pytest
@@ -65,3 +67,28 @@ def test_domain_alone_equals_domain_in_the_mix():
         for k in FIELDS:
             assert np.array_equal(alone.download(k), finals[r.rid][k]), (r.rid, k)
         alone.close()
+
+
+@pytest.mark.parametrize("graph", [False, True], ids=["eager", "graph"])
+def test_domain_on_reused_memory_steps_like_a_fresh_one(graph):
+    """A destroyed domain's device memory is kept and handed to the next
+    domain of the same extents with its last contents still in it; the new
+    domain must step exactly as the first did on fresh memory (set-up clears
+    what must start at zero)."""
+    w = setup_gen.Workload("reuse", 48, 40, 24, seed=21)
+    s = setup_gen.build(w, dense=False)
+    print(f"\n[memory reuse] 48x40x24 graph={graph}", flush=True)
+    api.release_all()  # no kept memory: the first domain gets fresh blocks
+    runs = []
+    for _ in range(2):
+        d = api.Domain(s)
+        d.set_graph(graph)
+        for n in range(1, 6):
+            d.step(n, 5)
+        runs.append(({f: d.download(f) for f in FIELDS + ["rhs", "sm"]}, d.press_stats()))
+        d.close()
+    (a, sa), (b, sb) = runs
+    for f in a:
+        assert np.array_equal(a[f], b[f]), f
+    assert sa["iters"] == sb["iters"]
+    assert sa["sor"].tolist() == sb["sor"].tolist()
