@@ -317,8 +317,8 @@ def main(argv=None):
     dom.synchronize()
     t1 = time.perf_counter()
     st = dom.press_stats()
-    # the dominant kernel (SOR iterations 2..L, class "sor"; iteration 1 is
-    # class "sor1") bracketed by HIP events on the domain's stream over K
+    # the dominant kernel (the SOR iterations, class "sor"; iteration 1 on
+    # entry ghosts is class "sor1") bracketed by HIP events on the domain's stream over K
     # more steps launched eagerly (events cannot bracket kernels inside a
     # graph), one event pair per launch: the pair's marker packets add a
     # few us to each launch.  The kernel's own duration is then timed with
@@ -336,8 +336,10 @@ def main(argv=None):
         dom.step(n, nmax)
     kt_all = dom.timing()
     dom.set_timing(False)
-    if st["nonfinite"]:
-        print("bench.py: non-finite SOR residual", file=sys.stderr)
+    # a non-finite state makes every later SOR launch exit at once (its stop
+    # flag is set), which would time empty launches: refuse to report them
+    if st["nonfinite"] or dom.press_stats()["nonfinite"]:
+        print("bench.py: non-finite SOR residual (the flow diverged); no figures reported", file=sys.stderr)
         return 1
     elapsed = t1 - t0
     sps = K / elapsed
@@ -349,9 +351,14 @@ def main(argv=None):
     alg_bytes = CLASS_BYTES.get(dom_cls, 0) * w.cells
     if dom_cls == "sor":
         # a launch is one colour sweep (half an iteration) or one fused
-        # iteration: price it by the SOR iterations it carries (every timed
-        # step ran L iterations, of which the "sor1" launches carried one each)
-        iters = L * K - (kt["sor1"][1] if "sor1" in kt else 0)
+        # iteration: price it by the SOR iterations it carries.  Every timed
+        # step ran L iterations.  Iteration 1 is either a "sor" launch (p's
+        # ghosts equal their boundp copies, as after any press; the step's
+        # "sor1" launch then exits at once) or the "sor1" launch itself.
+        if dom_n == L * K:
+            iters = dom_n
+        else:
+            iters = L * K - (kt["sor1"][1] if "sor1" in kt else 0)
         alg_bytes = SOR_BYTES_PER_ITER * w.cells * iters / dom_n
     in_step_us = avg_s * 1e6
     timing_method = "one HIP event pair per launch in K eager steps"

@@ -74,6 +74,7 @@ __device__ __forceinline__ void rhs_body(const VBlk& vb, const Grid& g, const Me
     ctrl->sor_done = 0;
     ctrl->sor_iters = 0;
     ctrl->nonfinite = 0;
+    ctrl->p_ghost_diff = 0;
   }
   const int i = 1 + vb.x * BX + threadIdx.x, j = 1 + vb.y * BY + threadIdx.y;
   double acc = 0.0;
@@ -134,17 +135,24 @@ __global__ __launch_bounds__(256) void k_finalize(const double* __restrict__ par
 
 // rhs = rhs - rhsav (press.f95:81-87, the subtraction in FP64), and the
 // colour-split copies the SOR reads: rhs, the starting iterate p, and cn1
-// when it is not recomputed.
+// when it is not recomputed.  Also compares, bit for bit, every ghost of p
+// next to the interior with the value boundp gives it (i and k: the
+// neighbouring interior cell, boundp.f95:7-78; j: the periodic image) and
+// raises ctrl->p_ghost_diff on any difference.  Without one, SOR iteration 1
+// reading the entry ghosts equals an iteration that mirrors the current
+// values, so it can run as any later iteration.
 __device__ __forceinline__ void rhs_split_body(const VBlk& vb, const Grid& g, const SplitGrid& s,
                                                float* __restrict__ rhs, const float* __restrict__ p,
                                                const float* __restrict__ cn1, const SplitBufs& sb,
-                                               const Ctrl* ctrl) {
+                                               Ctrl* ctrl) {
   const int i = 1 + vb.x * BX + threadIdx.x, j = 1 + vb.y * BY + threadIdx.y;
   if (i > g.im || j > g.jm) return;
   const double av = ctrl->rhsav;
   const int t = (i - 1) >> 1;
   int k0, k1;
   kspan(vb, 1, g.km, k0, k1);
+  bool diff = false;
+  const auto same = [](float a, float b) { return __float_as_uint(a) == __float_as_uint(b); };
   for (int k = k0; k <= k1; k++) {
     const int c = ix(g, i, j, k);
     const int col = (i + j + k) & 1;  // 1 = red
@@ -152,13 +160,21 @@ __device__ __forceinline__ void rhs_split_body(const VBlk& vb, const Grid& g, co
     rhs[c] = r;
     const int q = cix(s, t, j, k);
     sb.rhs[col][q] = r;
-    sb.p[0][col][q] = p[c];
+    const float pc = p[c];
+    sb.p[0][col][q] = pc;
     if (sb.cn1[0]) sb.cn1[col][q] = cn1[c];
+    if (i == 1) diff |= !same(p[c - 1], pc);
+    if (i == g.im) diff |= !same(p[c + 1], pc);
+    if (j == 1) diff |= !same(p[ix(g, i, 0, k)], p[ix(g, i, g.jm, k)]);
+    if (j == g.jm) diff |= !same(p[ix(g, i, g.jm + 1, k)], p[ix(g, i, 1, k)]);
+    if (k == 1) diff |= !same(p[c - g.sk], pc);
+    if (k == g.km) diff |= !same(p[c + g.sk], pc);
   }
+  if (diff) ctrl->p_ghost_diff = 1;
 }
 __global__ __launch_bounds__(NT) void k_rhs_split(Grid g, SplitGrid s, float* __restrict__ rhs,
                                                   const float* __restrict__ p, const float* __restrict__ cn1,
-                                                  SplitBufs sb, const Ctrl* ctrl) {
+                                                  SplitBufs sb, Ctrl* ctrl) {
   rhs_split_body(hw_blk<false>(), g, s, rhs, p, cn1, sb, ctrl);
 }
 
@@ -366,13 +382,15 @@ __device__ __forceinline__ unsigned boff(const float* base, const float* p) {
 
 // two blocks per CU need at most 64 VGPRs per lane (8 waves per SIMD)
 // ENTRY: iteration 1, which reads the ghost values p had on entry to press
-// (pent); the other iterations are compiled without that code.
+// (pent); the other iterations are compiled without that code.  gate != 0:
+// an ENTRY launch that runs only when ctrl->p_ghost_diff is raised (a
+// launch of k_sor_iter2 for iteration 1 takes the other case).
 // bid: the block's index within the launch of this domain.
 template <bool SEP, bool ENTRY>
 __device__ __forceinline__ void sor_iter_block(const Grid& g, const SplitGrid& s, const Metrics& m,
                                                const SplitBufs& sb, Ctrl* ctrl, int iter, int nparts, float omega,
                                                float pjuge, int ntt, int ntj, int kchunk, int remap,
-                                               const float* __restrict__ pent, int nprev, int bid) {
+                                               const float* __restrict__ pent, int nprev, int gate, int bid) {
   __shared__ float ldsB[4][FY][FX];  // previous black iterate, levels mod 4
   __shared__ float ldsR[2][FY][FX];  // new red values, levels mod 2
   __shared__ double lds[FNT / kWave];
@@ -382,7 +400,7 @@ __device__ __forceinline__ void sor_iter_block(const Grid& g, const SplitGrid& s
   // a wave is one row of the block (blockDim.x == 64): row-derived values are wave-uniform
   const int row = __builtin_amdgcn_readfirstlane(threadIdx.y);
   const int tid = row * FX + lane;
-  if (tid == 0) skip = (ctrl->sor_done != 0);
+  if (tid == 0) skip = (ctrl->sor_done != 0) || (ENTRY && gate && ctrl->p_ghost_diff == 0);
   __syncthreads();
   if (skip) return;
   if (iter >= 2 && bid == 0) {  // stop test of the previous iteration (press.f95:119-121)
@@ -576,9 +594,9 @@ __device__ __forceinline__ void sor_iter_block(const Grid& g, const SplitGrid& s
 template <bool SEP, bool ENTRY>
 __global__ __launch_bounds__(FNT) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_sor_iter(
     Grid g, SplitGrid s, Metrics m, SplitBufs sb, Ctrl* ctrl, int iter, int nparts, float omega, float pjuge,
-    int ntt, int ntj, int kchunk, int remap, const float* __restrict__ pent, int nprev) {
+    int ntt, int ntj, int kchunk, int remap, const float* __restrict__ pent, int nprev, int gate) {
   sor_iter_block<SEP, ENTRY>(g, s, m, sb, ctrl, iter, nparts, omega, pjuge, ntt, ntj, kchunk, remap, pent, nprev,
-                             (int)blockIdx.x);
+                             gate, (int)blockIdx.x);
 }
 
 // ------------------------------- fused SOR iteration, two rows per thread
@@ -611,7 +629,8 @@ __global__ __launch_bounds__(FNT) __attribute__((amdgpu_waves_per_eu(8, 8))) voi
 // rows, which fits narrow domains better).  The block has TY * W / 128
 // waves.  nprev: partial-sum slots written by the previous iteration's
 // launch.  One block's share of the iteration: bid is the block's index
-// within the launch of this domain.
+// within the launch of this domain.  Iteration 1 runs here only when
+// ctrl->p_ghost_diff is clear (then it equals the entry-ghost iteration).
 template <bool SEP, int TY, bool OM1, int W>
 __device__ __forceinline__ void sor_iter2_block(Grid g, SplitGrid s, Metrics m, const SplitBufs& sb, Ctrl* ctrl, int iter,
                                                 int nparts, int nprev, float omega, float pjuge, int ntt, int ntj,
@@ -628,7 +647,7 @@ __device__ __forceinline__ void sor_iter2_block(Grid g, SplitGrid s, Metrics m, 
   // lane = pair of the tile row; vw = the thread's pair of rows (wave-uniform when W == FX)
   const int lane = (W == FX) ? (int)threadIdx.x : (int)threadIdx.x % W;
   const int vw = (W == FX) ? w : w * (FX / W) + (int)threadIdx.x / W;
-  if (tid == 0) skip = (ctrl->sor_done != 0);
+  if (tid == 0) skip = (ctrl->sor_done != 0) || (iter == 1 && ctrl->p_ghost_diff != 0);
   __syncthreads();
   if (skip) return;
   if (iter >= 2 && bid == 0) {  // stop test of the previous iteration (press.f95:119-121)
@@ -1236,9 +1255,10 @@ static SorShape sor_shape(const DevDomain& d) {
 
 // One fused SOR iteration l with shape h; nprev is the previous launch's
 // number of partial-sum slots; pent != nullptr for iteration 1 of press,
-// which reads the entry ghosts of p (k_sor_iter only).
+// which reads the entry ghosts of p (k_sor_iter only); gate: that launch
+// runs only when a ghost of p differs from its boundp copy.
 static void launch_sor_iter(const DevDomain& d, const SplitBufs& sb, int l, float pjuge, const SorShape& h, int nprev,
-                            const float* pent = nullptr) {
+                            const float* pent = nullptr, int gate = 0) {
   const Grid& g = d.g;
   const SplitGrid& s = d.sg;
   if (h.rows == 1 || pent) {
@@ -1246,7 +1266,8 @@ static void launch_sor_iter(const DevDomain& d, const SplitBufs& sb, int l, floa
 #define SOR1(SEP, ENT)                                                                                          \
   TIMED(d, ENT ? KC_SOR1 : KC_SOR,                                                                              \
         hipLaunchKernelGGL((k_sor_iter<SEP, ENT>), dim3(h.nparts), dim3(FX, FY), lvm_bytes, d.stream, g, s, d.m, \
-                           sb, d.ctrl, l, h.nparts, d.omega, pjuge, h.ntt, h.ntj, h.kc, d.xcd_remap, pent, nprev))
+                           sb, d.ctrl, l, h.nparts, d.omega, pjuge, h.ntt, h.ntj, h.kc, d.xcd_remap, pent, nprev, \
+                           gate))
     if (d.cn1_separable) {
       if (pent)
         SOR1(true, true);
@@ -1342,7 +1363,7 @@ static void press_head(const DevDomain& d, float dt, const SplitBufs& sb, int gx
                                         d.area));
   }
   TIMED(d, KC_RHS, hipLaunchKernelGGL(k_rhs_split, dim3(gx, gy, g.km), blk(), 0, d.stream, g, s, d.F.rhs,
-                                      (const float*)d.F.p, (const float*)d.F.cn1, sb, (const Ctrl*)d.ctrl));
+                                      (const float*)d.F.p, (const float*)d.F.cn1, sb, d.ctrl));
 }
 
 bool sor_batchable(const DevDomain& d) {
@@ -1454,7 +1475,7 @@ __global__ __launch_bounds__(FNT) __attribute__((amdgpu_waves_per_eu(8, 8))) voi
     const BatchDom* __restrict__ bd, const StageSpan* __restrict__ sp, int nd) {
   BATCH_BLOCK(BS_SOR1, false);
   sor_iter_block<true, true>(d.g, d.sg, d.m, B.sb, d.ctrl, 1, B.nparts1, d.omega, d.pjuge, B.ntt1, B.ntj1, B.kc1,
-                             d.xcd_remap, d.F.p, B.nparts1, vb.x);
+                             d.xcd_remap, d.F.p, B.nparts1, 0, vb.x);
 }
 // one block per domain
 __global__ __launch_bounds__(256) void kb_sor_final(const BatchDom* __restrict__ bd) {
@@ -1531,9 +1552,18 @@ void launch_press(const DevDomain& d, float dt) {
   }
   if (d.sor_fused) {
     const SorShape h = sor_shape(d);
-    // iteration 1 reads the ghost values p had on entry (k_sor_iter, 16-row tiles)
+    // iteration 1 reads the ghost values p had on entry (k_sor_iter, 16-row
+    // tiles).  When the iteration's own kernel has the same tiles and chunks
+    // (so the same partial-sum slots), both are launched: it runs iteration
+    // 1 when every ghost equals its boundp copy, as after any press, and the
+    // entry-ghost kernel only otherwise (ctrl->p_ghost_diff).
     const SorShape h1 = sor_shape(g, s, 1, FY, h.kc);
-    launch_sor_iter(d, sb, 1, d.pjuge, h1, h1.nparts, d.F.p);
+    if (h.rows == 2 && h.ty == FY && h.tw == FX && h.nparts == h1.nparts && d.sor_entry_fast) {
+      launch_sor_iter(d, sb, 1, d.pjuge, h1, h1.nparts, d.F.p, 1);
+      launch_sor_iter(d, sb, 1, d.pjuge, h, h1.nparts);
+    } else {
+      launch_sor_iter(d, sb, 1, d.pjuge, h1, h1.nparts, d.F.p);
+    }
     for (int l = 2; l <= d.nmaxp; l++) launch_sor_iter(d, sb, l, d.pjuge, h, l == 2 ? h1.nparts : h.nparts);
     launch_press_tail(d, sb, d.nmaxp == 1 ? h1.nparts : h.nparts, gx, gy, gz, nred);
     return;

@@ -59,6 +59,7 @@ struct Ctrl {
   int sor_done;            // SOR iteration (1-based) whose residual fell below pjuge, 0 = none
   int sor_iters;           // iterations executed in the current press call
   int nonfinite;           // a non-finite residual was met
+  int p_ghost_diff;        // a ghost cell of p differs from the copy boundp would make (set by the rhs split)
   double rhsav;            // volume-weighted mean of rhs
   double pav;              // volume-weighted mean of p
   double sor_first;        // sum of reltmp^2 of the current iteration's first colour

@@ -79,6 +79,7 @@ struct DevDomain {
   int sor_kchunk;    // k levels per block of the fused SOR iteration (0: from sor_fused_blocks)
   int sor_tile_rows; // tile rows (with the ring) of k_sor_iter2: 16 or 32
   int sor_tile_width;  // tile width in pairs (with the ring) of k_sor_iter2: 64, or 32 with 32 rows
+  int sor_entry_fast;  // SOR iteration 1 on k_sor_iter2 when p's ghosts equal their boundp copies
 };
 
 int sor_kchunk(const Grid& g, const SplitGrid& s, int target);

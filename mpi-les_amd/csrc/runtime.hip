@@ -647,6 +647,7 @@ int domain_alloc(int im, int jm, int km, les_domain** out) {
   D->d.sor_kchunk = std::max(0, env_int("LES_SOR_KCHUNK", 0));
   D->d.sor_tile_rows = env_int("LES_SOR_TILE_ROWS", 16) == 32 ? 32 : 16;
   D->d.sor_tile_width = (D->d.sor_tile_rows == 32 && env_int("LES_SOR_TILE_WIDTH", 64) == 32) ? 32 : 64;
+  D->d.sor_entry_fast = env_int("LES_SOR_ENTRY_FAST", 1) != 0;
   {
     int dev = 0, ncu = 0;
     if (hipGetDevice(&dev) == hipSuccess &&
