@@ -325,8 +325,10 @@ def main(argv=None):
     # one pair around a run of back-to-back launches (below).
     dom.set_timing(True, ["sor", "sor1"])
     dom.timing()
+    iters_run = []  # SOR iterations each of these steps executed (read from the device)
     for n in range(W + K + 2, W + 2 * K + 2):
         dom.step(n, nmax)
+        iters_run.append(dom.press_stats()["iters"])
     kt = dom.timing()
     # short second pass with every launch timed: the per-kernel table
     dom.set_timing(True)
@@ -349,20 +351,21 @@ def main(argv=None):
     dom_ms, dom_n = kt[dom_cls]
     avg_s = dom_ms / 1e3 / dom_n
     alg_bytes = CLASS_BYTES.get(dom_cls, 0) * w.cells
+    fused = os.environ.get("LES_SOR_FUSED", "1") != "0"
     if dom_cls == "sor":
-        # a launch is one colour sweep (half an iteration) or one fused
-        # iteration: price it by the SOR iterations it carries.  Every timed
-        # step ran L iterations.  Iteration 1 is either a "sor" launch (p's
-        # ghosts equal their boundp copies, as after any press; the step's
-        # "sor1" launch then exits at once) or the "sor1" launch itself.
-        if dom_n == L * K:
-            iters = dom_n
-        else:
-            iters = L * K - (kt["sor1"][1] if "sor1" in kt else 0)
-        alg_bytes = SOR_BYTES_PER_ITER * w.cells * iters / dom_n
+        # Class "sor" holds the launches of iterations 2..nmaxp of each step
+        # (iteration 1 is class "sor1"), one fused iteration per launch, or
+        # two colour sweeps per iteration (LES_SOR_FUSED=0, where iteration 1
+        # is in "sor").  Launches after a step's stop test exit at once: the
+        # iterations that ran come from each step's own count, and the
+        # launches' total time is divided among them only.
+        iters = sum(it - 1 for it in iters_run) if fused else sum(iters_run)
+        per_launch = 1 if fused else 2
+        alg_bytes = SOR_BYTES_PER_ITER * w.cells / per_launch
+        avg_s = dom_ms / 1e3 / max(1, iters * per_launch)
     in_step_us = avg_s * 1e6
     timing_method = "one HIP event pair per launch in K eager steps"
-    if dom_cls == "sor" and iters == dom_n:
+    if dom_cls == "sor" and fused and iters == dom_n:
         # one fused iteration per launch: its duration from SOR_RUN
         # back-to-back launches of the same kernel and launch shape on the
         # domain's stream between one event pair (les_domain_time_sor)

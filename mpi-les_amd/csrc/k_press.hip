@@ -629,12 +629,15 @@ __global__ __launch_bounds__(FNT) __attribute__((amdgpu_waves_per_eu(8, 8))) voi
 // rows, which fits narrow domains better).  The block has TY * W / 128
 // waves.  nprev: partial-sum slots written by the previous iteration's
 // launch.  One block's share of the iteration: bid is the block's index
-// within the launch of this domain.  Iteration 1 runs here only when
+// within the launch of this domain.  gate1 != 0: this is press's iteration
+// 1 launched beside the entry-ghost kernel, and runs only when
 // ctrl->p_ghost_diff is clear (then it equals the entry-ghost iteration).
+// Other iteration-1 launches (the inner solves of press mode 1, which start
+// from e = 0 with mirrored ghosts) pass gate1 == 0 and always run.
 template <bool SEP, int TY, bool OM1, int W>
 __device__ __forceinline__ void sor_iter2_block(Grid g, SplitGrid s, Metrics m, const SplitBufs& sb, Ctrl* ctrl, int iter,
                                                 int nparts, int nprev, float omega, float pjuge, int ntt, int ntj,
-                                                int kchunk, int remap, int bid) {
+                                                int kchunk, int remap, int bid, int gate1) {
   static_assert(W == FX || (W == FX / 2 && TY == 32), "tile shapes: 16 or 32 rows x 64 pairs, 32 x 32");
   constexpr int GNT = W * TY / 2, FTYT = TY - 4, FTXW = W - 4;
   __shared__ float ldsB[3][TY][W];  // previous black iterate of level kr (planes by level mod 3)
@@ -647,7 +650,7 @@ __device__ __forceinline__ void sor_iter2_block(Grid g, SplitGrid s, Metrics m, 
   // lane = pair of the tile row; vw = the thread's pair of rows (wave-uniform when W == FX)
   const int lane = (W == FX) ? (int)threadIdx.x : (int)threadIdx.x % W;
   const int vw = (W == FX) ? w : w * (FX / W) + (int)threadIdx.x / W;
-  if (tid == 0) skip = (ctrl->sor_done != 0) || (iter == 1 && ctrl->p_ghost_diff != 0);
+  if (tid == 0) skip = (ctrl->sor_done != 0) || (gate1 && iter == 1 && ctrl->p_ghost_diff != 0);
   __syncthreads();
   if (skip) return;
   if (iter >= 2 && bid == 0) {  // stop test of the previous iteration (press.f95:119-121)
@@ -928,9 +931,9 @@ __device__ __forceinline__ void sor_iter2_block(Grid g, SplitGrid s, Metrics m, 
 template <bool SEP, int TY, bool OM1, int W>
 __global__ __launch_bounds__(W * TY / 2) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_sor_iter2(
     Grid g, SplitGrid s, Metrics m, SplitBufs sb, Ctrl* ctrl, int iter, int nparts, int nprev, float omega,
-    float pjuge, int ntt, int ntj, int kchunk, int remap) {
+    float pjuge, int ntt, int ntj, int kchunk, int remap, int gate1) {
   sor_iter2_block<SEP, TY, OM1, W>(g, s, m, sb, ctrl, iter, nparts, nprev, omega, pjuge, ntt, ntj, kchunk, remap,
-                                   (int)blockIdx.x);
+                                   (int)blockIdx.x, gate1);
 }
 
 // The same iteration for several domains in one launch: each domain owns a
@@ -948,7 +951,7 @@ __global__ __launch_bounds__(W * TY / 2) __attribute__((amdgpu_waves_per_eu(4, 4
   const int bid = (int)blockIdx.x - E.block0;
   if (bid >= E.nparts || iter > E.nmaxp) return;  // padding block, or this domain has finished its iterations
   sor_iter2_block<true, TY, OM1, W>(E.g, E.s, E.m, E.sb, E.ctrl, iter, E.nparts, iter == 2 ? E.nparts1 : E.nparts,
-                                    E.omega, E.pjuge, E.ntt, E.ntj, E.kchunk, remap, bid);
+                                    E.omega, E.pjuge, E.ntt, E.ntj, E.kchunk, remap, bid, 0);
 }
 
 void launch_sor_batch(const SorBatchEntry* tab, const int* block0, int ntab, int nblocks, int iter, int kcmax,
@@ -1255,8 +1258,10 @@ static SorShape sor_shape(const DevDomain& d) {
 
 // One fused SOR iteration l with shape h; nprev is the previous launch's
 // number of partial-sum slots; pent != nullptr for iteration 1 of press,
-// which reads the entry ghosts of p (k_sor_iter only); gate: that launch
-// runs only when a ghost of p differs from its boundp copy.
+// which reads the entry ghosts of p (k_sor_iter only); gate: with pent, that
+// launch runs only when a ghost of p differs from its boundp copy; without
+// pent (k_sor_iter2, iteration 1 of press beside the gated entry-ghost
+// launch), only when none does.
 static void launch_sor_iter(const DevDomain& d, const SplitBufs& sb, int l, float pjuge, const SorShape& h, int nprev,
                             const float* pent = nullptr, int gate = 0) {
   const Grid& g = d.g;
@@ -1264,7 +1269,7 @@ static void launch_sor_iter(const DevDomain& d, const SplitBufs& sb, int l, floa
   if (h.rows == 1 || pent) {
     const size_t lvm_bytes = 3 * (size_t)(h.kc + 2) * sizeof(float);
 #define SOR1(SEP, ENT)                                                                                          \
-  TIMED(d, ENT ? KC_SOR1 : KC_SOR,                                                                              \
+  TIMED(d, (ENT || l == 1) ? KC_SOR1 : KC_SOR,                                                                              \
         hipLaunchKernelGGL((k_sor_iter<SEP, ENT>), dim3(h.nparts), dim3(FX, FY), lvm_bytes, d.stream, g, s, d.m, \
                            sb, d.ctrl, l, h.nparts, d.omega, pjuge, h.ntt, h.ntj, h.kc, d.xcd_remap, pent, nprev, \
                            gate))
@@ -1283,18 +1288,19 @@ static void launch_sor_iter(const DevDomain& d, const SplitBufs& sb, int l, floa
     return;
   }
   const size_t lvm_bytes = (size_t)(h.kc + 2) * sizeof(float4);
+  const int cls = l == 1 ? KC_SOR1 : KC_SOR;  // iteration 1 (entry ghosts or not) is timed apart
 #define SOR2(SEP, TY, W)                                                                                     \
   do {                                                                                                       \
     if (d.omega == 1.0f)                                                                                     \
-      TIMED(d, KC_SOR, hipLaunchKernelGGL((k_sor_iter2<SEP, TY, true, W>), dim3(h.nparts),                   \
+      TIMED(d, cls, hipLaunchKernelGGL((k_sor_iter2<SEP, TY, true, W>), dim3(h.nparts),                   \
                                           dim3(FX, TY * W / (2 * FX)), lvm_bytes, d.stream, g, s, d.m, sb,   \
                                           d.ctrl, l, h.nparts, nprev, d.omega, pjuge, h.ntt, h.ntj, h.kc,    \
-                                          d.xcd_remap));                                                     \
+                                          d.xcd_remap, gate));                                               \
     else                                                                                                     \
-      TIMED(d, KC_SOR, hipLaunchKernelGGL((k_sor_iter2<SEP, TY, false, W>), dim3(h.nparts),                  \
+      TIMED(d, cls, hipLaunchKernelGGL((k_sor_iter2<SEP, TY, false, W>), dim3(h.nparts),                  \
                                           dim3(FX, TY * W / (2 * FX)), lvm_bytes, d.stream, g, s, d.m, sb,   \
                                           d.ctrl, l, h.nparts, nprev, d.omega, pjuge, h.ntt, h.ntj, h.kc,    \
-                                          d.xcd_remap));                                                     \
+                                          d.xcd_remap, gate));                                               \
   } while (0)
   if (d.cn1_separable) {
     if (h.tw == FX / 2)
@@ -1560,7 +1566,7 @@ void launch_press(const DevDomain& d, float dt) {
     const SorShape h1 = sor_shape(g, s, 1, FY, h.kc);
     if (h.rows == 2 && h.ty == FY && h.tw == FX && h.nparts == h1.nparts && d.sor_entry_fast) {
       launch_sor_iter(d, sb, 1, d.pjuge, h1, h1.nparts, d.F.p, 1);
-      launch_sor_iter(d, sb, 1, d.pjuge, h, h1.nparts);
+      launch_sor_iter(d, sb, 1, d.pjuge, h, h1.nparts, nullptr, 1);
     } else {
       launch_sor_iter(d, sb, 1, d.pjuge, h1, h1.nparts, d.F.p);
     }

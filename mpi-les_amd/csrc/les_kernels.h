@@ -39,7 +39,7 @@ struct SplitBufs {
 enum KClass {
   KC_VELNW, KC_BONDV1, KC_VEL2, KC_VEL2_GHOST, KC_VELFG, KC_FEEDBF, KC_LES_SM, KC_BOUNDSM,
   KC_LES_VISC, KC_ADAM, KC_RHS, KC_SOR, KC_BOUNDP, KC_PAV, KC_PADJ, KC_COPY,
-  KC_SOR1,  // SOR iteration 1 of a press call (k_sor_iter reading the entry ghosts)
+  KC_SOR1,  // SOR iteration 1 of a press call (either kernel; KC_SOR: iterations 2..nmaxp)
   KC_N
 };
 

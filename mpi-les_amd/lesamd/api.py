@@ -290,6 +290,11 @@ class Domain:
     def press_stats(self):
         return _stats(lib().les_domain_press_stats, self._h)
 
+    def nonfinite(self) -> bool:
+        """True when the last pressure solve met a non-finite residual (the
+        state has diverged).  Waits for the domain's stream."""
+        return self.press_stats()["nonfinite"] != 0
+
     def autotune(self, reps: int = 0) -> dict:
         """Pick the SOR launch shape for this grid shape (cached per shape;
         reps > 0 times it again with that many iterations per candidate)."""

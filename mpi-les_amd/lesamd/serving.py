@@ -48,10 +48,29 @@ class Result:
     arrival: float
     admitted: float
     finished: float
+    # "ok", or "failed: ..." when the domain's state was not finite at its
+    # last pressure solve (the simulation diverged).  A failed request fails
+    # alone: the others sharing the device and the batch are unaffected.
+    status: str = "ok"
 
     @property
     def latency(self) -> float:
         return self.finished - self.arrival
+
+    @property
+    def ok(self) -> bool:
+        return self.status == "ok"
+
+
+def _status(dom) -> str:
+    """The request's outcome from its finished domain: a non-finite
+    pressure-solve residual (the SOR's stop-test sum, press.f95:119-121, is
+    NaN or infinite) marks a diverged state.  Domain objects without the
+    check (stand-ins) count as ok."""
+    check = getattr(dom, "nonfinite", None)
+    if check is not None and check():
+        return "failed: non-finite state (diverged)"
+    return "ok"
 
 
 def sample_requests(n: int, nsteps: int, rate: float | None = None, seed: int = 0,
@@ -243,7 +262,8 @@ class Server:
         results = []
 
         def done_with(a):
-            results.append(Result(a.req.rid, a.req.cells, a.req.nsteps, a.req.arrival, a.admitted, lc.now()))
+            results.append(Result(a.req.rid, a.req.cells, a.req.nsteps, a.req.arrival, a.admitted, lc.now(),
+                                  _status(a.dom)))
             lc.release(a)
 
         try:
@@ -323,7 +343,7 @@ class Server:
                 for a in active:
                     if a.issued >= a.req.nsteps and a.dom.done():
                         results.append(Result(a.req.rid, a.req.cells, a.req.nsteps, a.req.arrival, a.admitted,
-                                              lc.now()))
+                                              lc.now(), _status(a.dom)))
                         lc.release(a)
                         progressed = True
                     else:
@@ -338,14 +358,18 @@ class Server:
 
 def summarize(results: list[Result]) -> dict:
     """Aggregate cell updates per second over the run and the latency
-    percentiles."""
+    percentiles of the requests that completed; failed requests are counted
+    apart and add no cell updates (a diverged domain's later solves exit at
+    once, so its steps would inflate the rate)."""
     if not results:
         return {}
+    good = [r for r in results if r.ok]
     span = max(r.finished for r in results) - min(r.arrival for r in results)
-    updates = sum(r.cells * r.nsteps for r in results)
-    lat = np.array([r.latency for r in results])
+    updates = sum(r.cells * r.nsteps for r in good)
+    lat = np.array([r.latency for r in good]) if good else np.array([math.nan])
     return {
         "requests": len(results),
+        "failed": len(results) - len(good),
         "mcell_updates_per_s": updates / span / 1e6 if span > 0 else math.inf,
         "span_s": span,
         "p50_s": float(np.percentile(lat, 50)),

@@ -210,7 +210,8 @@ class OracleBackend:
 class RefBackend:
     """The reference's own subroutines (flang build).  Arrays are allocated
     at the build's compile-time extents and the domain's index range is
-    embedded in them; the reference's inflow is its fixed 5 m/s."""
+    embedded in them; the reference's inflow is its fixed 5 m/s.  The o64
+    variants (kind-promoted FP64 builds) take and return FP64 arrays."""
 
     def __init__(self, setup, variant="small"):
         path = ref_lib_path(variant)
@@ -223,12 +224,13 @@ class RefBackend:
             raise ValueError(f"domain {w.im}x{w.jm}x{w.km} exceeds the {variant} build ({ip}x{jp}x{kp})")
         if setup.inflow is not None:
             raise ValueError("the reference build only has its fixed 5 m/s inflow (bondv1.f95:33-54)")
-        self.dom = _Domain(setup, np.float32, ext=(ip, jp, kp))
+        self.fp64 = variant.startswith("o64")
+        self.dom = _Domain(setup, np.float64 if self.fp64 else np.float32, ext=(ip, jp, kp))
         self.setup = setup
         self.fn = {}
         for s, names in SIGNATURES.items():
             f = getattr(self.lib, "ref_" + s)
-            f.argtypes = _argtypes(names, ctypes.c_float)
+            f.argtypes = _argtypes(names, ctypes.c_double if self.fp64 else ctypes.c_float)
             f.restype = None
             self.fn[s] = f
 

@@ -8,6 +8,12 @@
 ! the same name; array dummies are assumed-size so the caller's storage is
 ! passed by sequence association to the reference's explicit-shape dummies,
 ! whose extents come from the compile-time ip,jp,kp of params_common_sn.
+! RK is the kind of every real argument: c_float for the reference as shipped
+! and for the -fdefault-real-8 build (whose fields stay real(kind=4)),
+! c_double for the kind-promoted build (oracle/Makefile, o64 variants).
+#ifndef RK
+#define RK c_float
+#endif
 module ref_harness
     use iso_c_binding
     use params_common_sn, only: ip, jp, kp, ipmax, jpmax
@@ -36,7 +42,7 @@ contains
     ! set.f95:5 -- run constants
     subroutine ref_set(im, jm, km, n0, n1, nmax, dt, ro, vn, alpha, beta) bind(C, name='ref_set')
         integer(c_int), intent(out) :: im, jm, km, n0, n1, nmax
-        real(c_float), intent(out) :: dt, ro, vn, alpha, beta
+        real(RK), intent(out) :: dt, ro, vn, alpha, beta
         character(len=70) :: d10, d11, d12, d13, d14, d20, d21, d22, d23, d24, d25, d26, d27, d30, d31
         integer :: ifbf, ianime, ical, idata24, nspec, jdata24
         call set(d10, d11, d20, d21, d22, d23, d24, d25, d26, d27, d30, d31, im, jm, km, ifbf, &
@@ -46,7 +52,7 @@ contains
 
     ! grid.f95:5 -- metrics
     subroutine ref_grid(dx1, dxl, dy1, dyl, z2, dzn, dzs, dxs, dys) bind(C, name='ref_grid')
-        real(c_float) :: dx1(*), dxl(*), dy1(*), dyl(*), z2(*), dzn(*), dzs(*), dxs(*), dys(*)
+        real(RK) :: dx1(*), dxl(*), dy1(*), dyl(*), z2(*), dzn(*), dzs(*), dxs(*), dys(*)
         call grid(dx1, dxl, dy1, dyl, z2, dzn, dzs, dxs, dys)
     end subroutine
 
@@ -55,9 +61,9 @@ contains
     subroutine ref_init(km, jm, im, u, v, w, p, cn2s, dxs, cn2l, cn3s, dys, cn3l, dzs, cn4s, &
                         cn4l, cn1, amask1, bmask1, cmask1, dmask1, zbm, z2, dzn) bind(C, name='ref_init')
         integer(c_int), value :: km, jm, im
-        real(c_float) :: u(*), v(*), w(*), p(*), cn2s(*), dxs(*), cn2l(*), cn3s(*), dys(*), cn3l(*)
-        real(c_float) :: dzs(*), cn4s(*), cn4l(*), cn1(*), amask1(*), bmask1(*), cmask1(*), dmask1(*)
-        real(c_float) :: zbm(*), z2(*), dzn(*)
+        real(RK) :: u(*), v(*), w(*), p(*), cn2s(*), dxs(*), cn2l(*), cn3s(*), dys(*), cn3l(*)
+        real(RK) :: dzs(*), cn4s(*), cn4l(*), cn1(*), amask1(*), bmask1(*), cmask1(*), dmask1(*)
+        real(RK) :: zbm(*), z2(*), dzn(*)
         call init(km, jm, im, u, v, w, p, cn2s, dxs, cn2l, cn3s, dys, cn3l, dzs, cn4s, cn4l, cn1, &
                   amask1, bmask1, cmask1, dmask1, zbm, z2, dzn)
     end subroutine
@@ -65,16 +71,16 @@ contains
     ! velnw.f95:5
     subroutine ref_velnw(km, jm, im, p, ro, dxs, u, dt, f, dys, v, g, dzs, w, h) bind(C, name='ref_velnw')
         integer(c_int), value :: km, jm, im
-        real(c_float), value :: ro, dt
-        real(c_float) :: p(*), dxs(*), u(*), f(*), dys(*), v(*), g(*), dzs(*), w(*), h(*)
+        real(RK), value :: ro, dt
+        real(RK) :: p(*), dxs(*), u(*), f(*), dys(*), v(*), g(*), dzs(*), w(*), h(*)
         call velnw(km, jm, im, p, ro, dxs, u, dt, f, dys, v, g, dzs, w, h)
     end subroutine
 
     ! bondv1.f95:5
     subroutine ref_bondv1(jm, u, z2, dzn, v, w, km, n, im, dt, dxs) bind(C, name='ref_bondv1')
         integer(c_int), value :: jm, km, n, im
-        real(c_float), value :: dt
-        real(c_float) :: u(*), z2(*), dzn(*), v(*), w(*), dxs(*)
+        real(RK), value :: dt
+        real(RK) :: u(*), z2(*), dzn(*), v(*), w(*), dxs(*)
         call bondv1(jm, u, z2, dzn, v, w, km, n, im, dt, dxs)
     end subroutine
 
@@ -84,12 +90,12 @@ contains
                          diu8, diu9, dzs, h, nou1, u, nou5, v, nou9, w, nou2, nou3, nou4, nou6, nou7, &
                          nou8, uspd, vspd) bind(C, name='ref_velfg')
         integer(c_int), value :: km, jm, im
-        real(c_float), value :: vn
-        real(c_float) :: dx1(*), cov1(*), cov2(*), cov3(*), dfu1(*), diu1(*), diu2(*), dy1(*), diu3(*)
-        real(c_float) :: dzn(*), f(*), cov4(*), cov5(*), cov6(*), dfv1(*), diu4(*), diu5(*), diu6(*)
-        real(c_float) :: g(*), cov7(*), cov8(*), cov9(*), dfw1(*), diu7(*), diu8(*), diu9(*), dzs(*)
-        real(c_float) :: h(*), nou1(*), u(*), nou5(*), v(*), nou9(*), w(*), nou2(*), nou3(*), nou4(*)
-        real(c_float) :: nou6(*), nou7(*), nou8(*), uspd(*), vspd(*)
+        real(RK), value :: vn
+        real(RK) :: dx1(*), cov1(*), cov2(*), cov3(*), dfu1(*), diu1(*), diu2(*), dy1(*), diu3(*)
+        real(RK) :: dzn(*), f(*), cov4(*), cov5(*), cov6(*), dfv1(*), diu4(*), diu5(*), diu6(*)
+        real(RK) :: g(*), cov7(*), cov8(*), cov9(*), dfw1(*), diu7(*), diu8(*), diu9(*), dzs(*)
+        real(RK) :: h(*), nou1(*), u(*), nou5(*), v(*), nou9(*), w(*), nou2(*), nou3(*), nou4(*)
+        real(RK) :: nou6(*), nou7(*), nou8(*), uspd(*), vspd(*)
         call velfg(km, jm, im, dx1, cov1, cov2, cov3, dfu1, diu1, diu2, dy1, diu3, dzn, vn, f, cov4, &
                    cov5, cov6, dfv1, diu4, diu5, diu6, g, cov7, cov8, cov9, dfw1, diu7, diu8, diu9, &
                    dzs, h, nou1, u, nou5, v, nou9, w, nou2, nou3, nou4, nou6, nou7, nou8, uspd, vspd)
@@ -99,9 +105,9 @@ contains
     subroutine ref_feedbf(km, jm, im, usum, u, bmask1, vsum, v, cmask1, wsum, w, dmask1, alpha, dt, &
                           beta, fx, fy, fz, f, g, h) bind(C, name='ref_feedbf')
         integer(c_int), value :: km, jm, im
-        real(c_float), value :: alpha, dt, beta
-        real(c_float) :: usum(*), u(*), bmask1(*), vsum(*), v(*), cmask1(*), wsum(*), w(*), dmask1(*)
-        real(c_float) :: fx(*), fy(*), fz(*), f(*), g(*), h(*)
+        real(RK), value :: alpha, dt, beta
+        real(RK) :: usum(*), u(*), bmask1(*), vsum(*), v(*), cmask1(*), wsum(*), w(*), dmask1(*)
+        real(RK) :: fx(*), fy(*), fz(*), f(*), g(*), h(*)
         call feedbf(km, jm, im, usum, u, bmask1, vsum, v, cmask1, wsum, w, dmask1, alpha, dt, beta, &
                     fx, fy, fz, f, g, h)
     end subroutine
@@ -110,9 +116,9 @@ contains
     subroutine ref_les(km, delx1, dx1, dy1, dzn, jm, im, diu1, diu2, diu3, diu4, diu5, diu6, diu7, &
                        diu8, diu9, sm, f, g, h, uspd, vspd, dxs, dys) bind(C, name='ref_les')
         integer(c_int), value :: km, jm, im
-        real(c_float) :: delx1(*), dx1(*), dy1(*), dzn(*), diu1(*), diu2(*), diu3(*), diu4(*), diu5(*)
-        real(c_float) :: diu6(*), diu7(*), diu8(*), diu9(*), sm(*), f(*), g(*), h(*), uspd(*), vspd(*)
-        real(c_float) :: dxs(*), dys(*)
+        real(RK) :: delx1(*), dx1(*), dy1(*), dzn(*), diu1(*), diu2(*), diu3(*), diu4(*), diu5(*)
+        real(RK) :: diu6(*), diu7(*), diu8(*), diu9(*), sm(*), f(*), g(*), h(*), uspd(*), vspd(*)
+        real(RK) :: dxs(*), dys(*)
         call les(km, delx1, dx1, dy1, dzn, jm, im, diu1, diu2, diu3, diu4, diu5, diu6, diu7, diu8, &
                  diu9, sm, f, g, h, uspd, vspd, dxs, dys)
     end subroutine
@@ -120,14 +126,14 @@ contains
     ! boundsm.f95:5
     subroutine ref_boundsm(km, jm, sm, im) bind(C, name='ref_boundsm')
         integer(c_int), value :: km, jm, im
-        real(c_float) :: sm(*)
+        real(RK) :: sm(*)
         call boundsm(km, jm, sm, im)
     end subroutine
 
     ! adam.f95:5 (data21 dump compiled out by NO_IO)
     subroutine ref_adam(n, nmax, fold, im, jm, km, gold, hold, fghold, f, g, h) bind(C, name='ref_adam')
         integer(c_int), value :: n, nmax, im, jm, km
-        real(c_float) :: fold(*), gold(*), hold(*), fghold(*), f(*), g(*), h(*)
+        real(RK) :: fold(*), gold(*), hold(*), fghold(*), f(*), g(*), h(*)
         character(len=70) :: data21
         data21 = ''
         call adam(n, nmax, data21, fold, im, jm, km, gold, hold, fghold, f, g, h)
@@ -137,10 +143,10 @@ contains
     subroutine ref_press(km, jm, im, rhs, u, dx1, v, dy1, w, dzn, f, g, h, dt, cn1, cn2l, p, cn2s, &
                          cn3l, cn3s, cn4l, cn4s, n, nmax, usum, vsum, wsum) bind(C, name='ref_press')
         integer(c_int), value :: km, jm, im, n, nmax
-        real(c_float), value :: dt
-        real(c_float) :: rhs(*), u(*), dx1(*), v(*), dy1(*), w(*), dzn(*), f(*), g(*), h(*), cn1(*)
-        real(c_float) :: cn2l(*), p(*), cn2s(*), cn3l(*), cn3s(*), cn4l(*), cn4s(*)
-        real(c_float) :: usum(*), vsum(*), wsum(*)
+        real(RK), value :: dt
+        real(RK) :: rhs(*), u(*), dx1(*), v(*), dy1(*), w(*), dzn(*), f(*), g(*), h(*), cn1(*)
+        real(RK) :: cn2l(*), p(*), cn2s(*), cn3l(*), cn3s(*), cn4l(*), cn4s(*)
+        real(RK) :: usum(*), vsum(*), wsum(*)
         character(len=70) :: data20
         data20 = ''
         call press(km, jm, im, rhs, u, dx1, v, dy1, w, dzn, f, g, h, dt, cn1, cn2l, p, cn2s, cn3l, &
@@ -150,21 +156,21 @@ contains
     ! bondFG.f95:5
     subroutine ref_bondfg(km, jm, f, im, g, h) bind(C, name='ref_bondfg')
         integer(c_int), value :: km, jm, im
-        real(c_float) :: f(*), g(*), h(*)
+        real(RK) :: f(*), g(*), h(*)
         call bondfg(km, jm, f, im, g, h)
     end subroutine
 
     ! boundp.f95:28
     subroutine ref_boundp1(km, jm, p, im) bind(C, name='ref_boundp1')
         integer(c_int), value :: km, jm, im
-        real(c_float) :: p(*)
+        real(RK) :: p(*)
         call boundp1(km, jm, p, im)
     end subroutine
 
     ! boundp.f95:7
     subroutine ref_boundp2(jm, im, p, km) bind(C, name='ref_boundp2')
         integer(c_int), value :: km, jm, im
-        real(c_float) :: p(*)
+        real(RK) :: p(*)
         call boundp2(jm, im, p, km)
     end subroutine
 

@@ -15,6 +15,8 @@ outputs are stored):
   the reference inflow) for 20 steps: per step the sum, L2 norm and max|.|
   of u, v, w, p, sm, the centre values of p,u,v,w (press.f95:197-201), the
   SOR iteration count and sum(reltmp^2) per iteration.
+* c1_norms.npz: the same records for the 300x300x90 plain workload (its
+  30x30-block city, the reference inflow), first 5 steps.
 """
 from __future__ import annotations
 
@@ -76,7 +78,15 @@ def norms(a):
 
 
 def write_c0(nsteps=20):
-    w = dataclasses.replace(setup_gen.workloads()["plain-150x150x90"], ustar=None)
+    write_norms("plain-150x150x90", nsteps, "c0_norms.npz")
+
+
+def write_c1(nsteps=5):
+    write_norms("plain-300x300x90", nsteps, "c1_norms.npz")
+
+
+def write_norms(wname, nsteps, fname):
+    w = dataclasses.replace(setup_gen.workloads()[wname], ustar=None)
     s = setup_gen.build(w)
     ref = oracle.RefBackend(s, "large")
     # The SOR iteration count and residuals are loop variables of press,
@@ -105,13 +115,18 @@ def write_c0(nsteps=20):
         h[: st["iters"]] = st["sor"]
         sor.append(h)
         print(f"step {n}: {time.time() - t0:.1f} s, SOR {st['iters']} iterations", flush=True)
-    np.savez_compressed(HERE / "c0_norms.npz", **{f"norm_{k}": np.array(v) for k, v in rec.items()},
+    np.savez_compressed(HERE / fname, **{f"norm_{k}": np.array(v) for k, v in rec.items()},
                         centre=np.array(centre), iters=np.array(iters), sor=np.array(sor))
-    print("c0_norms.npz written")
+    print(f"{fname} written")
 
 
 if __name__ == "__main__":
     if not oracle.ref_available("small") or not oracle.ref_available("large"):
         raise SystemExit("build the reference first: make -C oracle ref")
-    write_stages()
-    write_c0()
+    which = sys.argv[1:] or ["stages", "c0", "c1"]
+    if "stages" in which:
+        write_stages()
+    if "c0" in which:
+        write_c0()
+    if "c1" in which:
+        write_c1()

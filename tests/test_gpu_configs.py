@@ -1,0 +1,277 @@
+"""The GPU path on the benchmark workloads at their full sizes, against the
+C restatement of the reference (oracle/les_oracle.c, pinned bit for bit to
+the reference's own Fortran by tests/test_oracle_vs_ref.py) and against the
+fixtures written from the reference build (tests/golden/).
+
+Tolerances: per-step relative L2 <= 1e-5 on u, v, w,
+p and sm, and equal SOR iteration counts.  The restatement runs with its
+FP64 reductions (redmode=1, the GPU's arithmetic); the reference reduces
+rhsav, sum(reltmp^2) and pav in FP32, so p differs from it at rounding level
+and the differences feed the later steps.
+
+* plain-300x300x90 (log-law inflow, the benchmarked workload): 5 steps,
+  replayed from the step graph as the benchmark does;
+* plain-300x300x90 with the reference's own 5 m/s inflow: 5 steps against
+  the reference's norms, centre values and SOR iteration counts (c1_norms);
+* large-1024x1024x128: one step against the restatement with the SOR capped
+  at 10 iterations (the restatement needs about a minute per full step at
+  this size), then 3 steps at the workload's own 100 iterations through
+  size-independent properties;
+* precision-512x512x128: one press, plain FP32 SOR and the FP64 defect
+  correction against the FP64 restatement (orc64, pinned bit for bit to the
+  kind-promoted reference build);
+* the serving mix: a request of each size class (96 to 512 wide, nz 64 and
+  128) served in one batch, each equal bit for bit to its domain alone and
+  within the tolerance of the restatement; a request whose state turns
+  non-finite fails alone.
+"""
+import dataclasses
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from backends import rel_l2
+from lesamd import api, layout, serving, setup_gen
+from oracle import oracle
+
+pytestmark = pytest.mark.gpu
+GOLDEN = Path(__file__).resolve().parent / "golden"
+FIELDS = ("u", "v", "w", "p", "sm")
+
+
+def _compare(dom, orc, n, tol=1e-5):
+    ref = orc.fetch(list(FIELDS))
+    errs = {}
+    for k in FIELDS:
+        errs[k] = rel_l2(dom.download(k), ref[k])
+    print(f"  step {n}: " + " ".join(f"{k} {e:.1e}" for k, e in errs.items()), flush=True)
+    for k, e in errs.items():
+        assert e <= tol, (n, k, e)
+    assert dom.press_stats()["iters"] == orc.press_stats()["iters"], n
+
+
+def test_plain_300_against_restatement():
+    w = dataclasses.replace(setup_gen.workloads()["plain-300x300x90"], nsteps=5)
+    s = setup_gen.build(w)
+    print(f"\n[c1 parity] {w.name}, 5 steps, graph replay", flush=True)
+    dom = api.Domain(s)
+    dom.set_graph(True)
+    orc = oracle.OracleBackend(s, 32, redmode=1)
+    for n in range(1, 6):
+        dom.step(n, 5)
+        orc.step(n, 5)
+        _compare(dom, orc, n)
+    dom.close()
+
+
+def test_plain_300_against_reference_norms():
+    """The reference's own 300x300x90 run (flang build, its 5 m/s inflow):
+    norms, the press.f95:197-201 centre values and SOR iteration counts of
+    its first 5 steps."""
+    fx = np.load(GOLDEN / "c1_norms.npz")
+    w = dataclasses.replace(setup_gen.workloads()["plain-300x300x90"], ustar=None)
+    s = setup_gen.build(w)
+    print(f"\n[golden c1] {w.name} (reference inflow), 5 steps", flush=True)
+    dom = api.Domain(s)
+    dom.set_graph(True)
+    for n in range(1, 6):
+        dom.step(n, 10 ** 9)
+        st = dom.press_stats()
+        assert st["iters"] == fx["iters"][n - 1], n
+        # the reference sums 4e6 squares per colour in FP32 (biased low);
+        # the GPU sums them in FP64
+        np.testing.assert_allclose(st["sor"], fx["sor"][n - 1][: st["iters"]], rtol=5e-3)
+        for k in FIELDS:
+            a = dom.download(k).astype(np.float64)
+            got = np.array([a.sum(), np.sqrt((a * a).sum()), np.abs(a).max()])
+            want = fx[f"norm_{k}"][n - 1]
+            assert abs(got[1] - want[1]) <= 1e-5 * want[1], (n, k, got, want)
+            assert abs(got[2] - want[2]) <= 1e-5 * want[2], (n, k, got, want)
+        c = (w.im // 2, w.jm // 2, w.km // 2)
+        cen = [layout.at(k, dom.download(k), *c) for k in ("u", "v", "w")]
+        np.testing.assert_allclose(cen, fx["centre"][n - 1][1:], rtol=1e-4, atol=1e-6)
+        print(f"  step {n}: SOR {st['iters']} iterations, norms within 1e-5", flush=True)
+    dom.close()
+
+
+def test_large_1024_against_restatement_and_properties():
+    base = setup_gen.workloads()["large-1024x1024x128"]
+    w = dataclasses.replace(base, nmaxp=10)
+    s = setup_gen.build(w)
+    print(f"\n[c2 parity] {w.name}, step 1 with the SOR capped at 10 iterations", flush=True)
+    dom = api.Domain(s)
+    orc = oracle.OracleBackend(s, 32, redmode=1)
+    dom.step(1, 3)
+    for st in oracle.STAGES:
+        kw = {"n": 1, "nmax": 3} if st in ("bondv1", "adam", "press") else {}
+        orc.call(st, **kw)
+        print(f"  restatement: {st} done", flush=True)
+    _compare(dom, orc, 1)
+    dom.close()
+    del orc
+
+    s = setup_gen.build(base, dense=False)
+    print(f"[c2 properties] {base.name}, 3 steps, SOR at most 100 iterations", flush=True)
+    dom = api.Domain(s)
+    dom.set_graph(True)
+    for n in range(1, 4):
+        dom.step(n, 3)
+        st = dom.press_stats()
+        print(f"  step {n}: SOR {st['iters']} iterations, last sum {st['sor'][-1]:.3e}", flush=True)
+        assert st["nonfinite"] == 0 and np.all(np.isfinite(st["sor"]))
+        assert 1 <= st["iters"] <= 100
+        assert st["iters"] == 100 or st["sor"][-1] < base.pjuge
+    u = dom.download("u")
+    for i in (0, 1):
+        np.testing.assert_array_equal(u[layout.ref_slice("u", (i, 1, 1), (i, base.jm, base.km))][0], s.inflow)
+    np.testing.assert_array_equal(u[layout.ref_slice("u", (0, 0, 1), (base.im + 1, 0, base.km))][:, 0],
+                                  u[layout.ref_slice("u", (0, base.jm, 1), (base.im + 1, base.jm, base.km))][:, 0])
+    del u
+    p = layout.interior("p", dom.download("p"), base.im, base.jm, base.km).astype(np.float64)
+    dv = (base.dx * base.dy) * s.metrics["dzn"][2: base.km + 2].astype(np.float64)[None, None, :]
+    assert abs((p * dv).sum()) <= 1e-6 * (np.abs(p) * dv).sum()
+    for k in FIELDS:
+        assert np.all(np.isfinite(dom.download(k))), k
+    dom.close()
+
+
+def test_precision_512_press_against_fp64():
+    """One press at 512x512x128 from the GPU's FP32 state after step 1:
+    100 FP32 SOR iterations (the workload's cap) and 4 x 25 iterations of
+    the FP64 defect correction (equal in exact arithmetic to 100 SOR
+    iterations) against 100 iterations of the FP64 restatement on the same
+    inputs.  The defect correction must not land farther from FP64."""
+    base = setup_gen.workloads()["precision-512x512x128"]
+    w = dataclasses.replace(base, pjuge=1e-30)  # every solve runs its 100 iterations
+    s = setup_gen.build(w)
+    print(f"\n[c3 precision] {w.name}, one press vs orc64", flush=True)
+    src = api.Domain(s)
+    src.step(1, 2)
+    for st in ("velnw", "bondv1", "velfg", "feedbf", "les", "adam"):
+        src.stage(st, 2, 2)
+    names = ["u", "v", "w", "f", "g", "h", "p"]
+    state = {n: src.download(n) for n in names}
+    src.close()
+    o64 = oracle.OracleBackend(s, 64, redmode=1)
+    o64.load(state)
+    o64.call("press", n=2, nmax=2)
+    p64 = o64.fetch(["p"])["p"]
+    assert o64.press_stats()["iters"] == 100
+    del o64
+    errs = {}
+    for mode in (0, 1):
+        d = api.Domain(s)
+        if mode:
+            d.set_press_mode(1, outer=4, inner=25)
+        for n in names:
+            d.upload(n, state[n])
+        d.stage("press", 2, 2)
+        errs[mode] = rel_l2(d.download("p"), p64)
+        d.close()
+    print(f"  p rel L2 vs FP64: FP32 SOR {errs[0]:.2e}, defect correction {errs[1]:.2e}", flush=True)
+    assert errs[0] <= 1e-5
+    assert errs[1] <= errs[0]
+
+
+SIZE_CLASSES = [(96, 96, 64), (256, 224, 128), (512, 448, 64)]
+
+
+def test_serving_size_classes_against_restatement():
+    """A request of each size class served together by continuous batching
+    (4 steps: step 1 on each domain's own stream, steps 2-4 in one batched
+    launch per stage): every domain equals its run alone bit for bit, and
+    the restatement within the tolerance, with the same SOR counts."""
+    reqs = serving.sample_requests(len(SIZE_CLASSES), 4, seed=12, sizes=SIZE_CLASSES)
+    print("\n[serving size classes] " + ", ".join(r.workload.name for r in reqs), flush=True)
+    finals = {}
+
+    class Keep:
+        def __init__(self, req):
+            self.req = req
+            self.d = api.Domain(setup_gen.build(req.workload, dense=False))
+            self.d.set_graph(True)
+
+        def step(self, n, nmax):
+            self.d.step(n, nmax)
+
+        def done(self):
+            return self.d.done()
+
+        def nonfinite(self):
+            return self.d.nonfinite()
+
+        def close(self):
+            finals[self.req.rid] = ({k: self.d.download(k) for k in FIELDS}, self.d.press_stats()["iters"])
+            self.d.close()
+
+    srv = serving.Server(64 << 30, burst=2, domain_factory=Keep,
+                         bytes_of=lambda r: api.bytes_needed(r.workload.im, r.workload.jm, r.workload.km))
+    res = srv.run_batched(reqs)
+    assert [r.ok for r in res] == [True] * len(reqs)
+    for r in reqs:
+        s = setup_gen.build(r.workload, dense=False)
+        alone = api.Domain(s)
+        alone.run(1, r.nsteps, r.nsteps)
+        got, iters = finals[r.rid]
+        for k in FIELDS:
+            assert np.array_equal(alone.download(k), got[k]), (r.rid, k)
+        assert alone.press_stats()["iters"] == iters
+        alone.close()
+        orc = oracle.OracleBackend(setup_gen.build(r.workload), 32, redmode=1)
+        for n in range(1, r.nsteps + 1):
+            orc.step(n, r.nsteps)
+        ref = orc.fetch(list(FIELDS))
+        errs = {k: rel_l2(got[k], ref[k]) for k in FIELDS}
+        print(f"  {r.workload.name}: " + " ".join(f"{k} {e:.1e}" for k, e in errs.items()), flush=True)
+        assert all(e <= 1e-5 for e in errs.values()), errs
+        assert orc.press_stats()["iters"] == iters
+
+
+def test_diverged_request_fails_alone():
+    """A request whose state is made non-finite (a NaN planted in p before
+    its first step) is reported failed; the others in the same batch finish
+    ok and equal their runs alone bit for bit."""
+    sizes = [(64, 48, 24), (96, 64, 32), (64, 64, 24)]
+    reqs = serving.sample_requests(3, 4, seed=7, sizes=sizes)
+    print("\n[serving failure isolation] request 1 of 3 gets a NaN", flush=True)
+    finals = {}
+
+    class Keep:
+        def __init__(self, req):
+            self.req = req
+            self.d = api.Domain(setup_gen.build(req.workload, dense=False))
+            if req.rid == 1:
+                # p, not u: bondv1 at n == 1 overwrites the interior of u, v, w
+                # with the inflow column (bondv1.f95:79-87)
+                p = self.d.download("p")
+                p[10, 10, 5] = np.nan
+                self.d.upload("p", p)
+
+        def step(self, n, nmax):
+            self.d.step(n, nmax)
+
+        def done(self):
+            return self.d.done()
+
+        def nonfinite(self):
+            return self.d.nonfinite()
+
+        def close(self):
+            finals[self.req.rid] = {k: self.d.download(k) for k in FIELDS}
+            self.d.close()
+
+    srv = serving.Server(64 << 30, burst=2, domain_factory=Keep,
+                         bytes_of=lambda r: api.bytes_needed(r.workload.im, r.workload.jm, r.workload.km))
+    res = srv.run_batched(reqs)
+    assert [r.ok for r in res] == [True, False, True]
+    summ = serving.summarize(res)
+    assert summ["failed"] == 1
+    for r in reqs:
+        if r.rid == 1:
+            continue
+        alone = api.Domain(setup_gen.build(r.workload, dense=False))
+        alone.run(1, r.nsteps, r.nsteps)
+        for k in FIELDS:
+            assert np.array_equal(alone.download(k), finals[r.rid][k]), (r.rid, k)
+        alone.close()

@@ -93,3 +93,58 @@ def test_defect_correction_matches_the_fp64_restatement():
     e0, e1 = rel_l2(got[0], p64), rel_l2(got[1], p64)
     print(f"rel L2 vs orc64: FP32 SOR {e0:.2e}, defect correction {e1:.2e}", flush=True)
     assert e1 <= 0.5 * e0
+
+
+def _env_domain(s, **env):
+    import os
+    old = {k: os.environ.get(k) for k in env}
+    os.environ.update({k: str(v) for k, v in env.items()})
+    try:
+        return api.Domain(s)
+    finally:
+        for k, v in old.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
+
+
+@pytest.mark.parametrize("face", ["i0", "im1", "j0", "jm1", "k0", "km1"])
+def test_defect_correction_ignores_the_ghosts_of_p(face):
+    """Press mode 1 reads p only on the interior (its defect mirrors the
+    interior in i and k and wraps j), so ghosts of p that differ from their
+    boundp copies -- an upload of p, or a restart file without ghosts --
+    must not change a bit of the result.  Compared with the same press on
+    press-consistent ghosts and with the one-row SOR kernel
+    (LES_SOR_ROWS=1), whose iterates equal the two-row kernel's."""
+    w = setup_gen.Workload("dc-ghosts", 48, 40, 18, seed=5)
+    s = setup_gen.build(w, dense=False)
+    print(f"\n[defect correction] perturbed ghost face {face}, 48x40x18", flush=True)
+    src = api.Domain(s)
+    src.run(1, 3, 8)
+    for st in ("velnw", "bondv1", "velfg", "feedbf", "les", "adam"):
+        src.stage(st, 4, 8)
+    state = {n: src.download(n) for n in ("u", "v", "w", "p", "f", "g", "h")}
+    src.close()
+    im, jm, km = w.im, w.jm, w.km
+    bad = state["p"].copy()  # p(0:im+2, 0:jm+2, 0:km+1)
+    rng = np.random.default_rng(3)
+    sl = {"i0": np.s_[0, 1:jm + 1, 1:km + 1], "im1": np.s_[im + 1, 1:jm + 1, 1:km + 1],
+          "j0": np.s_[1:im + 1, 0, 1:km + 1], "jm1": np.s_[1:im + 1, jm + 1, 1:km + 1],
+          "k0": np.s_[1:im + 1, 1:jm + 1, 0], "km1": np.s_[1:im + 1, 1:jm + 1, km + 1]}[face]
+    bad[sl] += rng.uniform(0.5, 1.0, bad[sl].shape).astype(np.float32)
+    out = {}
+    for name, p, env in (("clean", state["p"], {}), ("perturbed", bad, {}),
+                         ("rows1", bad, {"LES_SOR_ROWS": 1})):
+        d = _env_domain(s, **env)
+        d.set_press_mode(1, outer=3, inner=7)
+        for n, a in state.items():
+            d.upload(n, p if n == "p" else a)
+        d.stage("press", 4, 8)
+        out[name] = {f: d.download(f) for f in ("p", "rhs", "f", "g", "h")}
+        out[name]["hist"] = d.press_stats()["sor"]
+        d.close()
+    for name in ("perturbed", "rows1"):
+        for f in ("p", "rhs", "f", "g", "h"):
+            assert np.array_equal(out[name][f], out["clean"][f]), (name, f)
+        np.testing.assert_array_equal(out[name]["hist"], out["clean"]["hist"])

@@ -3,7 +3,9 @@ builds under oracle/_ref), bit for bit, and against the committed golden
 fixtures written from those subroutines (tests/golden/make_golden.py).
 
 These tests pin the oracle: every function of les_oracle.c is executed on
-seeded inputs and compared with the reference after every stage.
+seeded inputs and compared with the reference after every stage, in FP32
+(against the reference as shipped) and in FP64 (against the kind-promoted
+build, the accuracy oracle).
 """
 import dataclasses
 
@@ -80,6 +82,28 @@ def test_fp64_reduction_mode_equals_mixed_reference_press():
             assert np.array_equal(a[name], b[name]), (n, name)
 
 
+@pytest.mark.parametrize("case", REF_CASES, ids=lambda c: f"{c['im']}x{c['jm']}x{c['km']}")
+def test_fp64_restatement_equals_kind_promoted_reference(case):
+    """The FP64 instantiation of the restatement (orc64, the accuracy
+    oracle) against the reference with every real promoted to real(kind=8)
+    (oracle/Makefile, o64_small): bit for bit after every stage of steps
+    1-3."""
+    if not oracle.ref_available("o64_small"):
+        pytest.skip("reference build oracle/_ref/o64_small absent")
+    s = _setup(case)
+    ref = oracle.RefBackend(s, "o64_small")
+    orc = oracle.OracleBackend(s, 64, redmode=0)
+    for n in range(1, 4):
+        for st in oracle.STAGES:
+            kw = {"n": n, "nmax": 10 ** 6} if st in ("bondv1", "adam", "press") else {}
+            ref.call(st, **kw)
+            orc.call(st, **kw)
+            a, b = ref.fetch(STEP_ARRAYS), orc.fetch(STEP_ARRAYS)
+            for name in STEP_ARRAYS:
+                assert a[name].dtype == np.float64
+                assert np.array_equal(a[name], b[name]), (n, st, name)
+
+
 # --------------------------------------------------- committed fixtures
 from pathlib import Path  # noqa: E402
 
@@ -122,3 +146,23 @@ def test_restatement_reproduces_c0_norms_first_steps():
         st = orc.press_stats()
         assert st["iters"] == fx["iters"][n - 1]
         np.testing.assert_array_equal(st["sor"], fx["sor"][n - 1][: st["iters"]])
+
+
+def test_restatement_reproduces_c1_norms_first_step():
+    """300x300x90 plain workload, reference inflow: step 1's norms, centre
+    values and SOR history (steps 2-5 are checked on the GPU path; the
+    generator compared all 5 steps bit for bit with the reference)."""
+    fx = np.load(GOLDEN / "c1_norms.npz")
+    w = dataclasses.replace(setup_gen.workloads()["plain-300x300x90"], ustar=None)
+    s = setup_gen.build(w)
+    orc = oracle.OracleBackend(s, 32, redmode=0)
+    orc.step(1, 10 ** 9)
+    f = orc.fetch(["u", "v", "w", "p", "sm"])
+    for k in ("u", "v", "w", "p", "sm"):
+        a = f[k].astype(np.float64)
+        np.testing.assert_array_equal([a.sum(), np.sqrt((a * a).sum()), np.abs(a).max()], fx[f"norm_{k}"][0])
+    c = (w.im // 2, w.jm // 2, w.km // 2)
+    np.testing.assert_array_equal([layout.at(k, f[k], *c) for k in ("p", "u", "v", "w")], fx["centre"][0])
+    st = orc.press_stats()
+    assert st["iters"] == fx["iters"][0]
+    np.testing.assert_array_equal(st["sor"], fx["sor"][0][: st["iters"]])

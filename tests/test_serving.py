@@ -165,3 +165,39 @@ def test_batched_serving_steps_large_domains_on_their_own(workers):
     assert len(opens) == sum(1 for kind, _ in blog if kind == "close")
     assert FakeDomain.live == 0
     assert all(r.finished >= r.admitted >= r.arrival for r in res)
+
+
+class DivergingDomain(FakeDomain):
+    """A stand-in whose state turns non-finite (the request with rid 2)."""
+
+    def nonfinite(self):
+        return self.req.rid == 2
+
+
+@pytest.mark.parametrize("mode", ["streams", "sequential", "batched"])
+def test_a_diverged_request_fails_alone(mode):
+    FakeDomain.live = FakeDomain.peak = 0
+    reqs = _reqs(5)
+    log, blog = [], []
+    srv = serving.Server(10 ** 9, burst=2, domain_factory=lambda r: DivergingDomain(r, log),
+                         bytes_of=lambda r: 1, clock=FakeClock(),
+                         batch_factory=lambda doms: FakeBatch(doms, blog), workers=1)
+    if mode == "batched":
+        res = srv.run_batched(reqs)
+    else:
+        res = srv.run(reqs, sequential=(mode == "sequential"))
+    assert [r.rid for r in res] == list(range(5))
+    assert [r.ok for r in res] == [True, True, False, True, True]
+    assert res[2].status.startswith("failed")
+    # every request still ran all its steps and was released
+    assert sorted(log) == [(r, tuple(range(1, 8))) for r in range(5)]
+    assert FakeDomain.live == 0
+
+
+def test_summary_leaves_failed_requests_out_of_the_rates():
+    res = [serving.Result(i, 1000, 10, 0.0, 0.0, float(i + 1)) for i in range(4)]
+    res.append(serving.Result(4, 10 ** 6, 10, 0.0, 0.0, 4.0, status="failed: non-finite state (diverged)"))
+    s = serving.summarize(res)
+    assert s["requests"] == 5 and s["failed"] == 1
+    assert abs(s["mcell_updates_per_s"] - 4 * 1000 * 10 / 4.0 / 1e6) < 1e-12
+    assert s["p99_s"] <= 4.0
