@@ -91,6 +91,30 @@ int les_press(int km, int jm, int im, float* rhs, const float* u, const float* d
               const float* cn3l, const float* cn3s, const float* cn4l, const float* cn4s, int n,
               int nmax, const float* usum, const float* vsum, const float* wsum);
 
+/* press without allocation.  No counterpart in the reference: for callers
+ * that cannot allocate device memory (code under HIP graph capture, serving
+ * loops that forbid hipMalloc).  The argument list of les_press (press.f95:
+ * 9-10) with every array a DEVICE pointer in the reference layout (extents
+ * from im, jm, km), plus the stream to run on (a hipStream_t; NULL: the null
+ * stream) and stats, a device int[2] that receives the SOR iteration count
+ * and a non-finite flag (NULL: not written).  Asynchronous; nothing is
+ * allocated and nothing waits.  nmaxp, pjuge and omega are those of
+ * les_press_config.  The reductions' partial sums and control block live in
+ * a fixed device array of 270 592 bytes per GPU, held until the process ends and
+ * shared by every call on that GPU: calls that overlap on one GPU race, so
+ * order them.  At most ceil(im/64)*ceil(jm/4) <= 8192 (im, jm up to 512 x
+ * 4096, 1024 x 2048, ...); larger domains are refused.  Results are those of
+ * les_press (p, rhs, f, g, h bit for bit where the FP64 volume sum of
+ * dx1*dy1*dzn is exact, as for power-of-two dx1*dy1 and FP32 dzn); the
+ * first call on a device must not be under graph capture (it looks up the
+ * array's device address). */
+int les_press_noalloc(int km, int jm, int im, float* rhs, const float* u, const float* dx1, const float* v,
+                      const float* dy1, const float* w, const float* dzn, float* f, float* g, float* h,
+                      float dt, const float* cn1, const float* cn2l, float* p, const float* cn2s,
+                      const float* cn3l, const float* cn3s, const float* cn4l, const float* cn4s, int n,
+                      int nmax, const float* usum, const float* vsum, const float* wsum, void* stream,
+                      int* stats);
+
 /* Replaces the values of the hard-coded inflow (bondv1.f95:33-54) with
  * u_in(j,k), j=1..jm fastest then k=1..km, for the domain of extents
  * (im,jm,km); prof == NULL restores 5 m/s.  No reference counterpart: the

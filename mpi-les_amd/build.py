@@ -21,7 +21,7 @@ ROOT = HERE.parent
 CSRC = HERE / "csrc"
 OBJ = HERE / "_build"
 LIB = HERE / "lesamd" / "liblesamd.so"
-SOURCES = ["k_step.hip", "k_press.hip", "k_output.hip", "runtime.hip"]
+SOURCES = ["k_step.hip", "k_press.hip", "k_output.hip", "k_noalloc.hip", "runtime.hip"]
 
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("LES_OFFLOAD_ARCH", "gfx950")

@@ -15,7 +15,7 @@
 #include <tuple>
 #include <vector>
 
-#include "../../include/les_amd.h"
+#include "inc/les_amd.h"
 #include "les_kernels.h"
 
 using namespace les;
@@ -974,6 +974,17 @@ int bind(les_domain* D, int f, const float* host) {
   if (ref_domain(im, jm, km, &D)) return 1
 
 }  // namespace
+
+namespace les {
+// For the entry points defined in other files (k_noalloc.hip).
+int report_error(const std::string& msg) { return fail(msg); }
+int press_config_values(int* nmaxp, float* pjuge, float* omega) {
+  *nmaxp = nmaxp_default;
+  *pjuge = pjuge_default;
+  *omega = omega_default;
+  return 0;
+}
+}  // namespace les
 
 extern "C" {
 
