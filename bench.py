@@ -109,16 +109,17 @@ def cpu_baseline(setup, state, n0, nsteps, budget_s):
     }
 
 
-def profiled_traffic(kernel_class: str, workload: str):
+def profiled_traffic(kernel_class: str, workload: str, sor_rows: int = 4):
     """HBM bytes per launch of the dominant kernel, as measured by the
     rocprofv3 FETCH_SIZE and WRITE_SIZE counters (scaled by a copy of known
     size) and recorded in profiles/<kernel>_<workload>.md; None when no such
-    measurement is recorded for this kernel and workload."""
+    measurement is recorded for this kernel and workload.  sor_rows: the SOR
+    kernel in use (4 two iterations per launch, 2 or 1 one)."""
     if kernel_class == "sor":
         if os.environ.get("LES_SOR_FUSED", "1") == "0":
             name = "sor_sweep"
         else:
-            name = "sor_iter" if os.environ.get("LES_SOR_ROWS", "2") == "1" else "sor_iter2"
+            name = {1: "sor_iter", 2: "sor_iter2"}.get(sor_rows, "sor_pair")
     else:
         name = kernel_class
     f = ROOT / "profiles" / f"{name}_{workload}.md"
@@ -352,25 +353,31 @@ def main(argv=None):
     avg_s = dom_ms / 1e3 / dom_n
     alg_bytes = CLASS_BYTES.get(dom_cls, 0) * w.cells
     fused = os.environ.get("LES_SOR_FUSED", "1") != "0"
+    # SOR iterations per launch: 2 for the two-iteration kernel (the tuned or
+    # default shape reports rows 4), 1 for one fused iteration, 1/2 for the
+    # colour sweeps (LES_SOR_FUSED=0)
+    sor_rows = (tuned or {}).get("rows", 4)  # untuned: the default shape (two iterations per launch)
+    ipl = 0.5 if not fused else (2 if sor_rows == 4 else 1)
     if dom_cls == "sor":
-        # Class "sor" holds the launches of iterations 2..nmaxp of each step
-        # (iteration 1 is class "sor1"), one fused iteration per launch, or
-        # two colour sweeps per iteration (LES_SOR_FUSED=0, where iteration 1
-        # is in "sor").  Launches after a step's stop test exit at once: the
-        # iterations that ran come from each step's own count, and the
-        # launches' total time is divided among them only.
-        iters = sum(it - 1 for it in iters_run) if fused else sum(iters_run)
-        per_launch = 1 if fused else 2
-        alg_bytes = SOR_BYTES_PER_ITER * w.cells / per_launch
-        avg_s = dom_ms / 1e3 / max(1, iters * per_launch)
+        # Class "sor" holds the SOR launches after the one holding iteration
+        # 1 (class "sor1"), or all colour sweeps (LES_SOR_FUSED=0).
+        # Launches after a step's stop test exit at once: the iterations
+        # that ran come from each step's own count, and the launches' total
+        # time is divided among the launches that carried them.
+        first = {0.5: 0, 1: 1, 2: 2}[ipl]
+        iters = sum(max(0, it - first) for it in iters_run)
+        alg_bytes = SOR_BYTES_PER_ITER * w.cells * ipl
+        avg_s = dom_ms / 1e3 / max(1, round(iters / ipl))
     in_step_us = avg_s * 1e6
     timing_method = "one HIP event pair per launch in K eager steps"
-    if dom_cls == "sor" and fused and iters == dom_n:
-        # one fused iteration per launch: its duration from SOR_RUN
-        # back-to-back launches of the same kernel and launch shape on the
-        # domain's stream between one event pair (les_domain_time_sor)
-        avg_s = dom.time_sor(SOR_RUN) / 1e6
-        timing_method = f"one HIP event pair around {SOR_RUN} back-to-back launches on the domain's stream"
+    if dom_cls == "sor" and fused and round(iters / ipl) == dom_n:
+        # every launch carried its iterations: the kernel's duration from
+        # SOR_RUN back-to-back iterations of the same kernel and launch shape
+        # on the domain's stream between one event pair (les_domain_time_sor
+        # reports per iteration)
+        avg_s = dom.time_sor(SOR_RUN) * ipl / 1e6
+        timing_method = (f"one HIP event pair around {SOR_RUN} back-to-back iterations "
+                         f"({int(SOR_RUN / ipl)} launches) on the domain's stream")
     achieved = alg_bytes / avg_s / 1e9
     step_bytes = w.cells * (STEP_BYTES_FIXED + SOR_BYTES_PER_ITER * L)
     per_kernel = {c: {"ms_per_step": round(v[0] / nprof, 4), "launches_per_step": v[1] / nprof,
@@ -401,8 +408,9 @@ def main(argv=None):
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": profiled_traffic(dom_cls, w.name),
+            "traffic": profiled_traffic(dom_cls, w.name, sor_rows),
             "alg_bytes_per_launch": int(alg_bytes),
+            "sor_iterations_per_launch": ipl if dom_cls == "sor" else None,
             "avg_launch_us": round(avg_s * 1e6, 3),
             "timing": timing_method,
             "in_step_us_per_launch": round(in_step_us, 3),
@@ -410,7 +418,9 @@ def main(argv=None):
         "kernels": per_kernel,
     }
     if tuned is not None:
-        result["sor_launch"] = {"rows_per_thread": tuned["rows"], "tile_rows": tuned["tile_rows"],
+        result["sor_launch"] = {"kernel": {4: "k_sor_pair (two iterations per launch)",
+                                           2: "k_sor_iter2", 1: "k_sor_iter"}[tuned["rows"]],
+                                "rows_per_thread": min(tuned["rows"], 2), "tile_rows": tuned["tile_rows"],
                                 "tile_pairs": tuned["tile_width"], "k_levels_per_block": tuned["kchunk"],
                                 "tuned_us_per_iter": round(tuned["us_per_iter"], 2),
                                 "default_us_per_iter": round(tuned["us_per_iter_default"], 2)}

@@ -211,9 +211,12 @@ int les_domain_step(les_domain* d, int n, int nmax);
 /* With on != 0, steps n > 1 replay one captured HIP graph of the whole
  * step (recaptured when the domain's constants change); results are the
  * same as eager launches.  Off by default; ignored while timing is on. */
-/* Picks the launch shape of the fused SOR iteration (tile rows per thread: 1
- * or 2; tile rows x pairs with the ring: 16 x 64, 32 x 64 or 32 x 32; k
- * levels per block) for this grid shape: times `reps` iterations of each
+/* Picks the launch shape of the fused SOR iteration (kernel: rows = 4 runs
+ * two iterations per launch on 32 x 64 tiles with a four-cell ring (even jm,
+ * separable cn1), rows = 2 or 1 one iteration with two or one tile rows per
+ * thread; tile rows x pairs with the ring of the one-iteration kernel: 16 x
+ * 64, 32 x 64 or 32 x 32; k levels per block) for this grid shape: times
+ * `reps` iterations of each
  * candidate on the domain's SOR scratch arrays and keeps the fastest.  The
  * choice is remembered per (im, jm, km) for the process; later calls with
  * reps <= 0 reuse it without timing.  Results do not depend on the shape.
@@ -224,7 +227,8 @@ int les_domain_step(les_domain* d, int n, int nmax);
 int les_domain_autotune(les_domain* d, int reps, int* rows, int* tile_rows, int* tile_width, int* kchunk,
                         double* us_best, double* us_default);
 /* Microseconds per fused SOR iteration with the domain's current launch
- * shape: `iters` back-to-back iterations (numbered 2.., never stopping) on
+ * shape (a launch of two iterations counts as two; an odd iters is rounded
+ * up to whole launches): `iters` back-to-back iterations (numbered 2.., never stopping) on
  * the domain's SOR scratch arrays between one pair of HIP events on its
  * stream.  Leaves the fields and the last press statistics unchanged;
  * synchronises.  No reference counterpart: the measurement of the dominant

@@ -75,6 +75,8 @@ __device__ __forceinline__ void rhs_body(const VBlk& vb, const Grid& g, const Me
     ctrl->sor_iters = 0;
     ctrl->nonfinite = 0;
     ctrl->p_ghost_diff = 0;
+    ctrl->p_final = -1;
+    ctrl->fix_iter = 0;
   }
   const int i = 1 + vb.x * BX + threadIdx.x, j = 1 + vb.y * BY + threadIdx.y;
   double acc = 0.0;
@@ -975,6 +977,529 @@ void launch_sor_batch(const SorBatchEntry* tab, const int* block0, int ntab, int
 #undef SORB
 }
 
+// ------------------------- two SOR iterations per launch (temporal blocking)
+// One launch runs iterations l and l+1 of the fused red-black iteration in a
+// single k-march, so the iterates and the rhs cross the memory system once
+// per two iterations.  At step kr a thread computes, for its two tile rows:
+//   A  red of iteration l at level kr (from the black iterate of l-1),
+//   B  black of iteration l at level kr-1 (from A's red values),
+//   C  red of iteration l+1 at level kr-2 (from B's black values),
+//   D  black of iteration l+1 at level kr-3 (from C's red values).
+// Each stage needs the previous stage's values one cell around, so the tile
+// carries a four-cell ring (32 rows of which 24 are the tile, 64 pairs of
+// which 56): A is valid on the tile and three rings, B on two, C on one, D
+// on the tile alone, and the outermost waves skip the stages they cannot
+// complete.  The march starts three levels below the block's chunk and ends
+// three above it, each stage active only on the levels its successor reads.
+// In-plane neighbours owned by other threads come through LDS, one set of
+// planes per stage input (the black iterate of l-1 one level ahead, then
+// two planes each for A, B and C); the k neighbours and the thread's other
+// row come from register rings.  The red cell of level L and the black cell
+// of level L-1 share their position in the pair, and levels kr, kr-2 (red)
+// and kr-1, kr-3 (black) have the same parity, so all four stages of a step
+// use the same positions.  Only iteration l+1 is stored (C and D); iteration
+// l lives in registers and LDS.  Every update is the expression of
+// k_sor_iter2 on the same operands (the boundp1/boundp2 ghost rules included:
+// i and k ghosts mirror the cell, j wraps), so the iterates are those of two
+// one-iteration launches bit for bit.  A launch of iteration 1 reads the
+// ghost values p had on entry to press (pent) where they differ from their
+// boundp copies (ctrl->p_ghost_diff), as k_sor_iter<SEP, true>.
+//
+// Iterate sets rotate over three: a launch reads in_set (p_{l-1}) and writes
+// out_set, leaving the set before in_set intact.  The stop test runs one
+// launch behind in block 0; when iteration l, the first of a launch, met the
+// threshold, every later launch exits at once and a closing fix-up launch
+// recomputes iteration l alone from that launch's in_set into its out_set.
+struct PairLaunch {
+  int iter;         // first iteration l of the launch
+  int two;          // 1: iterations l and l+1; 0: l alone
+  int in_set, out_set;  // iterate sets read (p_{l-1}) and written
+  int wpart;        // partial-sum slots written: sb.part[wpart][0] iteration l, [1] iteration l+1
+  int prev_iter;    // the previous launch (0: none), whose stop test block 0 runs ...
+  int prev_two, prev_in, prev_out, prev_nparts, rpart;  // ... its shape, sets, slots
+  int fixup;        // the fix-up launch: iteration and sets from ctrl->fix_*, one iteration
+};
+
+// The stop test (press.f95:119-121) of a launch of iteration L (and L+1 when
+// two) with FP64 sums s1 (and s2) of reltmp^2: records them and, at the first
+// iteration below pjuge or non-finite, stops the loop with the result in
+// out_set -- for the first iteration of a pair after the fix-up recomputes it.
+__device__ __forceinline__ void pair_stop(Ctrl* ctrl, int L, int two, double s1, double s2, int in_set, int out_set,
+                                          float pjuge) {
+  if (L <= kSorHist) ctrl->sor_hist[L - 1] = s1;
+  ctrl->sor_iters = L;
+  ctrl->p_final = out_set;
+  if (!(s1 == s1) || s1 > 1.0e300) ctrl->nonfinite = 1;
+  if (s1 < (double)pjuge || ctrl->nonfinite) {
+    ctrl->sor_done = L;
+    if (two) {
+      ctrl->fix_iter = L;
+      ctrl->fix_in = in_set;
+      ctrl->fix_out = out_set;
+    }
+    return;
+  }
+  if (!two) return;
+  if (L + 1 <= kSorHist) ctrl->sor_hist[L] = s2;
+  ctrl->sor_iters = L + 1;
+  if (!(s2 == s2) || s2 > 1.0e300) ctrl->nonfinite = 1;
+  if (s2 < (double)pjuge || ctrl->nonfinite) ctrl->sor_done = L + 1;
+}
+
+constexpr int PTY = 32, PR = 4, PNT = FX * PTY / 2, PFTY = PTY - 2 * PR, PFTX = FX - 2 * PR;
+
+// ENT: the launch may hold iteration 1 (the entry-ghost code is compiled in).
+template <bool OM1, bool ENT>
+__device__ __forceinline__ void sor_pair_block(const Grid& g, const SplitGrid& s, const Metrics& m,
+                                               const SplitBufs& sb, Ctrl* ctrl, const PairLaunch& pl, int nparts,
+                                               float omega, float pjuge, int ntt, int ntj, int kchunk, int remap,
+                                               const float* __restrict__ pent, int bid) {
+  constexpr int TY = PTY, W = FX, R = PR, GNT = PNT, FTYT = PFTY, FTXW = PFTX, PL = TY * W;
+  __shared__ float ldsB[3 * PL];   // black iterate of l-1, level L in plane L mod 3 (by step)
+  __shared__ float ldsR[2 * PL];   // A: red of l
+  __shared__ float ldsB2[2 * PL];  // B: black of l
+  __shared__ float ldsR2[2 * PL];  // C: red of l+1
+  __shared__ double lds[GNT / kWave];
+  __shared__ int sh[5];
+  extern __shared__ float4 lvm4[];  // per level from krs: cn4l, cn4s, cncz
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.y);
+  const int lane = (int)threadIdx.x;
+  const int tid = w * FX + lane;
+  if (tid == 0) {
+    int skip = ctrl->sor_done != 0, it = pl.iter, si = pl.in_set, so = pl.out_set;
+    if (pl.fixup) {
+      skip = ctrl->fix_iter == 0;
+      it = ctrl->fix_iter;
+      si = ctrl->fix_in;
+      so = ctrl->fix_out;
+    }
+    sh[0] = skip;
+    sh[1] = it;
+    sh[2] = si;
+    sh[3] = so;
+    sh[4] = (it == 1) && ctrl->p_ghost_diff != 0;
+  }
+  __syncthreads();
+  if (sh[0]) return;
+  const int in_set = sh[2], out_set = sh[3];
+  const bool two = pl.two != 0 && !pl.fixup;
+  const bool ent = ENT && sh[4] != 0;  // iteration 1 on the entry ghosts of p
+  if (!pl.fixup && pl.prev_iter > 0 && bid == 0) {  // the previous launch's stop test
+    double s1 = 0.0, s2 = 0.0;
+    for (int b = tid; b < pl.prev_nparts; b += GNT) {
+      s1 += sb.part[pl.rpart][0][b];
+      s2 += sb.part[pl.rpart][1][b];
+    }
+    s1 = block_sum<GNT>(s1, tid, lds);
+    s2 = block_sum<GNT>(s2, tid, lds);
+    if (tid == 0) pair_stop(ctrl, pl.prev_iter, pl.prev_two, s1, s2, pl.prev_in, pl.prev_out, pjuge);
+  }
+  const int im = g.im, jm = g.jm, km = g.km, nt = s.nt, cp = s.cp;
+  const unsigned cplb = 4u * (unsigned)s.cplane;
+  const float* base = sb.base;
+  const Rsrc S = brsrc(base, (unsigned)sb.nplanes * cplb);
+  // byte offset of level 0 of each array: level L is at o + L * cplb
+  const unsigned oRo = boff(base, sb.p[in_set][1]) - cplb, oBo = boff(base, sb.p[in_set][0]) - cplb;
+  const unsigned oRn = boff(base, sb.p[out_set][1]) - cplb, oBn = boff(base, sb.p[out_set][0]) - cplb;
+  const unsigned orR = boff(base, sb.rhs[1]) - cplb, orB = boff(base, sb.rhs[0]) - cplb;
+  const int lb = remap ? xcd_contiguous(bid, nparts) : bid;
+  const int bt = lb % ntt, bj = (lb / ntt) % ntj, bk = lb / (ntt * ntj);
+  const int t = bt * FTXW - R + lane;
+  const bool tin = (t >= 0) && (t < nt);
+  const int rA = 2 * w, rB = rA + 1;
+  const int jlA = 1 + bj * FTYT - R + rA, jlB = jlA + 1;  // tile rows (may lie outside 1..jm); jlA odd
+  const int jA = jwrap(jlA, jm), jB = jwrap(jlB, jm);     // the rows they stand for
+  const int k0 = 1 + bk * kchunk, k1 = min(km, k0 + kchunk - 1);
+  // levels each stage computes: the next stage reads one level around
+  const int ra = two ? 3 : 1;
+  const int aLo = max(1, k0 - ra), aHi = min(km, k1 + ra);
+  const int bLo = max(1, k0 - ra + 1), bHi = min(km, k1 + ra - 1);
+  const int cLo = max(1, k0 - 1), cHi = min(km, k1 + 1);
+  const int krs = aLo, kend = k1 + ra;  // the steps: the last stage reaches k1 at kend
+  const int tq = tin ? t : 0;
+  const unsigned voA = 4u * (unsigned)(tq + (jA - 1) * cp), voB = 4u * (unsigned)(tq + (jB - 1) * cp);
+  const unsigned trA = 4u * (unsigned)(nt + (jA - 1) * cp), trB = 4u * (unsigned)(nt + (jB - 1) * cp);
+  const bool lane_tile = tin && lane >= R && lane <= W - 1 - R;
+  const bool tileA = rA >= R && rA <= TY - 1 - R && jlA <= jm, tileB = rB >= R && rB <= TY - 1 - R && jlB <= jm;
+  // waves whose rows a stage's result is needed on (wave-uniform): B on
+  // rows 2..TY-3, C on 3..TY-4, D on the tile rows 4..TY-5
+  const bool wB = 2 * w + 1 >= 2 && 2 * w <= TY - 3;
+  const bool wC = two && 2 * w + 1 >= 3 && 2 * w <= TY - 4;
+  const bool wD = two && 2 * w + 1 >= 4 && 2 * w <= TY - 5;
+  float c2l[2] = {0.f, 0.f}, c2s[2] = {0.f, 0.f}, axp[2] = {0.f, 0.f};
+  if (tin) {
+    for (int p = 0; p < 2; p++) {
+      c2l[p] = M1(m.cn2l, 2 * t + 1 + p);
+      c2s[p] = M1(m.cn2s, 2 * t + 1 + p);
+      axp[p] = M1(m.cnax, 2 * t + 1 + p);
+    }
+  }
+  const float c3lA = M1(m.cn3l, jA), c3sA = M1(m.cn3s, jA), c3lB = M1(m.cn3l, jB), c3sB = M1(m.cn3s, jB);
+  const float byA = M1(m.cnby, jA), byB = M1(m.cnby, jB);
+  float sA[2], sB[2];
+  unsigned svA[2], svB[2];
+  bool okA[2], okB[2], last[2];
+  for (int p = 0; p < 2; p++) {
+    sA[p] = axp[p] + byA;
+    sB[p] = axp[p] + byB;
+    okA[p] = lane_tile && tileA && (2 * t + 1 + p <= im);
+    okB[p] = lane_tile && tileB && (2 * t + 1 + p <= im);
+    svA[p] = okA[p] ? voA : trA;
+    svB[p] = okB[p] ? voB : trB;
+    last[p] = (2 * t + 1 + p == im);
+  }
+  const bool first = (t == 0);  // i = 1 (position 0)
+  const int ownA = rA * W + lane;
+  const int bsA = min(max(ownA - 1, 0), PL - 3), bsB = min(max(ownA + W - 1, 0), PL - 3);
+  const int upA = (rA == 0 ? rA : rA - 1) * W + lane;       // row A-1 (row 0 reads itself, unused)
+  const int dnB = (rB == TY - 1 ? rB : rB + 1) * W + lane;  // row B+1
+  const int nsteps = kend - krs + 1;
+  for (int x = tid; x < nsteps; x += GNT) {
+    const int k = krs + x;
+    float4 v = {0.f, 0.f, 0.f, 0.f};
+    if (k <= km) {
+      v.x = M1(m.cn4l, k);
+      v.y = M1(m.cn4s, k);
+      v.z = M1(m.cncz, k);
+    }
+    lvm4[x] = v;
+  }
+  // streams, two steps ahead (slots by step mod 2): A's own red of l-1 and
+  // rhs at level L, B's rhs at level L-1; levels past their last use are
+  // clamped to it (a reload of a line just read costs no HBM traffic)
+  struct Sl {
+    float roA, rrA, rbA, roB, rrB, rbB;
+  };
+  auto fill = [&](Sl& sl, int L) {
+    const unsigned lr = (unsigned)min(L, aHi) * cplb, lk = (unsigned)min(max(L - 1, 1), bHi) * cplb;
+    sl.roA = bload(S, voA, oRo + lr);
+    sl.roB = bload(S, voB, oRo + lr);
+    sl.rrA = bload(S, voA, orR + lr);
+    sl.rrB = bload(S, voB, orR + lr);
+    sl.rbA = bload(S, voA, orB + lk);
+    sl.rbB = bload(S, voB, orB + lk);
+  };
+  Sl sl[2];
+  // rings, by level: the black iterate of l-1 at (L - krs + 1) mod 6, four
+  // levels ahead; A's red, B's black and C's red at (L - krs) mod 3
+  float bqA[6], bqB[6];
+  float rdA[3] = {0.f, 0.f, 0.f}, rdB[3] = {0.f, 0.f, 0.f};
+  float blA[3] = {0.f, 0.f, 0.f}, blB[3] = {0.f, 0.f, 0.f};
+  float r2A[3] = {0.f, 0.f, 0.f}, r2B[3] = {0.f, 0.f, 0.f};
+  // Level km+1 is the ghost boundp2 fills from level km: for A's red update
+  // at km the cell itself, so that slot takes the red iterate of level km.
+  const int lbq = min(aHi + 1, km + 1);
+  auto loadbq = [&](float& a, float& b, int L) {
+    const int Lc = min(L, lbq);
+    const unsigned o = (Lc == km + 1) ? oRo + (unsigned)km * cplb : oBo + (unsigned)Lc * cplb;
+    a = bload(S, voA, o);
+    b = bload(S, voB, o);
+  };
+  fill(sl[0], krs);
+  fill(sl[1], krs + 1);
+  for (int q = 0; q < 5; q++) loadbq(bqA[q], bqB[q], krs - 1 + q);  // levels krs-1 .. krs+3
+  float* const LB = ldsB;
+  float* const LR = ldsR;
+  float* const LB2 = ldsB2;
+  float* const LR2 = ldsR2;
+  LB[ownA] = bqA[1];  // level krs into plane 0
+  LB[ownA + W] = bqB[1];
+  if (krs == 1) {
+    // level 0 is the ghost boundp2 fills from level 1: the k-1 neighbour of
+    // A's red update at level 1 is the red cell itself, that of B's black
+    // update at level 1 the black cell itself
+    bqA[0] = sl[0].roA;
+    bqB[0] = sl[0].roB;
+    rdA[2] = bqA[1];
+    rdB[2] = bqB[1];
+  }
+  double acc1 = 0.0, acc2 = 0.0;
+  __syncthreads();  // lvm4 and plane 0 are complete
+
+  auto march = [&](auto D0C) {
+    constexpr int D0 = decltype(D0C)::value;
+    auto step = [&](auto UC, int kr) {
+      constexpr int U = decltype(UC)::value;
+      constexpr int P = U & 1;
+      constexpr int pa = D0 ^ P, pb = pa ^ 1;
+      const int kb = kr - 1, kc = kr - 2, kd = kr - 3;
+      const bool actA = kr <= aHi;
+      const bool actB = kb >= bLo && kb <= bHi;
+      const bool actC = two && kc >= cLo && kc <= cHi;
+      const bool actD = two && kd >= k0 && kd <= k1;
+      // 1. the rhs of C's and D's cells, read again (the lines were read two
+      // steps earlier for A and B, so they come from the caches), and the
+      // black iterate of l-1 at level kr+1 into its plane
+      float hrA = 0.f, hrB = 0.f, hbA = 0.f, hbB = 0.f;
+      if (two) {
+        const unsigned lc = (unsigned)min(max(kc, 1), km) * cplb, ld = (unsigned)min(max(kd, 1), km) * cplb;
+        hrA = bload(S, voA, orR + lc);
+        hrB = bload(S, voB, orR + lc);
+        hbA = bload(S, voA, orB + ld);
+        hbB = bload(S, voB, orB + ld);
+      }
+      LB[((U + 1) % 3) * PL + ownA] = bqA[(U + 2) % 6];
+      LB[((U + 1) % 3) * PL + ownA + W] = bqB[(U + 2) % 6];
+      __syncthreads();
+      Sl& q = sl[U % 2];
+      // ---- A: red of iteration l, level kr (press.f95:93-104)
+      float redA = 0.f, redB = 0.f;
+      if (actA) {
+        const float4 mt = lvm4[kr - krs];
+        const float* pp = LB + (U % 3) * PL;
+        const bool kin = kr >= k0 && kr <= k1;
+        {
+          const float own = q.roA;
+          float pim = (pa == 0 && first) ? own : pp[bsA + pa];
+          float pip = last[pa] ? own : pp[bsA + pa + 1];
+          float pjm = pp[upA];
+          float pjp = bqB[(U + 1) % 6];
+          float pkm = bqA[U % 6];
+          float pkp = bqA[(U + 2) % 6];
+          if (ENT && ent && tin) {  // iteration 1: the ghost values p had on entry to press
+            const int i = 2 * t + 1 + pa;
+            if (pa == 0 && first) pim = pent[ix(g, 0, jA, kr)];
+            if (last[pa]) pip = pent[ix(g, im + 1, jA, kr)];
+            if (jA == 1) pjm = pent[ix(g, i, 0, kr)];
+            if (jA == jm) pjp = pent[ix(g, i, jm + 1, kr)];
+            if (kr == 1) pkm = pent[ix(g, i, jA, 0)];
+            if (kr == km) pkp = pent[ix(g, i, jA, km + 1)];
+          }
+          const float cn = rcp_rn(sA[pa] + mt.z);  // init.f95:100-107
+          const float reltmp = relax<OM1>(omega, cn * (c2l[pa] * pip + c2s[pa] * pim + c3lA * pjp + c3sA * pjm +
+                                                       mt.x * pkp + mt.y * pkm - q.rrA) -
+                                                     own);
+          redA = own + reltmp;
+          acc1 = acc_sq(acc1, reltmp, okA[pa] && kin);
+          if (!two) bstore(S, redA, svA[pa], kin ? oRn + (unsigned)kr * cplb : 0u);
+        }
+        {
+          const float own = q.roB;
+          float pim = (pb == 0 && first) ? own : pp[bsB + pb];
+          float pip = last[pb] ? own : pp[bsB + pb + 1];
+          float pjm = bqA[(U + 1) % 6];
+          float pjp = pp[dnB];
+          float pkm = bqB[U % 6];
+          float pkp = bqB[(U + 2) % 6];
+          if (ENT && ent && tin) {
+            const int i = 2 * t + 1 + pb;
+            if (pb == 0 && first) pim = pent[ix(g, 0, jB, kr)];
+            if (last[pb]) pip = pent[ix(g, im + 1, jB, kr)];
+            if (jB == 1) pjm = pent[ix(g, i, 0, kr)];
+            if (jB == jm) pjp = pent[ix(g, i, jm + 1, kr)];
+            if (kr == 1) pkm = pent[ix(g, i, jB, 0)];
+            if (kr == km) pkp = pent[ix(g, i, jB, km + 1)];
+          }
+          const float cn = rcp_rn(sB[pb] + mt.z);
+          const float reltmp = relax<OM1>(omega, cn * (c2l[pb] * pip + c2s[pb] * pim + c3lB * pjp + c3sB * pjm +
+                                                       mt.x * pkp + mt.y * pkm - q.rrB) -
+                                                     own);
+          redB = own + reltmp;
+          acc1 = acc_sq(acc1, reltmp, okB[pb] && kin);
+          if (!two) bstore(S, redB, svB[pb], kin ? oRn + (unsigned)kr * cplb : 0u);
+        }
+        LR[P * PL + ownA] = redA;
+        LR[P * PL + ownA + W] = redB;
+        rdA[U % 3] = redA;
+        rdB[U % 3] = redB;
+      }
+      // ---- B: black of iteration l, level kb = kr-1, from A's red of levels kb-1..kb+1
+      if (actB && wB) {
+        const float4 mt = lvm4[kb - krs];
+        const float* pr = LR + (P ^ 1) * PL;
+        const bool kin = kb >= k0 && kb <= k1;
+        const bool top = kb == km;  // level km+1 is the ghost boundp2 fills from level km: the cell itself
+        float bA, bB;
+        {
+          const float own = bqA[U % 6];
+          const float pim = (pa == 0 && first) ? own : pr[bsA + pa];
+          const float pip = last[pa] ? own : pr[bsA + pa + 1];
+          const float pjm = pr[upA];
+          const float pjp = rdB[(U + 2) % 3];
+          float pkm = rdA[(U + 1) % 3];
+          float pkp = top ? own : redA;
+          if (ENT && ent && tin) {  // iteration 1: boundp2 refreshes the k ghosts only after the black sweep
+            const int i = 2 * t + 1 + pa;
+            if (kb == 1) pkm = pent[ix(g, i, jA, 0)];
+            if (kb == km) pkp = pent[ix(g, i, jA, km + 1)];
+          }
+          const float cn = rcp_rn(sA[pa] + mt.z);
+          const float reltmp = relax<OM1>(omega, cn * (c2l[pa] * pip + c2s[pa] * pim + c3lA * pjp + c3sA * pjm +
+                                                       mt.x * pkp + mt.y * pkm - q.rbA) -
+                                                     own);
+          bA = own + reltmp;
+          acc1 = acc_sq(acc1, reltmp, okA[pa] && kin);
+          if (!two) bstore(S, bA, svA[pa], kin ? oBn + (unsigned)kb * cplb : 0u);
+        }
+        {
+          const float own = bqB[U % 6];
+          const float pim = (pb == 0 && first) ? own : pr[bsB + pb];
+          const float pip = last[pb] ? own : pr[bsB + pb + 1];
+          const float pjm = rdA[(U + 2) % 3];
+          const float pjp = pr[dnB];
+          float pkm = rdB[(U + 1) % 3];
+          float pkp = top ? own : redB;
+          if (ENT && ent && tin) {
+            const int i = 2 * t + 1 + pb;
+            if (kb == 1) pkm = pent[ix(g, i, jB, 0)];
+            if (kb == km) pkp = pent[ix(g, i, jB, km + 1)];
+          }
+          const float cn = rcp_rn(sB[pb] + mt.z);
+          const float reltmp = relax<OM1>(omega, cn * (c2l[pb] * pip + c2s[pb] * pim + c3lB * pjp + c3sB * pjm +
+                                                       mt.x * pkp + mt.y * pkm - q.rbB) -
+                                                     own);
+          bB = own + reltmp;
+          acc1 = acc_sq(acc1, reltmp, okB[pb] && kin);
+          if (!two) bstore(S, bB, svB[pb], kin ? oBn + (unsigned)kb * cplb : 0u);
+        }
+        LB2[P * PL + ownA] = bA;
+        LB2[P * PL + ownA + W] = bB;
+        blA[(U + 2) % 3] = bA;
+        blB[(U + 2) % 3] = bB;
+      }
+      // ---- C: red of iteration l+1, level kc = kr-2, from B's black of levels kc-1..kc+1
+      float r2a = 0.f, r2b = 0.f;
+      if (actC && wC) {
+        const float4 mt = lvm4[kc - krs];
+        const float* pq = LB2 + (P ^ 1) * PL;
+        const bool kin = kc >= k0 && kc <= k1;
+        const unsigned so = kin ? oRn + (unsigned)kc * cplb : 0u;
+        {
+          const float own = rdA[(U + 1) % 3];
+          const float pim = (pa == 0 && first) ? own : pq[bsA + pa];
+          const float pip = last[pa] ? own : pq[bsA + pa + 1];
+          const float pjm = pq[upA];
+          const float pjp = blB[(U + 1) % 3];
+          const float pkm = (kc == 1) ? own : blA[U % 3];
+          const float pkp = (kc == km) ? own : blA[(U + 2) % 3];
+          const float cn = rcp_rn(sA[pa] + mt.z);
+          const float reltmp = relax<OM1>(omega, cn * (c2l[pa] * pip + c2s[pa] * pim + c3lA * pjp + c3sA * pjm +
+                                                       mt.x * pkp + mt.y * pkm - hrA) -
+                                                     own);
+          r2a = own + reltmp;
+          acc2 = acc_sq(acc2, reltmp, okA[pa] && kin);
+          bstore(S, r2a, svA[pa], so);
+        }
+        {
+          const float own = rdB[(U + 1) % 3];
+          const float pim = (pb == 0 && first) ? own : pq[bsB + pb];
+          const float pip = last[pb] ? own : pq[bsB + pb + 1];
+          const float pjm = blA[(U + 1) % 3];
+          const float pjp = pq[dnB];
+          const float pkm = (kc == 1) ? own : blB[U % 3];
+          const float pkp = (kc == km) ? own : blB[(U + 2) % 3];
+          const float cn = rcp_rn(sB[pb] + mt.z);
+          const float reltmp = relax<OM1>(omega, cn * (c2l[pb] * pip + c2s[pb] * pim + c3lB * pjp + c3sB * pjm +
+                                                       mt.x * pkp + mt.y * pkm - hrB) -
+                                                     own);
+          r2b = own + reltmp;
+          acc2 = acc_sq(acc2, reltmp, okB[pb] && kin);
+          bstore(S, r2b, svB[pb], so);
+        }
+        LR2[P * PL + ownA] = r2a;
+        LR2[P * PL + ownA + W] = r2b;
+        r2A[(U + 1) % 3] = r2a;
+        r2B[(U + 1) % 3] = r2b;
+      }
+      // ---- D: black of iteration l+1, level kd = kr-3, from C's red of levels kd-1..kd+1
+      if (actD && wD) {
+        const float4 mt = lvm4[kd - krs];
+        const float* pr = LR2 + (P ^ 1) * PL;
+        const unsigned so = oBn + (unsigned)kd * cplb;  // kd is in the chunk whenever D runs
+        {
+          const float own = blA[U % 3];
+          const float pim = (pa == 0 && first) ? own : pr[bsA + pa];
+          const float pip = last[pa] ? own : pr[bsA + pa + 1];
+          const float pjm = pr[upA];
+          const float pjp = r2B[U % 3];
+          const float pkm = (kd == 1) ? own : r2A[(U + 2) % 3];
+          const float pkp = (kd == km) ? own : r2a;
+          const float cn = rcp_rn(sA[pa] + mt.z);
+          const float reltmp = relax<OM1>(omega, cn * (c2l[pa] * pip + c2s[pa] * pim + c3lA * pjp + c3sA * pjm +
+                                                       mt.x * pkp + mt.y * pkm - hbA) -
+                                                     own);
+          acc2 = acc_sq(acc2, reltmp, okA[pa]);
+          bstore(S, own + reltmp, svA[pa], so);
+        }
+        {
+          const float own = blB[U % 3];
+          const float pim = (pb == 0 && first) ? own : pr[bsB + pb];
+          const float pip = last[pb] ? own : pr[bsB + pb + 1];
+          const float pjm = r2A[U % 3];
+          const float pjp = pr[dnB];
+          const float pkm = (kd == 1) ? own : r2B[(U + 2) % 3];
+          const float pkp = (kd == km) ? own : r2b;
+          const float cn = rcp_rn(sB[pb] + mt.z);
+          const float reltmp = relax<OM1>(omega, cn * (c2l[pb] * pip + c2s[pb] * pim + c3lB * pjp + c3sB * pjm +
+                                                       mt.x * pkp + mt.y * pkm - hbB) -
+                                                     own);
+          acc2 = acc_sq(acc2, reltmp, okB[pb]);
+          bstore(S, own + reltmp, svB[pb], so);
+        }
+      }
+      // 4. operands of later steps
+      fill(q, kr + 2);
+      loadbq(bqA[(U + 5) % 6], bqB[(U + 5) % 6], kr + 4);
+    };
+    using I0 = std::integral_constant<int, 0>;
+    using I1 = std::integral_constant<int, 1>;
+    using I2 = std::integral_constant<int, 2>;
+    using I3 = std::integral_constant<int, 3>;
+    using I4 = std::integral_constant<int, 4>;
+    using I5 = std::integral_constant<int, 5>;
+    int kr = krs;
+    for (; kr + 5 <= kend; kr += 6) {
+      step(I0{}, kr);
+      step(I1{}, kr + 1);
+      step(I2{}, kr + 2);
+      step(I3{}, kr + 3);
+      step(I4{}, kr + 4);
+      step(I5{}, kr + 5);
+    }
+    const int rest = kend - kr + 1;  // 0..5 steps left, block-uniform
+    if (rest > 0) step(I0{}, kr);
+    if (rest > 1) step(I1{}, kr + 1);
+    if (rest > 2) step(I2{}, kr + 2);
+    if (rest > 3) step(I3{}, kr + 3);
+    if (rest > 4) step(I4{}, kr + 4);
+  };
+  // row A (jA odd) at level krs: position (jA + krs) & 1
+  if (((1 + krs) & 1) == 0)
+    march(std::integral_constant<int, 0>{});
+  else
+    march(std::integral_constant<int, 1>{});
+  const double t1 = block_sum<GNT>(acc1, tid, lds);
+  const double t2 = block_sum<GNT>(acc2, tid, lds);
+  if (tid == 0) {
+    sb.part[pl.wpart][0][bid] = t1;
+    sb.part[pl.wpart][1][bid] = t2;
+  }
+}
+
+template <bool OM1, bool ENT>
+__global__ __launch_bounds__(PNT) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_sor_pair(
+    Grid g, SplitGrid s, Metrics m, SplitBufs sb, Ctrl* ctrl, PairLaunch pl, int nparts, float omega, float pjuge,
+    int ntt, int ntj, int kchunk, int remap, const float* __restrict__ pent) {
+  sor_pair_block<OM1, ENT>(g, s, m, sb, ctrl, pl, nparts, omega, pjuge, ntt, ntj, kchunk, remap, pent,
+                           (int)blockIdx.x);
+}
+
+// After the last two-iteration launch: its stop test, unless an earlier one
+// stopped the loop.
+__global__ __launch_bounds__(256) void k_sor_pair_final(SplitBufs sb, Ctrl* ctrl, PairLaunch last, int nparts,
+                                                        float pjuge) {
+  __shared__ double lds[4];
+  if (ctrl->sor_done != 0) return;  // uniform: written by earlier launches
+  double s1 = 0.0, s2 = 0.0;
+  for (int b = threadIdx.x; b < nparts; b += 256) {
+    s1 += sb.part[last.wpart][0][b];
+    s2 += sb.part[last.wpart][1][b];
+  }
+  s1 = block_sum<256>(s1, threadIdx.x, lds);
+  s2 = block_sum<256>(s2, threadIdx.x, lds);
+  if (threadIdx.x == 0) pair_stop(ctrl, last.iter, last.two, s1, s2, last.in_set, last.out_set, pjuge);
+}
+
 // After the iteration loop: when no iteration met the stop test, the last
 // one (nmaxp) is summed here.  Records the iteration count and sum.
 __device__ __forceinline__ void sor_final_body(const SplitBufs& sb, Ctrl* ctrl, int nmaxp, int nparts) {
@@ -1000,9 +1525,9 @@ __device__ __forceinline__ void pav_body(const VBlk& vb, const Grid& g, const Sp
   __shared__ double lds[NT / kWave];
   const int tid = threadIdx.y * BX + threadIdx.x;
   const int bid = vlinear(vb);
-  const int L = ctrl->sor_iters;
-  const float* __restrict__ red = sb.p[L & 1][1];
-  const float* __restrict__ blk = sb.p[L & 1][0];
+  const int fs = ctrl->p_final >= 0 ? ctrl->p_final : (ctrl->sor_iters & 1);  // the set holding the result
+  const float* __restrict__ red = sb.p[fs][1];
+  const float* __restrict__ blk = sb.p[fs][0];
   const int i = 1 + vb.x * BX + threadIdx.x, j = 1 + vb.y * BY + threadIdx.y;
   double acc = 0.0;
   if (i <= g.im && j <= g.jm) {
@@ -1032,9 +1557,9 @@ __device__ __forceinline__ void padj_body(const VBlk& vb, const Grid& g, const S
                                           float* __restrict__ p, const Ctrl* ctrl) {
   const int i = vb.x * BX + threadIdx.x, j = vb.y * BY + threadIdx.y;
   if (i > g.im + 1 || j > g.jm + 1) return;
-  const int L = ctrl->sor_iters;
-  const float* __restrict__ red = sb.p[L & 1][1];
-  const float* __restrict__ blk = sb.p[L & 1][0];
+  const int fs = ctrl->p_final >= 0 ? ctrl->p_final : (ctrl->sor_iters & 1);  // the set holding the result
+  const float* __restrict__ red = sb.p[fs][1];
+  const float* __restrict__ blk = sb.p[fs][0];
   const double av = ctrl->pav;
   const int ii = (i == 0) ? 1 : ((i == g.im + 1) ? g.im : i);
   const int jj = (j == 0) ? g.jm : ((j == g.jm + 1) ? 1 : j);
@@ -1054,11 +1579,9 @@ __global__ __launch_bounds__(NT) void k_padj(Grid g, SplitGrid s, SplitBufs sb, 
 
 // After the SOR loop: the closing stop-test sum, the pressure mean and the
 // adjusted p with its ghosts.
-static void launch_press_tail(const DevDomain& d, const SplitBufs& sb, int nparts, int gx, int gy, int gz,
-                              int nred) {
+static void launch_press_mean(const DevDomain& d, const SplitBufs& sb, int gx, int gy, int gz, int nred) {
   const Grid& g = d.g;
   const SplitGrid& s = d.sg;
-  TIMED(d, KC_PAV, hipLaunchKernelGGL(k_sor_final, dim3(1), dim3(256), 0, d.stream, sb, d.ctrl, d.nmaxp, nparts));
   TIMED(d, KC_PAV, hipLaunchKernelGGL(k_pav, dim3(gx, gy, gz), blk(), 0, d.stream, g, s, d.m, sb, d.partials,
                                       (const Ctrl*)d.ctrl));
   TIMED(d, KC_PAV, hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, d.stream, d.partials, nred, d.ctrl, 1,
@@ -1067,6 +1590,11 @@ static void launch_press_tail(const DevDomain& d, const SplitBufs& sb, int npart
   const int pz = g.km + 2;
   TIMED(d, KC_PADJ, hipLaunchKernelGGL(k_padj, dim3(px, py, pz),
                                        blk(), 0, d.stream, g, s, sb, d.F.p, (const Ctrl*)d.ctrl));
+}
+static void launch_press_tail(const DevDomain& d, const SplitBufs& sb, int nparts, int gx, int gy, int gz,
+                              int nred) {
+  TIMED(d, KC_PAV, hipLaunchKernelGGL(k_sor_final, dim3(1), dim3(256), 0, d.stream, sb, d.ctrl, d.nmaxp, nparts));
+  launch_press_mean(d, sb, gx, gy, gz, nred);
 }
 
 // ------------------------------------------- FP64 defect correction
@@ -1238,9 +1766,20 @@ struct SorShape {
   int rows, ty, tw, kc, ntt, ntj, nparts;  // tw: tile width in pairs with the ring (64, or 32 with 32 rows)
 };
 
+// rows 4: two iterations per launch (k_sor_pair), 32 x 64 tiles with a
+// four-cell ring (callers pass it only for separable cn1).
 static SorShape sor_shape(const Grid& g, const SplitGrid& s, int rows, int ty, int kc, int tw = FX) {
   SorShape h;
   h.rows = ((g.jm & 1) != 0) ? 1 : rows;  // odd jm: one row per thread
+  if (h.rows == 4) {
+    h.ty = PTY;
+    h.tw = FX;
+    h.kc = std::max(1, std::min(kc, g.km));
+    h.ntt = (s.nt + PFTX - 1) / PFTX;
+    h.ntj = (g.jm + PFTY - 1) / PFTY;
+    h.nparts = h.ntt * h.ntj * ((g.km + h.kc - 1) / h.kc);
+    return h;
+  }
   h.ty = (h.rows == 1) ? FY : ty;          // k_sor_iter has 16-row tiles
   h.tw = (h.rows == 2 && h.ty == 32 && tw == FX / 2) ? FX / 2 : FX;
   h.kc = std::max(1, std::min(kc, g.km));
@@ -1250,10 +1789,77 @@ static SorShape sor_shape(const Grid& g, const SplitGrid& s, int rows, int ty, i
   return h;
 }
 
-// The domain's shape: its settings, or the automatic k chunk.
+// The domain's shape: its settings, or the automatic k chunk.  Two
+// iterations per launch need separable cn1 (the third iterate set takes the
+// storage of the split cn1) and even jm.
 static SorShape sor_shape(const DevDomain& d) {
-  const int kc = d.sor_kchunk > 0 ? d.sor_kchunk : sor_fused_kchunk(d.g, d.sg, d.sor_fused_blocks, d.ncu);
-  return sor_shape(d.g, d.sg, d.sor_rows, d.sor_tile_rows, kc, d.sor_tile_width);
+  const int rows = (d.sor_rows == 4 && !d.cn1_separable) ? 2 : d.sor_rows;
+  int kc = d.sor_kchunk;
+  if (kc <= 0)
+    kc = (rows == 4 && (d.g.jm & 1) == 0) ? sor_pair_kchunk(d.g, d.sg, d.ncu)
+                                           : sor_fused_kchunk(d.g, d.sg, d.sor_fused_blocks, d.ncu);
+  return sor_shape(d.g, d.sg, rows, d.sor_tile_rows, kc, d.sor_tile_width);
+}
+
+// The shape of one iteration per launch (the paths that run iterations one
+// at a time: press mode 1's inner solves).
+static SorShape sor_shape_single(const DevDomain& d) {
+  if (d.sor_rows != 4) return sor_shape(d);
+  return sor_shape(d.g, d.sg, 2, d.sor_tile_rows, sor_fused_kchunk(d.g, d.sg, d.sor_fused_blocks, d.ncu),
+                   d.sor_tile_width);
+}
+
+// One launch of k_sor_pair (two iterations, one, or the fix-up).
+static void launch_pair(const DevDomain& d, const SplitBufs& sb, const PairLaunch& pl, const SorShape& h, float pjuge,
+                        const float* pent) {
+  const size_t lvm_bytes = (size_t)(h.kc + 7) * sizeof(float4);
+  const bool ent = (pl.iter == 1 || pl.fixup) && pent;  // may run iteration 1 on the entry ghosts
+  const int cls = ent ? KC_SOR1 : KC_SOR;
+#define PAIR(OM1, ENT)                                                                                          \
+  TIMED(d, cls, hipLaunchKernelGGL((k_sor_pair<OM1, ENT>), dim3(h.nparts), dim3(FX, PTY / 2), lvm_bytes, d.stream, \
+                                   d.g, d.sg, d.m, sb, d.ctrl, pl, h.nparts, d.omega, pjuge, h.ntt, h.ntj, h.kc,   \
+                                   d.xcd_remap, pent))
+  if (d.omega == 1.0f) {
+    if (ent)
+      PAIR(true, true);
+    else
+      PAIR(true, false);
+  } else {
+    if (ent)
+      PAIR(false, true);
+    else
+      PAIR(false, false);
+  }
+#undef PAIR
+}
+
+// Iterations l0..l1 in launches of two (the last alone when their number is
+// odd), reading iterate set in0 first and rotating over the three sets;
+// returns the last launch.  pent: the entry values of p for a launch of
+// iteration 1.
+static PairLaunch launch_sor_pairs(const DevDomain& d, const SplitBufs& sb, const SorShape& h, float pjuge, int l0,
+                                   int l1, int in0, const float* pent) {
+  PairLaunch prev{};
+  int in = in0, q = 0;
+  for (int l = l0; l <= l1; l += 2, q++) {
+    PairLaunch pl{};
+    pl.iter = l;
+    pl.two = (l + 1 <= l1) ? 1 : 0;
+    pl.in_set = in;
+    pl.out_set = (in + 1) % 3;  // the set before in_set stays intact for a fix-up
+    pl.wpart = q & 1;
+    pl.prev_iter = prev.iter;
+    pl.prev_two = prev.two;
+    pl.prev_in = prev.in_set;
+    pl.prev_out = prev.out_set;
+    pl.prev_nparts = h.nparts;
+    pl.rpart = prev.wpart;
+    pl.fixup = 0;
+    launch_pair(d, sb, pl, h, pjuge, pent);
+    prev = pl;
+    in = pl.out_set;
+  }
+  return prev;
 }
 
 // One fused SOR iteration l with shape h; nprev is the previous launch's
@@ -1323,7 +1929,7 @@ static void launch_sor_iter(const DevDomain& d, const SplitBufs& sb, int l, floa
 static void launch_press_dc(const DevDomain& d, const SplitBufs& sb, int gx, int gy, int gz, int nred) {
   const Grid& g = d.g;
   const SplitGrid& s = d.sg;
-  const SorShape h = sor_shape(d);
+  const SorShape h = sor_shape_single(d);
   double* P64 = d.dc64;
   double* R64 = d.dc64 + d.n3;
   double* hist = d.dc64 + 2 * (size_t)d.n3;
@@ -1558,6 +2164,19 @@ void launch_press(const DevDomain& d, float dt) {
   }
   if (d.sor_fused) {
     const SorShape h = sor_shape(d);
+    if (h.rows == 4) {
+      // two iterations per launch; the launch of iteration 1 reads the entry
+      // ghosts of p where they differ from their boundp copies
+      const PairLaunch last = launch_sor_pairs(d, sb, h, d.pjuge, 1, d.nmaxp, 0, d.F.p);
+      TIMED(d, KC_PAV, hipLaunchKernelGGL(k_sor_pair_final, dim3(1), dim3(256), 0, d.stream, sb, d.ctrl, last,
+                                          h.nparts, d.pjuge));
+      PairLaunch fix{};
+      fix.fixup = 1;
+      fix.wpart = last.wpart ^ 1;
+      launch_pair(d, sb, fix, h, d.pjuge, d.F.p);
+      launch_press_mean(d, sb, gx, gy, gz, nred);
+      return;
+    }
     // iteration 1 reads the ghost values p had on entry (k_sor_iter, 16-row
     // tiles).  When the iteration's own kernel has the same tiles and chunks
     // (so the same partial-sum slots), both are launched: it runs iteration
@@ -1661,8 +2280,37 @@ void launch_sor_trial(const DevDomain& d0, int rows, int ty, int tw, int kc, int
   d.timer = nullptr;
   SplitBufs sb = d.sb;
   if (d.cn1_separable) sb.cn1[0] = sb.cn1[1] = nullptr;
-  const SorShape h = sor_shape(d.g, d.sg, rows, ty, kc, tw);
+  const SorShape h = sor_shape(d.g, d.sg, (rows == 4 && !d.cn1_separable) ? 2 : rows, ty, kc, tw);
+  if (h.rows == 4) {  // iterations 2 .. iters+1 in launches of two (an even count: whole launches)
+    launch_sor_pairs(d, sb, h, 0.0f, 2, 1 + (iters + 1) / 2 * 2, 1, nullptr);
+    return;
+  }
   for (int l = 2; l < 2 + iters; l++) launch_sor_iter(d, sb, l, 0.0f, h, h.nparts);
+}
+
+// k levels per block of the two-iteration launch: the split that minimises
+// (rounds of resident blocks) x (steps per block), one 1024-thread block per
+// CU, each block marching its chunk and six more levels.
+int sor_pair_kchunk(const Grid& g, const SplitGrid& s, int ncu) {
+  const int tiles = ((s.nt + PFTX - 1) / PFTX) * ((g.jm + PFTY - 1) / PFTY);
+  const long slots = ncu > 0 ? ncu : 256;
+  long best = -1;
+  int best_kc = g.km;
+  for (int tz = 1; tz <= g.km; tz++) {
+    const int kc = (g.km + tz - 1) / tz;
+    const int nz = (g.km + kc - 1) / kc;
+    const long rounds = ((long)tiles * nz + slots - 1) / slots;
+    const long cost = rounds * (kc + 6);
+    if (best < 0 || cost < best) {
+      best = cost;
+      best_kc = kc;
+    }
+  }
+  return best_kc;
+}
+
+int sor_pair_tiles(const Grid& g, const SplitGrid& s) {
+  return ((s.nt + PFTX - 1) / PFTX) * ((g.jm + PFTY - 1) / PFTY);
 }
 
 int reduction_parts_needed(const Grid& g) {

@@ -60,6 +60,9 @@ struct Ctrl {
   int sor_iters;           // iterations executed in the current press call
   int nonfinite;           // a non-finite residual was met
   int p_ghost_diff;        // a ghost cell of p differs from the copy boundp would make (set by the rhs split)
+  int p_final;             // iterate set holding the SOR result (-1: the parity of sor_iters)
+  int fix_iter;            // two-iteration launches: iteration to recompute alone (0: none) ...
+  int fix_in, fix_out;     // ... from iterate set fix_in into set fix_out
   double rhsav;            // volume-weighted mean of rhs
   double pav;              // volume-weighted mean of p
   double sor_first;        // sum of reltmp^2 of the current iteration's first colour

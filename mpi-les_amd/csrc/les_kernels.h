@@ -27,7 +27,10 @@ struct SplitGrid {
   int cplane;  // plane pitch (floats)
 };
 struct SplitBufs {
-  float* p[2][2];       // [iterate parity][colour: 0 black, 1 red]
+  // [iterate set][colour: 0 black, 1 red].  One iteration per launch uses
+  // sets 0 and 1 by iteration parity; two iterations per launch rotate over
+  // all three (set 2 shares the storage of cn1, which is then recomputed).
+  float* p[3][2];
   float* rhs[2];        // [colour]
   float* cn1[2];        // [colour]; unused while cn1 is recomputed
   double* part[2][2];   // [iterate parity][colour] per-block sums of reltmp^2
@@ -75,7 +78,7 @@ struct DevDomain {
   double* dc64;      // P64, R64 (n3 each) and the defect history (kDcMaxOuter)
   int sor_fused_blocks;  // target number of blocks of one fused SOR iteration (0: automatic)
   int ncu;           // compute units of the device
-  int sor_rows;      // tile rows per thread of the fused SOR iteration (1: k_sor_iter, 2: k_sor_iter2)
+  int sor_rows;      // fused SOR kernel: 1 k_sor_iter, 2 k_sor_iter2 (one iteration per launch), 4 k_sor_pair (two)
   int sor_kchunk;    // k levels per block of the fused SOR iteration (0: from sor_fused_blocks)
   int sor_tile_rows; // tile rows (with the ring) of k_sor_iter2: 16 or 32
   int sor_tile_width;  // tile width in pairs (with the ring) of k_sor_iter2: 64, or 32 with 32 rows
@@ -87,6 +90,9 @@ int sor_parts_needed(const Grid& g, const SplitGrid& s, int target);
 int sor_fused_kchunk(const Grid& g, const SplitGrid& s, int target, int ncu);
 int sor_fused_parts_needed(const Grid& g, const SplitGrid& s, int target, int ncu);
 int sor_fused_tiles(const Grid& g, const SplitGrid& s);
+// the two-iterations-per-launch SOR (32 x 64 tiles with a four-cell ring)
+int sor_pair_kchunk(const Grid& g, const SplitGrid& s, int ncu);
+int sor_pair_tiles(const Grid& g, const SplitGrid& s);
 
 // One domain of a batched SOR iteration (k_sor_iter2_batch): its arguments
 // of k_sor_iter2<true, 16> and its range of blocks in the launch.

@@ -11,11 +11,12 @@
 #include <array>
 #include <map>
 #include <mutex>
+#include <set>
 #include <string>
 #include <tuple>
 #include <vector>
 
-#include "inc/les_amd.h"
+#include "../../include/les_amd.h"
 #include "les_kernels.h"
 
 using namespace les;
@@ -601,6 +602,7 @@ int domain_alloc(int im, int jm, int km, les_domain** out) {
     for (int c = 0; c < 2; c++) {
       sb.rhs[c] = arr0 + (4 + c) * cs;
       sb.cn1[c] = arr0 + (6 + c) * cs;
+      sb.p[2][c] = sb.cn1[c];  // the third iterate set of two-iteration launches (separable cn1 only)
     }
     sb.base = D->split;
     sb.nplanes = (int)nplanes;
@@ -643,7 +645,12 @@ int domain_alloc(int im, int jm, int km, les_domain** out) {
   D->d.visc_kchunk = std::max(0, env_int("LES_VISC_KCHUNK", 0));
   D->d.fuse_les = env_int("LES_FUSE_LES", 1);
   D->d.sor_fused_blocks = env_int("LES_SOR_FUSED_BLOCKS", 0);
-  D->d.sor_rows = env_int("LES_SOR_ROWS", 2) == 1 ? 1 : 2;
+  {
+    // fused SOR kernel: 4 = two iterations per launch (default; even jm and
+    // separable cn1, else 2), 2 = one iteration, two rows per thread, 1 = one row
+    const int r = env_int("LES_SOR_ROWS", 4);
+    D->d.sor_rows = (r == 1 || r == 2) ? r : 4;
+  }
   D->d.sor_kchunk = std::max(0, env_int("LES_SOR_KCHUNK", 0));
   D->d.sor_tile_rows = env_int("LES_SOR_TILE_ROWS", 16) == 32 ? 32 : 16;
   D->d.sor_tile_width = (D->d.sor_tile_rows == 32 && env_int("LES_SOR_TILE_WIDTH", 64) == 32) ? 32 : 64;
@@ -656,8 +663,9 @@ int domain_alloc(int im, int jm, int km, les_domain** out) {
   }
   // SOR partial slots for any k chunk of the fused iteration (down to one
   // level per block) and for the colour sweeps
-  const int nsor = std::max(sor_parts_needed(D->d.g, D->d.sg, D->d.sor_blocks),
-                            sor_fused_tiles(D->d.g, D->d.sg) * D->d.g.km);
+  const int nsor = std::max({sor_parts_needed(D->d.g, D->d.sg, D->d.sor_blocks),
+                             sor_fused_tiles(D->d.g, D->d.sg) * D->d.g.km,
+                             sor_pair_tiles(D->d.g, D->d.sg) * D->d.g.km});
   e = dev_alloc(&D->d.partials, ((size_t)D->d.npartials + 4 * (size_t)nsor) * sizeof(double));
   if (e == hipSuccess)
     for (int q = 0; q < 4; q++) D->d.sb.part[q >> 1][q & 1] = D->d.partials + D->d.npartials + (size_t)q * nsor;
@@ -866,27 +874,39 @@ int tune_sor(les_domain* D, int reps, TuneResult* out) {
   HIP_TRY(hipStreamSynchronize(D->d.stream));
   Ctrl saved;
   HIP_TRY(copy_sync(&saved, D->d.ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, D->d.stream));
-  const int tiles = sor_fused_tiles(g, D->d.sg);
   const int ncu = D->d.ncu > 0 ? D->d.ncu : 256;
-  const int kdef = sor_fused_kchunk(g, D->d.sg, D->d.sor_fused_blocks, D->d.ncu);
-  // candidate k chunks: every distinct split of km into nz chunks with
-  // between a quarter and eight times as many blocks as compute units
-  std::vector<int> kcs{kdef};
-  for (int nz = 1; nz <= g.km; nz++) {
-    const int kc = (g.km + nz - 1) / nz;
-    const long blocks = (long)tiles * ((g.km + kc - 1) / kc);
-    if (blocks * 4 < ncu || blocks > 8L * ncu) continue;
-    if (std::find(kcs.begin(), kcs.end(), kc) == kcs.end()) kcs.push_back(kc);
-  }
-  // (rows per thread, tile rows, tile width): k_sor_iter2 with 16 x 64,
-  // 32 x 64 or 32 x 32 tiles (even jm only), k_sor_iter
+  const bool pair_ok = (g.jm & 1) == 0 && D->d.cn1_separable;
+  // candidate k chunks of a kernel with `tiles` tiles: its automatic chunk
+  // and every distinct split of km into nz chunks with between a quarter
+  // and eight times as many blocks as compute units
+  auto chunks = [&](int tiles, int kdef) {
+    std::vector<int> kcs{kdef};
+    for (int nz = 1; nz <= g.km; nz++) {
+      const int kc = (g.km + nz - 1) / nz;
+      const long blocks = (long)tiles * ((g.km + kc - 1) / kc);
+      if (blocks * 4 < ncu || blocks > 8L * ncu) continue;
+      if (std::find(kcs.begin(), kcs.end(), kc) == kcs.end()) kcs.push_back(kc);
+    }
+    return kcs;
+  };
+  const int kdef1 = sor_fused_kchunk(g, D->d.sg, D->d.sor_fused_blocks, D->d.ncu);
+  const int kdefp = sor_pair_kchunk(g, D->d.sg, D->d.ncu);
+  const std::vector<int> kcs1 = chunks(sor_fused_tiles(g, D->d.sg), kdef1);
+  const std::vector<int> kcsp = chunks(sor_pair_tiles(g, D->d.sg), kdefp);
+  // (kernel, tile rows, tile width): k_sor_pair (4; two iterations per
+  // launch, 32 x 64 tiles), k_sor_iter2 (2) with 16 x 64, 32 x 64 or 32 x 32
+  // tiles (even jm only), k_sor_iter (1)
   std::vector<std::tuple<int, int, int>> kernels{{1, 16, 64}};
   if ((g.jm & 1) == 0) kernels.insert(kernels.begin(), {{2, 16, 64}, {2, 32, 64}, {2, 32, 32}});
+  if (pair_ok) kernels.insert(kernels.begin(), {4, 32, 64});
   // the shape used without tuning
+  const int rdef = (D->d.sor_rows == 4 && !pair_ok) ? 2 : D->d.sor_rows;
   const std::tuple<int, int, int> kdef2 =
-      ((g.jm & 1) == 0 && D->d.sor_rows == 2)
+      (rdef == 4 && pair_ok) ? std::make_tuple(4, 32, 64)
+      : ((g.jm & 1) == 0 && rdef >= 2)
           ? std::make_tuple(2, D->d.sor_tile_rows, D->d.sor_tile_rows == 32 ? D->d.sor_tile_width : 64)
           : std::make_tuple(1, 16, 64);
+  const int kdef = std::get<0>(kdef2) == 4 ? kdefp : kdef1;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   HIP_TRY(hipEventCreate(&e0));
   if (hipEventCreate(&e1) != hipSuccess) {
@@ -902,7 +922,7 @@ int tune_sor(les_domain* D, int reps, TuneResult* out) {
   r.kchunk = 0;
   for (const auto& kt : kernels) {
     const int rows = std::get<0>(kt), ty = std::get<1>(kt), tw = std::get<2>(kt);
-    for (int kc : kcs) {
+    for (int kc : (rows == 4 ? kcsp : kcs1)) {
       double us = 0.0;
       if ((rc = time_sor_trial(D, saved, rows, ty, tw, kc, reps, e0, e1, &us)) != 0) break;
       if (kt == kdef2 && kc == kdef) r.us_default = us;
@@ -974,6 +994,26 @@ int bind(les_domain* D, int f, const float* host) {
   if (ref_domain(im, jm, km, &D)) return 1
 
 }  // namespace
+
+// Handles handed out by les_domain_create and les_batch_create and not yet
+// destroyed.  Every domain and batch entry point checks its handle here, so
+// a call on a destroyed domain (or a batch step after one of its domains
+// was destroyed) is refused with an error instead of touching freed memory.
+std::mutex g_live_mu;
+std::set<const void*> g_live;
+void live_add(const void* p) {
+  std::lock_guard<std::mutex> lk(g_live_mu);
+  g_live.insert(p);
+}
+bool live_remove(const void* p) {
+  std::lock_guard<std::mutex> lk(g_live_mu);
+  return g_live.erase(p) == 1;
+}
+bool live_has(const void* p) {
+  if (!p) return false;
+  std::lock_guard<std::mutex> lk(g_live_mu);
+  return g_live.count(p) == 1;
+}
 
 namespace les {
 // For the entry points defined in other files (k_noalloc.hip).
@@ -1324,12 +1364,13 @@ int les_domain_create(const les_domain_desc* s, les_domain** out) {
   D->d.nmaxp = s->nmaxp;
   D->d.pjuge = s->pjuge;
   D->d.omega = s->omega;
+  live_add(D);
   *out = D;
   return 0;
 }
 
 int les_domain_set_press_mode(les_domain* D, int mode, int outer, int inner) {
-  if (!D) return fail("null domain");
+  if (!live_has(D)) return fail("null, unknown or destroyed domain");
   if (mode == 0) {
     D->d.press_mode = 0;
     return 0;
@@ -1386,13 +1427,14 @@ long long les_domain_bytes_needed(int im, int jm, int km) {
 
 int les_domain_destroy(les_domain* D) {
   if (!D) return 0;
+  if (!live_remove(D)) return fail("les_domain_destroy: unknown or already destroyed domain");
   hipStreamSynchronize(D->d.stream);
   domain_free(D);
   return 0;
 }
 
 int les_domain_stage(les_domain* D, int stage, int n, int nmax) {
-  if (!D) return fail("null domain");
+  if (!live_has(D)) return fail("null, unknown or destroyed domain");
   return run_stage(D, stage, n, nmax);
 }
 
@@ -1437,24 +1479,24 @@ static int step_graph(les_domain* D, int n, int nmax) {
 }
 
 int les_domain_step(les_domain* D, int n, int nmax) {
-  if (!D) return fail("null domain");
+  if (!live_has(D)) return fail("null, unknown or destroyed domain");
   if (D->use_graph && n != 1 && !D->d.timer) return step_graph(D, n, nmax);
   return run_step(D, n, nmax);
 }
 
 int les_domain_aveflow(les_domain* D, int n, int n1, int nmax) {
-  if (!D) return fail("null domain");
+  if (!live_has(D)) return fail("null, unknown or destroyed domain");
   return run_aveflow(D, n, n1, nmax);
 }
 
 int les_domain_set_inflow(les_domain* D, const float* u_in, const float* v_in, const float* w_in) {
-  if (!D) return fail("null domain");
+  if (!live_has(D)) return fail("null, unknown or destroyed domain");
   return set_inflow(D, u_in, v_in, w_in);
 }
 
 int les_domain_autotune(les_domain* D, int reps, int* rows, int* tile_rows, int* tile_width, int* kchunk,
                         double* us_best, double* us_default) {
-  if (!D) return fail("null domain");
+  if (!live_has(D)) return fail("null, unknown or destroyed domain");
   std::lock_guard<std::mutex> lk(g_tune_mu);
   const Grid& g = D->d.g;
   const TuneKey key{g.im, g.jm, g.km, D->d.cn1_separable, D->d.ncu};
@@ -1470,7 +1512,7 @@ int les_domain_autotune(les_domain* D, int reps, int* rows, int* tile_rows, int*
   D->d.sor_tile_width = r.tile_width;
   D->d.sor_kchunk = r.kchunk;
   if (rows) *rows = r.rows;
-  if (tile_rows) *tile_rows = r.rows == 2 ? r.tile_rows : 16;
+  if (tile_rows) *tile_rows = r.rows == 2 ? r.tile_rows : (r.rows == 4 ? 32 : 16);
   if (tile_width) *tile_width = r.rows == 2 ? r.tile_width : 64;
   if (kchunk) *kchunk = r.kchunk;
   if (us_best) *us_best = r.us_best;
@@ -1485,16 +1527,18 @@ int les_domain_time_sor(les_domain* D, int iters, double* us_per_iter) {
   HIP_TRY(hipStreamSynchronize(D->d.stream));
   Ctrl saved;
   HIP_TRY(copy_sync(&saved, D->d.ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, D->d.stream));
+  const bool pair = D->d.sor_rows == 4 && D->d.cn1_separable && (D->d.g.jm & 1) == 0;
   const int kc = D->d.sor_kchunk > 0 ? D->d.sor_kchunk
-                                     : sor_fused_kchunk(D->d.g, D->d.sg, D->d.sor_fused_blocks, D->d.ncu);
+                 : pair             ? sor_pair_kchunk(D->d.g, D->d.sg, D->d.ncu)
+                                    : sor_fused_kchunk(D->d.g, D->d.sg, D->d.sor_fused_blocks, D->d.ncu);
   hipEvent_t e0 = nullptr, e1 = nullptr;
   HIP_TRY(hipEventCreate(&e0));
   if (hipEventCreate(&e1) != hipSuccess) {
     hipEventDestroy(e0);
     return fail("hipEventCreate");
   }
-  int rc = time_sor_trial(D, saved, D->d.sor_rows, D->d.sor_tile_rows, D->d.sor_tile_width, kc, iters, e0, e1,
-                          us_per_iter);
+  int rc = time_sor_trial(D, saved, pair ? 4 : std::min(D->d.sor_rows, 2), D->d.sor_tile_rows,
+                          D->d.sor_tile_width, kc, iters + (pair ? iters % 2 : 0), e0, e1, us_per_iter);
   hipEventDestroy(e0);
   hipEventDestroy(e1);
   if (copy_sync(D->d.ctrl, &saved, sizeof(Ctrl), hipMemcpyHostToDevice, D->d.stream) != hipSuccess && !rc)
@@ -1510,7 +1554,7 @@ int les_domain_time_sor(les_domain* D, int iters, double* us_per_iter) {
 // events.  Steps with every n > 1 replay one graph.
 
 int les_domain_set_graph(les_domain* D, int on) {
-  if (!D) return fail("null domain");
+  if (!live_has(D)) return fail("null, unknown or destroyed domain");
   D->use_graph = on ? 1 : 0;
   return 0;
 }
@@ -1536,21 +1580,21 @@ int les_domain_download(les_domain* D, const char* field, float* host) {
 }
 
 int les_domain_synchronize(les_domain* D) {
-  if (!D) return fail("null domain");
+  if (!live_has(D)) return fail("null, unknown or destroyed domain");
   HIP_TRY(hipStreamSynchronize(D->d.stream));
   return 0;
 }
 
 int les_domain_press_stats(les_domain* D, int* iters, double* sor_hist, int hist_len, double* rhsav,
                            double* pav, int* nonfinite) {
-  if (!D) return fail("null domain");
+  if (!live_has(D)) return fail("null, unknown or destroyed domain");
   return read_stats(D, iters, sor_hist, hist_len, rhsav, pav, nonfinite);
 }
 
-long long les_domain_bytes(const les_domain* D) { return D ? D->bytes : 0; }
+long long les_domain_bytes(const les_domain* D) { return live_has(D) ? D->bytes : 0; }
 
 int les_domain_set_timing(les_domain* D, int on) {
-  if (!D) return fail("null domain");
+  if (!live_has(D)) return fail("null, unknown or destroyed domain");
   HIP_TRY(hipStreamSynchronize(D->d.stream));
   if (on && !D->d.timer) D->d.timer = new Timer();
   if (on) ((Timer*)D->d.timer)->mask = (unsigned int)on;
@@ -1559,7 +1603,7 @@ int les_domain_set_timing(les_domain* D, int on) {
 }
 
 int les_domain_timing(les_domain* D, double* ms, long long* count) {
-  if (!D) return fail("null domain");
+  if (!live_has(D)) return fail("null, unknown or destroyed domain");
   for (int c = 0; c < KC_N; c++) {
     ms[c] = 0.0;
     count[c] = 0;
@@ -1579,7 +1623,7 @@ int les_domain_timing(les_domain* D, double* ms, long long* count) {
 }
 
 int les_domain_set_timing_runs(les_domain* D, int on) {
-  if (!D) return fail("null domain");
+  if (!live_has(D)) return fail("null, unknown or destroyed domain");
   Timer* T = (Timer*)D->d.timer;
   if (!T) return fail("timing is off");
   HIP_TRY(hipStreamSynchronize(D->d.stream));
@@ -1822,7 +1866,7 @@ extern "C" {
 int les_batch_create(les_domain* const* doms, int n, les_batch** out) {
   if (!doms || n < 1 || !out) return fail("les_batch_create: need at least one domain");
   for (int q = 0; q < n; q++) {
-    if (!doms[q]) return fail("les_batch_create: null domain");
+    if (!live_has(doms[q])) return fail("les_batch_create: null, unknown or destroyed domain");
     for (int r = 0; r < q; r++)
       if (doms[r] == doms[q]) return fail("les_batch_create: a domain appears twice");
     if (doms[q]->d.timer) return fail("les_batch_create: a domain has timing on");
@@ -1844,23 +1888,28 @@ int les_batch_create(les_domain* const* doms, int n, les_batch** out) {
     batch_free(B);
     return fail(std::string("les_batch_create: ") + hipGetErrorString(e));
   }
+  live_add(B);
   *out = B;
   return 0;
 }
 
 int les_batch_destroy(les_batch* B) {
-  if (B) batch_free(B);
+  if (!B) return 0;
+  if (!live_remove(B)) return fail("les_batch_destroy: unknown or already destroyed batch");
+  batch_free(B);
   return 0;
 }
 
 int les_batch_set_graph(les_batch* B, int on) {
-  if (!B) return fail("null batch");
+  if (!live_has(B)) return fail("null, unknown or destroyed batch");
   B->use_graph = on ? 1 : 0;
   return 0;
 }
 
 int les_batch_step(les_batch* B, const int* ns, const int* nmaxs) {
-  if (!B || !ns || !nmaxs) return fail("null argument");
+  if (!live_has(B) || !ns || !nmaxs) return fail("les_batch_step: null argument, or unknown or destroyed batch");
+  for (les_domain* D : B->doms)  // a domain must outlive the batches holding it
+    if (!live_has(D)) return fail("les_batch_step: a domain of the batch was destroyed");
   if (!batch_key_same(B)) {
     if (B->gexec) {
       HIP_TRY(hipStreamSynchronize(B->stream));
@@ -1907,7 +1956,7 @@ int les_batch_step(les_batch* B, const int* ns, const int* nmaxs) {
 }
 
 int les_batch_query(les_batch* B) {
-  if (!B) return fail("null batch");
+  if (!live_has(B)) return fail("null, unknown or destroyed batch");
   const hipError_t e = hipStreamQuery(B->stream);
   if (e == hipSuccess) return 1;
   if (e == hipErrorNotReady) return 0;
@@ -1916,7 +1965,7 @@ int les_batch_query(les_batch* B) {
 }
 
 int les_batch_synchronize(les_batch* B) {
-  if (!B) return fail("null batch");
+  if (!live_has(B)) return fail("null, unknown or destroyed batch");
   HIP_TRY(hipStreamSynchronize(B->stream));
   return 0;
 }

@@ -39,6 +39,23 @@ typedef float real;
 #endif
 #define C(x) ((real)(x))
 
+/* The timing build of the restatement (liblesoracle_omp.so, -fopenmp)
+ * splits the k loops of the stencil stages, the rhs and mean loops of press
+ * and the SOR colour sweeps over threads; the serial build (liblesoracle.so,
+ * the checker) compiles without OpenMP and ignores them.  The cells of one
+ * such loop are independent -- each is written from arrays the loop does not
+ * write, and a colour sweep reads only the other colour -- so the fields are
+ * those of the serial loops; the FP64 sums of press (redmode 1) are reduced
+ * per thread, in another order. */
+#define PRAGMA_(x) _Pragma(#x)
+#ifdef _OPENMP
+#define PAR_K PRAGMA_(omp parallel for schedule(static))
+#define PAR_K_SUM(...) PRAGMA_(omp parallel for schedule(static) reduction(+ : __VA_ARGS__))
+#else
+#define PAR_K
+#define PAR_K_SUM(...)
+#endif
+
 /* ---- reference array layouts (main.f95:63-189 and the callee dummies) ---- */
 typedef struct {
   long li, lj, lk, ni, nj;
@@ -67,18 +84,21 @@ void ORC(velnw)(int km, int jm, int im, const real* p, real ro, const real* dxs,
                 const real* h) {
   lay LP = layP(im, jm), LU = layU(im, jm), LW = layW(im, jm), LF = layF(im, jm);
   const real* DZS = dzs + 1; /* dzs(-1:kp+2) */
+  PAR_K
   for (int k = 1; k <= km; k++) /* velnw.f95:27-35 */
     for (int j = 1; j <= jm; j++)
       for (int i = 1; i <= im; i++) {
         real pz = (-AT(p, LP, i, j, k) + AT(p, LP, i + 1, j, k)) / ro / dxs[i];
         AT(u, LU, i, j, k) = AT(u, LU, i, j, k) + dt * (AT(f, LF, i, j, k) - pz);
       }
+  PAR_K
   for (int k = 1; k <= km; k++) /* velnw.f95:38-48 */
     for (int j = 1; j <= jm; j++)
       for (int i = 1; i <= im; i++) {
         real pz = (-AT(p, LP, i, j, k) + AT(p, LP, i, j + 1, k)) / ro / dys[j];
         AT(v, LU, i, j, k) = AT(v, LU, i, j, k) + dt * (AT(g, LF, i, j, k) - pz);
       }
+  PAR_K
   for (int k = 1; k <= km - 1; k++) /* velnw.f95:51-59 */
     for (int j = 1; j <= jm; j++)
       for (int i = 1; i <= im; i++) {
@@ -239,6 +259,7 @@ static void vel2(int km, int jm, int im, real* nou1, const real* u, real* diu1, 
                (dy1[j] + dy1[j + 1]);
       S2(vspd, i, j) = SQRT(V_(i, j, 1) * V_(i, j, 1) + t * t);
     }
+  PAR_K
   for (int k = 1; k <= km; k++) /* vel2.f95:74-89 */
     for (int j = 1; j <= jm; j++)
       for (int i = 1; i <= im; i++) {
@@ -252,6 +273,7 @@ static void vel2(int km, int jm, int im, real* nou1, const real* u, real* diu1, 
         AT(cov5, LD1, i, j, k) = AT(nou5, LD1, i, j, k) * AT(diu5, LD1, i, j, k);
         AT(cov9, LD, i, j, k) = AT(nou9, LD, i, j, k) * AT(diu9, LD, i, j, k);
       }
+  PAR_K
   for (int k = 1; k <= km; k++) /* vel2.f95:103-111 */
     for (int j = 1; j <= jm; j++)
       for (int i = 1; i <= im; i++) {
@@ -260,6 +282,7 @@ static void vel2(int km, int jm, int im, real* nou1, const real* u, real* diu1, 
         AT(diu2, LD, i, j, k) = C(2.) * (-U_(i, j - 1, k) + U_(i, j, k)) / (dy1[j - 1] + dy1[j]);
         AT(cov2, LD, i, j, k) = AT(nou2, LD, i, j, k) * AT(diu2, LD, i, j, k);
       }
+  PAR_K
   for (int k = 2; k <= km + 1; k++) /* vel2.f95:119-127 */
     for (int j = 1; j <= jm; j++)
       for (int i = 1; i <= im; i++) {
@@ -276,6 +299,7 @@ static void vel2(int km, int jm, int im, real* nou1, const real* u, real* diu1, 
                               (C(0.5) * DZN[1]) / C(0.4) * U_(i, j, 1) / S2(uspd, i, j);
       AT(cov3, LD, i, j, 1) = AT(nou3, LD, i, j, 1) * AT(diu3, LD, i, j, 1);
     }
+  PAR_K
   for (int k = 1; k <= km; k++) /* vel2.f95:144-152 (u0 = 0) */
     for (int j = 1; j <= jm; j++)
       for (int i = 1; i <= im; i++) {
@@ -284,6 +308,7 @@ static void vel2(int km, int jm, int im, real* nou1, const real* u, real* diu1, 
         AT(diu4, LD, i, j, k) = C(2.) * (-V_(i - 1, j, k) + V_(i, j, k)) / (DX1[i - 1] + DX1[i]);
         AT(cov4, LD, i, j, k) = (AT(nou4, LD, i, j, k) - C(0)) * AT(diu4, LD, i, j, k);
       }
+  PAR_K
   for (int k = 2; k <= km + 1; k++) /* vel2.f95:154-162 */
     for (int j = 1; j <= jm; j++)
       for (int i = 1; i <= im; i++) {
@@ -300,6 +325,7 @@ static void vel2(int km, int jm, int im, real* nou1, const real* u, real* diu1, 
                               (C(0.5) * DZN[1]) / C(0.4) * V_(i, j, 1) / S2(vspd, i, j);
       AT(cov6, LD, i, j, 1) = AT(nou6, LD, i, j, 1) * AT(diu6, LD, i, j, 1);
     }
+  PAR_K
   for (int k = 1; k <= km - 1; k++) /* vel2.f95:179-187 */
     for (int j = 1; j <= jm; j++)
       for (int i = 1; i <= im; i++) {
@@ -308,6 +334,7 @@ static void vel2(int km, int jm, int im, real* nou1, const real* u, real* diu1, 
         AT(diu7, LD, i, j, k) = C(2.) * (-W_(i - 1, j, k) + W_(i, j, k)) / (DX1[i - 1] + DX1[i]);
         AT(cov7, LD, i, j, k) = (AT(nou7, LD, i, j, k) - C(0)) * AT(diu7, LD, i, j, k);
       }
+  PAR_K
   for (int k = 1; k <= km - 1; k++) /* vel2.f95:189-197 */
     for (int j = 1; j <= jm; j++)
       for (int i = 1; i <= im; i++) {
@@ -316,12 +343,14 @@ static void vel2(int km, int jm, int im, real* nou1, const real* u, real* diu1, 
         AT(diu8, LD, i, j, k) = C(2.) * (-W_(i, j - 1, k) + W_(i, j, k)) / (dy1[j - 1] + dy1[j]);
         AT(cov8, LD, i, j, k) = AT(nou8, LD, i, j, k) * AT(diu8, LD, i, j, k);
       }
+  PAR_K
   for (int k = 1; k <= km; k++) /* vel2.f95:202-208 */
     for (int j = 1; j <= jm; j++) {
       AT(nou1, LD1, im + 1, j, k) = AT(nou1, LD1, im, j, k);
       AT(diu1, LD1, im + 1, j, k) = AT(diu1, LD1, im, j, k);
       AT(cov1, LD1, im + 1, j, k) = AT(cov1, LD1, im, j, k);
     }
+  PAR_K
   for (int k = 1; k <= km; k++) /* vel2.f95:213-222 */
     for (int i = 1; i <= im; i++) {
       AT(nou2, LD, i, 0, k) = AT(nou2, LD, i, jm, k);
@@ -331,12 +360,14 @@ static void vel2(int km, int jm, int im, real* nou1, const real* u, real* diu1, 
       AT(diu2, LD, i, jm + 1, k) = AT(diu2, LD, i, 1, k);
       AT(cov2, LD, i, jm + 1, k) = AT(cov2, LD, i, 1, k);
     }
+  PAR_K
   for (int k = 1; k <= km; k++) /* vel2.f95:234-240 */
     for (int j = 1; j <= jm; j++) {
       AT(nou4, LD, im + 1, j, k) = AT(nou4, LD, im, j, k);
       AT(diu4, LD, im + 1, j, k) = AT(diu4, LD, im, j, k);
       AT(cov4, LD, im + 1, j, k) = AT(cov4, LD, im, j, k);
     }
+  PAR_K
   for (int k = 1; k <= km; k++) /* vel2.f95:245-254 */
     for (int i = 1; i <= im; i++) {
       AT(nou5, LD1, i, 0, k) = AT(nou5, LD1, i, jm, k);
@@ -346,12 +377,14 @@ static void vel2(int km, int jm, int im, real* nou1, const real* u, real* diu1, 
       AT(diu5, LD1, i, jm + 1, k) = AT(diu5, LD1, i, 1, k);
       AT(cov5, LD1, i, jm + 1, k) = AT(cov5, LD1, i, 1, k);
     }
+  PAR_K
   for (int k = 1; k <= km - 1; k++) /* vel2.f95:266-272 */
     for (int j = 1; j <= jm; j++) {
       AT(nou7, LD, im + 1, j, k) = AT(nou7, LD, im, j, k);
       AT(diu7, LD, im + 1, j, k) = AT(diu7, LD, im, j, k);
       AT(cov7, LD, im + 1, j, k) = AT(cov7, LD, im, j, k);
     }
+  PAR_K
   for (int k = 1; k <= km - 1; k++) /* vel2.f95:277-286 */
     for (int i = 1; i <= im; i++) {
       AT(nou8, LD, i, 0, k) = AT(nou8, LD, i, jm, k);
@@ -361,11 +394,13 @@ static void vel2(int km, int jm, int im, real* nou1, const real* u, real* diu1, 
       AT(diu8, LD, i, jm + 1, k) = AT(diu8, LD, i, 1, k);
       AT(cov8, LD, i, jm + 1, k) = AT(cov8, LD, i, 1, k);
     }
+  PAR_K
   for (int k = 1; k <= km + 1; k++) /* vel2.f95:299-304 */
     for (int j = 1; j <= jm + 1; j++) {
       AT(diu2, LD, im + 1, j, k) = AT(diu2, LD, im, j, k);
       AT(diu3, LD, im + 1, j, k) = AT(diu3, LD, im, j, k);
     }
+  PAR_K
   for (int k = 1; k <= km + 1; k++) /* vel2.f95:309-314 */
     for (int i = 1; i <= im + 1; i++) {
       AT(diu4, LD, i, 0, k) = AT(diu4, LD, i, jm, k);
@@ -398,6 +433,7 @@ void ORC(velfg)(int km, int jm, int im, const real* dx1, real* cov1, real* cov2,
   vel2(km, jm, im, nou1, u, diu1, dx1, nou5, v, diu5, dy1, nou9, w, diu9, dzn, cov1, cov5, cov9,
        nou2, diu2, cov2, nou3, diu3, dzs, cov3, nou4, diu4, cov4, nou6, diu6, cov6, nou7, diu7,
        cov7, nou8, diu8, cov8, uspd, vspd); /* velFG.f95:61-62 */
+  PAR_K
   for (int k = 1; k <= km; k++) /* velFG.f95:64-77 */
     for (int j = 1; j <= jm; j++)
       for (int i = 1; i <= im; i++) {
@@ -413,6 +449,7 @@ void ORC(velfg)(int km, int jm, int im, const real* dx1, real* cov1, real* cov2,
         real df = vn * d;
         AT(f, LF, i, j, k) = (-covc + df);
       }
+  PAR_K
   for (int k = 1; k <= km; k++) /* velFG.f95:80-93 */
     for (int j = 1; j <= jm; j++)
       for (int i = 1; i <= im; i++) {
@@ -428,6 +465,7 @@ void ORC(velfg)(int km, int jm, int im, const real* dx1, real* cov1, real* cov2,
         real df = vn * d;
         AT(g, LF, i, j, k) = (-covc + df);
       }
+  PAR_K
   for (int k = 1; k <= km - 1; k++) /* velFG.f95:97-110 */
     for (int j = 1; j <= jm; j++)
       for (int i = 1; i <= im; i++) {
@@ -452,6 +490,7 @@ void ORC(feedbf)(int km, int jm, int im, real* usum, const real* u, const real* 
                  real alpha, real dt, real beta, real* fx, real* fy, real* fz, real* f, real* g,
                  real* h) {
   lay LF = layF(im, jm), LU = layU(im, jm), LW = layW(im, jm), LB = layB(im, jm), LR = layR(im, jm);
+  PAR_K
   for (int k = 1; k <= km; k++) /* feedbf.f95:35-50 */
     for (int j = 1; j <= jm; j++)
       for (int i = 1; i <= im; i++) {
@@ -468,6 +507,7 @@ void ORC(feedbf)(int km, int jm, int im, real* usum, const real* u, const real* 
         AT(fy, LF, i, j, k) = f1y + f2y;
         AT(fz, LF, i, j, k) = f1z + f2z;
       }
+  PAR_K
   for (int k = 1; k <= km; k++) /* feedbf.f95:53-61 */
     for (int j = 1; j <= jm; j++)
       for (int i = 1; i <= im; i++) {
@@ -513,8 +553,10 @@ void ORC(les)(int km, real* delx1, const real* dx1, const real* dy1, const real*
 #define SM(i, j, k) AT(sm, LS, i, j, k)
 #define D1(a, i, j, k) AT(a, LD1, i, j, k)
 #define DD(a, i, j, k) AT(a, LD, i, j, k)
+  PAR_K
   for (int k = 1; k <= km; k++) /* les.f95:38-42 */
     DELX1[k] = POW(DX1[0] * dy1[0] * DZN[k], C(1.) / C(3.));
+  PAR_K
   for (int k = 1; k <= km; k++) /* les.f95:45-65 */
     for (int j = 1; j <= jm; j++)
       for (int i = 1; i <= im; i++) {
@@ -542,6 +584,7 @@ void ORC(les)(int km, real* delx1, const real* dx1, const real* dy1, const real*
       }
   ORC(boundsm)(km, jm, sm, im); /* les.f95:74 */
   const real wl = LOG(C(0.5) * DZN[1] / C(0.1));
+  PAR_K
   for (int k = 2; k <= km; k++) /* les.f95:83-114 */
     for (int j = 1; j <= jm; j++)
       for (int i = 1; i <= im; i++) {
@@ -592,6 +635,7 @@ void ORC(les)(int km, real* delx1, const real* dx1, const real* dy1, const real*
                  (visuy2 - visuy1) / dy1[j] + (visuz2 - visuz1) / DZN[1];
       AT(f, LF, i, j, 1) = (AT(f, LF, i, j, 1) + vfu);
     }
+  PAR_K
   for (int k = 2; k <= km; k++) /* les.f95:143-174 */
     for (int j = 1; j <= jm; j++)
       for (int i = 1; i <= im; i++) {
@@ -641,6 +685,7 @@ void ORC(les)(int km, real* delx1, const real* dx1, const real* dy1, const real*
       real vfv = (visvx2 - visvx1) / DX1[i] + (visvy2 - visvy1) / dys[j] + (visvz2 - visvz1) / DZN[1];
       AT(g, LF, i, j, 1) = (AT(g, LF, i, j, 1) + vfv);
     }
+  PAR_K
   for (int k = 1; k <= km; k++) /* les.f95:206-242 */
     for (int j = 1; j <= jm; j++)
       for (int i = 1; i <= im; i++) {
@@ -680,6 +725,7 @@ void ORC(adam)(int n, int nmax, real* fold, int im, int jm, int km, real* gold, 
   (void)n;
   (void)nmax;
   (void)fghold;
+  PAR_K
   for (int k = 1; k <= km; k++) /* adam.f95:30-44 */
     for (int j = 1; j <= jm; j++)
       for (int i = 1; i <= im; i++) {
@@ -787,7 +833,8 @@ void ORC(press)(int km, int jm, int im, real* rhs, const real* u, const real* dx
   (void)wsum;
 
   ORC(bondfg)(km, jm, f, im, g, h); /* press.f95:47 */
-  for (int k = 1; k <= km; k++)     /* press.f95:49-60 */
+  PAR_K
+  for (int k = 1; k <= km; k++) /* press.f95:49-60 */
     for (int j = 1; j <= jm; j++)
       for (int i = 1; i <= im; i++) {
         real r = (-AT(u, LU, i - 1, j, k) + AT(u, LU, i, j, k)) / DX1[i] +
@@ -800,6 +847,7 @@ void ORC(press)(int km, int jm, int im, real* rhs, const real* u, const real* dx
   /* press.f95:63-87: volume-weighted mean of rhs, subtracted */
   if (dmode) {
     double rs = 0.0, ar = 0.0;
+    PAR_K_SUM(rs, ar)
     for (int k = 1; k <= km; k++)
       for (int j = 1; j <= jm; j++)
         for (int i = 1; i <= im; i++) {
@@ -809,6 +857,7 @@ void ORC(press)(int km, int jm, int im, real* rhs, const real* u, const real* dx
     rs = rs / ar;
     g_rhsav = rs;
     g_area = ar;
+    PAR_K
     for (int k = 1; k <= km; k++)
       for (int j = 1; j <= jm; j++)
         for (int i = 1; i <= im; i++) AT(rhs, LR, i, j, k) = (real)((double)AT(rhs, LR, i, j, k) - rs);
@@ -834,6 +883,7 @@ void ORC(press)(int km, int jm, int im, real* rhs, const real* u, const real* dx
     double sord = 0.0;
     real sorf = C(0.0);
     for (int nrd = 0; nrd <= 1; nrd++) {
+      PAR_K_SUM(sord, sorf)
       for (int k = 1; k <= km; k++)
         for (int j = 1; j <= jm; j++)
           for (int i = 1 + (k + j + nrd) % 2; i <= im; i += 2) {
@@ -860,6 +910,7 @@ void ORC(press)(int km, int jm, int im, real* rhs, const real* u, const real* dx
   /* press.f95:124-150: volume-weighted mean of p, subtracted */
   if (dmode) {
     double pa = 0.0, pc = 0.0;
+    PAR_K_SUM(pa, pc)
     for (int k = 1; k <= km; k++)
       for (int j = 1; j <= jm; j++)
         for (int i = 1; i <= im; i++) {
@@ -869,6 +920,7 @@ void ORC(press)(int km, int jm, int im, real* rhs, const real* u, const real* dx
     pa = pa / pc;
     g_pav = pa;
     g_pco = pc;
+    PAR_K
     for (int k = 1; k <= km; k++)
       for (int j = 1; j <= jm; j++)
         for (int i = 1; i <= im; i++) AT(p, LP, i, j, k) = (real)((double)AT(p, LP, i, j, k) - pa);

@@ -71,8 +71,9 @@ def _argtypes(names, real_ct):
     return out
 
 
-def oracle_lib_path() -> Path:
-    return HERE / "liblesoracle.so"
+def oracle_lib_path(omp: bool = False) -> Path:
+    """The restatement; omp: its OpenMP build (CPU timing baseline only)."""
+    return HERE / ("liblesoracle_omp.so" if omp else "liblesoracle.so")
 
 
 def ref_lib_path(variant: str) -> Path:
@@ -143,10 +144,11 @@ class _Domain:
 class OracleBackend:
     """The C restatement.  prec=32 or 64; redmode 0 = sequential FP32
     reductions (the shipped build), 1 = FP64 accumulation (the GPU path's
-    arithmetic)."""
+    arithmetic).  omp: the OpenMP build (k loops over threads; only for the
+    CPU timing baseline, with redmode 1)."""
 
-    def __init__(self, setup, prec=32, redmode=1):
-        self.lib = ctypes.CDLL(str(oracle_lib_path()))
+    def __init__(self, setup, prec=32, redmode=1, omp=False):
+        self.lib = ctypes.CDLL(str(oracle_lib_path(omp)))
         self.prefix = "orc32_" if prec == 32 else "orc64_"
         real_ct = ctypes.c_float if prec == 32 else ctypes.c_double
         self.dom = _Domain(setup, np.float32 if prec == 32 else np.float64)

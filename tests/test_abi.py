@@ -66,3 +66,21 @@ def test_library_reports_missing_device_cleanly(lib):
 def test_bench_refuses_more_than_one_gpu():
     r = subprocess.run([sys.executable, str(ROOT / "bench.py"), "--gpus", "2"], capture_output=True, text=True)
     assert r.returncode == 2 and "one GPU" in r.stderr
+
+
+def test_calls_on_unknown_or_destroyed_handles_are_refused(lib):
+    """Every domain and batch entry point checks its handle against the live
+    ones, so a call on a destroyed (or never created) domain returns an
+    error instead of touching freed memory; no GPU is needed to reach it."""
+    lib.les_last_error.restype = ctypes.c_char_p
+    bogus = ctypes.c_void_p(0x1234560)
+    for name, args in (("les_domain_step", (bogus, 2, 10)), ("les_domain_destroy", (bogus,)),
+                       ("les_domain_synchronize", (bogus,)), ("les_batch_step", (bogus, None, None)),
+                       ("les_batch_destroy", (bogus,)), ("les_batch_query", (bogus,))):
+        f = getattr(lib, name)
+        f.restype = ctypes.c_int
+        rc = f(*args)
+        assert rc != 0, name
+        assert b"destroyed" in lib.les_last_error() or b"unknown" in lib.les_last_error(), name
+    lib.les_domain_destroy.restype = ctypes.c_int
+    assert lib.les_domain_destroy(None) == 0  # NULL is a no-op, as free(NULL)

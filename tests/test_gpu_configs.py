@@ -79,9 +79,13 @@ def test_plain_300_against_reference_norms():
         dom.step(n, 10 ** 9)
         st = dom.press_stats()
         assert st["iters"] == fx["iters"][n - 1], n
-        # the reference sums 4e6 squares per colour in FP32 (biased low);
-        # the GPU sums them in FP64
-        np.testing.assert_allclose(st["sor"], fx["sor"][n - 1][: st["iters"]], rtol=5e-3)
+        # The reference adds 4e6 squares per colour one by one into an FP32
+        # sum; once the sum is large, each addition drops low bits of the
+        # small terms, so its total is biased low, here by up to 0.6 % (0.4 %
+        # at 150x150x90, half the terms).  The GPU sums in FP64 (its history
+        # matches the FP64-accumulating restatement to 1e-9 in
+        # tests/test_gpu_parity.py); here only agreement to 1e-2.
+        np.testing.assert_allclose(st["sor"], fx["sor"][n - 1][: st["iters"]], rtol=1e-2)
         for k in FIELDS:
             a = dom.download(k).astype(np.float64)
             got = np.array([a.sum(), np.sqrt((a * a).sum()), np.abs(a).max()])
@@ -201,6 +205,10 @@ def test_serving_size_classes_against_restatement():
         def nonfinite(self):
             return self.d.nonfinite()
 
+        @property
+        def handle(self):  # the batch steps the library domain itself
+            return self.d.handle
+
         def close(self):
             finals[self.req.rid] = ({k: self.d.download(k) for k in FIELDS}, self.d.press_stats()["iters"])
             self.d.close()
@@ -256,6 +264,10 @@ def test_diverged_request_fails_alone():
 
         def nonfinite(self):
             return self.d.nonfinite()
+
+        @property
+        def handle(self):
+            return self.d.handle
 
         def close(self):
             finals[self.req.rid] = {k: self.d.download(k) for k in FIELDS}

@@ -22,18 +22,24 @@ class FakeDomain:
 
     def __init__(self, req, log):
         self.req, self.log, self.steps, self.polls = req, log, [], 0
+        self.closed = False
         FakeDomain.live += 1
         FakeDomain.peak = max(FakeDomain.peak, FakeDomain.live)
 
     def step(self, n, nmax):
+        # the life cycle never steps, polls or closes a closed domain
+        assert not self.closed, f"step {n} issued on closed domain {self.req.rid}"
         assert nmax == self.req.nsteps
         self.steps.append(n)
 
     def done(self):
+        assert not self.closed
         self.polls += 1
         return self.polls >= 2
 
     def close(self):
+        assert not self.closed, f"domain {self.req.rid} closed twice"
+        self.closed = True
         FakeDomain.live -= 1
         self.log.append((self.req.rid, tuple(self.steps)))
 
@@ -113,7 +119,9 @@ def test_summary_percentiles():
 
 class FakeBatch:
     def __init__(self, doms, blog):
+        assert not any(d.closed for d in doms), "batch built from a closed domain"
         self.doms, self.blog, self.polls = doms, blog, 0
+        self.closed = False
         blog.append(("open", tuple(d.req.rid for d in doms)))
 
     def step(self, ns, nmaxs):
@@ -126,6 +134,9 @@ class FakeBatch:
         return self.polls >= 2
 
     def close(self):
+        # a batch is closed before any of its domains (they must outlive it)
+        assert not self.closed and not any(d.closed for d in self.doms), "batch outlived one of its domains"
+        self.closed = True
         self.blog.append(("close", tuple(d.req.rid for d in self.doms)))
 
 

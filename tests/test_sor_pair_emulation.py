@@ -1,0 +1,583 @@
+"""CPU emulation of the two-iterations-per-launch SOR (k_sor_pair in
+mpi-les_amd/csrc/k_press.hip): its block/tile/row/lane index arithmetic,
+four-cell ring, per-stage LDS planes, register rings, stage activity by
+level and by wave, store redirection and per-cell FP32 arithmetic,
+statement by statement, checked against plain red-black iterations with the
+ghost rules of press.f95:89-122 / boundp.f95:7-78, bit for bit: two
+iterations per launch, one (the odd last launch of a loop), iteration 1 on
+the entry ghosts of p, and the three-set rotation with the stop test and
+fix-up of a pair whose first iteration meets the threshold.  It also checks
+that every buffer and LDS index the kernel forms stays inside its
+allocation, that every interior cell is stored exactly once by its owner,
+and that the redirected stores land only in the row padding or the scratch
+plane.  The GPU tests check the kernel itself against the one-iteration
+kernel; this pins the index arithmetic on the CPU."""
+import numpy as np
+import pytest
+
+FX = 64
+f32 = np.float32
+TY, R = 32, 4
+FTY, FTXW = TY - 2 * R, FX - 2 * R
+NW = TY // 2
+PL = TY * FX
+
+
+def jwrap(j, jm):
+    return (j - 1) % jm + 1
+
+
+def xcd(b, nb):
+    x, l = b % 8, b // 8
+    base, rem = nb // 8, nb % 8
+    return x * base + (x if x < rem else rem) + l
+
+
+class Err(Exception):
+    pass
+
+
+class Problem:
+    """The split allocation (eight arrays of km planes between two scratch
+    planes, garbage everywhere), metrics and the reference iterations."""
+
+    def __init__(self, im, jm, km, seed=0, omega=1.0):
+        assert jm % 2 == 0
+        self.im, self.jm, self.km = im, jm, km
+        rng = np.random.default_rng(seed)
+        self.nt = (im + 1) // 2
+        self.cp = (self.nt + 1 + 31) // 32 * 32
+        self.cpl = self.cp * jm
+        self.nplanes = 8 * km + 2
+        self.mem = rng.uniform(-1, 1, self.nplanes * self.cpl).astype(f32)
+        self.omega = f32(omega)
+        n = lambda c: np.zeros(c, f32)  # noqa: E731
+        self.c2l, self.c2s, self.ax = n(im + 4), n(im + 4), n(im + 4)
+        self.c3l, self.c3s, self.by = n(jm + 4), n(jm + 4), n(jm + 4)
+        self.c4l, self.c4s, self.cz = n(km + 4), n(km + 4), n(km + 4)
+        for a in (self.c2l, self.c2s, self.c3l, self.c3s, self.c4l, self.c4s):
+            a[:] = rng.uniform(0.05, 0.3, a.size).astype(f32)
+        for a in (self.ax, self.by, self.cz):
+            a[:] = rng.uniform(0.1, 0.5, a.size).astype(f32)
+        # entry values of the ghosts of p (reference layout, indices 0..n+1)
+        self.pent = rng.uniform(-1, 1, (im + 2, jm + 2, km + 2)).astype(f32)
+
+    # arrays: 0 p[0][0] (set 0 black), 1 p[0][1] (set 0 red), 2 p[1][0], 3 p[1][1], 4 rhs black,
+    # 5 rhs red, 6 p[2][0] (the cn1 storage), 7 p[2][1]
+    @staticmethod
+    def arr(set_, colour):
+        return {0: 0, 1: 2, 2: 6}[set_] + colour
+
+    def plane(self, a, L):
+        return 1 + a * self.km + (L - 1)
+
+    def get(self, a, t, j, L, mem=None):
+        mem = self.mem if mem is None else mem
+        return mem[self.plane(a, L) * self.cpl + (j - 1) * self.cp + t]
+
+    def cells(self):
+        for k in range(1, self.km + 1):
+            for j in range(1, self.jm + 1):
+                for i in range(1, self.im + 1):
+                    yield i, j, k
+
+    def read_set(self, set_, mem=None):
+        out = {}
+        for i, j, k in self.cells():
+            red = (i + j + k) & 1
+            out[(i, j, k)] = self.get(self.arr(set_, red), (i - 1) // 2, j, k, mem)
+        return out
+
+    def rhs(self):
+        out = {}
+        for i, j, k in self.cells():
+            red = (i + j + k) & 1
+            out[(i, j, k)] = self.get(5 if red else 4, (i - 1) // 2, j, k)
+        return out
+
+    def upd(self, i, j, k, own, pim, pip, pjm, pjp, pkm, pkp, rr):
+        M1 = lambda a, x: a[x + 1]  # noqa: E731
+        cn = f32(1.0) / f32(f32(M1(self.ax, i) + M1(self.by, j)) + M1(self.cz, k))
+        s = f32(M1(self.c2l, i) * pip)
+        s = f32(s + f32(M1(self.c2s, i) * pim))
+        s = f32(s + f32(M1(self.c3l, j) * pjp))
+        s = f32(s + f32(M1(self.c3s, j) * pjm))
+        s = f32(s + f32(M1(self.c4l, k) * pkp))
+        s = f32(s + f32(M1(self.c4s, k) * pkm))
+        s = f32(s - rr)
+        r = f32(self.omega * f32(f32(cn * s) - own))
+        return f32(own + r), r
+
+    def iteration(self, old, rhs, entry=False):
+        """One red-black iteration (press.f95:90-108): red from the previous
+        iterate, boundp1, black from the new red, boundp1, boundp2.  entry:
+        the ghosts read before the first boundp are the entry values."""
+        im, jm, km, pe = self.im, self.jm, self.km, self.pent
+        new = dict(old)
+        s = 0.0
+        for i, j, k in self.cells():
+            if not (i + j + k) & 1:
+                continue
+            own = old[(i, j, k)]
+            pim = (pe[0, j, k] if entry else own) if i == 1 else old[(i - 1, j, k)]
+            pip = (pe[im + 1, j, k] if entry else own) if i == im else old[(i + 1, j, k)]
+            pjm = pe[i, 0, k] if (entry and j == 1) else old[(i, jwrap(j - 1, jm), k)]
+            pjp = pe[i, jm + 1, k] if (entry and j == jm) else old[(i, jwrap(j + 1, jm), k)]
+            pkm = (pe[i, j, 0] if entry else own) if k == 1 else old[(i, j, k - 1)]
+            pkp = (pe[i, j, km + 1] if entry else own) if k == km else old[(i, j, k + 1)]
+            v, r = self.upd(i, j, k, own, pim, pip, pjm, pjp, pkm, pkp, rhs[(i, j, k)])
+            new[(i, j, k)] = v
+            s += float(r) * float(r)
+        out = dict(new)
+        for i, j, k in self.cells():
+            if (i + j + k) & 1:
+                continue
+            own = old[(i, j, k)]
+            pim = own if i == 1 else new[(i - 1, j, k)]
+            pip = own if i == im else new[(i + 1, j, k)]
+            pjm = new[(i, jwrap(j - 1, jm), k)]
+            pjp = new[(i, jwrap(j + 1, jm), k)]
+            pkm = (pe[i, j, 0] if entry else own) if k == 1 else new[(i, j, k - 1)]
+            pkp = (pe[i, j, km + 1] if entry else own) if k == km else new[(i, j, k + 1)]
+            v, r = self.upd(i, j, k, own, pim, pip, pjm, pjp, pkm, pkp, rhs[(i, j, k)])
+            out[(i, j, k)] = v
+            s += float(r) * float(r)
+        return out, s
+
+
+def run_launch(pb, mem, kc, iter_, two, in_set, out_set, ent, remap=1):
+    """Emulate one launch of k_sor_pair on the allocation `mem` (updated in
+    place).  Returns (written, sum1, sum2, nparts)."""
+    im, jm, km, nt, cp, cpl = pb.im, pb.jm, pb.km, pb.nt, pb.cp, pb.cpl
+    ntt, ntj = (nt + FTXW - 1) // FTXW, (jm + FTY - 1) // FTY
+    nz = (km + kc - 1) // kc
+    nparts = ntt * ntj * nz
+    cplb = 4 * cpl
+    nbytes = pb.nplanes * cplb
+    off0 = lambda a: (pb.plane(a, 1) - 1) * cplb  # noqa: E731
+    oRo, oBo = off0(pb.arr(in_set, 1)), off0(pb.arr(in_set, 0))
+    oRn, oBn = off0(pb.arr(out_set, 1)), off0(pb.arr(out_set, 0))
+    orR, orB = off0(5), off0(4)
+    src = mem.copy()  # what the launch reads (it never reads what it writes)
+    written = {}
+    sums = [0.0, 0.0]
+    M1 = lambda a, x: a[x + 1]  # noqa: E731
+
+    def load(voff, soff):
+        addr = voff + soff
+        if np.any(addr < 0) or np.any(addr + 4 > nbytes) or np.any(addr % 4):
+            raise Err(f"load out of range {addr.min()}..{addr.max()} of {nbytes}")
+        return src[addr // 4].copy()
+
+    def store(val, voff, soff, bid, real_mask):
+        addr = voff + soff
+        if np.any(addr < 0) or np.any(addr + 4 > nbytes):
+            raise Err("store out of range")
+        idx = (addr // 4).ravel()
+        for q, v, rm in zip(idx, val.ravel(), real_mask.ravel()):
+            written.setdefault(int(q), []).append((bid, float(v), bool(rm)))
+            mem[q] = v
+
+    def pent_at(i, j, k, mask):
+        # device layout covers -1..n+2: any index in 0..n+1 is inside
+        ii = np.clip(i, 0, im + 1)
+        if np.any(mask & ((i < 0) | (i > im + 1))):
+            raise Err("pent i index")
+        return pb.pent[ii, j, k]
+
+    lane = np.arange(FX)[None, :].repeat(NW, 0)
+    w = np.arange(NW)[:, None].repeat(FX, 1)
+    for bid in range(nparts):
+        lb = xcd(bid, nparts) if remap else bid
+        bt, bj, bk = lb % ntt, (lb // ntt) % ntj, lb // (ntt * ntj)
+        t = bt * FTXW - R + lane
+        tin = (t >= 0) & (t < nt)
+        rA = 2 * w
+        rB = rA + 1
+        jlA = 1 + bj * FTY - R + rA
+        jlB = jlA + 1
+        jA = jwrap(jlA, jm)
+        jB = jwrap(jlB, jm)
+        assert np.all(jA % 2 == 1)
+        k0 = 1 + bk * kc
+        k1 = min(km, k0 + kc - 1)
+        ra = 3 if two else 1
+        aLo, aHi = max(1, k0 - ra), min(km, k1 + ra)
+        bLo, bHi = max(1, k0 - ra + 1), min(km, k1 + ra - 1)
+        cLo, cHi = max(1, k0 - 1), min(km, k1 + 1)
+        krs, kend = aLo, k1 + ra
+        tq = np.where(tin, t, 0)
+        voA = 4 * (tq + (jA - 1) * cp)
+        voB = 4 * (tq + (jB - 1) * cp)
+        trA = 4 * (nt + (jA - 1) * cp)
+        trB = 4 * (nt + (jB - 1) * cp)
+        lane_tile = tin & (lane >= R) & (lane <= FX - 1 - R)
+        tileA = (rA >= R) & (rA <= TY - 1 - R) & (jlA <= jm)
+        tileB = (rB >= R) & (rB <= TY - 1 - R) & (jlB <= jm)
+        wB = (2 * w[:, 0] + 1 >= 2) & (2 * w[:, 0] <= TY - 3)
+        wC = two & (2 * w[:, 0] + 1 >= 3) & (2 * w[:, 0] <= TY - 4)
+        wD = two & (2 * w[:, 0] + 1 >= 4) & (2 * w[:, 0] <= TY - 5)
+        cl = lambda x: np.clip(x, -1, im + 2) + 1  # noqa: E731
+        c2lp = [np.where(tin, pb.c2l[cl(2 * t + 1 + p)], 0).astype(f32) for p in (0, 1)]
+        c2sp = [np.where(tin, pb.c2s[cl(2 * t + 1 + p)], 0).astype(f32) for p in (0, 1)]
+        axp = [np.where(tin, pb.ax[cl(2 * t + 1 + p)], 0).astype(f32) for p in (0, 1)]
+        c3lA, c3sA = pb.c3l[jA + 1], pb.c3s[jA + 1]
+        c3lB, c3sB = pb.c3l[jB + 1], pb.c3s[jB + 1]
+        byA, byB = pb.by[jA + 1], pb.by[jB + 1]
+        sA = [(axp[p] + byA).astype(f32) for p in (0, 1)]
+        sB = [(axp[p] + byB).astype(f32) for p in (0, 1)]
+        okA = [lane_tile & tileA & (2 * t + 1 + p <= im) for p in (0, 1)]
+        okB = [lane_tile & tileB & (2 * t + 1 + p <= im) for p in (0, 1)]
+        svA = [np.where(okA[p], voA, trA) for p in (0, 1)]
+        svB = [np.where(okB[p], voB, trB) for p in (0, 1)]
+        last = [(2 * t + 1 + p == im) for p in (0, 1)]
+        first = (t == 0)
+        ownA = rA * FX + lane
+        bsA = np.clip(ownA - 1, 0, PL - 3)
+        bsB = np.clip(ownA + FX - 1, 0, PL - 3)
+        upA = np.where(rA == 0, rA, rA - 1) * FX + lane
+        dnB = np.where(rB == TY - 1, rB, rB + 1) * FX + lane
+        nsteps = kend - krs + 1
+        if nsteps > kc + 7:
+            raise Err("lvm4 larger than its allocation")
+        lvm = np.zeros((nsteps, 3), f32)
+        for x in range(nsteps):
+            k = krs + x
+            if k <= km:
+                lvm[x] = (M1(pb.c4l, k), M1(pb.c4s, k), M1(pb.cz, k))
+
+        def fill(L):
+            lr = min(L, aHi) * cplb
+            lk = min(max(L - 1, 1), bHi) * cplb
+            return dict(roA=load(voA, oRo + lr), roB=load(voB, oRo + lr), rrA=load(voA, orR + lr),
+                        rrB=load(voB, orR + lr), rbA=load(voA, orB + lk), rbB=load(voB, orB + lk))
+
+        lbq = min(aHi + 1, km + 1)
+
+        def loadbq(L):
+            Lc = min(L, lbq)
+            o = oRo + km * cplb if Lc == km + 1 else oBo + Lc * cplb
+            return load(voA, o), load(voB, o)
+
+        sl = [fill(krs), fill(krs + 1)]
+        bqA, bqB = [None] * 6, [None] * 6
+        for q in range(5):
+            bqA[q], bqB[q] = loadbq(krs - 1 + q)
+        z = np.zeros(voA.shape, f32)
+        rdA, rdB, blA, blB, r2A, r2B = ([z] * 3 for _ in range(6))
+        rdA, rdB, blA, blB, r2A, r2B = (list(x) for x in (rdA, rdB, blA, blB, r2A, r2B))
+        LB = np.full(3 * PL, np.nan, f32)
+        LR = np.full(2 * PL, np.nan, f32)
+        LB2 = np.full(2 * PL, np.nan, f32)
+        LR2 = np.full(2 * PL, np.nan, f32)
+        LB[ownA] = bqA[1]
+        LB[ownA + FX] = bqB[1]
+        if krs == 1:
+            bqA[0], bqB[0] = sl[0]["roA"], sl[0]["roB"]
+            rdA[2], rdB[2] = bqA[1], bqB[1]
+        acc1 = np.zeros(voA.shape)
+        acc2 = np.zeros(voA.shape)
+        D0 = (1 + krs) & 1
+
+        def lds(arr, idx, size):
+            if np.any(idx < 0) or np.any(idx >= size):
+                raise Err("LDS out of range")
+            return arr[idx]
+
+        def cellupd(own, pim, pip, pjm, pjp, pkm, pkp, p, c3l_, c3s_, s_, mt, rr):
+            cn = (f32(1) / (s_ + mt[2]).astype(f32)).astype(f32)
+            s = (c2lp[p] * pip).astype(f32)
+            s = (s + (c2sp[p] * pim).astype(f32)).astype(f32)
+            s = (s + (c3l_ * pjp).astype(f32)).astype(f32)
+            s = (s + (c3s_ * pjm).astype(f32)).astype(f32)
+            s = (s + (mt[0] * pkp).astype(f32)).astype(f32)
+            s = (s + (mt[1] * pkm).astype(f32)).astype(f32)
+            s = (s - rr).astype(f32)
+            r = (pb.omega * ((cn * s).astype(f32) - own).astype(f32)).astype(f32)
+            return (own + r).astype(f32), r
+
+        def inplane(plane, base, p, bs, first_, last_, own):
+            lo = lds(plane, base + bs + p, plane.size)
+            hi = lds(plane, base + bs + p + 1, plane.size)
+            pim = np.where(first_, own, lo) if p == 0 else lo
+            pip = np.where(last_, own, hi)
+            return pim, pip
+
+        def wrows(mask_w):  # (NW,) wave mask -> lanes
+            return mask_w[:, None].repeat(FX, 1)
+
+        for U_abs, kr in enumerate(range(krs, kend + 1)):
+            U = U_abs % 6
+            P = U & 1
+            pa = D0 ^ P
+            pbb = pa ^ 1
+            kb, kc_, kd = kr - 1, kr - 2, kr - 3
+            actA = kr <= aHi
+            actB = bLo <= kb <= bHi
+            actC = two and cLo <= kc_ <= cHi
+            actD = two and k0 <= kd <= k1
+            if two:
+                lc = min(max(kc_, 1), km) * cplb
+                ld = min(max(kd, 1), km) * cplb
+                hrA, hrB = load(voA, orR + lc), load(voB, orR + lc)
+                hbA, hbB = load(voA, orB + ld), load(voB, orB + ld)
+            LB[((U + 1) % 3) * PL + ownA] = bqA[(U + 2) % 6]
+            LB[((U + 1) % 3) * PL + ownA + FX] = bqB[(U + 2) % 6]
+            q = sl[U % 2]
+            redA = redB = z
+            if actA:
+                mt = lvm[kr - krs]
+                base = (U % 3) * PL
+                kin = k0 <= kr <= k1
+                # row A
+                own = q["roA"]
+                pim, pip = inplane(LB, base, pa, bsA, first, last[pa], own)
+                pjm, pjp = lds(LB, base + upA, 3 * PL), bqB[(U + 1) % 6]
+                pkm, pkp = bqA[U % 6], bqA[(U + 2) % 6]
+                if ent:
+                    i = 2 * t + 1 + pa
+                    m = tin
+                    if pa == 0:
+                        pim = np.where(m & first, pent_at(np.zeros_like(t), jA, kr, m & first), pim)
+                    pip = np.where(m & last[pa], pent_at(np.full_like(t, im + 1), jA, kr, m & last[pa]), pip)
+                    pjm = np.where(m & (jA == 1), pent_at(i, np.zeros_like(jA), kr, m & (jA == 1)), pjm)
+                    pjp = np.where(m & (jA == jm), pent_at(i, np.full_like(jA, jm + 1), kr, m & (jA == jm)), pjp)
+                    if kr == 1:
+                        pkm = np.where(m, pent_at(i, jA, 0, m), pkm)
+                    if kr == km:
+                        pkp = np.where(m, pent_at(i, jA, km + 1, m), pkp)
+                redA, r = cellupd(own, pim, pip, pjm, pjp, pkm, pkp, pa, c3lA, c3sA, sA[pa], mt, q["rrA"])
+                acc1 += np.where(okA[pa] & kin, r.astype(float) ** 2, 0)
+                if not two:
+                    store(redA, svA[pa], oRn + kr * cplb if kin else 0, bid, okA[pa] & kin)
+                # row B
+                own = q["roB"]
+                pim, pip = inplane(LB, base, pbb, bsB, first, last[pbb], own)
+                pjm, pjp = bqA[(U + 1) % 6], lds(LB, base + dnB, 3 * PL)
+                pkm, pkp = bqB[U % 6], bqB[(U + 2) % 6]
+                if ent:
+                    i = 2 * t + 1 + pbb
+                    m = tin
+                    if pbb == 0:
+                        pim = np.where(m & first, pent_at(np.zeros_like(t), jB, kr, m & first), pim)
+                    pip = np.where(m & last[pbb], pent_at(np.full_like(t, im + 1), jB, kr, m & last[pbb]), pip)
+                    pjm = np.where(m & (jB == 1), pent_at(i, np.zeros_like(jB), kr, m & (jB == 1)), pjm)
+                    pjp = np.where(m & (jB == jm), pent_at(i, np.full_like(jB, jm + 1), kr, m & (jB == jm)), pjp)
+                    if kr == 1:
+                        pkm = np.where(m, pent_at(i, jB, 0, m), pkm)
+                    if kr == km:
+                        pkp = np.where(m, pent_at(i, jB, km + 1, m), pkp)
+                redB, r = cellupd(own, pim, pip, pjm, pjp, pkm, pkp, pbb, c3lB, c3sB, sB[pbb], mt, q["rrB"])
+                acc1 += np.where(okB[pbb] & kin, r.astype(float) ** 2, 0)
+                if not two:
+                    store(redB, svB[pbb], oRn + kr * cplb if kin else 0, bid, okB[pbb] & kin)
+                LR[P * PL + ownA] = redA
+                LR[P * PL + ownA + FX] = redB
+                rdA[U % 3], rdB[U % 3] = redA, redB
+            wBl = wrows(wB)
+            if actB and wB.any():
+                mt = lvm[kb - krs]
+                base = (P ^ 1) * PL
+                kin = k0 <= kb <= k1
+                top = kb == km
+                res = []
+                for row in ("A", "B"):
+                    if row == "A":
+                        own, p, bs, c3l_, c3s_, s_ = bqA[U % 6], pa, bsA, c3lA, c3sA, sA[pa]
+                        pjm, pjp = lds(LR, base + upA, 2 * PL), rdB[(U + 2) % 3]
+                        pkm, pkp = rdA[(U + 1) % 3], (own if top else redA)
+                        jj, rr, ok, sv = jA, q["rbA"], okA[pa], svA[pa]
+                    else:
+                        own, p, bs, c3l_, c3s_, s_ = bqB[U % 6], pbb, bsB, c3lB, c3sB, sB[pbb]
+                        pjm, pjp = rdA[(U + 2) % 3], lds(LR, base + dnB, 2 * PL)
+                        pkm, pkp = rdB[(U + 1) % 3], (own if top else redB)
+                        jj, rr, ok, sv = jB, q["rbB"], okB[pbb], svB[pbb]
+                    pim, pip = inplane(LR, base, p, bs, first, last[p], own)
+                    if ent:
+                        i = 2 * t + 1 + p
+                        if kb == 1:
+                            pkm = np.where(tin, pent_at(i, jj, 0, tin), pkm)
+                        if kb == km:
+                            pkp = np.where(tin, pent_at(i, jj, km + 1, tin), pkp)
+                    v, r = cellupd(own, pim, pip, pjm, pjp, pkm, pkp, p, c3l_, c3s_, s_, mt, rr)
+                    v = np.where(wBl, v, np.nan).astype(f32)  # waves that skip B
+                    acc1 += np.where(ok & kin & wBl, r.astype(float) ** 2, 0)
+                    if not two:
+                        m = wBl
+                        store(v[m], sv[m], oBn + kb * cplb if kin else 0, bid, (ok & kin)[m])
+                    res.append(v)
+                bA, bB = res
+                LB2[P * PL + ownA] = np.where(wBl, bA, LB2[P * PL + ownA])
+                LB2[P * PL + ownA + FX] = np.where(wBl, bB, LB2[P * PL + ownA + FX])
+                blA[(U + 2) % 3] = np.where(wBl, bA, blA[(U + 2) % 3])
+                blB[(U + 2) % 3] = np.where(wBl, bB, blB[(U + 2) % 3])
+            r2a = r2b = z
+            wCl = wrows(wC)
+            if actC and wC.any():
+                mt = lvm[kc_ - krs]
+                base = (P ^ 1) * PL
+                kin = k0 <= kc_ <= k1
+                so = oRn + kc_ * cplb if kin else 0
+                res = []
+                for row in ("A", "B"):
+                    if row == "A":
+                        own, p, bs, c3l_, c3s_, s_ = rdA[(U + 1) % 3], pa, bsA, c3lA, c3sA, sA[pa]
+                        pjm, pjp = lds(LB2, base + upA, 2 * PL), blB[(U + 1) % 3]
+                        pkm = own if kc_ == 1 else blA[U % 3]
+                        pkp = own if kc_ == km else blA[(U + 2) % 3]
+                        rr, ok, sv = hrA, okA[pa], svA[pa]
+                    else:
+                        own, p, bs, c3l_, c3s_, s_ = rdB[(U + 1) % 3], pbb, bsB, c3lB, c3sB, sB[pbb]
+                        pjm, pjp = blA[(U + 1) % 3], lds(LB2, base + dnB, 2 * PL)
+                        pkm = own if kc_ == 1 else blB[U % 3]
+                        pkp = own if kc_ == km else blB[(U + 2) % 3]
+                        rr, ok, sv = hrB, okB[pbb], svB[pbb]
+                    pim, pip = inplane(LB2, base, p, bs, first, last[p], own)
+                    v, r = cellupd(own, pim, pip, pjm, pjp, pkm, pkp, p, c3l_, c3s_, s_, mt, rr)
+                    v = np.where(wCl, v, np.nan).astype(f32)
+                    acc2 += np.where(ok & kin & wCl, r.astype(float) ** 2, 0)
+                    m = wCl
+                    store(v[m], sv[m], so, bid, (ok & kin)[m])
+                    res.append(v)
+                r2a, r2b = res
+                LR2[P * PL + ownA] = np.where(wCl, r2a, LR2[P * PL + ownA])
+                LR2[P * PL + ownA + FX] = np.where(wCl, r2b, LR2[P * PL + ownA + FX])
+                r2A[(U + 1) % 3] = np.where(wCl, r2a, r2A[(U + 1) % 3])
+                r2B[(U + 1) % 3] = np.where(wCl, r2b, r2B[(U + 1) % 3])
+            wDl = wrows(wD)
+            if actD and wD.any():
+                mt = lvm[kd - krs]
+                base = (P ^ 1) * PL
+                so = oBn + kd * cplb
+                for row in ("A", "B"):
+                    if row == "A":
+                        own, p, bs, c3l_, c3s_, s_ = blA[U % 3], pa, bsA, c3lA, c3sA, sA[pa]
+                        pjm, pjp = lds(LR2, base + upA, 2 * PL), r2B[U % 3]
+                        pkm = own if kd == 1 else r2A[(U + 2) % 3]
+                        pkp = own if kd == km else r2a
+                        rr, ok, sv = hbA, okA[pa], svA[pa]
+                    else:
+                        own, p, bs, c3l_, c3s_, s_ = blB[U % 3], pbb, bsB, c3lB, c3sB, sB[pbb]
+                        pjm, pjp = r2A[U % 3], lds(LR2, base + dnB, 2 * PL)
+                        pkm = own if kd == 1 else r2B[(U + 2) % 3]
+                        pkp = own if kd == km else r2b
+                        rr, ok, sv = hbB, okB[pbb], svB[pbb]
+                    pim, pip = inplane(LR2, base, p, bs, first, last[p], own)
+                    v, r = cellupd(own, pim, pip, pjm, pjp, pkm, pkp, p, c3l_, c3s_, s_, mt, rr)
+                    acc2 += np.where(ok & wDl, r.astype(float) ** 2, 0)
+                    m = wDl
+                    store(v[m], sv[m], so, bid, ok[m])
+            sl[U % 2] = fill(kr + 2)
+            bqA[(U + 5) % 6], bqB[(U + 5) % 6] = loadbq(kr + 4)
+        sums[0] += acc1.sum()
+        sums[1] += acc2.sum()
+    return written, sums[0], sums[1], nparts
+
+
+def check_written(pb, written, out_set, expect):
+    """Every interior cell of out_set stored once by its owner with the
+    expected value; other stores only in padding or the scratch plane."""
+    bad = 0
+    for (i, j, k), v in expect.items():
+        red = (i + j + k) & 1
+        qidx = pb.plane(pb.arr(out_set, red), k) * pb.cpl + (j - 1) * pb.cp + (i - 1) // 2
+        ws = written.get(qidx)
+        if not ws:
+            bad += 1
+            if bad < 5:
+                print("unwritten", (i, j, k), "red" if red else "black")
+            continue
+        reals = [x for x in ws if x[2]]
+        if len(reals) != 1:
+            bad += 1
+            if bad < 5:
+                print("owned writes", len(reals), (i, j, k))
+        if f32(reals[-1][1] if reals else ws[-1][1]).tobytes() != f32(v).tobytes():
+            bad += 1
+            if bad < 5:
+                print("mismatch", (i, j, k), "red" if red else "black", [x[1] for x in ws], float(v))
+    for qidx, ws in written.items():
+        pl, rem = divmod(qidx, pb.cpl)
+        jrow, tt = divmod(rem, pb.cp)
+        if pl == 0 or pl == pb.nplanes - 1 or tt >= pb.nt:
+            if any(x[2] for x in ws):
+                bad += 1
+                print("owned write into scratch", qidx)
+            continue
+        if not any(x[2] for x in ws):
+            bad += 1
+            if bad < 10:
+                print("non-owned write into a real cell", pl, jrow + 1, tt)
+    return bad
+
+
+CASES = [(20, 14, 9, 9), (20, 14, 9, 3), (21, 14, 9, 4), (130, 26, 12, 5), (7, 2, 2, 1), (9, 4, 3, 2),
+         (125, 30, 20, 7), (64, 16, 13, 13), (3, 2, 5, 5), (121, 50, 8, 6), (70, 60, 11, 4), (113, 48, 14, 14)]
+
+
+@pytest.mark.parametrize("im,jm,km,kc", CASES)
+@pytest.mark.parametrize("mode", ["pair", "single", "entry-pair"])
+def test_sor_pair_emulation(im, jm, km, kc, mode):
+    pb = Problem(im, jm, km, seed=im * 7 + jm + km)
+    rhs = pb.rhs()
+    in_set, out_set = (0, 1) if mode == "entry-pair" else (1, 2)
+    old = pb.read_set(in_set)
+    it1, s1 = pb.iteration(old, rhs, entry=(mode == "entry-pair"))
+    mem = pb.mem.copy()
+    two = mode != "single"
+    written, e1, e2, _ = run_launch(pb, mem, kc, 1 if mode == "entry-pair" else 5, two, in_set, out_set,
+                                    ent=(mode == "entry-pair"))
+    if two:
+        it2, s2 = pb.iteration(it1, rhs)
+        assert check_written(pb, written, out_set, it2) == 0
+        assert abs(e2 - s2) <= 1e-12 * max(s2, 1e-30)
+    else:
+        assert check_written(pb, written, out_set, it1) == 0
+        assert e2 == 0.0
+    assert abs(e1 - s1) <= 1e-12 * max(s1, 1e-30)
+    # the set before in_set and in_set itself are untouched
+    for st in {0, 1, 2} - {out_set}:
+        for colour in (0, 1):
+            a = pb.arr(st, colour)
+            sl = slice(pb.plane(a, 1) * pb.cpl, (pb.plane(a, km) + 1) * pb.cpl)
+            assert np.array_equal(mem[sl], pb.mem[sl]), (st, colour)
+
+
+def test_rotation_stop_and_fixup():
+    """Launches of two over the three sets with the host's rotation, the stop
+    test one launch behind and the fix-up: when iteration L, the first of a
+    launch, meets the threshold, recomputing it alone from that launch's
+    in_set gives the L-th iterate, the same bits as L plain iterations."""
+    im, jm, km, kc = 23, 12, 7, 4
+    pb = Problem(im, jm, km, seed=3)
+    rhs = pb.rhs()
+    mem = pb.mem.copy()
+    # iterate set 0 holds p; iterations 1..5 in launches (1,2), (3,4), (5)
+    ref, its, sums = pb.read_set(0), [], []
+    for lnum in range(1, 6):
+        ref, s = pb.iteration(ref, rhs, entry=(lnum == 1))
+        its.append(ref)
+        sums.append(s)
+    launches = []
+    in_ = 0
+    for l0 in (1, 3, 5):
+        two = l0 + 1 <= 5
+        out = (in_ + 1) % 3
+        _, e1, e2, _ = run_launch(pb, mem, kc, l0, two, in_, out, ent=(l0 == 1))
+        assert abs(e1 - sums[l0 - 1]) <= 1e-12 * sums[l0 - 1]
+        if two:
+            assert abs(e2 - sums[l0]) <= 1e-12 * sums[l0]
+        launches.append((l0, two, in_, out))
+        in_ = out
+    # the result of each launch is in its out set
+    for (l0, two, _, out), expect in zip(launches, (its[1], its[3], its[4])):
+        if (l0, two, out) != launches[-1][:2] + (launches[-1][3],):
+            continue
+        assert pb.read_set(out, mem) == expect
+    # a stop at iteration 3 (the first of launch (3, 4)) is found by the
+    # launch of 5, which still runs; the fix-up recomputes 3 from that
+    # launch's in set, which no launch after it wrote
+    l0, two, in3, out3 = launches[1]
+    assert in3 not in (launches[2][3],)  # the launch of 5 wrote another set
+    fix_written, _, _, _ = run_launch(pb, mem, kc, 3, False, in3, out3, ent=False)
+    assert check_written(pb, fix_written, out3, its[2]) == 0
