@@ -60,24 +60,11 @@ STATE_FIELDS = ("u", "v", "w", "p", "f", "g", "h", "usum", "vsum", "wsum", "fold
                 "sm", "uspd", "vspd") + tuple(f"diu{q}" for q in range(1, 10))
 
 
-def cpu_baseline(setup, state, n0, nsteps, budget_s):
-    """Time the CPU checker on this host for up to nsteps steps (stopping
-    once budget_s is spent), from the GPU state `state` before step n0."""
-    sys.path.insert(0, str(ROOT))
-    from oracle import oracle
-    w = setup.workload
-    kind = "reference" if oracle.ref_available("large") and w.im <= 300 and w.jm <= 300 and w.km <= 105 else "port"
-    if kind == "reference":
-        ref_setup = dataclasses.replace(setup, inflow=None)
-        be = oracle.RefBackend(ref_setup, "large")
-        what = "reference Fortran subroutines (flang -O2, oracle/_ref/large) with its fixed 5 m/s inflow"
-    else:
-        be = oracle.OracleBackend(setup, 32, redmode=0)
-        what = "C restatement (gcc -O2, FP32)"
-    be.load(state)
+def _time_steps(be, n0, nsteps, budget_s):
+    """Wall time of each of up to nsteps steps of a CPU backend, stopping
+    once budget_s is spent; the reference's progress prints are kept off
+    stdout (the one-line JSON output)."""
     times = []
-    # the reference prints progress lines on stdout: keep them out of the
-    # one-line JSON output
     sys.stdout.flush()
     saved = os.dup(1)
     devnull = os.open(os.devnull, os.O_WRONLY)
@@ -96,8 +83,39 @@ def cpu_baseline(setup, state, n0, nsteps, budget_s):
         os.dup2(saved, 1)
         os.close(saved)
         os.close(devnull)
+    return times
+
+
+def host_threads() -> int:
+    """Threads for the multi-core baseline: OMP_NUM_THREADS when set (the
+    process's share of a shared host), else the cores this process may run on."""
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        return int(env)
+    return len(os.sched_getaffinity(0))
+
+
+def cpu_baseline(setup, state, n0, nsteps, budget_s):
+    """Time the CPU checker on this host for up to nsteps steps (stopping
+    once budget_s is spent), from the GPU state `state` before step n0: the
+    reference's own subroutines (or the C restatement) on one core, and the
+    C restatement's OpenMP build on every thread of the process's share."""
+    sys.path.insert(0, str(ROOT))
+    from oracle import oracle
+    w = setup.workload
+    kind = "reference" if oracle.ref_available("large") and w.im <= 300 and w.jm <= 300 and w.km <= 105 else "port"
+    if kind == "reference":
+        ref_setup = dataclasses.replace(setup, inflow=None)
+        be = oracle.RefBackend(ref_setup, "large")
+        what = "reference Fortran subroutines (flang -O2, oracle/_ref/large) with its fixed 5 m/s inflow"
+    else:
+        be = oracle.OracleBackend(setup, 32, redmode=0)
+        what = "C restatement (gcc -O2, FP32)"
+    be.load(state)
+    times = _time_steps(be, n0, nsteps, budget_s)
+    del be
     sps = len(times) / sum(times)
-    return {
+    out = {
         "value": round(sps, 4),
         "unit": "steps/s",
         "cores": 1,
@@ -107,6 +125,21 @@ def cpu_baseline(setup, state, n0, nsteps, budget_s):
                   f"{sum(times):.1f} s of CPU time",
         "mcell_updates_per_s": round(sps * w.cells / 1e6, 3),
     }
+    if oracle.oracle_lib_path(omp=True).exists():
+        nthr = host_threads()
+        os.environ["OMP_NUM_THREADS"] = str(nthr)  # read by the OpenMP runtime when the library loads
+        par = oracle.OracleBackend(setup, 32, redmode=1, omp=True)
+        par.load(state)
+        pt = _time_steps(par, n0, 4 * nsteps, budget_s)
+        psps = len(pt) / sum(pt)
+        out["all_cores"] = {
+            "value": round(psps, 4), "unit": "steps/s", "cores": nthr, "kind": "port",
+            "sample": f"{len(pt)} steps (n={n0}..{n0 + len(pt) - 1}) of the same domain and inflow as the GPU, "
+                      f"C restatement (gcc -O2, FP32 fields, FP64 sums) with its k loops on {nthr} OpenMP "
+                      f"threads; {sum(pt):.1f} s wall time",
+            "mcell_updates_per_s": round(psps * w.cells / 1e6, 3),
+        }
+    return out
 
 
 def profiled_traffic(kernel_class: str, workload: str, sor_rows: int = 4):
@@ -195,18 +228,24 @@ def poisson_sweep(rates: list[float], nreq: int, nsteps: int) -> int:
     return 0
 
 
-def precision_comparison(K: int, W: int) -> int:
+def precision_comparison(K: int, W: int, omegas, outer: int, inner: int) -> int:
     """The precision-512x512x128 workload (dx=dy=2 m, 35 % cover, u*=0.6,
     z0=0.3, dt=0.02; SOR at most 100 iterations, stop at 1e-6) stepped with
-    the reference's FP32 SOR and with the FP64 defect correction doing the
-    same number of SOR iterations (4 outer steps of 25): time per step from
-    graph replay, and the SOR statistics of the last step."""
+    the reference's FP32 SOR and with the FP64 defect correction (at most
+    `outer` steps of `inner` FP32 SOR iterations, stopping once
+    sum((cn1*d)^2) < 1e-6) for each inner relaxation in omegas: time per
+    step from graph replay, and per step of the timed run the measure the
+    stop test compares with 1e-6 and the sweeps taken."""
     w = setup_gen.workloads()["precision-512x512x128"]
-    out = {"workload": w.name, "grid": [w.im, w.jm, w.km], "steps": K, "warmup": W}
-    for mode, tag in ((0, "fp32_sor"), (1, "fp64_defect_correction")):
+    out = {"workload": w.name, "grid": [w.im, w.jm, w.km], "steps": K, "warmup": W, "target": w.pjuge,
+           "defect_correction": {"max_outer": outer, "inner": inner}}
+    runs = [("fp32_sor", None)] + [(f"fp64_dc_omega_{om:g}", om) for om in omegas]
+    for tag, om in runs:
+        print(f"[bench] precision: {tag}", file=sys.stderr, flush=True)
         dom = api.Domain(setup_gen.build(w, dense=False))
-        if mode:
-            dom.set_press_mode(1, outer=4, inner=25)
+        if om is not None:
+            dom.set_press_mode(1, outer=outer, inner=inner)
+            dom.set_press_omega(om)
         nmax = W + K + 1
         for n in range(1, W + 1):
             dom.step(n, nmax)
@@ -218,17 +257,79 @@ def precision_comparison(K: int, W: int) -> int:
             dom.step(n, nmax)
         dom.synchronize()
         dt_ = (time.perf_counter() - t0) / K
-        st = dom.press_stats()
-        rec = {"ms_per_step": round(1e3 * dt_, 3), "mcell_updates_per_s": round(w.cells / dt_ / 1e6, 1)}
-        if mode:
+        # one more step with its statistics read back (graph replays as timed)
+        sweeps, finals = [], []
+        for n in range(W + K + 2, W + K + 4):
+            dom.step(n, nmax)
+            st = dom.press_stats()
             h = st["sor"]
-            rec.update({"sor_iterations": 4 * 25, "defect_sumsq_per_outer_step": [float(x) for x in h],
-                        "defect_reduction": float(h[-1] / h[0])})
-        else:
-            rec.update({"sor_iterations": st["iters"], "last_sum_reltmp_sq": float(st["sor"][-1])})
+            if om is None:
+                sweeps.append(st["iters"])
+            else:
+                conv = len(h) and h[-1] < w.pjuge
+                sweeps.append(inner * (len(h) - 1 if conv else len(h)))
+            finals.append(float(h[-1]))
+        rec = {"ms_per_step": round(1e3 * dt_, 3), "mcell_updates_per_s": round(w.cells / dt_ / 1e6, 1),
+               "sweeps_per_step": sweeps, "final_residual_measure": finals,
+               "reached_target": [f < w.pjuge for f in finals]}
+        if om is not None:
+            rec["defect_history_last_step"] = [float(f"{x:.3e}") for x in h]
         out[tag] = rec
         dom.close()
-    out["time_ratio"] = round(out["fp64_defect_correction"]["ms_per_step"] / out["fp32_sor"]["ms_per_step"], 3)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def accuracy_table(wname: str, nsteps: int, dc: tuple | None, dc_omega: float = 1.0) -> int:
+    """Per-field error against the FP64 oracle: the workload stepped on the
+    GPU (FP32 stencils, FP64 sums), by the C restatement as the reference
+    ships it (FP32 everywhere, sequential FP32 sums; equal to the
+    reference's own build bit for bit) and by the restatement's FP64
+    instantiation (pinned bit for bit to the reference with every real
+    promoted to kind 8), from the same seeded set-up.  Reports per step the
+    relative L2 error of u, v, w, p, sm against FP64 for the GPU, for the
+    FP32 reference and, with dc = (outer, inner), for the GPU's FP64
+    defect-correction press."""
+    sys.path.insert(0, str(ROOT))
+    from oracle import oracle
+    w = dataclasses.replace(setup_gen.workloads()[wname], nsteps=nsteps)
+    s = setup_gen.build(w)
+    fields = ("u", "v", "w", "p", "sm")
+
+    def rel(a, b):
+        a = np.asarray(a, np.float64)
+        b = np.asarray(b, np.float64)
+        return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+    runs = {"gpu": api.Domain(s)}
+    if dc:
+        runs["gpu_defect_correction"] = api.Domain(s)
+        runs["gpu_defect_correction"].set_press_mode(1, outer=dc[0], inner=dc[1])
+        runs["gpu_defect_correction"].set_press_omega(dc_omega)
+    o64 = oracle.OracleBackend(s, 64, redmode=0)
+    r32 = oracle.OracleBackend(s, 32, redmode=0)
+    table = {k: [] for k in list(runs) + ["reference_fp32"]}
+    print(f"[bench] accuracy: {w.name}, {nsteps} steps", file=sys.stderr, flush=True)
+    for n in range(1, nsteps + 1):
+        for d in runs.values():
+            d.step(n, nsteps)
+        o64.step(n, nsteps)
+        r32.step(n, nsteps)
+        ref = o64.fetch(list(fields))
+        got32 = r32.fetch(list(fields))
+        for k, d in runs.items():
+            table[k].append({f: rel(d.download(f), ref[f]) for f in fields})
+        table["reference_fp32"].append({f: rel(got32[f], ref[f]) for f in fields})
+        print(f"[bench] accuracy step {n}: gpu p {table['gpu'][-1]['p']:.2e}, "
+              f"fp32 reference p {table['reference_fp32'][-1]['p']:.2e}", file=sys.stderr, flush=True)
+    for d in runs.values():
+        d.close()
+    out = {"accuracy": {"workload": w.name, "grid": [w.im, w.jm, w.km], "steps": nsteps,
+                        "oracle": "FP64 restatement (orc64, bit for bit the kind-promoted reference build)",
+                        "metric": "relative L2 error per field after each step",
+                        "defect_correction": {"outer": dc[0], "inner": dc[1], "omega": dc_omega} if dc else None,
+                        "rel_l2": {k: [{f: float(f"{x:.3e}") for f, x in row.items()} for row in v]
+                                   for k, v in table.items()}}}
     print(json.dumps(out), flush=True)
     return 0
 
@@ -270,6 +371,15 @@ def main(argv=None):
     ap.add_argument("--precision", action="store_true",
                     help="only compare the reference SOR with the FP64 defect-correction press on the "
                          "precision-512x512x128 workload (prints its own JSON line)")
+    ap.add_argument("--accuracy", type=int, default=0, metavar="STEPS",
+                    help="only report the per-field error against the FP64 oracle over this many steps of the "
+                         "workload (GPU, FP32 reference, and with --dc the defect-correction press); prints its "
+                         "own JSON line")
+    ap.add_argument("--dc", default="", metavar="OUTER,INNER",
+                    help="with --accuracy: also the FP64 defect-correction press with these outer and inner steps; "
+                         "with --precision: its largest outer and its inner steps (default 64,32)")
+    ap.add_argument("--dc-omegas", default="", metavar="LIST",
+                    help="with --precision: inner relaxations of the defect correction (default 1,1.9)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run device-to-device copies of this size (HBM counter calibration)")
     ap.add_argument("--serve-poisson", default="", metavar="RATES",
@@ -288,8 +398,15 @@ def main(argv=None):
                              args.serve_steps)
     if args.copy_calibration:
         return copy_calibration(args.copy_calibration)
+    if args.accuracy:
+        dc = tuple(int(x) for x in args.dc.split(",")) if args.dc else None
+        om = float(args.dc_omegas.split(",")[0]) if args.dc_omegas else 1.0
+        return accuracy_table(args.workload, args.accuracy, dc, om)
     if args.precision:
-        return precision_comparison(args.steps if args.steps != 100 else 20, args.warmup if args.warmup != 10 else 3)
+        oms = [float(x) for x in args.dc_omegas.split(",")] if args.dc_omegas else [1.0, 1.9]
+        outer, inner = (int(x) for x in (args.dc or "64,32").split(","))
+        return precision_comparison(args.steps if args.steps != 100 else 5, args.warmup if args.warmup != 10 else 2,
+                                    oms, outer, inner)
 
     w = setup_gen.workloads()[args.workload]
     # the CPU baseline needs the host set-up arrays (masks, cn1); without it

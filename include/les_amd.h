@@ -149,6 +149,8 @@ int les_aveflow(int n, int n1, int km, int jm, int im, float* aveu, float* avev,
  * nmaxp = 50, pjuge = 1e-4, omega = 1.  Applies to domains created after
  * the call and to reference-signature domains. */
 int les_press_config(int nmaxp, float pjuge, float omega);
+/* The values les_press_config set (the reference's parameters by default). */
+int les_press_config_get(int* nmaxp, float* pjuge, float* omega);
 
 /* Statistics of the last press call on the reference-signature domain of
  * extents (im,jm,km): SOR iterations, sum(reltmp^2) per iteration, the
@@ -260,10 +262,19 @@ int les_domain_set_inflow(les_domain* d, const float* u_in, const float* v_in, c
  * `outer` times, the defect d = (rhs - rhsav) - B(p) of the SOR's equation
  * is formed in FP64 and `inner` FP32 SOR iterations (from zero, no stop
  * test) solve B(e) = d, then p += e in FP64; p = p - pav in FP64 at the
- * end.  In mode 1 les_domain_press_stats reports `outer` as the iteration
- * count and sum(d^2) of each outer step as the history.  No reference
+ * end.  In mode 1 les_domain_press_stats reports the number of defect norms
+ * recorded as the iteration count and sum((cn1*d)^2) before each outer
+ * step's correction as the history.  No reference
  * counterpart: a precision variant (1 <= outer <= 64, 1 <= inner <= 1024). */
 int les_domain_set_press_mode(les_domain* d, int mode, int outer, int inner);
+/* Press mode 1: the relaxation of the inner FP32 SOR iterations (default 1,
+ * the reference's; 0 < omega < 2).  Over-relaxation cuts the defect in far
+ * fewer sweeps; the FP64 outer loop keeps the result that of the exact
+ * equation.  The outer loop stops once sum((cn1*d)^2), the reference's
+ * stop-test measure for the FP64 iterate, is below the domain's pjuge; the
+ * statistics then report the norms up to that one.  No reference
+ * counterpart: a precision variant. */
+int les_domain_set_press_omega(les_domain* d, float omega);
 /* 1 when every call issued on the domain has completed, 0 while work is
  * pending, -1 on error (does not wait).  No reference counterpart: used by
  * hosts that serve several domains at once. */

@@ -1616,11 +1616,15 @@ __global__ __launch_bounds__(NT) void k_dc_init(Grid g, Metrics m, const float* 
                                                 const float* __restrict__ f, const float* __restrict__ gg,
                                                 const float* __restrict__ h, const float* __restrict__ p,
                                                 double* __restrict__ R64, double* __restrict__ P64,
-                                                double* __restrict__ part, double dt) {
+                                                double* __restrict__ part, double dt, Ctrl* ctrl) {
   __shared__ double lds[NT / kWave];
   const int tid = threadIdx.y * BX + threadIdx.x;
   const int bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
   const int i = 1 + blockIdx.x * BX + threadIdx.x, j = 1 + blockIdx.y * BY + threadIdx.y;
+  if (bid == 0 && tid == 0) {  // a new press call: no defect-correction step taken yet
+    ctrl->dc_done = 0;
+    ctrl->dc_norms = 0;
+  }
   double acc = 0.0;
   if (i <= g.im && j <= g.jm) {
     const double dxi = M1(m.dx1, i), dyj = M1(m.dy1, j);
@@ -1645,9 +1649,13 @@ __global__ __launch_bounds__(NT) void k_dc_init(Grid g, Metrics m, const float* 
 
 // d = (R64 - rhsav64) - B(P64) in FP64: its FP32 copy into the split rhs
 // the SOR reads, e = 0 in the split iterate of parity 0, block partials of
-// d^2; block 0 resets the SOR control for the inner solve.  (The defect is
-// not projected: K inner iterations from e = 0 are then, in exact
-// arithmetic, K further iterations of the reference's SOR.)
+// (cn1*d)^2 -- the measure of the reference's stop test (press.f95:93-104,
+// reltmp = -cn1*d at omega 1 for a cell updated from P64); block 0 resets
+// the SOR control for the inner solve.  (The defect is not projected: K
+// inner iterations from e = 0 are then, in exact arithmetic, K further
+// iterations of the SOR with the inner relaxation.)  Once an earlier outer
+// step met the stop threshold (ctrl->dc_done) the launch only keeps the
+// inner solve from running.
 template <bool SEP>
 __global__ __launch_bounds__(NT) void k_dc_defect(Grid g, SplitGrid s, Metrics m, const float* __restrict__ cn1f,
                                                   const double* __restrict__ R64,
@@ -1657,11 +1665,13 @@ __global__ __launch_bounds__(NT) void k_dc_defect(Grid g, SplitGrid s, Metrics m
   const int tid = threadIdx.y * BX + threadIdx.x;
   const int bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
   const double rav = ctrl->rhsav64;
+  const int done = ctrl->dc_done;
   __syncthreads();  // every thread has read the control block before block 0 writes it
   if (bid == 0 && tid == 0) {
-    ctrl->sor_done = 0;
+    ctrl->sor_done = done ? 1 : 0;  // a finished correction: the inner launches exit at once
     ctrl->nonfinite = 0;
   }
+  if (done) return;  // uniform
   const int i = 1 + blockIdx.x * BX + threadIdx.x, j = 1 + blockIdx.y * BY + threadIdx.y;
   double acc = 0.0;
   if (i <= g.im && j <= g.jm) {
@@ -1689,26 +1699,35 @@ __global__ __launch_bounds__(NT) void k_dc_defect(Grid g, SplitGrid s, Metrics m
       const int q = cix(s, t, j, k);
       sb.rhs[col][q] = (float)dd;
       sb.p[0][col][q] = 0.0f;
-      acc += dd * dd;
+      const double rd = (double)cn * dd;
+      acc += rd * rd;
     }
   }
   const double sum = block_sum<NT>(acc, tid, lds);
   if (tid == 0) part[bid] = sum;
 }
 
-// dc_hist[step] = sum d^2 of the outer step
+// dc_hist[step] = sum (cn1*d)^2 before the outer step's correction; below
+// pjuge (or non-finite) the iterate is final and the later steps skip
 __global__ __launch_bounds__(256) void k_dc_norm(const double* __restrict__ part, int n, double* __restrict__ hist,
-                                                 int step) {
+                                                 int step, Ctrl* ctrl, float pjuge) {
   __shared__ double lds[4];
+  if (ctrl->dc_done) return;  // uniform
   double s = 0.0;
   for (int b = threadIdx.x; b < n; b += 256) s += part[b];
   s = block_sum<256>(s, threadIdx.x, lds);
-  if (threadIdx.x == 0) hist[step] = s;
+  if (threadIdx.x == 0) {
+    hist[step] = s;
+    ctrl->dc_norms = step + 1;
+    if (s < (double)pjuge || !(s == s) || s > 1.0e300) ctrl->dc_done = 1;
+  }
 }
 
-// P64 += e (the inner solve's last iterate, parity `par`)
+// P64 += e (the inner solve's last iterate, parity `par`), unless the
+// defect met the stop threshold
 __global__ __launch_bounds__(NT) void k_dc_correct(Grid g, SplitGrid s, SplitBufs sb, double* __restrict__ P64,
-                                                   int par) {
+                                                   int par, const Ctrl* ctrl) {
+  if (ctrl->dc_done) return;  // uniform
   const int i = 1 + blockIdx.x * BX + threadIdx.x, j = 1 + blockIdx.y * BY + threadIdx.y;
   if (i > g.im || j > g.jm) return;
   const int t = (i - 1) >> 1;
@@ -1739,13 +1758,16 @@ __global__ __launch_bounds__(NT) void k_dc_pav(Grid g, Metrics m, const double* 
 }
 
 // p = P64 - pav with the ghost cells of boundp1/boundp2 (as k_padj), and
-// the defect history into the statistics (sor_iters = outer steps)
+// the defect history into the statistics (sor_iters = defect norms
+// recorded: the outer steps taken, plus the final one below the threshold
+// when the correction stopped early)
 __global__ __launch_bounds__(NT) void k_dc_padj(Grid g, const double* __restrict__ P64, float* __restrict__ p,
-                                                Ctrl* ctrl, const double* __restrict__ hist, int nouter) {
+                                                Ctrl* ctrl, const double* __restrict__ hist) {
   const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
   if (blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0 && threadIdx.x == 0 && threadIdx.y == 0) {
-    for (int q = 0; q < nouter; q++) ctrl->sor_hist[q] = hist[q];
-    ctrl->sor_iters = nouter;
+    const int nn = ctrl->dc_norms;
+    for (int q = 0; q < nn; q++) ctrl->sor_hist[q] = hist[q];
+    ctrl->sor_iters = nn;
     ctrl->sor_done = 0;
   }
   if (i > g.im + 1 || j > g.jm + 1) return;
@@ -1933,6 +1955,8 @@ static void launch_press_dc(const DevDomain& d, const SplitBufs& sb, int gx, int
   double* P64 = d.dc64;
   double* R64 = d.dc64 + d.n3;
   double* hist = d.dc64 + 2 * (size_t)d.n3;
+  DevDomain di = d;  // the inner iterations run with the inner relaxation
+  di.omega = d.dc_omega;
   for (int o = 0; o < d.dc_outer; o++) {
     if (d.cn1_separable)
       TIMED(d, KC_RHS, hipLaunchKernelGGL((k_dc_defect<true>), dim3(gx, gy, gz), blk(), 0, d.stream, g, s, d.m,
@@ -1943,10 +1967,10 @@ static void launch_press_dc(const DevDomain& d, const SplitBufs& sb, int gx, int
                                           (const float*)d.F.cn1, (const double*)R64, (const double*)P64, sb,
                                           d.partials, d.ctrl));
     TIMED(d, KC_RHS, hipLaunchKernelGGL(k_dc_norm, dim3(1), dim3(256), 0, d.stream, (const double*)d.partials, nred,
-                                        hist, o));
-    for (int l = 1; l <= d.dc_inner; l++) launch_sor_iter(d, sb, l, 0.0f, h, h.nparts);
+                                        hist, o, d.ctrl, d.pjuge));
+    for (int l = 1; l <= d.dc_inner; l++) launch_sor_iter(di, sb, l, 0.0f, h, h.nparts);
     TIMED(d, KC_PAV, hipLaunchKernelGGL(k_dc_correct, dim3(gx, gy, g.km), blk(), 0, d.stream, g, s, sb, P64,
-                                        d.dc_inner & 1));
+                                        d.dc_inner & 1, (const Ctrl*)d.ctrl));
   }
   TIMED(d, KC_PAV, hipLaunchKernelGGL(k_dc_pav, dim3(gx, gy, gz), blk(), 0, d.stream, g, d.m, (const double*)P64,
                                       d.partials));
@@ -1954,7 +1978,7 @@ static void launch_press_dc(const DevDomain& d, const SplitBufs& sb, int gx, int
                                       d.area));
   const int px = (g.im + 2 + BX - 1) / BX, py = (g.jm + 2 + BY - 1) / BY;
   TIMED(d, KC_PADJ, hipLaunchKernelGGL(k_dc_padj, dim3(px, py, g.km + 2), blk(), 0, d.stream, g, (const double*)P64,
-                                       d.F.p, d.ctrl, (const double*)hist, d.dc_outer));
+                                       d.F.p, d.ctrl, (const double*)hist));
 }
 
 // The right-hand side (with bondfg), its mean, and the split arrays the SOR
@@ -1970,7 +1994,8 @@ static void press_head(const DevDomain& d, float dt, const SplitBufs& sb, int gx
     TIMED(d, KC_RHS, hipLaunchKernelGGL(k_dc_init, dim3(gx, gy, gz), blk(), 0, d.stream, g, d.m,
                                         (const float*)d.F.u, (const float*)d.F.v, (const float*)d.F.w,
                                         (const float*)d.F.f, (const float*)d.F.g, (const float*)d.F.h,
-                                        (const float*)d.F.p, d.dc64 + d.n3, d.dc64, d.partials, (double)dt));
+                                        (const float*)d.F.p, d.dc64 + d.n3, d.dc64, d.partials, (double)dt,
+                                        d.ctrl));
     TIMED(d, KC_RHS, hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, d.stream, d.partials, nred, d.ctrl, 2,
                                         d.area));
   }

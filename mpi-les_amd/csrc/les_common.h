@@ -63,6 +63,8 @@ struct Ctrl {
   int p_final;             // iterate set holding the SOR result (-1: the parity of sor_iters)
   int fix_iter;            // two-iteration launches: iteration to recompute alone (0: none) ...
   int fix_in, fix_out;     // ... from iterate set fix_in into set fix_out
+  int dc_done;             // press mode 1: the defect met the stop threshold (later outer steps skip)
+  int dc_norms;            // press mode 1: defect norms recorded in this press call
   double rhsav;            // volume-weighted mean of rhs
   double pav;              // volume-weighted mean of p
   double sor_first;        // sum of reltmp^2 of the current iteration's first colour

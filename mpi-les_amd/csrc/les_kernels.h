@@ -74,6 +74,7 @@ struct DevDomain {
   int press_mode;    // 0: the reference's SOR; 1: FP64 defect correction around FP32 SOR
   int dc_outer;      // defect-correction steps per press call
   int dc_inner;      // FP32 SOR iterations per defect-correction step
+  float dc_omega;    // relaxation of those inner iterations (1: the reference's)
   int n3;            // doubles per padded field (FP64 buffers of press mode 1)
   double* dc64;      // P64, R64 (n3 each) and the defect history (kDcMaxOuter)
   int sor_fused_blocks;  // target number of blocks of one fused SOR iteration (0: automatic)

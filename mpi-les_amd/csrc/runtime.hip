@@ -655,6 +655,7 @@ int domain_alloc(int im, int jm, int km, les_domain** out) {
   D->d.sor_tile_rows = env_int("LES_SOR_TILE_ROWS", 16) == 32 ? 32 : 16;
   D->d.sor_tile_width = (D->d.sor_tile_rows == 32 && env_int("LES_SOR_TILE_WIDTH", 64) == 32) ? 32 : 64;
   D->d.sor_entry_fast = env_int("LES_SOR_ENTRY_FAST", 1) != 0;
+  D->d.dc_omega = 1.0f;
   {
     int dev = 0, ncu = 0;
     if (hipGetDevice(&dev) == hipSuccess &&
@@ -1225,6 +1226,14 @@ int les_bondv1_set_inflow(int im, int jm, int km, const float* prof) {
   return les_bondv1_set_inflow_uvw(im, jm, km, prof, nullptr, nullptr);
 }
 
+int les_press_config_get(int* nmaxp, float* pjuge, float* omega) {
+  if (!nmaxp || !pjuge || !omega) return fail("null argument");
+  *nmaxp = nmaxp_default;
+  *pjuge = pjuge_default;
+  *omega = omega_default;
+  return 0;
+}
+
 int les_press_config(int nmaxp, float pjuge, float omega) {
   if (nmaxp < 1 || nmaxp > kSorHist) return fail("nmaxp out of range (1..1024)");
   nmaxp_default = nmaxp;
@@ -1392,6 +1401,13 @@ int les_domain_set_press_mode(les_domain* D, int mode, int outer, int inner) {
   D->d.press_mode = 1;
   D->d.dc_outer = outer;
   D->d.dc_inner = inner;
+  return 0;
+}
+
+int les_domain_set_press_omega(les_domain* D, float omega) {
+  if (!live_has(D)) return fail("null, unknown or destroyed domain");
+  if (!(omega > 0.0f && omega < 2.0f)) return fail("inner relaxation must lie in (0, 2)");
+  D->d.dc_omega = omega;
   return 0;
 }
 

@@ -114,6 +114,12 @@ module les_amd_c
             real(c_double) :: sor_hist(*), rhsav, pav
         end function
 
+        integer(c_int) function les_press_config_get(nmaxp,pjuge,omega) bind(C, name='les_press_config_get')
+            import :: c_int, c_float
+            integer(c_int) :: nmaxp
+            real(c_float) :: pjuge, omega
+        end function
+
         integer(c_int) function les_sync_host_c(a) bind(C, name='les_sync_host')
             import :: c_int, c_float
             real(c_float) :: a(*)
@@ -442,8 +448,9 @@ contains
         real(kind=4), dimension(0:im,0:jm,0:km), intent(InOut) :: usum, vsum, wsum
         integer :: i, j, k, nn
 #ifdef VERBOSE
-        integer(c_int) :: iters, nonfinite, l
+        integer(c_int) :: iters, nonfinite, l, nmaxp
         real(c_double) :: hist(1024), rhsav, pav
+        real(c_float) :: pjuge, omega
         real(kind=4) :: cflu, cflv, cflw
 #endif
         call les_check(les_press(km,jm,im,rhs,u,dx1,v,dy1,w,dzn,f,g,h,dt,cn1,cn2l,p,cn2s,cn3l,cn3s, &
@@ -463,7 +470,12 @@ contains
             call les_sync(u)
             call les_sync(v)
             call les_sync(w)
-            write(6,*) '=mac= time step, iteration step, conv =', n, iters, real(hist(min(iters, 1024)), 4)
+            ! the reference prints its DO variable l (press.f95:89,166-168): the
+            ! iteration that met the stop test, or nmaxp+1 when none did
+            call les_check(les_press_config_get(nmaxp, pjuge, omega), 'press_config_get')
+            l = iters
+            if (.not. (hist(min(iters, 1024)) < real(pjuge, c_double))) l = nmaxp + 1
+            write(6,*) '=mac= time step, iteration step, conv =', n, l, real(hist(min(iters, 1024)), 4)
             do k = 1,km,10
                 write(6,*) 'Inflow=',k,'u=',u(0,jm/2,k),v(0,jm/2,k),w(0,jm/2,k)
             end do

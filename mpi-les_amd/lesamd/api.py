@@ -93,6 +93,7 @@ def lib():
     L.les_domain_synchronize.argtypes = [vp]
     L.les_domain_query.argtypes = [vp]
     L.les_domain_set_press_mode.argtypes = [vp, ci, ci, ci]
+    L.les_domain_set_press_omega.argtypes = [vp, cf]
     L.les_domain_bytes_needed.argtypes = [ci, ci, ci]
     L.les_domain_bytes_needed.restype = ctypes.c_longlong
     L.les_domain_press_stats.argtypes = [vp, vp, vp, ci, vp, vp, vp]
@@ -279,6 +280,10 @@ class Domain:
         """0: the reference's SOR; 1: FP64 defect correction (`outer` steps
         of `inner` FP32 SOR iterations)."""
         _check(lib().les_domain_set_press_mode(self._h, mode, outer, inner))
+
+    def set_press_omega(self, omega: float):
+        """Relaxation of press mode 1's inner SOR iterations."""
+        _check(lib().les_domain_set_press_omega(self._h, float(omega)))
 
     def done(self) -> bool:
         """True when every call issued on the domain has completed (no wait)."""

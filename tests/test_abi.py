@@ -84,3 +84,76 @@ def test_calls_on_unknown_or_destroyed_handles_are_refused(lib):
         assert b"destroyed" in lib.les_last_error() or b"unknown" in lib.les_last_error(), name
     lib.les_domain_destroy.restype = ctypes.c_int
     assert lib.les_domain_destroy(None) == 0  # NULL is a no-op, as free(NULL)
+
+
+FORTRAN = ROOT / "mpi-les_amd" / "fortran" / "les_amd_modules.F90"
+
+
+def _fortran_interfaces():
+    """name -> list of C types of the bind(C) interfaces in the drop-in
+    modules, from their Fortran declarations: value integer(c_int) -> int,
+    value real(c_float) -> float, an assumed-size real(c_float) array ->
+    float*, a non-value scalar integer(c_int) / real(c_double) /
+    real(c_float) -> int* / double* / float*, type(c_ptr) value -> void*."""
+    text = FORTRAN.read_text()
+    text = re.sub(r"&\s*\n\s*", " ", text)  # join continuation lines
+    out = {}
+    for m in re.finditer(r"function\s+(\w+)\s*\(([^)]*)\)\s*bind\(C,\s*name='(\w+)'\)(.*?)end function", text,
+                         flags=re.S | re.I):
+        args = [a.strip() for a in m.group(2).split(",") if a.strip()]
+        body = m.group(4)
+        kinds = {}
+        for dm in re.finditer(r"^\s*(integer\(c_int\)|real\(c_float\)|real\(c_double\)|type\(c_ptr\))"
+                              r"(\s*,\s*value)?\s*::\s*(.*)$", body, flags=re.M | re.I):
+            base = {"integer(c_int)": "int", "real(c_float)": "float", "real(c_double)": "double",
+                    "type(c_ptr)": "void*"}[dm.group(1).lower()]
+            for var in re.findall(r"(\w+)(\(\*\))?", dm.group(3)):
+                name, arr = var
+                if dm.group(2) or base == "void*":
+                    kinds[name.lower()] = base
+                else:
+                    kinds[name.lower()] = base + "*"
+        out[m.group(3)] = [kinds.get(a.lower(), "?" + a) for a in args]
+    return out
+
+
+def _c_prototypes():
+    text = re.sub(r"/\*.*?\*/", "", HEADER.read_text(), flags=re.S)
+    out = {}
+    for m in re.finditer(r"\bint\s+(les_\w+)\s*\(([^)]*)\);", text, flags=re.S):
+        types = []
+        for p in m.group(2).split(","):
+            p = p.strip().replace("const ", "")
+            if not p or p == "void":
+                continue
+            t = p.rsplit(None, 1)[0] + ("*" * p.split()[-1].count("*"))
+            types.append(t.replace(" ", ""))
+        out[m.group(1)] = types
+    return out
+
+
+def test_fortran_interfaces_match_the_c_prototypes():
+    """Every bind(C) interface the drop-in Fortran modules declare has the
+    argument count and types of the prototype in include/les_amd.h (checked
+    from the two sources, without a GPU)."""
+    f, c = _fortran_interfaces(), _c_prototypes()
+    assert len(f) >= 12
+    for name, ftypes in f.items():
+        if name == "les_last_error":
+            continue
+        assert name in c, name
+        ctypes_ = c[name]
+        assert len(ftypes) == len(ctypes_), (name, ftypes, ctypes_)
+        for ft, ct in zip(ftypes, ctypes_):
+            # a C void* parameter takes any array (les_sync_host's host array)
+            assert ft == ct or (ct == "void*" and ft.endswith("*")), (name, ftypes, ctypes_)
+
+
+def test_fortran_modules_compile_with_verbose(tmp_path):
+    """The VERBOSE prints of the press module (press.f95:164-195) compile."""
+    flang = Path("/opt/rocm/lib/llvm/bin/flang")
+    if not flang.exists():
+        pytest.skip("flang absent")
+    r = subprocess.run([str(flang), "-cpp", "-O0", "-ffree-form", "-DVERBOSE", "-module-dir", str(tmp_path),
+                        "-c", str(FORTRAN), "-o", str(tmp_path / "m.o")], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
