@@ -165,37 +165,49 @@ def profiled_traffic(kernel_class: str, workload: str, sor_rows: int = 4):
 SERVE_BATCH_CELLS = 2 << 20  # domains above this many cells fill the GPU alone and are not batched
 
 
-def serve_measurement(nreq: int, nsteps: int, sizes=None) -> dict:
+def serve_measurement(nreq: int, nsteps: int, sizes=None, rate: float | None = None, streams: bool = True) -> dict:
     """Mixed-size requests (nx, ny in 96..512, nz in 64..128, seeded urban
-    set-ups; or the given sizes) all arriving at once: served with continuous
-    batching (the headline figures: domains of at most SERVE_BATCH_CELLS
-    cells stepped as one batch, larger ones on their own streams alongside),
-    concurrently on streams (one stream and step graph per domain) and one
+    set-ups with cover, heights and log-law inflow drawn per request; or the
+    given sizes), arriving as a Poisson process of the given rate (requests
+    per second; all at t=0 when None): served with continuous batching (the
+    headline figures: domains of at most SERVE_BATCH_CELLS cells stepped as
+    one batch, larger ones on their own streams alongside), optionally
+    concurrently on streams (one stream and step graph per domain), and one
     after another, admission by device memory in every mode, domains built
     and released on a worker thread.  Every grid shape's SOR launch shape is
     tuned before the timed runs, so that no mode pays for it.  Reports the
-    aggregate cell updates per second and the time-to-result percentiles of
-    each."""
+    aggregate cell updates per second of the requests that completed, the
+    time-to-result percentiles and the requests that failed (a diverged
+    flow) in each mode."""
     from lesamd import serving
-    reqs = serving.sample_requests(nreq, nsteps, seed=2024, sizes=sizes)
+    reqs = serving.sample_requests(nreq, nsteps, rate=rate, seed=2024, sizes=sizes)
     srv = serving.Server(budget_bytes=200 << 30, burst=4)
-    print(f"[bench] serving {nreq} requests x {nsteps} steps", file=sys.stderr, flush=True)
+    print(f"[bench] serving {nreq} requests x {nsteps} steps, "
+          f"{'Poisson arrivals at %g/s' % rate if rate else 'all at t=0'}", file=sys.stderr, flush=True)
     srv.warm(reqs)
     seq = serving.summarize(srv.run(reqs, sequential=True))
-    con = serving.summarize(srv.run(reqs))
+    con = serving.summarize(srv.run(reqs)) if streams else None
     bat = serving.summarize(srv.run_batched(reqs, max_batch_cells=SERVE_BATCH_CELLS))
-    return {
-        "requests": nreq, "steps_each": nsteps, "arrivals": "all at t=0",
+    offered = sum(r.cells * r.nsteps for r in reqs) / max(reqs[-1].arrival, 1e-9) / 1e6 if rate else None
+    out = {
+        "requests": nreq, "steps_each": nsteps,
+        "arrivals": f"Poisson, {rate:g} requests/s" if rate else "all at t=0",
+        "offered_mcell_updates_per_s": round(offered, 1) if offered else None,
         "mode": f"continuous batching of the domains of at most {SERVE_BATCH_CELLS} cells (each stage one launch "
                 "for all of them, SOR one launch per iteration), larger domains on their own streams",
         "mcell_updates_per_s": round(bat["mcell_updates_per_s"], 1),
-        "p50_s": round(bat["p50_s"], 4), "p99_s": round(bat["p99_s"], 4),
+        "p50_s": round(bat["p50_s"], 4), "p99_s": round(bat["p99_s"], 4), "failed": bat["failed"],
         "speedup_vs_sequential": round(bat["mcell_updates_per_s"] / seq["mcell_updates_per_s"], 3),
         "sequential_mcell_updates_per_s": round(seq["mcell_updates_per_s"], 1),
         "sequential_p50_s": round(seq["p50_s"], 4), "sequential_p99_s": round(seq["p99_s"], 4),
-        "streams_mcell_updates_per_s": round(con["mcell_updates_per_s"], 1),
-        "streams_p50_s": round(con["p50_s"], 4), "streams_p99_s": round(con["p99_s"], 4),
+        "sequential_failed": seq["failed"],
+        "fp64_oracle_error": "the same as one-at-a-time runs: served fields are bit-identical to the domain "
+                             "stepped alone (tests/test_gpu_batch.py, tests/test_gpu_configs.py)",
     }
+    if con is not None:
+        out.update({"streams_mcell_updates_per_s": round(con["mcell_updates_per_s"], 1),
+                    "streams_p50_s": round(con["p50_s"], 4), "streams_p99_s": round(con["p99_s"], 4)})
+    return out
 
 
 def poisson_sweep(rates: list[float], nreq: int, nsteps: int) -> int:
@@ -365,9 +377,12 @@ def main(argv=None):
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--eager", action="store_true", help="launch kernels one by one (no graph replay)")
     ap.add_argument("--no-tune", action="store_true", help="keep the default SOR launch shape (no autotuning)")
-    ap.add_argument("--serve-requests", type=int, default=24,
+    ap.add_argument("--serve-rate", type=float, default=4.0,
+                    help="Poisson arrival rate of the served mixed stream, requests per second (0: all at t=0); "
+                         "the default is past the rate where admission saturates")
+    ap.add_argument("--serve-requests", type=int, default=64,
                     help="mixed-size requests of the serving measurement (0: skip it)")
-    ap.add_argument("--serve-steps", type=int, default=40, help="time steps per served request")
+    ap.add_argument("--serve-steps", type=int, default=200, help="time steps per served request")
     ap.add_argument("--precision", action="store_true",
                     help="only compare the reference SOR with the FP64 defect-correction press on the "
                          "precision-512x512x128 workload (prints its own JSON line)")
@@ -543,11 +558,12 @@ def main(argv=None):
                                 "default_us_per_iter": round(tuned["us_per_iter_default"], 2)}
     dom.close()
     if args.serve_requests > 0:
-        result["served"] = serve_measurement(args.serve_requests, args.serve_steps)
-        # the same service on small domains only (nx, ny in 96..160, nz 64),
-        # where one domain cannot fill the GPU
+        result["served"] = serve_measurement(args.serve_requests, args.serve_steps,
+                                             rate=args.serve_rate if args.serve_rate > 0 else None, streams=False)
+        # small domains only (nx, ny in 96..160, nz 64), where one domain
+        # cannot fill the GPU: 24 requests of 40 steps arriving at once
         small = [(a, b, 64) for a in (96, 128, 160) for b in (96, 128, 160)]
-        result["served_small"] = serve_measurement(args.serve_requests, args.serve_steps, sizes=small)
+        result["served_small"] = serve_measurement(24, 40, sizes=small)
     if snapshot is not None:
         result["cpu_baseline"] = cpu_baseline(setup, snapshot, W + 1, args.cpu_steps, args.cpu_budget)
     print(json.dumps(result), flush=True)
