@@ -240,22 +240,27 @@ def poisson_sweep(rates: list[float], nreq: int, nsteps: int) -> int:
     return 0
 
 
-def precision_comparison(K: int, W: int, omegas, outer: int, inner: int) -> int:
+def precision_comparison(K: int, W: int, configs) -> int:
     """The precision-512x512x128 workload (dx=dy=2 m, 35 % cover, u*=0.6,
     z0=0.3, dt=0.02; SOR at most 100 iterations, stop at 1e-6) stepped with
-    the reference's FP32 SOR and with the FP64 defect correction (at most
-    `outer` steps of `inner` FP32 SOR iterations, stopping once
-    sum((cn1*d)^2) < 1e-6) for each inner relaxation in omegas: time per
-    step from graph replay, and per step of the timed run the measure the
-    stop test compares with 1e-6 and the sweeps taken."""
+    the reference's FP32 SOR and with the FP64 defect correction for each
+    (outer, inner, omega) in configs (at most `outer` steps of `inner` FP32
+    SOR iterations with relaxation omega, stopping once sum((cn1*d)^2) <
+    1e-6): time per step from graph replay, and per step of the timed run
+    the measure the stop test compares with 1e-6 and the sweeps taken."""
     w = setup_gen.workloads()["precision-512x512x128"]
-    out = {"workload": w.name, "grid": [w.im, w.jm, w.km], "steps": K, "warmup": W, "target": w.pjuge,
-           "defect_correction": {"max_outer": outer, "inner": inner}}
-    runs = [("fp32_sor", None)] + [(f"fp64_dc_omega_{om:g}", om) for om in omegas]
-    for tag, om in runs:
+    out = {"workload": w.name, "grid": [w.im, w.jm, w.km], "steps": K, "warmup": W, "target": w.pjuge}
+    # the reference's SOR as shipped (100 iterations), and with 1024 to show
+    # where its FP32 iterate stops improving
+    runs = [("fp32_sor", None), ("fp32_sor_1024", None)]
+    runs += [(f"fp64_dc_{o}x{i}_omega_{om:g}", (o, i, om)) for o, i, om in configs]
+    for tag, cfg in runs:
         print(f"[bench] precision: {tag}", file=sys.stderr, flush=True)
-        dom = api.Domain(setup_gen.build(w, dense=False))
-        if om is not None:
+        wt = dataclasses.replace(w, nmaxp=1024) if tag == "fp32_sor_1024" else w
+        dom = api.Domain(setup_gen.build(wt, dense=False))
+        om = None
+        if cfg is not None:
+            outer, inner, om = cfg
             dom.set_press_mode(1, outer=outer, inner=inner)
             dom.set_press_omega(om)
         nmax = W + K + 1
@@ -286,6 +291,10 @@ def precision_comparison(K: int, W: int, omegas, outer: int, inner: int) -> int:
                "reached_target": [f < w.pjuge for f in finals]}
         if om is not None:
             rec["defect_history_last_step"] = [float(f"{x:.3e}") for x in h]
+        elif len(h) > 100:
+            rec["residual_measure_at_iteration"] = {str(q): float(f"{h[q - 1]:.4e}")
+                                                    for q in (100, 200, 400, 800, len(h)) if q <= len(h)}
+            rec["smallest_residual_measure"] = float(f"{float(np.min(h)):.4e}")
         out[tag] = rec
         dom.close()
     print(json.dumps(out), flush=True)
@@ -391,10 +400,13 @@ def main(argv=None):
                          "workload (GPU, FP32 reference, and with --dc the defect-correction press); prints its "
                          "own JSON line")
     ap.add_argument("--dc", default="", metavar="OUTER,INNER",
-                    help="with --accuracy: also the FP64 defect-correction press with these outer and inner steps; "
-                         "with --precision: its largest outer and its inner steps (default 64,32)")
-    ap.add_argument("--dc-omegas", default="", metavar="LIST",
-                    help="with --precision: inner relaxations of the defect correction (default 1,1.9)")
+                    help="with --accuracy: also the FP64 defect-correction press with these outer and inner steps")
+    ap.add_argument("--dc-omegas", default="", metavar="OMEGA",
+                    help="with --accuracy: the inner relaxation of that press (default 1)")
+    ap.add_argument("--dc-configs", default="4x25@1,4x25@1.9,64x32@1.9", metavar="LIST",
+                    help="with --precision: defect-correction runs OUTERxINNER@OMEGA, comma-separated (the "
+                         "default: the reference's 100 sweeps as 4 x 25 at omega 1 and 1.9, and up to 2048 "
+                         "sweeps at 1.9)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run device-to-device copies of this size (HBM counter calibration)")
     ap.add_argument("--serve-poisson", default="", metavar="RATES",
@@ -418,10 +430,13 @@ def main(argv=None):
         om = float(args.dc_omegas.split(",")[0]) if args.dc_omegas else 1.0
         return accuracy_table(args.workload, args.accuracy, dc, om)
     if args.precision:
-        oms = [float(x) for x in args.dc_omegas.split(",")] if args.dc_omegas else [1.0, 1.9]
-        outer, inner = (int(x) for x in (args.dc or "64,32").split(","))
+        cfgs = []
+        for c in args.dc_configs.split(","):
+            oi, om = c.split("@")
+            o, i = oi.split("x")
+            cfgs.append((int(o), int(i), float(om)))
         return precision_comparison(args.steps if args.steps != 100 else 5, args.warmup if args.warmup != 10 else 2,
-                                    oms, outer, inner)
+                                    cfgs)
 
     w = setup_gen.workloads()[args.workload]
     # the CPU baseline needs the host set-up arrays (masks, cn1); without it
@@ -435,8 +450,9 @@ def main(argv=None):
     for n in range(1, W + 1):
         dom.step(n, nmax)
     dom.synchronize()
-    # the SOR launch shape for this grid shape (les_domain_autotune), picked
-    # before the timed steps as a served domain gets it at admission
+    # the SOR launch shape and the stencil kernels' k levels per thread for
+    # this grid shape (les_domain_autotune), picked before the timed steps as
+    # a served domain gets them at admission
     tuned = None if args.no_tune else dom.autotune(reps=10)
     snapshot = None if args.no_cpu_baseline else {n: dom.download(n) for n in STATE_FIELDS}
 
@@ -556,6 +572,12 @@ def main(argv=None):
                                 "tile_pairs": tuned["tile_width"], "k_levels_per_block": tuned["kchunk"],
                                 "tuned_us_per_iter": round(tuned["us_per_iter"], 2),
                                 "default_us_per_iter": round(tuned["us_per_iter_default"], 2)}
+        result["stencil_launch"] = {"k_levels_per_thread": {"velfg_feedbf": tuned["velfg_kchunk"],
+                                                            "les_sm": tuned["sm_kchunk"],
+                                                            "les_visc_adam": tuned["visc_kchunk"]},
+                                    "note": "0: the automatic chunk (about 36 blocks per compute unit)",
+                                    "tuned_us": round(tuned["stencil_us"], 2),
+                                    "default_us": round(tuned["stencil_us_default"], 2)}
     dom.close()
     if args.serve_requests > 0:
         result["served"] = serve_measurement(args.serve_requests, args.serve_steps,

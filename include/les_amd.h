@@ -228,6 +228,15 @@ int les_domain_step(les_domain* d, int n, int nmax);
  * counterpart: the per-shape autotuner of the serving path. */
 int les_domain_autotune(les_domain* d, int reps, int* rows, int* tile_rows, int* tile_width, int* kchunk,
                         double* us_best, double* us_default);
+/* The k levels per thread of the fused stencil kernels (velfg+feedbf,
+ * les_sm, les viscous+adam; 0 = automatic) in use by the domain, as
+ * les_domain_autotune picks them with the SOR shape (timing each candidate
+ * on the domain's fields, which it restores), and the microseconds of one
+ * velfg+feedbf plus one les+adam launch sequence with the tuned and with
+ * the automatic chunks (0 when not timed).  Results do not depend on the
+ * chunks.  No reference counterpart: part of the per-shape autotuner. */
+int les_domain_stencil_chunks(les_domain* d, int* velfg_kchunk, int* sm_kchunk, int* visc_kchunk, double* us_best,
+                              double* us_default);
 /* Microseconds per fused SOR iteration with the domain's current launch
  * shape (a launch of two iterations counts as two; an odd iters is rounded
  * up to whole launches): `iters` back-to-back iterations (numbered 2.., never stopping) on

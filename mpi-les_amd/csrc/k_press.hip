@@ -180,6 +180,122 @@ __global__ __launch_bounds__(NT) void k_rhs_split(Grid g, SplitGrid s, float* __
   rhs_split_body(hw_blk<false>(), g, s, rhs, p, cn1, sb, ctrl);
 }
 
+// The head of press with two SOR iterations per launch, in one pass: the
+// rhs of k_rhs (the same expression, grid, block partials and ghost stores
+// of f, g, h), stored into the colour-split rhsr before its mean is removed
+// (the launch of SOR iterations 1 and 2 subtracts the mean and stores the
+// split rhs), the split copy of p (iterate set 0), and the ghost comparison
+// of k_rhs_split.  Each block's comparison result goes to part[nred + block]
+// for k_finalize_head, so that no block clears a flag another one may have
+// raised.  The reference-layout rhs is not written here (a download gathers
+// it from the split rhs).
+__global__ __launch_bounds__(NT) void k_rhs_head(Grid g, SplitGrid s, Metrics m, const float* __restrict__ u,
+                                                 const float* __restrict__ v, const float* __restrict__ w,
+                                                 float* __restrict__ f, float* __restrict__ gg,
+                                                 float* __restrict__ h, const float* __restrict__ p, SplitBufs sb,
+                                                 double* __restrict__ part, int nred, Ctrl* ctrl, float dt) {
+  __shared__ double lds[NT / kWave];
+  const VBlk vb = hw_blk<false>();
+  const int tid = threadIdx.y * BX + threadIdx.x;
+  const int bid = vlinear(vb);
+  if (bid == 0 && tid == 0) {
+    ctrl->sor_done = 0;
+    ctrl->sor_iters = 0;
+    ctrl->nonfinite = 0;
+    ctrl->p_final = -1;
+    ctrl->fix_iter = 0;
+  }
+  const int i = 1 + vb.x * BX + threadIdx.x, j = 1 + vb.y * BY + threadIdx.y;
+  double acc = 0.0;
+  bool diff = false;
+  if (i <= g.im && j <= g.jm) {
+    const float dxi = M1(m.dx1, i), dyj = M1(m.dy1, j);
+    const int t = (i - 1) >> 1;
+    const auto same = [](float a, float b) { return __float_as_uint(a) == __float_as_uint(b); };
+    const bool lane0 = threadIdx.x == 0;
+    int k0, k1;
+    kspan(vb, 1, g.km, k0, k1);
+    // w and h of level k-1 come from the thread's previous level, u and f
+    // of cell i-1 from the neighbouring lane: the same values as the loads
+    // of k_rhs (nothing here writes them before they are read)
+    float wkm = 0.0f, hkm = 0.0f;
+    if (k0 <= k1) {
+      wkm = w[ix(g, i, j, k0 - 1)];
+      hkm = h[ix(g, i, j, k0 - 1)];
+    }
+    for (int k = k0; k <= k1; k++) {
+      const int c = ix(g, i, j, k);
+      const float dzk = M1(m.dzn, k);
+      const float fc = f[c], gc = gg[c], uc = u[c], wc = w[c], hk = h[c];
+      float fl = __shfl_up(fc, 1, kWave), ul = __shfl_up(uc, 1, kWave);
+      if (lane0) {  // the block's first lane: cell i-1 belongs to the previous block
+        fl = f[c - 1];
+        ul = u[c - 1];
+      }
+      const float fm = (i == 1) ? fc : fl;
+      const float gm = (j == 1) ? gg[ix(g, i, g.jm, k)] : gg[c - g.sj];
+      const float hc = (k == g.km) ? 0.0f : hk;
+      const float hm = (k == 1) ? 0.0f : hkm;
+      if (i == 1) f[c - 1] = fc;
+      if (j == 1) gg[c - g.sj] = gm;
+      if (k == 1) h[c - g.sk] = 0.0f;
+      if (k == g.km) h[c] = 0.0f;
+      const float r = (-ul + uc) / dxi + (-v[c - g.sj] + v[c]) / dyj + (-wkm + wc) / dzk;
+      const float rr = (fc - fm) / dxi + (gc - gm) / dyj + (hc - hm) / dzk + r / dt;
+      wkm = wc;
+      hkm = hk;
+      acc += (double)(dxi * dyj * dzk * rr);
+      const int col = (i + j + k) & 1;  // 1 = red
+      const int q = cix(s, t, j, k);
+      sb.rhsr[col][q] = rr;
+      const float pc = p[c];
+      sb.p[0][col][q] = pc;
+      if (i == 1) diff |= !same(p[c - 1], pc);
+      if (i == g.im) diff |= !same(p[c + 1], pc);
+      if (j == 1) diff |= !same(p[ix(g, i, 0, k)], p[ix(g, i, g.jm, k)]);
+      if (j == g.jm) diff |= !same(p[ix(g, i, g.jm + 1, k)], p[ix(g, i, 1, k)]);
+      if (k == 1) diff |= !same(p[c - g.sk], pc);
+      if (k == g.km) diff |= !same(p[c + g.sk], pc);
+    }
+  }
+  const double sum = block_sum<NT>(acc, tid, lds);
+  const int anyd = __syncthreads_or(diff ? 1 : 0);
+  if (tid == 0) {
+    part[bid] = sum;
+    part[nred + bid] = anyd ? 1.0 : 0.0;
+  }
+}
+
+// rhsav from the partials of k_rhs_head (as k_finalize: the same order and
+// sum), and ctrl->p_ghost_diff from its per-block flags.
+__global__ __launch_bounds__(256) void k_finalize_head(const double* __restrict__ part, int n, Ctrl* ctrl,
+                                                       double denom) {
+  __shared__ double lds[4];
+  double s = 0.0, fl = 0.0;
+  for (int b = threadIdx.x; b < n; b += 256) {
+    s += part[b];
+    fl += part[n + b];
+  }
+  s = block_sum<256>(s, threadIdx.x, lds);
+  fl = block_sum<256>(fl, threadIdx.x, lds);
+  if (threadIdx.x == 0) {
+    ctrl->rhsav = s / denom;
+    ctrl->p_ghost_diff = fl != 0.0 ? 1 : 0;
+  }
+}
+
+// rhs(i,j,k) = the split rhs, on the interior (press.f95:81-87 leave the
+// ghosts of rhs untouched).
+__global__ __launch_bounds__(NT) void k_rhs_gather(Grid g, SplitGrid s, SplitBufs sb, float* __restrict__ rhs) {
+  const VBlk vb = hw_blk<false>();
+  const int i = 1 + vb.x * BX + threadIdx.x, j = 1 + vb.y * BY + threadIdx.y;
+  if (i > g.im || j > g.jm) return;
+  const int t = (i - 1) >> 1;
+  int k0, k1;
+  kspan(vb, 1, g.km, k0, k1);
+  for (int k = k0; k <= k1; k++) rhs[ix(g, i, j, k)] = sb.rhs[(i + j + k) & 1][cix(s, t, j, k)];
+}
+
 // ---------------------------------------------------------------- SOR
 // One colour sweep of iteration `iter` (press.f95:92-104).  COL 1 = red
 // (the first sweep), 0 = black.  Red sweeps of iteration >= 2 run the
@@ -1018,6 +1134,8 @@ struct PairLaunch {
   int prev_iter;    // the previous launch (0: none), whose stop test block 0 runs ...
   int prev_two, prev_in, prev_out, prev_nparts, rpart;  // ... its shape, sets, slots
   int fixup;        // the fix-up launch: iteration and sets from ctrl->fix_*, one iteration
+  int raw;          // the launch of iteration 1 after k_rhs_head: rhs from rhsr less the mean, stored into rhs
+  int gate;         // 0: runs; 1: runs only when no ghost of p differs from its boundp copy; 2: only when one does
 };
 
 // The stop test (press.f95:119-121) of a launch of iteration L (and L+1 when
@@ -1048,8 +1166,9 @@ __device__ __forceinline__ void pair_stop(Ctrl* ctrl, int L, int two, double s1,
 
 constexpr int PTY = 32, PR = 4, PNT = FX * PTY / 2, PFTY = PTY - 2 * PR, PFTX = FX - 2 * PR;
 
-// ENT: the launch may hold iteration 1 (the entry-ghost code is compiled in).
-template <bool OM1, bool ENT>
+// ENT: the launch may hold iteration 1 (the entry-ghost code is compiled in);
+// RAW: the launch of iteration 1 after k_rhs_head.
+template <bool OM1, bool ENT, bool RAW>
 __device__ __forceinline__ void sor_pair_block(const Grid& g, const SplitGrid& s, const Metrics& m,
                                                const SplitBufs& sb, Ctrl* ctrl, const PairLaunch& pl, int nparts,
                                                float omega, float pjuge, int ntt, int ntj, int kchunk, int remap,
@@ -1073,17 +1192,24 @@ __device__ __forceinline__ void sor_pair_block(const Grid& g, const SplitGrid& s
       si = ctrl->fix_in;
       so = ctrl->fix_out;
     }
-    sh[0] = skip;
     sh[1] = it;
     sh[2] = si;
     sh[3] = so;
     sh[4] = (it == 1) && ctrl->p_ghost_diff != 0;
+    if (pl.gate != 0) skip = skip || ((ctrl->p_ghost_diff != 0) != (pl.gate == 2));
+    sh[0] = skip;
   }
   __syncthreads();
   if (sh[0]) return;
   const int in_set = sh[2], out_set = sh[3];
   const bool two = pl.two != 0 && !pl.fixup;
   const bool ent = ENT && sh[4] != 0;  // iteration 1 on the entry ghosts of p
+  // after k_rhs_head, the launch of iteration 1 reads the rhs before its
+  // mean is removed, subtracts the mean as k_rhs_split does (in FP64) on
+  // every value it loads, and stores each owned cell's result once into the
+  // split rhs that later launches read
+  constexpr bool raw = RAW;  // never a fix-up launch
+  const double rav = raw ? ctrl->rhsav : 0.0;
   if (!pl.fixup && pl.prev_iter > 0 && bid == 0) {  // the previous launch's stop test
     double s1 = 0.0, s2 = 0.0;
     for (int b = tid; b < pl.prev_nparts; b += GNT) {
@@ -1101,7 +1227,13 @@ __device__ __forceinline__ void sor_pair_block(const Grid& g, const SplitGrid& s
   // byte offset of level 0 of each array: level L is at o + L * cplb
   const unsigned oRo = boff(base, sb.p[in_set][1]) - cplb, oBo = boff(base, sb.p[in_set][0]) - cplb;
   const unsigned oRn = boff(base, sb.p[out_set][1]) - cplb, oBn = boff(base, sb.p[out_set][0]) - cplb;
-  const unsigned orR = boff(base, sb.rhs[1]) - cplb, orB = boff(base, sb.rhs[0]) - cplb;
+  const unsigned orR = boff(base, raw ? sb.rhsr[1] : sb.rhs[1]) - cplb;
+  const unsigned orB = boff(base, raw ? sb.rhsr[0] : sb.rhs[0]) - cplb;
+  const unsigned ocR = boff(base, sb.rhs[1]) - cplb, ocB = boff(base, sb.rhs[0]) - cplb;
+  const auto rload = [&](unsigned vo, unsigned so) {
+    const float x = bload(S, vo, so);
+    return raw ? (float)((double)x - rav) : x;
+  };
   const int lb = remap ? xcd_contiguous(bid, nparts) : bid;
   const int bt = lb % ntt, bj = (lb / ntt) % ntj, bk = lb / (ntt * ntj);
   const int t = bt * FTXW - R + lane;
@@ -1174,10 +1306,10 @@ __device__ __forceinline__ void sor_pair_block(const Grid& g, const SplitGrid& s
     const unsigned lr = (unsigned)min(L, aHi) * cplb, lk = (unsigned)min(max(L - 1, 1), bHi) * cplb;
     sl.roA = bload(S, voA, oRo + lr);
     sl.roB = bload(S, voB, oRo + lr);
-    sl.rrA = bload(S, voA, orR + lr);
-    sl.rrB = bload(S, voB, orR + lr);
-    sl.rbA = bload(S, voA, orB + lk);
-    sl.rbB = bload(S, voB, orB + lk);
+    sl.rrA = rload(voA, orR + lr);
+    sl.rrB = rload(voB, orR + lr);
+    sl.rbA = rload(voA, orB + lk);
+    sl.rbB = rload(voB, orB + lk);
   };
   Sl sl[2];
   // rings, by level: the black iterate of l-1 at (L - krs + 1) mod 6, four
@@ -1233,10 +1365,10 @@ __device__ __forceinline__ void sor_pair_block(const Grid& g, const SplitGrid& s
       float hrA = 0.f, hrB = 0.f, hbA = 0.f, hbB = 0.f;
       if (two) {
         const unsigned lc = (unsigned)min(max(kc, 1), km) * cplb, ld = (unsigned)min(max(kd, 1), km) * cplb;
-        hrA = bload(S, voA, orR + lc);
-        hrB = bload(S, voB, orR + lc);
-        hbA = bload(S, voA, orB + ld);
-        hbB = bload(S, voB, orB + ld);
+        hrA = rload(voA, orR + lc);
+        hrB = rload(voB, orR + lc);
+        hbA = rload(voA, orB + ld);
+        hbB = rload(voB, orB + ld);
       }
       LB[((U + 1) % 3) * PL + ownA] = bqA[(U + 2) % 6];
       LB[((U + 1) % 3) * PL + ownA + W] = bqB[(U + 2) % 6];
@@ -1298,6 +1430,11 @@ __device__ __forceinline__ void sor_pair_block(const Grid& g, const SplitGrid& s
           acc1 = acc_sq(acc1, reltmp, okB[pb] && kin);
           if (!two) bstore(S, redB, svB[pb], kin ? oRn + (unsigned)kr * cplb : 0u);
         }
+        if (raw) {  // the red rhs of level kr, as the red iterate's stores
+          const unsigned so = kin ? ocR + (unsigned)kr * cplb : 0u;
+          bstore(S, q.rrA, svA[pa], so);
+          bstore(S, q.rrB, svB[pb], so);
+        }
         LR[P * PL + ownA] = redA;
         LR[P * PL + ownA + W] = redB;
         rdA[U % 3] = redA;
@@ -1351,6 +1488,11 @@ __device__ __forceinline__ void sor_pair_block(const Grid& g, const SplitGrid& s
           bB = own + reltmp;
           acc1 = acc_sq(acc1, reltmp, okB[pb] && kin);
           if (!two) bstore(S, bB, svB[pb], kin ? oBn + (unsigned)kb * cplb : 0u);
+        }
+        if (raw) {  // the black rhs of level kb
+          const unsigned so = kin ? ocB + (unsigned)kb * cplb : 0u;
+          bstore(S, q.rbA, svA[pa], so);
+          bstore(S, q.rbB, svB[pb], so);
         }
         LB2[P * PL + ownA] = bA;
         LB2[P * PL + ownA + W] = bB;
@@ -1476,12 +1618,12 @@ __device__ __forceinline__ void sor_pair_block(const Grid& g, const SplitGrid& s
   }
 }
 
-template <bool OM1, bool ENT>
+template <bool OM1, bool ENT, bool RAW>
 __global__ __launch_bounds__(PNT) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_sor_pair(
     Grid g, SplitGrid s, Metrics m, SplitBufs sb, Ctrl* ctrl, PairLaunch pl, int nparts, float omega, float pjuge,
     int ntt, int ntj, int kchunk, int remap, const float* __restrict__ pent) {
-  sor_pair_block<OM1, ENT>(g, s, m, sb, ctrl, pl, nparts, omega, pjuge, ntt, ntj, kchunk, remap, pent,
-                           (int)blockIdx.x);
+  sor_pair_block<OM1, ENT, RAW>(g, s, m, sb, ctrl, pl, nparts, omega, pjuge, ntt, ntj, kchunk, remap, pent,
+                                (int)blockIdx.x);
 }
 
 // After the last two-iteration launch: its stop test, unless an earlier one
@@ -1587,7 +1729,8 @@ static void launch_press_mean(const DevDomain& d, const SplitBufs& sb, int gx, i
   TIMED(d, KC_PAV, hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, d.stream, d.partials, nred, d.ctrl, 1,
                                       d.area));
   const int px = (g.im + 2 + BX - 1) / BX, py = (g.jm + 2 + BY - 1) / BY;
-  const int pz = g.km + 2;
+  // a few levels per thread (a copy: no value depends on the split)
+  const int pz = kblocks(px, py, g.km + 2, 8192);
   TIMED(d, KC_PADJ, hipLaunchKernelGGL(k_padj, dim3(px, py, pz),
                                        blk(), 0, d.stream, g, s, sb, d.F.p, (const Ctrl*)d.ctrl));
 }
@@ -1836,21 +1979,30 @@ static void launch_pair(const DevDomain& d, const SplitBufs& sb, const PairLaunc
                         const float* pent) {
   const size_t lvm_bytes = (size_t)(h.kc + 7) * sizeof(float4);
   const bool ent = (pl.iter == 1 || pl.fixup) && pent;  // may run iteration 1 on the entry ghosts
-  const int cls = ent ? KC_SOR1 : KC_SOR;
-#define PAIR(OM1, ENT)                                                                                          \
-  TIMED(d, cls, hipLaunchKernelGGL((k_sor_pair<OM1, ENT>), dim3(h.nparts), dim3(FX, PTY / 2), lvm_bytes, d.stream, \
-                                   d.g, d.sg, d.m, sb, d.ctrl, pl, h.nparts, d.omega, pjuge, h.ntt, h.ntj, h.kc,   \
-                                   d.xcd_remap, pent))
+  const bool raw = pl.raw && !pl.fixup;
+  const int cls = (pl.iter == 1 || pl.fixup) ? KC_SOR1 : KC_SOR;
+#define PAIR(OM1, ENT, RAW)                                                                                    \
+  TIMED(d, cls, hipLaunchKernelGGL((k_sor_pair<OM1, ENT, RAW>), dim3(h.nparts), dim3(FX, PTY / 2), lvm_bytes,    \
+                                   d.stream, d.g, d.sg, d.m, sb, d.ctrl, pl, h.nparts, d.omega, pjuge, h.ntt,     \
+                                   h.ntj, h.kc, d.xcd_remap, pent))
   if (d.omega == 1.0f) {
-    if (ent)
-      PAIR(true, true);
+    if (raw && ent)
+      PAIR(true, true, true);
+    else if (raw)
+      PAIR(true, false, true);
+    else if (ent)
+      PAIR(true, true, false);
     else
-      PAIR(true, false);
+      PAIR(true, false, false);
   } else {
-    if (ent)
-      PAIR(false, true);
+    if (raw && ent)
+      PAIR(false, true, true);
+    else if (raw)
+      PAIR(false, false, true);
+    else if (ent)
+      PAIR(false, true, false);
     else
-      PAIR(false, false);
+      PAIR(false, false, false);
   }
 #undef PAIR
 }
@@ -1858,9 +2010,9 @@ static void launch_pair(const DevDomain& d, const SplitBufs& sb, const PairLaunc
 // Iterations l0..l1 in launches of two (the last alone when their number is
 // odd), reading iterate set in0 first and rotating over the three sets;
 // returns the last launch.  pent: the entry values of p for a launch of
-// iteration 1.
+// iteration 1; raw: the first launch takes the rhs from k_rhs_head.
 static PairLaunch launch_sor_pairs(const DevDomain& d, const SplitBufs& sb, const SorShape& h, float pjuge, int l0,
-                                   int l1, int in0, const float* pent) {
+                                   int l1, int in0, const float* pent, bool raw = false) {
   PairLaunch prev{};
   int in = in0, q = 0;
   for (int l = l0; l <= l1; l += 2, q++) {
@@ -1877,7 +2029,17 @@ static PairLaunch launch_sor_pairs(const DevDomain& d, const SplitBufs& sb, cons
     pl.prev_nparts = h.nparts;
     pl.rpart = prev.wpart;
     pl.fixup = 0;
+    pl.raw = (raw && l == l0) ? 1 : 0;
+    if (l == 1 && pent) {
+      // iteration 1: the kernel without the entry-ghost code runs when every
+      // ghost of p equals its boundp copy (as after any press), the one with
+      // it otherwise; both fill the same partial-sum slots
+      pl.gate = 1;
+      launch_pair(d, sb, pl, h, pjuge, nullptr);
+      pl.gate = 2;
+    }
     launch_pair(d, sb, pl, h, pjuge, pent);
+    pl.gate = 0;
     prev = pl;
     in = pl.out_set;
   }
@@ -2182,6 +2344,26 @@ void launch_press(const DevDomain& d, float dt) {
   const int nred = gx * gy * gz;
   SplitBufs sb = d.sb;
   if (d.cn1_separable) sb.cn1[0] = sb.cn1[1] = nullptr;
+  if (d.press_mode == 0 && d.sor_fused && d.nmaxp >= 1 && sor_shape(d).rows == 4) {
+    // two iterations per launch: the head in one pass (k_rhs_head), its mean
+    // removed by the launch of iterations 1 and 2, which also reads the entry
+    // ghosts of p where they differ from their boundp copies
+    const SorShape h = sor_shape(d);
+    TIMED(d, KC_RHS, hipLaunchKernelGGL(k_rhs_head, dim3(gx, gy, gz), blk(), 0, d.stream, g, s, d.m,
+                                        (const float*)d.F.u, (const float*)d.F.v, (const float*)d.F.w, d.F.f,
+                                        d.F.g, d.F.h, (const float*)d.F.p, sb, d.partials, nred, d.ctrl, dt));
+    TIMED(d, KC_RHS, hipLaunchKernelGGL(k_finalize_head, dim3(1), dim3(256), 0, d.stream,
+                                        (const double*)d.partials, nred, d.ctrl, d.area));
+    const PairLaunch last = launch_sor_pairs(d, sb, h, d.pjuge, 1, d.nmaxp, 0, d.F.p, true);
+    TIMED(d, KC_PAV, hipLaunchKernelGGL(k_sor_pair_final, dim3(1), dim3(256), 0, d.stream, sb, d.ctrl, last,
+                                        h.nparts, d.pjuge));
+    PairLaunch fix{};
+    fix.fixup = 1;
+    fix.wpart = last.wpart ^ 1;
+    launch_pair(d, sb, fix, h, d.pjuge, d.F.p);
+    launch_press_mean(d, sb, gx, gy, gz, nred);
+    return;
+  }
   press_head(d, dt, sb, gx, gy, gz, nred);
   if (d.press_mode == 1) {
     launch_press_dc(d, sb, gx, gy, gz, nred);
@@ -2189,9 +2371,7 @@ void launch_press(const DevDomain& d, float dt) {
   }
   if (d.sor_fused) {
     const SorShape h = sor_shape(d);
-    if (h.rows == 4) {
-      // two iterations per launch; the launch of iteration 1 reads the entry
-      // ghosts of p where they differ from their boundp copies
+    if (h.rows == 4) {  // nmaxp < 1: no SOR launch to remove the mean of rhs
       const PairLaunch last = launch_sor_pairs(d, sb, h, d.pjuge, 1, d.nmaxp, 0, d.F.p);
       TIMED(d, KC_PAV, hipLaunchKernelGGL(k_sor_pair_final, dim3(1), dim3(256), 0, d.stream, sb, d.ctrl, last,
                                           h.nparts, d.pjuge));
@@ -2241,6 +2421,13 @@ void launch_press(const DevDomain& d, float dt) {
     }
   }
   launch_press_tail(d, sb, nparts, gx, gy, gz, nred);
+}
+
+void launch_rhs_gather(const DevDomain& d) {
+  const Grid& g = d.g;
+  const int gx = (g.im + BX - 1) / BX, gy = (g.jm + BY - 1) / BY;
+  hipLaunchKernelGGL(k_rhs_gather, dim3(gx, gy, kblocks(gx, gy, g.km, kMaxRedBlocks)), blk(), 0, d.stream, g, d.sg,
+                     d.sb, d.F.rhs);
 }
 
 // k levels per SOR block so that the sweep has about `target` blocks

@@ -31,11 +31,12 @@ struct SplitBufs {
   // sets 0 and 1 by iteration parity; two iterations per launch rotate over
   // all three (set 2 shares the storage of cn1, which is then recomputed).
   float* p[3][2];
-  float* rhs[2];        // [colour]
+  float* rhs[2];        // [colour] the rhs the SOR reads (its mean removed)
   float* cn1[2];        // [colour]; unused while cn1 is recomputed
+  float* rhsr[2];       // [colour] two iterations per launch: rhs before its mean is removed
   double* part[2][2];   // [iterate parity][colour] per-block sums of reltmp^2
-  float* base;          // start of the allocation: a scratch plane, the eight arrays, a scratch plane
-  int nplanes;          // planes in the allocation (8 km + 2)
+  float* base;          // start of the allocation: a scratch plane, the ten arrays, a scratch plane
+  int nplanes;          // planes in the allocation (10 km + 2)
 };
 
 // Kernel classes for the optional per-launch timing (les_domain_set_timing).
@@ -51,8 +52,8 @@ struct DevDomain {
   Metrics m;
   DevFields F;
   Ctrl* ctrl;
-  double* partials;
-  int npartials;
+  double* partials;  // per-block partials of the reductions outside the SOR: sums, then per-block flags
+  int npartials;     // slots before the SOR's partial slots (twice the largest reduction grid)
   hipStream_t stream;
   float wl;          // logf(0.5*dzn(1)/0.1), the wall-function log (vel2.f95:138, les.f95:133)
   double area;       // sum of dx1*dy1*dzn over the interior (press.f95:63-72)
@@ -221,6 +222,9 @@ void launch_masks(const DevDomain& d, const float* zbm, const float* top);
 void launch_les(const DevDomain& d);
 void launch_adam(const DevDomain& d);
 void launch_press(const DevDomain& d, float dt);
+// Copy the split rhs (its mean removed) into the reference-layout rhs, which
+// press with two SOR iterations per launch does not write.
+void launch_rhs_gather(const DevDomain& d);
 
 // The time statistics of aveflow (aveflow.f95:8-23), device layout.
 struct AveBufs {

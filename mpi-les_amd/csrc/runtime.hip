@@ -295,6 +295,10 @@ struct les_domain {
   const void* bound[NFIELDS] = {};  // host address each field mirrors (reference-signature layer)
   float ro = 1.1763f, vn = 0.0f, alpha = -10.0f, beta = -1.0f, dt = 0.05f;
   long long bytes = 0;
+  // the reference-layout rhs is behind the split rhs (a press of mode 0 ran
+  // since it was last written); a download gathers it first
+  bool rhs_lazy = false;
+  double st_us_best = 0.0, st_us_default = 0.0;  // the stencil tuner's timings for this shape
 };
 
 namespace {
@@ -399,6 +403,7 @@ int ensure_field(les_domain* D, int f) {
 int upload_field(les_domain* D, int f, const float* host) {
   if (ensure_field(D, f)) return 1;
   if (f == FCN1) check_cn1(D, host);
+  if (f == FRHS) D->rhs_lazy = false;
   const Kind k = kFields[f].kind;
   int ni, nj, nk;
   ref_extent(D, k, &ni, &nj, &nk);
@@ -414,6 +419,11 @@ int upload_field(les_domain* D, int f, const float* host) {
 
 int download_field(les_domain* D, int f, float* host) {
   if (ensure_field(D, f)) return 1;
+  if (f == FRHS && D->rhs_lazy) {
+    launch_rhs_gather(D->d);
+    HIP_TRY(hipGetLastError());
+    D->rhs_lazy = false;
+  }
   const Kind k = kFields[f].kind;
   int ni, nj, nk;
   ref_extent(D, k, &ni, &nj, &nk);
@@ -580,8 +590,8 @@ int domain_alloc(int im, int jm, int km, les_domain** out) {
   // allocated on its first use (ensure_cov), the whole-step path never
   // stores them
   {
-    // colour-split SOR arrays: 4 iterate buffers, 2 rhs, 2 cn1, after one
-    // scratch plane and followed by another.  Rows are padded to at least
+    // colour-split SOR arrays: 4 iterate buffers, 2 rhs, 2 cn1, 2 rhs before
+    // the mean removal, after one scratch plane and followed by another.  Rows are padded to at least
     // one pair beyond the last (cp > nt): the SOR kernel sends the stores of
     // lanes that own no cell there, and the planes at either end take the
     // loads and stores of levels outside 1..km.  Nothing reads either.
@@ -590,7 +600,7 @@ int domain_alloc(int im, int jm, int km, les_domain** out) {
     sg.cp = (sg.nt + 1 + 31) / 32 * 32;
     sg.cplane = sg.cp * jm;
     const size_t cs = (size_t)sg.cplane * km;
-    const size_t nplanes = 8 * (size_t)km + 2;
+    const size_t nplanes = 10 * (size_t)km + 2;
     const size_t nb = nplanes * sg.cplane * sizeof(float);
     e = dev_alloc(&D->split, nb);
     if (e == hipSuccess) e = hipMemsetAsync(D->split, 0, nb, D->d.stream);
@@ -603,6 +613,7 @@ int domain_alloc(int im, int jm, int km, les_domain** out) {
       sb.rhs[c] = arr0 + (4 + c) * cs;
       sb.cn1[c] = arr0 + (6 + c) * cs;
       sb.p[2][c] = sb.cn1[c];  // the third iterate set of two-iteration launches (separable cn1 only)
+      sb.rhsr[c] = arr0 + (8 + c) * cs;
     }
     sb.base = D->split;
     sb.nplanes = (int)nplanes;
@@ -635,7 +646,7 @@ int domain_alloc(int im, int jm, int km, les_domain** out) {
   m.cncz = D->mbuf + D->moff[MCNCZ];
   m.inflow = nullptr;
   // reduction scratch: reduction partials, then the four SOR partial slots
-  D->d.npartials = reduction_parts_needed(D->d.g);
+  D->d.npartials = 2 * reduction_parts_needed(D->d.g);
   D->d.sor_blocks = env_int("LES_SOR_BLOCKS", 2048);
   D->d.xcd_remap = env_int("LES_XCD_REMAP", 1);
   D->d.sor_fused = env_int("LES_SOR_FUSED", 1);
@@ -655,6 +666,9 @@ int domain_alloc(int im, int jm, int km, les_domain** out) {
   D->d.sor_tile_rows = env_int("LES_SOR_TILE_ROWS", 16) == 32 ? 32 : 16;
   D->d.sor_tile_width = (D->d.sor_tile_rows == 32 && env_int("LES_SOR_TILE_WIDTH", 64) == 32) ? 32 : 64;
   D->d.sor_entry_fast = env_int("LES_SOR_ENTRY_FAST", 1) != 0;
+  // the fused SOR kernels address the split arrays through one buffer
+  // descriptor with 32-bit byte offsets; beyond 4 GiB the colour sweeps run
+  if ((size_t)D->d.sb.nplanes * D->d.sg.cplane * sizeof(float) > 0xffffffffull) D->d.sor_fused = 0;
   D->d.dc_omega = 1.0f;
   {
     int dev = 0, ncu = 0;
@@ -796,7 +810,13 @@ int run_stage(les_domain* D, int s, int n, int nmax) {
     case 3: launch_feedbf(D->d, D->alpha, D->dt, D->beta); break;
     case 4: launch_les(D->d); break;
     case 5: launch_adam(D->d); break;
-    case 6: launch_press(D->d, D->dt); break;
+    case 6:
+      launch_press(D->d, D->dt);
+      // press mode 0 leaves the split rhs equal to rhs on the interior (two
+      // iterations per launch write only the split rhs); mode 1 overwrites
+      // the split rhs with its defects and writes rhs itself
+      D->rhs_lazy = D->d.press_mode == 0;
+      break;
     default: return fail("unknown stage");
   }
   HIP_TRY(hipGetLastError());
@@ -839,6 +859,11 @@ struct TuneKey {
 struct TuneResult {
   int rows = 2, tile_rows = 16, tile_width = 64, kchunk = 0;
   double us_best = 0.0, us_default = 0.0;
+  // k levels per thread of k_velfg_feedbf, k_les_sm and k_les_visc<true> (0:
+  // automatic) and the microseconds of one velfg+feedbf and one les+adam
+  // launch sequence with them and with the automatic ones
+  int velfg_kc = 0, sm_kc = 0, visc_kc = 0;
+  double st_us_best = 0.0, st_us_default = 0.0;
 };
 std::mutex g_tune_mu;
 std::map<TuneKey, TuneResult> g_tuned;
@@ -945,6 +970,122 @@ int tune_sor(les_domain* D, int reps, TuneResult* out) {
   if (rc) return rc;
   r.us_best = best;
   *out = r;
+  return 0;
+}
+
+// The stencil kernels' k levels per thread -- k_velfg_feedbf, k_les_sm and
+// k_les_visc<true> -- change only the speed: a thread computes each of its
+// cells with the same expressions whatever its run, and no partial sum
+// depends on it.  Each candidate is timed (`reps` launches between one pair
+// of events, after one untimed launch) on the domain's own fields, which
+// are restored from a copy before each candidate and at the end: every
+// field those kernels write (velfg+feedbf: diu1..9, f, g, h, usum, vsum,
+// wsum, uspd, vspd; les+adam: sm, f, g, h, fold, gold, hold).  Velfg+feedbf
+// is timed alone, les_sm and les_visc by the les+adam sequence (with
+// boundsm), one chunk varied at a time.
+int tune_stencils(les_domain* D, int reps, TuneResult* r) {
+  const Grid& g = D->d.g;
+  if (!D->d.fuse_velfg || !D->d.fuse_les) return 0;  // the unfused stages keep their shapes
+  if (flush_metrics(D)) return 1;
+  static const int kSaved[] = {FF,    FG,    FH,    FUSUM, FVSUM, FWSUM, FFOLD, FGOLD, FHOLD, FSM,  FUSPD,
+                               FVSPD, FDIU1, FDIU2, FDIU3, FDIU4, FDIU5, FDIU6, FDIU7, FDIU8, FDIU9};
+  constexpr int kNs = (int)(sizeof(kSaved) / sizeof(kSaved[0]));
+  size_t off[kNs + 1];
+  off[0] = 0;
+  for (int q = 0; q < kNs; q++)
+    off[q + 1] = off[q] + (kFields[kSaved[q]].kind == K2D ? (size_t)D->d.g.sk : (size_t)D->n3);
+  float* save = nullptr;
+  hipError_t e = dev_alloc(&save, off[kNs] * sizeof(float));
+  if (e != hipSuccess) return fail(std::string("autotune: field copy allocation: ") + hipGetErrorString(e));
+  auto copy_all = [&](bool back) {
+    for (int q = 0; q < kNs; q++) {
+      float* f = D->fields[kSaved[q]];
+      const size_t nb = (off[q + 1] - off[q]) * sizeof(float);
+      const hipError_t ce = back ? hipMemcpyAsync(f, save + off[q], nb, hipMemcpyDeviceToDevice, D->d.stream)
+                                 : hipMemcpyAsync(save + off[q], f, nb, hipMemcpyDeviceToDevice, D->d.stream);
+      if (ce != hipSuccess) return ce;
+    }
+    return hipSuccess;
+  };
+  int rc = 0;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if ((e = copy_all(false)) != hipSuccess) rc = fail(std::string("autotune: field copy: ") + hipGetErrorString(e));
+  if (!rc && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)) rc = fail("hipEventCreate");
+  // candidates: the automatic chunk (0) and every distinct chunk whose
+  // grid has between one and 64 blocks per compute unit
+  const int gx = (g.im + 63) / 64, gy = (g.jm + 3) / 4;
+  const long ncu = D->d.ncu > 0 ? D->d.ncu : 256;
+  std::vector<int> kcs{0};
+  for (int nz = 1; nz <= g.km; nz++) {
+    const int kc = (g.km + nz - 1) / nz;
+    const long blocks = (long)gx * gy * ((g.km + kc - 1) / kc);
+    if (blocks < ncu || blocks > 64 * ncu) continue;
+    if (std::find(kcs.begin(), kcs.end(), kc) == kcs.end()) kcs.push_back(kc);
+  }
+  // which: 0 velfg+feedbf, 1 les+adam
+  auto time_one = [&](int which, int vkc, int skc, int ckc, double* us) -> int {
+    DevDomain d = D->d;
+    d.timer = nullptr;
+    d.velfg_kchunk = vkc;
+    d.sm_kchunk = skc;
+    d.visc_kchunk = ckc;
+    hipError_t te = copy_all(true);
+    for (int q = -1; q < reps && te == hipSuccess; q++) {
+      if (q == 0) te = hipEventRecord(e0, d.stream);
+      if (which == 0)
+        launch_velfg_feedbf(d, D->vn, D->alpha, D->dt, D->beta);
+      else
+        launch_les_adam(d);
+      if (te == hipSuccess) te = hipGetLastError();
+    }
+    if (te == hipSuccess) te = hipEventRecord(e1, d.stream);
+    if (te == hipSuccess) te = hipEventSynchronize(e1);
+    float ms = 0.0f;
+    if (te == hipSuccess) te = hipEventElapsedTime(&ms, e0, e1);
+    if (te != hipSuccess) return fail(std::string("autotune: stencil timing: ") + hipGetErrorString(te));
+    *us = 1e3 * (double)ms / reps;
+    return 0;
+  };
+  double vdef = 0.0, vbest = -1.0, ldef = 0.0, lbest = -1.0;
+  int vkc = 0, skc = 0, ckc = 0;
+  for (int kc : kcs) {
+    double us = 0.0;
+    if (rc || (rc = time_one(0, kc, 0, 0, &us))) break;
+    if (kc == 0) vdef = us;
+    if (vbest < 0.0 || us < vbest) {
+      vbest = us;
+      vkc = kc;
+    }
+  }
+  for (int kc : kcs) {  // les_sm's chunk, les_visc's automatic
+    double us = 0.0;
+    if (rc || (rc = time_one(1, vkc, kc, 0, &us))) break;
+    if (kc == 0) ldef = us;
+    if (lbest < 0.0 || us < lbest) {
+      lbest = us;
+      skc = kc;
+    }
+  }
+  for (int kc : kcs) {  // then les_visc's
+    double us = 0.0;
+    if (kc == 0) continue;  // timed above with les_sm's best
+    if (rc || (rc = time_one(1, vkc, skc, kc, &us))) break;
+    if (us < lbest) {
+      lbest = us;
+      ckc = kc;
+    }
+  }
+  if (e0) hipEventDestroy(e0);
+  if (e1) hipEventDestroy(e1);
+  if ((e = copy_all(true)) != hipSuccess && !rc) rc = fail(std::string("autotune: field restore: ") + hipGetErrorString(e));
+  if (hipStreamSynchronize(D->d.stream) != hipSuccess && !rc) rc = fail("autotune: synchronize");
+  dev_release(save);
+  if (rc) return rc;
+  r->velfg_kc = vkc;
+  r->sm_kc = skc;
+  r->visc_kc = ckc;
+  r->st_us_best = vbest + lbest;
+  r->st_us_default = vdef + ldef;
   return 0;
 }
 
@@ -1437,7 +1578,7 @@ long long les_domain_bytes_needed(int im, int jm, int km) {
   const long long nt = (im + 1) / 2, cp = (nt + 1 + 31) / 32 * 32, cs = cp * jm * km;
   long long b = 0;
   for (int f = 0; f < kLazyFirst; f++) b += (kFields[f].kind == K2D ? sk : n3) * 4;
-  b += (8 * cs + 2 * cp * jm) * 4;   // colour-split SOR arrays and their two scratch planes
+  b += (10 * cs + 2 * cp * jm) * 4;  // colour-split SOR arrays and their two scratch planes
   return b + (1LL << 20);  // metrics, partial sums, control block (bounded by 1 MiB here)
 }
 
@@ -1491,6 +1632,7 @@ static int step_graph(les_domain* D, int n, int nmax) {
     memcpy(D->gscal, sc, sizeof(sc));
   }
   HIP_TRY(hipGraphLaunch(D->gexec, D->d.stream));
+  D->rhs_lazy = D->d.press_mode == 0;
   return 0;
 }
 
@@ -1520,6 +1662,7 @@ int les_domain_autotune(les_domain* D, int reps, int* rows, int* tile_rows, int*
   if (it == g_tuned.end() || reps > 0) {
     TuneResult t;
     if (tune_sor(D, reps > 0 ? reps : 10, &t)) return 1;
+    if (tune_stencils(D, reps > 0 ? reps : 10, &t)) return 1;
     it = g_tuned.insert_or_assign(key, t).first;
   }
   const TuneResult& r = it->second;
@@ -1527,12 +1670,28 @@ int les_domain_autotune(les_domain* D, int reps, int* rows, int* tile_rows, int*
   D->d.sor_tile_rows = r.tile_rows;
   D->d.sor_tile_width = r.tile_width;
   D->d.sor_kchunk = r.kchunk;
+  D->d.velfg_kchunk = r.velfg_kc;
+  D->d.sm_kchunk = r.sm_kc;
+  D->d.visc_kchunk = r.visc_kc;
+  D->st_us_best = r.st_us_best;
+  D->st_us_default = r.st_us_default;
   if (rows) *rows = r.rows;
   if (tile_rows) *tile_rows = r.rows == 2 ? r.tile_rows : (r.rows == 4 ? 32 : 16);
   if (tile_width) *tile_width = r.rows == 2 ? r.tile_width : 64;
   if (kchunk) *kchunk = r.kchunk;
   if (us_best) *us_best = r.us_best;
   if (us_default) *us_default = r.us_default;
+  return 0;
+}
+
+int les_domain_stencil_chunks(les_domain* D, int* velfg_kchunk, int* sm_kchunk, int* visc_kchunk, double* us_best,
+                              double* us_default) {
+  if (!live_has(D)) return fail("null, unknown or destroyed domain");
+  if (velfg_kchunk) *velfg_kchunk = D->d.velfg_kchunk;
+  if (sm_kchunk) *sm_kchunk = D->d.sm_kchunk;
+  if (visc_kchunk) *visc_kchunk = D->d.visc_kchunk;
+  if (us_best) *us_best = D->st_us_best;
+  if (us_default) *us_default = D->st_us_default;
   return 0;
 }
 
@@ -1829,6 +1988,7 @@ int batch_enqueue(les_batch* B, const int* ns, const int* nmaxs, bool capturing)
   std::vector<char> split(nd, 0);
   for (size_t q = 0; q < nd; q++) {
     les_domain* D = B->doms[q];
+    D->rhs_lazy = D->d.press_mode == 0;  // a batched press also leaves the split rhs equal to rhs
     if (staged && B->full[q]) {
       if (flush_metrics(D)) return 1;
       if (!capturing) {  // after the work already on the domain's stream
@@ -1934,8 +2094,10 @@ int les_batch_step(les_batch* B, const int* ns, const int* nmaxs) {
     }
     if (batch_prepare(B)) return 1;
   }
-  for (les_domain* D : B->doms)
+  for (les_domain* D : B->doms) {
     if (flush_metrics(D)) return 1;  // synchronous: not under capture
+    D->rhs_lazy = D->d.press_mode == 0;
+  }
   bool replay = B->use_graph != 0;
   for (size_t q = 0; q < B->doms.size(); q++) replay = replay && ns[q] != 1;
   if (!replay) return batch_enqueue(B, ns, nmaxs, false);

@@ -104,6 +104,7 @@ def lib():
     L.les_domain_set_graph.argtypes = [vp, ci]
     L.les_domain_autotune.argtypes = [vp, ci, vp, vp, vp, vp, vp, vp]
     L.les_domain_time_sor.argtypes = [vp, ci, vp]
+    L.les_domain_stencil_chunks.argtypes = [vp, vp, vp, vp, vp, vp]
     L.les_domain_timing.argtypes = [vp, vp, vp]
     L.les_batch_create.argtypes = [vp, ci, ctypes.POINTER(vp)]
     L.les_batch_destroy.argtypes = [vp]
@@ -301,14 +302,28 @@ class Domain:
         return self.press_stats()["nonfinite"] != 0
 
     def autotune(self, reps: int = 0) -> dict:
-        """Pick the SOR launch shape for this grid shape (cached per shape;
-        reps > 0 times it again with that many iterations per candidate)."""
+        """Pick the SOR launch shape and the stencil kernels' k levels per
+        thread for this grid shape (cached per shape; reps > 0 times them
+        again with that many launches per candidate)."""
         r, ty, tw, kc = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
         ub, ud = ctypes.c_double(), ctypes.c_double()
         _check(lib().les_domain_autotune(self._h, reps, ctypes.byref(r), ctypes.byref(ty), ctypes.byref(tw),
                                          ctypes.byref(kc), ctypes.byref(ub), ctypes.byref(ud)))
-        return dict(rows=r.value, tile_rows=ty.value, tile_width=tw.value, kchunk=kc.value, us_per_iter=ub.value,
-                    us_per_iter_default=ud.value)
+        out = dict(rows=r.value, tile_rows=ty.value, tile_width=tw.value, kchunk=kc.value, us_per_iter=ub.value,
+                   us_per_iter_default=ud.value)
+        out.update(self.stencil_chunks())
+        return out
+
+    def stencil_chunks(self) -> dict:
+        """k levels per thread of velfg+feedbf, les_sm and les viscous+adam
+        (0: automatic) and the tuner's timings of one velfg+feedbf and one
+        les+adam launch sequence, tuned and automatic (us)."""
+        v, sm, vc = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        ub, ud = ctypes.c_double(), ctypes.c_double()
+        _check(lib().les_domain_stencil_chunks(self._h, ctypes.byref(v), ctypes.byref(sm), ctypes.byref(vc),
+                                               ctypes.byref(ub), ctypes.byref(ud)))
+        return dict(velfg_kchunk=v.value, sm_kchunk=sm.value, visc_kchunk=vc.value, stencil_us=ub.value,
+                    stencil_us_default=ud.value)
 
     def time_sor(self, iters: int = 200) -> float:
         """Microseconds per fused SOR iteration of the current launch shape:

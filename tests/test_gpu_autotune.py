@@ -1,5 +1,6 @@
-"""The per-shape autotuner of the fused SOR iteration's launch shape
-(les_domain_autotune): it picks a valid shape, leaves the domain's state
+"""The per-shape autotuner of the fused SOR iteration's launch shape and
+of the stencil kernels' k levels per thread (les_domain_autotune): it
+picks a valid shape, leaves the domain's state
 and last press statistics untouched, is remembered per grid shape, and the
 tuned domain steps bit for bit like an untuned one (every tiling gives the
 colour sweeps' iterates) with the same SOR iteration counts.  The residual
@@ -12,7 +13,8 @@ import pytest
 from lesamd import api, setup_gen
 
 pytestmark = pytest.mark.gpu
-FIELDS = ("u", "v", "w", "p", "rhs", "sm", "f", "g", "h")
+FIELDS = ("u", "v", "w", "p", "rhs", "sm", "f", "g", "h", "usum", "vsum", "wsum", "fold", "gold", "hold", "uspd",
+          "vspd") + tuple(f"diu{q}" for q in range(1, 10))
 
 
 @pytest.mark.parametrize("case", [dict(im=90, jm=40, km=24, seed=31), dict(im=61, jm=27, km=18, seed=32)],
@@ -35,6 +37,11 @@ def test_autotune_keeps_state_and_results(case):
     assert 1 <= r["kchunk"] <= w.km, r
     assert r["us_per_iter"] > 0.0 and r["us_per_iter_default"] > 0.0, r
     assert r["us_per_iter"] <= r["us_per_iter_default"], r
+    # the stencil kernels' k levels per thread (0: automatic), timed on the
+    # domain's fields and restored (every field they write is checked below)
+    for key in ("velfg_kchunk", "sm_kchunk", "visc_kchunk"):
+        assert 0 <= r[key] <= w.km, r
+    assert 0.0 < r["stencil_us"] <= r["stencil_us_default"], r
     for f in FIELDS:
         assert np.array_equal(a.download(f), before[f]), f
     st_after = a.press_stats()

@@ -151,3 +151,40 @@ def test_pair_graph_replay_and_autotune():
             assert np.array_equal(a, tuned.download(f)), (n, f)
     for d in (eager, graph, tuned):
         d.close()
+
+
+def test_pair_head_rhs_downloads():
+    """The one-pass head leaves the reference-layout rhs to a gather on
+    download: rhs equals the one-iteration path's after eager steps, after
+    graph replays with downloads in between, after an upload (which the
+    download returns as uploaded), and after a press of mode 1, whose head
+    writes rhs itself."""
+    w = setup_gen.Workload("sor-pair-head", 70, 36, 22, seed=45)
+    s = setup_gen.build(w, dense=False)
+    print("\n[sor pair head] rhs downloads 70x36x22", flush=True)
+    ref = _domain(s, LES_SOR_ROWS=2)
+    pair = _domain(s, LES_SOR_ROWS=4)
+    for n in (1, 2):
+        ref.step(n, 8)
+        pair.step(n, 8)
+    _compare(ref, pair, 2)
+    for d in (ref, pair):
+        d.set_graph(True)
+    for n in (3, 4, 5):
+        ref.step(n, 8)
+        pair.step(n, 8)
+        if n != 4:
+            _compare(ref, pair, n)
+    _compare(ref, pair, 5)
+    r = np.random.default_rng(3).uniform(-1, 1, ref.download("rhs").shape).astype(np.float32)
+    for d in (ref, pair):  # both: press leaves the ghosts of rhs as they are
+        d.upload("rhs", r)
+    assert np.array_equal(pair.download("rhs"), r)
+    for d in (ref, pair):
+        d.set_press_mode(1, outer=2, inner=5)
+    ref.step(6, 8)
+    pair.step(6, 8)
+    for f in FIELDS:
+        assert np.array_equal(ref.download(f), pair.download(f)), f
+    ref.close()
+    pair.close()
