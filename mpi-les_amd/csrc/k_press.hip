@@ -110,7 +110,7 @@ __global__ __launch_bounds__(NT) void k_rhs(Grid g, Metrics m, const float* __re
                                             float* __restrict__ f, float* __restrict__ gg,
                                             float* __restrict__ h, float* __restrict__ rhs,
                                             double* __restrict__ part, Ctrl* ctrl, float dt) {
-  rhs_body(hw_blk<false>(), g, m, u, v, w, f, gg, h, rhs, part, ctrl, dt);
+  rhs_body(hw_blk<true>(), g, m, u, v, w, f, gg, h, rhs, part, ctrl, dt);
 }
 
 // One block: total of n partials in a fixed order; what=0 -> rhsav,
@@ -183,9 +183,8 @@ __global__ __launch_bounds__(NT) void k_rhs_split(Grid g, SplitGrid s, float* __
 // The head of press with two SOR iterations per launch, in one pass: the
 // rhs of k_rhs (the same expression, grid, block partials and ghost stores
 // of f, g, h), stored into the colour-split rhsr before its mean is removed
-// (the launch of SOR iterations 1 and 2 subtracts the mean and stores the
-// split rhs), the split copy of p (iterate set 0), and the ghost comparison
-// of k_rhs_split.  Each block's comparison result goes to part[nred + block]
+// (k_rhs_sub then writes the split rhs), the split copy of p (iterate set
+// 0), and the ghost comparison of k_rhs_split.  Each block's comparison result goes to part[nred + block]
 // for k_finalize_head, so that no block clears a flag another one may have
 // raised.  The reference-layout rhs is not written here (a download gathers
 // it from the split rhs).
@@ -195,7 +194,7 @@ __global__ __launch_bounds__(NT) void k_rhs_head(Grid g, SplitGrid s, Metrics m,
                                                  float* __restrict__ h, const float* __restrict__ p, SplitBufs sb,
                                                  double* __restrict__ part, int nred, Ctrl* ctrl, float dt) {
   __shared__ double lds[NT / kWave];
-  const VBlk vb = hw_blk<false>();
+  const VBlk vb = hw_blk<true>();
   const int tid = threadIdx.y * BX + threadIdx.x;
   const int bid = vlinear(vb);
   if (bid == 0 && tid == 0) {
@@ -217,30 +216,54 @@ __global__ __launch_bounds__(NT) void k_rhs_head(Grid g, SplitGrid s, Metrics m,
     kspan(vb, 1, g.km, k0, k1);
     // w and h of level k-1 come from the thread's previous level, u and f
     // of cell i-1 from the neighbouring lane: the same values as the loads
-    // of k_rhs (nothing here writes them before they are read)
+    // of k_rhs.  The operands of level k+1 are loaded before level k is
+    // computed; nothing the loop writes (ghosts of f, g and h) is read by a
+    // later load.
+    struct Lv {
+      float f, g, u, w, h, v, vm, gm, p, fl, ul;
+    };
+    const auto load = [&](int k) {
+      const int c = ix(g, i, j, k);
+      Lv a;
+      a.f = f[c];
+      a.g = gg[c];
+      a.u = u[c];
+      a.w = w[c];
+      a.h = h[c];
+      a.v = v[c];
+      a.vm = v[c - g.sj];
+      a.gm = (j == 1) ? gg[ix(g, i, g.jm, k)] : gg[c - g.sj];
+      a.p = p[c];
+      a.fl = lane0 ? f[c - 1] : 0.0f;  // the block's first lane: cell i-1 belongs to the previous block
+      a.ul = lane0 ? u[c - 1] : 0.0f;
+      return a;
+    };
     float wkm = 0.0f, hkm = 0.0f;
+    Lv cur{};
     if (k0 <= k1) {
       wkm = w[ix(g, i, j, k0 - 1)];
       hkm = h[ix(g, i, j, k0 - 1)];
+      cur = load(k0);
     }
     for (int k = k0; k <= k1; k++) {
+      const Lv nxt = (k < k1) ? load(k + 1) : cur;
       const int c = ix(g, i, j, k);
       const float dzk = M1(m.dzn, k);
-      const float fc = f[c], gc = gg[c], uc = u[c], wc = w[c], hk = h[c];
+      const float fc = cur.f, gc = cur.g, uc = cur.u, wc = cur.w, hk = cur.h;
       float fl = __shfl_up(fc, 1, kWave), ul = __shfl_up(uc, 1, kWave);
-      if (lane0) {  // the block's first lane: cell i-1 belongs to the previous block
-        fl = f[c - 1];
-        ul = u[c - 1];
+      if (lane0) {
+        fl = cur.fl;
+        ul = cur.ul;
       }
       const float fm = (i == 1) ? fc : fl;
-      const float gm = (j == 1) ? gg[ix(g, i, g.jm, k)] : gg[c - g.sj];
+      const float gm = cur.gm;
       const float hc = (k == g.km) ? 0.0f : hk;
       const float hm = (k == 1) ? 0.0f : hkm;
       if (i == 1) f[c - 1] = fc;
       if (j == 1) gg[c - g.sj] = gm;
       if (k == 1) h[c - g.sk] = 0.0f;
       if (k == g.km) h[c] = 0.0f;
-      const float r = (-ul + uc) / dxi + (-v[c - g.sj] + v[c]) / dyj + (-wkm + wc) / dzk;
+      const float r = (-ul + uc) / dxi + (-cur.vm + cur.v) / dyj + (-wkm + wc) / dzk;
       const float rr = (fc - fm) / dxi + (gc - gm) / dyj + (hc - hm) / dzk + r / dt;
       wkm = wc;
       hkm = hk;
@@ -248,8 +271,9 @@ __global__ __launch_bounds__(NT) void k_rhs_head(Grid g, SplitGrid s, Metrics m,
       const int col = (i + j + k) & 1;  // 1 = red
       const int q = cix(s, t, j, k);
       sb.rhsr[col][q] = rr;
-      const float pc = p[c];
+      const float pc = cur.p;
       sb.p[0][col][q] = pc;
+      cur = nxt;
       if (i == 1) diff |= !same(p[c - 1], pc);
       if (i == g.im) diff |= !same(p[c + 1], pc);
       if (j == 1) diff |= !same(p[ix(g, i, 0, k)], p[ix(g, i, g.jm, k)]);
@@ -281,6 +305,24 @@ __global__ __launch_bounds__(256) void k_finalize_head(const double* __restrict_
   if (threadIdx.x == 0) {
     ctrl->rhsav = s / denom;
     ctrl->p_ghost_diff = fl != 0.0 ? 1 : 0;
+  }
+}
+
+// rhs = rhsr - rhsav over both colour-split arrays (press.f95:81-87, the
+// subtraction in FP64 as in k_rhs_split), which lie one after the other:
+// n4 groups of four floats (the row padding included, which nothing reads).
+constexpr int kRhsSubBlocks = 2048;
+__global__ __launch_bounds__(256) void k_rhs_sub(const float4* __restrict__ in, float4* __restrict__ out, long n4,
+                                                 const Ctrl* ctrl) {
+  const double av = ctrl->rhsav;
+  for (long q = blockIdx.x * 256L + threadIdx.x; q < n4; q += (long)gridDim.x * 256L) {
+    const float4 x = in[q];
+    float4 y;
+    y.x = (float)((double)x.x - av);
+    y.y = (float)((double)x.y - av);
+    y.z = (float)((double)x.z - av);
+    y.w = (float)((double)x.w - av);
+    out[q] = y;
   }
 }
 
@@ -1134,7 +1176,6 @@ struct PairLaunch {
   int prev_iter;    // the previous launch (0: none), whose stop test block 0 runs ...
   int prev_two, prev_in, prev_out, prev_nparts, rpart;  // ... its shape, sets, slots
   int fixup;        // the fix-up launch: iteration and sets from ctrl->fix_*, one iteration
-  int raw;          // the launch of iteration 1 after k_rhs_head: rhs from rhsr less the mean, stored into rhs
   int gate;         // 0: runs; 1: runs only when no ghost of p differs from its boundp copy; 2: only when one does
 };
 
@@ -1166,9 +1207,8 @@ __device__ __forceinline__ void pair_stop(Ctrl* ctrl, int L, int two, double s1,
 
 constexpr int PTY = 32, PR = 4, PNT = FX * PTY / 2, PFTY = PTY - 2 * PR, PFTX = FX - 2 * PR;
 
-// ENT: the launch may hold iteration 1 (the entry-ghost code is compiled in);
-// RAW: the launch of iteration 1 after k_rhs_head.
-template <bool OM1, bool ENT, bool RAW>
+// ENT: the launch may hold iteration 1 (the entry-ghost code is compiled in).
+template <bool OM1, bool ENT>
 __device__ __forceinline__ void sor_pair_block(const Grid& g, const SplitGrid& s, const Metrics& m,
                                                const SplitBufs& sb, Ctrl* ctrl, const PairLaunch& pl, int nparts,
                                                float omega, float pjuge, int ntt, int ntj, int kchunk, int remap,
@@ -1204,12 +1244,6 @@ __device__ __forceinline__ void sor_pair_block(const Grid& g, const SplitGrid& s
   const int in_set = sh[2], out_set = sh[3];
   const bool two = pl.two != 0 && !pl.fixup;
   const bool ent = ENT && sh[4] != 0;  // iteration 1 on the entry ghosts of p
-  // after k_rhs_head, the launch of iteration 1 reads the rhs before its
-  // mean is removed, subtracts the mean as k_rhs_split does (in FP64) on
-  // every value it loads, and stores each owned cell's result once into the
-  // split rhs that later launches read
-  constexpr bool raw = RAW;  // never a fix-up launch
-  const double rav = raw ? ctrl->rhsav : 0.0;
   if (!pl.fixup && pl.prev_iter > 0 && bid == 0) {  // the previous launch's stop test
     double s1 = 0.0, s2 = 0.0;
     for (int b = tid; b < pl.prev_nparts; b += GNT) {
@@ -1227,13 +1261,7 @@ __device__ __forceinline__ void sor_pair_block(const Grid& g, const SplitGrid& s
   // byte offset of level 0 of each array: level L is at o + L * cplb
   const unsigned oRo = boff(base, sb.p[in_set][1]) - cplb, oBo = boff(base, sb.p[in_set][0]) - cplb;
   const unsigned oRn = boff(base, sb.p[out_set][1]) - cplb, oBn = boff(base, sb.p[out_set][0]) - cplb;
-  const unsigned orR = boff(base, raw ? sb.rhsr[1] : sb.rhs[1]) - cplb;
-  const unsigned orB = boff(base, raw ? sb.rhsr[0] : sb.rhs[0]) - cplb;
-  const unsigned ocR = boff(base, sb.rhs[1]) - cplb, ocB = boff(base, sb.rhs[0]) - cplb;
-  const auto rload = [&](unsigned vo, unsigned so) {
-    const float x = bload(S, vo, so);
-    return raw ? (float)((double)x - rav) : x;
-  };
+  const unsigned orR = boff(base, sb.rhs[1]) - cplb, orB = boff(base, sb.rhs[0]) - cplb;
   const int lb = remap ? xcd_contiguous(bid, nparts) : bid;
   const int bt = lb % ntt, bj = (lb / ntt) % ntj, bk = lb / (ntt * ntj);
   const int t = bt * FTXW - R + lane;
@@ -1306,10 +1334,10 @@ __device__ __forceinline__ void sor_pair_block(const Grid& g, const SplitGrid& s
     const unsigned lr = (unsigned)min(L, aHi) * cplb, lk = (unsigned)min(max(L - 1, 1), bHi) * cplb;
     sl.roA = bload(S, voA, oRo + lr);
     sl.roB = bload(S, voB, oRo + lr);
-    sl.rrA = rload(voA, orR + lr);
-    sl.rrB = rload(voB, orR + lr);
-    sl.rbA = rload(voA, orB + lk);
-    sl.rbB = rload(voB, orB + lk);
+    sl.rrA = bload(S, voA, orR + lr);
+    sl.rrB = bload(S, voB, orR + lr);
+    sl.rbA = bload(S, voA, orB + lk);
+    sl.rbB = bload(S, voB, orB + lk);
   };
   Sl sl[2];
   // rings, by level: the black iterate of l-1 at (L - krs + 1) mod 6, four
@@ -1365,10 +1393,10 @@ __device__ __forceinline__ void sor_pair_block(const Grid& g, const SplitGrid& s
       float hrA = 0.f, hrB = 0.f, hbA = 0.f, hbB = 0.f;
       if (two) {
         const unsigned lc = (unsigned)min(max(kc, 1), km) * cplb, ld = (unsigned)min(max(kd, 1), km) * cplb;
-        hrA = rload(voA, orR + lc);
-        hrB = rload(voB, orR + lc);
-        hbA = rload(voA, orB + ld);
-        hbB = rload(voB, orB + ld);
+        hrA = bload(S, voA, orR + lc);
+        hrB = bload(S, voB, orR + lc);
+        hbA = bload(S, voA, orB + ld);
+        hbB = bload(S, voB, orB + ld);
       }
       LB[((U + 1) % 3) * PL + ownA] = bqA[(U + 2) % 6];
       LB[((U + 1) % 3) * PL + ownA + W] = bqB[(U + 2) % 6];
@@ -1430,11 +1458,6 @@ __device__ __forceinline__ void sor_pair_block(const Grid& g, const SplitGrid& s
           acc1 = acc_sq(acc1, reltmp, okB[pb] && kin);
           if (!two) bstore(S, redB, svB[pb], kin ? oRn + (unsigned)kr * cplb : 0u);
         }
-        if (raw) {  // the red rhs of level kr, as the red iterate's stores
-          const unsigned so = kin ? ocR + (unsigned)kr * cplb : 0u;
-          bstore(S, q.rrA, svA[pa], so);
-          bstore(S, q.rrB, svB[pb], so);
-        }
         LR[P * PL + ownA] = redA;
         LR[P * PL + ownA + W] = redB;
         rdA[U % 3] = redA;
@@ -1488,11 +1511,6 @@ __device__ __forceinline__ void sor_pair_block(const Grid& g, const SplitGrid& s
           bB = own + reltmp;
           acc1 = acc_sq(acc1, reltmp, okB[pb] && kin);
           if (!two) bstore(S, bB, svB[pb], kin ? oBn + (unsigned)kb * cplb : 0u);
-        }
-        if (raw) {  // the black rhs of level kb
-          const unsigned so = kin ? ocB + (unsigned)kb * cplb : 0u;
-          bstore(S, q.rbA, svA[pa], so);
-          bstore(S, q.rbB, svB[pb], so);
         }
         LB2[P * PL + ownA] = bA;
         LB2[P * PL + ownA + W] = bB;
@@ -1618,12 +1636,12 @@ __device__ __forceinline__ void sor_pair_block(const Grid& g, const SplitGrid& s
   }
 }
 
-template <bool OM1, bool ENT, bool RAW>
+template <bool OM1, bool ENT>
 __global__ __launch_bounds__(PNT) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_sor_pair(
     Grid g, SplitGrid s, Metrics m, SplitBufs sb, Ctrl* ctrl, PairLaunch pl, int nparts, float omega, float pjuge,
     int ntt, int ntj, int kchunk, int remap, const float* __restrict__ pent) {
-  sor_pair_block<OM1, ENT, RAW>(g, s, m, sb, ctrl, pl, nparts, omega, pjuge, ntt, ntj, kchunk, remap, pent,
-                                (int)blockIdx.x);
+  sor_pair_block<OM1, ENT>(g, s, m, sb, ctrl, pl, nparts, omega, pjuge, ntt, ntj, kchunk, remap, pent,
+                           (int)blockIdx.x);
 }
 
 // After the last two-iteration launch: its stop test, unless an earlier one
@@ -1866,10 +1884,10 @@ __global__ __launch_bounds__(256) void k_dc_norm(const double* __restrict__ part
   }
 }
 
-// P64 += e (the inner solve's last iterate, parity `par`), unless the
-// defect met the stop threshold
+// P64 += e (the inner solve's last iterate, in iterate set `set`), unless
+// the defect met the stop threshold
 __global__ __launch_bounds__(NT) void k_dc_correct(Grid g, SplitGrid s, SplitBufs sb, double* __restrict__ P64,
-                                                   int par, const Ctrl* ctrl) {
+                                                   int set, const Ctrl* ctrl) {
   if (ctrl->dc_done) return;  // uniform
   const int i = 1 + blockIdx.x * BX + threadIdx.x, j = 1 + blockIdx.y * BY + threadIdx.y;
   if (i > g.im || j > g.jm) return;
@@ -1878,7 +1896,7 @@ __global__ __launch_bounds__(NT) void k_dc_correct(Grid g, SplitGrid s, SplitBuf
   kspan(1, g.km, k0, k1);
   for (int k = k0; k <= k1; k++) {
     const int col = (i + j + k) & 1;
-    P64[ix(g, i, j, k)] += (double)sb.p[par][col][cix(s, t, j, k)];
+    P64[ix(g, i, j, k)] += (double)sb.p[set][col][cix(s, t, j, k)];
   }
 }
 
@@ -1966,8 +1984,8 @@ static SorShape sor_shape(const DevDomain& d) {
   return sor_shape(d.g, d.sg, rows, d.sor_tile_rows, kc, d.sor_tile_width);
 }
 
-// The shape of one iteration per launch (the paths that run iterations one
-// at a time: press mode 1's inner solves).
+// The shape of one iteration per launch (press mode 1's inner solves where
+// two iterations per launch are not available).
 static SorShape sor_shape_single(const DevDomain& d) {
   if (d.sor_rows != 4) return sor_shape(d);
   return sor_shape(d.g, d.sg, 2, d.sor_tile_rows, sor_fused_kchunk(d.g, d.sg, d.sor_fused_blocks, d.ncu),
@@ -1979,30 +1997,21 @@ static void launch_pair(const DevDomain& d, const SplitBufs& sb, const PairLaunc
                         const float* pent) {
   const size_t lvm_bytes = (size_t)(h.kc + 7) * sizeof(float4);
   const bool ent = (pl.iter == 1 || pl.fixup) && pent;  // may run iteration 1 on the entry ghosts
-  const bool raw = pl.raw && !pl.fixup;
   const int cls = (pl.iter == 1 || pl.fixup) ? KC_SOR1 : KC_SOR;
-#define PAIR(OM1, ENT, RAW)                                                                                    \
-  TIMED(d, cls, hipLaunchKernelGGL((k_sor_pair<OM1, ENT, RAW>), dim3(h.nparts), dim3(FX, PTY / 2), lvm_bytes,    \
-                                   d.stream, d.g, d.sg, d.m, sb, d.ctrl, pl, h.nparts, d.omega, pjuge, h.ntt,     \
-                                   h.ntj, h.kc, d.xcd_remap, pent))
+#define PAIR(OM1, ENT)                                                                                          \
+  TIMED(d, cls, hipLaunchKernelGGL((k_sor_pair<OM1, ENT>), dim3(h.nparts), dim3(FX, PTY / 2), lvm_bytes, d.stream, \
+                                   d.g, d.sg, d.m, sb, d.ctrl, pl, h.nparts, d.omega, pjuge, h.ntt, h.ntj, h.kc,   \
+                                   d.xcd_remap, pent))
   if (d.omega == 1.0f) {
-    if (raw && ent)
-      PAIR(true, true, true);
-    else if (raw)
-      PAIR(true, false, true);
-    else if (ent)
-      PAIR(true, true, false);
+    if (ent)
+      PAIR(true, true);
     else
-      PAIR(true, false, false);
+      PAIR(true, false);
   } else {
-    if (raw && ent)
-      PAIR(false, true, true);
-    else if (raw)
-      PAIR(false, false, true);
-    else if (ent)
-      PAIR(false, true, false);
+    if (ent)
+      PAIR(false, true);
     else
-      PAIR(false, false, false);
+      PAIR(false, false);
   }
 #undef PAIR
 }
@@ -2010,9 +2019,9 @@ static void launch_pair(const DevDomain& d, const SplitBufs& sb, const PairLaunc
 // Iterations l0..l1 in launches of two (the last alone when their number is
 // odd), reading iterate set in0 first and rotating over the three sets;
 // returns the last launch.  pent: the entry values of p for a launch of
-// iteration 1; raw: the first launch takes the rhs from k_rhs_head.
+// iteration 1.
 static PairLaunch launch_sor_pairs(const DevDomain& d, const SplitBufs& sb, const SorShape& h, float pjuge, int l0,
-                                   int l1, int in0, const float* pent, bool raw = false) {
+                                   int l1, int in0, const float* pent) {
   PairLaunch prev{};
   int in = in0, q = 0;
   for (int l = l0; l <= l1; l += 2, q++) {
@@ -2029,7 +2038,6 @@ static PairLaunch launch_sor_pairs(const DevDomain& d, const SplitBufs& sb, cons
     pl.prev_nparts = h.nparts;
     pl.rpart = prev.wpart;
     pl.fixup = 0;
-    pl.raw = (raw && l == l0) ? 1 : 0;
     if (l == 1 && pent) {
       // iteration 1: the kernel without the entry-ghost code runs when every
       // ghost of p equals its boundp copy (as after any press), the one with
@@ -2113,7 +2121,12 @@ static void launch_sor_iter(const DevDomain& d, const SplitBufs& sb, int l, floa
 static void launch_press_dc(const DevDomain& d, const SplitBufs& sb, int gx, int gy, int gz, int nred) {
   const Grid& g = d.g;
   const SplitGrid& s = d.sg;
-  const SorShape h = sor_shape_single(d);
+  // the inner iterations two per launch where the domain's SOR runs so
+  // (iterate sets rotating over three from set 0, which k_dc_defect clears),
+  // else one per launch; they never stop on their own (threshold 0)
+  const SorShape hp = sor_shape(d);
+  const bool pairs = hp.rows == 4;
+  const SorShape h = pairs ? hp : sor_shape_single(d);
   double* P64 = d.dc64;
   double* R64 = d.dc64 + d.n3;
   double* hist = d.dc64 + 2 * (size_t)d.n3;
@@ -2130,9 +2143,13 @@ static void launch_press_dc(const DevDomain& d, const SplitBufs& sb, int gx, int
                                           d.partials, d.ctrl));
     TIMED(d, KC_RHS, hipLaunchKernelGGL(k_dc_norm, dim3(1), dim3(256), 0, d.stream, (const double*)d.partials, nred,
                                         hist, o, d.ctrl, d.pjuge));
-    for (int l = 1; l <= d.dc_inner; l++) launch_sor_iter(di, sb, l, 0.0f, h, h.nparts);
-    TIMED(d, KC_PAV, hipLaunchKernelGGL(k_dc_correct, dim3(gx, gy, g.km), blk(), 0, d.stream, g, s, sb, P64,
-                                        d.dc_inner & 1, (const Ctrl*)d.ctrl));
+    int set = d.dc_inner & 1;  // the iterate set holding the inner result
+    if (pairs)
+      set = launch_sor_pairs(di, sb, h, 0.0f, 1, d.dc_inner, 0, nullptr).out_set;
+    else
+      for (int l = 1; l <= d.dc_inner; l++) launch_sor_iter(di, sb, l, 0.0f, h, h.nparts);
+    TIMED(d, KC_PAV, hipLaunchKernelGGL(k_dc_correct, dim3(gx, gy, g.km), blk(), 0, d.stream, g, s, sb, P64, set,
+                                        (const Ctrl*)d.ctrl));
   }
   TIMED(d, KC_PAV, hipLaunchKernelGGL(k_dc_pav, dim3(gx, gy, gz), blk(), 0, d.stream, g, d.m, (const double*)P64,
                                       d.partials));
@@ -2345,16 +2362,20 @@ void launch_press(const DevDomain& d, float dt) {
   SplitBufs sb = d.sb;
   if (d.cn1_separable) sb.cn1[0] = sb.cn1[1] = nullptr;
   if (d.press_mode == 0 && d.sor_fused && d.nmaxp >= 1 && sor_shape(d).rows == 4) {
-    // two iterations per launch: the head in one pass (k_rhs_head), its mean
-    // removed by the launch of iterations 1 and 2, which also reads the entry
-    // ghosts of p where they differ from their boundp copies
+    // two iterations per launch: the head in one pass (k_rhs_head), the mean
+    // removal over the split rhs alone (k_rhs_sub); the launch of iterations
+    // 1 and 2 reads the entry ghosts of p where they differ from their boundp
+    // copies
     const SorShape h = sor_shape(d);
     TIMED(d, KC_RHS, hipLaunchKernelGGL(k_rhs_head, dim3(gx, gy, gz), blk(), 0, d.stream, g, s, d.m,
                                         (const float*)d.F.u, (const float*)d.F.v, (const float*)d.F.w, d.F.f,
                                         d.F.g, d.F.h, (const float*)d.F.p, sb, d.partials, nred, d.ctrl, dt));
     TIMED(d, KC_RHS, hipLaunchKernelGGL(k_finalize_head, dim3(1), dim3(256), 0, d.stream,
                                         (const double*)d.partials, nred, d.ctrl, d.area));
-    const PairLaunch last = launch_sor_pairs(d, sb, h, d.pjuge, 1, d.nmaxp, 0, d.F.p, true);
+    const long n4 = (long)s.cplane * g.km / 2;  // both colours, four floats at a time
+    TIMED(d, KC_RHS, hipLaunchKernelGGL(k_rhs_sub, dim3(kRhsSubBlocks), dim3(256), 0, d.stream,
+                                        (const float4*)sb.rhsr[0], (float4*)sb.rhs[0], n4, (const Ctrl*)d.ctrl));
+    const PairLaunch last = launch_sor_pairs(d, sb, h, d.pjuge, 1, d.nmaxp, 0, d.F.p);
     TIMED(d, KC_PAV, hipLaunchKernelGGL(k_sor_pair_final, dim3(1), dim3(256), 0, d.stream, sb, d.ctrl, last,
                                         h.nparts, d.pjuge));
     PairLaunch fix{};

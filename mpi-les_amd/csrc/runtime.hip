@@ -613,7 +613,7 @@ int domain_alloc(int im, int jm, int km, les_domain** out) {
       sb.rhs[c] = arr0 + (4 + c) * cs;
       sb.cn1[c] = arr0 + (6 + c) * cs;
       sb.p[2][c] = sb.cn1[c];  // the third iterate set of two-iteration launches (separable cn1 only)
-      sb.rhsr[c] = arr0 + (8 + c) * cs;
+      sb.rhsr[c] = arr0 + (8 + c) * cs;  // rhs[0..1] and rhsr[0..1] each one after the other (k_rhs_sub)
     }
     sb.base = D->split;
     sb.nplanes = (int)nplanes;

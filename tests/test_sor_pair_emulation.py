@@ -6,10 +6,7 @@ statement by statement, checked against plain red-black iterations with the
 ghost rules of press.f95:89-122 / boundp.f95:7-78, bit for bit: two
 iterations per launch, one (the odd last launch of a loop), iteration 1 on
 the entry ghosts of p, and the three-set rotation with the stop test and
-fix-up of a pair whose first iteration meets the threshold, and the launch
-of iteration 1 after the one-pass press head, which reads the rhs before its
-mean is removed, subtracts the mean from every value it loads and stores
-each owned cell's result once.  It also checks
+fix-up of a pair whose first iteration meets the threshold.  It also checks
 that every buffer and LDS index the kernel forms stays inside its
 allocation, that every interior cell is stored exactly once by its owner,
 and that the redirected stores land only in the row padding or the scratch
@@ -91,16 +88,11 @@ class Problem:
             out[(i, j, k)] = self.get(self.arr(set_, red), (i - 1) // 2, j, k, mem)
         return out
 
-    def rhs(self, raw_mean=None):
-        """The rhs the SOR reads; with raw_mean, the one the launch of
-        iteration 1 forms from rhsr: (float)((double)rhsr - mean)."""
+    def rhs(self):
         out = {}
         for i, j, k in self.cells():
             red = (i + j + k) & 1
-            if raw_mean is None:
-                out[(i, j, k)] = self.get(5 if red else 4, (i - 1) // 2, j, k)
-            else:
-                out[(i, j, k)] = f32(np.float64(self.get(9 if red else 8, (i - 1) // 2, j, k)) - raw_mean)
+            out[(i, j, k)] = self.get(5 if red else 4, (i - 1) // 2, j, k)
         return out
 
     def upd(self, i, j, k, own, pim, pip, pjm, pjp, pkm, pkp, rr):
@@ -153,12 +145,9 @@ class Problem:
         return out, s
 
 
-def run_launch(pb, mem, kc, iter_, two, in_set, out_set, ent, remap=1, raw_mean=None, rhs_written=None):
+def run_launch(pb, mem, kc, iter_, two, in_set, out_set, ent, remap=1):
     """Emulate one launch of k_sor_pair on the allocation `mem` (updated in
-    place).  Returns (written, sum1, sum2, nparts).  raw_mean: the launch of
-    iteration 1 after the one-pass head (rhs from rhsr less this mean, its
-    stores recorded in rhs_written)."""
-    raw = raw_mean is not None
+    place).  Returns (written, sum1, sum2, nparts)."""
     im, jm, km, nt, cp, cpl = pb.im, pb.jm, pb.km, pb.nt, pb.cp, pb.cpl
     ntt, ntj = (nt + FTXW - 1) // FTXW, (jm + FTY - 1) // FTY
     nz = (km + kc - 1) // kc
@@ -168,8 +157,7 @@ def run_launch(pb, mem, kc, iter_, two, in_set, out_set, ent, remap=1, raw_mean=
     off0 = lambda a: (pb.plane(a, 1) - 1) * cplb  # noqa: E731
     oRo, oBo = off0(pb.arr(in_set, 1)), off0(pb.arr(in_set, 0))
     oRn, oBn = off0(pb.arr(out_set, 1)), off0(pb.arr(out_set, 0))
-    orR, orB = (off0(9), off0(8)) if raw else (off0(5), off0(4))
-    ocR, ocB = off0(5), off0(4)
+    orR, orB = off0(5), off0(4)
     src = mem.copy()  # what the launch reads (it never reads what it writes)
     written = {}
     sums = [0.0, 0.0]
@@ -181,18 +169,13 @@ def run_launch(pb, mem, kc, iter_, two, in_set, out_set, ent, remap=1, raw_mean=
             raise Err(f"load out of range {addr.min()}..{addr.max()} of {nbytes}")
         return src[addr // 4].copy()
 
-    def rload(voff, soff):
-        x = load(voff, soff)
-        return (x.astype(np.float64) - raw_mean).astype(f32) if raw else x
-
-    def store(val, voff, soff, bid, real_mask, rec=None):
-        rec = written if rec is None else rec
+    def store(val, voff, soff, bid, real_mask):
         addr = voff + soff
         if np.any(addr < 0) or np.any(addr + 4 > nbytes):
             raise Err("store out of range")
         idx = (addr // 4).ravel()
         for q, v, rm in zip(idx, val.ravel(), real_mask.ravel()):
-            rec.setdefault(int(q), []).append((bid, float(v), bool(rm)))
+            written.setdefault(int(q), []).append((bid, float(v), bool(rm)))
             mem[q] = v
 
     def pent_at(i, j, k, mask):
@@ -266,8 +249,8 @@ def run_launch(pb, mem, kc, iter_, two, in_set, out_set, ent, remap=1, raw_mean=
         def fill(L):
             lr = min(L, aHi) * cplb
             lk = min(max(L - 1, 1), bHi) * cplb
-            return dict(roA=load(voA, oRo + lr), roB=load(voB, oRo + lr), rrA=rload(voA, orR + lr),
-                        rrB=rload(voB, orR + lr), rbA=rload(voA, orB + lk), rbB=rload(voB, orB + lk))
+            return dict(roA=load(voA, oRo + lr), roB=load(voB, oRo + lr), rrA=load(voA, orR + lr),
+                        rrB=load(voB, orR + lr), rbA=load(voA, orB + lk), rbB=load(voB, orB + lk))
 
         lbq = min(aHi + 1, km + 1)
 
@@ -336,8 +319,8 @@ def run_launch(pb, mem, kc, iter_, two, in_set, out_set, ent, remap=1, raw_mean=
             if two:
                 lc = min(max(kc_, 1), km) * cplb
                 ld = min(max(kd, 1), km) * cplb
-                hrA, hrB = rload(voA, orR + lc), rload(voB, orR + lc)
-                hbA, hbB = rload(voA, orB + ld), rload(voB, orB + ld)
+                hrA, hrB = load(voA, orR + lc), load(voB, orR + lc)
+                hbA, hbB = load(voA, orB + ld), load(voB, orB + ld)
             LB[((U + 1) % 3) * PL + ownA] = bqA[(U + 2) % 6]
             LB[((U + 1) % 3) * PL + ownA + FX] = bqB[(U + 2) % 6]
             q = sl[U % 2]
@@ -388,10 +371,6 @@ def run_launch(pb, mem, kc, iter_, two, in_set, out_set, ent, remap=1, raw_mean=
                 acc1 += np.where(okB[pbb] & kin, r.astype(float) ** 2, 0)
                 if not two:
                     store(redB, svB[pbb], oRn + kr * cplb if kin else 0, bid, okB[pbb] & kin)
-                if raw:
-                    so = ocR + kr * cplb if kin else 0
-                    store(q["rrA"], svA[pa], so, bid, okA[pa] & kin, rhs_written)
-                    store(q["rrB"], svB[pbb], so, bid, okB[pbb] & kin, rhs_written)
                 LR[P * PL + ownA] = redA
                 LR[P * PL + ownA + FX] = redB
                 rdA[U % 3], rdB[U % 3] = redA, redB
@@ -427,11 +406,6 @@ def run_launch(pb, mem, kc, iter_, two, in_set, out_set, ent, remap=1, raw_mean=
                         m = wBl
                         store(v[m], sv[m], oBn + kb * cplb if kin else 0, bid, (ok & kin)[m])
                     res.append(v)
-                if raw:
-                    m = wBl
-                    so = ocB + kb * cplb if kin else 0
-                    store(q["rbA"][m], svA[pa][m], so, bid, (okA[pa] & kin)[m], rhs_written)
-                    store(q["rbB"][m], svB[pbb][m], so, bid, (okB[pbb] & kin)[m], rhs_written)
                 bA, bB = res
                 LB2[P * PL + ownA] = np.where(wBl, bA, LB2[P * PL + ownA])
                 LB2[P * PL + ownA + FX] = np.where(wBl, bB, LB2[P * PL + ownA + FX])
@@ -608,58 +582,3 @@ def test_rotation_stop_and_fixup():
     fix_written, _, _, _ = run_launch(pb, mem, kc, 3, False, in3, out3, ent=False)
     assert check_written(pb, fix_written, out3, its[2]) == 0
 
-
-def check_rhs_written(pb, written, expect):
-    """Every interior cell's split rhs stored once by its owner with the
-    expected value; other stores only in padding or the scratch plane."""
-    bad = 0
-    for (i, j, k), v in expect.items():
-        red = (i + j + k) & 1
-        qidx = pb.plane(5 if red else 4, k) * pb.cpl + (j - 1) * pb.cp + (i - 1) // 2
-        reals = [x for x in written.get(qidx, []) if x[2]]
-        if len(reals) != 1 or f32(reals[0][1]).tobytes() != f32(v).tobytes():
-            bad += 1
-            if bad < 5:
-                print("rhs", (i, j, k), written.get(qidx, []), float(v))
-    for qidx, ws in written.items():
-        pl, rem = divmod(qidx, pb.cpl)
-        tt = rem % pb.cp
-        scratch = pl == 0 or pl == pb.nplanes - 1 or tt >= pb.nt
-        if scratch == any(x[2] for x in ws):
-            bad += 1
-            if bad < 10:
-                print("rhs store", "owned into scratch" if scratch else "non-owned into a real cell", qidx)
-    return bad
-
-
-@pytest.mark.parametrize("im,jm,km,kc", [(20, 14, 9, 3), (21, 14, 9, 4), (130, 26, 12, 5), (7, 2, 2, 1),
-                                         (125, 30, 20, 7), (3, 2, 5, 5)])
-@pytest.mark.parametrize("two", [True, False])
-@pytest.mark.parametrize("ent", [True, False])
-def test_first_launch_removes_the_rhs_mean(im, jm, km, kc, two, ent):
-    """The launch of iterations 1 and 2 after the one-pass head (with the
-    entry-ghost code, or without it as when every ghost of p equals its
-    boundp copy): rhs read from rhsr less the mean (FP64 subtraction, as
-    k_rhs_split), the iterates those of plain iterations on that rhs, and
-    every owned cell's rhs stored once into the split rhs the later launches
-    read."""
-    pb = Problem(im, jm, km, seed=im * 5 + jm + km)
-    mean = 0.1234567890123
-    rhs = pb.rhs(raw_mean=mean)
-    old = pb.read_set(0)
-    it1, s1 = pb.iteration(old, rhs, entry=ent)
-    mem = pb.mem.copy()
-    rw = {}
-    written, e1, e2, _ = run_launch(pb, mem, kc, 1, two, 0, 1, ent=ent, raw_mean=mean, rhs_written=rw)
-    if two:
-        it2, s2 = pb.iteration(it1, rhs)
-        assert check_written(pb, written, 1, it2) == 0
-        assert abs(e2 - s2) <= 1e-12 * max(s2, 1e-30)
-    else:
-        assert check_written(pb, written, 1, it1) == 0
-    assert abs(e1 - s1) <= 1e-12 * max(s1, 1e-30)
-    assert check_rhs_written(pb, rw, rhs) == 0
-    # rhsr is left as it was
-    for a in (8, 9):
-        sl = slice(pb.plane(a, 1) * pb.cpl, (pb.plane(a, km) + 1) * pb.cpl)
-        assert np.array_equal(mem[sl], pb.mem[sl])
