@@ -162,6 +162,11 @@ def profiled_traffic(kernel_class: str, workload: str, sor_rows: int = 4):
     return int(m.group(1)) if m else None
 
 
+def _tuning_progress(done: int, total: int):
+    if done % 10 == 0 or done == total:
+        print(f"[bench] tuned {done} of {total} grid shapes", file=sys.stderr, flush=True)
+
+
 SERVE_BATCH_CELLS = 2 << 20  # domains above this many cells fill the GPU alone and are not batched
 
 
@@ -184,7 +189,7 @@ def serve_measurement(nreq: int, nsteps: int, sizes=None, rate: float | None = N
     srv = serving.Server(budget_bytes=200 << 30, burst=4)
     print(f"[bench] serving {nreq} requests x {nsteps} steps, "
           f"{'Poisson arrivals at %g/s' % rate if rate else 'all at t=0'}", file=sys.stderr, flush=True)
-    srv.warm(reqs)
+    srv.warm(reqs, progress=_tuning_progress)
     seq = serving.summarize(srv.run(reqs, sequential=True))
     con = serving.summarize(srv.run(reqs)) if streams else None
     bat = serving.summarize(srv.run_batched(reqs, max_batch_cells=SERVE_BATCH_CELLS))
@@ -224,7 +229,7 @@ def poisson_sweep(rates: list[float], nreq: int, nsteps: int) -> int:
     out = {"requests": nreq, "steps_each": nsteps, "batch_cells_max": SERVE_BATCH_CELLS, "sweep": []}
     for rate in rates:
         reqs = serving.sample_requests(nreq, nsteps, rate=rate, seed=2024)
-        srv.warm(reqs)
+        srv.warm(reqs, progress=_tuning_progress)
         rec = {"rate_per_s": rate, "offered_mcell_updates_per_s":
                round(sum(r.cells * r.nsteps for r in reqs) / max(reqs[-1].arrival, 1e-9) / 1e6, 1)}
         for tag, fn in (("batched", lambda: srv.run_batched(reqs, max_batch_cells=SERVE_BATCH_CELLS)),
@@ -240,6 +245,29 @@ def poisson_sweep(rates: list[float], nreq: int, nsteps: int) -> int:
     return 0
 
 
+def defect_measure(setup, p: np.ndarray, rhs: np.ndarray) -> float:
+    """sum((cn1*d)^2) over the interior in FP64, d = rhs - B(p) with B the
+    pressure operator of the SOR (press.f95:95-98 at its fixed point, the
+    ghosts of p as press leaves them), cn1 = 1/((ax+by)+cz) in FP32 as the
+    kernels evaluate it: the measure press mode 1 stops on, evaluated on any
+    result p (a constant shift of p does not change it)."""
+    w = setup.workload
+    im, jm, km = w.im, w.jm, w.km
+    c = setup.coeffs
+    ax, by, cz = (np.asarray(t, np.float32) for t in c["_cn1_terms"])
+    s32 = (ax[:, None, None] + by[None, :, None]) + cz[None, None, :]
+    cn = (np.float32(1.0) / s32).astype(np.float64)
+    P = np.asarray(p, np.float64)
+    co = {n: np.asarray(c[n], np.float64) for n in ("cn2l", "cn2s", "cn3l", "cn3s", "cn4l", "cn4s")}
+    I, J, K = slice(1, im + 1), slice(1, jm + 1), slice(1, km + 1)
+    b = (co["cn2l"][:im, None, None] * P[2:im + 2, J, K] + co["cn2s"][:im, None, None] * P[0:im, J, K]
+         + co["cn3l"][None, :jm, None] * P[I, 2:jm + 2, K] + co["cn3s"][None, :jm, None] * P[I, 0:jm, K]
+         + co["cn4l"][None, None, :km] * P[I, J, 2:km + 2] + co["cn4s"][None, None, :km] * P[I, J, 0:km]
+         - P[I, J, K] / cn)
+    rd = cn * (np.asarray(rhs, np.float64)[I, J, K] - b)
+    return float(np.sum(rd * rd))
+
+
 def precision_comparison(K: int, W: int, configs) -> int:
     """The precision-512x512x128 workload (dx=dy=2 m, 35 % cover, u*=0.6,
     z0=0.3, dt=0.02; SOR at most 100 iterations, stop at 1e-6) stepped with
@@ -247,7 +275,11 @@ def precision_comparison(K: int, W: int, configs) -> int:
     (outer, inner, omega) in configs (at most `outer` steps of `inner` FP32
     SOR iterations with relaxation omega, stopping once sum((cn1*d)^2) <
     1e-6): time per step from graph replay, and per step of the timed run
-    the measure the stop test compares with 1e-6 and the sweeps taken."""
+    the measure the stop test compares with 1e-6 and the sweeps taken.  The
+    reference's stop measure is sum(reltmp^2), the size of the last
+    iteration's update; the defect correction's is sum((cn1*d)^2) of its
+    iterate; result_defect_measure evaluates the latter on every press's
+    final p, so the presses compare on one measure."""
     w = setup_gen.workloads()["precision-512x512x128"]
     out = {"workload": w.name, "grid": [w.im, w.jm, w.km], "steps": K, "warmup": W, "target": w.pjuge}
     # the reference's SOR as shipped (100 iterations), and with 1024 to show
@@ -257,7 +289,8 @@ def precision_comparison(K: int, W: int, configs) -> int:
     for tag, cfg in runs:
         print(f"[bench] precision: {tag}", file=sys.stderr, flush=True)
         wt = dataclasses.replace(w, nmaxp=1024) if tag == "fp32_sor_1024" else w
-        dom = api.Domain(setup_gen.build(wt, dense=False))
+        setup = setup_gen.build(wt, dense=False)
+        dom = api.Domain(setup)
         om = None
         if cfg is not None:
             outer, inner, om = cfg
@@ -288,7 +321,9 @@ def precision_comparison(K: int, W: int, configs) -> int:
             finals.append(float(h[-1]))
         rec = {"ms_per_step": round(1e3 * dt_, 3), "mcell_updates_per_s": round(w.cells / dt_ / 1e6, 1),
                "sweeps_per_step": sweeps, "final_residual_measure": finals,
-               "reached_target": [f < w.pjuge for f in finals]}
+               "reached_target": [f < w.pjuge for f in finals],
+               # the same measure for every press: sum((cn1*d)^2) of the last step's result p
+               "result_defect_measure": float(f"{defect_measure(setup, dom.download('p'), dom.download('rhs')):.4e}")}
         if om is not None:
             rec["defect_history_last_step"] = [float(f"{x:.3e}") for x in h]
         elif len(h) > 100:

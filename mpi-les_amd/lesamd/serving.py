@@ -234,17 +234,21 @@ class Server:
         # batch_factory(domains) -> object with step(ns, nmaxs), done(), close()
         self.batch_factory = batch_factory
 
-    def warm(self, requests) -> None:
+    def warm(self, requests, progress=None) -> None:
         """Create (and close) one domain of every grid shape among the
-        requests, untimed: the default factory's per-shape SOR tuning is timed
-        once per shape and process, so after this every serving mode starts
-        from the same tuned state."""
-        seen = set()
+        requests, untimed: the default factory's per-shape tuning (SOR launch
+        shape, stencil k levels) is timed once per shape and process, so
+        after this every serving mode starts from the same tuned state.
+        progress(done, total) is called after each shape."""
+        shapes = []
         for r in requests:
             w = r.workload
-            if (w.im, w.jm, w.km) not in seen:
-                seen.add((w.im, w.jm, w.km))
-                self.factory(r).close()
+            if (w.im, w.jm, w.km) not in {(x.workload.im, x.workload.jm, x.workload.km) for x in shapes}:
+                shapes.append(r)
+        for q, r in enumerate(shapes):
+            self.factory(r).close()
+            if progress is not None:
+                progress(q + 1, len(shapes))
 
     def run_batched(self, requests, max_batch_cells: int | None = None) -> list[Result]:
         """Continuous batching: the admitted domains of at most
