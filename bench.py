@@ -167,6 +167,11 @@ def _tuning_progress(done: int, total: int):
         print(f"[bench] tuned {done} of {total} grid shapes", file=sys.stderr, flush=True)
 
 
+def _served_progress(done: int, total: int):
+    if done % 16 == 0 or done == total:
+        print(f"[bench] served {done} of {total} requests", file=sys.stderr, flush=True)
+
+
 SERVE_BATCH_CELLS = 2 << 20  # domains above this many cells fill the GPU alone and are not batched
 
 
@@ -190,6 +195,7 @@ def serve_measurement(nreq: int, nsteps: int, sizes=None, rate: float | None = N
     print(f"[bench] serving {nreq} requests x {nsteps} steps, "
           f"{'Poisson arrivals at %g/s' % rate if rate else 'all at t=0'}", file=sys.stderr, flush=True)
     srv.warm(reqs, progress=_tuning_progress)
+    srv.on_result = _served_progress
     seq = serving.summarize(srv.run(reqs, sequential=True))
     con = serving.summarize(srv.run(reqs)) if streams else None
     bat = serving.summarize(srv.run_batched(reqs, max_batch_cells=SERVE_BATCH_CELLS))
@@ -226,6 +232,7 @@ def poisson_sweep(rates: list[float], nreq: int, nsteps: int) -> int:
     saturates where the wait and p99 climb while the throughput flattens."""
     from lesamd import serving
     srv = serving.Server(budget_bytes=200 << 30, burst=4)
+    srv.on_result = _served_progress
     out = {"requests": nreq, "steps_each": nsteps, "batch_cells_max": SERVE_BATCH_CELLS, "sweep": []}
     for rate in rates:
         reqs = serving.sample_requests(nreq, nsteps, rate=rate, seed=2024)

@@ -213,13 +213,14 @@ class Server:
         self.max_active = int(max_active)
         self.clock = clock
         self.workers = int(workers)
+        self.on_result = None  # called as on_result(done, total) after each completed request
         if domain_factory is None or bytes_of is None:
             from . import api
 
             def factory(req):
                 d = api.Domain(setup_gen.build(req.workload, dense=False))
                 d.set_graph(True)
-                d.autotune()  # SOR launch shape for this grid shape (timed once per shape)
+                d.autotune()  # SOR launch shape and stencil k levels for this grid shape (timed once per shape)
                 return d
 
             domain_factory = domain_factory or factory
@@ -269,6 +270,8 @@ class Server:
             results.append(Result(a.req.rid, a.req.cells, a.req.nsteps, a.req.arrival, a.admitted, lc.now(),
                                   _status(a.dom)))
             lc.release(a)
+            if self.on_result is not None:
+                self.on_result(len(results), len(requests))
 
         try:
             while lc.waiting() or batched or alone or joining:
@@ -350,6 +353,8 @@ class Server:
                                               lc.now(), _status(a.dom)))
                         lc.release(a)
                         progressed = True
+                        if self.on_result is not None:
+                            self.on_result(len(results), len(requests))
                     else:
                         still.append(a)
                 active = still
