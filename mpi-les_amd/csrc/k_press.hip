@@ -1177,6 +1177,8 @@ struct PairLaunch {
   int prev_two, prev_in, prev_out, prev_nparts, rpart;  // ... its shape, sets, slots
   int fixup;        // the fix-up launch: iteration and sets from ctrl->fix_*, one iteration
   int gate;         // 0: runs; 1: runs only when no ghost of p differs from its boundp copy; 2: only when one does
+  int prev_colours; // the previous launch ran one iteration in a one-iteration kernel (iteration 1 of a batch):
+                    // its partials are sb.part[rpart][1] (red) and [0] (black) of that iteration
 };
 
 // The stop test (press.f95:119-121) of a launch of iteration L (and L+1 when
@@ -1246,9 +1248,13 @@ __device__ __forceinline__ void sor_pair_block(const Grid& g, const SplitGrid& s
   const bool ent = ENT && sh[4] != 0;  // iteration 1 on the entry ghosts of p
   if (!pl.fixup && pl.prev_iter > 0 && bid == 0) {  // the previous launch's stop test
     double s1 = 0.0, s2 = 0.0;
-    for (int b = tid; b < pl.prev_nparts; b += GNT) {
-      s1 += sb.part[pl.rpart][0][b];
-      s2 += sb.part[pl.rpart][1][b];
+    if (pl.prev_colours) {
+      for (int b = tid; b < pl.prev_nparts; b += GNT) s1 += sb.part[pl.rpart][1][b] + sb.part[pl.rpart][0][b];
+    } else {
+      for (int b = tid; b < pl.prev_nparts; b += GNT) {
+        s1 += sb.part[pl.rpart][0][b];
+        s2 += sb.part[pl.rpart][1][b];
+      }
     }
     s1 = block_sum<GNT>(s1, tid, lds);
     s2 = block_sum<GNT>(s2, tid, lds);
@@ -1660,6 +1666,31 @@ __global__ __launch_bounds__(256) void k_sor_pair_final(SplitBufs sb, Ctrl* ctrl
   if (threadIdx.x == 0) pair_stop(ctrl, last.iter, last.two, s1, s2, last.in_set, last.out_set, pjuge);
 }
 
+// Iterations 2.. of the batched domains (k_sor_iter2_batch's table), two
+// per launch: launch pl of the sequence that starts at iteration 2 from
+// iterate set 1 (iteration 1, run by the one-iteration entry kernel, wrote
+// set 1 and its partials to slot 1).  Each domain's blocks run the launch
+// with its own shape, skip the launches past its nmaxp, and run the last one
+// alone when it holds only nmaxp; the fix-up (pl.fixup) takes its state from
+// each domain's control block.
+template <bool OM1>
+__global__ __launch_bounds__(PNT) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_sor_pair_batch(
+    const SorBatchEntry* __restrict__ tab, const int* __restrict__ block0, int ntab, PairLaunch pl, int remap) {
+  int e = 0;
+  for (int q = 1; q < ntab; q++)  // block0[q] == tab[q].block0, packed
+    if ((int)blockIdx.x >= block0[q]) e = q;
+  const SorBatchEntry& E = tab[e];
+  const int bid = (int)blockIdx.x - E.block0;
+  if (bid >= E.nparts) return;  // padding block
+  if (!pl.fixup) {
+    if (pl.iter > E.nmaxp) return;  // this domain has finished its iterations
+    pl.two = pl.two && pl.iter + 1 <= E.nmaxp;
+    pl.prev_nparts = pl.prev_colours ? E.nparts1 : E.nparts;
+  }
+  sor_pair_block<OM1, false>(E.g, E.s, E.m, E.sb, E.ctrl, pl, E.nparts, E.omega, E.pjuge, E.ntt, E.ntj, E.kchunk,
+                             remap, nullptr, bid);
+}
+
 // After the iteration loop: when no iteration met the stop test, the last
 // one (nmaxp) is summed here.  Records the iteration count and sum.
 __device__ __forceinline__ void sor_final_body(const SplitBufs& sb, Ctrl* ctrl, int nmaxp, int nparts) {
@@ -1677,6 +1708,30 @@ __device__ __forceinline__ void sor_final_body(const SplitBufs& sb, Ctrl* ctrl, 
 }
 __global__ __launch_bounds__(256) void k_sor_final(SplitBufs sb, Ctrl* ctrl, int nmaxp, int nparts) {
   sor_final_body(sb, ctrl, nmaxp, nparts);
+}
+
+// One block per batched domain after its two-iteration launches: the stop
+// test of its last launch (k_sor_pair_final's, with that launch's iteration,
+// sets and slots derived from the domain's nmaxp as k_sor_pair_batch's
+// sequence gives them), or, for nmaxp = 1, the closing sum of iteration 1.
+__global__ __launch_bounds__(256) void k_sor_pair_batch_final(const SorBatchEntry* __restrict__ tab) {
+  __shared__ double lds[4];
+  const SorBatchEntry& E = tab[blockIdx.x];
+  Ctrl* ctrl = E.ctrl;
+  if (E.nmaxp <= 1) {
+    sor_final_body(E.sb, ctrl, E.nmaxp, E.nparts1);
+    return;
+  }
+  if (ctrl->sor_done != 0) return;  // uniform: written by earlier launches
+  const int q = (E.nmaxp - 2) / 2, L = 2 + 2 * q, two = L + 1 <= E.nmaxp;
+  double s1 = 0.0, s2 = 0.0;
+  for (int b = threadIdx.x; b < E.nparts; b += 256) {
+    s1 += E.sb.part[q & 1][0][b];
+    s2 += E.sb.part[q & 1][1][b];
+  }
+  s1 = block_sum<256>(s1, threadIdx.x, lds);
+  s2 = block_sum<256>(s2, threadIdx.x, lds);
+  if (threadIdx.x == 0) pair_stop(ctrl, L, two, s1, s2, (1 + q) % 3, (2 + q) % 3, E.pjuge);
 }
 
 // pav partials: p*dx1*dy1*dzn over the SOR result (press.f95:124-133).
@@ -2222,9 +2277,76 @@ int sor_batch_kchunk(const DevDomain* const* ds, int n, int ty, int tw, int ncu)
   return best_kc;
 }
 
-void sor_batch_entry(const DevDomain& d, int kc, int ty, int tw, SorBatchEntry* e) {
+// The batch's k chunk for two iterations per launch: the same cost model
+// with k_sor_pair's tiles, one 1024-thread block resident per CU, each block
+// marching its chunk and six more levels.
+int sor_pair_batch_kchunk(const DevDomain* const* ds, int n, int ncu) {
+  int kmax = 1;
+  for (int q = 0; q < n; q++) kmax = std::max(kmax, ds[q]->g.km);
+  const long slots = ncu > 0 ? ncu : 256;
+  long best = -1;
+  int best_kc = kmax;
+  for (int kc = 1; kc <= kmax; kc++) {
+    long blocks = 0;
+    for (int q = 0; q < n; q++) blocks += sor_shape(ds[q]->g, ds[q]->sg, 4, PTY, kc).nparts;
+    const long cost = (blocks + slots - 1) / slots * (std::min(kc, kmax) + 6);
+    if (best < 0 || cost < best) {
+      best = cost;
+      best_kc = kc;
+    }
+  }
+  return best_kc;
+}
+
+// Iterations 2..lmax of the batched domains in launches of two (each domain
+// stopping at its own nmaxp), the closing stop test of each domain's last
+// launch and the fix-up launch.  Iteration 1 ran in the entry kernel.
+void launch_sor_pair_batch(const SorBatchEntry* tab, const int* block0, int ntab, int nblocks, int lmax, int kcmax,
+                           bool om1, int remap, hipStream_t stream) {
+  const size_t lds = (size_t)(kcmax + 7) * sizeof(float4);
+  auto launch = [&](const PairLaunch& pl) {
+    if (om1)
+      hipLaunchKernelGGL((k_sor_pair_batch<true>), dim3(nblocks), dim3(FX, PTY / 2), lds, stream, tab, block0, ntab,
+                         pl, remap);
+    else
+      hipLaunchKernelGGL((k_sor_pair_batch<false>), dim3(nblocks), dim3(FX, PTY / 2), lds, stream, tab, block0, ntab,
+                         pl, remap);
+  };
+  PairLaunch prev{};
+  prev.iter = 1;  // iteration 1: set 1, partials of both colours in slot 1
+  prev.two = 0;
+  prev.in_set = 0;
+  prev.out_set = 1;
+  prev.wpart = 1;
+  int q = 0;
+  for (int l = 2; l <= lmax; l += 2, q++) {
+    PairLaunch pl{};
+    pl.iter = l;
+    pl.two = (l + 1 <= lmax) ? 1 : 0;  // narrowed per domain
+    pl.in_set = (1 + q) % 3;
+    pl.out_set = (2 + q) % 3;
+    pl.wpart = q & 1;
+    pl.prev_iter = prev.iter;
+    pl.prev_two = prev.two;
+    pl.prev_in = prev.in_set;
+    pl.prev_out = prev.out_set;
+    pl.rpart = prev.wpart;
+    pl.prev_colours = (q == 0) ? 1 : 0;
+    launch(pl);
+    prev = pl;
+  }
+  hipLaunchKernelGGL(k_sor_pair_batch_final, dim3((unsigned)ntab), dim3(256), 0, stream, tab);
+  if (lmax >= 2) {
+    PairLaunch fix{};
+    fix.fixup = 1;
+    launch(fix);
+  }
+}
+
+void sor_batch_entry(const DevDomain& d, int kc, int ty, int tw, SorBatchEntry* e, bool pairs) {
   SorShape h, h1;
   batch_shapes(d, kc, ty, tw, &h, &h1);
+  if (pairs) h = sor_shape(d.g, d.sg, 4, PTY, h.kc);  // the same k chunk as iteration 1's (h1)
   e->g = d.g;
   e->s = d.sg;
   e->m = d.m;
@@ -2242,7 +2364,7 @@ void sor_batch_entry(const DevDomain& d, int kc, int ty, int tw, SorBatchEntry* 
   e->pjuge = d.pjuge;
 }
 
-void launch_press_part(const DevDomain& d, float dt, int part, int kc, int ty, int tw) {
+void launch_press_part(const DevDomain& d, float dt, int part, int kc, int ty, int tw, bool pairs) {
   const Grid& g = d.g;
   const int gx = (g.im + BX - 1) / BX, gy = (g.jm + BY - 1) / BY;
   const int gz = kblocks(gx, gy, g.km, kMaxRedBlocks);
@@ -2255,7 +2377,10 @@ void launch_press_part(const DevDomain& d, float dt, int part, int kc, int ty, i
     press_head(d, dt, sb, gx, gy, gz, nred);
     launch_sor_iter(d, sb, 1, d.pjuge, h1, h1.nparts, d.F.p);
   } else {
-    launch_press_tail(d, sb, d.nmaxp == 1 ? h1.nparts : h.nparts, gx, gy, gz, nred);
+    if (pairs)  // the stop test of the last launch and the fix-up ran batched (launch_sor_pair_batch)
+      launch_press_mean(d, sb, gx, gy, gz, nred);
+    else
+      launch_press_tail(d, sb, d.nmaxp == 1 ? h1.nparts : h.nparts, gx, gy, gz, nred);
   }
 }
 
@@ -2334,7 +2459,7 @@ void press_batch_spans(const DevDomain& d, int kc, int ty, int tw, int nzb, Batc
   sp[BS_PADJ] = {0, px, py, nzb > 0 ? std::min(nzb, g.km + 2) : g.km + 2};
 }
 
-void launch_press_batch(const StageBatch& b, int part, hipStream_t stream) {
+void launch_press_batch(const StageBatch& b, int part, hipStream_t stream, bool pairs) {
   const BatchDom* bd = b.doms;
   const StageSpan* sp = b.spans;
   const int nd = b.nd;
@@ -2346,7 +2471,8 @@ void launch_press_batch(const StageBatch& b, int part, hipStream_t stream) {
     hipLaunchKernelGGL(kb_sor_entry, nb(BS_SOR1), dim3(FX, FY), 3 * (size_t)(b.kc1max + 2) * sizeof(float), stream,
                        bd, sp, nd);
   } else {
-    hipLaunchKernelGGL(kb_sor_final, dim3((unsigned)nd), dim3(256), 0, stream, bd);
+    if (!pairs)  // with pairs: launch_sor_pair_batch ran the closing stop test
+      hipLaunchKernelGGL(kb_sor_final, dim3((unsigned)nd), dim3(256), 0, stream, bd);
     hipLaunchKernelGGL(kb_pav, nb(BS_PAV), blk(), 0, stream, bd, sp, nd);
     hipLaunchKernelGGL(kb_finalize, dim3((unsigned)nd), dim3(256), 0, stream, bd, 1);
     hipLaunchKernelGGL(kb_padj, nb(BS_PADJ), blk(), 0, stream, bd, sp, nd);

@@ -120,13 +120,20 @@ void launch_sor_batch(const SorBatchEntry* tab, const int* block0, int ntab, int
 // iteration 1, part 1 the stop test of the last iteration and the mean
 // removal.  kc, ty, tw: the SOR k chunk (0: the domain's own), tile rows
 // and tile width of the batch.
-void launch_press_part(const DevDomain& d, float dt, int part, int kc, int ty, int tw);
+void launch_press_part(const DevDomain& d, float dt, int part, int kc, int ty, int tw, bool pairs = false);
 // The domain's entry in a batched SOR launch with k chunk kc, tile rows ty
 // and tile width tw (block0 left to the caller).
-void sor_batch_entry(const DevDomain& d, int kc, int ty, int tw, SorBatchEntry* e);
+void sor_batch_entry(const DevDomain& d, int kc, int ty, int tw, SorBatchEntry* e, bool pairs = false);
 // The k chunk of a batch of domains' SOR launches with tile rows ty and
 // tile width tw.
 int sor_batch_kchunk(const DevDomain* const* ds, int n, int ty, int tw, int ncu);
+// The batch's k chunk with two SOR iterations per launch (k_sor_pair tiles).
+int sor_pair_batch_kchunk(const DevDomain* const* ds, int n, int ncu);
+// Iterations 2..lmax of the batched domains two per launch (each domain up to
+// its nmaxp; tables as built with sor_batch_entry(..., pairs = true)), then
+// each domain's closing stop test and the fix-up launch.
+void launch_sor_pair_batch(const SorBatchEntry* tab, const int* block0, int ntab, int nblocks, int lmax, int kcmax,
+                           bool om1, int remap, hipStream_t stream);
 // Whether a domain's SOR iterations can run in a batch (fused reference SOR,
 // separable cn1, even jm).
 bool sor_batchable(const DevDomain& d);
@@ -197,7 +204,7 @@ void press_batch_spans(const DevDomain& d, int kc, int ty, int tw, int nzb, Batc
 void launch_step_batch(const StageBatch& b, hipStream_t stream);
 // part 0: rhs, its mean, the split arrays and SOR iteration 1; part 1 (after
 // iterations 2..nmaxp): the closing stop-test sum, pav and the adjusted p.
-void launch_press_batch(const StageBatch& b, int part, hipStream_t stream);
+void launch_press_batch(const StageBatch& b, int part, hipStream_t stream, bool pairs = false);
 void launch_sor_trial(const DevDomain& d, int rows, int ty, int tw, int kc, int iters);
 int reduction_parts_needed(const Grid& g);
 

@@ -1822,6 +1822,7 @@ struct les_batch {
   const int* dblock0 = nullptr;
   int ntab = 0, nblocks = 0, kcmax = 0, lmax = 0, remap = 1;
   int sor_kc = 0, sor_ty = 16, sor_tw = 64;  // SOR k chunk, tile rows and width of the batched domains
+  bool pairs = true;              // batched SOR iterations 2.. two per launch (k_sor_pair_batch)
   bool om1 = false;               // every batched domain has omega == 1
   // Domains whose whole step runs as batched stage launches (fused velfg and
   // les, batchable SOR); the others step on their own streams.
@@ -1891,14 +1892,17 @@ int batch_prepare(les_batch* B) {
   // splits of the copy-like stages giving about 36 blocks per CU in all
   B->sor_ty = env_int("LES_BATCH_SOR_TILE_ROWS", 16) == 32 ? 32 : 16;
   B->sor_tw = (B->sor_ty == 32 && env_int("LES_BATCH_SOR_TILE_WIDTH", 64) == 32) ? 32 : 64;
+  B->pairs = env_int("LES_BATCH_SOR_PAIRS", 1) != 0;
   B->sor_kc = fd.empty() ? 0 : env_int("LES_BATCH_SOR_KCHUNK", 0);
-  if (!fd.empty() && B->sor_kc <= 0) B->sor_kc = sor_batch_kchunk(fd.data(), (int)fd.size(), B->sor_ty, B->sor_tw, ncu);
+  if (!fd.empty() && B->sor_kc <= 0)
+    B->sor_kc = B->pairs ? sor_pair_batch_kchunk(fd.data(), (int)fd.size(), ncu)
+                         : sor_batch_kchunk(fd.data(), (int)fd.size(), B->sor_ty, B->sor_tw, ncu);
   const int nzb = tiles > 0 ? (int)std::max(1L, 36L * ncu / tiles) : 0;
   for (size_t q = 0; q < B->doms.size(); q++) {
     if (!B->full[q]) continue;
     les_domain* D = B->doms[q];
     SorBatchEntry e;
-    sor_batch_entry(D->d, B->sor_kc, B->sor_ty, B->sor_tw, &e);
+    sor_batch_entry(D->d, B->sor_kc, B->sor_ty, B->sor_tw, &e, B->pairs);
     e.block0 = block;
     block += (e.nparts + 7) / 8 * 8;
     B->kcmax = std::max(B->kcmax, e.kchunk);
@@ -2012,10 +2016,12 @@ int batch_enqueue(les_batch* B, const int* ns, const int* nmaxs, bool capturing)
     launch_step_batch(B->stages, B->stream);
     launch_press_batch(B->stages, 0, B->stream);
   }
-  for (int l = 2; l <= B->lmax && B->ntab > 0; l++)
+  if (B->pairs && B->ntab > 0)
+    launch_sor_pair_batch(B->dtab, B->dblock0, B->ntab, B->nblocks, B->lmax, B->kcmax, B->om1, B->remap, B->stream);
+  for (int l = 2; !B->pairs && l <= B->lmax && B->ntab > 0; l++)
     launch_sor_batch(B->dtab, B->dblock0, B->ntab, B->nblocks, l, B->kcmax, B->sor_ty, B->sor_tw, B->om1, B->remap,
                      B->stream);
-  if (staged) launch_press_batch(B->stages, 1, B->stream);
+  if (staged) launch_press_batch(B->stages, 1, B->stream, B->pairs);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipEventRecord(B->sor_done, B->stream));
   for (size_t q = 0; q < nd; q++) {
@@ -2026,7 +2032,7 @@ int batch_enqueue(les_batch* B, const int* ns, const int* nmaxs, bool capturing)
     }
     if (split[q]) {
       HIP_TRY(hipStreamWaitEvent(D->d.stream, B->sor_done, 0));
-      launch_press_part(D->d, D->dt, 1, B->sor_kc, B->sor_ty, B->sor_tw);
+      launch_press_part(D->d, D->dt, 1, B->sor_kc, B->sor_ty, B->sor_tw, B->pairs);
       HIP_TRY(hipGetLastError());
     }
     HIP_TRY(hipEventRecord(B->tail_done[q], D->d.stream));

@@ -1,6 +1,6 @@
 """Batched steps (les_batch_*): domains advanced together -- every stage of
-a step one launch for all of them, the SOR iterations one launch per
-iteration -- give each domain exactly the results of stepping it alone (the
+a step one launch for all of them, the SOR iterations 2.. two per launch
+(one per launch with LES_BATCH_SOR_PAIRS=0) -- give each domain exactly the results of stepping it alone (the
 same result alone and in the mix), with eager steps and graph replays, with
 domains that step on their own streams in the mix (odd jm: SOR not
 batchable; les and adam unfused: stages not batched), with domains joining
@@ -53,13 +53,15 @@ def _check(doms, alone):
         np.testing.assert_allclose(got["sor"], st["sor"], rtol=1e-12, atol=0, err_msg=str(q))
 
 
+@pytest.mark.parametrize("pairs", [True, False])
 @pytest.mark.parametrize("graph", [True, False])
-def test_batch_equals_alone(graph, monkeypatch):
+def test_batch_equals_alone(graph, pairs, monkeypatch):
     setups = [setup_gen.build(setup_gen.small_workload(**c)) for c in CASES]
-    print(f"\n[batch] {len(setups)} domains, {NSTEPS} steps, graph={graph}", flush=True)
+    print(f"\n[batch] {len(setups)} domains, {NSTEPS} steps, graph={graph}, pairs={pairs}", flush=True)
     api.release_all()
     alone = _alone(setups, [NSTEPS] * len(setups), monkeypatch)
     doms = [_domain(q, s, monkeypatch) for q, s in enumerate(setups)]
+    monkeypatch.setenv("LES_BATCH_SOR_PAIRS", "1" if pairs else "0")
     b = api.Batch(doms, graph=graph)
     mid = None
     for n in range(1, NSTEPS + 1):
@@ -102,5 +104,50 @@ def test_batch_staggered_joins(monkeypatch):
     b.synchronize()
     b.close()
     _check(doms, alone)
+    for d in doms:
+        d.close()
+
+
+def test_batch_pairs_stop_and_cap_per_domain(monkeypatch):
+    """The batched SOR runs iterations 2.. two per launch over the shared
+    launch sequence: each domain runs up to its own iteration cap (1, 4, 7
+    and 50: none, whole launches, a lone last iteration) and stops at its own
+    iteration (1, the first of the sequence's launches, the second of one,
+    and a later first one, which need the fix-up), with the results of the
+    domain stepped alone."""
+    import dataclasses
+    shapes = [dict(im=40, jm=28, km=14, seed=51 + q) for q in range(8)]
+    # thresholds between two residuals of a run that never stops, so that
+    # step 1 stops exactly at the given iteration
+    stops = {4: 1, 5: 2, 6: 3, 7: 6}
+    setups = []
+    for q, c in enumerate(shapes):
+        w = setup_gen.small_workload(**c)
+        if q < 4:
+            w = dataclasses.replace(w, nmaxp=(1, 4, 7, 50)[q], pjuge=1e-30)
+        else:
+            probe = api.Domain(setup_gen.build(dataclasses.replace(w, pjuge=1e-30)))
+            probe.step(1, 3)
+            h = probe.press_stats()["sor"]
+            probe.close()
+            at = stops[q]
+            lo, hi = h[at - 1], (min(h[: at - 1]) if at > 1 else np.inf)
+            assert lo < hi, (q, h[: at + 1])
+            w = dataclasses.replace(w, pjuge=float(np.sqrt(lo * hi)) if np.isfinite(hi) else float(lo * 2))
+        setups.append(setup_gen.build(w))
+    print("\n[batch pairs] caps 1/4/7/50, stops at 1/2/3/6", flush=True)
+    api.release_all()
+    alone = _alone(setups, [3] * len(setups), monkeypatch)
+    doms = [api.Domain(s) for s in setups]
+    b = api.Batch(doms, graph=True)
+    for n in range(1, 4):
+        b.step([n] * len(doms), [3] * len(doms))
+        if n == 1:
+            b.synchronize()
+            for q in range(4, 8):
+                assert doms[q].press_stats()["iters"] == stops[q], (q, doms[q].press_stats()["iters"])
+    b.synchronize()
+    _check(doms, alone)
+    b.close()
     for d in doms:
         d.close()
