@@ -1,6 +1,6 @@
 """Batched steps (les_batch_*): domains advanced together -- every stage of
-a step one launch for all of them, the SOR iterations 2.. two per launch
-(one per launch with LES_BATCH_SOR_PAIRS=0) -- give each domain exactly the results of stepping it alone (the
+a step one launch for all of them, the SOR iterations one per launch (two
+per launch with LES_BATCH_SOR_PAIRS=1) -- give each domain exactly the results of stepping it alone (the
 same result alone and in the mix), with eager steps and graph replays, with
 domains that step on their own streams in the mix (odd jm: SOR not
 batchable; les and adam unfused: stages not batched), with domains joining
@@ -139,6 +139,7 @@ def test_batch_pairs_stop_and_cap_per_domain(monkeypatch):
     api.release_all()
     alone = _alone(setups, [3] * len(setups), monkeypatch)
     doms = [api.Domain(s) for s in setups]
+    monkeypatch.setenv("LES_BATCH_SOR_PAIRS", "1")
     b = api.Batch(doms, graph=True)
     for n in range(1, 4):
         b.step([n] * len(doms), [3] * len(doms))
