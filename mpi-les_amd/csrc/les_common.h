@@ -110,9 +110,10 @@ struct VBlk {
 // XCD one contiguous range of the grid (x fastest, then y, then z): a tile's
 // neighbours in j and k then mostly run on the same XCD and share its L2.
 // Measured at 300x300x90: velnw 78 -> 67 us, les_visc 214 -> 195 us,
-// les_sm 95 -> 92 us; padj 23 -> 36 us and velfg and the reductions
-// unchanged or slower, so only the first three use it.  The work of a
-// position, and so every value and partial sum, is the same either way.
+// les_sm 95 -> 92 us, the rhs kernels the same time with a quarter less
+// traffic; padj 23 -> 36 us and velfg unchanged, so those two keep the
+// hardware order.  The work of a position, and so every value and partial
+// sum (partials are indexed by position), is the same either way.
 template <bool XCD>
 __device__ __forceinline__ VBlk hw_blk() {
   const int nx = gridDim.x, ny = gridDim.y, nz = gridDim.z;
