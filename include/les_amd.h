@@ -210,22 +210,23 @@ int les_domain_create(const les_domain_desc* desc, les_domain** out);
 int les_domain_destroy(les_domain* d);
 /* One time step n (main.f95:260-296): the seven stages in order. */
 int les_domain_step(les_domain* d, int n, int nmax);
-/* With on != 0, steps n > 1 replay one captured HIP graph of the whole
- * step (recaptured when the domain's constants change); results are the
- * same as eager launches.  Off by default; ignored while timing is on. */
 /* Picks the launch shape of the fused SOR iteration (kernel: rows = 4 runs
  * two iterations per launch on 32 x 64 tiles with a four-cell ring (even jm,
  * separable cn1), rows = 2 or 1 one iteration with two or one tile rows per
  * thread; tile rows x pairs with the ring of the one-iteration kernel: 16 x
  * 64, 32 x 64 or 32 x 32; k levels per block) for this grid shape: times
- * `reps` iterations of each
- * candidate on the domain's SOR scratch arrays and keeps the fastest.  The
- * choice is remembered per (im, jm, km) for the process; later calls with
- * reps <= 0 reuse it without timing.  Results do not depend on the shape.
- * Leaves the fields and the last press statistics unchanged; synchronises.
- * Reports the choice and the microseconds per iteration measured, when the
- * shape was timed, for it and for the untuned default.  No reference
- * counterpart: the per-shape autotuner of the serving path. */
+ * `reps` iterations of each candidate on the domain's SOR scratch arrays and
+ * keeps the fastest.  With fused stages it also picks the k levels per
+ * thread of velfg+feedbf, les_sm and les viscous+adam, timing `reps`
+ * launches of each candidate on the domain's own fields, which it copies to
+ * a temporary device allocation first and restores (les_domain_stencil_chunks
+ * reports them).  The choices are remembered per (im, jm, km) for the
+ * process; later calls with reps <= 0 reuse them without timing.  Results
+ * do not depend on them.  Leaves the fields and the last press statistics
+ * unchanged; synchronises.  Reports the SOR choice and the microseconds per
+ * iteration measured, when the shape was timed, for it and for the untuned
+ * default.  No reference counterpart: the per-shape autotuner of the serving
+ * path. */
 int les_domain_autotune(les_domain* d, int reps, int* rows, int* tile_rows, int* tile_width, int* kchunk,
                         double* us_best, double* us_default);
 /* The k levels per thread of the fused stencil kernels (velfg+feedbf,
@@ -245,6 +246,9 @@ int les_domain_stencil_chunks(les_domain* d, int* velfg_kchunk, int* sm_kchunk, 
  * synchronises.  No reference counterpart: the measurement of the dominant
  * kernel. */
 int les_domain_time_sor(les_domain* d, int iters, double* us_per_iter);
+/* With on != 0, steps n > 1 replay one captured HIP graph of the whole
+ * step (recaptured when the domain's constants change); results are the
+ * same as eager launches.  Off by default; ignored while timing is on. */
 int les_domain_set_graph(les_domain* d, int on);
 /* Advance steps n0..n1 inclusive. */
 int les_domain_run(les_domain* d, int n0, int n1, int nmax);
