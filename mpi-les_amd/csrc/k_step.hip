@@ -201,8 +201,9 @@ __global__ __launch_bounds__(256) void k_bondv1_ghost(Grid g, float* __restrict_
   bondv1_ghost_body(hw_blk<false>(), g, u, v, w);
 }
 
-// Grids of the four bondv1 launches (one-dimensional but the first, which
-// covers (i, j, k) at n == 1 and (j, k) for the two inflow planes after).
+// Grids of the four bondv1 launches of step 1 and of the batched steps
+// (one-dimensional but the first, which covers (i, j, k) at n == 1 and
+// (j, k) for the two inflow planes after).
 static dim3 bondv1_grid(const Grid& g, int ilast, int part) {
   const int nplane = g.jm * g.km;
   switch (part) {
@@ -213,9 +214,180 @@ static dim3 bondv1_grid(const Grid& g, int ilast, int part) {
   }
 }
 
+// Steps n > 1 in two launches instead of four.  The inflow planes i = 0, 1
+// and the outflow column i = im (im >= 2) do not overlap, so
+//   k_bondv1_in_uout writes the inflow planes (one thread per (j,k)) and, in
+//     the blocks after them, the block max/min of u(im,:,:) into the
+//     domain's partial slots (block b: max at b, min at 256+b, at most 256
+//     blocks);
+//   k_bondv1_out_ghost folds those (max and min are exact in any order) in
+//     each of its outflow blocks, applies the outflow with the j and k ghost
+//     copies of every cell it writes, and in the blocks after them writes
+//     the ghost rows of bondv1_ghost_body for every other cell.
+// Each cell gets the same expression on the same operands as in the four
+// launches, and its final value: w(im+1,j,km), which the outflow computes
+// and the top row then zeroes, is left to the zeroing.
+__global__ __launch_bounds__(256) void k_bondv1_in_uout(Grid g, Metrics m, float* __restrict__ u,
+                                                       float* __restrict__ v, float* __restrict__ w,
+                                                       double* __restrict__ part, int nin) {
+  const int b = blockIdx.x, tid = threadIdx.x;
+  const int n = g.jm * g.km;
+  if (b < nin) {  // bondv1.f95:33-54
+    const int t = b * 256 + tid;
+    if (t >= n) return;
+    const int j = 1 + t % g.jm, k = 1 + t / g.jm;
+    const int q = (j - 1) + (k - 1) * g.jm;
+    for (int i = 0; i <= 1; i++) {
+      const int c = ix(g, i, j, k);
+      u[c] = m.inflow ? m.inflow[q] : 5.0f;
+      v[c] = m.inflow_v ? m.inflow_v[q] : 0.0f;
+      w[c] = m.inflow_w ? m.inflow_w[q] : 0.0f;
+    }
+    return;
+  }
+  // bondv1.f95:98-114, as bondv1_uout_body, one partial per block
+  const int bu = b - nin, nu = (int)gridDim.x - nin;
+  __shared__ float smax[4], smin[4];
+  float mx = 0.0f, mn = 0.0f;
+  for (int t = bu * 256 + tid; t < n; t += nu * 256) {
+    const int j = 1 + t % g.jm, k = 1 + t / g.jm;
+    const float x = u[ix(g, g.im, j, k)];
+    mx = (x > mx) ? x : mx;
+    mn = (x < mn) ? x : mn;
+  }
+  for (int o = kWave / 2; o > 0; o >>= 1) {
+    const float a = __shfl_xor(mx, o, kWave), c = __shfl_xor(mn, o, kWave);
+    mx = (a > mx) ? a : mx;
+    mn = (c < mn) ? c : mn;
+  }
+  if (tid % kWave == 0) {
+    smax[tid / kWave] = mx;
+    smin[tid / kWave] = mn;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    for (int q = 1; q < 4; q++) {
+      mx = (smax[q] > mx) ? smax[q] : mx;
+      mn = (smin[q] < mn) ? smin[q] : mn;
+    }
+    part[bu] = (double)mx;  // exact both ways
+    part[256 + bu] = (double)mn;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_bondv1_out_ghost(Grid g, Metrics m, float* __restrict__ u,
+                                                         float* __restrict__ v, float* __restrict__ w,
+                                                         const double* __restrict__ part, int nu, int nout,
+                                                         float dt) {
+  const int b = blockIdx.x, tid = threadIdx.x;
+  const int im = g.im, jm = g.jm, km = g.km;
+  if (b < nout) {  // bondv1.f95:116-134 with the copies of bondv1.f95:137-156, 176-195
+    __shared__ float smax[4], smin[4];
+    float mx = 0.0f, mn = 0.0f;
+    if (tid < nu) {
+      mx = (float)part[tid];
+      mn = (float)part[256 + tid];
+    }
+    for (int o = kWave / 2; o > 0; o >>= 1) {
+      const float a = __shfl_xor(mx, o, kWave), c = __shfl_xor(mn, o, kWave);
+      mx = (a > mx) ? a : mx;
+      mn = (c < mn) ? c : mn;
+    }
+    if (tid % kWave == 0) {
+      smax[tid / kWave] = mx;
+      smin[tid / kWave] = mn;
+    }
+    __syncthreads();
+    for (int q = 0; q < 4; q++) {
+      mx = (smax[q] > mx) ? smax[q] : mx;
+      mn = (smin[q] < mn) ? smin[q] : mn;
+    }
+    const int t = b * 256 + tid;
+    if (t >= jm * km) return;
+    const int j = 1 + t % jm, k = 1 + t / jm;
+    const float aaa = mx;
+    const float bbb = -(0.0f - mn);  // the sign-bit round trip of bondv1_uout_body's accumulator
+    const float uout = (aaa + bbb) / 2.0f;
+    const float dxs = M1(m.dxs, im);
+    int c = ix(g, im, j, k);
+    const float nu_ = u[c] - dt * uout * (u[c] - u[c - 1]) / dxs;
+    c = ix(g, im + 1, j, k);
+    const float nv = v[c] - dt * uout * (v[c] - v[c - 1]) / dxs;
+    float nw = 0.0f;
+    if (k <= km - 1) nw = w[c] - dt * uout * (w[c] - w[c - 1]) / dxs;
+    // the rows holding this cell's value: its own, and j = 0 (the periodic
+    // copy of jm) or j = jm+1 (of 1)
+    const int jr1 = (j == jm) ? 0 : ((j == 1) ? jm + 1 : -2);
+    for (int r = 0; r < 2; r++) {
+      const int jj = r == 0 ? j : jr1;
+      if (jj < 0) continue;
+      u[ix(g, im, jj, k)] = nu_;
+      v[ix(g, im + 1, jj, k)] = nv;
+      if (k <= km - 1) w[ix(g, im + 1, jj, k)] = nw;
+      if (k == 1) {
+        u[ix(g, im, jj, 0)] = -nu_;
+        v[ix(g, im + 1, jj, 0)] = -nv;
+      }
+      if (k == km) {
+        u[ix(g, im, jj, km + 1)] = nu_;
+        v[ix(g, im + 1, jj, km + 1)] = nv;
+      }
+    }
+    return;
+  }
+  // bondv1_ghost_body for the cells the outflow blocks do not write: u at
+  // i = im and v, w at i = im+1 are theirs (w's top row excepted)
+  const int ni = im + 2;
+  const int nA = ni * km, nB = ni * (jm + 3);
+  int t = (b - nout) * 256 + tid;
+  if (t < nA) {
+    const int i = t % ni, k = 1 + t / ni;
+    const int c0 = ix(g, i, 0, k), cjm = ix(g, i, jm, k), c1 = ix(g, i, 1, k), cj1 = ix(g, i, jm + 1, k);
+    if (i != im) {
+      u[c0] = u[cjm];
+      u[cj1] = u[c1];
+    }
+    if (i != im + 1) {
+      v[c0] = v[cjm];
+      v[cj1] = v[c1];
+      if (k <= km - 1) {
+        w[c0] = w[cjm];
+        w[cj1] = w[c1];
+      }
+    }
+    return;
+  }
+  t -= nA;
+  if (t >= nB) return;
+  const int i = t % ni, j = -1 + t / ni;
+  if (j >= 0) {
+    const int cj = (j == 0) ? jm : ((j == jm + 1) ? 1 : j);
+    if (i != im) {
+      u[ix(g, i, j, 0)] = -u[ix(g, i, cj, 1)];
+      u[ix(g, i, j, km + 1)] = u[ix(g, i, cj, km)];
+    }
+    if (i != im + 1) {
+      v[ix(g, i, j, 0)] = -v[ix(g, i, cj, 1)];
+      v[ix(g, i, j, km + 1)] = v[ix(g, i, cj, km)];
+    }
+  }
+  w[ix(g, i, j, 0)] = 0.0f;
+  w[ix(g, i, j, km)] = 0.0f;
+}
+
 void launch_bondv1(const DevDomain& d, int n, float dt) {
   const Grid& g = d.g;
   const int ilast = (n == 1) ? g.im : 1;
+  if (ilast == 1) {  // steps n > 1: two launches
+    const int nplane = g.jm * g.km;
+    const int nin = (nplane + 255) / 256, nu = std::min(nin, 256);
+    const int ngh = ((g.im + 2) * (g.km + g.jm + 3) + 255) / 256;
+    TIMED(d, KC_BONDV1, hipLaunchKernelGGL(k_bondv1_in_uout, dim3((unsigned)(nin + nu)), dim3(256), 0, d.stream,
+                                           g, d.m, d.F.u, d.F.v, d.F.w, d.partials, nin));
+    TIMED(d, KC_BONDV1, hipLaunchKernelGGL(k_bondv1_out_ghost, dim3((unsigned)(nin + ngh)), dim3(256), 0, d.stream,
+                                           g, d.m, d.F.u, d.F.v, d.F.w, (const double*)d.partials, nu, nin, dt));
+    return;
+  }
   TIMED(d, KC_BONDV1, hipLaunchKernelGGL(k_bondv1_inflow, bondv1_grid(g, ilast, 0), block2d(), 0, d.stream, g, d.m,
                      d.F.u, d.F.v, d.F.w, ilast, d.ctrl));
   TIMED(d, KC_BONDV1, hipLaunchKernelGGL(k_bondv1_uout, bondv1_grid(g, ilast, 1), dim3(256), 0, d.stream, g, d.F.u,
@@ -543,7 +715,8 @@ __device__ __forceinline__ void velfg_feedbf_body(
     const VBlk& vb, const VelCtx& X, const P10& diu, float* __restrict__ f, float* __restrict__ gg,
     float* __restrict__ h, float* __restrict__ usum, float* __restrict__ vsum, float* __restrict__ wsum,
     const float* __restrict__ bm, const float* __restrict__ cm, const float* __restrict__ dm,
-    float* __restrict__ uspd, float* __restrict__ vspd, float vn, float alpha, float dt, float beta, int kchunk) {
+    const float* __restrict__ ms, float* __restrict__ uspd, float* __restrict__ vspd, float vn, float alpha,
+    float dt, float beta, int kchunk) {
   const Grid& g = X.g;
   const Metrics& m = X.m;
   const int i = 1 + vb.x * BX + threadIdx.x, j = 1 + vb.y * BY + threadIdx.y;
@@ -555,6 +728,12 @@ __device__ __forceinline__ void velfg_feedbf_body(
   const bool ilast = (i == im);
   const float dxi = X.dx(i), dxp = X.dx(i + 1);
   const float dyj = X.dy(j), dyp = X.dy(j + 1);
+  // Masks built from the height map (ms: zbm(i,j) at (i-1)+(j-1)*im, then
+  // each level's threshold): the three are equal and are 1 exactly where
+  // zbm(i,j) exceeds the threshold of level k, k_masks' expression, so the
+  // three mask fields need not be read.  ms is null when the masks were given.
+  const float* const mtop = ms ? ms + (size_t)im * jm : nullptr;
+  const float zb = ms ? ms[(i - 1) + (j - 1) * im] : 0.0f;
   float* const* D = diu.a;
   auto put = [&](int q, int ii, int jj, int kk, float val) { D[q][ix(g, ii, jj, kk)] = val; };
   auto cpy = [&](int q, int ii, int jj, int kk, int si, int sj, int sk) {
@@ -628,7 +807,17 @@ __device__ __forceinline__ void velfg_feedbf_body(
     }
     // ---- feedbf.f95:35-61
     {
-      const float uc = X.u[c], vc = X.v[c], wc = X.w[c], b = bm[c], cc = cm[c], dd = dm[c];
+      const float uc = X.u[c], vc = X.v[c], wc = X.w[c];
+      float b, cc, dd;
+      if (ms) {
+        b = (zb > mtop[k - 1]) ? 1.0f : 0.0f;
+        cc = b;
+        dd = b;
+      } else {
+        b = bm[c];
+        cc = cm[c];
+        dd = dm[c];
+      }
       const float su = (usum[c] + uc) * b, sv = (vsum[c] + vc) * cc, sw = (wsum[c] + wc) * dd;
       usum[c] = su;
       vsum[c] = sv;
@@ -707,10 +896,11 @@ __global__ __launch_bounds__(NT) void k_velfg_feedbf(VelCtx X, P10 diu, float* _
                                                      float* __restrict__ usum, float* __restrict__ vsum,
                                                      float* __restrict__ wsum, const float* __restrict__ bm,
                                                      const float* __restrict__ cm, const float* __restrict__ dm,
-                                                     float* __restrict__ uspd, float* __restrict__ vspd, float vn,
-                                                     float alpha, float dt, float beta, int kchunk) {
-  velfg_feedbf_body(hw_blk<false>(), X, diu, f, gg, h, usum, vsum, wsum, bm, cm, dm, uspd, vspd, vn, alpha, dt, beta,
-                    kchunk);
+                                                     const float* __restrict__ ms, float* __restrict__ uspd,
+                                                     float* __restrict__ vspd, float vn, float alpha, float dt,
+                                                     float beta, int kchunk) {
+  velfg_feedbf_body(hw_blk<false>(), X, diu, f, gg, h, usum, vsum, wsum, bm, cm, dm, ms, uspd, vspd, vn, alpha, dt,
+                    beta, kchunk);
 }
 
 // Grid of k_velfg_feedbf and its k levels per thread.
@@ -735,7 +925,7 @@ void launch_velfg_feedbf(const DevDomain& d, float vn, float alpha, float dt, fl
   VelCtx X{d.g, d.m, d.F.u, d.F.v, d.F.w, d.wl};
   TIMED(d, KC_VEL2, hipLaunchKernelGGL(k_velfg_feedbf, grid, block2d(), 0, d.stream, X, p10(d.F.diu),
                                         d.F.f, d.F.g, d.F.h, d.F.usum, d.F.vsum, d.F.wsum, d.F.bmask, d.F.cmask,
-                                        d.F.dmask, d.F.uspd, d.F.vspd, vn, alpha, dt, beta, kc));
+                                        d.F.dmask, d.mask_src, d.F.uspd, d.F.vspd, vn, alpha, dt, beta, kc));
 }
 
 // ----------------------------------------------------------------- feedbf
@@ -1254,7 +1444,7 @@ __global__ __launch_bounds__(NT) void kb_velfg_feedbf(const BatchDom* __restrict
   BATCH_BLOCK(BS_VELFG, false);
   const VelCtx X{d.g, d.m, d.F.u, d.F.v, d.F.w, d.wl};
   velfg_feedbf_body(vb, X, p10(d.F.diu), d.F.f, d.F.g, d.F.h, d.F.usum, d.F.vsum, d.F.wsum, d.F.bmask, d.F.cmask,
-                    d.F.dmask, d.F.uspd, d.F.vspd, B.vn, B.alpha, B.dt, B.beta, B.velfg_kc);
+                    d.F.dmask, d.mask_src, d.F.uspd, d.F.vspd, B.vn, B.alpha, B.dt, B.beta, B.velfg_kc);
 }
 __global__ __launch_bounds__(NT) void kb_les_sm(const BatchDom* __restrict__ bd, const StageSpan* __restrict__ sp,
                                                 int nd) {

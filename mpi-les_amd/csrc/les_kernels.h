@@ -53,7 +53,7 @@ struct DevDomain {
   DevFields F;
   Ctrl* ctrl;
   double* partials;  // per-block partials of the reductions outside the SOR: sums, then per-block flags
-  int npartials;     // slots before the SOR's partial slots (twice the largest reduction grid)
+  int npartials;     // slots before the SOR's partial slots (twice the largest reduction grid, at least 512)
   hipStream_t stream;
   float wl;          // logf(0.5*dzn(1)/0.1), the wall-function log (vel2.f95:138, les.f95:133)
   double area;       // sum of dx1*dy1*dzn over the interior (press.f95:63-72)
@@ -85,6 +85,9 @@ struct DevDomain {
   int sor_tile_rows; // tile rows (with the ring) of k_sor_iter2: 16 or 32
   int sor_tile_width;  // tile width in pairs (with the ring) of k_sor_iter2: 64, or 32 with 32 rows
   int sor_entry_fast;  // SOR iteration 1 on k_sor_iter2 when p's ghosts equal their boundp copies
+  // masks built on the device from the height map: zbm(i,j) at (i-1)+(j-1)*im,
+  // then the km level thresholds; nullptr when the masks were given as fields
+  const float* mask_src;
 };
 
 int sor_kchunk(const Grid& g, const SplitGrid& s, int target);

@@ -286,6 +286,7 @@ struct les_domain {
   bool mdirty = true;
   bool have_inflow = false;
   float* split = nullptr;           // colour-split SOR arrays
+  float* mask_src = nullptr;        // height map and level thresholds of device-built masks
   int use_graph = 0;                // replay steps n > 1 from a captured graph
   hipGraphExec_t gexec = nullptr;   // captured step
   DevDomain gkey{};                 // domain state the graph was captured with
@@ -414,6 +415,14 @@ int upload_field(les_domain* D, int f, const float* host) {
   launch_unpack(D->d, D->staging, D->fields[f], K.li, K.lj, K.lk, ni, nj, nk);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipStreamSynchronize(D->d.stream));  // the host array may change after return
+  if ((f == FBMASK || f == FCMASK || f == FDMASK) && D->mask_src) {
+    // a given mask replaces the height map's: k_velfg_feedbf reads the mask
+    // fields again (the changed DevDomain also retires a captured step)
+    D->d.mask_src = nullptr;
+    dev_release(D->mask_src);
+    D->mask_src = nullptr;
+    D->bytes -= ((long long)D->d.g.im * D->d.g.jm + D->d.g.km) * (long long)sizeof(float);
+  }
   return 0;
 }
 
@@ -548,6 +557,7 @@ void domain_free(les_domain* D) {
   for (int f = 0; f < NFIELDS; f++) dev_release(D->fields[f]);
   for (int q = 1; q <= 9; q++) dev_release(D->cov[q]);
   dev_release(D->split);
+  dev_release(D->mask_src);
   dev_release(D->mbuf);
   dev_release(D->inflow_dev);
   dev_release(D->staging);
@@ -646,7 +656,8 @@ int domain_alloc(int im, int jm, int km, les_domain** out) {
   m.cncz = D->mbuf + D->moff[MCNCZ];
   m.inflow = nullptr;
   // reduction scratch: reduction partials, then the four SOR partial slots
-  D->d.npartials = 2 * reduction_parts_needed(D->d.g);
+  // at least the 2 x 256 slots of the bondv1 outflow's block max/min
+  D->d.npartials = std::max(2 * reduction_parts_needed(D->d.g), 512);
   D->d.sor_blocks = env_int("LES_SOR_BLOCKS", 2048);
   D->d.xcd_remap = env_int("LES_XCD_REMAP", 1);
   D->d.sor_fused = env_int("LES_SOR_FUSED", 1);
@@ -1449,6 +1460,7 @@ int build_masks(les_domain* D, const float* zbm, const float* z2, const float* d
   float* buf = nullptr;
   const size_t nz = (size_t)g.im * g.jm;
   HIP_TRY(dev_alloc(&buf, (nz + g.km) * sizeof(float)));
+  D->bytes += (long long)((nz + g.km) * sizeof(float));
   hipError_t e = hipMemcpyAsync(buf, zbm, nz * sizeof(float), hipMemcpyHostToDevice, D->d.stream);
   if (e == hipSuccess)
     e = hipMemcpyAsync(buf + nz, top.data(), g.km * sizeof(float), hipMemcpyHostToDevice, D->d.stream);
@@ -1457,7 +1469,9 @@ int build_masks(les_domain* D, const float* zbm, const float* z2, const float* d
     e = hipGetLastError();
   }
   if (e == hipSuccess) e = hipStreamSynchronize(D->d.stream);
-  dev_release(buf);
+  // kept: k_velfg_feedbf evaluates the masks from it instead of reading them
+  D->mask_src = buf;
+  D->d.mask_src = buf;
   if (e != hipSuccess) return fail(std::string("mask generation: ") + hipGetErrorString(e));
   return 0;
 }
@@ -1579,6 +1593,7 @@ long long les_domain_bytes_needed(int im, int jm, int km) {
   long long b = 0;
   for (int f = 0; f < kLazyFirst; f++) b += (kFields[f].kind == K2D ? sk : n3) * 4;
   b += (10 * cs + 2 * cp * jm) * 4;  // colour-split SOR arrays and their two scratch planes
+  b += ((long long)im * jm + km) * 4;  // the height map and level thresholds of device-built masks
   return b + (1LL << 20);  // metrics, partial sums, control block (bounded by 1 MiB here)
 }
 
