@@ -969,6 +969,39 @@ void launch_feedbf(const DevDomain& d, float alpha, float dt, float beta) {
 
 // -------------------------------------------------------------------- les
 // Eddy viscosity sm (les.f95:45-65); delx1 arrives precomputed per level.
+// boundsm.f95:18-64 for the ghost cells whose composite copy (boundsm_body)
+// reads interior cell (i,j,k), whose new value is val: its images on the
+// i, j and k faces it touches, negated at k = 0.  The threads of row j = 1
+// also write the ghosts of row j = -1, whose copies read row j = -1 itself,
+// which nothing writes.
+__device__ __forceinline__ void sm_ghosts(const Grid& g, int i, int j, int k, float val, float* __restrict__ sm) {
+  constexpr int kNone = -1000;
+  const int ie = (i == 1) ? 0 : ((i == g.im) ? g.im + 1 : kNone);
+  const int je = (j == 1) ? 0 : ((j == g.jm) ? g.jm + 1 : kNone);
+  const int ke = (k == 1) ? 0 : ((k == g.km) ? g.km + 1 : kNone);
+  if (ie == kNone && je == kNone && ke == kNone) return;
+  for (int a = 0; a < 2; a++) {
+    const int gi = a ? ie : i;
+    if (gi == kNone) continue;
+    for (int c = 0; c < 2; c++) {
+      const int gk = c ? ke : k;
+      if (gk == kNone) continue;
+      for (int b = 0; b < 2; b++) {
+        const int gj = b ? je : j;
+        if (gj == kNone || (a | b | c) == 0) continue;
+        sm[ix(g, gi, gj, gk)] = (gk == 0) ? -val : val;
+      }
+      if (j == 1 && (a | c) != 0) {
+        const float s = sm[ix(g, i, -1, k)];
+        sm[ix(g, gi, -1, gk)] = (gk == 0) ? -s : s;
+      }
+    }
+  }
+}
+
+// GHOSTS: also write the ghost cells boundsm sets (k_les_sm of a domain's
+// own step; the batched step keeps its boundsm launch).
+template <bool GHOSTS>
 __device__ __forceinline__ void les_sm_body(const VBlk& vb, const Grid& g, const P10& diu,
                                             const float* __restrict__ delx1, float* __restrict__ sm) {
   const int i = 1 + vb.x * BX + threadIdx.x, j = 1 + vb.y * BY + threadIdx.y;
@@ -1013,12 +1046,15 @@ __device__ __forceinline__ void les_sm_body(const VBlk& vb, const Grid& g, const
     const float s = 2.0f * (dudxx1 * dudxx1 + dvdyx1 * dvdyx1 + dwdzx1 * dwdzx1) +
                     (dudyx1 + dvdxx1) * (dudyx1 + dvdxx1) + (dwdyx1 + dvdzx1) * (dwdyx1 + dvdzx1) +
                     (dudzx1 + dwdxx1) * (dudzx1 + dwdxx1);
-    sm[c] = a * a * sqrtf(s);
+    const float val = a * a * sqrtf(s);
+    sm[c] = val;
+    if (GHOSTS) sm_ghosts(g, i, j, k, val, sm);
   }
 }
+// les.f95:44-79 with boundsm.f95:18-64
 __global__ __launch_bounds__(NT) void k_les_sm(Grid g, P10 diu, const float* __restrict__ delx1,
                                                float* __restrict__ sm) {
-  les_sm_body(hw_blk<true>(), g, diu, delx1, sm);
+  les_sm_body<true>(hw_blk<true>(), g, diu, delx1, sm);
 }
 // Grid of a stencil kernel whose threads walk runs of kchunk levels
 // (0: automatic, about 36 blocks per compute unit, as velfg).
@@ -1088,7 +1124,6 @@ __device__ __forceinline__ void boundsm_body(const VBlk& vb, const Grid& g, floa
   const float s = sm[ix(g, ii, jj, kk)];
   sm[ix(g, i, j, k)] = (sgn < 0.0f) ? -s : s;
 }
-__global__ __launch_bounds__(256) void k_boundsm(Grid g, float* __restrict__ sm) { boundsm_body(hw_blk<false>(), g, sm); }
 
 // Viscous terms added to f (k=2..km, wall k=1), g (same) and h (k=1..km),
 // les.f95:83-242.
@@ -1249,8 +1284,6 @@ void launch_les(const DevDomain& d) {
   const P10 diu = p10(d.F.diu);
   TIMED(d, KC_LES_SM, hipLaunchKernelGGL(k_les_sm, krun_grid(d, d.sm_kchunk), block2d(), 0, d.stream, g, diu,
                      d.m.delx1 + 1, d.F.sm));
-  TIMED(d, KC_BOUNDSM, hipLaunchKernelGGL(k_boundsm, dim3((unsigned)boundsm_blocks(g)), dim3(256), 0, d.stream, g,
-                                          d.F.sm));
   TIMED(d, KC_LES_VISC, hipLaunchKernelGGL(k_les_visc<false>, krun_grid(d, d.visc_kchunk), block2d(), 0, d.stream, g,
                      d.m, diu, d.F.sm, d.F.uspd, d.F.vspd, d.F.f, d.F.g, d.F.h, nullptr, nullptr, nullptr, d.wl));
 }
@@ -1261,8 +1294,6 @@ void launch_les_adam(const DevDomain& d) {
   const P10 diu = p10(d.F.diu);
   TIMED(d, KC_LES_SM, hipLaunchKernelGGL(k_les_sm, krun_grid(d, d.sm_kchunk), block2d(), 0, d.stream, g, diu,
                      d.m.delx1 + 1, d.F.sm));
-  TIMED(d, KC_BOUNDSM, hipLaunchKernelGGL(k_boundsm, dim3((unsigned)boundsm_blocks(g)), dim3(256), 0, d.stream, g,
-                                          d.F.sm));
   TIMED(d, KC_LES_VISC, hipLaunchKernelGGL(k_les_visc<true>, krun_grid(d, d.visc_kchunk), block2d(), 0, d.stream, g,
                      d.m, diu, d.F.sm, d.F.uspd, d.F.vspd, d.F.f, d.F.g, d.F.h, d.F.fold, d.F.gold, d.F.hold,
                      d.wl));
@@ -1449,7 +1480,7 @@ __global__ __launch_bounds__(NT) void kb_velfg_feedbf(const BatchDom* __restrict
 __global__ __launch_bounds__(NT) void kb_les_sm(const BatchDom* __restrict__ bd, const StageSpan* __restrict__ sp,
                                                 int nd) {
   BATCH_BLOCK(BS_LES_SM, true);
-  les_sm_body(vb, d.g, p10(d.F.diu), d.m.delx1 + 1, d.F.sm);
+  les_sm_body<false>(vb, d.g, p10(d.F.diu), d.m.delx1 + 1, d.F.sm);
 }
 __global__ __launch_bounds__(256) void kb_boundsm(const BatchDom* __restrict__ bd, const StageSpan* __restrict__ sp,
                                                   int nd) {
