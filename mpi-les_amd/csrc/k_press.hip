@@ -586,14 +586,17 @@ __device__ __forceinline__ void sor_iter_block(const Grid& g, const SplitGrid& s
   const unsigned ocR = SEP ? 0u : boff(base, sb.cn1[1]), ocB = SEP ? 0u : boff(base, sb.cn1[0]);
   const int lb = remap ? xcd_contiguous(bid, nparts) : bid;
   const int bt = lb % ntt, bj = (lb / ntt) % ntj, bk = lb / (ntt * ntj);
-  const int t = bt * FTX - 2 + lane;
+  // a tile as wide as the domain has no ring lanes: its i edges are the
+  // walls, whose neighbours are mirrored
+  const bool whole = ntt == 1 && nt <= FX;
+  const int t = whole ? lane : bt * FTX - 2 + lane;
   const int jl = 1 + bj * FTY - 2 + row;  // row of the tile (may lie outside 1..jm)
   const int j = jwrap(jl, jm);            // the row it stands for
   const int k0 = 1 + bk * kchunk, k1 = min(km, k0 + kchunk - 1);
   const bool tin = (t >= 0) && (t < nt);
   const bool row_red = row >= 1 && row <= FY - 2;               // wave-uniform
   const bool row_tile = row >= 2 && row <= FY - 3 && jl <= jm;  // wave-uniform
-  const bool lane_tile = tin && lane >= 2 && lane <= FX - 3;
+  const bool lane_tile = tin && (whole || (lane >= 2 && lane <= FX - 3));
   const bool jodd = (jm & 1) != 0;
   const bool wlo = jodd && j == 1, whi = jodd && j == jm;  // same-colour periodic neighbours
   const int wofs = (jm - 1) * cp;
@@ -837,7 +840,10 @@ __device__ __forceinline__ void sor_iter2_block(Grid g, SplitGrid s, Metrics m, 
   const unsigned ocR = SEP ? 0u : boff(base, sb.cn1[1]) - cplb, ocB = SEP ? 0u : boff(base, sb.cn1[0]) - cplb;
   const int lb = remap ? xcd_contiguous(bid, nparts) : bid;
   const int bt = lb % ntt, bj = (lb / ntt) % ntj, bk = lb / (ntt * ntj);
-  const int t = bt * FTXW - 2 + lane;
+  // a tile as wide as the domain has no ring lanes: its i edges are the
+  // walls, whose neighbours are mirrored
+  const bool whole = ntt == 1 && nt <= W;
+  const int t = whole ? lane : bt * FTXW - 2 + lane;
   const bool tin = (t >= 0) && (t < nt);
   const int rA = 2 * vw, rB = rA + 1;
   const int jlA = 1 + bj * FTYT - 2 + rA, jlB = jlA + 1;  // tile rows (may lie outside 1..jm)
@@ -847,7 +853,7 @@ __device__ __forceinline__ void sor_iter2_block(Grid g, SplitGrid s, Metrics m, 
   const int tq = tin ? t : 0;
   const unsigned voA = 4u * (unsigned)(tq + (jA - 1) * cp), voB = 4u * (unsigned)(tq + (jB - 1) * cp);
   const unsigned trA = 4u * (unsigned)(nt + (jA - 1) * cp), trB = 4u * (unsigned)(nt + (jB - 1) * cp);
-  const bool lane_tile = tin && lane >= 2 && lane <= W - 3;
+  const bool lane_tile = tin && (whole || (lane >= 2 && lane <= W - 3));
   const bool tileA = rA >= 2 && rA <= TY - 3 && jlA <= jm, tileB = rB >= 2 && rB <= TY - 3 && jlB <= jm;
   // per position p of the pair (i = 2t+1+p)
   float c2l[2] = {0.f, 0.f}, c2s[2] = {0.f, 0.f}, axp[2] = {0.f, 0.f};
@@ -1270,7 +1276,10 @@ __device__ __forceinline__ void sor_pair_block(const Grid& g, const SplitGrid& s
   const unsigned orR = boff(base, sb.rhs[1]) - cplb, orB = boff(base, sb.rhs[0]) - cplb;
   const int lb = remap ? xcd_contiguous(bid, nparts) : bid;
   const int bt = lb % ntt, bj = (lb / ntt) % ntj, bk = lb / (ntt * ntj);
-  const int t = bt * FTXW - R + lane;
+  // a tile as wide as the domain has no ring lanes: its i edges are the
+  // walls, whose neighbours are mirrored
+  const bool whole = ntt == 1 && nt <= W;
+  const int t = whole ? lane : bt * FTXW - R + lane;
   const bool tin = (t >= 0) && (t < nt);
   const int rA = 2 * w, rB = rA + 1;
   const int jlA = 1 + bj * FTYT - R + rA, jlB = jlA + 1;  // tile rows (may lie outside 1..jm); jlA odd
@@ -1285,7 +1294,7 @@ __device__ __forceinline__ void sor_pair_block(const Grid& g, const SplitGrid& s
   const int tq = tin ? t : 0;
   const unsigned voA = 4u * (unsigned)(tq + (jA - 1) * cp), voB = 4u * (unsigned)(tq + (jB - 1) * cp);
   const unsigned trA = 4u * (unsigned)(nt + (jA - 1) * cp), trB = 4u * (unsigned)(nt + (jB - 1) * cp);
-  const bool lane_tile = tin && lane >= R && lane <= W - 1 - R;
+  const bool lane_tile = tin && (whole || (lane >= R && lane <= W - 1 - R));
   const bool tileA = rA >= R && rA <= TY - 1 - R && jlA <= jm, tileB = rB >= R && rB <= TY - 1 - R && jlB <= jm;
   // waves whose rows a stage's result is needed on (wave-uniform): B on
   // rows 2..TY-3, C on 3..TY-4, D on the tile rows 4..TY-5
@@ -2013,7 +2022,7 @@ static SorShape sor_shape(const Grid& g, const SplitGrid& s, int rows, int ty, i
     h.ty = PTY;
     h.tw = FX;
     h.kc = std::max(1, std::min(kc, g.km));
-    h.ntt = (s.nt + PFTX - 1) / PFTX;
+    h.ntt = (s.nt <= FX) ? 1 : (s.nt + PFTX - 1) / PFTX;  // one tile without ring lanes when the width fits
     h.ntj = (g.jm + PFTY - 1) / PFTY;
     h.nparts = h.ntt * h.ntj * ((g.km + h.kc - 1) / h.kc);
     return h;
@@ -2021,7 +2030,8 @@ static SorShape sor_shape(const Grid& g, const SplitGrid& s, int rows, int ty, i
   h.ty = (h.rows == 1) ? FY : ty;          // k_sor_iter has 16-row tiles
   h.tw = (h.rows == 2 && h.ty == 32 && tw == FX / 2) ? FX / 2 : FX;
   h.kc = std::max(1, std::min(kc, g.km));
-  h.ntt = (s.nt + (h.tw - 4) - 1) / (h.tw - 4);
+  // one tile without ring lanes when the width fits
+  h.ntt = (s.nt <= h.tw) ? 1 : (s.nt + (h.tw - 4) - 1) / (h.tw - 4);
   h.ntj = (g.jm + (h.ty - 4) - 1) / (h.ty - 4);
   h.nparts = h.ntt * h.ntj * ((g.km + h.kc - 1) / h.kc);
   return h;
@@ -2596,7 +2606,7 @@ int sor_parts_needed(const Grid& g, const SplitGrid& s, int target) {
 // blocks.  target <= 0: the split that minimises (rounds of resident blocks)
 // x (steps per block), with two 1024-thread blocks resident per CU.
 int sor_fused_kchunk(const Grid& g, const SplitGrid& s, int target, int ncu) {
-  const int ntt = (s.nt + FTX - 1) / FTX, ntj = (g.jm + FTY - 1) / FTY;
+  const int ntt = s.nt <= FX ? 1 : (s.nt + FTX - 1) / FTX, ntj = (g.jm + FTY - 1) / FTY;
   const int tiles = ntt * ntj;
   if (target > 0) {
     int tz = target / tiles;
@@ -2651,7 +2661,7 @@ void launch_sor_trial(const DevDomain& d0, int rows, int ty, int tw, int kc, int
 // (rounds of resident blocks) x (steps per block), one 1024-thread block per
 // CU, each block marching its chunk and six more levels.
 int sor_pair_kchunk(const Grid& g, const SplitGrid& s, int ncu) {
-  const int tiles = ((s.nt + PFTX - 1) / PFTX) * ((g.jm + PFTY - 1) / PFTY);
+  const int tiles = (s.nt <= FX ? 1 : (s.nt + PFTX - 1) / PFTX) * ((g.jm + PFTY - 1) / PFTY);
   const long slots = ncu > 0 ? ncu : 256;
   long best = -1;
   int best_kc = g.km;

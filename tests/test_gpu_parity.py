@@ -161,6 +161,9 @@ FUSED_CASES = [
     (dict(im=250, jm=30, km=20, seed=18), 640),  # three pair tiles, three row tiles, even jm
     (dict(im=121, jm=14, km=17, seed=19), 0),    # automatic k chunks, odd im, even jm
     (dict(im=300, jm=24, km=90, seed=20), 0),    # the 300-wide rows of the bench domain
+    (dict(im=128, jm=27, km=10, seed=21), 0),    # 64 pairs: one tile as wide as the domain, odd jm
+    (dict(im=127, jm=20, km=9, seed=22), 640),   # 64 pairs, odd im, even jm
+    (dict(im=66, jm=18, km=7, seed=23), 0),      # 33 pairs: one whole-width tile of 64, two of 32
 ]
 
 

@@ -49,7 +49,9 @@ CASES = [
     (dict(im=9, jm=2, km=5, seed=35), 1, 9),        # jm = 2, one level per block, odd cap
     (dict(im=300, jm=24, km=90, seed=36), 0, 50),   # 300-wide rows of the benchmark domain
     (dict(im=130, jm=50, km=12, seed=37), 4, 1),    # a cap of one iteration
-    (dict(im=120, jm=48, km=9, seed=38), 2, 4),     # 60 pairs: a whole tile of 56 and 4 more
+    (dict(im=120, jm=48, km=9, seed=38), 2, 4),     # 60 pairs: one tile as wide as the domain
+    (dict(im=128, jm=20, km=12, seed=39), 0, 50),   # 64 pairs: the widest single tile
+    (dict(im=129, jm=22, km=9, seed=40), 3, 7),     # 65 pairs: two tiles with their rings
 ]
 
 

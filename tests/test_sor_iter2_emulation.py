@@ -147,7 +147,8 @@ def run(im, jm, km, kc, ty=16, seed=0, omega=1.0, remap=1, tw=FX):
                     ref_sum += float(r) * float(r)
 
     # ---------------- emulation of the kernel
-    ntt, ntj = (nt + FTXW - 1) // FTXW, (jm + FTY - 1) // FTY
+    whole = nt <= W  # one tile spans the width: no ring lanes (the i walls are mirrored)
+    ntt, ntj = (1 if whole else (nt + FTXW - 1) // FTXW), (jm + FTY - 1) // FTY
     nz = (km + kc - 1) // kc
     nparts = ntt * ntj * nz
     cplb = 4 * cpl
@@ -182,7 +183,7 @@ def run(im, jm, km, kc, ty=16, seed=0, omega=1.0, remap=1, tw=FX):
     for bid in range(nparts):
         lb = xcd(bid, nparts) if remap else bid
         bt, bj, bk = lb % ntt, (lb // ntt) % ntj, lb // (ntt * ntj)
-        t = bt * FTXW - 2 + lane
+        t = lane if whole else bt * FTXW - 2 + lane
         tin = (t >= 0) & (t < nt)
         rA = 2 * wv
         rB = rA + 1
@@ -200,7 +201,7 @@ def run(im, jm, km, kc, ty=16, seed=0, omega=1.0, remap=1, tw=FX):
         voB = 4 * (tq + (jB - 1) * cp)
         trA = 4 * (nt + (jA - 1) * cp)
         trB = 4 * (nt + (jB - 1) * cp)
-        lane_tile = tin & (lane >= 2) & (lane <= W - 3)
+        lane_tile = tin & (whole | ((lane >= 2) & (lane <= W - 3)))
         tileA = (rA >= 2) & (rA <= FY - 3) & (jlA <= jm)
         tileB = (rB >= 2) & (rB <= FY - 3) & (jlB <= jm)
         c2lp = [np.where(tin, c2l[np.clip(2 * t + 1 + p, -1, im + 2) + 1], 0).astype(f32) for p in (0, 1)]
@@ -379,7 +380,8 @@ def run(im, jm, km, kc, ty=16, seed=0, omega=1.0, remap=1, tw=FX):
 
 
 CASES = [(20, 14, 9, 9), (20, 14, 9, 3), (21, 14, 9, 4), (130, 26, 12, 5), (7, 2, 2, 1), (9, 4, 3, 2),
-         (125, 30, 20, 7), (64, 16, 13, 13), (3, 2, 5, 5), (121, 12, 8, 6), (70, 60, 11, 4)]
+         (125, 30, 20, 7), (64, 16, 13, 13), (3, 2, 5, 5), (121, 12, 8, 6), (70, 60, 11, 4),
+         (128, 18, 6, 3), (127, 10, 5, 5), (129, 8, 4, 2)]  # 64 pairs: one whole-width tile; 65: two
 
 
 @pytest.mark.parametrize("ty,tw", [(16, FX), (32, FX), (32, FX // 2)])

@@ -149,7 +149,8 @@ def run_launch(pb, mem, kc, iter_, two, in_set, out_set, ent, remap=1):
     """Emulate one launch of k_sor_pair on the allocation `mem` (updated in
     place).  Returns (written, sum1, sum2, nparts)."""
     im, jm, km, nt, cp, cpl = pb.im, pb.jm, pb.km, pb.nt, pb.cp, pb.cpl
-    ntt, ntj = (nt + FTXW - 1) // FTXW, (jm + FTY - 1) // FTY
+    whole = nt <= FX  # one tile spans the width: no ring lanes (the i walls are mirrored)
+    ntt, ntj = (1 if whole else (nt + FTXW - 1) // FTXW), (jm + FTY - 1) // FTY
     nz = (km + kc - 1) // kc
     nparts = ntt * ntj * nz
     cplb = 4 * cpl
@@ -190,7 +191,7 @@ def run_launch(pb, mem, kc, iter_, two, in_set, out_set, ent, remap=1):
     for bid in range(nparts):
         lb = xcd(bid, nparts) if remap else bid
         bt, bj, bk = lb % ntt, (lb // ntt) % ntj, lb // (ntt * ntj)
-        t = bt * FTXW - R + lane
+        t = lane if whole else bt * FTXW - R + lane
         tin = (t >= 0) & (t < nt)
         rA = 2 * w
         rB = rA + 1
@@ -211,7 +212,7 @@ def run_launch(pb, mem, kc, iter_, two, in_set, out_set, ent, remap=1):
         voB = 4 * (tq + (jB - 1) * cp)
         trA = 4 * (nt + (jA - 1) * cp)
         trB = 4 * (nt + (jB - 1) * cp)
-        lane_tile = tin & (lane >= R) & (lane <= FX - 1 - R)
+        lane_tile = tin & (whole | ((lane >= R) & (lane <= FX - 1 - R)))
         tileA = (rA >= R) & (rA <= TY - 1 - R) & (jlA <= jm)
         tileB = (rB >= R) & (rB <= TY - 1 - R) & (jlB <= jm)
         wB = (2 * w[:, 0] + 1 >= 2) & (2 * w[:, 0] <= TY - 3)
@@ -512,7 +513,8 @@ def check_written(pb, written, out_set, expect):
 
 
 CASES = [(20, 14, 9, 9), (20, 14, 9, 3), (21, 14, 9, 4), (130, 26, 12, 5), (7, 2, 2, 1), (9, 4, 3, 2),
-         (125, 30, 20, 7), (64, 16, 13, 13), (3, 2, 5, 5), (121, 50, 8, 6), (70, 60, 11, 4), (113, 48, 14, 14)]
+         (125, 30, 20, 7), (64, 16, 13, 13), (3, 2, 5, 5), (121, 50, 8, 6), (70, 60, 11, 4), (113, 48, 14, 14),
+         (128, 18, 6, 3), (127, 10, 5, 5), (129, 8, 4, 2)]  # 64 pairs: one whole-width tile; 65: two
 
 
 @pytest.mark.parametrize("im,jm,km,kc", CASES)
