@@ -728,10 +728,13 @@ __device__ __forceinline__ void velfg_feedbf_body(
   const bool ilast = (i == im);
   const float dxi = X.dx(i), dxp = X.dx(i + 1);
   const float dyj = X.dy(j), dyp = X.dy(j + 1);
-  // Masks built from the height map (ms: zbm(i,j) at (i-1)+(j-1)*im, then
-  // each level's threshold): the three are equal and are 1 exactly where
-  // zbm(i,j) exceeds the threshold of level k, k_masks' expression, so the
-  // three mask fields need not be read.  ms is null when the masks were given.
+  // Masks of buildings standing on the ground (ms: a value per column at
+  // (i-1)+(j-1)*im, then each level's threshold): the three are equal and
+  // are 1 exactly where the column's value exceeds the level's threshold --
+  // zbm(i,j) and z2(k)+0.5*dzn(k) for masks built from the height map
+  // (k_masks' expression), the top solid level K(i,j) and k - 1/2 for given
+  // masks of that form -- so the three mask fields need not be read.  ms is
+  // null for other masks.
   const float* const mtop = ms ? ms + (size_t)im * jm : nullptr;
   const float zb = ms ? ms[(i - 1) + (j - 1) * im] : 0.0f;
   float* const* D = diu.a;

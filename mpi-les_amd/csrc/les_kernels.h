@@ -85,8 +85,10 @@ struct DevDomain {
   int sor_tile_rows; // tile rows (with the ring) of k_sor_iter2: 16 or 32
   int sor_tile_width;  // tile width in pairs (with the ring) of k_sor_iter2: 64, or 32 with 32 rows
   int sor_entry_fast;  // SOR iteration 1 on k_sor_iter2 when p's ghosts equal their boundp copies
-  // masks built on the device from the height map: zbm(i,j) at (i-1)+(j-1)*im,
-  // then the km level thresholds; nullptr when the masks were given as fields
+  // masks of buildings standing on the ground, as a value per column at
+  // (i-1)+(j-1)*im and km level thresholds (mask = value > threshold): the
+  // height map zbm for masks built from it, the top solid level for given
+  // masks of that form; nullptr for other masks (the fields are read)
   const float* mask_src;
 };
 

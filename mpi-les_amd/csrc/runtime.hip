@@ -286,7 +286,7 @@ struct les_domain {
   bool mdirty = true;
   bool have_inflow = false;
   float* split = nullptr;           // colour-split SOR arrays
-  float* mask_src = nullptr;        // height map and level thresholds of device-built masks
+  float* mask_src = nullptr;        // column values and level thresholds of the masks (DevDomain::mask_src)
   int use_graph = 0;                // replay steps n > 1 from a captured graph
   hipGraphExec_t gexec = nullptr;   // captured step
   DevDomain gkey{};                 // domain state the graph was captured with
@@ -1469,10 +1469,66 @@ int build_masks(les_domain* D, const float* zbm, const float* z2, const float* d
     e = hipGetLastError();
   }
   if (e == hipSuccess) e = hipStreamSynchronize(D->d.stream);
-  // kept: k_velfg_feedbf evaluates the masks from it instead of reading them
-  D->mask_src = buf;
-  D->d.mask_src = buf;
+  if (env_int("LES_MASK_FIELDS", 0)) {  // the fused feedbf reads the mask fields
+    dev_release(buf);
+    D->bytes -= (long long)((nz + g.km) * sizeof(float));
+  } else {  // kept: k_velfg_feedbf evaluates the masks from it instead of reading them
+    D->mask_src = buf;
+    D->d.mask_src = buf;
+  }
   if (e != hipSuccess) return fail(std::string("mask generation: ") + hipGetErrorString(e));
+  return 0;
+}
+
+// Masks given as fields whose three arrays are equal on the interior, hold
+// only 0 and 1, and in every column (i,j) are 1 up to some level K(i,j) and
+// 0 above it (buildings standing on the ground, as feedbfm.f95:87-116 makes
+// them from a height map): the domain then keeps K(i,j) and the thresholds
+// k - 1/2 in the height map's layout, so that k_velfg_feedbf's comparison
+// K(i,j) > k - 1/2, true exactly for k <= K(i,j), gives each field's value
+// without reading the fields.  Any other masks (or LES_MASK_FIELDS=1) leave
+// the fields to be read.  Returns nonzero only on a device error.
+int mask_columns(les_domain* D, const float* bm, const float* cm, const float* dm) {
+  if (env_int("LES_MASK_FIELDS", 0)) return 0;
+  const Grid& g = D->d.g;
+  const size_t nz = (size_t)g.im * g.jm;
+  const auto base = [&](Kind k, int& ni, int& nj) {
+    int nk;
+    ref_extent(D, k, &ni, &nj, &nk);
+    const KindInfo& K = kKinds[k];
+    return (size_t)(-K.li) + (size_t)ni * ((size_t)(-K.lj) + (size_t)nj * (size_t)(-K.lk));
+  };
+  int bi, bj, ci, cj, di, dj;
+  const size_t b0 = base(KB, bi, bj), c0 = base(KU, ci, cj), d0 = base(KR, di, dj);
+  const unsigned one = 0x3f800000u, zero = 0u;
+  std::vector<float> col(nz + g.km, 0.0f);  // K(i,j), then the thresholds
+  std::vector<unsigned char> top_seen(nz, 0);  // a 0 was met below in the column
+  for (int k = 1; k <= g.km; k++)
+    for (int j = 1; j <= g.jm; j++)
+      for (int i = 1; i <= g.im; i++) {
+        unsigned b, c, d;
+        memcpy(&b, bm + b0 + i + (size_t)bi * (j + (size_t)bj * k), 4);
+        memcpy(&c, cm + c0 + i + (size_t)ci * (j + (size_t)cj * k), 4);
+        memcpy(&d, dm + d0 + i + (size_t)di * (j + (size_t)dj * k), 4);
+        const size_t q = (size_t)(i - 1) + (size_t)(j - 1) * g.im;
+        if (b != c || b != d) return 0;
+        if (b == one) {
+          if (top_seen[q]) return 0;
+          col[q] = (float)k;
+        } else if (b == zero) {
+          top_seen[q] = 1;
+        } else {
+          return 0;
+        }
+      }
+  for (int k = 1; k <= g.km; k++) col[nz + k - 1] = (float)k - 0.5f;
+  float* buf = nullptr;
+  HIP_TRY(dev_alloc(&buf, col.size() * sizeof(float)));
+  D->bytes += (long long)(col.size() * sizeof(float));
+  D->mask_src = buf;
+  const hipError_t e = copy_sync(buf, col.data(), col.size() * sizeof(float), hipMemcpyHostToDevice, D->d.stream);
+  if (e != hipSuccess) return fail(std::string("mask columns: ") + hipGetErrorString(e));
+  D->d.mask_src = buf;
   return 0;
 }
 
@@ -1511,6 +1567,7 @@ int les_domain_create(const les_domain_desc* s, les_domain** out) {
   rc |= set_inflow(D, s->inflow);
   rc |= flush_metrics(D);
   if (!rc && gen_masks) rc |= build_masks(D, s->zbm, s->z2, s->dzn);
+  if (!rc && s->bmask1 && s->cmask1 && s->dmask1) rc |= mask_columns(D, s->bmask1, s->cmask1, s->dmask1);
   if (rc) {
     std::string e = g_err;
     domain_free(D);
