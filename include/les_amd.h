@@ -189,9 +189,11 @@ typedef struct {
    * feedbfm.f95:87-116 (1 inside buildings, zbm(i,j) > z2(k)+0.5*dzn(k),
    * 0 elsewhere and on every ghost cell), and the domain keeps zbm and the
    * level thresholds (im*jm + km floats), from which the whole step's
-   * feedbf evaluates them instead of reading the mask fields.  Uploading
-   * any mask field afterwards drops the height map: the step reads the
-   * fields from then on. */
+   * feedbf evaluates them instead of reading the mask fields.  Given masks
+   * that are equal, 0 or 1, and solid from the ground up to a level in every
+   * column are kept the same way (that level per column).  Uploading any
+   * mask field afterwards drops that form: the step reads the fields from
+   * then on. */
   const float* bmask1;   /* (-1:im+1,0:jm+1,0:km+1) */
   const float* cmask1;   /* (0:im+1,-1:jm+1,0:km+1) */
   const float* dmask1;   /* (0:im+1,0:jm+1,0:km+1) */
