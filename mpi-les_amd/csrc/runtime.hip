@@ -1964,8 +1964,9 @@ int batch_prepare(les_batch* B) {
   // splits of the copy-like stages giving about 36 blocks per CU in all
   B->sor_ty = env_int("LES_BATCH_SOR_TILE_ROWS", 16) == 32 ? 32 : 16;
   B->sor_tw = (B->sor_ty == 32 && env_int("LES_BATCH_SOR_TILE_WIDTH", 64) == 32) ? 32 : 64;
-  // two iterations per launch measured slower on batches of small domains
-  // (24 domains of 96..160 x 96..160 x 64: 10.4 against 8.7 ms per step), so
+  // two iterations per launch measured no faster on batches of small domains
+  // (24 domains of 96..160 x 96..160 x 64: 8.0 against 8.1 ms per step, and
+  // 10.4 against 8.7 before tiles as wide as a domain lost their ring), so
   // the batch keeps one iteration per launch unless LES_BATCH_SOR_PAIRS=1
   B->pairs = env_int("LES_BATCH_SOR_PAIRS", 0) != 0;
   B->sor_kc = fd.empty() ? 0 : env_int("LES_BATCH_SOR_KCHUNK", 0);
