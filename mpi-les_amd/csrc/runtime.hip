@@ -416,8 +416,8 @@ int upload_field(les_domain* D, int f, const float* host) {
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipStreamSynchronize(D->d.stream));  // the host array may change after return
   if ((f == FBMASK || f == FCMASK || f == FDMASK) && D->mask_src) {
-    // a given mask replaces the height map's: k_velfg_feedbf reads the mask
-    // fields again (the changed DevDomain also retires a captured step)
+    // an uploaded mask replaces the masks' column form: k_velfg_feedbf reads
+    // the mask fields again (the changed DevDomain also retires a captured step)
     D->d.mask_src = nullptr;
     dev_release(D->mask_src);
     D->mask_src = nullptr;
@@ -1650,7 +1650,7 @@ long long les_domain_bytes_needed(int im, int jm, int km) {
   long long b = 0;
   for (int f = 0; f < kLazyFirst; f++) b += (kFields[f].kind == K2D ? sk : n3) * 4;
   b += (10 * cs + 2 * cp * jm) * 4;  // colour-split SOR arrays and their two scratch planes
-  b += ((long long)im * jm + km) * 4;  // the height map and level thresholds of device-built masks
+  b += ((long long)im * jm + km) * 4;  // the masks' column values and level thresholds
   return b + (1LL << 20);  // metrics, partial sums, control block (bounded by 1 MiB here)
 }
 
