@@ -25,6 +25,7 @@ CASES = [
     dict(im=24, jm=20, km=16, seed=1),                      # even jm, log-law inflow
     dict(im=23, jm=19, km=13, seed=2),                      # odd extents: one-sided/periodic quirks
     dict(im=33, jm=18, km=21, seed=3, ustar=None),          # the reference's fixed 5 m/s inflow
+    dict(im=5, jm=2, km=2, seed=4),                         # smallest extents: every cell on a face
 ]
 
 
