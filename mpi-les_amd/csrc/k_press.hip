@@ -1425,8 +1425,9 @@ __device__ __forceinline__ void sor_pair_block(const Grid& g, const SplitGrid& s
         const bool kin = kr >= k0 && kr <= k1;
         {
           const float own = q.roA;
-          float pim = (pa == 0 && first) ? own : pp[bsA + pa];
-          float pip = last[pa] ? own : pp[bsA + pa + 1];
+          const float lo = pp[bsA + pa], hi = pp[bsA + pa + 1];  // read by every lane
+          float pim = (pa == 0 && first) ? own : lo;
+          float pip = last[pa] ? own : hi;
           float pjm = pp[upA];
           float pjp = bqB[(U + 1) % 6];
           float pkm = bqA[U % 6];
@@ -1450,8 +1451,9 @@ __device__ __forceinline__ void sor_pair_block(const Grid& g, const SplitGrid& s
         }
         {
           const float own = q.roB;
-          float pim = (pb == 0 && first) ? own : pp[bsB + pb];
-          float pip = last[pb] ? own : pp[bsB + pb + 1];
+          const float lo = pp[bsB + pb], hi = pp[bsB + pb + 1];  // read by every lane
+          float pim = (pb == 0 && first) ? own : lo;
+          float pip = last[pb] ? own : hi;
           float pjm = bqA[(U + 1) % 6];
           float pjp = pp[dnB];
           float pkm = bqB[U % 6];
@@ -1487,8 +1489,9 @@ __device__ __forceinline__ void sor_pair_block(const Grid& g, const SplitGrid& s
         float bA, bB;
         {
           const float own = bqA[U % 6];
-          const float pim = (pa == 0 && first) ? own : pr[bsA + pa];
-          const float pip = last[pa] ? own : pr[bsA + pa + 1];
+          const float lo = pr[bsA + pa], hi = pr[bsA + pa + 1];  // read by every lane
+          const float pim = (pa == 0 && first) ? own : lo;
+          const float pip = last[pa] ? own : hi;
           const float pjm = pr[upA];
           const float pjp = rdB[(U + 2) % 3];
           float pkm = rdA[(U + 1) % 3];
@@ -1508,8 +1511,9 @@ __device__ __forceinline__ void sor_pair_block(const Grid& g, const SplitGrid& s
         }
         {
           const float own = bqB[U % 6];
-          const float pim = (pb == 0 && first) ? own : pr[bsB + pb];
-          const float pip = last[pb] ? own : pr[bsB + pb + 1];
+          const float lo = pr[bsB + pb], hi = pr[bsB + pb + 1];  // read by every lane
+          const float pim = (pb == 0 && first) ? own : lo;
+          const float pip = last[pb] ? own : hi;
           const float pjm = rdA[(U + 2) % 3];
           const float pjp = pr[dnB];
           float pkm = rdB[(U + 1) % 3];
@@ -1541,8 +1545,9 @@ __device__ __forceinline__ void sor_pair_block(const Grid& g, const SplitGrid& s
         const unsigned so = kin ? oRn + (unsigned)kc * cplb : 0u;
         {
           const float own = rdA[(U + 1) % 3];
-          const float pim = (pa == 0 && first) ? own : pq[bsA + pa];
-          const float pip = last[pa] ? own : pq[bsA + pa + 1];
+          const float lo = pq[bsA + pa], hi = pq[bsA + pa + 1];  // read by every lane
+          const float pim = (pa == 0 && first) ? own : lo;
+          const float pip = last[pa] ? own : hi;
           const float pjm = pq[upA];
           const float pjp = blB[(U + 1) % 3];
           const float pkm = (kc == 1) ? own : blA[U % 3];
@@ -1557,8 +1562,9 @@ __device__ __forceinline__ void sor_pair_block(const Grid& g, const SplitGrid& s
         }
         {
           const float own = rdB[(U + 1) % 3];
-          const float pim = (pb == 0 && first) ? own : pq[bsB + pb];
-          const float pip = last[pb] ? own : pq[bsB + pb + 1];
+          const float lo = pq[bsB + pb], hi = pq[bsB + pb + 1];  // read by every lane
+          const float pim = (pb == 0 && first) ? own : lo;
+          const float pip = last[pb] ? own : hi;
           const float pjm = blA[(U + 1) % 3];
           const float pjp = pq[dnB];
           const float pkm = (kc == 1) ? own : blB[U % 3];
@@ -1583,8 +1589,9 @@ __device__ __forceinline__ void sor_pair_block(const Grid& g, const SplitGrid& s
         const unsigned so = oBn + (unsigned)kd * cplb;  // kd is in the chunk whenever D runs
         {
           const float own = blA[U % 3];
-          const float pim = (pa == 0 && first) ? own : pr[bsA + pa];
-          const float pip = last[pa] ? own : pr[bsA + pa + 1];
+          const float lo = pr[bsA + pa], hi = pr[bsA + pa + 1];  // read by every lane
+          const float pim = (pa == 0 && first) ? own : lo;
+          const float pip = last[pa] ? own : hi;
           const float pjm = pr[upA];
           const float pjp = r2B[U % 3];
           const float pkm = (kd == 1) ? own : r2A[(U + 2) % 3];
@@ -1598,8 +1605,9 @@ __device__ __forceinline__ void sor_pair_block(const Grid& g, const SplitGrid& s
         }
         {
           const float own = blB[U % 3];
-          const float pim = (pb == 0 && first) ? own : pr[bsB + pb];
-          const float pip = last[pb] ? own : pr[bsB + pb + 1];
+          const float lo = pr[bsB + pb], hi = pr[bsB + pb + 1];  // read by every lane
+          const float pim = (pb == 0 && first) ? own : lo;
+          const float pip = last[pb] ? own : hi;
           const float pjm = r2A[U % 3];
           const float pjp = pr[dnB];
           const float pkm = (kd == 1) ? own : r2B[(U + 2) % 3];
