@@ -78,7 +78,7 @@ __device__ __forceinline__ void rhs_body(const VBlk& vb, const Grid& g, const Me
     ctrl->p_final = -1;
     ctrl->fix_iter = 0;
   }
-  const int i = 1 + vb.x * BX + threadIdx.x, j = 1 + vb.y * BY + threadIdx.y;
+  const int i = 1 + vb.x * BX + threadIdx.x, j = 1 + vb.y * BY + wave_row();
   double acc = 0.0;
   if (i <= g.im && j <= g.jm) {
     const float dxi = M1(m.dx1, i), dyj = M1(m.dy1, j);
@@ -147,7 +147,7 @@ __device__ __forceinline__ void rhs_split_body(const VBlk& vb, const Grid& g, co
                                                float* __restrict__ rhs, const float* __restrict__ p,
                                                const float* __restrict__ cn1, const SplitBufs& sb,
                                                Ctrl* ctrl) {
-  const int i = 1 + vb.x * BX + threadIdx.x, j = 1 + vb.y * BY + threadIdx.y;
+  const int i = 1 + vb.x * BX + threadIdx.x, j = 1 + vb.y * BY + wave_row();
   if (i > g.im || j > g.jm) return;
   const double av = ctrl->rhsav;
   const int t = (i - 1) >> 1;
@@ -204,7 +204,7 @@ __global__ __launch_bounds__(NT) void k_rhs_head(Grid g, SplitGrid s, Metrics m,
     ctrl->p_final = -1;
     ctrl->fix_iter = 0;
   }
-  const int i = 1 + vb.x * BX + threadIdx.x, j = 1 + vb.y * BY + threadIdx.y;
+  const int i = 1 + vb.x * BX + threadIdx.x, j = 1 + vb.y * BY + wave_row();
   double acc = 0.0;
   bool diff = false;
   if (i <= g.im && j <= g.jm) {
@@ -330,7 +330,7 @@ __global__ __launch_bounds__(256) void k_rhs_sub(const float4* __restrict__ in, 
 // ghosts of rhs untouched).
 __global__ __launch_bounds__(NT) void k_rhs_gather(Grid g, SplitGrid s, SplitBufs sb, float* __restrict__ rhs) {
   const VBlk vb = hw_blk<false>();
-  const int i = 1 + vb.x * BX + threadIdx.x, j = 1 + vb.y * BY + threadIdx.y;
+  const int i = 1 + vb.x * BX + threadIdx.x, j = 1 + vb.y * BY + wave_row();
   if (i > g.im || j > g.jm) return;
   const int t = (i - 1) >> 1;
   int k0, k1;
@@ -1760,7 +1760,7 @@ __device__ __forceinline__ void pav_body(const VBlk& vb, const Grid& g, const Sp
   const int fs = ctrl->p_final >= 0 ? ctrl->p_final : (ctrl->sor_iters & 1);  // the set holding the result
   const float* __restrict__ red = sb.p[fs][1];
   const float* __restrict__ blk = sb.p[fs][0];
-  const int i = 1 + vb.x * BX + threadIdx.x, j = 1 + vb.y * BY + threadIdx.y;
+  const int i = 1 + vb.x * BX + threadIdx.x, j = 1 + vb.y * BY + wave_row();
   double acc = 0.0;
   if (i <= g.im && j <= g.jm) {
     const float dxi = M1(m.dx1, i), dyj = M1(m.dy1, j);
@@ -1787,7 +1787,7 @@ __global__ __launch_bounds__(NT) void k_pav(Grid g, SplitGrid s, Metrics m, Spli
 // i,j,k = 0..im+1, 0..jm+1, 0..km+1.
 __device__ __forceinline__ void padj_body(const VBlk& vb, const Grid& g, const SplitGrid& s, const SplitBufs& sb,
                                           float* __restrict__ p, const Ctrl* ctrl) {
-  const int i = vb.x * BX + threadIdx.x, j = vb.y * BY + threadIdx.y;
+  const int i = vb.x * BX + threadIdx.x, j = vb.y * BY + wave_row();
   if (i > g.im + 1 || j > g.jm + 1) return;
   const int fs = ctrl->p_final >= 0 ? ctrl->p_final : (ctrl->sor_iters & 1);  // the set holding the result
   const float* __restrict__ red = sb.p[fs][1];

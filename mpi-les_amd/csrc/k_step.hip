@@ -29,7 +29,7 @@ __device__ __forceinline__ void velnw_body(const VBlk& vb, const Grid& g, const 
                                            const float* __restrict__ gg, const float* __restrict__ h,
                                            float* __restrict__ u, float* __restrict__ v, float* __restrict__ w,
                                            float ro, float dt) {
-  const int i = 1 + vb.x * BX + threadIdx.x, j = 1 + vb.y * BY + threadIdx.y;
+  const int i = 1 + vb.x * BX + threadIdx.x, j = 1 + vb.y * BY + wave_row();
   if (i > g.im || j > g.jm) return;
   const float rdx = M1(m.dxs, i), rdy = M1(m.dys, j);
   int k0, k1;
@@ -719,7 +719,8 @@ __device__ __forceinline__ void velfg_feedbf_body(
     float dt, float beta, int kchunk) {
   const Grid& g = X.g;
   const Metrics& m = X.m;
-  const int i = 1 + vb.x * BX + threadIdx.x, j = 1 + vb.y * BY + threadIdx.y;
+  // a wave is one row of the block (blockDim.x == 64): j is wave-uniform
+  const int i = 1 + vb.x * BX + threadIdx.x, j = 1 + vb.y * BY + wave_row();
   if (i > g.im || j > g.jm) return;
   const int im = g.im, jm = g.jm, km = g.km;
   const int k0 = 1 + vb.z * kchunk, k1 = min(km, k0 + kchunk - 1);
@@ -770,8 +771,11 @@ __device__ __forceinline__ void velfg_feedbf_body(
     const DC n3 = X.q3(i, j, k + 1), n6 = X.q6(i, j, k + 1);
     const DC n9 = (k + 1 <= km) ? X.q9(i, j, k + 1) : DC{0.f, 0.f};
     // neighbours at i+1 (the copy of im at i=im) and j+1 (row 1 at j=jm)
-    const DC b1 = ilast ? a1 : X.q1(i + 1, j, k);
-    const DC b4 = ilast ? a4 : X.q4(i + 1, j, k);
+    // computed by every lane (at i = im from the ghost cells, then replaced):
+    // no lane-dependent branch around the loads
+    const DC c1 = X.q1(i + 1, j, k), c4 = X.q4(i + 1, j, k);
+    const DC b1 = ilast ? a1 : c1;
+    const DC b4 = ilast ? a4 : c4;
     const DC b2 = X.q2(i, jn, k);
     const DC b5 = X.q5(i, jn, k);
     // ---- velFG.f95:64-77
@@ -796,7 +800,8 @@ __device__ __forceinline__ void velfg_feedbf_body(
     // ---- velFG.f95:97-110 (k <= km-1; h(km) keeps its value)
     float hv;
     if (hk) {
-      const DC b7 = ilast ? a7 : X.q7(i + 1, j, k);
+      const DC c7 = X.q7(i + 1, j, k);
+      const DC b7 = ilast ? a7 : c7;
       const DC b8 = X.q8(i, jn, k);
       const float dzp = M1(m.dzn, k + 1);
       const float covx1 = (a7.c + b7.c) / 2.0f;
@@ -1007,7 +1012,7 @@ __device__ __forceinline__ void sm_ghosts(const Grid& g, int i, int j, int k, fl
 template <bool GHOSTS>
 __device__ __forceinline__ void les_sm_body(const VBlk& vb, const Grid& g, const P10& diu,
                                             const float* __restrict__ delx1, float* __restrict__ sm) {
-  const int i = 1 + vb.x * BX + threadIdx.x, j = 1 + vb.y * BY + threadIdx.y;
+  const int i = 1 + vb.x * BX + threadIdx.x, j = 1 + vb.y * BY + wave_row();
   if (i > g.im || j > g.jm) return;
   const int sj = g.sj, sk = g.sk;
   const float* d2 = diu.a[2];
@@ -1138,7 +1143,7 @@ __device__ __forceinline__ void les_visc_body(const VBlk& vb, const Grid& g, con
                                               const float* __restrict__ vspd, float* __restrict__ f,
                                               float* __restrict__ gg, float* __restrict__ h, float* __restrict__ fo,
                                               float* __restrict__ go, float* __restrict__ ho, float wl) {
-  const int i = 1 + vb.x * BX + threadIdx.x, j = 1 + vb.y * BY + threadIdx.y;
+  const int i = 1 + vb.x * BX + threadIdx.x, j = 1 + vb.y * BY + wave_row();
   if (i > g.im || j > g.jm) return;
   const int sj = g.sj, sk = g.sk;
   const float dxi = M1(m.dx1, i), dxp = M1(m.dx1, i + 1), dxm = M1(m.dx1, i - 1);

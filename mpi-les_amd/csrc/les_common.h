@@ -74,6 +74,11 @@ struct Ctrl {
 
 constexpr int kWave = 64;
 
+// The row of a block whose rows are one wave wide (blockDim.x == 64): the
+// same for every lane of a wave, so the compiler can keep it, and whatever
+// derives from it, in scalar registers.
+__device__ __forceinline__ int wave_row() { return __builtin_amdgcn_readfirstlane(threadIdx.y); }
+
 // 1/x correctly rounded for positive normal x whose reciprocal is normal:
 // the hardware reciprocal (within 1 ulp) refined by one Newton step with
 // fused multiply-adds (the residual 1 - x*r is exact in FMA).  It equals
