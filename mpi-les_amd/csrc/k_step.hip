@@ -3,6 +3,7 @@
 // order of the cited Fortran lines and the library is built without FMA
 // contraction, so each FP32 result equals the reference's.
 #include <algorithm>
+#include <type_traits>
 
 #include "les_kernels.h"
 
@@ -756,12 +757,15 @@ __device__ __forceinline__ void velfg_feedbf_body(
     z6 = X.q6(i, j, k0);
   }
   z9 = X.q9(i, j, k0);
-  for (int k = k0; k <= k1; k++) {
+  // One level.  HK: k <= km-1, where h, diu7, diu8 and diu9's k+1 term
+  // exist; the levels below km run the HK body, which has no branch on k,
+  // and the top level km its own.
+  auto level = [&](auto HKC, int k) {
+    constexpr bool hk = decltype(HKC)::value;
     const int c = ix(g, i, j, k);
     const float dzk = M1(m.dzn, k);
     const DC a1 = X.q1(i, j, k), a2 = X.q2(i, j, k), a4 = X.q4(i, j, k), a5 = X.q5(i, j, k);
     const DC a3 = z3, a6 = z6, a9 = z9;
-    const bool hk = (k <= km - 1);
     DC a7 = {0.f, 0.f}, a8 = {0.f, 0.f};
     if (hk) {
       a7 = X.q7(i, j, k);
@@ -769,7 +773,7 @@ __device__ __forceinline__ void velfg_feedbf_body(
     }
     // next level (k+1 <= km+1 for diu3/diu6, <= km for diu9)
     const DC n3 = X.q3(i, j, k + 1), n6 = X.q6(i, j, k + 1);
-    const DC n9 = (k + 1 <= km) ? X.q9(i, j, k + 1) : DC{0.f, 0.f};
+    const DC n9 = hk ? X.q9(i, j, k + 1) : DC{0.f, 0.f};
     // neighbours at i+1 (the copy of im at i=im) and j+1 (row 1 at j=jm)
     // computed by every lane (at i = im from the ghost cells, then replaced):
     // no lane-dependent branch around the loads
@@ -876,7 +880,10 @@ __device__ __forceinline__ void velfg_feedbf_body(
     z3 = n3;
     z6 = n6;
     z9 = n9;
-  }
+  };
+  const int kh = min(k1, km - 1);
+  for (int k = k0; k <= kh; k++) level(std::true_type{}, k);
+  if (k1 == km) level(std::false_type{}, km);
   if (k1 == km) {  // level km+1: diu3, diu6 (vel2.f95:119-176) and the copies from it
     const int k = km + 1;
     put(3, i, j, k, z3.d);
