@@ -1216,7 +1216,8 @@ __device__ __forceinline__ void pair_stop(Ctrl* ctrl, int L, int two, double s1,
 constexpr int PTY = 32, PR = 4, PNT = FX * PTY / 2, PFTY = PTY - 2 * PR, PFTX = FX - 2 * PR;
 
 // ENT: the launch may hold iteration 1 (the entry-ghost code is compiled in).
-template <bool OM1, bool ENT>
+// TW: 1 two iterations, 0 one (compiled in), -1 pl.two at run time (batches).
+template <bool OM1, bool ENT, int TW>
 __device__ __forceinline__ void sor_pair_block(const Grid& g, const SplitGrid& s, const Metrics& m,
                                                const SplitBufs& sb, Ctrl* ctrl, const PairLaunch& pl, int nparts,
                                                float omega, float pjuge, int ntt, int ntj, int kchunk, int remap,
@@ -1250,7 +1251,7 @@ __device__ __forceinline__ void sor_pair_block(const Grid& g, const SplitGrid& s
   __syncthreads();
   if (sh[0]) return;
   const int in_set = sh[2], out_set = sh[3];
-  const bool two = pl.two != 0 && !pl.fixup;
+  const bool two = (TW < 0) ? (pl.two != 0 && !pl.fixup) : (TW == 1);
   const bool ent = ENT && sh[4] != 0;  // iteration 1 on the entry ghosts of p
   if (!pl.fixup && pl.prev_iter > 0 && bid == 0) {  // the previous launch's stop test
     double s1 = 0.0, s2 = 0.0;
@@ -1659,11 +1660,11 @@ __device__ __forceinline__ void sor_pair_block(const Grid& g, const SplitGrid& s
   }
 }
 
-template <bool OM1, bool ENT>
+template <bool OM1, bool ENT, int TW>
 __global__ __launch_bounds__(PNT) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_sor_pair(
     Grid g, SplitGrid s, Metrics m, SplitBufs sb, Ctrl* ctrl, PairLaunch pl, int nparts, float omega, float pjuge,
     int ntt, int ntj, int kchunk, int remap, const float* __restrict__ pent) {
-  sor_pair_block<OM1, ENT>(g, s, m, sb, ctrl, pl, nparts, omega, pjuge, ntt, ntj, kchunk, remap, pent,
+  sor_pair_block<OM1, ENT, TW>(g, s, m, sb, ctrl, pl, nparts, omega, pjuge, ntt, ntj, kchunk, remap, pent,
                            (int)blockIdx.x);
 }
 
@@ -1704,7 +1705,7 @@ __global__ __launch_bounds__(PNT) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
     pl.two = pl.two && pl.iter + 1 <= E.nmaxp;
     pl.prev_nparts = pl.prev_colours ? E.nparts1 : E.nparts;
   }
-  sor_pair_block<OM1, false>(E.g, E.s, E.m, E.sb, E.ctrl, pl, E.nparts, E.omega, E.pjuge, E.ntt, E.ntj, E.kchunk,
+  sor_pair_block<OM1, false, -1>(E.g, E.s, E.m, E.sb, E.ctrl, pl, E.nparts, E.omega, E.pjuge, E.ntt, E.ntj, E.kchunk,
                              remap, nullptr, bid);
 }
 
@@ -2071,10 +2072,18 @@ static void launch_pair(const DevDomain& d, const SplitBufs& sb, const PairLaunc
   const size_t lvm_bytes = (size_t)(h.kc + 7) * sizeof(float4);
   const bool ent = (pl.iter == 1 || pl.fixup) && pent;  // may run iteration 1 on the entry ghosts
   const int cls = (pl.iter == 1 || pl.fixup) ? KC_SOR1 : KC_SOR;
-#define PAIR(OM1, ENT)                                                                                          \
-  TIMED(d, cls, hipLaunchKernelGGL((k_sor_pair<OM1, ENT>), dim3(h.nparts), dim3(FX, PTY / 2), lvm_bytes, d.stream, \
-                                   d.g, d.sg, d.m, sb, d.ctrl, pl, h.nparts, d.omega, pjuge, h.ntt, h.ntj, h.kc,   \
-                                   d.xcd_remap, pent))
+  const bool two = pl.two != 0 && !pl.fixup;  // compiled in: no branch on it in the march
+#define PAIR1(OM1, ENT, TW)                                                                                     \
+  TIMED(d, cls, hipLaunchKernelGGL((k_sor_pair<OM1, ENT, TW>), dim3(h.nparts), dim3(FX, PTY / 2), lvm_bytes,       \
+                                   d.stream, d.g, d.sg, d.m, sb, d.ctrl, pl, h.nparts, d.omega, pjuge, h.ntt,      \
+                                   h.ntj, h.kc, d.xcd_remap, pent))
+#define PAIR(OM1, ENT)       \
+  do {                       \
+    if (two)                 \
+      PAIR1(OM1, ENT, 1);    \
+    else                     \
+      PAIR1(OM1, ENT, 0);    \
+  } while (0)
   if (d.omega == 1.0f) {
     if (ent)
       PAIR(true, true);
@@ -2087,6 +2096,7 @@ static void launch_pair(const DevDomain& d, const SplitBufs& sb, const PairLaunc
       PAIR(false, false);
   }
 #undef PAIR
+#undef PAIR1
 }
 
 // Iterations l0..l1 in launches of two (the last alone when their number is
