@@ -469,7 +469,7 @@ __global__ __launch_bounds__(NT) void k_vel2(Grid g, Metrics m, const float* __r
       d = vs * 0.4f / wl / (0.5f * dz1) / 0.4f * v[c] / vs;
       diu.a[6][c] = d;
       cov.a[6][c] = nou * d;
-      const int c2 = (i + 1) + (j + 1) * sj;  // vel2.f95:351-356
+      const int c2 = ix(g, i, j, -1);  // vel2.f95:351-356 (uspd, vspd: plane k = -1)
       uspd[c2] = u[c] / us;
       vspd[c2] = v[c] / vs;
     }
@@ -749,7 +749,7 @@ __device__ __forceinline__ void velfg_feedbf_body(
   if (k0 == 1) {
     float us, vs;
     X.wall(i, j, z3, z6, us, vs);
-    const int c2 = (i + 1) + (j + 1) * g.sj;
+    const int c2 = ix(g, i, j, -1);  // uspd, vspd: plane k = -1
     uspd[c2] = us;
     vspd[c2] = vs;
   } else {
@@ -1210,8 +1210,8 @@ __device__ __forceinline__ void les_visc_body(const VBlk& vb, const Grid& g, con
         const float visuz1 = cUz1;
         vfu = (visux2 - visux1) / dxs + (visuy2 - visuy1) / dyj + (visuz2 - visuz1) / dzk;
       } else {
-        const float t = 0.4f * uspd[(i + 1) + (j + 1) * sj] / wl;
-        const float visuz1 = t * t * uspd[(i + 1) + (j + 1) * sj];
+        const float t = 0.4f * uspd[ix(g, i, j, -1)] / wl;
+        const float visuz1 = t * t * uspd[ix(g, i, j, -1)];
         vfu = (visux2 - visux1) / dxs + (visuy2 - visuy1) / dyj + (visuz2 - visuz1) / dzk +
               (visuy2 - visuy1) / dyj + (visuz2 - visuz1) / dzk;
       }
@@ -1232,8 +1232,8 @@ __device__ __forceinline__ void les_visc_body(const VBlk& vb, const Grid& g, con
       if (k >= 2) {
         visvz1 = cVz1;
       } else {
-        const float t = 0.4f * vspd[(i + 1) + (j + 1) * sj] / wl;
-        visvz1 = t * t * vspd[(i + 1) + (j + 1) * sj];
+        const float t = 0.4f * vspd[ix(g, i, j, -1)] / wl;
+        visvz1 = t * t * vspd[ix(g, i, j, -1)];
       }
       const float vfv = (visvx2 - visvx1) / dxi + (visvy2 - visvy1) / dys + (visvz2 - visvz1) / dzk;
       gv = (gg[c] + vfv);

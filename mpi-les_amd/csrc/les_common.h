@@ -4,10 +4,12 @@
 // i,j,k = -1 .. n+2 in each direction, which contains every reference
 // extent (u,v: (0:ip+1,-1:jp+1,0:kp+1); w: (0:ip+1,-1:jp+1,-1:kp+1);
 // p: (0:ip+2,0:jp+2,0:kp+1); diu1: (-1:ip+2,0:jp+2,0:kp+2); sm:
-// (-1:ip+1,-1:jp+1,0:kp+1); ... main.f95:63-189).  i is fastest and the row
-// pitch sj is padded to a multiple of 64 floats (256 B).  Cells a reference
-// array does not have are never read by the kernels.  1-D metrics are
-// stored with index x at slot x+1.
+// (-1:ip+1,-1:jp+1,0:kp+1); ... main.f95:63-189).  i is fastest; i = 1
+// starts a 256-byte segment of its row (i = 0 and -1 end the segment before)
+// and the row pitch sj is a multiple of 64 floats (256 B), so the 64 lanes of
+// a wave on i = 1..64 (and on every later run of 64) touch two 128-byte lines,
+// not three.  Cells a reference array does not have are never read by the
+// kernels.  1-D metrics are stored with index x at slot x+1.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -20,8 +22,13 @@ struct Grid {
   int sk;  // floats between consecutive k planes
 };
 
+constexpr int kIOrg = 63;  // row offset of i = 0
+
+// Row pitch of an im-wide domain: i runs to im+2.
+__host__ __device__ constexpr long row_pitch(long im) { return (im + 2 + kIOrg + 1 + 63) / 64 * 64; }
+
 __host__ __device__ __forceinline__ int ix(const Grid& g, int i, int j, int k) {
-  return (i + 1) + (j + 1) * g.sj + (k + 1) * g.sk;
+  return (i + kIOrg) + (j + 1) * g.sj + (k + 1) * g.sk;
 }
 
 // 1-D metrics (slot x+1 holds index x).  Pointers into one device buffer.

@@ -570,7 +570,7 @@ void domain_free(les_domain* D) {
 
 int domain_alloc(int im, int jm, int km, les_domain** out) {
   if (im < 2 || jm < 2 || km < 2) return fail("domain extents must be at least 2 in every direction");
-  const long sj = ((long)im + 4 + 63) / 64 * 64;
+  const long sj = row_pitch(im);
   const long sk = sj * (jm + 4);
   const long n3 = sk * (km + 4);
   if (n3 >= (1L << 31)) return fail("domain too large for 32-bit field indexing");
@@ -1645,7 +1645,7 @@ int les_domain_query(les_domain* D) {
 
 long long les_domain_bytes_needed(int im, int jm, int km) {
   if (im < 2 || jm < 2 || km < 2) return -1;
-  const long long sj = ((long long)im + 4 + 63) / 64 * 64, sk = sj * (jm + 4), n3 = sk * (km + 4);
+  const long long sj = row_pitch(im), sk = sj * (jm + 4), n3 = sk * (km + 4);
   const long long nt = (im + 1) / 2, cp = (nt + 1 + 31) / 32 * 32, cs = cp * jm * km;
   long long b = 0;
   for (int f = 0; f < kLazyFirst; f++) b += (kFields[f].kind == K2D ? sk : n3) * 4;
