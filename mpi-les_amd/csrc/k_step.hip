@@ -606,6 +606,60 @@ void launch_velfg(const DevDomain& d, float vn) {
                      d.F.f, d.F.g, d.F.h, vn));
 }
 
+// boundsm.f95:18-64 for the ghost cells whose composite copy (boundsm_body)
+// reads interior cell (i,j,k), whose new value is val: its images on the
+// i, j and k faces it touches, negated at k = 0.  The threads of row j = 1
+// also write the ghosts of row j = -1, whose copies read row j = -1 itself,
+// which nothing writes.
+__device__ __forceinline__ void sm_ghosts(const Grid& g, int i, int j, int k, float val, float* __restrict__ sm) {
+  constexpr int kNone = -1000;
+  const int ie = (i == 1) ? 0 : ((i == g.im) ? g.im + 1 : kNone);
+  const int je = (j == 1) ? 0 : ((j == g.jm) ? g.jm + 1 : kNone);
+  const int ke = (k == 1) ? 0 : ((k == g.km) ? g.km + 1 : kNone);
+  if (ie == kNone && je == kNone && ke == kNone) return;
+  for (int a = 0; a < 2; a++) {
+    const int gi = a ? ie : i;
+    if (gi == kNone) continue;
+    for (int c = 0; c < 2; c++) {
+      const int gk = c ? ke : k;
+      if (gk == kNone) continue;
+      for (int b = 0; b < 2; b++) {
+        const int gj = b ? je : j;
+        if (gj == kNone || (a | b | c) == 0) continue;
+        sm[ix(g, gi, gj, gk)] = (gk == 0) ? -val : val;
+      }
+      if (j == 1 && (a | c) != 0) {
+        const float s = sm[ix(g, i, -1, k)];
+        sm[ix(g, gi, -1, gk)] = (gk == 0) ? -s : s;
+      }
+    }
+  }
+}
+
+// sm of one cell (les.f95:44-79) from the diu values around it, in the
+// reference's order of operations: a = 0.1*delx1(k); the diu2, diu4, diu6,
+// diu8 (x, y, z) averages of the four values at the cell's corners.
+__device__ __forceinline__ float sm_value(float dlx, float d1c, float d2im, float d2imjp, float d2c, float d2jp,
+                                          float d3im, float d3imkp, float d3c, float d3kp, float d4c, float d4jm,
+                                          float d4ip, float d4ipjm, float d5c, float d6jm, float d6jmkp, float d6c,
+                                          float d6kp, float d7c, float d7km, float d7ip, float d7ipkm, float d8c,
+                                          float d8km, float d8jp, float d8jpkm, float d9c) {
+  const float dudxx1 = d1c;
+  const float dudyx1 = (d2im + d2imjp + d2c + d2jp) * 0.25f;
+  const float dudzx1 = (d3im + d3imkp + d3c + d3kp) * 0.25f;
+  const float dvdxx1 = (d4c + d4jm + d4ip + d4ipjm) * 0.25f;
+  const float dvdyx1 = d5c;
+  const float dvdzx1 = (d6jm + d6jmkp + d6c + d6kp) * 0.25f;
+  const float dwdxx1 = (d7c + d7km + d7ip + d7ipkm) * 0.25f;
+  const float dwdyx1 = (d8c + d8km + d8jp + d8jpkm) * 0.25f;
+  const float dwdzx1 = d9c;
+  const float a = 0.1f * dlx;
+  const float s = 2.0f * (dudxx1 * dudxx1 + dvdyx1 * dvdyx1 + dwdzx1 * dwdzx1) +
+                  (dudyx1 + dvdxx1) * (dudyx1 + dvdxx1) + (dwdyx1 + dvdzx1) * (dwdyx1 + dvdzx1) +
+                  (dudzx1 + dwdxx1) * (dudzx1 + dwdxx1);
+  return a * a * sqrtf(s);
+}
+
 // ------------------------------------------- fused vel2 + velfg + feedbf
 // One pass of vel2 (vel2.f95:54-356), velfg (velFG.f95:64-110) and feedbf
 // (feedbf.f95:35-61) for the whole-step path.  A thread owns the column
@@ -712,12 +766,19 @@ struct VelCtx {
   }
 };
 
+// SM: also sm with its ghost cells (les.f95:44-79, boundsm.f95:18-64), the
+// whole-step path's k_les_sm.  The diu values sm reads around the cell that
+// other threads store in this launch are recomputed (at i-1, j-1 and j+1,
+// the j=0 and j=jm+1 ghosts as the periodic images they copy), or carried
+// from level to level (k-1, k+1); those no thread stores (at i=0, and diu7,
+// diu8 at k=0 and k=km) are read from the fields.
+template <bool SM>
 __device__ __forceinline__ void velfg_feedbf_body(
     const VBlk& vb, const VelCtx& X, const P10& diu, float* __restrict__ f, float* __restrict__ gg,
     float* __restrict__ h, float* __restrict__ usum, float* __restrict__ vsum, float* __restrict__ wsum,
     const float* __restrict__ bm, const float* __restrict__ cm, const float* __restrict__ dm,
     const float* __restrict__ ms, float* __restrict__ uspd, float* __restrict__ vspd, float vn, float alpha,
-    float dt, float beta, int kchunk) {
+    float dt, float beta, int kchunk, const float* __restrict__ delx1, float* __restrict__ sm) {
   const Grid& g = X.g;
   const Metrics& m = X.m;
   // a wave is one row of the block (blockDim.x == 64): j is wave-uniform
@@ -757,6 +818,36 @@ __device__ __forceinline__ void velfg_feedbf_body(
     z6 = X.q6(i, j, k0);
   }
   z9 = X.q9(i, j, k0);
+  // SM: jp is the row whose diu4/diu6 the cell reads at j-1 (the periodic
+  // image jm at j=1); s3m, s6m are diu3(i-1,j,k), diu6(i,j-1,k); p7c, p7p,
+  // p8c, p8j diu7 at (i,j),(i+1,j) and diu8 at (i,j),(i,j+1) of level k-1
+  const int jp = (j == 1) ? jm : j - 1;
+  const bool ifirst = (i == 1);
+  float s3m = 0.f, s6m = 0.f, p7c = 0.f, p7p = 0.f, p8c = 0.f, p8j = 0.f;
+  if (SM) {
+    const float m3 = D[3][ix(g, 0, j, k0)];  // i=0: never stored
+    if (k0 == 1) {
+      DC r3, r6, t3, t6;
+      float us, vs;
+      X.wall(i - 1, j, r3, t6, us, vs);
+      X.wall(i, jp, t3, r6, us, vs);
+      s3m = ifirst ? m3 : r3.d;
+      s6m = r6.d;
+      p7c = D[7][ix(g, i, j, 0)];  // k=0: never stored
+      p7p = D[7][ix(g, i + 1, j, 0)];
+      p8c = D[8][ix(g, i, j, 0)];
+      p8j = D[8][ix(g, i, j + 1, 0)];
+    } else {
+      const float r3 = X.q3(i - 1, j, k0).d;
+      s3m = ifirst ? m3 : r3;
+      s6m = X.q6(i, jp, k0).d;
+      const DC e7 = X.q7(i, j, k0 - 1), f7 = X.q7(i + 1, j, k0 - 1);
+      p7c = e7.d;
+      p7p = ilast ? e7.d : f7.d;
+      p8c = X.q8(i, j, k0 - 1).d;
+      p8j = X.q8(i, jn, k0 - 1).d;
+    }
+  }
   // One level.  HK: k <= km-1, where h, diu7, diu8 and diu9's k+1 term
   // exist; the levels below km run the HK body, which has no branch on k,
   // and the top level km its own.
@@ -803,10 +894,11 @@ __device__ __forceinline__ void velfg_feedbf_body(
     }
     // ---- velFG.f95:97-110 (k <= km-1; h(km) keeps its value)
     float hv;
+    DC b7 = {0.f, 0.f}, b8 = {0.f, 0.f};
     if (hk) {
       const DC c7 = X.q7(i + 1, j, k);
-      const DC b7 = ilast ? a7 : c7;
-      const DC b8 = X.q8(i, jn, k);
+      b7 = ilast ? a7 : c7;
+      b8 = X.q8(i, jn, k);
       const float dzp = M1(m.dzn, k + 1);
       const float covx1 = (a7.c + b7.c) / 2.0f;
       const float covy1 = (a8.c + b8.c) / 2.0f;
@@ -877,6 +969,37 @@ __device__ __forceinline__ void velfg_feedbf_body(
         cpy(3, im + 1, jm + 1, k, im, jm + 1, k); // diu3(im,jm+1) is never written
       }
     }
+    if (SM) {  // les.f95:44-79 and boundsm.f95:18-64
+      const float m2a = D[2][ix(g, 0, j, k)], m2b = D[2][ix(g, 0, j + 1, k)];  // i=0: never stored
+      const float m3 = D[3][ix(g, 0, j, k + 1)];
+      const float r2a = X.q2(i - 1, j, k).d, r2b = X.q2(i - 1, jn, k).d, r3 = X.q3(i - 1, j, k + 1).d;
+      const float d4jm = X.q4(i, jp, k).d, r4 = X.q4(i + 1, jp, k).d;
+      const float n6m = X.q6(i, jp, k + 1).d;
+      const float n3m = ifirst ? m3 : r3;
+      float c7c, c7p, c8c, c8j;
+      if (hk) {
+        c7c = a7.d;
+        c7p = b7.d;
+        c8c = a8.d;
+        c8j = b8.d;
+      } else {  // diu7, diu8 at k=km: never stored
+        c7c = D[7][c];
+        c7p = D[7][c + 1];
+        c8c = D[8][c];
+        c8j = D[8][c + g.sj];
+      }
+      const float val = sm_value(delx1[k], a1.d, ifirst ? m2a : r2a, ifirst ? m2b : r2b, a2.d, b2.d, s3m, n3m, a3.d,
+                                 n3.d, a4.d, d4jm, b4.d, ilast ? d4jm : r4, a5.d, s6m, n6m, a6.d, n6.d, c7c, p7c, c7p,
+                                 p7p, c8c, p8c, c8j, p8j, a9.d);
+      sm[c] = val;
+      sm_ghosts(g, i, j, k, val, sm);
+      s3m = n3m;
+      s6m = n6m;
+      p7c = c7c;
+      p7p = c7p;
+      p8c = c8c;
+      p8j = c8j;
+    }
     z3 = n3;
     z6 = n6;
     z9 = n9;
@@ -906,6 +1029,7 @@ __device__ __forceinline__ void velfg_feedbf_body(
     }
   }
 }
+template <bool SM>
 __global__ __launch_bounds__(NT) void k_velfg_feedbf(VelCtx X, P10 diu, float* __restrict__ f,
                                                      float* __restrict__ gg, float* __restrict__ h,
                                                      float* __restrict__ usum, float* __restrict__ vsum,
@@ -913,9 +1037,10 @@ __global__ __launch_bounds__(NT) void k_velfg_feedbf(VelCtx X, P10 diu, float* _
                                                      const float* __restrict__ cm, const float* __restrict__ dm,
                                                      const float* __restrict__ ms, float* __restrict__ uspd,
                                                      float* __restrict__ vspd, float vn, float alpha, float dt,
-                                                     float beta, int kchunk) {
-  velfg_feedbf_body(hw_blk<false>(), X, diu, f, gg, h, usum, vsum, wsum, bm, cm, dm, ms, uspd, vspd, vn, alpha, dt,
-                    beta, kchunk);
+                                                     float beta, int kchunk, const float* __restrict__ delx1,
+                                                     float* __restrict__ sm) {
+  velfg_feedbf_body<SM>(hw_blk<false>(), X, diu, f, gg, h, usum, vsum, wsum, bm, cm, dm, ms, uspd, vspd, vn, alpha,
+                        dt, beta, kchunk, delx1, sm);
 }
 
 // Grid of k_velfg_feedbf and its k levels per thread.
@@ -938,9 +1063,16 @@ void launch_velfg_feedbf(const DevDomain& d, float vn, float alpha, float dt, fl
   int kc = 0;
   const dim3 grid = velfg_grid(d, &kc);
   VelCtx X{d.g, d.m, d.F.u, d.F.v, d.F.w, d.wl};
-  TIMED(d, KC_VEL2, hipLaunchKernelGGL(k_velfg_feedbf, grid, block2d(), 0, d.stream, X, p10(d.F.diu),
-                                        d.F.f, d.F.g, d.F.h, d.F.usum, d.F.vsum, d.F.wsum, d.F.bmask, d.F.cmask,
-                                        d.F.dmask, d.mask_src, d.F.uspd, d.F.vspd, vn, alpha, dt, beta, kc));
+  if (d.fuse_sm)
+    TIMED(d, KC_VEL2, hipLaunchKernelGGL(k_velfg_feedbf<true>, grid, block2d(), 0, d.stream, X, p10(d.F.diu),
+                                          d.F.f, d.F.g, d.F.h, d.F.usum, d.F.vsum, d.F.wsum, d.F.bmask, d.F.cmask,
+                                          d.F.dmask, d.mask_src, d.F.uspd, d.F.vspd, vn, alpha, dt, beta, kc,
+                                          d.m.delx1 + 1, d.F.sm));
+  else
+    TIMED(d, KC_VEL2, hipLaunchKernelGGL(k_velfg_feedbf<false>, grid, block2d(), 0, d.stream, X, p10(d.F.diu),
+                                          d.F.f, d.F.g, d.F.h, d.F.usum, d.F.vsum, d.F.wsum, d.F.bmask, d.F.cmask,
+                                          d.F.dmask, d.mask_src, d.F.uspd, d.F.vspd, vn, alpha, dt, beta, kc,
+                                          nullptr, nullptr));
 }
 
 // ----------------------------------------------------------------- feedbf
@@ -984,35 +1116,6 @@ void launch_feedbf(const DevDomain& d, float alpha, float dt, float beta) {
 
 // -------------------------------------------------------------------- les
 // Eddy viscosity sm (les.f95:45-65); delx1 arrives precomputed per level.
-// boundsm.f95:18-64 for the ghost cells whose composite copy (boundsm_body)
-// reads interior cell (i,j,k), whose new value is val: its images on the
-// i, j and k faces it touches, negated at k = 0.  The threads of row j = 1
-// also write the ghosts of row j = -1, whose copies read row j = -1 itself,
-// which nothing writes.
-__device__ __forceinline__ void sm_ghosts(const Grid& g, int i, int j, int k, float val, float* __restrict__ sm) {
-  constexpr int kNone = -1000;
-  const int ie = (i == 1) ? 0 : ((i == g.im) ? g.im + 1 : kNone);
-  const int je = (j == 1) ? 0 : ((j == g.jm) ? g.jm + 1 : kNone);
-  const int ke = (k == 1) ? 0 : ((k == g.km) ? g.km + 1 : kNone);
-  if (ie == kNone && je == kNone && ke == kNone) return;
-  for (int a = 0; a < 2; a++) {
-    const int gi = a ? ie : i;
-    if (gi == kNone) continue;
-    for (int c = 0; c < 2; c++) {
-      const int gk = c ? ke : k;
-      if (gk == kNone) continue;
-      for (int b = 0; b < 2; b++) {
-        const int gj = b ? je : j;
-        if (gj == kNone || (a | b | c) == 0) continue;
-        sm[ix(g, gi, gj, gk)] = (gk == 0) ? -val : val;
-      }
-      if (j == 1 && (a | c) != 0) {
-        const float s = sm[ix(g, i, -1, k)];
-        sm[ix(g, gi, -1, gk)] = (gk == 0) ? -s : s;
-      }
-    }
-  }
-}
 
 // GHOSTS: also write the ghost cells boundsm sets (k_les_sm of a domain's
 // own step; the batched step keeps its boundsm launch).
@@ -1044,24 +1147,13 @@ __device__ __forceinline__ void les_sm_body(const VBlk& vb, const Grid& g, const
     h6m = d6[c - sj + sk];
     h6c = d6[c + sk];
     const float c7c = d7[c], c7p = d7[c + 1], c8c = d8[c], c8j = d8[c + sj];
-    const float dudxx1 = diu.a[1][c];
-    const float dudyx1 = (d2[c - 1] + d2[c - 1 + sj] + d2[c] + d2[c + sj]) * 0.25f;
-    const float dudzx1 = (l3m + h3m + l3c + h3c) * 0.25f;
-    const float dvdxx1 = (d4[c] + d4[c - sj] + d4[c + 1] + d4[c + 1 - sj]) * 0.25f;
-    const float dvdyx1 = diu.a[5][c];
-    const float dvdzx1 = (l6m + h6m + l6c + h6c) * 0.25f;
-    const float dwdxx1 = (c7c + p7c + c7p + p7p) * 0.25f;
-    const float dwdyx1 = (c8c + p8c + c8j + p8j) * 0.25f;
+    const float val = sm_value(delx1[k], diu.a[1][c], d2[c - 1], d2[c - 1 + sj], d2[c], d2[c + sj], l3m, h3m, l3c,
+                               h3c, d4[c], d4[c - sj], d4[c + 1], d4[c + 1 - sj], diu.a[5][c], l6m, h6m, l6c, h6c,
+                               c7c, p7c, c7p, p7p, c8c, p8c, c8j, p8j, diu.a[9][c]);
     p7c = c7c;
     p7p = c7p;
     p8c = c8c;
     p8j = c8j;
-    const float dwdzx1 = diu.a[9][c];
-    const float a = 0.1f * delx1[k];
-    const float s = 2.0f * (dudxx1 * dudxx1 + dvdyx1 * dvdyx1 + dwdzx1 * dwdzx1) +
-                    (dudyx1 + dvdxx1) * (dudyx1 + dvdxx1) + (dwdyx1 + dvdzx1) * (dwdyx1 + dvdzx1) +
-                    (dudzx1 + dwdxx1) * (dudzx1 + dwdxx1);
-    const float val = a * a * sqrtf(s);
     sm[c] = val;
     if (GHOSTS) sm_ghosts(g, i, j, k, val, sm);
   }
@@ -1303,12 +1395,14 @@ void launch_les(const DevDomain& d) {
                      d.m, diu, d.F.sm, d.F.uspd, d.F.vspd, d.F.f, d.F.g, d.F.h, nullptr, nullptr, nullptr, d.wl));
 }
 
-// les followed by adam (whole-step path)
+// les followed by adam (whole-step path); with fuse_sm the fused velfg
+// kernel has computed sm
 void launch_les_adam(const DevDomain& d) {
   const Grid& g = d.g;
   const P10 diu = p10(d.F.diu);
-  TIMED(d, KC_LES_SM, hipLaunchKernelGGL(k_les_sm, krun_grid(d, d.sm_kchunk), block2d(), 0, d.stream, g, diu,
-                     d.m.delx1 + 1, d.F.sm));
+  if (!d.fuse_sm)
+    TIMED(d, KC_LES_SM, hipLaunchKernelGGL(k_les_sm, krun_grid(d, d.sm_kchunk), block2d(), 0, d.stream, g, diu,
+                       d.m.delx1 + 1, d.F.sm));
   TIMED(d, KC_LES_VISC, hipLaunchKernelGGL(k_les_visc<true>, krun_grid(d, d.visc_kchunk), block2d(), 0, d.stream, g,
                      d.m, diu, d.F.sm, d.F.uspd, d.F.vspd, d.F.f, d.F.g, d.F.h, d.F.fold, d.F.gold, d.F.hold,
                      d.wl));
@@ -1489,8 +1583,9 @@ __global__ __launch_bounds__(NT) void kb_velfg_feedbf(const BatchDom* __restrict
                                                       const StageSpan* __restrict__ sp, int nd) {
   BATCH_BLOCK(BS_VELFG, false);
   const VelCtx X{d.g, d.m, d.F.u, d.F.v, d.F.w, d.wl};
-  velfg_feedbf_body(vb, X, p10(d.F.diu), d.F.f, d.F.g, d.F.h, d.F.usum, d.F.vsum, d.F.wsum, d.F.bmask, d.F.cmask,
-                    d.F.dmask, d.mask_src, d.F.uspd, d.F.vspd, B.vn, B.alpha, B.dt, B.beta, B.velfg_kc);
+  velfg_feedbf_body<false>(vb, X, p10(d.F.diu), d.F.f, d.F.g, d.F.h, d.F.usum, d.F.vsum, d.F.wsum, d.F.bmask,
+                           d.F.cmask, d.F.dmask, d.mask_src, d.F.uspd, d.F.vspd, B.vn, B.alpha, B.dt, B.beta,
+                           B.velfg_kc, nullptr, nullptr);
 }
 __global__ __launch_bounds__(NT) void kb_les_sm(const BatchDom* __restrict__ bd, const StageSpan* __restrict__ sp,
                                                 int nd) {

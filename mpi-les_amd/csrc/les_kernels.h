@@ -72,6 +72,7 @@ struct DevDomain {
   int sm_kchunk;     // k levels per thread of k_les_sm (0: automatic)
   int visc_kchunk;   // k levels per thread of k_les_visc (0: automatic)
   int fuse_les;      // whole steps apply adam inside the les viscous kernel
+  int fuse_sm;       // whole steps compute sm (and its ghosts) inside the fused velfg kernel
   int press_mode;    // 0: the reference's SOR; 1: FP64 defect correction around FP32 SOR
   int dc_outer;      // defect-correction steps per press call
   int dc_inner;      // FP32 SOR iterations per defect-correction step

@@ -666,6 +666,7 @@ int domain_alloc(int im, int jm, int km, les_domain** out) {
   D->d.sm_kchunk = std::max(0, env_int("LES_SM_KCHUNK", 0));
   D->d.visc_kchunk = std::max(0, env_int("LES_VISC_KCHUNK", 0));
   D->d.fuse_les = env_int("LES_FUSE_LES", 1);
+  D->d.fuse_sm = env_int("LES_FUSE_SM", 1) != 0 && D->d.fuse_velfg && D->d.fuse_les;
   D->d.sor_fused_blocks = env_int("LES_SOR_FUSED_BLOCKS", 0);
   {
     // fused SOR kernel: 4 = two iterations per launch (default; even jm and
@@ -1070,6 +1071,7 @@ int tune_stencils(les_domain* D, int reps, TuneResult* r) {
   }
   for (int kc : kcs) {  // les_sm's chunk, les_visc's automatic
     double us = 0.0;
+    if (D->d.fuse_sm && kc != 0) continue;  // sm is computed in velfg+feedbf
     if (rc || (rc = time_one(1, vkc, kc, 0, &us))) break;
     if (kc == 0) ldef = us;
     if (lbest < 0.0 || us < lbest) {
