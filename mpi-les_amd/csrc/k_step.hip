@@ -1579,13 +1579,14 @@ __global__ __launch_bounds__(256) void kb_bondv1_ghost(const BatchDom* __restric
   BATCH_BLOCK(BS_GHOST, false);
   bondv1_ghost_body(vb, d.g, d.F.u, d.F.v, d.F.w);
 }
+template <bool SM>
 __global__ __launch_bounds__(NT) void kb_velfg_feedbf(const BatchDom* __restrict__ bd,
                                                       const StageSpan* __restrict__ sp, int nd) {
   BATCH_BLOCK(BS_VELFG, false);
   const VelCtx X{d.g, d.m, d.F.u, d.F.v, d.F.w, d.wl};
-  velfg_feedbf_body<false>(vb, X, p10(d.F.diu), d.F.f, d.F.g, d.F.h, d.F.usum, d.F.vsum, d.F.wsum, d.F.bmask,
-                           d.F.cmask, d.F.dmask, d.mask_src, d.F.uspd, d.F.vspd, B.vn, B.alpha, B.dt, B.beta,
-                           B.velfg_kc, nullptr, nullptr);
+  velfg_feedbf_body<SM>(vb, X, p10(d.F.diu), d.F.f, d.F.g, d.F.h, d.F.usum, d.F.vsum, d.F.wsum, d.F.bmask,
+                        d.F.cmask, d.F.dmask, d.mask_src, d.F.uspd, d.F.vspd, B.vn, B.alpha, B.dt, B.beta,
+                        B.velfg_kc, d.m.delx1 + 1, d.F.sm);
 }
 __global__ __launch_bounds__(NT) void kb_les_sm(const BatchDom* __restrict__ bd, const StageSpan* __restrict__ sp,
                                                 int nd) {
@@ -1640,9 +1641,13 @@ void launch_step_batch(const StageBatch& b, hipStream_t stream) {
   hipLaunchKernelGGL(kb_bondv1_uout, nb(BS_UOUT), dim3(256), 0, stream, bd, sp, nd);
   hipLaunchKernelGGL(kb_bondv1_outflow, nb(BS_OUTFLOW), dim3(256), 0, stream, bd, sp, nd);
   hipLaunchKernelGGL(kb_bondv1_ghost, nb(BS_GHOST), dim3(256), 0, stream, bd, sp, nd);
-  hipLaunchKernelGGL(kb_velfg_feedbf, nb(BS_VELFG), block2d(), 0, stream, bd, sp, nd);
-  hipLaunchKernelGGL(kb_les_sm, nb(BS_LES_SM), block2d(), 0, stream, bd, sp, nd);
-  hipLaunchKernelGGL(kb_boundsm, nb(BS_BOUNDSM), dim3(256), 0, stream, bd, sp, nd);
+  if (b.fuse_sm) {
+    hipLaunchKernelGGL(kb_velfg_feedbf<true>, nb(BS_VELFG), block2d(), 0, stream, bd, sp, nd);
+  } else {
+    hipLaunchKernelGGL(kb_velfg_feedbf<false>, nb(BS_VELFG), block2d(), 0, stream, bd, sp, nd);
+    hipLaunchKernelGGL(kb_les_sm, nb(BS_LES_SM), block2d(), 0, stream, bd, sp, nd);
+    hipLaunchKernelGGL(kb_boundsm, nb(BS_BOUNDSM), dim3(256), 0, stream, bd, sp, nd);
+  }
   hipLaunchKernelGGL(kb_les_visc_adam, nb(BS_LES_VISC), block2d(), 0, stream, bd, sp, nd);
 }
 

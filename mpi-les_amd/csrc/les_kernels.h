@@ -176,6 +176,7 @@ struct StageBatch {
   int nblocks[BS_N];
   int kc1max;              // largest iteration-1 k chunk (dynamic LDS of that launch)
   int remap;
+  int fuse_sm;             // sm and its ghost cells inside kb_velfg_feedbf (no kb_les_sm, kb_boundsm)
 };
 // This block's domain e and position in that domain's grid, in a batched
 // launch over the spans sp[0..nd) (b0 ascending); false for the padding

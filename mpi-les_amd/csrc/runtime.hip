@@ -2025,6 +2025,7 @@ int batch_prepare(les_batch* B) {
     const int nd = (int)bdom.size();
     std::vector<StageSpan> flat((size_t)BS_N * nd);
     StageBatch& S = B->stages;
+    S.fuse_sm = env_int("LES_FUSE_SM", 1) != 0;
     for (int s = 0; s < BS_N; s++) {
       int b0 = 0;
       for (int q = 0; q < nd; q++) {
