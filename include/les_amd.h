@@ -223,7 +223,8 @@ int les_domain_step(les_domain* d, int n, int nmax);
  * 64, 32 x 64 or 32 x 32; k levels per block) for this grid shape: times
  * `reps` iterations of each candidate on the domain's SOR scratch arrays and
  * keeps the fastest.  With fused stages it also picks the k levels per
- * thread of velfg+feedbf, les_sm and les viscous+adam, timing `reps`
+ * thread of velfg+feedbf, les_sm (when sm is not computed inside
+ * velfg+feedbf) and les viscous+adam, timing `reps`
  * launches of each candidate on the domain's own fields, which it copies to
  * a temporary device allocation first and restores (les_domain_stencil_chunks
  * reports them).  The choices are remembered per (im, jm, km) for the
@@ -236,7 +237,8 @@ int les_domain_step(les_domain* d, int n, int nmax);
 int les_domain_autotune(les_domain* d, int reps, int* rows, int* tile_rows, int* tile_width, int* kchunk,
                         double* us_best, double* us_default);
 /* The k levels per thread of the fused stencil kernels (velfg+feedbf,
- * les_sm, les viscous+adam; 0 = automatic) in use by the domain, as
+ * les_sm, les viscous+adam; 0 = automatic, and always 0 for les_sm when
+ * velfg+feedbf computes sm) in use by the domain, as
  * les_domain_autotune picks them with the SOR shape (timing each candidate
  * on the domain's fields, which it restores), and the microseconds of one
  * velfg+feedbf plus one les+adam launch sequence with the tuned and with
