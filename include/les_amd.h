@@ -220,7 +220,7 @@ int les_domain_step(les_domain* d, int n, int nmax);
  * two iterations per launch on 32 x 64 tiles with a four-cell ring (even jm,
  * separable cn1), rows = 2 or 1 one iteration with two or one tile rows per
  * thread; tile rows x pairs with the ring of the one-iteration kernel: 16 x
- * 64, 32 x 64 or 32 x 32; k levels per block) for this grid shape: times
+ * 64 or 32 x 64; k levels per block) for this grid shape: times
  * `reps` iterations of each candidate on the domain's SOR scratch arrays and
  * keeps the fastest.  With fused stages it also picks the k levels per
  * thread of velfg+feedbf, les_sm (when sm is not computed inside
